@@ -1,0 +1,205 @@
+/*
+ * moris_gpu.h -- C ABI of the MI355X element K+R assembly library (libmoris_gpu.so).
+ *
+ * The library replaces the per-object loop of MORIS's bulk assembly
+ *
+ *   Solver_Interface::fill_matrix_and_RHS                projects/SOL/DLA/src/cl_DLA_Solver_Interface.cpp:317-365
+ *     -> MSI::Equation_Object::get_egn_obj_jacobian_and_residual
+ *                                                      projects/FEM/MSI/src/cl_MSI_Equation_Object.cpp:789-842
+ *        -> fem::Interpolation_Element::compute_jacobian_and_residual
+ *                                                      projects/FEM/INT/src/ELEM/cl_FEM_Interpolation_Element.cpp:632-690
+ *     -> Sparse_Matrix_EpetraFECrs::fill_matrix       projects/SOL/SOL_CORE/src/cl_Sparse_Matrix_EpetraFECrs.cpp:103-133
+ *
+ * for bulk sets whose IWGs are IWG_Diffusion_Bulk or IWG_Isotropic_Struc_Linear_Bulk, with one
+ * set-level batched call.  Everything crossing this boundary is plain C: POD structs, raw pointers,
+ * sizes.  No call throws; every entry point returns an mgpu_status and mgpu_last_error_string()
+ * describes the last failure of the calling thread.
+ *
+ * Index base is 0, ids are int32 (MORIS_USE_32BIT, CMakeLists.txt:156-157), all reals are FP64.
+ */
+#ifndef MORIS_GPU_H
+#define MORIS_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------------------------------
+ * status codes.  Replaces MORIS_ERROR (projects/MRS/ASR/src/fn_log_assert.hpp:41,92), which throws.
+ * ---------------------------------------------------------------------------------------------- */
+typedef int32_t mgpu_status;
+#define MGPU_OK                 0
+#define MGPU_ERR_INVALID_ARG    1   /* bad descriptor / pointer / size                          */
+#define MGPU_ERR_HIP            2   /* a HIP runtime call failed                                */
+#define MGPU_ERR_UNSUPPORTED    3   /* combination not implemented (e.g. IWG not in the set)    */
+#define MGPU_ERR_NO_DEVICE      4   /* no gfx950 device visible                                 */
+#define MGPU_ERR_STATE          5   /* call order violated (e.g. assemble before create)       */
+#define MGPU_ERR_GEOMETRY       6   /* level set vertex on the isocontour (|phi| <= 1e-12)      */
+
+/* IWG types (projects/FEM/INT/src/IWG/cl_FEM_IWG_Factory.cpp:153,191). */
+#define MGPU_IWG_SPATIALDIFF_BULK   0   /* IWG_Diffusion_Bulk, CM_Diffusion_Linear_Isotropic       */
+#define MGPU_IWG_STRUC_LINEAR_BULK  1   /* IWG_Isotropic_Struc_Linear_Bulk, CM_Struc_Linear_Isotropic
+                                           (3D full, 2D plane stress)                              */
+
+/* Cell geometries (mtk::Geometry_Type + interpolation order). */
+#define MGPU_GEOM_QUAD4  0
+#define MGPU_GEOM_HEX8   1
+#define MGPU_GEOM_TRI3   2
+#define MGPU_GEOM_TET4   3
+
+/* Arithmetic of the element integrands. */
+#define MGPU_PREC_FP64   0   /* plain path: everything FP64                                        */
+#define MGPU_PREC_MIXED  1   /* FP32 integrands with FP32 accumulation; FP64 cut-cell Jacobians and
+                                FP64 global accumulation                                           */
+
+/* Cut-cell quadrature. */
+#define MGPU_QUAD_SUBDIVISION  0   /* XTK regular subdivision + node hierarchy, TET_11 / TRI_7       */
+#define MGPU_QUAD_LEARNED      1   /* MLP weights on the HEX_3x3x3 Gauss points, moment-checked,
+                                      falling back to subdivision per element                      */
+
+/* Level-set kinds (GEN analytic geometries). */
+#define MGPU_LS_NONE     0   /* no geometry: every cell is uncut material                           */
+#define MGPU_LS_SPHERES  1   /* union of spherical (circular in 2D) holes: phi = min_i(|x-c_i|-r_i) */
+#define MGPU_LS_GYROID   2   /* phi = sin x' cos y' + sin y' cos z' + sin z' cos x', x' = s*x       */
+#define MGPU_LS_PLANE    3   /* phi = n.x + d                                                        */
+
+/* ------------------------------------------------------------------------------------------------
+ * Workload description: a level-0 HMR background grid, one GEN level set, XTK cut, MSI numbering.
+ * mgpu_problem_build() runs this set-up on the host and produces the set descriptor below.  It is
+ * the input side of the hot path (cl_XTK_Integration_Mesh_Generator.cpp:53-173,
+ * cl_MSI_Dof_Manager.cpp:684-990), not timed.
+ * ---------------------------------------------------------------------------------------------- */
+typedef struct mgpu_problem_spec {
+    int32_t dim;               /* 2 or 3                                                         */
+    int32_t n[3];              /* cells per axis (n[2] ignored in 2D)                            */
+    double  lo[3];             /* domain lower corner                                            */
+    double  hi[3];             /* domain upper corner                                            */
+    int32_t iwg;               /* MGPU_IWG_*                                                     */
+    int32_t ls_kind;           /* MGPU_LS_*                                                      */
+    int32_t n_spheres;         /* MGPU_LS_SPHERES: number of spheres                             */
+    const double* spheres;     /* MGPU_LS_SPHERES: [n_spheres][4] = cx, cy, cz, r               */
+    double  gyroid_scale;      /* MGPU_LS_GYROID: s                                              */
+    double  plane[4];          /* MGPU_LS_PLANE: nx, ny, nz, d                                   */
+    int32_t material_phase;    /* 1: phi > 0 is material (phase 1), 0: phi < 0 is material      */
+    double  E, nu;             /* elasticity                                                     */
+    double  k_mu, k_sigma;     /* diffusion: k_e = exp(k_mu + k_sigma * N(0,1)) per element     */
+    uint64_t seed;             /* splitmix64 seed of the per-element conductivity                */
+} mgpu_problem_spec;
+
+/* Uniform lattice the elements of a set sit on (HMR level 0).  n[0] == 0: no lattice. */
+typedef struct mgpu_lattice {
+    int32_t n[3];              /* cells per axis                                                 */
+    double  lo[3];             /* lower corner                                                   */
+    double  h[3];              /* cell size                                                      */
+} mgpu_lattice;
+
+/* ------------------------------------------------------------------------------------------------
+ * Set descriptor: element-major POD arrays of one or more bulk blocks (uncut *_n_p and cut *_c_p)
+ * assembled into one matrix.  Mirrors what fem::Set holds per equation object
+ * (projects/FEM/INT/src/CORE/cl_FEM_Set.hpp:64; clusters cl_FEM_Set.cpp:267-320).
+ * ---------------------------------------------------------------------------------------------- */
+typedef struct mgpu_set_desc {
+    int32_t spatial_dim;       /* 2 | 3                                                          */
+    int32_t iwg;               /* MGPU_IWG_*                                                     */
+    int32_t ip_geometry;       /* MGPU_GEOM_QUAD4 | MGPU_GEOM_HEX8 (IP cell, Lagrange linear)   */
+    int32_t ig_geometry;       /* MGPU_GEOM_TRI3 | MGPU_GEOM_TET4 (cut sub-cells)              */
+    int32_t n_elements;        /* equation objects                                               */
+    int32_t n_bases;           /* IP-cell bases per field (4 | 8)                                */
+    int32_t n_adofs_per_type;  /* adofs of one dof type; total adofs = n_fields * this           */
+
+    /* IP cell: axis-aligned box, [n_elements][2*dim] = lower corner then extent.                */
+    const double* ip_box;
+    /* material: diffusion [n_elements][1] = k; elasticity [n_elements][2] = E, nu.             */
+    const double* material;
+
+    /* T-matrix rows per (element, basis): type-local adof ids and weights (cl_MSI_Pdof_Host.cpp
+       :309-337 via cl_HMR_T_Matrix_Base.cpp:306-457).  Row r = e*n_bases + b spans
+       [t_ptr[r], t_ptr[r+1]).  The same rows serve every dof type of an element.               */
+    const int32_t* t_ptr;      /* [n_elements*n_bases + 1]                                       */
+    const int32_t* t_adof;
+    const double*  t_weight;
+
+    /* Cut sub-cells (IG cells of the element's material subphase).  ig_ptr[e] == ig_ptr[e+1]
+       means uncut: the IP cell itself is integrated with the auto rule (QUAD_2x2 / HEX_2x2x2).
+       ig_xi holds every IG vertex's parametric coordinate wrt the IP cell,
+       [n_ig][n_vertices(ig_geometry)][spatial_dim]
+       (Cluster::get_primary_cell_local_coords_on_side_wrt_interp_cell).                      */
+    const int32_t* ig_ptr;     /* [n_elements + 1]                                               */
+    const double*  ig_xi;
+
+    /* Optional lattice hint: elem_cell[e] = lattice cell (i + n0*(j + n1*k)).  With a lattice and
+       nodal T rows (one entry, weight 1) the owner-computes stencil path is used.               */
+    mgpu_lattice   lattice;
+    const int32_t* elem_cell;  /* [n_elements] or NULL                                          */
+
+    /* Per-element level-set values at the IP-cell vertices, [n_elements][n_bases] (optional,
+       needed by MGPU_QUAD_LEARNED).                                                             */
+    const double*  vertex_phi;
+} mgpu_set_desc;
+
+typedef struct mgpu_problem mgpu_problem;   /* host-side workload (cut mesh, numbering)        */
+typedef struct mgpu_set     mgpu_set;       /* device-resident set + CSR pattern + scratch     */
+
+/* Host set-up. */
+mgpu_status mgpu_problem_build(const mgpu_problem_spec* spec, mgpu_problem** out);
+mgpu_status mgpu_problem_desc(const mgpu_problem* p, mgpu_set_desc* out);
+/* Counts: [0] elements, [1] cut elements, [2] IG sub-cells, [3] adofs per type, [4] lattice cells,
+   [5] void cells.                                                                               */
+mgpu_status mgpu_problem_counts(const mgpu_problem* p, int64_t counts[6]);
+mgpu_status mgpu_problem_destroy(mgpu_problem* p);
+
+/* Set life cycle (fem::Set + MSI::Equation_Set scratch + SOL graph; cl_FEM_Set.cpp:35-100,
+   :132 initialize_set, cl_DLA_Solver_Interface.cpp:20-54 build_graph).                          */
+mgpu_status mgpu_set_create(const mgpu_set_desc* desc, mgpu_set** out);
+mgpu_status mgpu_set_destroy(mgpu_set* set);
+/* Precision and cut-cell quadrature of later assemble calls. */
+mgpu_status mgpu_set_options(mgpu_set* set, int32_t precision, int32_t quadrature);
+/* HIP stream (hipStream_t passed as void*) all work of the set is issued on; NULL = default. */
+mgpu_status mgpu_set_stream(mgpu_set* set, void* hip_stream);
+
+/* CSR pattern of the set's matrix (host arrays owned by the handle, valid until destroy):
+   union of element topologies (cl_Sparse_Matrix_EpetraFECrs.cpp:195-240), rows and columns
+   sorted ascending.                                                                             */
+mgpu_status mgpu_pattern_get(const mgpu_set* set, int32_t* n_rows, int64_t* nnz,
+                             const int32_t** row_ptr, const int32_t** col_idx);
+/* Element topology (cl_MSI_Equation_Object.cpp:505-554): sorted unique adof ids per dof type,
+   concatenated over types.  ids: [n_elements][n_fields*n_bases] (host, owned by the handle).  */
+mgpu_status mgpu_element_topology(const mgpu_set* set, int32_t* n_per_element, const int32_t** ids);
+
+/* K+R assembly of all elements of the set into caller-owned DEVICE arrays:
+   csr_vals[nnz] and rhs[n_rows] are overwritten (zeroed then summed, cl_DLA_Linear_Problem.cpp
+   :131-138 mat_put_scalar(0) + fill_matrix_and_RHS); u_adof[n_rows] is the adof vector
+   (compute_my_pdof_values, cl_MSI_Equation_Object.cpp:1100-1131).  Asynchronous on the set's
+   stream.                                                                                       */
+mgpu_status mgpu_set_assemble(mgpu_set* set, const double* u_adof, double* csr_vals, double* rhs);
+/* Same with HOST arrays; copies in and out, synchronous. */
+mgpu_status mgpu_set_assemble_host(mgpu_set* set, const double* u_adof, double* csr_vals, double* rhs);
+
+/* Statistics of the last assemble: [0] elements, [1] cut elements integrated by subdivision,
+   [2] cut elements integrated by learned quadrature, [3] learned elements rejected by the moment
+   check, [4] integration points evaluated.                                                      */
+mgpu_status mgpu_set_stats(const mgpu_set* set, int64_t stats[5]);
+
+/* Device helpers (so callers need no HIP headers). */
+mgpu_status mgpu_device_count(int32_t* n);
+mgpu_status mgpu_malloc(void** ptr, size_t bytes);
+mgpu_status mgpu_free(void* ptr);
+mgpu_status mgpu_memcpy_htod(void* dst, const void* src, size_t bytes);
+mgpu_status mgpu_memcpy_dtoh(void* dst, const void* src, size_t bytes);
+mgpu_status mgpu_synchronize(void);
+/* Time `iters` back-to-back mgpu_set_assemble calls with HIP events on the set's stream.
+   per_call_ms receives the mean; kernel_ms[4] the mean per-call duration of the cut-element,
+   stencil, generic and scatter kernels measured with events around each (0 where not used).  */
+mgpu_status mgpu_set_time_assemble(mgpu_set* set, const double* u_adof, double* csr_vals,
+                                   double* rhs, int32_t iters, double* per_call_ms,
+                                   double kernel_ms[4]);
+
+const char* mgpu_last_error_string(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MORIS_GPU_H */

@@ -1,0 +1,342 @@
+"""moris_amd -- MI355X-native element K+R assembly for XTK-cut HMR meshes.
+
+Python view of the C ABI in ``include/moris_gpu.h`` (libmoris_gpu.so, built in-tree by
+``moris_amd.build``).  The classes mirror the reference's operator surface for the bulk-assembly
+path: a :class:`Problem` is the HMR/GEN/XTK/MSI set-up that produces element-major set data, a
+:class:`Set` is ``fem::Set`` + the SOL graph, and :meth:`Set.compute_jacobian_and_residual`
+replaces ``Solver_Interface::fill_matrix_and_RHS`` over the set's equation objects
+(projects/SOL/DLA/src/cl_DLA_Solver_Interface.cpp:317-365).
+
+No PyTorch here: the library owns device memory through its own helpers, so the process holds a
+single HIP runtime.  There is no CPU fallback: every compute call goes through the HIP library and
+raises if it is missing.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libmoris_gpu.so")
+
+# constants (include/moris_gpu.h)
+MGPU_OK = 0
+IWG_SPATIALDIFF_BULK = 0
+IWG_STRUC_LINEAR_BULK = 1
+GEOM_QUAD4, GEOM_HEX8, GEOM_TRI3, GEOM_TET4 = 0, 1, 2, 3
+PREC_FP64, PREC_MIXED = 0, 1
+QUAD_SUBDIVISION, QUAD_LEARNED = 0, 1
+LS_NONE, LS_SPHERES, LS_GYROID, LS_PLANE = 0, 1, 2, 3
+
+_c_i32p = ctypes.POINTER(ctypes.c_int32)
+_c_f64p = ctypes.POINTER(ctypes.c_double)
+
+
+class ProblemSpec(ctypes.Structure):
+    _fields_ = [
+        ("dim", ctypes.c_int32),
+        ("n", ctypes.c_int32 * 3),
+        ("lo", ctypes.c_double * 3),
+        ("hi", ctypes.c_double * 3),
+        ("iwg", ctypes.c_int32),
+        ("ls_kind", ctypes.c_int32),
+        ("n_spheres", ctypes.c_int32),
+        ("spheres", _c_f64p),
+        ("gyroid_scale", ctypes.c_double),
+        ("plane", ctypes.c_double * 4),
+        ("material_phase", ctypes.c_int32),
+        ("E", ctypes.c_double),
+        ("nu", ctypes.c_double),
+        ("k_mu", ctypes.c_double),
+        ("k_sigma", ctypes.c_double),
+        ("seed", ctypes.c_uint64),
+    ]
+
+
+class Lattice(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int32 * 3), ("lo", ctypes.c_double * 3), ("h", ctypes.c_double * 3)]
+
+
+class SetDesc(ctypes.Structure):
+    _fields_ = [
+        ("spatial_dim", ctypes.c_int32),
+        ("iwg", ctypes.c_int32),
+        ("ip_geometry", ctypes.c_int32),
+        ("ig_geometry", ctypes.c_int32),
+        ("n_elements", ctypes.c_int32),
+        ("n_bases", ctypes.c_int32),
+        ("n_adofs_per_type", ctypes.c_int32),
+        ("ip_box", _c_f64p),
+        ("material", _c_f64p),
+        ("t_ptr", _c_i32p),
+        ("t_adof", _c_i32p),
+        ("t_weight", _c_f64p),
+        ("ig_ptr", _c_i32p),
+        ("ig_xi", _c_f64p),
+        ("lattice", Lattice),
+        ("elem_cell", _c_i32p),
+        ("vertex_phi", _c_f64p),
+    ]
+
+
+# every symbol include/moris_gpu.h declares
+EXPORTED = [
+    "mgpu_problem_build", "mgpu_problem_desc", "mgpu_problem_counts", "mgpu_problem_destroy",
+    "mgpu_set_create", "mgpu_set_destroy", "mgpu_set_options", "mgpu_set_stream", "mgpu_pattern_get",
+    "mgpu_element_topology", "mgpu_set_assemble", "mgpu_set_assemble_host", "mgpu_set_stats",
+    "mgpu_device_count", "mgpu_malloc", "mgpu_free", "mgpu_memcpy_htod", "mgpu_memcpy_dtoh",
+    "mgpu_synchronize", "mgpu_set_time_assemble", "mgpu_last_error_string",
+]
+
+_lib = None
+
+
+class MgpuError(RuntimeError):
+    pass
+
+
+def load_library(path=LIB_PATH):
+    """Load libmoris_gpu.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise MgpuError("libmoris_gpu.so not built at %s (run python -m moris_amd.build)" % path)
+    lib = ctypes.CDLL(path)
+    vp = ctypes.c_void_p
+    lib.mgpu_last_error_string.restype = ctypes.c_char_p
+    lib.mgpu_problem_build.argtypes = [ctypes.POINTER(ProblemSpec), ctypes.POINTER(vp)]
+    lib.mgpu_problem_desc.argtypes = [vp, ctypes.POINTER(SetDesc)]
+    lib.mgpu_problem_counts.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
+    lib.mgpu_problem_destroy.argtypes = [vp]
+    lib.mgpu_set_create.argtypes = [ctypes.POINTER(SetDesc), ctypes.POINTER(vp)]
+    lib.mgpu_set_destroy.argtypes = [vp]
+    lib.mgpu_set_options.argtypes = [vp, ctypes.c_int32, ctypes.c_int32]
+    lib.mgpu_set_stream.argtypes = [vp, vp]
+    lib.mgpu_pattern_get.argtypes = [vp, _c_i32p, ctypes.POINTER(ctypes.c_int64),
+                                     ctypes.POINTER(_c_i32p), ctypes.POINTER(_c_i32p)]
+    lib.mgpu_element_topology.argtypes = [vp, _c_i32p, ctypes.POINTER(_c_i32p)]
+    lib.mgpu_set_assemble.argtypes = [vp, vp, vp, vp]
+    lib.mgpu_set_assemble_host.argtypes = [vp, _c_f64p, _c_f64p, _c_f64p]
+    lib.mgpu_set_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
+    lib.mgpu_device_count.argtypes = [_c_i32p]
+    lib.mgpu_malloc.argtypes = [ctypes.POINTER(vp), ctypes.c_size_t]
+    lib.mgpu_free.argtypes = [vp]
+    lib.mgpu_memcpy_htod.argtypes = [vp, vp, ctypes.c_size_t]
+    lib.mgpu_memcpy_dtoh.argtypes = [vp, vp, ctypes.c_size_t]
+    lib.mgpu_synchronize.argtypes = []
+    lib.mgpu_set_time_assemble.argtypes = [vp, vp, vp, vp, ctypes.c_int32, _c_f64p, _c_f64p]
+    for name in EXPORTED:
+        getattr(lib, name).restype = ctypes.c_int32 if name != "mgpu_last_error_string" else ctypes.c_char_p
+    _lib = lib
+    return lib
+
+
+def _check(rc, what):
+    if rc != MGPU_OK:
+        msg = _lib.mgpu_last_error_string().decode(errors="replace")
+        raise MgpuError("%s failed (status %d): %s" % (what, rc, msg))
+
+
+def _ptr(a, ctype):
+    return a.ctypes.data_as(ctypes.POINTER(ctype))
+
+
+def make_spec(dim, n, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), iwg=IWG_SPATIALDIFF_BULK, spheres=None,
+              ls_kind=None, gyroid_scale=1.0, plane=(0.0, 0.0, 0.0, 0.0), material_phase=1, E=1.0, nu=0.3,
+              k_mu=0.0, k_sigma=0.0, seed=20261202):
+    """Build a ProblemSpec; returns (spec, keepalive) -- keep the second value alive with the spec."""
+    s = ProblemSpec()
+    s.dim = dim
+    nn = list(n) + [1] * (3 - len(n))
+    s.n[:] = nn[:3]
+    s.lo[:] = (list(lo) + [0.0] * 3)[:3]
+    s.hi[:] = (list(hi) + [1.0] * 3)[:3]
+    s.iwg = iwg
+    keep = None
+    if spheres is not None:
+        keep = np.ascontiguousarray(np.asarray(spheres, dtype=np.float64).reshape(-1, 4))
+        s.n_spheres = keep.shape[0]
+        s.spheres = _ptr(keep, ctypes.c_double)
+        s.ls_kind = LS_SPHERES if ls_kind is None else ls_kind
+    else:
+        s.ls_kind = LS_NONE if ls_kind is None else ls_kind
+    s.gyroid_scale = gyroid_scale
+    s.plane[:] = list(plane)
+    s.material_phase = material_phase
+    s.E, s.nu, s.k_mu, s.k_sigma = E, nu, k_mu, k_sigma
+    s.seed = seed
+    return s, keep
+
+
+class Problem:
+    """Host set-up of one workload (lattice, level set, XTK cut, adof numbering)."""
+
+    def __init__(self, spec, keepalive=None):
+        lib = load_library()
+        self._keep = keepalive
+        self.spec = spec
+        self._h = ctypes.c_void_p()
+        _check(lib.mgpu_problem_build(ctypes.byref(spec), ctypes.byref(self._h)), "mgpu_problem_build")
+        self.desc = SetDesc()
+        _check(lib.mgpu_problem_desc(self._h, ctypes.byref(self.desc)), "mgpu_problem_desc")
+        c = (ctypes.c_int64 * 6)()
+        _check(lib.mgpu_problem_counts(self._h, c), "mgpu_problem_counts")
+        (self.n_elements, self.n_cut, self.n_ig, self.n_adofs_per_type, self.n_cells, self.n_void) = list(c)
+
+    @property
+    def dim(self):
+        return self.desc.spatial_dim
+
+    def arrays(self):
+        """numpy views (copies) of the descriptor arrays."""
+        d = self.desc
+        ne, nb, dim = d.n_elements, d.n_bases, d.spatial_dim
+        nm = 2 if d.iwg == IWG_STRUC_LINEAR_BULK else 1
+        nv = dim + 1
+
+        def view(p, n, dt):
+            if n == 0:
+                return np.zeros(0, dtype=dt)
+            return np.ctypeslib.as_array(p, shape=(n,)).astype(dt, copy=True)
+
+        ig_ptr = view(d.ig_ptr, ne + 1, np.int32)
+        return {
+            "elem_cell": view(d.elem_cell, ne, np.int32),
+            "ig_ptr": ig_ptr,
+            "ig_xi": view(d.ig_xi, int(ig_ptr[-1]) * nv * dim, np.float64),
+            "material": view(d.material, ne * nm, np.float64),
+            "t_adof": view(d.t_adof, ne * nb, np.int32),
+            "ip_box": view(d.ip_box, ne * 2 * dim, np.float64),
+            "vertex_phi": view(d.vertex_phi, ne * nb, np.float64),
+        }
+
+    def close(self):
+        if self._h:
+            load_library().mgpu_problem_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceBuffer:
+    """Device allocation owned by the library's HIP runtime."""
+
+    def __init__(self, nbytes):
+        lib = load_library()
+        self.nbytes = int(nbytes)
+        self.ptr = ctypes.c_void_p()
+        _check(lib.mgpu_malloc(ctypes.byref(self.ptr), max(self.nbytes, 8)), "mgpu_malloc")
+
+    @classmethod
+    def from_array(cls, a):
+        a = np.ascontiguousarray(a)
+        b = cls(a.nbytes)
+        if a.nbytes:
+            _check(load_library().mgpu_memcpy_htod(b.ptr, a.ctypes.data_as(ctypes.c_void_p), a.nbytes), "htod")
+        return b
+
+    def to_array(self, dtype, count):
+        out = np.empty(count, dtype=dtype)
+        if out.nbytes:
+            _check(load_library().mgpu_memcpy_dtoh(out.ctypes.data_as(ctypes.c_void_p), self.ptr, out.nbytes), "dtoh")
+        return out
+
+    def free(self):
+        if self.ptr:
+            load_library().mgpu_free(self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Set:
+    """fem::Set of the bulk blocks of a problem, on the GPU."""
+
+    def __init__(self, desc, keepalive=None):
+        lib = load_library()
+        self._keep = keepalive
+        self.desc = desc
+        self._h = ctypes.c_void_p()
+        _check(lib.mgpu_set_create(ctypes.byref(desc), ctypes.byref(self._h)), "mgpu_set_create")
+        nr = ctypes.c_int32()
+        nnz = ctypes.c_int64()
+        rp = _c_i32p()
+        ci = _c_i32p()
+        _check(lib.mgpu_pattern_get(self._h, ctypes.byref(nr), ctypes.byref(nnz), ctypes.byref(rp),
+                                    ctypes.byref(ci)), "mgpu_pattern_get")
+        self.n_rows = nr.value
+        self.nnz = nnz.value
+        self._rp, self._ci = rp, ci
+
+    def options(self, precision=PREC_FP64, quadrature=QUAD_SUBDIVISION):
+        _check(load_library().mgpu_set_options(self._h, precision, quadrature), "mgpu_set_options")
+
+    def pattern(self):
+        rp = np.ctypeslib.as_array(self._rp, shape=(self.n_rows + 1,)).copy()
+        ci = np.ctypeslib.as_array(self._ci, shape=(max(self.nnz, 1),))[: self.nnz].copy()
+        return rp, ci
+
+    def topology(self):
+        n = ctypes.c_int32()
+        p = _c_i32p()
+        _check(load_library().mgpu_element_topology(self._h, ctypes.byref(n), ctypes.byref(p)), "topology")
+        ne = self.desc.n_elements
+        if ne == 0:
+            return np.zeros((0, n.value), dtype=np.int32)
+        return np.ctypeslib.as_array(p, shape=(ne * n.value,)).reshape(ne, n.value).copy()
+
+    def compute_jacobian_and_residual(self, u_adof):
+        """K values (CSR, pattern()) and R for adof vector u (host arrays)."""
+        u = np.ascontiguousarray(u_adof, dtype=np.float64)
+        if u.shape != (self.n_rows,):
+            raise ValueError("u must have %d entries" % self.n_rows)
+        vals = np.empty(self.nnz, dtype=np.float64)
+        rhs = np.empty(self.n_rows, dtype=np.float64)
+        _check(load_library().mgpu_set_assemble_host(self._h, _ptr(u, ctypes.c_double), _ptr(vals, ctypes.c_double),
+                                                     _ptr(rhs, ctypes.c_double)), "mgpu_set_assemble_host")
+        return vals, rhs
+
+    def assemble_device(self, u_buf, vals_buf, rhs_buf):
+        _check(load_library().mgpu_set_assemble(self._h, u_buf.ptr, vals_buf.ptr, rhs_buf.ptr), "mgpu_set_assemble")
+
+    def time_assemble(self, u_buf, vals_buf, rhs_buf, iters):
+        per = ctypes.c_double()
+        km = (ctypes.c_double * 4)()
+        _check(load_library().mgpu_set_time_assemble(self._h, u_buf.ptr, vals_buf.ptr, rhs_buf.ptr, iters,
+                                                     ctypes.byref(per), km), "mgpu_set_time_assemble")
+        return per.value, list(km)
+
+    def stats(self):
+        s = (ctypes.c_int64 * 5)()
+        _check(load_library().mgpu_set_stats(self._h, s), "mgpu_set_stats")
+        return list(s)
+
+    def close(self):
+        if self._h:
+            load_library().mgpu_set_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def synchronize():
+    _check(load_library().mgpu_synchronize(), "mgpu_synchronize")
+
+
+def device_count():
+    n = ctypes.c_int32()
+    load_library().mgpu_device_count(ctypes.byref(n))
+    return n.value

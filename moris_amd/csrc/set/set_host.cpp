@@ -1,0 +1,243 @@
+// set_host.cpp -- element topology, CSR pattern and lattice maps of an mgpu_set.
+//
+//   element topology   Equation_Object::get_equation_obj_dof_ids, cl_MSI_Equation_Object.cpp:505-554
+//                      (create_my_list_of_adof_ids :199-291 -- sorted unique adof ids per dof type,
+//                      concatenated over the requested types)
+//   CSR pattern        Sparse_Matrix_EpetraFECrs::build_graph, cl_Sparse_Matrix_EpetraFECrs.cpp:195-240
+//                      (union over elements of topology x topology), rows/cols ascending
+#include <algorithm>
+#include <atomic>
+#include <cstring>
+#include <numeric>
+#include <thread>
+
+#include "../common/fem_tables.hpp"
+#include "set_internal.hpp"
+
+namespace mgpu {
+
+namespace {
+int n_threads() {
+    unsigned t = std::thread::hardware_concurrency();
+    if (t == 0) t = 4;
+    return (int)std::min(t, 32u);
+}
+template <class F>
+void parallel_for(int64_t n, F f) {
+    const int T = n_threads();
+    std::vector<std::thread> th;
+    for (int t = 0; t < T; ++t)
+        th.emplace_back([&, t]() {
+            const int64_t a = n * t / T, b = n * (t + 1) / T;
+            for (int64_t i = a; i < b; ++i) f(i);
+        });
+    for (auto& x : th) x.join();
+}
+}  // namespace
+
+mgpu_status build_set_host(const mgpu_set_desc& d, SetHost& h, std::string* err) {
+    if (d.spatial_dim != 2 && d.spatial_dim != 3) { *err = "spatial_dim must be 2 or 3"; return MGPU_ERR_INVALID_ARG; }
+    if (d.iwg != MGPU_IWG_SPATIALDIFF_BULK && d.iwg != MGPU_IWG_STRUC_LINEAR_BULK) {
+        *err = "only SPATIALDIFF_BULK and STRUC_LINEAR_BULK are implemented";
+        return MGPU_ERR_UNSUPPORTED;
+    }
+    const int want_ip = d.spatial_dim == 3 ? MGPU_GEOM_HEX8 : MGPU_GEOM_QUAD4;
+    const int want_ig = d.spatial_dim == 3 ? MGPU_GEOM_TET4 : MGPU_GEOM_TRI3;
+    if (d.ip_geometry != want_ip || d.ig_geometry != want_ig) {
+        *err = "ip/ig geometry must be HEX8/TET4 (3D) or QUAD4/TRI3 (2D)";
+        return MGPU_ERR_UNSUPPORTED;
+    }
+    const int nb_expect = d.spatial_dim == 3 ? 8 : 4;
+    if (d.n_bases != nb_expect) { *err = "n_bases must be 8 (HEX8) or 4 (QUAD4)"; return MGPU_ERR_INVALID_ARG; }
+    if (d.n_elements < 0 || d.n_adofs_per_type < 0) { *err = "negative sizes"; return MGPU_ERR_INVALID_ARG; }
+    if (d.n_elements > 0 && (!d.ip_box || !d.material || !d.t_ptr || !d.t_adof || !d.t_weight || !d.ig_ptr)) {
+        *err = "descriptor array missing";
+        return MGPU_ERR_INVALID_ARG;
+    }
+    h.dim = d.spatial_dim;
+    h.iwg = d.iwg;
+    h.nf = d.iwg == MGPU_IWG_STRUC_LINEAR_BULK ? d.spatial_dim : 1;
+    h.nb = d.n_bases;
+    h.na = h.nf * h.nb;
+    h.nm = d.iwg == MGPU_IWG_STRUC_LINEAR_BULK ? 2 : 1;
+    h.n_el = d.n_elements;
+    h.n_adofs_type = d.n_adofs_per_type;
+    if ((int64_t)h.nf * h.n_adofs_type >= INT32_MAX) { *err = "too many adofs for int32 ids"; return MGPU_ERR_INVALID_ARG; }
+    h.n_rows = h.nf * h.n_adofs_type;
+    const int nb = h.nb, nf = h.nf, na = h.na;
+    const int32_t N = h.n_adofs_type;
+
+    // validate T rows and sub-cell offsets
+    const int64_t n_trows = (int64_t)h.n_el * nb;
+    if (d.t_ptr[0] != 0) { *err = "t_ptr[0] must be 0"; return MGPU_ERR_INVALID_ARG; }
+    for (int64_t r = 0; r < n_trows; ++r) {
+        if (d.t_ptr[r + 1] < d.t_ptr[r]) { *err = "t_ptr not monotone"; return MGPU_ERR_INVALID_ARG; }
+        for (int32_t q = d.t_ptr[r]; q < d.t_ptr[r + 1]; ++q)
+            if (d.t_adof[q] < 0 || d.t_adof[q] >= N) { *err = "t_adof out of range"; return MGPU_ERR_INVALID_ARG; }
+    }
+    if (d.ig_ptr[0] != 0) { *err = "ig_ptr[0] must be 0"; return MGPU_ERR_INVALID_ARG; }
+    for (int32_t e = 0; e < h.n_el; ++e)
+        if (d.ig_ptr[e + 1] < d.ig_ptr[e]) { *err = "ig_ptr not monotone"; return MGPU_ERR_INVALID_ARG; }
+    if (d.ig_ptr[h.n_el] > 0 && !d.ig_xi) { *err = "ig_xi missing"; return MGPU_ERR_INVALID_ARG; }
+
+    // the kernels implement p=1 nodal T rows (one adof of weight 1 per basis); refuse the rest
+    bool nodal = true;
+    for (int64_t r = 0; r < n_trows && nodal; ++r)
+        nodal = (d.t_ptr[r + 1] - d.t_ptr[r] == 1) && d.t_weight[d.t_ptr[r]] == 1.0;
+    if (!nodal) {
+        *err = "T rows with more than one entry or weight != 1 (B-spline p>1, THB) are not implemented";
+        return MGPU_ERR_UNSUPPORTED;
+    }
+
+    // element topology: sorted unique adofs per type; vertex -> sorted position
+    h.topo.resize((size_t)h.n_el * na);
+    h.vert_pos.resize((size_t)h.n_el * nb);
+    std::atomic<bool> dup{false};
+    parallel_for(h.n_el, [&](int64_t e) {
+        int32_t ids[8];
+        for (int b = 0; b < nb; ++b) ids[b] = d.t_adof[d.t_ptr[e * nb + b]];
+        int32_t s[8];
+        std::copy(ids, ids + nb, s);
+        std::sort(s, s + nb);
+        if (std::adjacent_find(s, s + nb) != s + nb) dup.store(true);
+        for (int b = 0; b < nb; ++b)
+            h.vert_pos[(size_t)e * nb + b] = (uint8_t)(std::lower_bound(s, s + nb, ids[b]) - s);
+        for (int f = 0; f < nf; ++f)
+            for (int b = 0; b < nb; ++b) h.topo[(size_t)e * na + f * nb + b] = s[b] + f * N;
+    });
+    if (dup.load()) { *err = "an element references one adof from two bases"; return MGPU_ERR_UNSUPPORTED; }
+
+    // adof -> elements (counting sort)
+    std::vector<int32_t> a_ptr((size_t)N + 1, 0), a_el;
+    for (int64_t e = 0; e < h.n_el; ++e)
+        for (int b = 0; b < nb; ++b) a_ptr[(size_t)h.topo[(size_t)e * na + b] + 1]++;
+    for (int32_t a = 0; a < N; ++a) a_ptr[(size_t)a + 1] += a_ptr[(size_t)a];
+    a_el.resize((size_t)a_ptr[(size_t)N]);
+    {
+        std::vector<int32_t> fill(a_ptr.begin(), a_ptr.end() - 1);
+        for (int64_t e = 0; e < h.n_el; ++e)
+            for (int b = 0; b < nb; ++b) a_el[(size_t)fill[(size_t)h.topo[(size_t)e * na + b]]++] = (int32_t)e;
+    }
+    // per type-local adof: sorted unique neighbour adofs (union of element topologies)
+    std::vector<int32_t> deg((size_t)N, 0);
+    parallel_for(N, [&](int64_t a) {
+        int32_t buf[8 * 27];
+        int n = 0;
+        for (int32_t q = a_ptr[(size_t)a]; q < a_ptr[(size_t)a + 1]; ++q) {
+            const int64_t e = a_el[(size_t)q];
+            for (int b = 0; b < nb; ++b) {
+                if (n < 8 * 27) buf[n++] = h.topo[(size_t)e * na + b];
+            }
+        }
+        std::sort(buf, buf + n);
+        deg[(size_t)a] = (int32_t)(std::unique(buf, buf + n) - buf);
+    });
+    // rows: type-major; row (f, a) has columns of every type g: g*N + nbr(a)
+    h.row_ptr.resize((size_t)h.n_rows + 1);
+    h.row_ptr[0] = 0;
+    int64_t nnz = 0;
+    for (int f = 0; f < nf; ++f)
+        for (int32_t a = 0; a < N; ++a) {
+            nnz += (int64_t)nf * deg[(size_t)a];
+            if (nnz >= INT32_MAX) { *err = "nnz exceeds int32"; return MGPU_ERR_INVALID_ARG; }
+            h.row_ptr[(size_t)f * N + a + 1] = (int32_t)nnz;
+        }
+    h.col_idx.resize((size_t)nnz);
+    parallel_for(N, [&](int64_t a) {
+        int32_t buf[8 * 27];
+        int n = 0;
+        for (int32_t q = a_ptr[(size_t)a]; q < a_ptr[(size_t)a + 1]; ++q) {
+            const int64_t e = a_el[(size_t)q];
+            for (int b = 0; b < nb; ++b)
+                if (n < 8 * 27) buf[n++] = h.topo[(size_t)e * na + b];
+        }
+        std::sort(buf, buf + n);
+        n = (int)(std::unique(buf, buf + n) - buf);
+        for (int f = 0; f < nf; ++f) {
+            int32_t* dst = h.col_idx.data() + h.row_ptr[(size_t)f * N + a];
+            for (int g = 0; g < nf; ++g)
+                for (int i = 0; i < n; ++i) *dst++ = buf[i] + g * N;
+        }
+    });
+    if (h.n_el > 0 && a_ptr[(size_t)N] > 0) {
+        // an adof of a single element has at most nb neighbours; an interior node 27 (3D) / 9 (2D)
+        for (int32_t a = 0; a < N; ++a)
+            if (deg[(size_t)a] > (h.dim == 3 ? 27 : 9)) {
+                *err = "adof with more neighbours than a lattice stencil";
+                return MGPU_ERR_UNSUPPORTED;
+            }
+    }
+
+    // cut elements
+    h.elem_cut.assign((size_t)h.n_el, -1);
+    h.cut_elems.clear();
+    for (int32_t e = 0; e < h.n_el; ++e)
+        if (d.ig_ptr[e + 1] > d.ig_ptr[e]) {
+            h.elem_cut[(size_t)e] = (int32_t)h.cut_elems.size();
+            h.cut_elems.push_back(e);
+        }
+
+    if (h.n_el == 0) return MGPU_OK;   // empty set: nothing to assemble
+
+    // lattice maps
+    h.lattice = d.lattice.n[0] > 0 && d.elem_cell != nullptr;
+    if (!h.lattice) {
+        *err = "sets without a lattice hint are not implemented";
+        return MGPU_ERR_UNSUPPORTED;
+    }
+    h.lat = d.lattice;
+    if (h.dim == 2) { h.lat.n[2] = 1; }
+    for (int k = 0; k < h.dim; ++k)
+        if (h.lat.n[k] < 1 || !(h.lat.h[k] > 0.0)) { *err = "bad lattice"; return MGPU_ERR_INVALID_ARG; }
+    const int64_t nx = h.lat.n[0], ny = h.lat.n[1], nz = h.dim == 3 ? h.lat.n[2] : 1;
+    h.n_cells = nx * ny * nz;
+    h.n_nodes = h.dim == 3 ? (nx + 1) * (ny + 1) * (nz + 1) : (nx + 1) * (ny + 1);
+    h.cell_elem.assign((size_t)h.n_cells, -1);
+    h.node_adof.assign((size_t)h.n_nodes, -1);
+    for (int32_t e = 0; e < h.n_el; ++e) {
+        const int64_t c = d.elem_cell[e];
+        if (c < 0 || c >= h.n_cells || h.cell_elem[(size_t)c] >= 0) {
+            *err = "elem_cell out of range or duplicated";
+            return MGPU_ERR_INVALID_ARG;
+        }
+        h.cell_elem[(size_t)c] = e;
+        const int64_t i = c % nx, j = (c / nx) % ny, k = c / (nx * ny);
+        // the IP box must be the lattice cell
+        const int64_t ijk[3] = {i, j, k};
+        for (int q = 0; q < h.dim; ++q) {
+            const double lo = h.lat.lo[q] + (double)ijk[q] * h.lat.h[q];
+            if (d.ip_box[(size_t)e * 2 * h.dim + q] != lo || d.ip_box[(size_t)e * 2 * h.dim + h.dim + q] != h.lat.h[q]) {
+                *err = "ip_box does not match the lattice cell";
+                return MGPU_ERR_INVALID_ARG;
+            }
+        }
+        for (int b = 0; b < nb; ++b) {
+            const int* off = h.dim == 3 ? kHex8Off[b] : kQuad4Off[b];
+            const int64_t nid = h.dim == 3 ? (i + off[0]) + (nx + 1) * ((j + off[1]) + (ny + 1) * (k + off[2]))
+                                           : (i + off[0]) + (nx + 1) * (j + off[1]);
+            const int32_t a = d.t_adof[d.t_ptr[(size_t)e * nb + b]];
+            if (h.node_adof[(size_t)nid] >= 0 && h.node_adof[(size_t)nid] != a) {
+                *err = "lattice node mapped to two adofs";
+                return MGPU_ERR_UNSUPPORTED;
+            }
+            h.node_adof[(size_t)nid] = a;
+        }
+    }
+    h.row_node.assign((size_t)N, -1);
+    int32_t last = -1;
+    for (int64_t nid = 0; nid < h.n_nodes; ++nid) {
+        const int32_t a = h.node_adof[(size_t)nid];
+        if (a < 0) continue;
+        if (a <= last) {
+            *err = "adof ids not ascending in lattice node order";
+            return MGPU_ERR_UNSUPPORTED;
+        }
+        last = a;
+        h.row_node[(size_t)a] = (int32_t)nid;
+    }
+    for (int32_t a = 0; a < N; ++a)
+        if (h.row_node[(size_t)a] < 0) { *err = "adof not used by any element"; return MGPU_ERR_INVALID_ARG; }
+    return MGPU_OK;
+}
+
+}  // namespace mgpu

@@ -1,0 +1,46 @@
+// set_internal.hpp -- the state behind an mgpu_set handle: host copies of the descriptor arrays the
+// set needs after creation, the CSR pattern, and device buffers.  Mirrors what fem::Set and
+// MSI::Equation_Set own (cl_FEM_Set.hpp:64, cl_MSI_Equation_Set.hpp:61-62) plus the SOL graph
+// (cl_Sparse_Matrix_EpetraFECrs.cpp:195-240).
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../../include/moris_gpu.h"
+
+namespace mgpu {
+
+// Host-side products of set creation (pure C++, no HIP).
+struct SetHost {
+    int dim = 3;
+    int iwg = MGPU_IWG_SPATIALDIFF_BULK;
+    int nf = 1;            // fields (dof types)
+    int nb = 8;            // bases per field
+    int na = 8;            // element adofs = nf * nb
+    int nm = 1;            // material parameters per element
+    int32_t n_el = 0;
+    int32_t n_adofs_type = 0;
+    int32_t n_rows = 0;    // nf * n_adofs_type
+    std::vector<int32_t> topo;       // [n_el][na] sorted unique per type, global ids
+    std::vector<int32_t> row_ptr;    // [n_rows + 1]
+    std::vector<int32_t> col_idx;    // [nnz]
+    // element vertex b -> position of its adof in the sorted per-type topology
+    std::vector<uint8_t> vert_pos;   // [n_el][nb]
+    // lattice (stencil) path
+    bool lattice = false;
+    mgpu_lattice lat{};
+    int64_t n_cells = 0, n_nodes = 0;
+    std::vector<int32_t> cell_elem;  // [n_cells] element or -1
+    std::vector<int32_t> node_adof;  // [n_nodes] type-local adof or -1
+    std::vector<int32_t> row_node;   // [n_adofs_type] lattice node of each adof
+    // cut elements
+    std::vector<int32_t> cut_elems;  // element ids with sub-cells
+    std::vector<int32_t> elem_cut;   // [n_el] cut index or -1
+};
+
+// Build topology, pattern and the lattice maps from a descriptor.  Returns MGPU_OK or an error
+// code with *err set.
+mgpu_status build_set_host(const mgpu_set_desc& d, SetHost& h, std::string* err);
+
+}  // namespace mgpu

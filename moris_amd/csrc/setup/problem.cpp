@@ -1,0 +1,436 @@
+// problem.cpp -- host set-up of one workload: level-0 HMR lattice, analytic GEN level set, XTK cut
+// of every intersected cell, MSI adof numbering and the element-major set descriptor consumed by
+// mgpu_set_create.  This is the input side of the hot path; nothing here is timed.
+//
+//   intersected background cells   Geometry_Engine::is_intersected, cl_GEN_Geometry_Engine.cpp:278-300
+//   cut of one cell                common/xtk_cut.hpp
+//   phase of a cell                material = phase 1 (phi > 0) by default; blocks *_n_p1 (uncut)
+//                                  and *_c_p1 (cut), UT_MDL_XTK_HMR_Linear_Struc_2D.cpp:355-359
+//   adof ids                       Dof_Manager::set_owned_adofs_ids_by_type,
+//                                  cl_MSI_Dof_Manager.cpp:703-745: by dof type, then ascending
+//                                  basis index over the bases some element uses
+//   T rows of p=1 B-splines        one entry of weight 1 per Lagrange node (HMR level 0)
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../common/fem_tables.hpp"
+#include "../common/levelset.hpp"
+#include "../common/xtk_cut.hpp"
+#include "internal.hpp"
+
+using namespace mgpu;
+
+struct mgpu_problem {
+    mgpu_problem_spec spec{};
+    std::vector<double> spheres;
+    LevelSet ls;
+    int dim = 3;
+    int n[3] = {1, 1, 1};
+    double lo[3] = {0, 0, 0};
+    double h[3] = {1, 1, 1};
+    int nb = 8;
+    int64_t n_cells = 0, n_nodes = 0, n_faces = 0;
+    int64_t n_cut = 0, n_void = 0;
+
+    std::vector<double> node_phi;
+    std::vector<int32_t> cell_elem;
+
+    std::vector<int32_t> elem_cell;
+    std::vector<double> ip_box, material, vertex_phi;
+    std::vector<int32_t> t_ptr, t_adof;
+    std::vector<double> t_weight;
+    std::vector<int32_t> ig_ptr;
+    std::vector<double> ig_xi;
+    int32_t n_adofs_per_type = 0;
+};
+
+namespace {
+
+int64_t node_index(const mgpu_problem& p, int i, int j, int k) {
+    return (int64_t)i + (int64_t)(p.n[0] + 1) * ((int64_t)j + (int64_t)(p.n[1] + 1) * k);
+}
+
+void node_coords(const mgpu_problem& p, int i, int j, int k, double* x) {
+    const int ijk[3] = {i, j, k};
+    for (int d = 0; d < p.dim; ++d) x[d] = p.lo[d] + (double)ijk[d] * p.h[d];
+    if (p.dim == 2) x[2] = 0.0;
+}
+
+// Trilinear / bilinear Lagrange basis at xi (vertex order of kHex8XiHat / kQuad4XiHat).
+void lagrange_N(int dim, const double* xi, double* N) {
+    if (dim == 3) {
+        for (int k = 0; k < 8; ++k)
+            N[k] = 0.125 * (1.0 + xi[0] * kHex8XiHat[k][0]) * (1.0 + xi[1] * kHex8XiHat[k][1]) *
+                   (1.0 + xi[2] * kHex8XiHat[k][2]);
+    } else {
+        for (int k = 0; k < 4; ++k)
+            N[k] = 0.25 * (1.0 + xi[0] * kQuad4XiHat[k][0]) * (1.0 + xi[1] * kQuad4XiHat[k][1]);
+    }
+}
+
+// Physical coordinates of the point with parametric coordinate xi in cell (i,j,k): N(xi) . X.
+void cell_point(const mgpu_problem& p, int i, int j, int k, const double* xi, double* x) {
+    double N[8];
+    lagrange_N(p.dim, xi, N);
+    const int nb = p.dim == 3 ? 8 : 4;
+    for (int d = 0; d < 3; ++d) x[d] = 0.0;
+    for (int v = 0; v < nb; ++v) {
+        double xv[3];
+        if (p.dim == 3)
+            node_coords(p, i + kHex8Off[v][0], j + kHex8Off[v][1], k + kHex8Off[v][2], xv);
+        else
+            node_coords(p, i + kQuad4Off[v][0], j + kQuad4Off[v][1], 0, xv);
+        for (int d = 0; d < p.dim; ++d) x[d] += N[v] * xv[d];
+    }
+}
+
+// Background cell intersected: its lattice vertices do not all lie in one region
+// (Geometry_Engine::is_intersected, cl_GEN_Geometry_Engine.cpp:278-300).
+bool cell_intersected(const mgpu_problem& p, int i, int j, int k) {
+    const int nb = p.dim == 3 ? 8 : 4;
+    int r0 = -1;
+    for (int v = 0; v < nb; ++v) {
+        const int64_t nid = p.dim == 3 ? node_index(p, i + kHex8Off[v][0], j + kHex8Off[v][1], k + kHex8Off[v][2])
+                                       : node_index(p, i + kQuad4Off[v][0], j + kQuad4Off[v][1], 0);
+        const int r = ls_region(p.node_phi[(size_t)nid]);
+        if (v == 0) r0 = r;
+        else if (r != r0) return true;
+    }
+    return false;
+}
+
+struct CellOut {
+    int32_t cell;
+    bool cut;
+    std::vector<double> xi;   // sub-cell vertex parametric coords
+    int n_sub;
+};
+
+// Process a contiguous range of cells; material cells appended in order.
+int process_cells(const mgpu_problem& p, int64_t c0, int64_t c1, std::vector<CellOut>& out,
+                  int64_t& n_void, std::string& err) {
+    const int phase = p.spec.material_phase;
+    static thread_local double sub3[kMaxSubTets][4][3];
+    static thread_local double sub2[kMaxSubTris][3][2];
+    for (int64_t c = c0; c < c1; ++c) {
+        const int i = (int)(c % p.n[0]);
+        const int j = (int)((c / p.n[0]) % p.n[1]);
+        const int k = p.dim == 3 ? (int)(c / ((int64_t)p.n[0] * p.n[1])) : 0;
+        const int nb = p.nb;
+        int regions[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        bool any_diff = false;
+        for (int v = 0; v < nb; ++v) {
+            const int64_t nid = p.dim == 3
+                                    ? node_index(p, i + kHex8Off[v][0], j + kHex8Off[v][1], k + kHex8Off[v][2])
+                                    : node_index(p, i + kQuad4Off[v][0], j + kQuad4Off[v][1], 0);
+            regions[v] = ls_region(p.node_phi[nid]);
+            if (regions[v] != regions[0]) any_diff = true;
+        }
+        if (!any_diff) {
+            if (regions[0] == phase) out.push_back(CellOut{(int32_t)c, false, {}, 0});
+            else ++n_void;
+            continue;
+        }
+        int n_sub = 0;
+        int rc;
+        CellOut co{(int32_t)c, true, {}, 0};
+        if (p.dim == 3) {
+            HexSubdivVertex hv[15];
+            for (int v = 0; v < 8; ++v) {
+                for (int d = 0; d < 3; ++d) hv[v].xi[d] = kHex8XiHat[v][d];
+                const int64_t nid = node_index(p, i + kHex8Off[v][0], j + kHex8Off[v][1], k + kHex8Off[v][2]);
+                hv[v].phi = p.node_phi[nid];
+                hv[v].gid = (uint32_t)nid;
+            }
+            // face nodes: coordinates interpolated in the lowest-index intersected cell sharing
+            // the face (the cell that first requests the node when cells are decomposed in order)
+            const int64_t nxf = (int64_t)(p.n[0] + 1) * p.n[1] * p.n[2];
+            const int64_t nyf = (int64_t)p.n[0] * (p.n[1] + 1) * p.n[2];
+            for (int f = 0; f < 6; ++f) {
+                HexSubdivVertex& fv = hv[8 + f];
+                for (int d = 0; d < 3; ++d) fv.xi[d] = kHexFaceXi[f][d];
+                int pi = i, pj = j, pk = k, pf = f;
+                int64_t fid;
+                switch (f) {
+                    case 0: if (j > 0 && cell_intersected(p, i, j - 1, k)) { pj = j - 1; pf = 2; }
+                            fid = nxf + i + (int64_t)p.n[0] * (j + (int64_t)(p.n[1] + 1) * k); break;
+                    case 1: fid = (i + 1) + (int64_t)(p.n[0] + 1) * (j + (int64_t)p.n[1] * k); break;
+                    case 2: fid = nxf + i + (int64_t)p.n[0] * ((j + 1) + (int64_t)(p.n[1] + 1) * k); break;
+                    case 3: if (i > 0 && cell_intersected(p, i - 1, j, k)) { pi = i - 1; pf = 1; }
+                            fid = i + (int64_t)(p.n[0] + 1) * (j + (int64_t)p.n[1] * k); break;
+                    case 4: if (k > 0 && cell_intersected(p, i, j, k - 1)) { pk = k - 1; pf = 5; }
+                            fid = nxf + nyf + i + (int64_t)p.n[0] * (j + (int64_t)p.n[1] * k); break;
+                    default: fid = nxf + nyf + i + (int64_t)p.n[0] * (j + (int64_t)p.n[1] * (k + 1)); break;
+                }
+                double x[3];
+                cell_point(p, pi, pj, pk, kHexFaceXi[pf], x);
+                fv.phi = p.ls(x);
+                fv.gid = (uint32_t)(p.n_nodes + fid);
+            }
+            HexSubdivVertex& cv = hv[14];
+            const double zero[3] = {0.0, 0.0, 0.0};
+            for (int d = 0; d < 3; ++d) cv.xi[d] = 0.0;
+            double x[3];
+            cell_point(p, i, j, k, zero, x);
+            cv.phi = p.ls(x);
+            cv.gid = (uint32_t)(p.n_nodes + p.n_faces + c);
+            rc = cut_hex8(hv, phase, sub3, &n_sub);
+            if (rc == kCutOk) co.xi.assign(&sub3[0][0][0], &sub3[0][0][0] + (size_t)n_sub * 12);
+        } else {
+            QuadSubdivVertex qv[5];
+            for (int v = 0; v < 4; ++v) {
+                for (int d = 0; d < 2; ++d) qv[v].xi[d] = kQuad4XiHat[v][d];
+                const int64_t nid = node_index(p, i + kQuad4Off[v][0], j + kQuad4Off[v][1], 0);
+                qv[v].phi = p.node_phi[nid];
+                qv[v].gid = (uint32_t)nid;
+            }
+            const double zero[2] = {0.0, 0.0};
+            double x[3];
+            cell_point(p, i, j, 0, zero, x);
+            qv[4].xi[0] = qv[4].xi[1] = 0.0;
+            qv[4].phi = p.ls(x);
+            qv[4].gid = (uint32_t)(p.n_nodes + c);
+            rc = cut_quad4(qv, phase, sub2, &n_sub);
+            if (rc == kCutOk) co.xi.assign(&sub2[0][0][0], &sub2[0][0][0] + (size_t)n_sub * 6);
+        }
+        if (rc != kCutOk) {
+            err = "node-hierarchy template not found in cell " + std::to_string(c);
+            return MGPU_ERR_GEOMETRY;
+        }
+        if (n_sub == 0) { ++n_void; continue; }
+        co.n_sub = n_sub;
+        out.push_back(std::move(co));
+    }
+    return MGPU_OK;
+}
+
+int n_threads() {
+    unsigned t = std::thread::hardware_concurrency();
+    if (t == 0) t = 4;
+    return (int)std::min(t, 32u);
+}
+
+}  // namespace
+
+extern "C" mgpu_status mgpu_problem_build(const mgpu_problem_spec* spec, mgpu_problem** out) {
+    MGPU_GUARD_BEGIN
+    if (!spec || !out) return mgpu::set_error(MGPU_ERR_INVALID_ARG, "null argument");
+    *out = nullptr;
+    if (spec->dim != 2 && spec->dim != 3) return mgpu::set_error(MGPU_ERR_INVALID_ARG, "dim must be 2 or 3");
+    if (spec->iwg != MGPU_IWG_SPATIALDIFF_BULK && spec->iwg != MGPU_IWG_STRUC_LINEAR_BULK)
+        return mgpu::set_error(MGPU_ERR_UNSUPPORTED, "iwg not supported");
+    for (int d = 0; d < spec->dim; ++d)
+        if (spec->n[d] < 1 || !(spec->hi[d] > spec->lo[d]))
+            return mgpu::set_error(MGPU_ERR_INVALID_ARG, "bad grid extent");
+    if (spec->ls_kind == MGPU_LS_SPHERES && (spec->n_spheres < 1 || !spec->spheres))
+        return mgpu::set_error(MGPU_ERR_INVALID_ARG, "sphere level set without spheres");
+
+    auto p = std::make_unique<mgpu_problem>();
+    p->spec = *spec;
+    p->dim = spec->dim;
+    for (int d = 0; d < 3; ++d) {
+        p->n[d] = d < p->dim ? spec->n[d] : 1;
+        p->lo[d] = d < p->dim ? spec->lo[d] : 0.0;
+        p->h[d] = d < p->dim ? (spec->hi[d] - spec->lo[d]) / (double)spec->n[d] : 1.0;
+    }
+    if (spec->ls_kind == MGPU_LS_SPHERES) {
+        p->spheres.assign(spec->spheres, spec->spheres + 4 * (size_t)spec->n_spheres);
+        p->spec.spheres = p->spheres.data();
+    } else {
+        p->spec.spheres = nullptr;
+    }
+    p->ls.kind = spec->ls_kind;
+    p->ls.dim = p->dim;
+    p->ls.n_spheres = spec->n_spheres;
+    p->ls.spheres = p->spheres.data();
+    p->ls.gyroid_scale = spec->gyroid_scale;
+    for (int d = 0; d < 4; ++d) p->ls.plane[d] = spec->plane[d];
+    p->nb = p->dim == 3 ? 8 : 4;
+    p->n_cells = (int64_t)p->n[0] * p->n[1] * p->n[2];
+    p->n_nodes = p->dim == 3 ? (int64_t)(p->n[0] + 1) * (p->n[1] + 1) * (p->n[2] + 1)
+                             : (int64_t)(p->n[0] + 1) * (p->n[1] + 1);
+    p->n_faces = p->dim == 3 ? (int64_t)(p->n[0] + 1) * p->n[1] * p->n[2] +
+                                   (int64_t)p->n[0] * (p->n[1] + 1) * p->n[2] +
+                                   (int64_t)p->n[0] * p->n[1] * (p->n[2] + 1)
+                             : 0;
+    if (p->n_nodes + p->n_faces + p->n_cells >= (int64_t)UINT32_MAX || p->n_cells >= INT32_MAX)
+        return mgpu::set_error(MGPU_ERR_INVALID_ARG, "grid too large for 32-bit ids");
+
+    const int T = n_threads();
+    // 1. level set at lattice nodes
+    p->node_phi.resize((size_t)p->n_nodes);
+    {
+        std::vector<std::thread> th;
+        for (int t = 0; t < T; ++t)
+            th.emplace_back([&, t]() {
+                const int64_t a = p->n_nodes * t / T, b = p->n_nodes * (t + 1) / T;
+                for (int64_t nid = a; nid < b; ++nid) {
+                    const int i = (int)(nid % (p->n[0] + 1));
+                    const int j = (int)((nid / (p->n[0] + 1)) % (p->n[1] + 1));
+                    const int k = p->dim == 3 ? (int)(nid / ((int64_t)(p->n[0] + 1) * (p->n[1] + 1))) : 0;
+                    double x[3];
+                    node_coords(*p, i, j, k, x);
+                    p->node_phi[(size_t)nid] = p->ls(x);
+                }
+            });
+        for (auto& t : th) t.join();
+    }
+    // 2. cut every cell (chunks in lattice order)
+    std::vector<std::vector<CellOut>> parts(T);
+    std::vector<int64_t> voids(T, 0);
+    std::vector<int> rcs(T, MGPU_OK);
+    std::vector<std::string> errs(T);
+    {
+        std::vector<std::thread> th;
+        for (int t = 0; t < T; ++t)
+            th.emplace_back([&, t]() {
+                const int64_t a = p->n_cells * t / T, b = p->n_cells * (t + 1) / T;
+                rcs[t] = process_cells(*p, a, b, parts[t], voids[t], errs[t]);
+            });
+        for (auto& t : th) t.join();
+    }
+    for (int t = 0; t < T; ++t)
+        if (rcs[t] != MGPU_OK) return mgpu::set_error(rcs[t], errs[t]);
+
+    // 3. element arrays
+    int64_t n_el = 0, n_ig = 0;
+    for (int t = 0; t < T; ++t) {
+        n_el += (int64_t)parts[t].size();
+        p->n_void += voids[t];
+        for (const auto& co : parts[t]) n_ig += co.n_sub;
+    }
+    const int d2 = 2 * p->dim;
+    const int nv_ig = p->dim == 3 ? 4 : 3;
+    p->cell_elem.assign((size_t)p->n_cells, -1);
+    p->elem_cell.resize((size_t)n_el);
+    p->ip_box.resize((size_t)n_el * d2);
+    p->vertex_phi.resize((size_t)n_el * p->nb);
+    p->ig_ptr.resize((size_t)n_el + 1);
+    p->ig_xi.resize((size_t)n_ig * nv_ig * p->dim);
+    const bool diff = spec->iwg == MGPU_IWG_SPATIALDIFF_BULK;
+    p->material.resize((size_t)n_el * (diff ? 1 : 2));
+    int64_t e = 0, ig = 0;
+    for (int t = 0; t < T; ++t) {
+        for (const auto& co : parts[t]) {
+            const int64_t c = co.cell;
+            const int i = (int)(c % p->n[0]);
+            const int j = (int)((c / p->n[0]) % p->n[1]);
+            const int k = p->dim == 3 ? (int)(c / ((int64_t)p->n[0] * p->n[1])) : 0;
+            const int ijk[3] = {i, j, k};
+            p->cell_elem[(size_t)c] = (int32_t)e;
+            p->elem_cell[(size_t)e] = (int32_t)c;
+            for (int d = 0; d < p->dim; ++d) {
+                p->ip_box[(size_t)e * d2 + d] = p->lo[d] + (double)ijk[d] * p->h[d];
+                p->ip_box[(size_t)e * d2 + p->dim + d] = p->h[d];
+            }
+            for (int v = 0; v < p->nb; ++v) {
+                const int64_t nid = p->dim == 3 ? node_index(*p, i + kHex8Off[v][0], j + kHex8Off[v][1], k + kHex8Off[v][2])
+                                                : node_index(*p, i + kQuad4Off[v][0], j + kQuad4Off[v][1], 0);
+                p->vertex_phi[(size_t)e * p->nb + v] = p->node_phi[(size_t)nid];
+            }
+            if (diff) {
+                p->material[(size_t)e] = spec->k_sigma == 0.0
+                                             ? std::exp(spec->k_mu)
+                                             : std::exp(spec->k_mu + spec->k_sigma * normal_at(spec->seed, (uint64_t)c));
+            } else {
+                p->material[(size_t)e * 2] = spec->E;
+                p->material[(size_t)e * 2 + 1] = spec->nu;
+            }
+            p->ig_ptr[(size_t)e] = (int32_t)ig;
+            if (co.cut) {
+                std::memcpy(&p->ig_xi[(size_t)ig * nv_ig * p->dim], co.xi.data(), co.xi.size() * sizeof(double));
+                ig += co.n_sub;
+                ++p->n_cut;
+            }
+            ++e;
+        }
+    }
+    p->ig_ptr[(size_t)n_el] = (int32_t)ig;
+
+    // 4. adof numbering over used lattice nodes, ascending node index
+    std::vector<int32_t> node_adof((size_t)p->n_nodes, -1);
+    for (int64_t ee = 0; ee < n_el; ++ee) {
+        const int64_t c = p->elem_cell[(size_t)ee];
+        const int i = (int)(c % p->n[0]);
+        const int j = (int)((c / p->n[0]) % p->n[1]);
+        const int k = p->dim == 3 ? (int)(c / ((int64_t)p->n[0] * p->n[1])) : 0;
+        for (int v = 0; v < p->nb; ++v) {
+            const int64_t nid = p->dim == 3 ? node_index(*p, i + kHex8Off[v][0], j + kHex8Off[v][1], k + kHex8Off[v][2])
+                                            : node_index(*p, i + kQuad4Off[v][0], j + kQuad4Off[v][1], 0);
+            node_adof[(size_t)nid] = 0;
+        }
+    }
+    int32_t next = 0;
+    for (int64_t nid = 0; nid < p->n_nodes; ++nid)
+        if (node_adof[(size_t)nid] == 0) node_adof[(size_t)nid] = next++;
+    p->n_adofs_per_type = next;
+
+    // 5. T rows: one entry of weight 1 per IP-cell vertex
+    p->t_ptr.resize((size_t)n_el * p->nb + 1);
+    p->t_adof.resize((size_t)n_el * p->nb);
+    p->t_weight.assign((size_t)n_el * p->nb, 1.0);
+    for (int64_t ee = 0; ee < n_el; ++ee) {
+        const int64_t c = p->elem_cell[(size_t)ee];
+        const int i = (int)(c % p->n[0]);
+        const int j = (int)((c / p->n[0]) % p->n[1]);
+        const int k = p->dim == 3 ? (int)(c / ((int64_t)p->n[0] * p->n[1])) : 0;
+        for (int v = 0; v < p->nb; ++v) {
+            const int64_t nid = p->dim == 3 ? node_index(*p, i + kHex8Off[v][0], j + kHex8Off[v][1], k + kHex8Off[v][2])
+                                            : node_index(*p, i + kQuad4Off[v][0], j + kQuad4Off[v][1], 0);
+            const size_t r = (size_t)ee * p->nb + v;
+            p->t_ptr[r] = (int32_t)r;
+            p->t_adof[r] = node_adof[(size_t)nid];
+        }
+    }
+    p->t_ptr[(size_t)n_el * p->nb] = (int32_t)((size_t)n_el * p->nb);
+
+    *out = p.release();
+    return MGPU_OK;
+    MGPU_GUARD_END
+}
+
+extern "C" mgpu_status mgpu_problem_desc(const mgpu_problem* p, mgpu_set_desc* d) {
+    if (!p || !d) return mgpu::set_error(MGPU_ERR_INVALID_ARG, "null argument");
+    std::memset(d, 0, sizeof(*d));
+    d->spatial_dim = p->dim;
+    d->iwg = p->spec.iwg;
+    d->ip_geometry = p->dim == 3 ? MGPU_GEOM_HEX8 : MGPU_GEOM_QUAD4;
+    d->ig_geometry = p->dim == 3 ? MGPU_GEOM_TET4 : MGPU_GEOM_TRI3;
+    d->n_elements = (int32_t)p->elem_cell.size();
+    d->n_bases = p->nb;
+    d->n_adofs_per_type = p->n_adofs_per_type;
+    d->ip_box = p->ip_box.data();
+    d->material = p->material.data();
+    d->t_ptr = p->t_ptr.data();
+    d->t_adof = p->t_adof.data();
+    d->t_weight = p->t_weight.data();
+    d->ig_ptr = p->ig_ptr.data();
+    d->ig_xi = p->ig_xi.data();
+    for (int k = 0; k < 3; ++k) {
+        d->lattice.n[k] = p->n[k];
+        d->lattice.lo[k] = p->lo[k];
+        d->lattice.h[k] = p->h[k];
+    }
+    d->elem_cell = p->elem_cell.data();
+    d->vertex_phi = p->vertex_phi.data();
+    return MGPU_OK;
+}
+
+extern "C" mgpu_status mgpu_problem_counts(const mgpu_problem* p, int64_t counts[6]) {
+    if (!p || !counts) return mgpu::set_error(MGPU_ERR_INVALID_ARG, "null argument");
+    counts[0] = (int64_t)p->elem_cell.size();
+    counts[1] = p->n_cut;
+    counts[2] = p->ig_ptr.empty() ? 0 : p->ig_ptr.back();
+    counts[3] = p->n_adofs_per_type;
+    counts[4] = p->n_cells;
+    counts[5] = p->n_void;
+    return MGPU_OK;
+}
+
+extern "C" mgpu_status mgpu_problem_destroy(mgpu_problem* p) {
+    delete p;
+    return MGPU_OK;
+}

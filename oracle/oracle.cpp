@@ -1,0 +1,1093 @@
+// oracle.cpp -- TEST INFRASTRUCTURE ONLY.  FP64 CPU restatement of MORIS's bulk K+R assembly and
+// of the set-up that feeds it (XTK cut of a level-0 HMR lattice, MSI adof numbering, CSR graph).
+// It is the checker of libmoris_gpu.so: only tests/, __graft_entry__.smoke() and bench.py's
+// cpu_baseline leg load it.  It is written in the reference's own flow (global vertex/edge maps,
+// template lookup by permutation id, dense T, dense B/D, per-entry row search on scatter) and
+// shares no code with the product except the public C header for the spec/descriptor structs.
+//
+// Reference functions restated (file:line under projects/):
+//   fill_matrix_and_RHS loop           SOL/DLA/src/cl_DLA_Solver_Interface.cpp:317-365
+//   K_e = T^T K_p T, R_e = T^T R_p      FEM/MSI/src/cl_MSI_Equation_Object.cpp:789-842, 426-493
+//   element topology (sorted unique)   FEM/MSI/src/cl_MSI_Equation_Object.cpp:199-291, 505-554
+//   pdof values u_p = T u_adof         FEM/MSI/src/cl_MSI_Equation_Object.cpp:1100-1131
+//   IG-cell loop, w* = w detJ, skip    FEM/INT/src/ELEM/cl_FEM_Element_Bulk.cpp:258-355
+//   IG->IP point map                   MTK/src/interpolation/cl_MTK_Space_Interpolator.cpp:1381-1397
+//   detJ hex / tet / tri                cl_MTK_Space_Interpolator.cpp:1006-1040, 1180-1210, 1120-1128
+//   rectangular inverse Jacobian       cl_MTK_Space_Interpolator.cpp:676-700
+//   dN/dx                              FEM/INT/src/IP/cl_FEM_Field_Interpolator.cpp:464-496
+//   test strain / strain / flux        FEM/INT/src/CM/cl_FEM_CM_Struc_Linear.cpp:280, 349-590
+//   D (3D, plane stress)               FEM/INT/src/CM/cl_FEM_CM_Struc_Linear_Isotropic.cpp:248-259, 374-390
+//   diffusion flux / test strain       FEM/INT/src/CM/cl_FEM_CM_Diffusion_Linear_Isotropic.cpp:95-119, 302-350
+//   IWG residual / jacobian            FEM/INT/src/IWG/cl_FEM_IWG_Isotropic_Struc_Linear_Bulk.cpp:43-205,
+//                                      FEM/INT/src/IWG/cl_FEM_IWG_Diffusion_Bulk.cpp:51-323
+//   regular subdivision                XTK/src/cl_XTK_Regular_Subdivision_Interface.hpp:110-360
+//   node hierarchy                     XTK/src/cl_XTK_Node_Hierarchy_Interface.cpp:160-287, 672-968
+//   templates                          XTK/src/cl_XTK_Node_Hierarchy_Template_Library.cpp:29-836
+//   edge intersection                  GEN/src/cl_GEN_Interpolation.hpp:107-132,
+//                                      GEN/src/PDV/cl_GEN_Intersection_Node.cpp:33,78-100
+//   adof ids by type                   FEM/MSI/src/cl_MSI_Dof_Manager.cpp:703-745
+//   CSR graph                          SOL/SOL_CORE/src/cl_Sparse_Matrix_EpetraFECrs.cpp:195-240
+//   SumIntoGlobalValues                cl_Sparse_Matrix_EpetraFECrs.cpp:103-133
+//   quadrature tables                  MTK/src/integration/cl_MTK_Integration_Coeffs_{Quad_2x2,
+//                                      Hex_2x2x2,Tri_7,Tet_11}.hpp
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../include/moris_gpu.h"
+
+namespace {
+
+// ------------------------------------------------------------------------------------------------
+// quadrature (reference literals, point-major here)
+// ------------------------------------------------------------------------------------------------
+const double G2 = 0.577350269189626;
+const double kQuad2[4][2] = {{-G2, -G2}, {G2, -G2}, {G2, G2}, {-G2, G2}};
+const double kHex2[8][3] = {{-G2, -G2, -G2}, {G2, -G2, -G2}, {G2, G2, -G2}, {-G2, G2, -G2},
+                            {-G2, -G2, G2},  {G2, -G2, G2},  {G2, G2, G2},  {-G2, G2, G2}};
+const double kTri7P[7][2] = {{0.333333333333333, 0.333333333333333}, {0.797426985353087, 0.101286507323456},
+                             {0.101286507323456, 0.797426985353087}, {0.101286507323456, 0.101286507323456},
+                             {0.059715871789770, 0.470142064105115}, {0.470142064105115, 0.059715871789770},
+                             {0.470142064105115, 0.470142064105115}};
+const double kTri7W[7] = {0.225000000000000, 0.125939180544827, 0.125939180544827, 0.125939180544827,
+                          0.132394152788506, 0.132394152788506, 0.132394152788506};
+const double A11 = 0.7857142857142860, B11 = 0.0714285714285714, C11 = 0.3994035761667990,
+             D11 = 0.1005964238332010;
+const double kTet11P[11][3] = {{A11, B11, B11}, {B11, A11, B11}, {B11, B11, B11}, {B11, B11, A11},
+                               {C11, C11, D11}, {C11, D11, D11}, {C11, D11, C11}, {D11, C11, D11},
+                               {D11, C11, C11}, {D11, D11, C11}, {0.25, 0.25, 0.25}};
+double tet11_weight(int q) {
+    const double w[11] = {0.0076222222222222, 0.0076222222222222, 0.0076222222222222, 0.0076222222222222,
+                          0.0248888888888889, 0.0248888888888889, 0.0248888888888889, 0.0248888888888889,
+                          0.0248888888888889, 0.0248888888888889, -0.0131555555555556};
+    return 6.0 * w[q];
+}
+
+// hex8 / quad4 parametric vertices
+const int kHexV[8][3] = {{-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {-1, 1, -1},
+                         {-1, -1, 1},  {1, -1, 1},  {1, 1, 1},  {-1, 1, 1}};
+const int kQuadV[4][2] = {{-1, -1}, {1, -1}, {1, 1}, {-1, 1}};
+
+void shape_N(int dim, const double* xi, double* N) {
+    if (dim == 3) {
+        const double x = xi[0], y = xi[1], z = xi[2];
+        // cl_MTK_Interpolation_Function_Lagrange_Hex8.hpp eval_N
+        N[0] = -(y - 1.0) * (x - 1.0) * (z - 1.0) * 0.125;
+        N[1] = (y - 1.0) * (x + 1.0) * (z - 1.0) * 0.125;
+        N[2] = -(y + 1.0) * (x + 1.0) * (z - 1.0) * 0.125;
+        N[3] = (y + 1.0) * (x - 1.0) * (z - 1.0) * 0.125;
+        N[4] = (y - 1.0) * (x - 1.0) * (z + 1.0) * 0.125;
+        N[5] = -(y - 1.0) * (x + 1.0) * (z + 1.0) * 0.125;
+        N[6] = (y + 1.0) * (x + 1.0) * (z + 1.0) * 0.125;
+        N[7] = -(y + 1.0) * (x - 1.0) * (z + 1.0) * 0.125;
+    } else {
+        const double x = xi[0], y = xi[1];
+        N[0] = (1.0 - x) * (1.0 - y) * 0.25;
+        N[1] = (1.0 + x) * (1.0 - y) * 0.25;
+        N[2] = (1.0 + x) * (1.0 + y) * 0.25;
+        N[3] = (1.0 - x) * (1.0 + y) * 0.25;
+    }
+}
+
+// dN/dxi [dim][nb]
+void shape_dN(int dim, const double* xi, double dN[3][8]) {
+    if (dim == 3) {
+        for (int a = 0; a < 8; ++a) {
+            const double sx = kHexV[a][0], sy = kHexV[a][1], sz = kHexV[a][2];
+            dN[0][a] = 0.125 * sx * (1.0 + sy * xi[1]) * (1.0 + sz * xi[2]);
+            dN[1][a] = 0.125 * sy * (1.0 + sx * xi[0]) * (1.0 + sz * xi[2]);
+            dN[2][a] = 0.125 * sz * (1.0 + sx * xi[0]) * (1.0 + sy * xi[1]);
+        }
+    } else {
+        for (int a = 0; a < 4; ++a) {
+            const double sx = kQuadV[a][0], sy = kQuadV[a][1];
+            dN[0][a] = 0.25 * sx * (1.0 + sy * xi[1]);
+            dN[1][a] = 0.25 * sy * (1.0 + sx * xi[0]);
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+// level sets (analytic GEN fields)
+// ------------------------------------------------------------------------------------------------
+struct LS {
+    int kind = MGPU_LS_NONE, dim = 3;
+    std::vector<double> spheres;
+    double gs = 1.0;
+    double plane[4] = {0, 0, 0, 0};
+    double eval(const double* x) const {
+        if (kind == MGPU_LS_SPHERES) {
+            double best = INFINITY;
+            for (size_t i = 0; i < spheres.size() / 4; ++i) {
+                const double* s = &spheres[4 * i];
+                double r2 = 0.0;
+                for (int d = 0; d < dim; ++d) r2 += (x[d] - s[d]) * (x[d] - s[d]);
+                best = std::min(best, std::sqrt(r2) - s[3]);
+            }
+            return best;
+        }
+        if (kind == MGPU_LS_GYROID) {
+            const double a = gs * x[0], b = gs * x[1], c = dim == 3 ? gs * x[2] : 0.0;
+            return std::sin(a) * std::cos(b) + std::sin(b) * std::cos(c) + std::sin(c) * std::cos(a);
+        }
+        if (kind == MGPU_LS_PLANE) {
+            double v = plane[3];
+            for (int d = 0; d < dim; ++d) v += plane[d] * x[d];
+            return v;
+        }
+        return 1.0;
+    }
+};
+int region(double phi) {
+    if (std::fabs(phi) <= 1e-12) return 2;
+    return phi < 0 ? 0 : 1;
+}
+
+uint64_t smix(uint64_t seed, uint64_t idx) {
+    uint64_t z = seed + (idx + 1) * 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+double gauss_draw(uint64_t seed, uint64_t cell) {
+    const double u1 = 1.0 - (double)(smix(seed, 2 * cell) >> 11) * (1.0 / 9007199254740992.0);
+    const double u2 = (double)(smix(seed, 2 * cell + 1) >> 11) * (1.0 / 9007199254740992.0);
+    return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+}
+
+// ------------------------------------------------------------------------------------------------
+// node-hierarchy template library: permutation id -> connectivity (cl_XTK_Node_Hierarchy_Template
+// _Library.cpp load_3d_template / load_2d_template)
+// ------------------------------------------------------------------------------------------------
+struct Tmpl { int n; int c[6][4]; };
+bool load_3d_template(int id, Tmpl& t) {
+    static const int g3a[] = {320, 32, 203, 251, 512, 125, 140, 401, 14, 453, 534, 345};
+    static const int g3b[] = {230, 302, 23, 521, 152, 215, 410, 41, 104, 543, 354, 435};
+    static const int g4[6][12] = {
+        {5420, 5240, 425, 3501, 1503, 3051, 1053, 4312, 2314, 4132, 2134, 245},
+        {4502, 4052, 1243, 2504, 2054, 5310, 5130, 135, 315, 3421, 3241, 1423},
+        {4250, 2450, 4205, 2405, 5031, 5013, 531, 513, 3124, 1342, 1324, 3142},
+        {5024, 524, 5042, 542, 3150, 1350, 3105, 1305, 4231, 2431, 4213, 2413},
+        {4520, 2540, 4025, 2045, 5301, 5103, 351, 153, 3412, 3214, 1432, 1234},
+        {5402, 452, 5204, 254, 3510, 1530, 3015, 1035, 4321, 2341, 4123, 2143}};
+    static const int c3[2][4][4] = {{{0, 4, 5, 6}, {4, 1, 5, 6}, {1, 2, 5, 6}, {1, 3, 2, 6}},
+                                    {{4, 0, 5, 6}, {1, 4, 5, 6}, {2, 1, 5, 6}, {3, 1, 2, 6}}};
+    static const int c4[6][6][4] = {
+        {{0, 3, 7, 6}, {4, 3, 6, 7}, {4, 3, 7, 5}, {1, 2, 7, 5}, {2, 4, 7, 5}, {4, 2, 7, 6}},
+        {{3, 0, 7, 6}, {3, 4, 6, 7}, {3, 4, 7, 5}, {2, 1, 7, 5}, {4, 2, 7, 5}, {2, 4, 7, 6}},
+        {{4, 0, 6, 7}, {0, 3, 6, 7}, {3, 5, 6, 7}, {5, 1, 6, 7}, {1, 2, 6, 7}, {2, 4, 6, 7}},
+        {{0, 4, 6, 7}, {3, 0, 6, 7}, {5, 3, 6, 7}, {1, 5, 6, 7}, {2, 1, 6, 7}, {4, 2, 6, 7}},
+        {{4, 0, 5, 7}, {0, 3, 5, 7}, {5, 3, 6, 7}, {1, 2, 6, 7}, {2, 4, 5, 7}, {2, 5, 6, 7}},
+        {{0, 4, 5, 7}, {3, 0, 5, 7}, {3, 5, 6, 7}, {2, 1, 6, 7}, {4, 2, 5, 7}, {5, 2, 6, 7}}};
+    for (int i = 0; i < 12; ++i) {
+        if (g3a[i] == id) { t.n = 4; std::memcpy(t.c, c3[0], sizeof(c3[0])); return true; }
+        if (g3b[i] == id) { t.n = 4; std::memcpy(t.c, c3[1], sizeof(c3[1])); return true; }
+    }
+    for (int g = 0; g < 6; ++g)
+        for (int i = 0; i < 12; ++i)
+            if (g4[g][i] == id) { t.n = 6; std::memcpy(t.c, c4[g], sizeof(c4[g])); return true; }
+    // vertices on the isocontour: one or two intersected edges
+    static const int c1[6][2][4] = {{{0, 4, 2, 3}, {4, 1, 2, 3}}, {{0, 1, 4, 3}, {0, 4, 2, 3}},
+                                    {{0, 1, 4, 3}, {4, 1, 2, 3}}, {{0, 1, 2, 4}, {4, 1, 2, 3}},
+                                    {{0, 4, 2, 3}, {0, 1, 2, 4}}, {{0, 1, 4, 3}, {0, 1, 2, 4}}};
+    if (id >= 10000 && id <= 10005) {
+        t.n = 2;
+        for (int i = 0; i < 2; ++i)
+            for (int k = 0; k < 4; ++k) t.c[i][k] = c1[id - 10000][i][k];
+        return true;
+    }
+    struct T2 { int id; int c[3][4]; };
+    static const T2 two[24] = {
+        {10250, {{0, 1, 4, 5}, {0, 4, 2, 5}, {0, 5, 2, 3}}}, {10520, {{0, 1, 5, 4}, {0, 4, 5, 3}, {0, 5, 2, 3}}},
+        {10260, {{0, 1, 4, 5}, {0, 1, 5, 3}, {0, 4, 2, 5}}}, {10620, {{0, 1, 5, 3}, {0, 5, 4, 3}, {0, 5, 2, 4}}},
+        {10560, {{0, 1, 5, 4}, {0, 1, 2, 5}, {0, 4, 5, 3}}}, {10650, {{0, 1, 2, 5}, {0, 5, 2, 4}, {0, 5, 4, 3}}},
+        {10640, {{0, 1, 2, 5}, {5, 1, 4, 3}, {1, 2, 5, 4}}}, {10460, {{0, 1, 2, 5}, {0, 1, 5, 4}, {4, 1, 5, 3}}},
+        {10230, {{5, 4, 2, 3}, {1, 4, 5, 3}, {0, 1, 5, 3}}}, {10320, {{5, 2, 4, 3}, {0, 5, 4, 3}, {0, 1, 5, 3}}},
+        {10130, {{5, 1, 2, 3}, {5, 4, 1, 3}, {0, 4, 5, 3}}}, {10310, {{5, 1, 2, 3}, {4, 5, 2, 3}, {0, 5, 4, 3}}},
+        {10120, {{0, 5, 2, 3}, {0, 4, 5, 3}, {4, 1, 5, 3}}}, {10210, {{0, 5, 2, 3}, {5, 4, 2, 3}, {5, 1, 4, 3}}},
+        {10140, {{0, 4, 2, 5}, {4, 1, 2, 5}, {5, 1, 2, 3}}}, {10410, {{0, 5, 2, 4}, {4, 5, 2, 3}, {5, 1, 2, 3}}},
+        {10150, {{0, 4, 2, 5}, {4, 1, 2, 5}, {0, 5, 2, 3}}}, {10510, {{5, 1, 2, 4}, {5, 4, 2, 3}, {0, 5, 2, 3}}},
+        {10450, {{0, 5, 2, 4}, {0, 1, 2, 5}, {4, 5, 2, 3}}}, {10540, {{0, 1, 2, 5}, {5, 1, 2, 4}, {5, 4, 2, 3}}},
+        {10360, {{0, 1, 5, 3}, {0, 1, 4, 5}, {1, 2, 4, 5}}}, {10630, {{0, 1, 5, 3}, {1, 4, 5, 3}, {1, 2, 5, 4}}},
+        {10340, {{0, 1, 4, 5}, {4, 1, 2, 5}, {1, 2, 5, 3}}}, {10430, {{0, 1, 5, 4}, {4, 1, 5, 3}, {1, 2, 5, 3}}}};
+    for (const T2& x : two)
+        if (x.id == id) {
+            t.n = 3;
+            for (int i = 0; i < 3; ++i)
+                for (int k = 0; k < 4; ++k) t.c[i][k] = x.c[i][k];
+            return true;
+        }
+    return false;
+}
+bool load_2d_template(int id, Tmpl& t) {
+    const int c1[3][2][3] = {{{0, 3, 2}, {3, 1, 2}}, {{0, 1, 3}, {0, 3, 2}}, {{0, 1, 3}, {1, 2, 3}}};
+    if (id >= 10 && id <= 12) {
+        t.n = 2;
+        for (int i = 0; i < 2; ++i)
+            for (int k = 0; k < 3; ++k) t.c[i][k] = c1[id - 10][i][k];
+        return true;
+    }
+    t.n = 3;
+    const int c[3][3][3] = {{{0, 3, 2}, {3, 1, 4}, {3, 4, 2}},
+                            {{0, 3, 4}, {3, 2, 4}, {3, 1, 2}},
+                            {{0, 1, 3}, {0, 3, 4}, {3, 2, 4}}};
+    if (id < 1 || id > 3) return false;
+    for (int i = 0; i < 3; ++i)
+        for (int k = 0; k < 3; ++k) t.c[i][k] = c[id - 1][i][k];
+    return true;
+}
+
+// regular subdivision tables (cl_XTK_Regular_Subdivision_Interface.hpp)
+const int kR24[24][4] = {{0, 8, 1, 14},  {1, 8, 5, 14},  {4, 5, 8, 14},  {0, 4, 8, 14},  {1, 9, 2, 14},
+                         {2, 9, 6, 14},  {5, 6, 9, 14},  {1, 5, 9, 14},  {2, 10, 3, 14}, {2, 6, 10, 14},
+                         {6, 7, 10, 14}, {3, 10, 7, 14}, {0, 3, 11, 14}, {3, 7, 11, 14}, {4, 11, 7, 14},
+                         {0, 11, 4, 14}, {0, 1, 12, 14}, {1, 2, 12, 14}, {2, 3, 12, 14}, {0, 12, 3, 14},
+                         {4, 13, 5, 14}, {5, 13, 6, 14}, {6, 13, 7, 14}, {4, 7, 13, 14}};
+const double kR24New[7][3] = {{0, -1, 0}, {1, 0, 0}, {0, 1, 0}, {-1, 0, 0}, {0, 0, -1}, {0, 0, 1}, {0, 0, 0}};
+const int kR4[4][3] = {{0, 1, 4}, {1, 2, 4}, {2, 3, 4}, {3, 0, 4}};
+const int kTetE[6][2] = {{0, 1}, {1, 2}, {0, 2}, {0, 3}, {1, 3}, {2, 3}};
+const int kTriE[3][2] = {{0, 1}, {1, 2}, {2, 0}};
+
+int tet_shared(int e1, int e2) {
+    for (int a : kTetE[e1])
+        for (int b : kTetE[e2])
+            if (a == b) return a;
+    return -1;
+}
+
+// ------------------------------------------------------------------------------------------------
+// problem
+// ------------------------------------------------------------------------------------------------
+struct OracleProblem {
+    mgpu_problem_spec spec{};
+    LS ls;
+    int dim = 3, nb = 8, nf = 1, nm = 1;
+    int n[3] = {1, 1, 1};
+    double lo[3] = {0, 0, 0}, h[3] = {1, 1, 1};
+    int64_t n_cells = 0, n_nodes = 0;
+    std::vector<double> node_phi;
+    // elements (material clusters) in lattice order
+    std::vector<int32_t> elem_cell;
+    std::vector<double> material;
+    std::vector<int32_t> ig_ptr;
+    std::vector<double> ig_xi;
+    std::vector<double> ip_box;
+    std::vector<int32_t> t_ptr, t_adof;
+    std::vector<double> t_weight;
+    int32_t N = 0;
+    int64_t n_cut = 0;
+    // graph
+    std::vector<int32_t> row_ptr, col_idx;
+};
+
+int64_t nid3(const OracleProblem& p, int i, int j, int k) {
+    return i + (int64_t)(p.n[0] + 1) * (j + (int64_t)(p.n[1] + 1) * k);
+}
+void node_x(const OracleProblem& p, int64_t nid, double* x) {
+    const int i = (int)(nid % (p.n[0] + 1));
+    const int j = (int)((nid / (p.n[0] + 1)) % (p.n[1] + 1));
+    const int k = p.dim == 3 ? (int)(nid / ((int64_t)(p.n[0] + 1) * (p.n[1] + 1))) : 0;
+    x[0] = p.lo[0] + (double)i * p.h[0];
+    x[1] = p.lo[1] + (double)j * p.h[1];
+    x[2] = p.dim == 3 ? p.lo[2] + (double)k * p.h[2] : 0.0;
+}
+void cell_nodes(const OracleProblem& p, int64_t c, int64_t* nodes) {
+    const int i = (int)(c % p.n[0]);
+    const int j = (int)((c / p.n[0]) % p.n[1]);
+    const int k = p.dim == 3 ? (int)(c / ((int64_t)p.n[0] * p.n[1])) : 0;
+    if (p.dim == 3) {
+        for (int v = 0; v < 8; ++v)
+            nodes[v] = nid3(p, i + (kHexV[v][0] + 1) / 2, j + (kHexV[v][1] + 1) / 2, k + (kHexV[v][2] + 1) / 2);
+    } else {
+        for (int v = 0; v < 4; ++v)
+            nodes[v] = (i + (kQuadV[v][0] + 1) / 2) + (int64_t)(p.n[0] + 1) * (j + (kQuadV[v][1] + 1) / 2);
+    }
+}
+// global coordinates of parametric point xi in cell c (interpolation over the cell vertices)
+void cell_x(const OracleProblem& p, int64_t c, const double* xi, double* x) {
+    int64_t nodes[8];
+    cell_nodes(p, c, nodes);
+    double N[8];
+    shape_N(p.dim, xi, N);
+    x[0] = x[1] = x[2] = 0.0;
+    for (int v = 0; v < p.nb; ++v) {
+        double xv[3];
+        node_x(p, nodes[v], xv);
+        for (int d = 0; d < p.dim; ++d) x[d] += N[v] * xv[d];
+    }
+}
+bool intersected(const OracleProblem& p, int64_t c) {
+    int64_t nodes[8];
+    cell_nodes(p, c, nodes);
+    const int r0 = region(p.node_phi[nodes[0]]);
+    for (int v = 1; v < p.nb; ++v)
+        if (region(p.node_phi[nodes[v]]) != r0) return true;
+    return false;
+}
+
+// XTK decomposition of all intersected cells; fills per-cell material sub-cells.
+void decompose(OracleProblem& p, std::map<int64_t, std::vector<double>>& cell_sub) {
+    const int D = p.dim;
+    const int nxf = (p.n[0] + 1) * p.n[1] * p.n[2];
+    const int nyf = p.n[0] * (p.n[1] + 1) * p.n[2];
+    const int64_t nfaces = D == 3 ? (int64_t)nxf + nyf + (int64_t)p.n[0] * p.n[1] * (p.n[2] + 1) : 0;
+    // global subdivision vertices created on request: gid -> (phi)
+    std::map<uint32_t, double> vphi;
+    for (int64_t c = 0; c < p.n_cells; ++c) {
+        if (!intersected(p, c)) continue;
+        const int i = (int)(c % p.n[0]);
+        const int j = (int)((c / p.n[0]) % p.n[1]);
+        const int k = D == 3 ? (int)(c / ((int64_t)p.n[0] * p.n[1])) : 0;
+        // local vertex table: gid, parametric coordinates wrt this cell
+        std::vector<uint32_t> gid;
+        std::vector<std::array<double, 3>> xi;
+        int64_t nodes[8];
+        cell_nodes(p, c, nodes);
+        for (int v = 0; v < p.nb; ++v) {
+            gid.push_back((uint32_t)nodes[v]);
+            std::array<double, 3> a{0, 0, 0};
+            for (int d = 0; d < D; ++d) a[d] = D == 3 ? kHexV[v][d] : kQuadV[v][d];
+            xi.push_back(a);
+            vphi[(uint32_t)nodes[v]] = p.node_phi[nodes[v]];
+        }
+        if (D == 3) {
+            const int64_t fids[6] = {nxf + i + (int64_t)p.n[0] * (j + (int64_t)(p.n[1] + 1) * k),
+                                     (i + 1) + (int64_t)(p.n[0] + 1) * (j + (int64_t)p.n[1] * k),
+                                     nxf + i + (int64_t)p.n[0] * ((j + 1) + (int64_t)(p.n[1] + 1) * k),
+                                     i + (int64_t)(p.n[0] + 1) * (j + (int64_t)p.n[1] * k),
+                                     (int64_t)nxf + nyf + i + (int64_t)p.n[0] * (j + (int64_t)p.n[1] * k),
+                                     (int64_t)nxf + nyf + i + (int64_t)p.n[0] * (j + (int64_t)p.n[1] * (k + 1))};
+            for (int f = 0; f < 7; ++f) {
+                const uint32_t g = f < 6 ? (uint32_t)(p.n_nodes + fids[f]) : (uint32_t)(p.n_nodes + nfaces + c);
+                std::array<double, 3> a{kR24New[f][0], kR24New[f][1], kR24New[f][2]};
+                if (!vphi.count(g)) {   // first requesting cell creates the vertex
+                    double x[3];
+                    cell_x(p, c, a.data(), x);
+                    vphi[g] = p.ls.eval(x);
+                }
+                gid.push_back(g);
+                xi.push_back(a);
+            }
+        } else {
+            const uint32_t g = (uint32_t)(p.n_nodes + c);
+            std::array<double, 3> a{0, 0, 0};
+            double x[3];
+            cell_x(p, c, a.data(), x);
+            vphi[g] = p.ls.eval(x);
+            gid.push_back(g);
+            xi.push_back(a);
+        }
+        // IG cells of the regular subdivision, then node hierarchy per IG cell
+        const int ncell = D == 3 ? 24 : 4;
+        const int nv = D + 1;
+        std::vector<double> sub;
+        for (int t = 0; t < ncell; ++t) {
+            int tv[4];
+            for (int q = 0; q < nv; ++q) tv[q] = D == 3 ? kR24[t][q] : kR4[t][q];
+            // intersected edges: (ordinal, key, parametric point)
+            struct IE { int ord; uint64_t key; double x[3]; };
+            std::vector<IE> ies;
+            const int ne = D == 3 ? 6 : 3;
+            for (int e = 0; e < ne; ++e) {
+                const int l0 = tv[D == 3 ? kTetE[e][0] : kTriE[e][0]];
+                const int l1 = tv[D == 3 ? kTetE[e][1] : kTriE[e][1]];
+                const double f0 = vphi[gid[l0]], f1 = vphi[gid[l1]];
+                // an endpoint on the interface: no general intersection on this edge
+                if (region(f0) == 2 || region(f1) == 2) continue;
+                if (region(f0) == region(f1)) continue;
+                // parent nodes ordered by global id
+                const int la = gid[l0] < gid[l1] ? l0 : l1;
+                const int lb = gid[l0] < gid[l1] ? l1 : l0;
+                const double fa = vphi[gid[la]], fb = vphi[gid[lb]];
+                const double loc = (2.0 * 0.0 - fb - fa) / (fb - fa);
+                if (!(std::fabs(loc) <= 1.0)) continue;
+                IE ie;
+                ie.ord = e;
+                ie.key = ((uint64_t)gid[la] << 32) | gid[lb];
+                for (int d = 0; d < 3; ++d) ie.x[d] = 0.5 * (1.0 - loc) * xi[la][d] + 0.5 * (1.0 + loc) * xi[lb][d];
+                ies.push_back(ie);
+            }
+            auto tet_vertex_xi = [&](int local_in_tet) { return xi[tv[local_in_tet]]; };
+            auto phase_of = [&](int local_in_tet) { return region(vphi[gid[tv[local_in_tet]]]); };
+            if (ies.empty()) {
+                int ph = -1;
+                for (int q = 0; q < nv && ph < 0; ++q)
+                    if (phase_of(q) != 2) ph = phase_of(q);
+                if (ph == p.spec.material_phase)
+                    for (int q = 0; q < nv; ++q)
+                        for (int d = 0; d < D; ++d) sub.push_back(tet_vertex_xi(q)[d]);
+                continue;
+            }
+            Tmpl tm;
+            int nodeord[4] = {0, 1, 2, 3};
+            std::vector<IE> sorted = ies;
+            if (D == 3) {
+                std::stable_sort(sorted.begin(), sorted.end(), [](const IE& a, const IE& b) { return a.key < b.key; });
+                int perm = 0;
+                if (sorted.size() == 1) {
+                    perm = 10000 + sorted[0].ord;
+                } else if (sorted.size() == 2) {
+                    perm = 10000 + 100 * (sorted[0].ord + 1) + 10 * (sorted[1].ord + 1);
+                } else if (sorted.size() == 3) {
+                    const int L = sorted[0].ord, M = sorted[1].ord, H = sorted[2].ord;
+                    const int n0 = tet_shared(L, M);
+                    auto other = [&](int e) { return kTetE[e][0] == n0 ? kTetE[e][1] : kTetE[e][0]; };
+                    nodeord[0] = n0; nodeord[1] = other(L); nodeord[2] = other(M); nodeord[3] = other(H);
+                    perm = L + 10 * M + 100 * H;
+                } else if (sorted.size() == 4) {
+                    const int L = sorted[0].ord, ML = sorted[1].ord, MH = sorted[2].ord, H = sorted[3].ord;
+                    perm = L + 10 * ML + 100 * MH + 1000 * H;
+                    const int HL = tet_shared(L, H), HMH = tet_shared(MH, H), HML = tet_shared(ML, H);
+                    const int LML = tet_shared(L, ML), LMH = tet_shared(L, MH), MLMH = tet_shared(ML, MH);
+                    if (HL < 0) { nodeord[0] = HMH; nodeord[1] = HML; nodeord[2] = LMH; nodeord[3] = LML; }
+                    else if (HMH < 0) { nodeord[0] = HL; nodeord[1] = HML; nodeord[2] = LMH; nodeord[3] = MLMH; }
+                    else { nodeord[0] = HL; nodeord[1] = HMH; nodeord[2] = LML; nodeord[3] = MLMH; }
+                } else {
+                    throw std::runtime_error("unexpected intersected edge count");
+                }
+                if (!load_3d_template(perm, tm)) throw std::runtime_error("template not found");
+            } else {
+                std::stable_sort(sorted.begin(), sorted.end(), [](const IE& a, const IE& b) { return a.ord < b.ord; });
+                int id;
+                if (sorted.size() == 1) id = 10 + sorted[0].ord;
+                else if (sorted.size() == 2) id = sorted[0].ord + sorted[1].ord;
+                else throw std::runtime_error("unexpected 2D cut");
+                if (!load_2d_template(id, tm)) throw std::runtime_error("template not found");
+            }
+            for (int cidx = 0; cidx < tm.n; ++cidx) {
+                int ph = -1;
+                for (int q = 0; q < nv && ph < 0; ++q)
+                    if (tm.c[cidx][q] < nv && phase_of(nodeord[tm.c[cidx][q]]) != 2) ph = phase_of(nodeord[tm.c[cidx][q]]);
+                if (ph != p.spec.material_phase) continue;
+                for (int q = 0; q < nv; ++q) {
+                    const int o = tm.c[cidx][q];
+                    if (o < nv) {
+                        auto a = tet_vertex_xi(nodeord[o]);
+                        for (int d = 0; d < D; ++d) sub.push_back(a[d]);
+                    } else {
+                        for (int d = 0; d < D; ++d) sub.push_back(sorted[o - nv].x[d]);
+                    }
+                }
+            }
+        }
+        cell_sub[c] = std::move(sub);
+    }
+}
+
+void build(OracleProblem& p, const mgpu_problem_spec& s) {
+    p.spec = s;
+    p.dim = s.dim;
+    p.nb = s.dim == 3 ? 8 : 4;
+    p.nf = s.iwg == MGPU_IWG_STRUC_LINEAR_BULK ? s.dim : 1;
+    p.nm = s.iwg == MGPU_IWG_STRUC_LINEAR_BULK ? 2 : 1;
+    for (int d = 0; d < 3; ++d) {
+        p.n[d] = d < s.dim ? s.n[d] : 1;
+        p.lo[d] = d < s.dim ? s.lo[d] : 0.0;
+        p.h[d] = d < s.dim ? (s.hi[d] - s.lo[d]) / (double)s.n[d] : 1.0;
+    }
+    p.ls.kind = s.ls_kind;
+    p.ls.dim = s.dim;
+    if (s.ls_kind == MGPU_LS_SPHERES) p.ls.spheres.assign(s.spheres, s.spheres + 4 * s.n_spheres);
+    p.spec.spheres = nullptr;
+    p.ls.gs = s.gyroid_scale;
+    for (int d = 0; d < 4; ++d) p.ls.plane[d] = s.plane[d];
+    p.n_cells = (int64_t)p.n[0] * p.n[1] * p.n[2];
+    p.n_nodes = s.dim == 3 ? (int64_t)(p.n[0] + 1) * (p.n[1] + 1) * (p.n[2] + 1) : (int64_t)(p.n[0] + 1) * (p.n[1] + 1);
+    p.node_phi.resize(p.n_nodes);
+    for (int64_t nid = 0; nid < p.n_nodes; ++nid) {
+        double x[3];
+        node_x(p, nid, x);
+        p.node_phi[nid] = p.ls.eval(x);
+    }
+    std::map<int64_t, std::vector<double>> cut;
+    decompose(p, cut);
+    const int nv = p.dim + 1;
+    p.ig_ptr.push_back(0);
+    std::vector<char> used(p.n_nodes, 0);
+    for (int64_t c = 0; c < p.n_cells; ++c) {
+        auto it = cut.find(c);
+        int64_t nodes[8];
+        cell_nodes(p, c, nodes);
+        if (it == cut.end()) {
+            if (region(p.node_phi[nodes[0]]) != s.material_phase) continue;
+        } else if (it->second.empty()) {
+            continue;
+        }
+        p.elem_cell.push_back((int32_t)c);
+        if (s.iwg == MGPU_IWG_SPATIALDIFF_BULK) {
+            p.material.push_back(s.k_sigma == 0.0 ? std::exp(s.k_mu) : std::exp(s.k_mu + s.k_sigma * gauss_draw(s.seed, (uint64_t)c)));
+        } else {
+            p.material.push_back(s.E);
+            p.material.push_back(s.nu);
+        }
+        if (it != cut.end()) {
+            p.ig_xi.insert(p.ig_xi.end(), it->second.begin(), it->second.end());
+            ++p.n_cut;
+        }
+        p.ig_ptr.push_back((int32_t)(p.ig_xi.size() / (nv * p.dim)));
+        const int i = (int)(c % p.n[0]);
+        const int j = (int)((c / p.n[0]) % p.n[1]);
+        const int k = p.dim == 3 ? (int)(c / ((int64_t)p.n[0] * p.n[1])) : 0;
+        const int ijk[3] = {i, j, k};
+        for (int d = 0; d < p.dim; ++d) p.ip_box.push_back(p.lo[d] + (double)ijk[d] * p.h[d]);
+        for (int d = 0; d < p.dim; ++d) p.ip_box.push_back(p.h[d]);
+        for (int v = 0; v < p.nb; ++v) used[nodes[v]] = 1;
+    }
+    std::vector<int32_t> adof(p.n_nodes, -1);
+    for (int64_t nid = 0; nid < p.n_nodes; ++nid)
+        if (used[nid]) adof[nid] = p.N++;
+    for (size_t e = 0; e < p.elem_cell.size(); ++e) {
+        int64_t nodes[8];
+        cell_nodes(p, p.elem_cell[e], nodes);
+        for (int v = 0; v < p.nb; ++v) {
+            p.t_ptr.push_back((int32_t)p.t_adof.size());
+            p.t_adof.push_back(adof[nodes[v]]);
+            p.t_weight.push_back(1.0);
+        }
+    }
+    p.t_ptr.push_back((int32_t)p.t_adof.size());
+}
+
+// element topology: per type sorted unique adof ids (global)
+void element_topology(const mgpu_set_desc& d, int nf, int64_t e, std::vector<int32_t>& ids) {
+    std::vector<int32_t> u;
+    for (int b = 0; b < d.n_bases; ++b)
+        for (int q = d.t_ptr[e * d.n_bases + b]; q < d.t_ptr[e * d.n_bases + b + 1]; ++q) u.push_back(d.t_adof[q]);
+    std::sort(u.begin(), u.end());
+    u.erase(std::unique(u.begin(), u.end()), u.end());
+    ids.clear();
+    for (int f = 0; f < nf; ++f)
+        for (int32_t a : u) ids.push_back(a + f * d.n_adofs_per_type);
+}
+
+void build_graph(const mgpu_set_desc& d, int nf, std::vector<int32_t>& row_ptr, std::vector<int32_t>& col_idx) {
+    const int64_t nrows = (int64_t)nf * d.n_adofs_per_type;
+    std::vector<std::vector<int32_t>> rows(nrows);
+    std::vector<int32_t> ids;
+    for (int64_t e = 0; e < d.n_elements; ++e) {
+        element_topology(d, nf, e, ids);
+        for (int32_t r : ids) rows[r].insert(rows[r].end(), ids.begin(), ids.end());
+    }
+    row_ptr.assign(nrows + 1, 0);
+    col_idx.clear();
+    for (int64_t r = 0; r < nrows; ++r) {
+        auto& v = rows[r];
+        std::sort(v.begin(), v.end());
+        v.erase(std::unique(v.begin(), v.end()), v.end());
+        col_idx.insert(col_idx.end(), v.begin(), v.end());
+        row_ptr[r + 1] = (int32_t)col_idx.size();
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+// element K_e, R_e and scatter (reference algorithm, dense small matrices)
+// ------------------------------------------------------------------------------------------------
+struct Scratch {
+    std::vector<double> Kp, Rp, up, T, Ke, Re;
+    std::vector<int32_t> ids;
+};
+
+void element_assemble(const mgpu_set_desc& d, int64_t e, const double* u, const int32_t* row_ptr,
+                      const int32_t* col_idx, double* vals, double* rhs, Scratch& s, int64_t* n_ip) {
+    const int D = d.spatial_dim, nb = d.n_bases;
+    const bool elast = d.iwg == MGPU_IWG_STRUC_LINEAR_BULK;
+    const int nf = elast ? D : 1;
+    const int np = nf * nb;
+    const int32_t N = d.n_adofs_per_type;
+    element_topology(d, nf, e, s.ids);
+    const int na = (int)s.ids.size();
+    const int na_t = na / nf;
+    // dense T (np x na), block diagonal per dof type
+    s.T.assign((size_t)np * na, 0.0);
+    for (int b = 0; b < nb; ++b)
+        for (int q = d.t_ptr[e * nb + b]; q < d.t_ptr[e * nb + b + 1]; ++q) {
+            const int col = (int)(std::lower_bound(s.ids.begin(), s.ids.begin() + na_t, d.t_adof[q]) - s.ids.begin());
+            for (int f = 0; f < nf; ++f) s.T[(size_t)(f * nb + b) * na + f * na_t + col] += d.t_weight[q];
+        }
+    // pdof values u_p = T u_adof
+    s.up.assign(np, 0.0);
+    for (int i = 0; i < np; ++i)
+        for (int a = 0; a < na; ++a) s.up[i] += s.T[(size_t)i * na + a] * u[s.ids[a]];
+    s.Kp.assign((size_t)np * np, 0.0);
+    s.Rp.assign(np, 0.0);
+    // IP cell geometry: vertices of the box
+    const double* box = d.ip_box + (size_t)e * 2 * D;
+    double X[8][3];
+    for (int v = 0; v < nb; ++v)
+        for (int q = 0; q < D; ++q) {
+            const int sgn = D == 3 ? kHexV[v][q] : kQuadV[v][q];
+            X[v][q] = box[q] + (sgn > 0 ? box[D + q] : 0.0);
+        }
+    // material
+    const int ns = elast ? (D == 3 ? 6 : 3) : D;
+    double Dm[6][6] = {{0}};
+    double kcond = 0.0;
+    if (elast) {
+        const double E = d.material[2 * e], nu = d.material[2 * e + 1];
+        if (D == 3) {
+            const double pre = E / (1.0 + nu) / (1.0 - 2.0 * nu);
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) Dm[i][j] = i == j ? pre * (1.0 - nu) : pre * nu;
+            for (int i = 3; i < 6; ++i) Dm[i][i] = pre * (1.0 - 2.0 * nu) / 2.0;
+        } else {
+            const double pre = E / (1.0 - std::pow(nu, 2));
+            Dm[0][0] = pre; Dm[1][1] = pre; Dm[0][1] = pre * nu; Dm[1][0] = pre * nu;
+            Dm[2][2] = pre * 0.5 * (1.0 - nu);
+        }
+    } else {
+        kcond = d.material[e];
+    }
+    // integration domains
+    const int ig0 = d.ig_ptr[e], ig1 = d.ig_ptr[e + 1];
+    const bool cut = ig1 > ig0;
+    const int n_dom = cut ? ig1 - ig0 : 1;
+    const int nv = D + 1;
+    std::vector<double> B((size_t)ns * np), DB((size_t)ns * np);
+    for (int dom = 0; dom < n_dom; ++dom) {
+        int nq;
+        if (!cut) nq = D == 3 ? 8 : 4;
+        else nq = D == 3 ? 11 : 7;
+        const double* Xh = cut ? d.ig_xi + (size_t)(ig0 + dom) * nv * D : nullptr;
+        // IG physical vertex coordinates (interpolated in the IP cell)
+        double Xig[4][3];
+        if (cut)
+            for (int v = 0; v < nv; ++v) {
+                double Nv[8];
+                shape_N(D, Xh + v * D, Nv);
+                for (int q = 0; q < D; ++q) {
+                    Xig[v][q] = 0.0;
+                    for (int a = 0; a < nb; ++a) Xig[v][q] += Nv[a] * X[a][q];
+                }
+            }
+        for (int iq = 0; iq < nq; ++iq) {
+            double xip[3] = {0, 0, 0};
+            double w, detJ;
+            if (!cut) {
+                for (int q = 0; q < D; ++q) xip[q] = D == 3 ? kHex2[iq][q] : kQuad2[iq][q];
+                w = 1.0;
+                // IG cell = IP cell: detJ of the hex / quad from physical coordinates
+                double dN[3][8];
+                shape_dN(D, xip, dN);
+                double Jt[3][3] = {{0}};
+                for (int r = 0; r < D; ++r)
+                    for (int q = 0; q < D; ++q)
+                        for (int a = 0; a < nb; ++a) Jt[r][q] += dN[r][a] * X[a][q];
+                detJ = D == 3 ? Jt[0][0] * (Jt[1][1] * Jt[2][2] - Jt[2][1] * Jt[1][2]) -
+                                    Jt[0][1] * (Jt[1][0] * Jt[2][2] - Jt[1][2] * Jt[2][0]) +
+                                    Jt[0][2] * (Jt[1][0] * Jt[2][1] - Jt[1][1] * Jt[2][0])
+                              : Jt[0][0] * Jt[1][1] - Jt[0][1] * Jt[1][0];
+            } else {
+                double xg[3];
+                double Ng[4];
+                if (D == 3) {
+                    for (int q = 0; q < 3; ++q) xg[q] = kTet11P[iq][q];
+                    Ng[0] = xg[0]; Ng[1] = xg[1]; Ng[2] = 1.0 - xg[0] - xg[1] - xg[2]; Ng[3] = xg[2];
+                    w = tet11_weight(iq);
+                    // dN/dxi of TET4: rows (1,0,-1,0), (0,1,-1,0), (0,0,-1,1)
+                    const double dNt[3][4] = {{1, 0, -1, 0}, {0, 1, -1, 0}, {0, 0, -1, 1}};
+                    double Jt[3][3] = {{0}};
+                    for (int r = 0; r < 3; ++r)
+                        for (int q = 0; q < 3; ++q)
+                            for (int v = 0; v < 4; ++v) Jt[r][q] += dNt[r][v] * Xig[v][q];
+                    detJ = (Jt[0][0] * (Jt[1][1] * Jt[2][2] - Jt[2][1] * Jt[1][2]) -
+                            Jt[0][1] * (Jt[1][0] * Jt[2][2] - Jt[1][2] * Jt[2][0]) +
+                            Jt[0][2] * (Jt[1][0] * Jt[2][1] - Jt[1][1] * Jt[2][0])) / 6.0;
+                } else {
+                    for (int q = 0; q < 2; ++q) xg[q] = kTri7P[iq][q];
+                    Ng[0] = xg[0]; Ng[1] = xg[1]; Ng[2] = 1.0 - xg[0] - xg[1];
+                    w = kTri7W[iq];
+                    const double dNt[2][3] = {{1, 0, -1}, {0, 1, -1}};
+                    double Jt[2][2] = {{0}};
+                    for (int r = 0; r < 2; ++r)
+                        for (int q = 0; q < 2; ++q)
+                            for (int v = 0; v < 3; ++v) Jt[r][q] += dNt[r][v] * Xig[v][q];
+                    detJ = (Jt[0][0] * Jt[1][1] - Jt[0][1] * Jt[1][0]) / 2.0;
+                }
+                for (int q = 0; q < D; ++q) {
+                    xip[q] = 0.0;
+                    for (int v = 0; v < nv; ++v) xip[q] += Ng[v] * Xh[v * D + q];
+                }
+            }
+            if (detJ < -1.0e-6) continue;
+            const double ws = w * detJ;
+            if (n_ip) ++*n_ip;
+            // IP cell Jacobian at xi_IP and its rectangular (diagonal) inverse
+            double dN[3][8];
+            shape_dN(D, xip, dN);
+            double Jt[3][3] = {{0}};
+            for (int r = 0; r < D; ++r)
+                for (int q = 0; q < D; ++q)
+                    for (int a = 0; a < nb; ++a) Jt[r][q] += dN[r][a] * X[a][q];
+            double dNdx[3][8];
+            for (int r = 0; r < D; ++r)
+                for (int a = 0; a < nb; ++a) dNdx[r][a] = dN[r][a] / Jt[r][r];
+            // B (test strain) and flux
+            std::fill(B.begin(), B.end(), 0.0);
+            if (elast) {
+                if (D == 3) {
+                    for (int a = 0; a < nb; ++a) {
+                        B[0 * np + a] = dNdx[0][a];
+                        B[4 * np + a] = dNdx[2][a];
+                        B[5 * np + a] = dNdx[1][a];
+                        B[1 * np + nb + a] = dNdx[1][a];
+                        B[3 * np + nb + a] = dNdx[2][a];
+                        B[5 * np + nb + a] = dNdx[0][a];
+                        B[2 * np + 2 * nb + a] = dNdx[2][a];
+                        B[3 * np + 2 * nb + a] = dNdx[1][a];
+                        B[4 * np + 2 * nb + a] = dNdx[0][a];
+                    }
+                } else {
+                    for (int a = 0; a < nb; ++a) {
+                        B[0 * np + a] = dNdx[0][a];
+                        B[2 * np + a] = dNdx[1][a];
+                        B[1 * np + nb + a] = dNdx[1][a];
+                        B[2 * np + nb + a] = dNdx[0][a];
+                    }
+                }
+                // D B
+                for (int i = 0; i < ns; ++i)
+                    for (int c = 0; c < np; ++c) {
+                        double v = 0.0;
+                        for (int k = 0; k < ns; ++k) v += Dm[i][k] * B[k * np + c];
+                        DB[i * np + c] = v;
+                    }
+            } else {
+                for (int i = 0; i < D; ++i)
+                    for (int a = 0; a < nb; ++a) {
+                        B[i * np + a] = dNdx[i][a];
+                        DB[i * np + a] = kcond * dNdx[i][a];
+                    }
+            }
+            // flux = D B u_p
+            double flux[6] = {0, 0, 0, 0, 0, 0};
+            for (int i = 0; i < ns; ++i)
+                for (int c = 0; c < np; ++c) flux[i] += DB[i * np + c] * s.up[c];
+            for (int r = 0; r < np; ++r) {
+                double rv = 0.0;
+                for (int i = 0; i < ns; ++i) rv += B[i * np + r] * flux[i];
+                s.Rp[r] += ws * rv;
+                for (int c = 0; c < np; ++c) {
+                    double kv = 0.0;
+                    for (int i = 0; i < ns; ++i) kv += B[i * np + r] * DB[i * np + c];
+                    s.Kp[(size_t)r * np + c] += ws * kv;
+                }
+            }
+        }
+    }
+    // K_e = T^T K_p T, R_e = T^T R_p
+    s.Ke.assign((size_t)na * na, 0.0);
+    s.Re.assign(na, 0.0);
+    std::vector<double> KT((size_t)np * na, 0.0);
+    for (int i = 0; i < np; ++i)
+        for (int a = 0; a < na; ++a) {
+            double v = 0.0;
+            for (int j = 0; j < np; ++j) v += s.Kp[(size_t)i * np + j] * s.T[(size_t)j * na + a];
+            KT[(size_t)i * na + a] = v;
+        }
+    for (int a = 0; a < na; ++a) {
+        for (int b = 0; b < na; ++b) {
+            double v = 0.0;
+            for (int i = 0; i < np; ++i) v += s.T[(size_t)i * na + a] * KT[(size_t)i * na + b];
+            s.Ke[(size_t)a * na + b] = v;
+        }
+        double r = 0.0;
+        for (int i = 0; i < np; ++i) r += s.T[(size_t)i * na + a] * s.Rp[i];
+        s.Re[a] = r;
+    }
+    (void)N;
+    // scatter: K[ids[a], ids[b]] += K_e[a][b] (row search), R[ids[a]] += R_e[a]
+    for (int a = 0; a < na; ++a) {
+        const int32_t row = s.ids[a];
+        const int32_t* c0 = col_idx + row_ptr[row];
+        const int32_t* c1 = col_idx + row_ptr[row + 1];
+        for (int b = 0; b < na; ++b) {
+            const int32_t* it = std::lower_bound(c0, c1, s.ids[b]);
+            if (it == c1 || *it != s.ids[b]) throw std::runtime_error("column not in graph");
+            vals[it - col_idx] += s.Ke[(size_t)a * na + b];
+        }
+        rhs[row] += s.Re[a];
+    }
+}
+
+thread_local std::string g_err;
+
+mgpu_set_desc desc_of(const OracleProblem& p) {
+    mgpu_set_desc d;
+    std::memset(&d, 0, sizeof(d));
+    d.spatial_dim = p.dim;
+    d.iwg = p.spec.iwg;
+    d.ip_geometry = p.dim == 3 ? MGPU_GEOM_HEX8 : MGPU_GEOM_QUAD4;
+    d.ig_geometry = p.dim == 3 ? MGPU_GEOM_TET4 : MGPU_GEOM_TRI3;
+    d.n_elements = (int32_t)p.elem_cell.size();
+    d.n_bases = p.nb;
+    d.n_adofs_per_type = p.N;
+    d.ip_box = p.ip_box.data();
+    d.material = p.material.data();
+    d.t_ptr = p.t_ptr.data();
+    d.t_adof = p.t_adof.data();
+    d.t_weight = p.t_weight.data();
+    d.ig_ptr = p.ig_ptr.data();
+    d.ig_xi = p.ig_xi.data();
+    d.elem_cell = p.elem_cell.data();
+    return d;
+}
+
+}  // namespace
+
+// ================================================================================================
+// C API (ctypes)
+// ================================================================================================
+extern "C" {
+
+const char* oracle_last_error(void) { return g_err.c_str(); }
+
+void* oracle_build(const mgpu_problem_spec* s) {
+    try {
+        auto p = std::make_unique<OracleProblem>();
+        build(*p, *s);
+        mgpu_set_desc d = desc_of(*p);
+        build_graph(d, p->nf, p->row_ptr, p->col_idx);
+        return p.release();
+    } catch (const std::exception& ex) {
+        g_err = ex.what();
+        return nullptr;
+    }
+}
+
+void oracle_destroy(void* h) { delete static_cast<OracleProblem*>(h); }
+
+// counts: [0] elements, [1] cut elements, [2] IG sub-cells, [3] adofs per type, [4] rows, [5] nnz
+void oracle_counts(void* h, int64_t* c) {
+    const OracleProblem& p = *static_cast<OracleProblem*>(h);
+    c[0] = (int64_t)p.elem_cell.size();
+    c[1] = p.n_cut;
+    c[2] = p.ig_ptr.back();
+    c[3] = p.N;
+    c[4] = (int64_t)p.row_ptr.size() - 1;
+    c[5] = (int64_t)p.col_idx.size();
+}
+
+// copies of the set-up arrays (sizes from oracle_counts)
+void oracle_get_setup(void* h, int32_t* elem_cell, int32_t* ig_ptr, double* ig_xi, double* material,
+                      int32_t* t_adof, int32_t* row_ptr, int32_t* col_idx) {
+    const OracleProblem& p = *static_cast<OracleProblem*>(h);
+    if (elem_cell) std::copy(p.elem_cell.begin(), p.elem_cell.end(), elem_cell);
+    if (ig_ptr) std::copy(p.ig_ptr.begin(), p.ig_ptr.end(), ig_ptr);
+    if (ig_xi) std::copy(p.ig_xi.begin(), p.ig_xi.end(), ig_xi);
+    if (material) std::copy(p.material.begin(), p.material.end(), material);
+    if (t_adof) std::copy(p.t_adof.begin(), p.t_adof.end(), t_adof);
+    if (row_ptr) std::copy(p.row_ptr.begin(), p.row_ptr.end(), row_ptr);
+    if (col_idx) std::copy(p.col_idx.begin(), p.col_idx.end(), col_idx);
+}
+
+// Assemble the oracle's own problem.  vals[nnz], rhs[rows] are zeroed then summed.
+int oracle_assemble(void* h, const double* u, double* vals, double* rhs) {
+    try {
+        const OracleProblem& p = *static_cast<OracleProblem*>(h);
+        mgpu_set_desc d = desc_of(p);
+        std::fill(vals, vals + p.col_idx.size(), 0.0);
+        std::fill(rhs, rhs + p.row_ptr.size() - 1, 0.0);
+        Scratch s;
+        for (int64_t e = 0; e < d.n_elements; ++e)
+            element_assemble(d, e, u, p.row_ptr.data(), p.col_idx.data(), vals, rhs, s, nullptr);
+        return 0;
+    } catch (const std::exception& ex) {
+        g_err = ex.what();
+        return 1;
+    }
+}
+
+// Reference algorithm on an external descriptor and graph, elements [e0, e1), with n_threads
+// workers over a static element partition; each worker scatters into private arrays that are summed
+// row-wise at the end (the stand-in for Epetra's GlobalAssemble).  Returns the number of IPs.
+int64_t oracle_assemble_desc(const mgpu_set_desc* d, const int32_t* row_ptr, const int32_t* col_idx,
+                             int32_t n_rows, const double* u, double* vals, double* rhs, int64_t e0,
+                             int64_t e1, int32_t n_threads) {
+    try {
+        const int64_t nnz = row_ptr[n_rows];
+        std::fill(vals, vals + nnz, 0.0);
+        std::fill(rhs, rhs + n_rows, 0.0);
+        if (n_threads <= 1) {
+            Scratch s;
+            int64_t nip = 0;
+            for (int64_t e = e0; e < e1; ++e) element_assemble(*d, e, u, row_ptr, col_idx, vals, rhs, s, &nip);
+            return nip;
+        }
+        std::vector<std::vector<double>> pv(n_threads), pr(n_threads);
+        std::vector<int64_t> nips(n_threads, 0);
+        std::vector<std::string> errs(n_threads);
+        std::vector<std::thread> th;
+        for (int t = 0; t < n_threads; ++t)
+            th.emplace_back([&, t]() {
+                try {
+                    pv[t].assign(nnz, 0.0);
+                    pr[t].assign(n_rows, 0.0);
+                    Scratch s;
+                    const int64_t a = e0 + (e1 - e0) * t / n_threads, b = e0 + (e1 - e0) * (t + 1) / n_threads;
+                    for (int64_t e = a; e < b; ++e)
+                        element_assemble(*d, e, u, row_ptr, col_idx, pv[t].data(), pr[t].data(), s, &nips[t]);
+                } catch (const std::exception& ex) {
+                    errs[t] = ex.what();
+                }
+            });
+        for (auto& x : th) x.join();
+        for (auto& e : errs)
+            if (!e.empty()) throw std::runtime_error(e);
+        for (int t = 0; t < n_threads; ++t) {
+            for (int64_t i = 0; i < nnz; ++i) vals[i] += pv[t][i];
+            for (int64_t i = 0; i < n_rows; ++i) rhs[i] += pr[t][i];
+        }
+        int64_t nip = 0;
+        for (auto v : nips) nip += v;
+        return nip;
+    } catch (const std::exception& ex) {
+        g_err = ex.what();
+        return -1;
+    }
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------------------------------------
+// building blocks exposed for the golden-vector / closed-form tests
+// ------------------------------------------------------------------------------------------------
+extern "C" {
+
+// Lagrange QUAD4 (dim 2) / HEX8 (dim 3) shape functions at n points xi[n][dim]:
+// N[n][nb], dN[n][dim][nb].
+void oracle_shape(int dim, int n, const double* xi, double* N, double* dN) {
+    const int nb = dim == 3 ? 8 : 4;
+    for (int k = 0; k < n; ++k) {
+        double xx[3] = {xi[k * dim], xi[k * dim + 1], dim == 3 ? xi[k * dim + 2] : 0.0};
+        if (N) shape_N(dim, xx, N + (size_t)k * nb);
+        if (dN) {
+            double d[3][8];
+            shape_dN(dim, xx, d);
+            for (int i = 0; i < dim; ++i)
+                for (int a = 0; a < nb; ++a) dN[((size_t)k * dim + i) * nb + a] = d[i][a];
+        }
+    }
+}
+
+// Quadrature rule: 0 QUAD_2x2, 1 HEX_2x2x2, 2 TRI_7, 3 TET_11.  Writes points [n][dim] and
+// weights [n]; returns n.
+int oracle_rule(int rule, double* xi, double* w) {
+    switch (rule) {
+        case 0:
+            for (int q = 0; q < 4; ++q) { xi[2 * q] = kQuad2[q][0]; xi[2 * q + 1] = kQuad2[q][1]; w[q] = 1.0; }
+            return 4;
+        case 1:
+            for (int q = 0; q < 8; ++q) { for (int d = 0; d < 3; ++d) xi[3 * q + d] = kHex2[q][d]; w[q] = 1.0; }
+            return 8;
+        case 2:
+            for (int q = 0; q < 7; ++q) { xi[2 * q] = kTri7P[q][0]; xi[2 * q + 1] = kTri7P[q][1]; w[q] = kTri7W[q]; }
+            return 7;
+        case 3:
+            for (int q = 0; q < 11; ++q) { for (int d = 0; d < 3; ++d) xi[3 * q + d] = kTet11P[q][d]; w[q] = tet11_weight(q); }
+            return 11;
+        default:
+            return 0;
+    }
+}
+
+// Dense element matrices of element e of the oracle's own problem (reference flow, before scatter):
+// ids[na], Ke[na][na], Re[na] for adof vector u.  Returns na.
+int oracle_element(void* h, int64_t e, const double* u, int32_t* ids, double* Ke, double* Re) {
+    try {
+        const OracleProblem& p = *static_cast<OracleProblem*>(h);
+        mgpu_set_desc d = desc_of(p);
+        Scratch s;
+        // scatter into a private one-element graph
+        std::vector<int32_t> tids;
+        element_topology(d, p.nf, e, tids);
+        const int na = (int)tids.size();
+        int32_t nrows = p.nf * p.N;
+        std::vector<int32_t> rp(nrows + 1, 0), ci;
+        for (int32_t r = 0; r < nrows; ++r) {
+            rp[r + 1] = rp[r];
+            if (std::binary_search(tids.begin(), tids.end(), r)) {
+                ci.insert(ci.end(), tids.begin(), tids.end());
+                rp[r + 1] += na;
+            }
+        }
+        std::vector<double> vals(ci.size(), 0.0), rhs(nrows, 0.0);
+        element_assemble(d, e, u, rp.data(), ci.data(), vals.data(), rhs.data(), s, nullptr);
+        for (int a = 0; a < na; ++a) {
+            ids[a] = tids[a];
+            Re[a] = s.Re[a];
+            for (int b = 0; b < na; ++b) Ke[a * na + b] = s.Ke[(size_t)a * na + b];
+        }
+        return na;
+    } catch (const std::exception& ex) {
+        g_err = ex.what();
+        return -1;
+    }
+}
+
+}  // extern "C"
+
+extern "C" {
+// Reference algorithm over an explicit element list (a sample of a workload), n_threads workers.
+int64_t oracle_assemble_list(const mgpu_set_desc* d, const int32_t* row_ptr, const int32_t* col_idx,
+                             int32_t n_rows, const double* u, double* vals, double* rhs, const int64_t* elems,
+                             int64_t n_elems, int32_t n_threads) {
+    try {
+        const int64_t nnz = row_ptr[n_rows];
+        std::fill(vals, vals + nnz, 0.0);
+        std::fill(rhs, rhs + n_rows, 0.0);
+        if (n_threads < 1) n_threads = 1;
+        std::vector<std::vector<double>> pv(n_threads), pr(n_threads);
+        std::vector<int64_t> nips(n_threads, 0);
+        std::vector<std::string> errs(n_threads);
+        std::vector<std::thread> th;
+        for (int t = 0; t < n_threads; ++t)
+            th.emplace_back([&, t]() {
+                try {
+                    double* v = vals;
+                    double* r = rhs;
+                    if (n_threads > 1) {
+                        pv[t].assign(nnz, 0.0);
+                        pr[t].assign(n_rows, 0.0);
+                        v = pv[t].data();
+                        r = pr[t].data();
+                    }
+                    Scratch s;
+                    const int64_t a = n_elems * t / n_threads, b = n_elems * (t + 1) / n_threads;
+                    for (int64_t i = a; i < b; ++i) element_assemble(*d, elems[i], u, row_ptr, col_idx, v, r, s, &nips[t]);
+                } catch (const std::exception& ex) {
+                    errs[t] = ex.what();
+                }
+            });
+        for (auto& x : th) x.join();
+        for (auto& e : errs)
+            if (!e.empty()) throw std::runtime_error(e);
+        if (n_threads > 1) {
+            std::vector<std::thread> red;
+            for (int t = 0; t < n_threads; ++t)
+                red.emplace_back([&, t]() {
+                    const int64_t a = nnz * t / n_threads, b = nnz * (t + 1) / n_threads;
+                    for (int q = 0; q < n_threads; ++q)
+                        for (int64_t i = a; i < b; ++i) vals[i] += pv[q][i];
+                    const int64_t ra = (int64_t)n_rows * t / n_threads, rb = (int64_t)n_rows * (t + 1) / n_threads;
+                    for (int q = 0; q < n_threads; ++q)
+                        for (int64_t i = ra; i < rb; ++i) rhs[i] += pr[q][i];
+                });
+            for (auto& x : red) x.join();
+        }
+        int64_t nip = 0;
+        for (auto v : nips) nip += v;
+        return nip;
+    } catch (const std::exception& ex) {
+        g_err = ex.what();
+        return -1;
+    }
+}
+}  // extern "C"

@@ -1,0 +1,176 @@
+"""TEST INFRASTRUCTURE ONLY -- ctypes binding of oracle/liboracle.so, the FP64 CPU restatement of the
+reference assembly.  Imported by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
+only; the product never loads it.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE, "-B", "liboracle.so"], check=True)
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("oracle not built at %s (make -C oracle)" % LIB_PATH)
+    lib = ctypes.CDLL(LIB_PATH)
+    vp = ctypes.c_void_p
+    lib.oracle_last_error.restype = ctypes.c_char_p
+    lib.oracle_build.argtypes = [vp]
+    lib.oracle_build.restype = vp
+    lib.oracle_destroy.argtypes = [vp]
+    lib.oracle_counts.argtypes = [vp, vp]
+    lib.oracle_get_setup.argtypes = [vp] * 8
+    lib.oracle_assemble.argtypes = [vp, vp, vp, vp]
+    lib.oracle_assemble.restype = ctypes.c_int
+    lib.oracle_assemble_desc.argtypes = [vp, vp, vp, ctypes.c_int32, vp, vp, vp, ctypes.c_int64, ctypes.c_int64,
+                                         ctypes.c_int32]
+    lib.oracle_assemble_desc.restype = ctypes.c_int64
+    _lib = lib
+    return lib
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+class OracleProblem:
+    """Independent set-up (XTK cut, adofs, graph) + FP64 assembly of one workload spec."""
+
+    def __init__(self, spec):
+        lib = load()
+        self._h = lib.oracle_build(ctypes.byref(spec))
+        if not self._h:
+            raise RuntimeError("oracle_build: " + lib.oracle_last_error().decode())
+        c = np.zeros(6, dtype=np.int64)
+        lib.oracle_counts(self._h, _p(c))
+        self.n_elements, self.n_cut, self.n_ig, self.n_adofs_per_type, self.n_rows, self.nnz = [int(x) for x in c]
+        self.dim = spec.dim
+        self.iwg = spec.iwg
+
+    def setup(self):
+        dim = self.dim
+        nb = 8 if dim == 3 else 4
+        nm = 2 if self.iwg == 1 else 1
+        out = {
+            "elem_cell": np.zeros(self.n_elements, np.int32),
+            "ig_ptr": np.zeros(self.n_elements + 1, np.int32),
+            "ig_xi": np.zeros(self.n_ig * (dim + 1) * dim, np.float64),
+            "material": np.zeros(self.n_elements * nm, np.float64),
+            "t_adof": np.zeros(self.n_elements * nb, np.int32),
+            "row_ptr": np.zeros(self.n_rows + 1, np.int32),
+            "col_idx": np.zeros(self.nnz, np.int32),
+        }
+        load().oracle_get_setup(self._h, *[_p(out[k]) for k in
+                                           ("elem_cell", "ig_ptr", "ig_xi", "material", "t_adof", "row_ptr",
+                                            "col_idx")])
+        return out
+
+    def assemble(self, u):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        vals = np.zeros(self.nnz)
+        rhs = np.zeros(self.n_rows)
+        if load().oracle_assemble(self._h, _p(u), _p(vals), _p(rhs)) != 0:
+            raise RuntimeError("oracle_assemble: " + load().oracle_last_error().decode())
+        return vals, rhs
+
+    def __del__(self):
+        try:
+            if self._h:
+                load().oracle_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+
+def assemble_desc(desc, row_ptr, col_idx, u, e0=0, e1=None, n_threads=1):
+    """Reference algorithm on an external set descriptor + graph; returns (vals, rhs, n_ip)."""
+    lib = load()
+    row_ptr = np.ascontiguousarray(row_ptr, dtype=np.int32)
+    col_idx = np.ascontiguousarray(col_idx, dtype=np.int32)
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    n_rows = row_ptr.shape[0] - 1
+    vals = np.zeros(col_idx.shape[0])
+    rhs = np.zeros(n_rows)
+    if e1 is None:
+        e1 = desc.n_elements
+    nip = lib.oracle_assemble_desc(ctypes.byref(desc), _p(row_ptr), _p(col_idx), n_rows, _p(u), _p(vals), _p(rhs),
+                                   e0, e1, n_threads)
+    if nip < 0:
+        raise RuntimeError("oracle_assemble_desc: " + lib.oracle_last_error().decode())
+    return vals, rhs, nip
+
+
+def shape(dim, xi):
+    """Lagrange QUAD4/HEX8 N [n][nb] and dN/dxi [n][dim][nb] at points xi [n][dim]."""
+    lib = load()
+    lib.oracle_shape.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    xi = np.ascontiguousarray(xi, dtype=np.float64).reshape(-1, dim)
+    n = xi.shape[0]
+    nb = 8 if dim == 3 else 4
+    N = np.zeros((n, nb))
+    dN = np.zeros((n, dim, nb))
+    lib.oracle_shape(dim, n, _p(xi), _p(N), _p(dN))
+    return N, dN
+
+
+RULES = {"QUAD_2x2": (0, 2), "HEX_2x2x2": (1, 3), "TRI_7": (2, 2), "TET_11": (3, 3)}
+
+
+def rule(name):
+    """Points [n][dim] and weights [n] of a quadrature rule of the oracle."""
+    lib = load()
+    lib.oracle_rule.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+    lib.oracle_rule.restype = ctypes.c_int
+    rid, dim = RULES[name]
+    xi = np.zeros(64 * 3)
+    w = np.zeros(64)
+    n = lib.oracle_rule(rid, _p(xi), _p(w))
+    return xi[: n * dim].reshape(n, dim), w[:n]
+
+
+def element(problem, e, u):
+    """(ids, K_e, R_e) of element e of an OracleProblem, before the scatter."""
+    lib = load()
+    lib.oracle_element.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p,
+                                   ctypes.c_void_p, ctypes.c_void_p]
+    lib.oracle_element.restype = ctypes.c_int
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    ids = np.zeros(64, np.int32)
+    Ke = np.zeros(64 * 64)
+    Re = np.zeros(64)
+    na = lib.oracle_element(problem._h, e, _p(u), _p(ids), _p(Ke), _p(Re))
+    if na < 0:
+        raise RuntimeError("oracle_element: " + lib.oracle_last_error().decode())
+    return ids[:na].copy(), Ke[: na * na].reshape(na, na).copy(), Re[:na].copy()
+
+
+def assemble_list(desc, row_ptr, col_idx, u, elems, n_threads=1):
+    """Reference algorithm over an explicit element list; returns (vals, rhs, n_ip)."""
+    lib = load()
+    lib.oracle_assemble_list.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32,
+                                         ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                         ctypes.c_int64, ctypes.c_int32]
+    lib.oracle_assemble_list.restype = ctypes.c_int64
+    row_ptr = np.ascontiguousarray(row_ptr, dtype=np.int32)
+    col_idx = np.ascontiguousarray(col_idx, dtype=np.int32)
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    elems = np.ascontiguousarray(elems, dtype=np.int64)
+    n_rows = row_ptr.shape[0] - 1
+    vals = np.zeros(col_idx.shape[0])
+    rhs = np.zeros(n_rows)
+    nip = lib.oracle_assemble_list(ctypes.byref(desc), _p(row_ptr), _p(col_idx), n_rows, _p(u), _p(vals), _p(rhs),
+                                   _p(elems), elems.shape[0], n_threads)
+    if nip < 0:
+        raise RuntimeError("oracle_assemble_list: " + lib.oracle_last_error().decode())
+    return vals, rhs, nip

@@ -1,0 +1,246 @@
+"""The oracle against what the mathematics fixes, so that a plausible mistake in it (a dropped term, a
+wrong sign or index, a transposed operand, a wrong weight or Jacobian) fails a test:
+
+* the unit-cube hex8 Laplace stiffness (tensor product of 1D stiffness/mass, closed form);
+* strain energy of affine fields on cut meshes: u^T K u = (eps : D : eps) * V_material and
+  k |g|^2 V_material, with V_material of a planar cut in closed form (half-space inside a box);
+* patch tests on cut meshes (uniform flux / uniform stress with zero traction on the planar
+  interface) solved with Dirichlet data on the box boundary -- the analogue of the reference's XTK-cut
+  known-answer test T = 5 + 20 z (projects/FEM/MDL/test/UT_MDL_XTK_DiffusionElement.cpp:91-340);
+* R(u) computed through the flux (IWG::compute_residual) equals K u (compute_jacobian) -- the
+  linear-IWG form of IWG::check_jacobian (projects/FEM/INT/src/IWG/cl_FEM_IWG.cpp:2498-2610);
+* rigid-body null space, symmetry, volume conservation of the XTK decomposition
+  (projects/XTK/test/xtk/UT_XTK_Cut_Mesh_RegSub.cpp:172-175) and O(h^2) convergence of the sphere
+  volume;
+* the CSR graph equals the brute-force union of element topologies.
+"""
+from itertools import product
+from math import factorial
+
+import numpy as np
+import pytest
+import scipy.sparse.linalg as spla
+
+import moris_amd as M
+from helpers import adof_coords, csr_matvec, csr_to_scipy, rel, spec_sphere3
+from oracle import py as O
+
+
+def _plane_spec(dim, n, normal, d, iwg, E=1.0, nu=0.3):
+    nrm = np.asarray(normal, float)
+    nrm = nrm / np.linalg.norm(nrm)
+    plane = list(nrm) + [0.0] * (3 - dim) + [d]
+    return M.make_spec(dim, (n,) * dim, iwg=iwg, ls_kind=M.LS_PLANE, plane=plane, E=E, nu=nu, k_sigma=0.0), nrm
+
+
+def _halfspace_volume(nrm, d):
+    """|{x in [0,1]^dim : n.x + d > 0}| for n with positive components (closed form)."""
+    dim = len(nrm)
+    below = 0.0
+    for v in product((0, 1), repeat=dim):
+        t = -d - np.dot(nrm, v)
+        below += (-1) ** sum(v) * max(0.0, t) ** dim
+    below /= factorial(dim) * np.prod(nrm)
+    return 1.0 - below
+
+
+def _sub_volumes(setup, dim):
+    nv = dim + 1
+    X = setup["ig_xi"].reshape(-1, nv, dim)
+    J = X[:, :dim, :] - X[:, dim:dim + 1, :] if dim == 2 else X[:, [0, 1, 3], :] - X[:, 2:3, :]
+    fac = 2.0 if dim == 2 else 6.0
+    return np.linalg.det(J) / fac   # in parametric units of the IP cell
+
+
+def _oracle(spec_keep):
+    spec, keep = spec_keep
+    return O.OracleProblem(spec), spec, keep
+
+
+def test_hex8_unit_cube_laplace_stiffness_closed_form():
+    n = 2
+    (spec, keep) = M.make_spec(3, (n, n, n), iwg=M.IWG_SPATIALDIFF_BULK, k_sigma=0.0)
+    P = O.OracleProblem(spec)
+    u = np.zeros(P.n_rows)
+    ids, Ke, Re = O.element(P, 0, u)
+    h = 1.0 / n
+    # 1D linear element on [0,h]: stiffness [[1,-1],[-1,1]]/h, mass [[2,1],[1,2]] h/6
+    k1 = np.array([[1.0, -1.0], [-1.0, 1.0]]) / h
+    m1 = np.array([[2.0, 1.0], [1.0, 2.0]]) * h / 6.0
+    offs = [(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0), (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)]
+    Kc = np.zeros((8, 8))
+    for a, oa in enumerate(offs):
+        for b, ob in enumerate(offs):
+            Kc[a, b] = (k1[oa[0], ob[0]] * m1[oa[1], ob[1]] * m1[oa[2], ob[2]] +
+                        m1[oa[0], ob[0]] * k1[oa[1], ob[1]] * m1[oa[2], ob[2]] +
+                        m1[oa[0], ob[0]] * m1[oa[1], ob[1]] * k1[oa[2], ob[2]])
+    # oracle K_e is in sorted adof order: vertex order 0,1,3,2,4,5,7,6 on the lattice
+    perm = [0, 1, 3, 2, 4, 5, 7, 6]
+    assert np.max(np.abs(Ke - Kc[np.ix_(perm, perm)])) < 1e-14
+
+
+@pytest.mark.parametrize("dim,iwg", [(3, 0), (3, 1), (2, 0), (2, 1)])
+def test_affine_field_energy_on_planar_cut(dim, iwg):
+    n = 6 if dim == 3 else 12
+    (spec, keep), nrm = _plane_spec(dim, n, [0.3, 0.5, 0.8][:dim], -0.45 * sum([0.3, 0.5, 0.8][:dim]) /
+                                    np.linalg.norm([0.3, 0.5, 0.8][:dim]), iwg)
+    P = O.OracleProblem(spec)
+    s = P.setup()
+    V = _halfspace_volume(nrm, spec.plane[3])
+    _, X = adof_coords(s, dim, (n,) * dim)
+    rp, ci = s["row_ptr"], s["col_idx"]
+    E, nu = 1.0, 0.3
+    if iwg == 0:
+        g = np.array([0.7, -0.4, 0.2][:dim])
+        u = X @ g
+        vals, _ = P.assemble(u)
+        energy = u @ csr_matvec(rp, ci, vals, u)
+        assert abs(energy - g @ g * V) < 1e-11 * max(1.0, g @ g * V)
+    else:
+        N = P.n_adofs_per_type
+        # u_x = a x (normal strain), then u_x = gam * y (engineering shear)
+        for case in ("normal", "shear"):
+            u = np.zeros(dim * N)
+            if case == "normal":
+                u[:N] = 0.01 * X[:, 0]
+                if dim == 3:
+                    expect = 0.01 ** 2 * E * (1 - nu) / ((1 + nu) * (1 - 2 * nu)) * V
+                else:
+                    expect = 0.01 ** 2 * E / (1 - nu ** 2) * V
+            else:
+                u[:N] = 0.01 * X[:, 1]
+                expect = 0.01 ** 2 * E / (2 * (1 + nu)) * V
+            vals, _ = P.assemble(u)
+            energy = u @ csr_matvec(rp, ci, vals, u)
+            assert abs(energy - expect) < 1e-12 * max(expect, 1e-30) + 1e-16, (case, energy, expect)
+
+
+@pytest.mark.parametrize("dim,iwg", [(3, 0), (3, 1), (2, 0), (2, 1)])
+def test_patch_test_on_planar_cut(dim, iwg):
+    n = 5 if dim == 3 else 10
+    nrm0 = [0.3, 0.5, 0.8][:dim]
+    (spec, keep), nrm = _plane_spec(dim, n, nrm0, -0.47 * sum(nrm0) / np.linalg.norm(nrm0), iwg)
+    P = O.OracleProblem(spec)
+    s = P.setup()
+    _, X = adof_coords(s, dim, (n,) * dim)
+    N = P.n_adofs_per_type
+    on_box = np.any((np.abs(X) < 1e-12) | (np.abs(X - 1.0) < 1e-12), axis=1)
+    # a field that satisfies the natural (zero-flux / traction-free) condition on the plane
+    t = np.cross(nrm, [1.0, 0.0, 0.0]) if dim == 3 else np.array([-nrm[1], nrm[0]])
+    t = t / np.linalg.norm(t)
+    if iwg == 0:
+        u_ex = X @ (0.8 * t)
+        fixed = on_box
+    else:
+        E, nu, sgm = 1.0, 0.3, 0.02
+        I = np.eye(dim)
+        eps = (sgm / E) * ((1 + nu) * np.outer(t, t) - nu * I)
+        u_ex = np.concatenate([X @ eps[i] for i in range(dim)])
+        fixed = np.tile(on_box, dim)
+    vals, R = P.assemble(u_ex)
+    K = csr_to_scipy(s["row_ptr"], s["col_idx"], vals, P.n_rows)
+    free = ~fixed
+    # the exact field is in equilibrium at every free row (residual at roundoff level) ...
+    scale = abs(K).max() * np.max(np.abs(u_ex))
+    assert np.max(np.abs(R[free])) < 1e-13 * scale
+    # ... and the Dirichlet solve reproduces it (limited by the conditioning of sliver cut cells)
+    rhs = -K[free][:, fixed] @ u_ex[fixed]
+    u_free = spla.spsolve(K[free][:, free].tocsc(), rhs)
+    assert rel(u_free, u_ex[free]) < 1e-6
+
+
+@pytest.mark.parametrize("iwg", [0, 1])
+def test_residual_equals_jacobian_times_u_and_symmetry(iwg):
+    P = O.OracleProblem(spec_sphere3(6, iwg=iwg)[0])
+    s = P.setup()
+    u = np.random.default_rng(3).uniform(-1, 1, P.n_rows)
+    vals, R = P.assemble(u)
+    Ku = csr_matvec(s["row_ptr"], s["col_idx"], vals, u)
+    assert rel(R, Ku) < 1e-13
+    K = csr_to_scipy(s["row_ptr"], s["col_idx"], vals, P.n_rows)
+    assert abs(K - K.T).max() <= 1e-14 * abs(K).max()
+
+
+def test_rigid_body_null_space():
+    # diffusion: constants; elasticity: three translations and three rotations
+    P = O.OracleProblem(spec_sphere3(5, iwg=0)[0])
+    vals, R = P.assemble(np.ones(P.n_rows))
+    assert np.max(np.abs(R)) < 1e-13 * np.max(np.abs(vals))
+    (spec, keep) = spec_sphere3(5, iwg=1)
+    P = O.OracleProblem(spec)
+    s = P.setup()
+    _, X = adof_coords(s, 3, (5, 5, 5))
+    N = P.n_adofs_per_type
+    modes = []
+    for d in range(3):
+        u = np.zeros(3 * N)
+        u[d * N:(d + 1) * N] = 1.0
+        modes.append(u)
+    for (a, b) in [(0, 1), (1, 2), (0, 2)]:
+        u = np.zeros(3 * N)
+        u[a * N:(a + 1) * N] = -X[:, b]
+        u[b * N:(b + 1) * N] = X[:, a]
+        modes.append(u)
+    for u in modes:
+        vals, R = P.assemble(u)
+        assert np.max(np.abs(R)) < 1e-13 * np.max(np.abs(vals))
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_decomposition_conserves_volume_and_orientation(dim):
+    n = 7
+    if dim == 3:
+        mk = lambda ph: M.make_spec(3, (n, n, n), spheres=[[0.43, 0.52, 0.47, 0.31]], material_phase=ph)
+    else:
+        mk = lambda ph: M.make_spec(2, (n, n), spheres=[[0.43, 0.52, 0.0, 0.31]], material_phase=ph)
+    vols = {}
+    for ph in (0, 1):
+        spec, keep = mk(ph)
+        P = O.OracleProblem(spec)
+        s = P.setup()
+        v = _sub_volumes(s, dim)
+        assert np.all(v > 0.0)            # templates keep the positive orientation
+        per = np.add.reduceat(v, s["ig_ptr"][:-1][np.diff(s["ig_ptr"]) > 0]) if v.size else v
+        cells = s["elem_cell"][np.diff(s["ig_ptr"]) > 0]
+        vols[ph] = dict(zip(cells.tolist(), per.tolist()))
+    full = 2.0 ** dim
+    common = set(vols[0]) & set(vols[1])
+    assert len(common) > 10
+    for c in common:
+        assert abs(vols[0][c] + vols[1][c] - full) < 1e-13
+
+
+def test_sphere_volume_converges_quadratically():
+    errs = []
+    exact = 1.0 - 4.0 / 3.0 * np.pi * 0.3 ** 3
+    for n in (8, 16, 32):
+        spec, keep = spec_sphere3(n, k_sigma=0.0)
+        P = O.OracleProblem(spec)
+        s = P.setup()
+        h = 1.0 / n
+        cut = np.diff(s["ig_ptr"]) > 0
+        V = (np.count_nonzero(~cut) + np.sum(_sub_volumes(s, 3)) / 8.0) * h ** 3
+        errs.append(abs(V - exact))
+    assert errs[2] < errs[1] < errs[0]
+    assert errs[1] / errs[2] > 3.0     # ~4 for O(h^2)
+
+
+def test_graph_is_union_of_element_topologies():
+    spec, keep = spec_sphere3(4, iwg=1)
+    P = O.OracleProblem(spec)
+    s = P.setup()
+    N = P.n_adofs_per_type
+    t = s["t_adof"].reshape(-1, 8)
+    rows = [set() for _ in range(3 * N)]
+    for e in range(t.shape[0]):
+        ids = sorted(set(t[e].tolist()))
+        full = [a + f * N for f in range(3) for a in ids]
+        for r in full:
+            rows[r].update(full)
+    rp = [0]
+    ci = []
+    for r in rows:
+        ci.extend(sorted(r))
+        rp.append(len(ci))
+    assert np.array_equal(np.array(rp), s["row_ptr"])
+    assert np.array_equal(np.array(ci), s["col_idx"])
