@@ -2,7 +2,8 @@
 // drive them.  Target: gfx950 (MI355X), 64-wide wavefronts.
 //
 // Path of one mgpu_set_assemble call on a lattice set:
-//   k_cut_moments   one wavefront per cut element.  For every IG sub-cell and every point of its
+//   k_cut_diffusion / k_cut_elasticity
+//                   one wavefront per cut element.  For every IG sub-cell and every point of its
 //                   rule (TET_11 / TRI_7): map the point to the IP cell (Space_Interpolator::
 //                   map_integration_point, cl_MTK_Space_Interpolator.cpp:1381-1397), take detJ of
 //                   the sub-cell (:1180-1210 tet, :1120-1128 tri), skip if detJ < -1e-6
@@ -55,6 +56,10 @@ struct DevView {
     const uint8_t* vert_pos;    // [n_el][nb]
     double* ke_cut;             // [n_cut][na][na]
     int32_t* err_flag;          // [1]
+    double* m_ref;              // reference moments of an uncut lattice cell
+    double* cell_rec;           // per lattice cell: diffusion [1] {k | -(cut+1) | 0 void},
+                                // elasticity [4] {flag, D_nn, D_nn', D_shear}
+    double* u_node;             // per lattice node: u of its adofs [n_nodes][nf] (0 if unused)
 };
 
 // Voigt index of the symmetric pair (i, j): 3D [xx,yy,zz,yz,xz,xy]
@@ -114,152 +119,282 @@ __device__ inline void lagrange_dNdxi(const double* xi, double (*out)[DIM == 3 ?
 }
 
 // ------------------------------------------------------------------------------------------------
-// k_cut_moments: one wavefront per cut element
+// Integration point ip of the cut element's IG sub-cells: weight w* and IP-cell gradients g[i][a].
 // ------------------------------------------------------------------------------------------------
-template <int DIM, int NF>
-__global__ __launch_bounds__(64) void k_cut_moments(DevView s) {
-    constexpr int NB = DIM == 3 ? 8 : 4;
-    constexpr int NG = DIM * NB;             // gradient vector length
-    constexpr int NA = NF * NB;              // element adofs
-    constexpr int NV = DIM + 1;              // IG simplex vertices
+template <int DIM>
+__device__ inline double cut_point(const DevView& s, int ig0, int ip, const double* hh,
+                                   double (*g)[DIM == 3 ? 8 : 4]) {
+    constexpr int NV = DIM + 1;
     constexpr int NQ = DIM == 3 ? kTet11Points : kTri7Points;
-    constexpr bool DIFF = NF == 1;
-    // diffusion needs S[a][b] = sum_i M[(i,a),(i,b)] only
-    constexpr int NACC_ENT = DIFF ? NB * NB : NG * NG;
-    constexpr int NACC = (NACC_ENT + 63) / 64;
-    constexpr int GPAD = NG + 1;
+    const int sc = ig0 + ip / NQ;
+    const int q = ip % NQ;
+    const double* X = s.ig_xi + (size_t)sc * NV * DIM;
+    double N[NV];
+    if constexpr (DIM == 3) {
+        N[0] = kTet11Xi[q][0];
+        N[1] = kTet11Xi[q][1];
+        N[2] = 1.0 - kTet11Xi[q][0] - kTet11Xi[q][1] - kTet11Xi[q][2];
+        N[3] = kTet11Xi[q][2];
+    } else {
+        N[0] = kTri7Xi[q][0];
+        N[1] = kTri7Xi[q][1];
+        N[2] = 1.0 - kTri7Xi[q][0] - kTri7Xi[q][1];
+    }
+    double xip[DIM];
+    for (int d = 0; d < DIM; ++d) {
+        double v = 0.0;
+        for (int k = 0; k < NV; ++k) v += N[k] * X[k * DIM + d];
+        xip[d] = v;
+    }
+    // detJ of the simplex in physical space: rows X_r - X_2 (r = 0, 1 [, 3]) scaled by h/2
+    double detJ;
+    if constexpr (DIM == 3) {
+        double J[3][3];
+        const int rr[3] = {0, 1, 3};
+        for (int r = 0; r < 3; ++r)
+            for (int d = 0; d < 3; ++d) J[r][d] = (X[rr[r] * 3 + d] - X[2 * 3 + d]) * (0.5 * hh[d]);
+        detJ = (J[0][0] * (J[1][1] * J[2][2] - J[2][1] * J[1][2]) -
+                J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0])) / 6.0;
+    } else {
+        double J[2][2];
+        for (int r = 0; r < 2; ++r)
+            for (int d = 0; d < 2; ++d) J[r][d] = (X[r * 2 + d] - X[2 * 2 + d]) * (0.5 * hh[d]);
+        detJ = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) / 2.0;
+    }
+    const double w = detJ < kDetJLowerLimit ? 0.0 : (DIM == 3 ? kTet11W[q] : kTri7W[q]) * detJ;
+    lagrange_dNdxi<DIM>(xip, g);
+    for (int i = 0; i < DIM; ++i)
+        for (int a = 0; a < (DIM == 3 ? 8 : 4); ++a) g[i][a] *= 2.0 / hh[i];
+    return w;
+}
 
-    __shared__ double sg[64 * GPAD];
-    __shared__ double sw[64];
-    __shared__ double sM[NACC_ENT];
+// ------------------------------------------------------------------------------------------------
+// k_cut_diffusion: one wavefront per cut element.  Each lane integrates every 64th point and keeps
+// the symmetric moments S[a][b] = sum_ip w* g_a . g_b in registers; one LDS reduction per element.
+// ------------------------------------------------------------------------------------------------
+template <int DIM>
+__global__ __launch_bounds__(64) void k_cut_diffusion(DevView s) {
+    constexpr int NB = DIM == 3 ? 8 : 4;
+    constexpr int NV = DIM + 1;
+    constexpr int NQ = DIM == 3 ? kTet11Points : kTri7Points;
+    constexpr int NSYM = NB * (NB + 1) / 2;
+    __shared__ double red[NSYM * 65];
 
     const int c = blockIdx.x;
     const int lane = threadIdx.x;
     const int e = s.cut_elems[c];
-    const int ig0 = s.ig_ptr[e], ig1 = s.ig_ptr[e + 1];
-    const int nip = (ig1 - ig0) * NQ;
+    const int ig0 = s.ig_ptr[e];
+    const int nip = (s.ig_ptr[e + 1] - ig0) * NQ;
     double hh[DIM];
     for (int d = 0; d < DIM; ++d) hh[d] = s.h[d];
+    (void)NV;
 
-    double acc[NACC];
+    double acc[NSYM];
 #pragma unroll
-    for (int t = 0; t < NACC; ++t) acc[t] = 0.0;
-
-    for (int base = 0; base < nip; base += 64) {
-        const int ip = base + lane;
-        double* g = sg + lane * GPAD;
-        if (ip < nip) {
-            const int sc = ig0 + ip / NQ;
-            const int q = ip % NQ;
-            const double* X = s.ig_xi + (size_t)sc * NV * DIM;
-            double N[NV];
-            double xq[DIM];
-            if constexpr (DIM == 3) {
-                xq[0] = kTet11Xi[q][0];
-                xq[1] = kTet11Xi[q][1];
-                xq[2] = kTet11Xi[q][2];
-                N[0] = xq[0];
-                N[1] = xq[1];
-                N[2] = 1.0 - xq[0] - xq[1] - xq[2];
-                N[3] = xq[2];
-            } else {
-                xq[0] = kTri7Xi[q][0];
-                xq[1] = kTri7Xi[q][1];
-                N[0] = xq[0];
-                N[1] = xq[1];
-                N[2] = 1.0 - xq[0] - xq[1];
-            }
-            double xip[DIM];
-            for (int d = 0; d < DIM; ++d) {
+    for (int t = 0; t < NSYM; ++t) acc[t] = 0.0;
+    for (int ip = lane; ip < nip; ip += 64) {
+        double g[DIM][NB];
+        const double w = cut_point<DIM>(s, ig0, ip, hh, g);
+        int t = 0;
+#pragma unroll
+        for (int a = 0; a < NB; ++a)
+#pragma unroll
+            for (int b = a; b < NB; ++b) {
                 double v = 0.0;
-                for (int k = 0; k < NV; ++k) v += N[k] * X[k * DIM + d];
-                xip[d] = v;
-            }
-            // detJ of the simplex in physical space: rows X_r - X_2 (r = 0, 1 [, 3]) scaled by h/2
-            double detJ;
-            if constexpr (DIM == 3) {
-                double J[3][3];
-                const int rr[3] = {0, 1, 3};
-                for (int r = 0; r < 3; ++r)
-                    for (int d = 0; d < 3; ++d) J[r][d] = (X[rr[r] * 3 + d] - X[2 * 3 + d]) * (0.5 * hh[d]);
-                detJ = (J[0][0] * (J[1][1] * J[2][2] - J[2][1] * J[1][2]) -
-                        J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
-                        J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0])) / 6.0;
-            } else {
-                double J[2][2];
-                for (int r = 0; r < 2; ++r)
-                    for (int d = 0; d < 2; ++d) J[r][d] = (X[r * 2 + d] - X[2 * 2 + d]) * (0.5 * hh[d]);
-                detJ = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) / 2.0;
-            }
-            const double w = detJ < kDetJLowerLimit ? 0.0 : (DIM == 3 ? kTet11W[q] : kTri7W[q]) * detJ;
-            double dN[DIM][NB];
-            lagrange_dNdxi<DIM>(xip, dN);
-            for (int i = 0; i < DIM; ++i)
-                for (int a = 0; a < NB; ++a) g[i * NB + a] = dN[i][a] * (2.0 / hh[i]);
-            sw[lane] = w;
-        } else {
-            for (int k = 0; k < NG; ++k) g[k] = 0.0;
-            sw[lane] = 0.0;
-        }
-        __syncthreads();
-        const int cnt = min(64, nip - base);
-        for (int p = 0; p < cnt; ++p) {
-            const double w = sw[p];
-            const double* gp = sg + p * GPAD;
 #pragma unroll
-            for (int t = 0; t < NACC; ++t) {
-                const int idx = lane + 64 * t;
-                if (idx < NACC_ENT) {
-                    if (DIFF) {
-                        const int a = idx / NB, b = idx % NB;
-                        double v = 0.0;
-#pragma unroll
-                        for (int i = 0; i < DIM; ++i) v += gp[i * NB + a] * gp[i * NB + b];
-                        acc[t] += w * v;
-                    } else {
-                        const int r = idx / NG, col = idx % NG;
-                        acc[t] += w * gp[r] * gp[col];
-                    }
-                }
+                for (int i = 0; i < DIM; ++i) v += g[i][a] * g[i][b];
+                acc[t++] += w * v;
             }
-        }
-        __syncthreads();
     }
 #pragma unroll
-    for (int t = 0; t < NACC; ++t) {
-        const int idx = lane + 64 * t;
-        if (idx < NACC_ENT) sM[idx] = acc[t];
-    }
+    for (int t = 0; t < NSYM; ++t) red[t * 65 + lane] = acc[t];
     __syncthreads();
-
-    // contraction with the constitutive matrix + projection to sorted adof order
     const uint8_t* vp = s.vert_pos + (size_t)e * NB;
-    double* ke = s.ke_cut + (size_t)c * NA * NA;
-    if (DIFF) {
-        const double k = s.material[e];
-        for (int idx = lane; idx < NB * NB; idx += 64) {
-            const int a = idx / NB, b = idx % NB;
-            ke[vp[a] * NA + vp[b]] = k * sM[idx];
-        }
-    } else {
-        constexpr int NS = DIM == 3 ? 6 : 3;
-        double D[NS][NS];
-        elastic_D<DIM>(s.material[2 * e], s.material[2 * e + 1], D);
-        for (int idx = lane; idx < NA * NA; idx += 64) {
-            const int A = idx / NA, B = idx % NA;
-            const int al = A / NB, a = A % NB, be = B / NB, b = B % NB;
-            double v = 0.0;
-            for (int i = 0; i < DIM; ++i)
-                for (int j = 0; j < DIM; ++j)
-                    if (d_nonzero<DIM>(voigt<DIM>(al, i), voigt<DIM>(be, j)))
-                        v += D[voigt<DIM>(al, i)][voigt<DIM>(be, j)] * sM[(i * NB + a) * NG + (j * NB + b)];
-            ke[(al * NB + vp[a]) * NA + (be * NB + vp[b])] = v;
-        }
+    double* ke = s.ke_cut + (size_t)c * NB * NB;
+    const double k = s.material[e];
+    if (lane < NSYM) {
+        double v = 0.0;
+        for (int l = 0; l < 64; ++l) v += red[lane * 65 + l];
+        // (a, b) of the packed upper triangle
+        int a = 0, t = lane;
+        while (t >= NB - a) { t -= NB - a; ++a; }
+        const int b = a + t;
+        ke[vp[a] * NB + vp[b]] = k * v;
+        ke[vp[b] * NB + vp[a]] = k * v;
     }
 }
 
 // ------------------------------------------------------------------------------------------------
-// k_stencil: one thread per CSR row (dof type AL given by blockIdx.y)
+// k_cut_elasticity: one wavefront per cut element.  Points are evaluated 64 at a time into LDS; lane
+// (a, b) accumulates the 3x3 (2x2) direction block M[(i,a),(j,b)] = sum w* g_ia g_jb over all points,
+// then contracts it with D (B^T D B) and projects to sorted adof order.
+// ------------------------------------------------------------------------------------------------
+template <int DIM>
+__global__ __launch_bounds__(64) void k_cut_elasticity(DevView s) {
+    constexpr int NB = DIM == 3 ? 8 : 4;
+    constexpr int NG = DIM * NB;
+    constexpr int NA = DIM * NB;
+    constexpr int NQ = DIM == 3 ? kTet11Points : kTri7Points;
+    constexpr int GPAD = NG + 1;
+    __shared__ double sgw[64 * GPAD];
+    __shared__ double sg[64 * GPAD];
+
+    const int c = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int e = s.cut_elems[c];
+    const int ig0 = s.ig_ptr[e];
+    const int nip = (s.ig_ptr[e + 1] - ig0) * NQ;
+    double hh[DIM];
+    for (int d = 0; d < DIM; ++d) hh[d] = s.h[d];
+    const int a = (lane / NB) % NB, b = lane % NB;
+    const bool active = lane < NB * NB;
+
+    double M[DIM][DIM];
+#pragma unroll
+    for (int i = 0; i < DIM; ++i)
+#pragma unroll
+        for (int j = 0; j < DIM; ++j) M[i][j] = 0.0;
+
+    for (int base = 0; base < nip; base += 64) {
+        const int ip = base + lane;
+        if (ip < nip) {
+            double g[DIM][NB];
+            const double w = cut_point<DIM>(s, ig0, ip, hh, g);
+            for (int i = 0; i < DIM; ++i)
+                for (int q = 0; q < NB; ++q) {
+                    sg[lane * GPAD + i * NB + q] = g[i][q];
+                    sgw[lane * GPAD + i * NB + q] = w * g[i][q];
+                }
+        } else {
+            for (int k = 0; k < NG; ++k) {
+                sg[lane * GPAD + k] = 0.0;
+                sgw[lane * GPAD + k] = 0.0;
+            }
+        }
+        __syncthreads();
+        const int cnt = min(64, nip - base);
+        if (active) {
+            for (int p = 0; p < cnt; ++p) {
+                double ga[DIM], gb[DIM];
+#pragma unroll
+                for (int i = 0; i < DIM; ++i) {
+                    ga[i] = sgw[p * GPAD + i * NB + a];
+                    gb[i] = sg[p * GPAD + i * NB + b];
+                }
+#pragma unroll
+                for (int i = 0; i < DIM; ++i)
+#pragma unroll
+                    for (int j = 0; j < DIM; ++j) M[i][j] += ga[i] * gb[j];
+            }
+        }
+        __syncthreads();
+    }
+    if (!active) return;
+    constexpr int NS = DIM == 3 ? 6 : 3;
+    double D[NS][NS];
+    elastic_D<DIM>(s.material[2 * e], s.material[2 * e + 1], D);
+    const uint8_t* vp = s.vert_pos + (size_t)e * NB;
+    double* ke = s.ke_cut + (size_t)c * NA * NA;
+#pragma unroll
+    for (int al = 0; al < DIM; ++al)
+#pragma unroll
+        for (int be = 0; be < DIM; ++be) {
+            double v = 0.0;
+#pragma unroll
+            for (int i = 0; i < DIM; ++i)
+#pragma unroll
+                for (int j = 0; j < DIM; ++j)
+                    if (d_nonzero<DIM>(voigt<DIM>(al, i), voigt<DIM>(be, j)))
+                        v += D[voigt<DIM>(al, i)][voigt<DIM>(be, j)] * M[i][j];
+            ke[(al * NB + vp[a]) * NA + (be * NB + vp[b])] = v;
+        }
+}
+
+// ------------------------------------------------------------------------------------------------
+// k_prep: per-call constants of the stencil kernel.  Block 0 computes the reference moments of an
+// uncut lattice cell (IG cell = IP cell, HEX_2x2x2 / QUAD_2x2, detJ = prod(h/2), dN/dx = dN/dxi 2/h):
+//   diffusion   S[a][b]        = sum_q w* g_a . g_b
+//   elasticity  M[(i,a),(j,b)] = sum_q w* g_ia g_jb
+// the other blocks write one record per lattice cell (void / uncut material / cut index) with the
+// material folded in: k (CM_Diffusion_Linear_Isotropic) or the three distinct entries of the
+// isotropic D (CM_Struc_Linear_Isotropic::full_3d / full_plane_stress).
+// ------------------------------------------------------------------------------------------------
+template <int DIM, int NF>
+__global__ __launch_bounds__(256) void k_prep(DevView s, const double* u) {
+    constexpr int NB = DIM == 3 ? 8 : 4;
+    constexpr int NG = DIM * NB;
+    constexpr bool DIFF = NF == 1;
+    constexpr int NM = DIFF ? NB * NB : NG * NG;
+    constexpr int NQ = DIM == 3 ? kHex2Points : kQuad2Points;
+    if (blockIdx.x == 0) {
+        double detJ = 1.0;
+        for (int d = 0; d < DIM; ++d) detJ *= 0.5 * s.h[d];
+        for (int idx = threadIdx.x; idx < NM; idx += blockDim.x) {
+            double v = 0.0;
+            for (int q = 0; q < NQ; ++q) {
+                double dN[DIM][NB];
+                double xq[DIM];
+                for (int d = 0; d < DIM; ++d) xq[d] = DIM == 3 ? kHex2Xi[q][d] : kQuad2Xi[q][d];
+                lagrange_dNdxi<DIM>(xq, dN);
+                const double w = (DIM == 3 ? kHex2W[q] : kQuad2W[q]) * detJ;
+                if (DIFF) {
+                    const int a = idx / NB, b = idx % NB;
+                    double t = 0.0;
+                    for (int i = 0; i < DIM; ++i) t += (dN[i][a] * (2.0 / s.h[i])) * (dN[i][b] * (2.0 / s.h[i]));
+                    v += w * t;
+                } else {
+                    const int ra = idx / NG, cb = idx % NG;
+                    const int i = ra / NB, a = ra % NB, jj = cb / NB, b = cb % NB;
+                    v += w * (dN[i][a] * (2.0 / s.h[i])) * (dN[jj][b] * (2.0 / s.h[jj]));
+                }
+            }
+            s.m_ref[idx] = v;
+        }
+        return;
+    }
+    const int64_t c = (int64_t)(blockIdx.x - 1) * blockDim.x + threadIdx.x;
+    const int64_t n_cells = (int64_t)s.nx * s.ny * s.nz;
+    const int64_t n_nodes = (int64_t)(s.nx + 1) * (s.ny + 1) * (DIM == 3 ? s.nz + 1 : 1);
+    if (c < n_nodes) {
+        // u gathered per lattice node so that rows read it without the node -> adof hop
+        const int a = s.node_adof[c];
+#pragma unroll
+        for (int f = 0; f < NF; ++f) s.u_node[(size_t)NF * c + f] = a >= 0 ? u[(size_t)f * s.N + a] : 0.0;
+    }
+    if (c >= n_cells) return;
+    const int e = s.cell_elem[c];
+    if (DIFF) {
+        double v = 0.0;
+        if (e >= 0) {
+            const int cut = s.elem_cut[e];
+            v = cut >= 0 ? -(double)(cut + 1) : s.material[e];
+        }
+        s.cell_rec[c] = v;
+    } else {
+        double r[4] = {0.0, 0.0, 0.0, 0.0};
+        if (e >= 0) {
+            const int cut = s.elem_cut[e];
+            if (cut >= 0) {
+                r[0] = -(double)(cut + 1);
+            } else {
+                constexpr int NS = DIM == 3 ? 6 : 3;
+                double D[NS][NS];
+                elastic_D<DIM>(s.material[2 * e], s.material[2 * e + 1], D);
+                r[0] = 1.0;
+                r[1] = D[0][0];
+                r[2] = D[0][1];
+                r[3] = D[NS - 1][NS - 1];
+            }
+        }
+        for (int k = 0; k < 4; ++k) s.cell_rec[4 * c + k] = r[k];
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+// k_stencil: one thread per (CSR row, column dof type BE); the dof type AL of the row is uniform
+// per block.  The thread sums the (AL, BE) block of the K_e rows of the lattice cells around its
+// node: uncut cells from the reference moments and the cell record, cut cells from the stored K_e.
 // ------------------------------------------------------------------------------------------------
 // vertex order -> sorted position for lattice cells (adof ids ascend with lattice node index)
 template <int DIM>
@@ -267,9 +402,15 @@ __device__ constexpr int lattice_sorted_pos(int b) {
     return DIM == 3 ? (b == 2 ? 3 : (b == 3 ? 2 : (b == 6 ? 7 : (b == 7 ? 6 : b)))) : (b == 2 ? 3 : (b == 3 ? 2 : b));
 }
 
-template <int DIM, int NF, int AL>
+// rows per block; a block has ROWS * NF threads and stages its contiguous CSR range in LDS
+template <int DIM, int NF>
+__host__ __device__ constexpr int stencil_rows() {
+    return NF == 1 ? 128 : (DIM == 3 ? 64 : 128);
+}
+
+template <int DIM, int NF, int AL, int BE>
 __device__ inline void stencil_row(const DevView& s, const double* sM, int r, const double* u,
-                                   double* vals, double* rhs) {
+                                   double* stage, int stage_base, double* r_part) {
     constexpr int NB = DIM == 3 ? 8 : 4;
     constexpr int NG = DIM * NB;
     constexpr int NA = NF * NB;
@@ -281,11 +422,9 @@ __device__ inline void stencil_row(const DevView& s, const double* sM, int r, co
     const int j = (node / (nx + 1)) % (ny + 1);
     const int k = DIM == 3 ? node / ((nx + 1) * (ny + 1)) : 0;
 
-    double acc[NF][NS27];
+    double acc[NS27];
 #pragma unroll
-    for (int f = 0; f < NF; ++f)
-#pragma unroll
-        for (int q = 0; q < NS27; ++q) acc[f][q] = 0.0;
+    for (int q = 0; q < NS27; ++q) acc[q] = 0.0;
     uint32_t mask = 0;
 
 #pragma unroll
@@ -296,14 +435,21 @@ __device__ inline void stencil_row(const DevView& s, const double* sM, int r, co
         const int ci = i + cx, cj = j + cy, ck = k + cz;
         if (ci < 0 || ci >= nx || cj < 0 || cj >= ny || (DIM == 3 && (ck < 0 || ck >= nz))) continue;
         const int cell = ci + nx * (cj + ny * ck);
-        const int e = s.cell_elem[cell];
-        if (e < 0) continue;
         // local vertex of the row node in this cell
-        int a_loc = 0;
-#pragma unroll
-        for (int b = 0; b < NB; ++b) {
-            const int* off = DIM == 3 ? kHex8Off[b] : kQuad4Off[b];
-            if (off[0] == -cx && off[1] == -cy && (DIM == 2 || off[2] == -cz)) a_loc = b;
+        constexpr int a_tab[8] = {6, 7, 5, 4, 2, 3, 1, 0};   // vertex with offset (-cx,-cy,-cz)
+        constexpr int a_tab2[4] = {2, 3, 1, 0};
+        const int a_loc = DIM == 3 ? a_tab[cc] : a_tab2[cc];
+        double flag, c0 = 0.0, c1 = 0.0, cs = 0.0;
+        if (DIFF) {
+            flag = s.cell_rec[cell];
+            if (flag == 0.0) continue;
+        } else {
+            const double4 rec = *reinterpret_cast<const double4*>(s.cell_rec + 4 * (size_t)cell);
+            flag = rec.x;
+            if (flag == 0.0) continue;
+            c0 = rec.y;
+            c1 = rec.z;
+            cs = rec.w;
         }
 #pragma unroll
         for (int b = 0; b < NB; ++b) {
@@ -311,44 +457,43 @@ __device__ inline void stencil_row(const DevView& s, const double* sM, int r, co
             const int q = (cx + off[0] + 1) + 3 * (cy + off[1] + 1) + (DIM == 3 ? 9 * (cz + off[2] + 1) : 0);
             mask |= 1u << q;
         }
-        const int cut = s.elem_cut[e];
-        if (cut >= 0) {
+        if (flag < 0.0) {
+            const int cut = (int)(-flag) - 1;
             const double* ke = s.ke_cut + (size_t)cut * NA * NA + (size_t)(AL * NB + lattice_sorted_pos<DIM>(a_loc)) * NA;
-#pragma unroll
-            for (int be = 0; be < NF; ++be)
-#pragma unroll
-                for (int b = 0; b < NB; ++b) {
-                    const int* off = DIM == 3 ? kHex8Off[b] : kQuad4Off[b];
-                    const int q = (cx + off[0] + 1) + 3 * (cy + off[1] + 1) + (DIM == 3 ? 9 * (cz + off[2] + 1) : 0);
-                    acc[be][q] += ke[be * NB + lattice_sorted_pos<DIM>(b)];
-                }
-        } else if (DIFF) {
-            const double kk = s.material[e];
 #pragma unroll
             for (int b = 0; b < NB; ++b) {
                 const int* off = DIM == 3 ? kHex8Off[b] : kQuad4Off[b];
                 const int q = (cx + off[0] + 1) + 3 * (cy + off[1] + 1) + (DIM == 3 ? 9 * (cz + off[2] + 1) : 0);
-                acc[0][q] += kk * sM[a_loc * NB + b];
+                acc[q] += ke[BE * NB + lattice_sorted_pos<DIM>(b)];
+            }
+        } else if (DIFF) {
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                const int* off = DIM == 3 ? kHex8Off[b] : kQuad4Off[b];
+                const int q = (cx + off[0] + 1) + 3 * (cy + off[1] + 1) + (DIM == 3 ? 9 * (cz + off[2] + 1) : 0);
+                acc[q] += flag * sM[a_loc * NB + b];
             }
         } else {
-            constexpr int NS = DIM == 3 ? 6 : 3;
-            double D[NS][NS];
-            elastic_D<DIM>(s.material[2 * e], s.material[2 * e + 1], D);
+            // isotropic B^T D B block (AL, BE):
+            //   D[AL][BE] M[(AL,a),(BE,b)] + D_s (delta_{AL,BE} sum_{i!=AL} M[(i,a),(i,b)]
+            //                                      + (1 - delta_{AL,BE}) M[(BE,a),(AL,b)])
+            const double cab = AL == BE ? c0 : c1;
 #pragma unroll
-            for (int be = 0; be < NF; ++be)
-#pragma unroll
-                for (int b = 0; b < NB; ++b) {
-                    const int* off = DIM == 3 ? kHex8Off[b] : kQuad4Off[b];
-                    const int q = (cx + off[0] + 1) + 3 * (cy + off[1] + 1) + (DIM == 3 ? 9 * (cz + off[2] + 1) : 0);
-                    double v = 0.0;
+            for (int b = 0; b < NB; ++b) {
+                const int* off = DIM == 3 ? kHex8Off[b] : kQuad4Off[b];
+                const int q = (cx + off[0] + 1) + 3 * (cy + off[1] + 1) + (DIM == 3 ? 9 * (cz + off[2] + 1) : 0);
+                double v = cab * sM[(AL * NB + a_loc) * NG + (BE * NB + b)];
+                if (AL == BE) {
+                    double t = 0.0;
 #pragma unroll
                     for (int ii = 0; ii < DIM; ++ii)
-#pragma unroll
-                        for (int jj = 0; jj < DIM; ++jj)
-                            if (d_nonzero<DIM>(voigt<DIM>(AL, ii), voigt<DIM>(be, jj)))
-                                v += D[voigt<DIM>(AL, ii)][voigt<DIM>(be, jj)] * sM[(ii * NB + a_loc) * NG + (jj * NB + b)];
-                    acc[be][q] += v;
+                        if (ii != AL) t += sM[(ii * NB + a_loc) * NG + (ii * NB + b)];
+                    v += cs * t;
+                } else {
+                    v += cs * sM[(BE * NB + a_loc) * NG + (AL * NB + b)];
                 }
+                acc[q] += v;
+            }
         }
     }
 
@@ -357,23 +502,32 @@ __device__ inline void stencil_row(const DevView& s, const double* sM, int r, co
     const int cnt = __popc(mask);
     if (s.row_ptr[row + 1] - start != NF * cnt) {
         atomicOr(s.err_flag, 1);
+        *r_part = 0.0;
         return;
     }
     double R = 0.0;
+    int idx = start - stage_base + BE * cnt;
 #pragma unroll
-    for (int be = 0; be < NF; ++be) {
-        int idx = start + be * cnt;
-#pragma unroll
-        for (int q = 0; q < NS27; ++q) {
-            if (mask & (1u << q)) {
-                const int dx = q % 3 - 1, dy = (q / 3) % 3 - 1, dz = DIM == 3 ? q / 9 - 1 : 0;
-                const int nbr = node + dx + (nx + 1) * (dy + (ny + 1) * dz);
-                vals[idx++] = acc[be][q];
-                R += acc[be][q] * u[(size_t)be * s.N + s.node_adof[nbr]];
-            }
+    for (int q = 0; q < NS27; ++q) {
+        if (mask & (1u << q)) {
+            const int dx = q % 3 - 1, dy = (q / 3) % 3 - 1, dz = DIM == 3 ? q / 9 - 1 : 0;
+            const int nbr = node + dx + (nx + 1) * (dy + (ny + 1) * dz);
+            stage[idx++] = acc[q];
+            R += acc[q] * s.u_node[(size_t)NF * nbr + BE];
         }
     }
-    rhs[row] = R;
+    *r_part = R;
+}
+
+template <int DIM, int NF, int AL>
+__device__ inline void stencil_dispatch_be(const DevView& s, const double* sM, int r, const double* u,
+                                           double* stage, int base, double* rp, int be) {
+    switch (be) {
+        case 0: stencil_row<DIM, NF, AL, 0>(s, sM, r, u, stage, base, rp); break;
+        case 1: if constexpr (NF > 1) stencil_row<DIM, NF, AL, 1>(s, sM, r, u, stage, base, rp); break;
+        case 2: if constexpr (NF > 2) stencil_row<DIM, NF, AL, 2>(s, sM, r, u, stage, base, rp); break;
+        default: break;
+    }
 }
 
 template <int DIM, int NF>
@@ -382,42 +536,40 @@ __global__ __launch_bounds__(256) void k_stencil(DevView s, const double* u, dou
     constexpr int NG = DIM * NB;
     constexpr bool DIFF = NF == 1;
     constexpr int NM = DIFF ? NB * NB : NG * NG;
-    constexpr int NQ = DIM == 3 ? kHex2Points : kQuad2Points;
+    constexpr int ROWS = stencil_rows<DIM, NF>();
+    constexpr int NS27 = DIM == 3 ? 27 : 9;
     __shared__ double sM[NM];
-    // reference moments of an uncut lattice cell: IG cell = IP cell, HEX_2x2x2 / QUAD_2x2,
-    // detJ = prod(h/2) (box), dN/dx = dN/dxi * 2/h
-    double detJ = 1.0;
-    for (int d = 0; d < DIM; ++d) detJ *= 0.5 * s.h[d];
-    for (int idx = threadIdx.x; idx < NM; idx += blockDim.x) {
-        double v = 0.0;
-        for (int q = 0; q < NQ; ++q) {
-            double dN[DIM][NB];
-            double xq[DIM];
-            for (int d = 0; d < DIM; ++d) xq[d] = DIM == 3 ? kHex2Xi[q][d] : kQuad2Xi[q][d];
-            lagrange_dNdxi<DIM>(xq, dN);
-            const double w = (DIM == 3 ? kHex2W[q] : kQuad2W[q]) * detJ;
-            if (DIFF) {
-                const int a = idx / NB, b = idx % NB;
-                double t = 0.0;
-                for (int i = 0; i < DIM; ++i) t += (dN[i][a] * (2.0 / s.h[i])) * (dN[i][b] * (2.0 / s.h[i]));
-                v += w * t;
-            } else {
-                const int ra = idx / NG, cb = idx % NG;
-                const int i = ra / NB, a = ra % NB, jj = cb / NB, b = cb % NB;
-                v += w * (dN[i][a] * (2.0 / s.h[i])) * (dN[jj][b] * (2.0 / s.h[jj]));
-            }
+    __shared__ double stage[ROWS * NF * NS27];
+    __shared__ double rpart[NF * ROWS];
+    for (int idx = threadIdx.x; idx < NM; idx += blockDim.x) sM[idx] = s.m_ref[idx];
+    const int al = blockIdx.y;
+    const int r0 = blockIdx.x * ROWS;
+    const int r1 = min(r0 + ROWS, s.N);
+    const int base = s.row_ptr[al * s.N + r0];
+    const int end = s.row_ptr[al * s.N + r1];
+    __syncthreads();
+    const int be = threadIdx.x / ROWS;
+    const int rr = threadIdx.x % ROWS;
+    const int r = r0 + rr;
+    if (r < r1) {
+        double* rp = rpart + be * ROWS + rr;
+        switch (al) {
+            case 0: stencil_dispatch_be<DIM, NF, 0>(s, sM, r, u, stage, base, rp, be); break;
+            case 1: if constexpr (NF > 1) stencil_dispatch_be<DIM, NF, 1>(s, sM, r, u, stage, base, rp, be); break;
+            case 2: if constexpr (NF > 2) stencil_dispatch_be<DIM, NF, 2>(s, sM, r, u, stage, base, rp, be); break;
+            default: break;
         }
-        sM[idx] = v;
     }
     __syncthreads();
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= s.N) return;
-    switch (blockIdx.y) {
-        case 0: stencil_row<DIM, NF, 0>(s, sM, r, u, vals, rhs); break;
-        case 1: if (NF > 1) stencil_row<DIM, NF, (NF > 1 ? 1 : 0)>(s, sM, r, u, vals, rhs); break;
-        case 2: if (NF > 2) stencil_row<DIM, NF, (NF > 2 ? 2 : 0)>(s, sM, r, u, vals, rhs); break;
-        default: break;
+    if (be == 0 && r < r1) {
+        double R = 0.0;
+#pragma unroll
+        for (int f = 0; f < NF; ++f) R += rpart[f * ROWS + rr];
+        rhs[al * s.N + r] = R;
     }
+    const int n = end - base;
+    if (n > ROWS * NF * NS27) return;   // flagged by the row-length check
+    for (int p = threadIdx.x; p < n; p += ROWS * NF) vals[base + p] = stage[p];
 }
 
 }  // namespace mgpu
@@ -472,19 +624,30 @@ mgpu_status launch_assemble(mgpu_set* S, const double* u, double* vals, double* 
     if (ev) HIP_TRY(hipEventRecord(ev[0], S->stream));
     if (S->v.n_cut > 0) {
         dim3 g((unsigned)S->v.n_cut), b(64);
-        if (dim == 3 && nf == 1) hipLaunchKernelGGL((k_cut_moments<3, 1>), g, b, 0, S->stream, S->v);
-        else if (dim == 3 && nf == 3) hipLaunchKernelGGL((k_cut_moments<3, 3>), g, b, 0, S->stream, S->v);
-        else if (dim == 2 && nf == 1) hipLaunchKernelGGL((k_cut_moments<2, 1>), g, b, 0, S->stream, S->v);
-        else hipLaunchKernelGGL((k_cut_moments<2, 2>), g, b, 0, S->stream, S->v);
+        if (dim == 3 && nf == 1) hipLaunchKernelGGL((k_cut_diffusion<3>), g, b, 0, S->stream, S->v);
+        else if (dim == 3 && nf == 3) hipLaunchKernelGGL((k_cut_elasticity<3>), g, b, 0, S->stream, S->v);
+        else if (dim == 2 && nf == 1) hipLaunchKernelGGL((k_cut_diffusion<2>), g, b, 0, S->stream, S->v);
+        else hipLaunchKernelGGL((k_cut_elasticity<2>), g, b, 0, S->stream, S->v);
         HIP_TRY(hipGetLastError());
     }
     if (ev) HIP_TRY(hipEventRecord(ev[1], S->stream));
     {
-        dim3 g((unsigned)((S->v.N + 255) / 256), (unsigned)nf), b(256);
-        if (dim == 3 && nf == 1) hipLaunchKernelGGL((k_stencil<3, 1>), g, b, 0, S->stream, S->v, u, vals, rhs);
-        else if (dim == 3 && nf == 3) hipLaunchKernelGGL((k_stencil<3, 3>), g, b, 0, S->stream, S->v, u, vals, rhs);
-        else if (dim == 2 && nf == 1) hipLaunchKernelGGL((k_stencil<2, 1>), g, b, 0, S->stream, S->v, u, vals, rhs);
-        else hipLaunchKernelGGL((k_stencil<2, 2>), g, b, 0, S->stream, S->v, u, vals, rhs);
+        const int64_t n_nodes = (int64_t)(S->v.nx + 1) * (S->v.ny + 1) * (dim == 3 ? S->v.nz + 1 : 1);
+        dim3 gp((unsigned)(1 + (n_nodes + 255) / 256)), bp(256);
+        if (dim == 3 && nf == 1) hipLaunchKernelGGL((k_prep<3, 1>), gp, bp, 0, S->stream, S->v, u);
+        else if (dim == 3 && nf == 3) hipLaunchKernelGGL((k_prep<3, 3>), gp, bp, 0, S->stream, S->v, u);
+        else if (dim == 2 && nf == 1) hipLaunchKernelGGL((k_prep<2, 1>), gp, bp, 0, S->stream, S->v, u);
+        else hipLaunchKernelGGL((k_prep<2, 2>), gp, bp, 0, S->stream, S->v, u);
+        HIP_TRY(hipGetLastError());
+        auto grid = [&](int rows) { return dim3((unsigned)((S->v.N + rows - 1) / rows), (unsigned)nf); };
+        if (dim == 3 && nf == 1)
+            hipLaunchKernelGGL((k_stencil<3, 1>), grid(stencil_rows<3, 1>()), dim3(stencil_rows<3, 1>() * 1), 0, S->stream, S->v, u, vals, rhs);
+        else if (dim == 3 && nf == 3)
+            hipLaunchKernelGGL((k_stencil<3, 3>), grid(stencil_rows<3, 3>()), dim3(stencil_rows<3, 3>() * 3), 0, S->stream, S->v, u, vals, rhs);
+        else if (dim == 2 && nf == 1)
+            hipLaunchKernelGGL((k_stencil<2, 1>), grid(stencil_rows<2, 1>()), dim3(stencil_rows<2, 1>() * 1), 0, S->stream, S->v, u, vals, rhs);
+        else
+            hipLaunchKernelGGL((k_stencil<2, 2>), grid(stencil_rows<2, 2>()), dim3(stencil_rows<2, 2>() * 2), 0, S->stream, S->v, u, vals, rhs);
         HIP_TRY(hipGetLastError());
     }
     if (ev) HIP_TRY(hipEventRecord(ev[2], S->stream));
@@ -556,6 +719,17 @@ extern "C" mgpu_status mgpu_set_create(const mgpu_set_desc* desc, mgpu_set** out
         S->allocs.push_back(p);
         HIP_TRY(hipMemset(p, 0, sizeof(int32_t)));
         v.err_flag = static_cast<int32_t*>(p);
+        const size_t nm_ref = (size_t)(h.nf == 1 ? h.nb * h.nb : (h.dim * h.nb) * (h.dim * h.nb));
+        HIP_TRY(hipMalloc(&p, nm_ref * sizeof(double)));
+        S->allocs.push_back(p);
+        v.m_ref = static_cast<double*>(p);
+        const size_t n_cells = (size_t)v.nx * v.ny * v.nz;
+        HIP_TRY(hipMalloc(&p, std::max<size_t>(n_cells, 1) * (h.nf == 1 ? 1 : 4) * sizeof(double)));
+        S->allocs.push_back(p);
+        v.cell_rec = static_cast<double*>(p);
+        HIP_TRY(hipMalloc(&p, std::max<size_t>((size_t)h.n_nodes, 1) * h.nf * sizeof(double)));
+        S->allocs.push_back(p);
+        v.u_node = static_cast<double*>(p);
     }
     *out = S.release();
     return MGPU_OK;
