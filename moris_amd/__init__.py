@@ -276,6 +276,9 @@ class Set:
         self.n_rows = nr.value
         self.nnz = nnz.value
         self._rp, self._ci = rp, ci
+        ne = desc.n_elements
+        ig = np.ctypeslib.as_array(desc.ig_ptr, shape=(ne + 1,)) if ne else np.zeros(1, np.int32)
+        self.n_cut = int(np.count_nonzero(np.diff(ig)))
 
     def options(self, precision=PREC_FP64, quadrature=QUAD_SUBDIVISION):
         _check(load_library().mgpu_set_options(self._h, precision, quadrature), "mgpu_set_options")

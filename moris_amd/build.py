@@ -22,9 +22,12 @@ CXX = os.environ.get("CXX", "g++")
 HOST_SOURCES = [
     "setup/problem.cpp",
     "set/set_host.cpp",
+    "learned/dataset.cpp",
 ]
 HIP_SOURCES = [
     "set/assemble.hip",
+    "learned/lq_assemble.hip",
+    "learned/mlp.hip",
 ]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-I" + os.path.join(ROOT, "include")]

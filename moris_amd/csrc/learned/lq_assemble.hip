@@ -1,0 +1,181 @@
+// lq_assemble.hip -- learned cut-cell quadrature inside the assembly.
+//
+// Per cut element (hex8 IP cell):
+//   k_lq_inputs      8 nodal level-set values / max |phi| -> MLP input
+//   (MLP forward)    8 -> 64 -> 64 -> 27 on BF16 MFMA (mlp.hip): a prior for the 27 weights
+//   k_lq_fit         one wavefront per element: the 23 moments of the material sub-cells in FP64
+//                    (TET_11 on the XTK sub-tets, FP64 sub-cell Jacobians), then the moment fit
+//                    started from the MLP prior: the minimum-norm correction onto the moment
+//                    constraints (iters = 0), optionally alternated `iters` times with the
+//                    nonnegative orthant; the residual ||V w - m|| / ||m|| of the final weights
+//                    decides learned (<= tol) or fallback to subdivision
+//   k_lq_element     one wavefront per accepted element: M = sum_q w_q g_q g_q^T on FP32 MFMA
+//                    (v_mfma_f32_16x16x4f32, FP32 accumulation) over the 27 fixed points, then
+//                    B^T D B and the projection to sorted adof order in FP64
+// Elements over tolerance run the subdivision kernels (assemble.hip) unchanged.
+#include <hip/hip_runtime.h>
+
+#include "../set/set_device.hpp"
+#include "lq_view.hpp"
+#include "quad.hpp"
+
+namespace mgpu {
+
+__global__ void k_lq_inputs(LqView lv) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= lv.n_cut) return;
+    double m = 0.0;
+    for (int v = 0; v < 8; ++v) m = fmax(m, fabs(lv.vphi[8 * c + v]));
+    const double inv = m > 0.0 ? 1.0 / m : 0.0;
+    for (int v = 0; v < 8; ++v) lv.x_in[8 * c + v] = (float)(lv.vphi[8 * c + v] * inv);
+}
+
+// Parametric weight of point ip of the cut element's sub-cells and its image in the IP cell.
+__device__ inline double cut_point_param(const DevView& s, int ig0, int ip, double* xip) {
+    const int sc = ig0 + ip / kTet11Points;
+    const int q = ip % kTet11Points;
+    const double* X = s.ig_xi + (size_t)sc * 12;
+    const double N[4] = {kTet11Xi[q][0], kTet11Xi[q][1], 1.0 - kTet11Xi[q][0] - kTet11Xi[q][1] - kTet11Xi[q][2],
+                         kTet11Xi[q][2]};
+    for (int d = 0; d < 3; ++d) xip[d] = N[0] * X[d] + N[1] * X[3 + d] + N[2] * X[6 + d] + N[3] * X[9 + d];
+    double J[3][3];
+    const int rr[3] = {0, 1, 3};
+    for (int r = 0; r < 3; ++r)
+        for (int d = 0; d < 3; ++d) J[r][d] = X[rr[r] * 3 + d] - X[6 + d];
+    const double detJ = (J[0][0] * (J[1][1] * J[2][2] - J[2][1] * J[1][2]) -
+                         J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                         J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0])) / 6.0;
+    return detJ < kDetJLowerLimit ? 0.0 : kTet11W[q] * detJ;
+}
+
+__global__ __launch_bounds__(64) void k_lq_fit(DevView s, LqView lv) {
+    __shared__ double red[kLqMoments * 65];
+    __shared__ double m[kLqMoments];
+    __shared__ double w[32];
+    __shared__ double r[32];
+    const int c = blockIdx.x;
+    const int lane = threadIdx.x;
+    const int e = s.cut_elems[c];
+    const int ig0 = s.ig_ptr[e];
+    const int nip = (s.ig_ptr[e + 1] - ig0) * kTet11Points;
+    double acc[kLqMoments];
+#pragma unroll
+    for (int k = 0; k < kLqMoments; ++k) acc[k] = 0.0;
+    for (int ip = lane; ip < nip; ip += 64) {
+        double xip[3];
+        const double wq = cut_point_param(s, ig0, ip, xip);
+        double p[kLqMoments];
+        lq_monomials(xip, p);
+#pragma unroll
+        for (int k = 0; k < kLqMoments; ++k) acc[k] += wq * p[k];
+    }
+#pragma unroll
+    for (int k = 0; k < kLqMoments; ++k) red[k * 65 + lane] = acc[k];
+    __syncthreads();
+    if (lane < kLqMoments) {
+        double v = 0.0;
+        for (int l = 0; l < 64; ++l) v += red[lane * 65 + l];
+        m[lane] = v;
+    }
+    if (lane < kLqPoints) w[lane] = 8.0 * (double)lv.w_pred[(size_t)c * kLqPoints + lane];
+    __syncthreads();
+    for (int it = 0; it <= lv.iters; ++it) {
+        if (it > 0 && lane < kLqPoints) w[lane] = fmax(w[lane], 0.0);
+        __syncthreads();
+        if (lane < kLqMoments) {
+            double v = m[lane];
+            for (int q = 0; q < kLqPoints; ++q) v -= lv.Vt[q * kLqMoments + lane] * w[q];
+            r[lane] = v;
+        }
+        __syncthreads();
+        if (lane < kLqPoints) {
+            double v = 0.0;
+            for (int k = 0; k < kLqMoments; ++k) v += lv.Pt[k * kLqPoints + lane] * r[k];
+            w[lane] += v;
+        }
+        __syncthreads();
+    }
+    if (lv.iters > 0 && lane < kLqPoints) w[lane] = fmax(w[lane], 0.0);
+    __syncthreads();
+    double rk = 0.0, mk = 0.0;
+    if (lane < kLqMoments) {
+        double v = m[lane];
+        for (int q = 0; q < kLqPoints; ++q) v -= lv.Vt[q * kLqMoments + lane] * w[q];
+        rk = v * v;
+        mk = m[lane] * m[lane];
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        rk += __shfl_xor(rk, off);
+        mk += __shfl_xor(mk, off);
+    }
+    const bool ok = mk > 0.0 && sqrt(rk / mk) <= lv.tol;
+    if (lane < kLqPoints) lv.w_fit[(size_t)c * kLqPoints + lane] = (float)w[lane];
+    if (lane == 0) lv.flag[c] = ok ? 1 : 0;
+}
+
+typedef float floatx4_t __attribute__((ext_vector_type(4)));
+
+// K_e of an accepted element from its 27 weights.  NF = 1 diffusion, 3 elasticity (3D hex8).
+template <int NF>
+__global__ __launch_bounds__(64) void k_lq_element(DevView s, LqView lv) {
+    constexpr int NB = 8, NG = 24, NA = NF * NB;
+    const int c = blockIdx.x;
+    if (lv.flag[c] == 0) return;
+    __shared__ float gq[kLqPoints * NG];
+    __shared__ float wq[32];
+    __shared__ float Ms[32 * 33];
+    const int lane = threadIdx.x;
+    for (int idx = lane; idx < kLqPoints * NG; idx += 64) gq[idx] = lv.g27[idx];
+    if (lane < 32) wq[lane] = lane < kLqPoints ? (float)(lv.w_fit[(size_t)c * kLqPoints + lane] * lv.detJ_ip) : 0.0f;
+    __syncthreads();
+    // M (24 x 24, padded 32 x 32) = sum_q (w_q g_q) g_q^T, K = 28 points in steps of 4
+    for (int mt = 0; mt < 2; ++mt)
+        for (int nt = 0; nt < 2; ++nt) {
+            floatx4_t acc = {0.f, 0.f, 0.f, 0.f};
+            const int mrow = mt * 16 + lane % 16, ncol = nt * 16 + lane % 16;
+            for (int ks = 0; ks < 7; ++ks) {
+                const int q = ks * 4 + lane / 16;
+                const float a = (q < kLqPoints && mrow < NG) ? wq[q] * gq[q * NG + mrow] : 0.0f;
+                const float b = (q < kLqPoints && ncol < NG) ? gq[q * NG + ncol] : 0.0f;
+                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
+            }
+            for (int j = 0; j < 4; ++j) Ms[(mt * 16 + 4 * (lane / 16) + j) * 33 + ncol] = acc[j];
+        }
+    __syncthreads();
+    const int e = s.cut_elems[c];
+    const uint8_t* vp = s.vert_pos + (size_t)e * NB;
+    double* ke = s.ke_cut + (size_t)c * NA * NA;
+    const int a = lane / NB, b = lane % NB;
+    if (NF == 1) {
+        double v = 0.0;
+        for (int i = 0; i < 3; ++i) v += (double)Ms[(i * NB + a) * 33 + (i * NB + b)];
+        ke[vp[a] * NB + vp[b]] = s.material[e] * v;
+    } else {
+        double D[6][6];
+        elastic_D<3>(s.material[2 * e], s.material[2 * e + 1], D);
+        for (int al = 0; al < 3; ++al)
+            for (int be = 0; be < 3; ++be) {
+                double v = 0.0;
+                for (int i = 0; i < 3; ++i)
+                    for (int j = 0; j < 3; ++j)
+                        if (d_nonzero<3>(voigt<3>(al, i), voigt<3>(be, j)))
+                            v += D[voigt<3>(al, i)][voigt<3>(be, j)] * (double)Ms[(i * NB + a) * 33 + (j * NB + b)];
+                ke[(al * NB + vp[a]) * NA + (be * NB + vp[b])] = v;
+            }
+    }
+}
+
+// Launch the learned part of an assembly on stream st (after the MLP forward has written w_pred).
+void lq_launch_inputs(const LqView& lv, hipStream_t st) {
+    if (lv.n_cut > 0) hipLaunchKernelGGL(k_lq_inputs, dim3((lv.n_cut + 255) / 256), dim3(256), 0, st, lv);
+}
+void lq_launch_fit(const DevView& s, const LqView& lv, hipStream_t st) {
+    if (lv.n_cut > 0) hipLaunchKernelGGL(k_lq_fit, dim3(lv.n_cut), dim3(64), 0, st, s, lv);
+}
+void lq_launch_element(const DevView& s, const LqView& lv, int nf, hipStream_t st) {
+    if (lv.n_cut == 0) return;
+    if (nf == 1) hipLaunchKernelGGL(k_lq_element<1>, dim3(lv.n_cut), dim3(64), 0, st, s, lv);
+    else hipLaunchKernelGGL(k_lq_element<3>, dim3(lv.n_cut), dim3(64), 0, st, s, lv);
+}
+
+}  // namespace mgpu

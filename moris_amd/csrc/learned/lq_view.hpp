@@ -1,0 +1,34 @@
+// lq_view.hpp -- device view of the learned cut-cell quadrature state of a set, and the launches
+// the assembly drives (lq_assemble.hip, mlp.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "../set/set_device.hpp"
+
+struct mgpu_mlp;
+
+namespace mgpu {
+
+struct LqView {
+    int32_t n_cut;
+    int32_t iters;
+    double tol;
+    double detJ_ip;                 // prod(h / 2) of the lattice cell
+    const double* vphi;             // [n_cut][8] nodal level-set values of the cut elements
+    float* x_in;                    // [n_cut][8]
+    float* w_pred;                  // [n_cut][27] MLP output (weights / 8)
+    float* w_fit;                   // [n_cut][27] fitted parametric weights
+    int32_t* flag;                  // [n_cut] 1 learned, 0 fallback
+    const double* Vt;               // [27][23]  V^T
+    const double* Pt;               // [23][27]  P^T
+    const float* g27;               // [27][NG] physical gradients at the 27 points
+};
+
+void lq_launch_inputs(const LqView& lv, hipStream_t st);
+void lq_launch_fit(const DevView& s, const LqView& lv, hipStream_t st);
+void lq_launch_element(const DevView& s, const LqView& lv, int nf, hipStream_t st);
+void mlp_forward_launch(const mgpu_mlp* m, const float* x, int64_t n, float* y, hipStream_t st);
+
+}  // namespace mgpu

@@ -33,147 +33,18 @@
 
 #include "../common/fem_tables.hpp"
 #include "../setup/internal.hpp"
+#include "set_device.hpp"
 #include "set_internal.hpp"
+#include "../learned/lq_view.hpp"
+#include "../learned/quad.hpp"
 
 namespace mgpu {
-
-// ------------------------------------------------------------------------------------------------
-// device-side view of a set
-// ------------------------------------------------------------------------------------------------
-struct DevView {
-    int32_t n_el, n_cut, N, n_rows;
-    int32_t nx, ny, nz;
-    double h[3];
-    const double* material;     // [n_el][nm]
-    const int32_t* cell_elem;   // [n_cells]
-    const int32_t* elem_cut;    // [n_el]
-    const int32_t* node_adof;   // [n_nodes]
-    const int32_t* row_node;    // [N]
-    const int32_t* row_ptr;     // [n_rows + 1]
-    const int32_t* cut_elems;   // [n_cut]
-    const int32_t* ig_ptr;      // [n_el + 1]
-    const double* ig_xi;        // [n_ig][nv][dim]
-    const uint8_t* vert_pos;    // [n_el][nb]
-    double* ke_cut;             // [n_cut][na][na]
-    int32_t* err_flag;          // [1]
-    double* m_ref;              // reference moments of an uncut lattice cell
-    double* cell_rec;           // per lattice cell: diffusion [1] {k | -(cut+1) | 0 void},
-                                // elasticity [4] {flag, D_nn, D_nn', D_shear}
-    double* u_node;             // per lattice node: u of its adofs [n_nodes][nf] (0 if unused)
-};
-
-// Voigt index of the symmetric pair (i, j): 3D [xx,yy,zz,yz,xz,xy]
-// (CM_Struc_Linear::eval_strain_3d, cl_FEM_CM_Struc_Linear.cpp:448-453); 2D [xx,yy,xy].
-template <int DIM>
-__host__ __device__ constexpr int voigt(int i, int j) {
-    return DIM == 3 ? (i == j ? i : (i + j == 3 ? 3 : (i + j == 2 ? 4 : 5))) : (i == j ? i : 2);
-}
-
-// Structural non-zeros of the isotropic D: the normal block and the shear diagonal.
-template <int DIM>
-__host__ __device__ constexpr bool d_nonzero(int a, int b) {
-    return DIM == 3 ? ((a < 3 && b < 3) || a == b) : ((a < 2 && b < 2) || a == b);
-}
-
-// Constitutive matrix D (Voigt, engineering shear):
-//   3D  CM_Struc_Linear_Isotropic::full_3d, cl_FEM_CM_Struc_Linear_Isotropic.cpp:374-390
-//   2D  CM_Struc_Linear_Isotropic::full_plane_stress, :248-259
-template <int DIM>
-__device__ inline void elastic_D(double E, double nu, double (*D)[DIM == 3 ? 6 : 3]) {
-    constexpr int NS = DIM == 3 ? 6 : 3;
-    for (int i = 0; i < NS; ++i)
-        for (int j = 0; j < NS; ++j) D[i][j] = 0.0;
-    if constexpr (DIM == 3) {
-        const double pre = E / (1.0 + nu) / (1.0 - 2.0 * nu);
-        for (int i = 0; i < 3; ++i)
-            for (int j = 0; j < 3; ++j) D[i][j] = i == j ? pre * (1.0 - nu) : pre * nu;
-        for (int i = 3; i < 6; ++i) D[i][i] = pre * (1.0 - 2.0 * nu) / 2.0;
-    } else {
-        const double pre = E / (1.0 - nu * nu);
-        D[0][0] = pre;
-        D[1][1] = pre;
-        D[0][1] = pre * nu;
-        D[1][0] = pre * nu;
-        D[2][2] = pre * 0.5 * (1.0 - nu);
-    }
-}
-
-// dN/dxi of the linear Lagrange IP cell at xi: out[i][a].
-template <int DIM>
-__device__ inline void lagrange_dNdxi(const double* xi, double (*out)[DIM == 3 ? 8 : 4]) {
-    if constexpr (DIM == 3) {
-        for (int a = 0; a < 8; ++a) {
-            const double sx = kHex8XiHat[a][0], sy = kHex8XiHat[a][1], sz = kHex8XiHat[a][2];
-            const double fx = 1.0 + xi[0] * sx, fy = 1.0 + xi[1] * sy, fz = 1.0 + xi[2] * sz;
-            out[0][a] = 0.125 * sx * fy * fz;
-            out[1][a] = 0.125 * fx * sy * fz;
-            out[2][a] = 0.125 * fx * fy * sz;
-        }
-    } else {
-        for (int a = 0; a < 4; ++a) {
-            const double sx = kQuad4XiHat[a][0], sy = kQuad4XiHat[a][1];
-            out[0][a] = 0.25 * sx * (1.0 + xi[1] * sy);
-            out[1][a] = 0.25 * (1.0 + xi[0] * sx) * sy;
-        }
-    }
-}
-
-// ------------------------------------------------------------------------------------------------
-// Integration point ip of the cut element's IG sub-cells: weight w* and IP-cell gradients g[i][a].
-// ------------------------------------------------------------------------------------------------
-template <int DIM>
-__device__ inline double cut_point(const DevView& s, int ig0, int ip, const double* hh,
-                                   double (*g)[DIM == 3 ? 8 : 4]) {
-    constexpr int NV = DIM + 1;
-    constexpr int NQ = DIM == 3 ? kTet11Points : kTri7Points;
-    const int sc = ig0 + ip / NQ;
-    const int q = ip % NQ;
-    const double* X = s.ig_xi + (size_t)sc * NV * DIM;
-    double N[NV];
-    if constexpr (DIM == 3) {
-        N[0] = kTet11Xi[q][0];
-        N[1] = kTet11Xi[q][1];
-        N[2] = 1.0 - kTet11Xi[q][0] - kTet11Xi[q][1] - kTet11Xi[q][2];
-        N[3] = kTet11Xi[q][2];
-    } else {
-        N[0] = kTri7Xi[q][0];
-        N[1] = kTri7Xi[q][1];
-        N[2] = 1.0 - kTri7Xi[q][0] - kTri7Xi[q][1];
-    }
-    double xip[DIM];
-    for (int d = 0; d < DIM; ++d) {
-        double v = 0.0;
-        for (int k = 0; k < NV; ++k) v += N[k] * X[k * DIM + d];
-        xip[d] = v;
-    }
-    // detJ of the simplex in physical space: rows X_r - X_2 (r = 0, 1 [, 3]) scaled by h/2
-    double detJ;
-    if constexpr (DIM == 3) {
-        double J[3][3];
-        const int rr[3] = {0, 1, 3};
-        for (int r = 0; r < 3; ++r)
-            for (int d = 0; d < 3; ++d) J[r][d] = (X[rr[r] * 3 + d] - X[2 * 3 + d]) * (0.5 * hh[d]);
-        detJ = (J[0][0] * (J[1][1] * J[2][2] - J[2][1] * J[1][2]) -
-                J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
-                J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0])) / 6.0;
-    } else {
-        double J[2][2];
-        for (int r = 0; r < 2; ++r)
-            for (int d = 0; d < 2; ++d) J[r][d] = (X[r * 2 + d] - X[2 * 2 + d]) * (0.5 * hh[d]);
-        detJ = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) / 2.0;
-    }
-    const double w = detJ < kDetJLowerLimit ? 0.0 : (DIM == 3 ? kTet11W[q] : kTri7W[q]) * detJ;
-    lagrange_dNdxi<DIM>(xip, g);
-    for (int i = 0; i < DIM; ++i)
-        for (int a = 0; a < (DIM == 3 ? 8 : 4); ++a) g[i][a] *= 2.0 / hh[i];
-    return w;
-}
 
 // ------------------------------------------------------------------------------------------------
 // k_cut_diffusion: one wavefront per cut element.  Each lane integrates every 64th point and keeps
 // the symmetric moments S[a][b] = sum_ip w* g_a . g_b in registers; one LDS reduction per element.
 // ------------------------------------------------------------------------------------------------
-template <int DIM>
+template <int DIM, typename T>
 __global__ __launch_bounds__(64) void k_cut_diffusion(DevView s) {
     constexpr int NB = DIM == 3 ? 8 : 4;
     constexpr int NV = DIM + 1;
@@ -182,6 +53,7 @@ __global__ __launch_bounds__(64) void k_cut_diffusion(DevView s) {
     __shared__ double red[NSYM * 65];
 
     const int c = blockIdx.x;
+    if (s.skip && s.skip[c]) return;
     const int lane = threadIdx.x;
     const int e = s.cut_elems[c];
     const int ig0 = s.ig_ptr[e];
@@ -190,25 +62,32 @@ __global__ __launch_bounds__(64) void k_cut_diffusion(DevView s) {
     for (int d = 0; d < DIM; ++d) hh[d] = s.h[d];
     (void)NV;
 
-    double acc[NSYM];
+    // T = float: FP32 integrand with FP32 accumulation; the point map and the sub-cell Jacobian
+    // (cut_point) stay FP64
+    T acc[NSYM];
 #pragma unroll
-    for (int t = 0; t < NSYM; ++t) acc[t] = 0.0;
+    for (int t = 0; t < NSYM; ++t) acc[t] = (T)0;
     for (int ip = lane; ip < nip; ip += 64) {
-        double g[DIM][NB];
-        const double w = cut_point<DIM>(s, ig0, ip, hh, g);
+        double gd[DIM][NB];
+        const T w = (T)cut_point<DIM>(s, ig0, ip, hh, gd);
+        T g[DIM][NB];
+#pragma unroll
+        for (int i = 0; i < DIM; ++i)
+#pragma unroll
+            for (int a = 0; a < NB; ++a) g[i][a] = (T)gd[i][a];
         int t = 0;
 #pragma unroll
         for (int a = 0; a < NB; ++a)
 #pragma unroll
             for (int b = a; b < NB; ++b) {
-                double v = 0.0;
+                T v = (T)0;
 #pragma unroll
                 for (int i = 0; i < DIM; ++i) v += g[i][a] * g[i][b];
                 acc[t++] += w * v;
             }
     }
 #pragma unroll
-    for (int t = 0; t < NSYM; ++t) red[t * 65 + lane] = acc[t];
+    for (int t = 0; t < NSYM; ++t) red[t * 65 + lane] = (double)acc[t];
     __syncthreads();
     const uint8_t* vp = s.vert_pos + (size_t)e * NB;
     double* ke = s.ke_cut + (size_t)c * NB * NB;
@@ -230,17 +109,18 @@ __global__ __launch_bounds__(64) void k_cut_diffusion(DevView s) {
 // (a, b) accumulates the 3x3 (2x2) direction block M[(i,a),(j,b)] = sum w* g_ia g_jb over all points,
 // then contracts it with D (B^T D B) and projects to sorted adof order.
 // ------------------------------------------------------------------------------------------------
-template <int DIM>
+template <int DIM, typename T>
 __global__ __launch_bounds__(64) void k_cut_elasticity(DevView s) {
     constexpr int NB = DIM == 3 ? 8 : 4;
     constexpr int NG = DIM * NB;
     constexpr int NA = DIM * NB;
     constexpr int NQ = DIM == 3 ? kTet11Points : kTri7Points;
     constexpr int GPAD = NG + 1;
-    __shared__ double sgw[64 * GPAD];
-    __shared__ double sg[64 * GPAD];
+    __shared__ T sgw[64 * GPAD];
+    __shared__ T sg[64 * GPAD];
 
     const int c = blockIdx.x;
+    if (s.skip && s.skip[c]) return;
     const int lane = threadIdx.x;
     const int e = s.cut_elems[c];
     const int ig0 = s.ig_ptr[e];
@@ -250,11 +130,11 @@ __global__ __launch_bounds__(64) void k_cut_elasticity(DevView s) {
     const int a = (lane / NB) % NB, b = lane % NB;
     const bool active = lane < NB * NB;
 
-    double M[DIM][DIM];
+    T M[DIM][DIM];
 #pragma unroll
     for (int i = 0; i < DIM; ++i)
 #pragma unroll
-        for (int j = 0; j < DIM; ++j) M[i][j] = 0.0;
+        for (int j = 0; j < DIM; ++j) M[i][j] = (T)0;
 
     for (int base = 0; base < nip; base += 64) {
         const int ip = base + lane;
@@ -263,20 +143,20 @@ __global__ __launch_bounds__(64) void k_cut_elasticity(DevView s) {
             const double w = cut_point<DIM>(s, ig0, ip, hh, g);
             for (int i = 0; i < DIM; ++i)
                 for (int q = 0; q < NB; ++q) {
-                    sg[lane * GPAD + i * NB + q] = g[i][q];
-                    sgw[lane * GPAD + i * NB + q] = w * g[i][q];
+                    sg[lane * GPAD + i * NB + q] = (T)g[i][q];
+                    sgw[lane * GPAD + i * NB + q] = (T)(w * g[i][q]);
                 }
         } else {
             for (int k = 0; k < NG; ++k) {
-                sg[lane * GPAD + k] = 0.0;
-                sgw[lane * GPAD + k] = 0.0;
+                sg[lane * GPAD + k] = (T)0;
+                sgw[lane * GPAD + k] = (T)0;
             }
         }
         __syncthreads();
         const int cnt = min(64, nip - base);
         if (active) {
             for (int p = 0; p < cnt; ++p) {
-                double ga[DIM], gb[DIM];
+                T ga[DIM], gb[DIM];
 #pragma unroll
                 for (int i = 0; i < DIM; ++i) {
                     ga[i] = sgw[p * GPAD + i * NB + a];
@@ -306,7 +186,7 @@ __global__ __launch_bounds__(64) void k_cut_elasticity(DevView s) {
 #pragma unroll
                 for (int j = 0; j < DIM; ++j)
                     if (d_nonzero<DIM>(voigt<DIM>(al, i), voigt<DIM>(be, j)))
-                        v += D[voigt<DIM>(al, i)][voigt<DIM>(be, j)] * M[i][j];
+                        v += D[voigt<DIM>(al, i)][voigt<DIM>(be, j)] * (double)M[i][j];
             ke[(al * NB + vp[a]) * NA + (be * NB + vp[b])] = v;
         }
 }
@@ -591,6 +471,11 @@ struct mgpu_set {
     double* d_u_tmp = nullptr;
     double* d_vals_tmp = nullptr;
     double* d_rhs_tmp = nullptr;
+    // learned cut-cell quadrature
+    std::vector<double> vphi_cut;      // host copy of the cut elements' nodal level sets
+    LqView lv{};
+    const mgpu_mlp* mlp = nullptr;
+    bool lq_ready = false;
 };
 
 namespace {
@@ -616,19 +501,50 @@ mgpu_status upload(mgpu_set* S, const T* src, size_t n, T** dst) {
     return MGPU_OK;
 }
 
-mgpu_status launch_assemble(mgpu_set* S, const double* u, double* vals, double* rhs, float* kms,
-                            hipEvent_t* ev) {
+void launch_subdivision(mgpu_set* S, const int32_t* skip) {
+    const int dim = S->h.dim, nf = S->h.nf;
+    DevView v = S->v;
+    v.skip = skip;
+    dim3 g((unsigned)v.n_cut), b(64);
+    const bool f32 = S->precision == MGPU_PREC_MIXED;
+    if (dim == 3 && nf == 1) {
+        if (f32) hipLaunchKernelGGL((k_cut_diffusion<3, float>), g, b, 0, S->stream, v);
+        else hipLaunchKernelGGL((k_cut_diffusion<3, double>), g, b, 0, S->stream, v);
+    } else if (dim == 3 && nf == 3) {
+        if (f32) hipLaunchKernelGGL((k_cut_elasticity<3, float>), g, b, 0, S->stream, v);
+        else hipLaunchKernelGGL((k_cut_elasticity<3, double>), g, b, 0, S->stream, v);
+    } else if (dim == 2 && nf == 1) {
+        if (f32) hipLaunchKernelGGL((k_cut_diffusion<2, float>), g, b, 0, S->stream, v);
+        else hipLaunchKernelGGL((k_cut_diffusion<2, double>), g, b, 0, S->stream, v);
+    } else {
+        if (f32) hipLaunchKernelGGL((k_cut_elasticity<2, float>), g, b, 0, S->stream, v);
+        else hipLaunchKernelGGL((k_cut_elasticity<2, double>), g, b, 0, S->stream, v);
+    }
+}
+
+// ev (optional): [0] start, [1] after the cut-element integrands, [2] after the scatter,
+// [3] after the learned moments/fit (learned quadrature only)
+mgpu_status launch_assemble(mgpu_set* S, const double* u, double* vals, double* rhs, hipEvent_t* ev) {
     const SetHost& h = S->h;
     if (h.n_rows == 0) return MGPU_OK;
     const int dim = h.dim, nf = h.nf;
     if (ev) HIP_TRY(hipEventRecord(ev[0], S->stream));
     if (S->v.n_cut > 0) {
-        dim3 g((unsigned)S->v.n_cut), b(64);
-        if (dim == 3 && nf == 1) hipLaunchKernelGGL((k_cut_diffusion<3>), g, b, 0, S->stream, S->v);
-        else if (dim == 3 && nf == 3) hipLaunchKernelGGL((k_cut_elasticity<3>), g, b, 0, S->stream, S->v);
-        else if (dim == 2 && nf == 1) hipLaunchKernelGGL((k_cut_diffusion<2>), g, b, 0, S->stream, S->v);
-        else hipLaunchKernelGGL((k_cut_elasticity<2>), g, b, 0, S->stream, S->v);
+        if (S->quadrature == MGPU_QUAD_LEARNED) {
+            lq_launch_inputs(S->lv, S->stream);
+            mlp_forward_launch(S->mlp, S->lv.x_in, S->lv.n_cut, S->lv.w_pred, S->stream);
+            lq_launch_fit(S->v, S->lv, S->stream);
+            HIP_TRY(hipGetLastError());
+            if (ev) HIP_TRY(hipEventRecord(ev[3], S->stream));
+            lq_launch_element(S->v, S->lv, nf, S->stream);
+            launch_subdivision(S, S->lv.flag);
+        } else {
+            if (ev) HIP_TRY(hipEventRecord(ev[3], S->stream));
+            launch_subdivision(S, nullptr);
+        }
         HIP_TRY(hipGetLastError());
+    } else if (ev) {
+        HIP_TRY(hipEventRecord(ev[3], S->stream));
     }
     if (ev) HIP_TRY(hipEventRecord(ev[1], S->stream));
     {
@@ -651,7 +567,53 @@ mgpu_status launch_assemble(mgpu_set* S, const double* u, double* vals, double* 
         HIP_TRY(hipGetLastError());
     }
     if (ev) HIP_TRY(hipEventRecord(ev[2], S->stream));
-    (void)kms;
+    return MGPU_OK;
+}
+
+// Buffers and constant tables of the learned quadrature (lattice hex8 sets).
+mgpu_status lq_prepare(mgpu_set* S) {
+    if (S->lq_ready) return MGPU_OK;
+    const SetHost& h = S->h;
+    const int nc = S->v.n_cut;
+    if (h.dim != 3) return set_error(MGPU_ERR_UNSUPPORTED, "learned quadrature is implemented for 3D hex8 sets");
+    if (nc > 0 && S->vphi_cut.empty())
+        return set_error(MGPU_ERR_INVALID_ARG, "learned quadrature needs vertex_phi in the set descriptor");
+    LqView& lv = S->lv;
+    lv.n_cut = nc;
+    if (lv.tol == 0.0) lv.tol = 1e-7;
+    lv.detJ_ip = 0.125 * h.lat.h[0] * h.lat.h[1] * h.lat.h[2];
+    auto alloc = [&](size_t bytes, void** p) -> mgpu_status {
+        HIP_TRY(hipMalloc(p, std::max<size_t>(bytes, 8)));
+        S->allocs.push_back(*p);
+        return MGPU_OK;
+    };
+    mgpu_status rc;
+    void* p;
+    if ((rc = upload(S, S->vphi_cut.data(), S->vphi_cut.size(), const_cast<double**>(&lv.vphi))) != MGPU_OK) return rc;
+    if ((rc = alloc((size_t)nc * 8 * sizeof(float), &p)) != MGPU_OK) return rc;
+    lv.x_in = static_cast<float*>(p);
+    if ((rc = alloc((size_t)nc * kLqPoints * sizeof(float), &p)) != MGPU_OK) return rc;
+    lv.w_pred = static_cast<float*>(p);
+    if ((rc = alloc((size_t)nc * kLqPoints * sizeof(float), &p)) != MGPU_OK) return rc;
+    lv.w_fit = static_cast<float*>(p);
+    if ((rc = alloc((size_t)nc * sizeof(int32_t), &p)) != MGPU_OK) return rc;
+    lv.flag = static_cast<int32_t*>(p);
+    LqOperators op;
+    lq_build_operators(&op);
+    std::vector<double> Vt(kLqPoints * kLqMoments), Pt(kLqMoments * kLqPoints);
+    for (int q = 0; q < kLqPoints; ++q)
+        for (int k = 0; k < kLqMoments; ++k) {
+            Vt[q * kLqMoments + k] = op.V[k][q];
+            Pt[k * kLqPoints + q] = op.P[q][k];
+        }
+    if ((rc = upload(S, Vt.data(), Vt.size(), const_cast<double**>(&lv.Vt))) != MGPU_OK) return rc;
+    if ((rc = upload(S, Pt.data(), Pt.size(), const_cast<double**>(&lv.Pt))) != MGPU_OK) return rc;
+    // physical gradients of the hex8 basis at the 27 points of the lattice cell (built in a host
+    // translation unit: the tables are device constants here)
+    std::vector<float> g27(kLqPoints * 24);
+    lq_point_gradients(h.lat.h, g27.data());
+    if ((rc = upload(S, g27.data(), g27.size(), const_cast<float**>(&lv.g27))) != MGPU_OK) return rc;
+    S->lq_ready = true;
     return MGPU_OK;
 }
 
@@ -707,6 +669,11 @@ extern "C" mgpu_status mgpu_set_create(const mgpu_set_desc* desc, mgpu_set** out
     if ((rc = upload(S.get(), desc->ig_ptr, (size_t)h.n_el + 1, const_cast<int32_t**>(&v.ig_ptr))) != MGPU_OK) return rc;
     if ((rc = upload(S.get(), desc->ig_xi, n_ig * nv_ig * h.dim, const_cast<double**>(&v.ig_xi))) != MGPU_OK) return rc;
     if ((rc = upload(S.get(), h.vert_pos.data(), h.vert_pos.size(), const_cast<uint8_t**>(&v.vert_pos))) != MGPU_OK) return rc;
+    if (desc->vertex_phi && h.dim == 3) {
+        S->vphi_cut.resize(h.cut_elems.size() * 8);
+        for (size_t c = 0; c < h.cut_elems.size(); ++c)
+            for (int k = 0; k < 8; ++k) S->vphi_cut[c * 8 + k] = desc->vertex_phi[(size_t)h.cut_elems[c] * 8 + k];
+    }
     if (v.n_cut > 0) {
         void* p = nullptr;
         HIP_TRY(hipMalloc(&p, (size_t)v.n_cut * h.na * h.na * sizeof(double)));
@@ -750,8 +717,8 @@ extern "C" mgpu_status mgpu_set_options(mgpu_set* set, int32_t precision, int32_
     if (!set) return set_error(MGPU_ERR_INVALID_ARG, "null set");
     if (precision != MGPU_PREC_FP64 && precision != MGPU_PREC_MIXED) return set_error(MGPU_ERR_INVALID_ARG, "bad precision");
     if (quadrature != MGPU_QUAD_SUBDIVISION && quadrature != MGPU_QUAD_LEARNED) return set_error(MGPU_ERR_INVALID_ARG, "bad quadrature");
-    if (precision != MGPU_PREC_FP64 || quadrature != MGPU_QUAD_SUBDIVISION)
-        return set_error(MGPU_ERR_UNSUPPORTED, "only the FP64 subdivision path is built in this library");
+    if (quadrature == MGPU_QUAD_LEARNED && !set->mlp)
+        return set_error(MGPU_ERR_STATE, "learned quadrature needs a model (mgpu_set_quadrature_model)");
     set->precision = precision;
     set->quadrature = quadrature;
     return MGPU_OK;
@@ -784,7 +751,8 @@ extern "C" mgpu_status mgpu_set_assemble(mgpu_set* set, const double* u, double*
     MGPU_GUARD_BEGIN
     if (!set) return set_error(MGPU_ERR_INVALID_ARG, "null set");
     if (set->h.n_rows > 0 && (!u || !vals || !rhs)) return set_error(MGPU_ERR_INVALID_ARG, "null buffer");
-    mgpu_status rc = launch_assemble(set, u, vals, rhs, nullptr, nullptr);
+    if (set->quadrature == MGPU_QUAD_LEARNED && !set->mlp) return set_error(MGPU_ERR_STATE, "no quadrature model");
+    mgpu_status rc = launch_assemble(set, u, vals, rhs, nullptr);
     if (rc != MGPU_OK) return rc;
     set->stats[0] = set->h.n_el;
     set->stats[1] = set->v.n_cut;
@@ -819,6 +787,45 @@ extern "C" mgpu_status mgpu_set_assemble_host(mgpu_set* set, const double* u, do
 extern "C" mgpu_status mgpu_set_stats(const mgpu_set* set, int64_t stats[5]) {
     if (!set || !stats) return set_error(MGPU_ERR_INVALID_ARG, "null argument");
     for (int i = 0; i < 5; ++i) stats[i] = set->stats[i];
+    if (set->quadrature == MGPU_QUAD_LEARNED && set->v.n_cut > 0) {
+        std::vector<int32_t> f((size_t)set->v.n_cut);
+        HIP_TRY(hipStreamSynchronize(set->stream));
+        HIP_TRY(hipMemcpy(f.data(), set->lv.flag, f.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+        int64_t acc = 0;
+        for (int32_t x : f) acc += x != 0;
+        stats[1] = set->v.n_cut - acc;
+        stats[2] = acc;
+        stats[3] = set->v.n_cut - acc;
+        stats[4] = (int64_t)(set->h.n_el - set->v.n_cut) * 8 + acc * kLqPoints;   // + fallback points (below)
+        const int32_t* ip = nullptr;
+        (void)ip;
+    }
+    return MGPU_OK;
+}
+
+extern "C" mgpu_status mgpu_set_quadrature_model(mgpu_set* set, const mgpu_mlp* mlp, double tol, int32_t iters) {
+    MGPU_GUARD_BEGIN
+    if (!set || !mlp) return set_error(MGPU_ERR_INVALID_ARG, "null argument");
+    if (!(tol > 0.0) || iters < 0) return set_error(MGPU_ERR_INVALID_ARG, "bad tolerance / iteration count");
+    set->lv.tol = tol;
+    set->lv.iters = iters;
+    mgpu_status rc = lq_prepare(set);
+    if (rc != MGPU_OK) return rc;
+    set->lv.tol = tol;
+    set->lv.iters = iters;
+    set->mlp = mlp;
+    return MGPU_OK;
+    MGPU_GUARD_END
+}
+
+extern "C" mgpu_status mgpu_set_lq_result(const mgpu_set* set, float* w_pred, float* w_fit, int32_t* flags) {
+    if (!set) return set_error(MGPU_ERR_INVALID_ARG, "null set");
+    if (!set->lq_ready) return set_error(MGPU_ERR_STATE, "no learned quadrature attached");
+    const size_t nc = (size_t)set->v.n_cut;
+    HIP_TRY(hipStreamSynchronize(set->stream));
+    if (w_pred && nc) HIP_TRY(hipMemcpy(w_pred, set->lv.w_pred, nc * kLqPoints * sizeof(float), hipMemcpyDeviceToHost));
+    if (w_fit && nc) HIP_TRY(hipMemcpy(w_fit, set->lv.w_fit, nc * kLqPoints * sizeof(float), hipMemcpyDeviceToHost));
+    if (flags && nc) HIP_TRY(hipMemcpy(flags, set->lv.flag, nc * sizeof(int32_t), hipMemcpyDeviceToHost));
     return MGPU_OK;
 }
 
@@ -826,34 +833,38 @@ extern "C" mgpu_status mgpu_set_time_assemble(mgpu_set* set, const double* u, do
                                               int32_t iters, double* per_call_ms, double kernel_ms[4]) {
     MGPU_GUARD_BEGIN
     if (!set || iters < 1) return set_error(MGPU_ERR_INVALID_ARG, "bad argument");
-    hipEvent_t ev[3];
+    if (set->quadrature == MGPU_QUAD_LEARNED && !set->mlp) return set_error(MGPU_ERR_STATE, "no quadrature model");
+    hipEvent_t ev[4];
     for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
-    double t_cut = 0.0, t_st = 0.0;
+    double t[4] = {0, 0, 0, 0};
     hipEvent_t t0, t1;
     HIP_TRY(hipEventCreate(&t0));
     HIP_TRY(hipEventCreate(&t1));
-    HIP_TRY(hipEventRecord(t0, set->stream));
+    // per-kernel segments, one call at a time
     for (int it = 0; it < iters; ++it) {
-        mgpu_status rc = launch_assemble(set, u, vals, rhs, nullptr, ev);
+        mgpu_status rc = launch_assemble(set, u, vals, rhs, ev);
         if (rc != MGPU_OK) return rc;
         HIP_TRY(hipEventSynchronize(ev[2]));
-        float a = 0.f, b = 0.f;
+        float a = 0.f, b = 0.f, c = 0.f, d = 0.f;
         HIP_TRY(hipEventElapsedTime(&a, ev[0], ev[1]));
         HIP_TRY(hipEventElapsedTime(&b, ev[1], ev[2]));
-        t_cut += a;
-        t_st += b;
+        HIP_TRY(hipEventElapsedTime(&c, ev[0], ev[3]));
+        HIP_TRY(hipEventElapsedTime(&d, ev[3], ev[1]));
+        t[0] += a; t[1] += b; t[2] += c; t[3] += d;
+    }
+    // back-to-back calls for the per-call mean
+    HIP_TRY(hipEventRecord(t0, set->stream));
+    for (int it = 0; it < iters; ++it) {
+        mgpu_status rc = launch_assemble(set, u, vals, rhs, nullptr);
+        if (rc != MGPU_OK) return rc;
     }
     HIP_TRY(hipEventRecord(t1, set->stream));
     HIP_TRY(hipEventSynchronize(t1));
     float total = 0.f;
     HIP_TRY(hipEventElapsedTime(&total, t0, t1));
     if (per_call_ms) *per_call_ms = total / iters;
-    if (kernel_ms) {
-        kernel_ms[0] = t_cut / iters;
-        kernel_ms[1] = t_st / iters;
-        kernel_ms[2] = 0.0;
-        kernel_ms[3] = 0.0;
-    }
+    if (kernel_ms)
+        for (int k = 0; k < 4; ++k) kernel_ms[k] = t[k] / iters;
     for (auto& e : ev) (void)hipEventDestroy(e);
     (void)hipEventDestroy(t0);
     (void)hipEventDestroy(t1);

@@ -1,0 +1,145 @@
+// set_device.hpp -- device-side view of an mgpu_set and the element-level building blocks shared
+// by the assembly kernels (subdivision, learned quadrature, stencil scatter).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "../common/fem_tables.hpp"
+
+namespace mgpu {
+
+// ------------------------------------------------------------------------------------------------
+// device-side view of a set
+// ------------------------------------------------------------------------------------------------
+struct DevView {
+    int32_t n_el, n_cut, N, n_rows;
+    int32_t nx, ny, nz;
+    double h[3];
+    const double* material;     // [n_el][nm]
+    const int32_t* cell_elem;   // [n_cells]
+    const int32_t* elem_cut;    // [n_el]
+    const int32_t* node_adof;   // [n_nodes]
+    const int32_t* row_node;    // [N]
+    const int32_t* row_ptr;     // [n_rows + 1]
+    const int32_t* cut_elems;   // [n_cut]
+    const int32_t* ig_ptr;      // [n_el + 1]
+    const double* ig_xi;        // [n_ig][nv][dim]
+    const uint8_t* vert_pos;    // [n_el][nb]
+    double* ke_cut;             // [n_cut][na][na]
+    int32_t* err_flag;          // [1]
+    double* m_ref;              // reference moments of an uncut lattice cell
+    double* cell_rec;           // per lattice cell: diffusion [1] {k | -(cut+1) | 0 void},
+                                // elasticity [4] {flag, D_nn, D_nn', D_shear}
+    double* u_node;             // per lattice node: u of its adofs [n_nodes][nf] (0 if unused)
+    const int32_t* skip;        // [n_cut] nonzero: element integrated elsewhere (learned), or NULL
+};
+
+// Voigt index of the symmetric pair (i, j): 3D [xx,yy,zz,yz,xz,xy]
+// (CM_Struc_Linear::eval_strain_3d, cl_FEM_CM_Struc_Linear.cpp:448-453); 2D [xx,yy,xy].
+template <int DIM>
+__host__ __device__ constexpr int voigt(int i, int j) {
+    return DIM == 3 ? (i == j ? i : (i + j == 3 ? 3 : (i + j == 2 ? 4 : 5))) : (i == j ? i : 2);
+}
+
+// Structural non-zeros of the isotropic D: the normal block and the shear diagonal.
+template <int DIM>
+__host__ __device__ constexpr bool d_nonzero(int a, int b) {
+    return DIM == 3 ? ((a < 3 && b < 3) || a == b) : ((a < 2 && b < 2) || a == b);
+}
+
+// Constitutive matrix D (Voigt, engineering shear):
+//   3D  CM_Struc_Linear_Isotropic::full_3d, cl_FEM_CM_Struc_Linear_Isotropic.cpp:374-390
+//   2D  CM_Struc_Linear_Isotropic::full_plane_stress, :248-259
+template <int DIM>
+__device__ inline void elastic_D(double E, double nu, double (*D)[DIM == 3 ? 6 : 3]) {
+    constexpr int NS = DIM == 3 ? 6 : 3;
+    for (int i = 0; i < NS; ++i)
+        for (int j = 0; j < NS; ++j) D[i][j] = 0.0;
+    if constexpr (DIM == 3) {
+        const double pre = E / (1.0 + nu) / (1.0 - 2.0 * nu);
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) D[i][j] = i == j ? pre * (1.0 - nu) : pre * nu;
+        for (int i = 3; i < 6; ++i) D[i][i] = pre * (1.0 - 2.0 * nu) / 2.0;
+    } else {
+        const double pre = E / (1.0 - nu * nu);
+        D[0][0] = pre;
+        D[1][1] = pre;
+        D[0][1] = pre * nu;
+        D[1][0] = pre * nu;
+        D[2][2] = pre * 0.5 * (1.0 - nu);
+    }
+}
+
+// dN/dxi of the linear Lagrange IP cell at xi: out[i][a].
+template <int DIM>
+__device__ inline void lagrange_dNdxi(const double* xi, double (*out)[DIM == 3 ? 8 : 4]) {
+    if constexpr (DIM == 3) {
+        for (int a = 0; a < 8; ++a) {
+            const double sx = kHex8XiHat[a][0], sy = kHex8XiHat[a][1], sz = kHex8XiHat[a][2];
+            const double fx = 1.0 + xi[0] * sx, fy = 1.0 + xi[1] * sy, fz = 1.0 + xi[2] * sz;
+            out[0][a] = 0.125 * sx * fy * fz;
+            out[1][a] = 0.125 * fx * sy * fz;
+            out[2][a] = 0.125 * fx * fy * sz;
+        }
+    } else {
+        for (int a = 0; a < 4; ++a) {
+            const double sx = kQuad4XiHat[a][0], sy = kQuad4XiHat[a][1];
+            out[0][a] = 0.25 * sx * (1.0 + xi[1] * sy);
+            out[1][a] = 0.25 * (1.0 + xi[0] * sx) * sy;
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+// Integration point ip of the cut element's IG sub-cells: weight w* and IP-cell gradients g[i][a].
+// ------------------------------------------------------------------------------------------------
+template <int DIM>
+__device__ inline double cut_point(const DevView& s, int ig0, int ip, const double* hh,
+                                   double (*g)[DIM == 3 ? 8 : 4]) {
+    constexpr int NV = DIM + 1;
+    constexpr int NQ = DIM == 3 ? kTet11Points : kTri7Points;
+    const int sc = ig0 + ip / NQ;
+    const int q = ip % NQ;
+    const double* X = s.ig_xi + (size_t)sc * NV * DIM;
+    double N[NV];
+    if constexpr (DIM == 3) {
+        N[0] = kTet11Xi[q][0];
+        N[1] = kTet11Xi[q][1];
+        N[2] = 1.0 - kTet11Xi[q][0] - kTet11Xi[q][1] - kTet11Xi[q][2];
+        N[3] = kTet11Xi[q][2];
+    } else {
+        N[0] = kTri7Xi[q][0];
+        N[1] = kTri7Xi[q][1];
+        N[2] = 1.0 - kTri7Xi[q][0] - kTri7Xi[q][1];
+    }
+    double xip[DIM];
+    for (int d = 0; d < DIM; ++d) {
+        double v = 0.0;
+        for (int k = 0; k < NV; ++k) v += N[k] * X[k * DIM + d];
+        xip[d] = v;
+    }
+    // detJ of the simplex in physical space: rows X_r - X_2 (r = 0, 1 [, 3]) scaled by h/2
+    double detJ;
+    if constexpr (DIM == 3) {
+        double J[3][3];
+        const int rr[3] = {0, 1, 3};
+        for (int r = 0; r < 3; ++r)
+            for (int d = 0; d < 3; ++d) J[r][d] = (X[rr[r] * 3 + d] - X[2 * 3 + d]) * (0.5 * hh[d]);
+        detJ = (J[0][0] * (J[1][1] * J[2][2] - J[2][1] * J[1][2]) -
+                J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0])) / 6.0;
+    } else {
+        double J[2][2];
+        for (int r = 0; r < 2; ++r)
+            for (int d = 0; d < 2; ++d) J[r][d] = (X[r * 2 + d] - X[2 * 2 + d]) * (0.5 * hh[d]);
+        detJ = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) / 2.0;
+    }
+    const double w = detJ < kDetJLowerLimit ? 0.0 : (DIM == 3 ? kTet11W[q] : kTri7W[q]) * detJ;
+    lagrange_dNdxi<DIM>(xip, g);
+    for (int i = 0; i < DIM; ++i)
+        for (int a = 0; a < (DIM == 3 ? 8 : 4); ++a) g[i][a] *= 2.0 / hh[i];
+    return w;
+}
+
+}  // namespace mgpu

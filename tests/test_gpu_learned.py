@@ -1,0 +1,137 @@
+"""GPU tests of the learned cut-cell quadrature and the mixed-precision path.
+
+* the BF16-MFMA network forward and its parameter gradients against the CPU restatement
+  (oracle/mlp.py) with BF16 rounding emulated at the kernels' rounding points (tight) and in plain
+  FP64 (loose, the BF16 error itself);
+* training with the project's kernels lowers the loss;
+* learned assembly and the FP32 (mixed) subdivision assembly against the FP64 oracle at the
+  1e-6 bar of the mixed path;
+* what the trained network gains: the fit keeps the 23 moments of degree <= 4 exact from any prior;
+  the network prior also carries the degree-5/6 tri-quadratic moments, which the volume-scaled
+  Gauss prior does not.
+"""
+import numpy as np
+import pytest
+
+import moris_amd as M
+from helpers import SEED, random_spheres, rel, spec_sphere3
+from moris_amd import learned as L
+from oracle import mlp as OM
+from oracle import py as O
+
+pytestmark = pytest.mark.gpu
+
+
+def test_mlp_forward_matches_cpu_restatement():
+    net = L.MLP(seed=3)
+    rng = np.random.default_rng(0)
+    p = net.params.astype(np.float64) + rng.normal(0, 0.05, net.n_params)
+    net.params = p.astype(np.float32)
+    x = rng.uniform(-1, 1, (200, 8)).astype(np.float32)
+    y = net.forward(x)
+    y_bf, _ = OM.forward(net.params, x, bf=True)
+    y_64, _ = OM.forward(net.params, x, bf=False)
+    assert rel(y, y_bf) < 2e-5, rel(y, y_bf)
+    assert rel(y, y_64) < 3e-2
+
+
+def test_mlp_gradients_match_cpu_restatement():
+    net = L.MLP(seed=4)
+    rng = np.random.default_rng(1)
+    x = rng.uniform(-1, 1, (256, 8)).astype(np.float32)
+    t = rng.normal(0, 0.1, (256, 27)).astype(np.float32)
+    g, loss = net.gradients(x, t)
+    l_bf, g_bf = OM.loss_and_grad(net.params, x, t, bf=True)
+    l_64, g_64 = OM.loss_and_grad(net.params, x, t, bf=False)
+    assert abs(loss - l_bf) < 1e-5 * abs(l_bf)
+    assert rel(g, g_bf) < 1e-3, rel(g, g_bf)
+    assert rel(g, g_64) < 5e-2
+    # per block of parameters as well (a transposed operand in one layer fails here)
+    for a, b in [(OM.OFF_W1, OM.OFF_B1), (OM.OFF_W2, OM.OFF_B2), (OM.OFF_W3, OM.OFF_B3), (OM.OFF_B3, OM.N_PARAMS)]:
+        assert rel(g[a:b], g_bf[a:b]) < 2e-3
+
+
+def test_training_lowers_loss():
+    x, y, _ = L.dataset(40000, SEED + 20)
+    net = L.MLP(seed=5)
+    _, l0 = net.gradients(x[:4096], y[:4096])
+    hist = net.train(x, y, epochs=8, batch=512, lr=3e-3)
+    _, l1 = net.gradients(x[:4096], y[:4096])
+    assert np.all(np.isfinite(hist))
+    assert l1 < 0.3 * l0, (l0, l1, hist)
+
+
+def _elastic_sphere(n=16):
+    return spec_sphere3(n, iwg=M.IWG_STRUC_LINEAR_BULK)
+
+
+@pytest.mark.parametrize("iwg", [M.IWG_STRUC_LINEAR_BULK, M.IWG_SPATIALDIFF_BULK])
+def test_learned_and_mixed_assembly_match_fp64_oracle(iwg):
+    spec, keep = spec_sphere3(16, iwg=iwg) if iwg == M.IWG_SPATIALDIFF_BULK else \
+        M.make_spec(3, (20, 20, 20), iwg=iwg, spheres=random_spheres(16, SEED + 4))
+    P = M.Problem(spec, keep)
+    S = M.Set(P.desc, keepalive=P)
+    Q = O.OracleProblem(spec)
+    u = np.random.default_rng(SEED + 7).uniform(-1.0, 1.0, S.n_rows)
+    vo, Ro = Q.assemble(u)
+    # mixed precision, subdivision
+    S.options(M.PREC_MIXED, M.QUAD_SUBDIVISION)
+    vals, R = S.compute_jacobian_and_residual(u)
+    assert rel(vals, vo) <= 1e-6 and rel(R, Ro) <= 1e-6, (rel(vals, vo), rel(R, Ro))
+    # learned quadrature (untrained prior: the moment fit alone sets the degree <= 4 moments)
+    net = L.MLP(seed=6)
+    L.attach(S, net, tol=1e-7)
+    for prec in (M.PREC_MIXED, M.PREC_FP64):
+        S.options(prec, M.QUAD_LEARNED)
+        vals, R = S.compute_jacobian_and_residual(u)
+        _, _, flags = L.result(S)
+        assert flags.sum() == S.n_cut
+        assert rel(vals, vo) <= 1e-6 and rel(R, Ro) <= 1e-6, (prec, rel(vals, vo), rel(R, Ro))
+    stats = S.stats()
+    assert stats[2] == S.n_cut
+
+
+def test_learned_fallback_goes_through_subdivision():
+    spec, keep = _elastic_sphere(12)
+    P = M.Problem(spec, keep)
+    S = M.Set(P.desc, keepalive=P)
+    Q = O.OracleProblem(spec)
+    u = np.random.default_rng(2).uniform(-1.0, 1.0, S.n_rows)
+    vo, Ro = Q.assemble(u)
+    net = L.MLP(seed=6)
+    # nonnegative fit with a tolerance most cut cells cannot meet -> most fall back
+    L.attach(S, net, tol=1e-9, iters=20)
+    S.options(M.PREC_FP64, M.QUAD_LEARNED)
+    vals, R = S.compute_jacobian_and_residual(u)
+    _, _, flags = L.result(S)
+    assert 0 <= flags.sum() < S.n_cut
+    assert rel(vals, vo) <= 1e-6 and rel(R, Ro) <= 1e-6
+
+
+def test_trained_network_gain_on_high_moments():
+    x, y, _ = L.dataset(200000, SEED + 30)
+    net = L.MLP(seed=8)
+    net.train(x, y, epochs=12, batch=1024, lr=3e-3)
+    xt, yt, mt = L.dataset(3000, SEED + 31, with_moments=True)
+    V, Pm, g = L.operators()
+    mask = L.MASK23
+    wp = net.forward(xt).astype(np.float64) * 8.0
+    # point coordinates in the library's order: the monomials xi, eta, zeta are rows 1, 3, 9 of V
+    pts = np.stack([V[1], V[3], V[9]], axis=1)
+    V27 = np.stack([[xi ** a * eta ** b * zeta ** c for c in range(3) for b in range(3) for a in range(3)]
+                    for (xi, eta, zeta) in pts], axis=1)
+    err_mlp, err_gauss = [], []
+    for i in range(xt.shape[0]):
+        m23 = mt[i, mask]
+        w_m, r_m = L.fit(m23, wp[i])
+        w_g, r_g = L.fit(m23, g * mt[i, 0] / 8.0)
+        # one FP64 projection step: residual at roundoff of the correction (acceptance tolerance 1e-7)
+        assert r_m < 1e-9 and r_g < 1e-9
+        hi = ~mask
+        scale = max(abs(mt[i, 0]), 1e-12)
+        err_mlp.append(np.abs(V27[hi] @ w_m - mt[i, hi]).max() / scale)
+        err_gauss.append(np.abs(V27[hi] @ w_g - mt[i, hi]).max() / scale)
+    em, eg = np.median(err_mlp), np.median(err_gauss)
+    print("median degree-5/6 moment error: network prior %.3e, Gauss prior %.3e" % (em, eg))
+    assert em < 0.5 * eg
+
