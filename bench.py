@@ -1,12 +1,22 @@
 """Benchmark of the MI355X element K+R assembly (one JSON line on stdout).
 
-A step is one mgpu_set_assemble call: K and R of every element of the workload (uncut and
-XTK-cut), written into the CSR values / R arrays, with u, the set data and the CSR pattern already
-resident in HBM.  Workload: 3D steady heat diffusion, 128^3 trilinear hex8 on the unit cube, sphere
-hole r = 0.3 at the centre cut by the XTK restatement (HEX_2x2x2 uncut, TET_11 on sub-tets), per-element
-log-normal conductivity (mu 0, sigma 0.5, seeded), plain FP64 path.
+Workload: a batch of design variants of 3D linear elasticity (E = 1, nu = 0.3), each a 64^3
+trilinear hex8 lattice of the unit cube with a 4 x 4 x 4 array of spherical holes (radius
+0.1 * U[0.9, 1.1] per sphere and variant, seeded), cut by the XTK restatement; the material is the
+region outside the spheres.  A step is one replay of the HIP graph that assembles K and R of every
+element of every variant (uncut and cut) into each variant's CSR values and R, with u, the set
+data and the CSR patterns resident in HBM:
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+  * measured path (`value`): mixed precision + learned cut-cell quadrature (network prior, FP64
+    moment fit, 27-point FP32 MFMA integrands, subdivision fallback in FP32 with FP64 sub-cell
+    Jacobians, FP64 global values);
+  * beside it: the plain FP64 path (XTK subdivision, TET_11 on every sub-tet) on the same sets,
+    captured the same way; `speedup_vs_fp64_gpu` is the ratio of the two.
+
+The learned path's K and R are compared with the FP64 path's for every variant (the FP64 path
+matches the CPU restatement to 1e-12 in the test suite).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--variants 32] [--n 64] [--no-cpu]
 """
 import argparse
 import json
@@ -20,61 +30,55 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 import moris_amd as M  # noqa: E402
+from moris_amd import learned as L  # noqa: E402
+from moris_amd import train_lq  # noqa: E402
+from moris_amd import workloads as W  # noqa: E402
 
 METRIC = "elements/s for K+R assembly, mixed+learned vs FP64 plain, ‖ΔK‖/‖K‖≤1e-6"
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
-SEED = 20261202
+FP32_PEAK_TFLOPS = 157.3
+SEED = W.BASE_SEED
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def workload(n=128):
-    return M.make_spec(3, (n, n, n), iwg=M.IWG_SPATIALDIFF_BULK, spheres=[[0.5, 0.5, 0.5, 0.3]], k_mu=0.0,
-                       k_sigma=0.5, seed=SEED + 1)
-
-
-def algorithmic_figures(P, S, dim=3, nf=1):
-    """Algorithmic bytes of the stencil (scatter) kernel and flops of the cut-element kernel."""
-    nb = 8 if dim == 3 else 4
+def stencil_bytes(spec, S, n_elements, n_cut, nf=3, nb=8):
+    """Algorithmic HBM bytes of one k_stencil launch: K values and R written once, u read once,
+    per-element material, the stored cut-element K_e, lattice maps and row offsets."""
     na = nf * nb
-    nm = 2 if nf > 1 else 1
-    n_nodes = (P.spec.n[0] + 1) * (P.spec.n[1] + 1) * (P.spec.n[2] + 1 if dim == 3 else 1)
-    stencil_bytes = (8 * S.nnz + 8 * S.n_rows            # K values and R written once
-                     + 8 * S.n_rows                       # u read
-                     + 8 * nm * P.n_elements              # material
-                     + 8 * na * na * P.n_cut              # cut-element K_e read back
-                     + 4 * P.n_cells + 4 * n_nodes        # lattice cell->element, node->adof maps
-                     + 4 * (S.n_rows + 1))                # row offsets
-    n_ip_cut = P.n_ig * (11 if dim == 3 else 7)
-    if nf == 1:
-        f_ip = 2 * nb * nb * dim + 4 * nb * dim
-        npd = nb
-    else:
-        ns = 6 if dim == 3 else 3
-        npd = nf * nb
-        f_ip = 2 * ns * ns * npd + 2 * npd * npd * ns + 2 * (ns * npd + ns * ns + npd * ns)
-    f_t = 2 * npd * npd * na + 2 * npd * na * na + 2 * npd * na
-    cut_flops = n_ip_cut * f_ip + P.n_cut * f_t
-    return stencil_bytes, cut_flops, n_ip_cut, f_ip, f_t
+    n_nodes = (spec.n[0] + 1) * (spec.n[1] + 1) * (spec.n[2] + 1)
+    n_cells = spec.n[0] * spec.n[1] * spec.n[2]
+    return (8 * S.nnz + 8 * S.n_rows + 8 * S.n_rows + 16 * n_elements + 8 * na * na * n_cut
+            + 4 * n_cells + 4 * n_nodes + 4 * (S.n_rows + 1))
+
+
+def elasticity_flops(n_ip, n_elem, dim=3, nb=8):
+    """Native-shape flops of the reference algorithm: per point D.B, B^T.DB and the residual
+    products; per element the T projection (dense T of size n_p x n_a)."""
+    ns = 6 if dim == 3 else 3
+    npd = dim * nb
+    f_ip = 2 * ns * ns * npd + 2 * npd * npd * ns + 2 * (ns * npd + ns * ns + npd * ns)
+    f_t = 2 * npd * npd * npd + 2 * npd * npd * npd + 2 * npd * npd
+    return n_ip * f_ip + n_elem * f_t, f_ip, f_t
 
 
 def cpu_baseline(P, S, u, budget_s=20.0):
-    """Oracle (FP64 C++ restatement of the reference algorithm) on the host cores, whole workload."""
+    """Oracle (FP64 C++ restatement of the reference algorithm) on the host cores, on a strided
+    sample of one variant's elements sized to the time budget."""
     from oracle import py as O
     cores = min(os.cpu_count() or 1, 16)
     rp, ci = S.pattern()
-    elems = np.arange(P.n_elements, dtype=np.int64)
-    # bounded sample: time a strided subset first, grow to the whole set if the budget allows
-    sample = elems[:: max(1, P.n_elements // 200000)]
+    ne = P.n_elements
+    elems = np.arange(ne, dtype=np.int64)
+    probe = elems[:: max(1, ne // 4000)]
     t0 = time.perf_counter()
-    O.assemble_list(P.desc, rp, ci, u, sample, n_threads=cores)
-    dt = time.perf_counter() - t0
-    est_full = dt * P.n_elements / len(sample)
-    if est_full * 2 < budget_s:
-        sample = elems
+    O.assemble_list(P.desc, rp, ci, u, probe, n_threads=cores)
+    per_elem = (time.perf_counter() - t0) / len(probe)
+    stride = max(1, int(np.ceil(ne * per_elem / (budget_s / 2))))
+    sample = elems[::stride]
     runs = []
     t_end = time.perf_counter() + budget_s
     while not runs or (time.perf_counter() < t_end and len(runs) < 3):
@@ -84,18 +88,30 @@ def cpu_baseline(P, S, u, budget_s=20.0):
     best = min(runs)
     cut_frac = float(np.mean(np.diff(P.arrays()["ig_ptr"])[sample] > 0))
     return {"value": len(sample) / best, "unit": "elements/s", "cores": cores, "kind": "port",
-            "sample": "%d of %d elements (%s), cut fraction %.4f, best of %d runs, %d worker threads with "
-                      "private CSR arrays + row-wise reduction" % (
-                          len(sample), P.n_elements, "whole workload" if len(sample) == P.n_elements else "strided",
-                          cut_frac, len(runs), cores)}
+            "sample": "every %d-th element of variant 0 (%d of %d elements, cut fraction %.3f), FP64 subdivision "
+                      "(TET_11), best of %d runs, %d worker threads with private CSR arrays + row-wise reduction"
+                      % (stride, len(sample), ne, cut_frac, len(runs), cores)}
+
+
+def timed_replays(B, steps, warmup):
+    for _ in range(warmup):
+        B.launch()
+    B.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        B.launch()
+    B.synchronize()
+    return (time.perf_counter() - t0) / steps
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--n", type=int, default=128)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--variants", type=int, default=32)
+    ap.add_argument("--n", type=int, default=64)
+    ap.add_argument("--branches", type=int, default=0, help="graph branches (0: autotuned)")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -105,47 +121,101 @@ def main():
         print("bench.py: no GPU visible", file=sys.stderr)
         sys.exit(1)
 
+    # network: the committed checkpoint, else trained here with the library's kernels
+    net = train_lq.load()
+    net_note = "checkpoint moris_amd/data/lq_mlp.f32"
+    if net is None:
+        log("no network checkpoint; training on 200000 cells ...")
+        net, info = train_lq.train(samples=200_000, epochs=8, log=log)
+        net_note = "trained in this run on %d cells, %d epochs" % (info["samples"], info["epochs"])
+
     t0 = time.time()
-    spec, keep = workload(args.n)
-    P = M.Problem(spec, keep)
-    S = M.Set(P.desc, keepalive=P)
-    log("setup %.1fs: elements %d cut %d sub-cells %d rows %d nnz %d" % (time.time() - t0, P.n_elements, P.n_cut,
-                                                                          P.n_ig, S.n_rows, S.nnz))
-    u = np.random.default_rng(SEED + 9).uniform(-1.0, 1.0, S.n_rows)
-    ub = M.DeviceBuffer.from_array(u)
-    vb = M.DeviceBuffer(8 * S.nnz)
-    rb = M.DeviceBuffer(8 * S.n_rows)
+    problems, sets, bufs = [], [], []
+    for k in range(args.variants):
+        spec, keep = W.elasticity_spec(args.n, W.sphere_array(k))
+        P = M.Problem(spec, keep)
+        S = M.Set(P.desc, keepalive=P)
+        L.attach(S, net, tol=1e-7)
+        u = np.random.default_rng(SEED + 100 + k).uniform(-1.0, 1.0, S.n_rows)
+        problems.append((P, u))
+        sets.append(S)
+        bufs.append((M.DeviceBuffer.from_array(u), M.DeviceBuffer(8 * S.nnz), M.DeviceBuffer(8 * S.n_rows)))
+    n_el = sum(P.n_elements for P, _ in problems)
+    n_cut = sum(P.n_cut for P, _ in problems)
+    n_ig = sum(P.n_ig for P, _ in problems)
+    nnz = sum(S.nnz for S in sets)
+    log("setup %.1fs: %d variants, elements %d cut %d sub-cells %d nnz %d" % (time.time() - t0, args.variants, n_el,
+                                                                              n_cut, n_ig, nnz))
 
-    for _ in range(args.warmup):
-        S.assemble_device(ub, vb, rb)
-    M.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        S.assemble_device(ub, vb, rb)
-    M.synchronize()
-    wall = time.perf_counter() - t0
-    ms_per_step = 1e3 * wall / args.steps
-    value = P.n_elements / (wall / args.steps)
+    def run_mode(prec, quad, label):
+        for S, (ub, vb, rb) in zip(sets, bufs):
+            S.options(prec, quad)
+            S.autotune(ub, vb, rb, iters=3)
+        # per-stage durations with HIP events on the set's stream (variant 0, eager calls)
+        per_call, stages = sets[0].time_assemble(*bufs[0], max(5, args.steps))
+        B = M.Batch(sets, [b[0] for b in bufs], [b[1] for b in bufs], [b[2] for b in bufs], n_branches=args.branches)
+        sec = timed_replays(B, args.steps, args.warmup)
+        ev_ms, n_br = B.time(max(3, args.steps // 2))
+        B.close()
+        log("%s: %.3f ms per replay (%.3f by events, %d branches), stages of variant 0 %s" % (
+            label, 1e3 * sec, ev_ms, n_br, ["%.4f" % t for t in stages]))
+        return sec, ev_ms, n_br, per_call, stages
 
-    # per-kernel durations with HIP events on the set's stream
-    per_call, km = S.time_assemble(ub, vb, rb, max(10, args.steps // 2))
-    stencil_bytes, cut_flops, n_ip_cut, f_ip, f_t = algorithmic_figures(P, S)
-    t_cut, t_st = km[0], km[1]
-    st_gbs = stencil_bytes / (t_st * 1e-3) / 1e9 if t_st > 0 else 0.0
-    cut_tf = cut_flops / (t_cut * 1e-3) / 1e12 if t_cut > 0 else 0.0
+    # plain FP64 path
+    sec64, ev64, nb64, call64, st64 = run_mode(M.PREC_FP64, M.QUAD_SUBDIVISION, "fp64 subdivision")
+    ref = [(vb.to_array(np.float64, S.nnz), rb.to_array(np.float64, S.n_rows)) for S, (_, vb, rb) in zip(sets, bufs)]
+    tune64 = sets[0].tuning()
+    # mixed + learned
+    sec, evl, nbl, call_l, st_l = run_mode(M.PREC_MIXED, M.QUAD_LEARNED, "mixed + learned")
+    tune_l = sets[0].tuning()
+    err_k, err_r, accepted = [], [], 0
+    for S, (_, vb, rb), (v0, r0) in zip(sets, bufs, ref):
+        v = vb.to_array(np.float64, S.nnz)
+        r = rb.to_array(np.float64, S.n_rows)
+        err_k.append(float(np.linalg.norm(v - v0) / np.linalg.norm(v0)))
+        err_r.append(float(np.linalg.norm(r - r0) / np.linalg.norm(r0)))
+        accepted += int(L.result(S)[2].sum())
+    del ref
+
+    value = n_el / sec
+    value64 = n_el / sec64
+    # dominant kernel of the measured path (variant 0 stages: learned pre-pass, cut integrands,
+    # cell records, stencil) and its roofline
+    P0, _ = problems[0]
+    S0 = sets[0]
+    names = ["k_lq_inputs+k_mlp_forward+k_lq_fit", "k_lq_element+k_cut_elasticity", "k_prep", "k_stencil"]
+    dom = int(np.argmax(st_l))
+    st_bytes = stencil_bytes(P0.spec, S0, P0.n_elements, P0.n_cut)
+    fl0 = L.result(S0)[2]
+    n_learned0 = int(fl0.sum())
+    ig = np.diff(P0.arrays()["ig_ptr"])
+    fallback_ip = int(ig[ig > 0][fl0 == 0].sum()) * 11
+    cut_flops_l, f_ip, f_t = elasticity_flops(27 * n_learned0 + fallback_ip, P0.n_cut)
+    cut_flops_64, _, _ = elasticity_flops(P0.n_ig * 11, P0.n_cut)
     kernels = {
-        "k_stencil": {"ms": t_st, "bound": "hbm", "algorithmic_bytes": stencil_bytes, "GB/s": st_gbs,
-                      "frac": st_gbs / HBM_PEAK_GBS},
-        "k_cut_moments": {"ms": t_cut, "bound": "fp64", "algorithmic_flops": cut_flops, "TFLOP/s": cut_tf,
-                          "frac": cut_tf / FP64_PEAK_TFLOPS, "integration_points": n_ip_cut,
-                          "flops_per_ip": f_ip, "flops_T_per_element": f_t},
+        "k_stencil": {"ms": st_l[3], "bound": "hbm", "algorithmic_bytes": st_bytes,
+                      "GB/s": st_bytes / (st_l[3] * 1e-3) / 1e9},
+        "cut_integrands_learned": {"ms": st_l[1], "algorithmic_flops": cut_flops_l,
+                                   "TFLOP/s": cut_flops_l / (st_l[1] * 1e-3) / 1e12 if st_l[1] > 0 else None,
+                                   "points": 27 * n_learned0 + fallback_ip},
+        "learned_prepass": {"ms": st_l[0]},
+        "k_prep": {"ms": st_l[2]},
+        "cut_integrands_fp64": {"ms": st64[1], "algorithmic_flops": cut_flops_64,
+                                "TFLOP/s": cut_flops_64 / (st64[1] * 1e-3) / 1e12 if st64[1] > 0 else None,
+                                "points": P0.n_ig * 11, "flops_per_point": f_ip, "flops_T_per_element": f_t},
     }
-    if t_st >= t_cut:
-        roof = {"bound": "hbm", "kernel": "k_stencil", "achieved": st_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": st_gbs / HBM_PEAK_GBS, "traffic": None}
+    if dom == 3:
+        gbs = kernels["k_stencil"]["GB/s"]
+        roof = {"bound": "hbm", "kernel": names[3], "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": gbs / HBM_PEAK_GBS, "traffic": None}
+    elif dom == 1:
+        tf = kernels["cut_integrands_learned"]["TFLOP/s"]
+        roof = {"bound": "mfma", "kernel": names[1], "achieved": tf, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": tf / FP32_PEAK_TFLOPS, "traffic": None}
     else:
-        roof = {"bound": "mfma", "kernel": "k_cut_moments", "achieved": cut_tf, "peak": FP64_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "frac": cut_tf / FP64_PEAK_TFLOPS, "traffic": None}
+        roof = {"bound": "hbm" if dom == 2 else "mfma", "kernel": names[dom], "achieved": None,
+                "peak": HBM_PEAK_GBS if dom == 2 else FP64_PEAK_TFLOPS, "unit": "GB/s" if dom == 2 else "TFLOP/s",
+                "frac": None, "traffic": None}
 
     out = {
         "metric": METRIC,
@@ -154,23 +224,34 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": 1e3 * sec,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32+f64",
         "data": "synthetic",
-        "config": {"workload": "3D steady heat diffusion, %d^3 hex8 unit cube, sphere hole r=0.3 (XTK cut, TET_11), "
-                               "log-normal k (sigma 0.5), plain FP64 GPU path" % args.n,
-                   "elements": P.n_elements, "cut_elements": P.n_cut, "ig_subcells": P.n_ig, "rows": S.n_rows,
-                   "nnz": S.nnz, "precision": "fp64", "quadrature": "subdivision"},
+        "config": {"workload": "%d design variants of 3D linear elasticity (E=1, nu=0.3), %d^3 hex8 unit cube, "
+                               "4x4x4 sphere-array holes r=0.1*U[0.9,1.1] (XTK cut), mixed precision + learned "
+                               "cut-cell quadrature, one HIP graph per step" % (args.variants, args.n),
+                   "elements": n_el, "cut_elements": n_cut, "ig_subcells": n_ig, "nnz": nnz,
+                   "network": net_note, "fit_tolerance": 1e-7},
+        "fp64_plain": {"value": value64, "ms_per_step": 1e3 * sec64, "graph_branches": nb64,
+                       "tuning_variant0": list(tune64), "stage_ms_variant0": st64},
+        "speedup_vs_fp64_gpu": value / value64,
+        "learned_accepted": accepted / max(n_cut, 1),
+        "max_rel_err_K_vs_fp64": max(err_k),
+        "max_rel_err_R_vs_fp64": max(err_r),
+        "graph_branches": nbl,
+        "tuning_variant0": list(tune_l),
+        "stage_ms_variant0": st_l,
+        "event_ms_per_replay": evl,
         "roofline": roof,
         "kernels": kernels,
-        "event_ms_per_call": per_call,
     }
     if not args.no_cpu:
         log("cpu baseline ...")
-        out["cpu_baseline"] = cpu_baseline(P, S, u)
+        P, u = problems[0]
+        out["cpu_baseline"] = cpu_baseline(P, sets[0], u)
         out["gpu_vs_cpu"] = value / out["cpu_baseline"]["value"]
     print(json.dumps(out), flush=True)
 

@@ -130,8 +130,10 @@ typedef struct mgpu_set_desc {
     const int32_t* ig_ptr;     /* [n_elements + 1]                                               */
     const double*  ig_xi;
 
-    /* Optional lattice hint: elem_cell[e] = lattice cell (i + n0*(j + n1*k)).  With a lattice and
-       nodal T rows (one entry, weight 1) the owner-computes stencil path is used.               */
+    /* Lattice hint: elem_cell[e] = lattice cell (i + n0*(j + n1*k)).  The implemented path is the
+       owner-computes stencil on a lattice with nodal T rows (one entry, weight 1, p = 1);
+       mgpu_set_create rejects a set without a lattice, or with T rows of more than one entry
+       (p = 2 / THB), with MGPU_ERR_UNSUPPORTED.                                                */
     mgpu_lattice   lattice;
     const int32_t* elem_cell;  /* [n_elements] or NULL                                          */
 
@@ -191,11 +193,49 @@ mgpu_status mgpu_memcpy_htod(void* dst, const void* src, size_t bytes);
 mgpu_status mgpu_memcpy_dtoh(void* dst, const void* src, size_t bytes);
 mgpu_status mgpu_synchronize(void);
 /* Time `iters` back-to-back mgpu_set_assemble calls with HIP events on the set's stream.
-   per_call_ms receives the mean; kernel_ms[4] the mean per-call duration of the cut-element,
-   stencil, generic and scatter kernels measured with events around each (0 where not used).  */
+   per_call_ms receives the mean; kernel_ms[4] the mean per-call durations, from events around
+   each stage of separate calls: [0] learned inputs + network + moment fit, [1] cut-element
+   integrands (learned element kernel and subdivision kernel), [2] cell records (k_prep),
+   [3] stencil scatter (k_stencil); 0 where a stage does not run.                               */
 mgpu_status mgpu_set_time_assemble(mgpu_set* set, const double* u_adof, double* csr_vals,
                                    double* rhs, int32_t iters, double* per_call_ms,
                                    double kernel_ms[4]);
+
+/* ---- launch shapes and the autotuner (no counterpart in the reference) -------------------------
+   A set's launch shape is the stencil row-block variant (rows per workgroup, see
+   mgpu_set_tuning_variants) and the order of the cut-element workgroups (0 natural, 1 longest
+   first by sub-cell count).  Results are bit-identical for every shape.                        */
+mgpu_status mgpu_set_tuning(mgpu_set* set, int32_t stencil_variant, int32_t cut_order);
+mgpu_status mgpu_set_get_tuning(const mgpu_set* set, int32_t* stencil_variant, int32_t* cut_order);
+/* Number of stencil variants of the set and the rows per workgroup of each (rows[] optional, >= 3). */
+mgpu_status mgpu_set_tuning_variants(const mgpu_set* set, int32_t* n_variants, int32_t* rows);
+/* Time every launch shape with `iters` assemble calls on the given device buffers and keep the
+   fastest.  The choice is cached per process by set shape (dimension, fields, rows, cut count
+   rounded to a power of two, precision, quadrature); flags bit 0 ignores the cache.
+   ms[2 * n_variants] (optional) receives the per-call mean of shape (variant, order) at
+   [2 * variant + order], or -1 where the cache answered.                                       */
+mgpu_status mgpu_set_autotune(mgpu_set* set, const double* u_adof, double* csr_vals, double* rhs,
+                              int32_t iters, int32_t flags, double* ms);
+
+/* ---- batches of sets captured into one HIP graph (no counterpart in the reference) ------------
+   Replaces the loop over sets of Solver_Interface::fill_matrix_and_RHS
+   (projects/SOL/DLA/src/cl_DLA_Solver_Interface.cpp:317-365) for sets that are assembled
+   together, e.g. design variants: one assemble call of each set, on its current options and
+   launch shape, captured once with fixed device buffers u[i], vals[i], rhs[i] (which must stay
+   valid) and replayed as one graph launch.  The sets are spread over n_branches parallel graph
+   branches (0: autotuned among 1, 2, 4, 8).  The batch does not own the sets; a set must not be
+   reconfigured (options, tuning, model) while a batch holds it.                                */
+typedef struct mgpu_batch mgpu_batch;
+mgpu_status mgpu_batch_create(mgpu_set* const* sets, int32_t n_sets, const double* const* u_adof,
+                              double* const* csr_vals, double* const* rhs, int32_t n_branches,
+                              void* hip_stream, mgpu_batch** out);
+mgpu_status mgpu_batch_destroy(mgpu_batch* batch);
+/* Enqueue one replay of the graph on the batch's stream (asynchronous). */
+mgpu_status mgpu_batch_launch(mgpu_batch* batch);
+/* Wait for the batch's stream and check every set's row-length flag. */
+mgpu_status mgpu_batch_synchronize(mgpu_batch* batch);
+/* Mean duration of `iters` back-to-back replays (HIP events), and the branch count in use. */
+mgpu_status mgpu_batch_time(mgpu_batch* batch, int32_t iters, double* per_launch_ms, int32_t* n_branches);
 
 const char* mgpu_last_error_string(void);
 

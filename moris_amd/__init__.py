@@ -86,6 +86,8 @@ EXPORTED = [
     "mgpu_element_topology", "mgpu_set_assemble", "mgpu_set_assemble_host", "mgpu_set_stats",
     "mgpu_device_count", "mgpu_malloc", "mgpu_free", "mgpu_memcpy_htod", "mgpu_memcpy_dtoh",
     "mgpu_synchronize", "mgpu_set_time_assemble", "mgpu_last_error_string",
+    "mgpu_set_tuning", "mgpu_set_get_tuning", "mgpu_set_tuning_variants", "mgpu_set_autotune",
+    "mgpu_batch_create", "mgpu_batch_destroy", "mgpu_batch_launch", "mgpu_batch_synchronize", "mgpu_batch_time",
 ]
 
 _lib = None
@@ -126,6 +128,16 @@ def load_library(path=LIB_PATH):
     lib.mgpu_memcpy_dtoh.argtypes = [vp, vp, ctypes.c_size_t]
     lib.mgpu_synchronize.argtypes = []
     lib.mgpu_set_time_assemble.argtypes = [vp, vp, vp, vp, ctypes.c_int32, _c_f64p, _c_f64p]
+    lib.mgpu_set_tuning.argtypes = [vp, ctypes.c_int32, ctypes.c_int32]
+    lib.mgpu_set_get_tuning.argtypes = [vp, _c_i32p, _c_i32p]
+    lib.mgpu_set_tuning_variants.argtypes = [vp, _c_i32p, _c_i32p]
+    lib.mgpu_set_autotune.argtypes = [vp, vp, vp, vp, ctypes.c_int32, ctypes.c_int32, _c_f64p]
+    lib.mgpu_batch_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int32, ctypes.POINTER(vp), ctypes.POINTER(vp),
+                                      ctypes.POINTER(vp), ctypes.c_int32, vp, ctypes.POINTER(vp)]
+    lib.mgpu_batch_destroy.argtypes = [vp]
+    lib.mgpu_batch_launch.argtypes = [vp]
+    lib.mgpu_batch_synchronize.argtypes = [vp]
+    lib.mgpu_batch_time.argtypes = [vp, ctypes.c_int32, _c_f64p, _c_i32p]
     for name in EXPORTED:
         getattr(lib, name).restype = ctypes.c_int32 if name != "mgpu_last_error_string" else ctypes.c_char_p
     _lib = lib
@@ -318,6 +330,34 @@ class Set:
                                                      ctypes.byref(per), km), "mgpu_set_time_assemble")
         return per.value, list(km)
 
+    def tuning_variants(self):
+        """Rows per workgroup of each stencil variant."""
+        n = ctypes.c_int32()
+        rows = (ctypes.c_int32 * 3)()
+        _check(load_library().mgpu_set_tuning_variants(self._h, ctypes.byref(n), rows), "mgpu_set_tuning_variants")
+        return list(rows)[: n.value]
+
+    def tuning(self, stencil_variant=None, cut_order=None):
+        """Get, or set and return, the launch shape (stencil variant, cut order)."""
+        lib = load_library()
+        v, o = ctypes.c_int32(), ctypes.c_int32()
+        _check(lib.mgpu_set_get_tuning(self._h, ctypes.byref(v), ctypes.byref(o)), "mgpu_set_get_tuning")
+        if stencil_variant is not None or cut_order is not None:
+            nv = v.value if stencil_variant is None else stencil_variant
+            no = o.value if cut_order is None else cut_order
+            _check(lib.mgpu_set_tuning(self._h, nv, no), "mgpu_set_tuning")
+            return nv, no
+        return v.value, o.value
+
+    def autotune(self, u_buf, vals_buf, rhs_buf, iters=5, ignore_cache=False):
+        """Time every launch shape and keep the fastest; returns {(variant, order): ms}."""
+        nv = len(self.tuning_variants())
+        ms = np.zeros(2 * nv, dtype=np.float64)
+        _check(load_library().mgpu_set_autotune(self._h, u_buf.ptr, vals_buf.ptr, rhs_buf.ptr, iters,
+                                                1 if ignore_cache else 0, _ptr(ms, ctypes.c_double)),
+               "mgpu_set_autotune")
+        return {(k // 2, k % 2): float(ms[k]) for k in range(2 * nv) if ms[k] >= 0.0}
+
     def stats(self):
         s = (ctypes.c_int64 * 5)()
         _check(load_library().mgpu_set_stats(self._h, s), "mgpu_set_stats")
@@ -326,6 +366,47 @@ class Set:
     def close(self):
         if self._h:
             load_library().mgpu_set_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """One assemble call of each set captured into a HIP graph, replayed with one launch."""
+
+    def __init__(self, sets, u_bufs, vals_bufs, rhs_bufs, n_branches=0):
+        lib = load_library()
+        n = len(sets)
+        if not (len(u_bufs) == len(vals_bufs) == len(rhs_bufs) == n):
+            raise ValueError("one u, vals and rhs buffer per set")
+        vp = ctypes.c_void_p
+        hs = (vp * n)(*[s._h for s in sets])
+        us = (vp * n)(*[b.ptr for b in u_bufs])
+        vs = (vp * n)(*[b.ptr for b in vals_bufs])
+        rs = (vp * n)(*[b.ptr for b in rhs_bufs])
+        self._keep = (list(sets), list(u_bufs), list(vals_bufs), list(rhs_bufs))
+        self._h = vp()
+        _check(lib.mgpu_batch_create(hs, n, us, vs, rs, n_branches, None, ctypes.byref(self._h)), "mgpu_batch_create")
+
+    def launch(self):
+        _check(load_library().mgpu_batch_launch(self._h), "mgpu_batch_launch")
+
+    def synchronize(self):
+        _check(load_library().mgpu_batch_synchronize(self._h), "mgpu_batch_synchronize")
+
+    def time(self, iters):
+        per = ctypes.c_double()
+        nb = ctypes.c_int32()
+        _check(load_library().mgpu_batch_time(self._h, iters, ctypes.byref(per), ctypes.byref(nb)), "mgpu_batch_time")
+        return per.value, nb.value
+
+    def close(self):
+        if self._h:
+            load_library().mgpu_batch_destroy(self._h)
             self._h = ctypes.c_void_p()
 
     def __del__(self):

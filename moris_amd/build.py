@@ -26,6 +26,7 @@ HOST_SOURCES = [
 ]
 HIP_SOURCES = [
     "set/assemble.hip",
+    "set/batch.hip",
     "learned/lq_assemble.hip",
     "learned/mlp.hip",
 ]
@@ -72,6 +73,9 @@ def build(verbose=False):
     objs = [_obj(s) for s in HOST_SOURCES + HIP_SOURCES]
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-pthread", "-o", tmp] + objs)
     os.replace(tmp, os.path.join(LIB, LIBNAME))
+    # HBM counter calibration program (copy / write-only kernels of a known size)
+    _run([HIPCC] + HIP_FLAGS + ["-x", "hip", os.path.join(CSRC, "calib", "hbm_calibrate.cpp"), "-o",
+                                os.path.join(LIB, "mgpu_hbm_calibrate")])
     return os.path.join(LIB, LIBNAME)
 
 

@@ -53,7 +53,7 @@ __global__ __launch_bounds__(64) void k_lq_fit(DevView s, LqView lv) {
     __shared__ double m[kLqMoments];
     __shared__ double w[32];
     __shared__ double r[32];
-    const int c = blockIdx.x;
+    const int c = s.cut_order ? s.cut_order[blockIdx.x] : (int)blockIdx.x;
     const int lane = threadIdx.x;
     const int e = s.cut_elems[c];
     const int ig0 = s.ig_ptr[e];

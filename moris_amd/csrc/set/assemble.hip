@@ -33,9 +33,7 @@
 
 #include "../common/fem_tables.hpp"
 #include "../setup/internal.hpp"
-#include "set_device.hpp"
-#include "set_internal.hpp"
-#include "../learned/lq_view.hpp"
+#include "set_state.hpp"
 #include "../learned/quad.hpp"
 
 namespace mgpu {
@@ -52,7 +50,7 @@ __global__ __launch_bounds__(64) void k_cut_diffusion(DevView s) {
     constexpr int NSYM = NB * (NB + 1) / 2;
     __shared__ double red[NSYM * 65];
 
-    const int c = blockIdx.x;
+    const int c = s.cut_order ? s.cut_order[blockIdx.x] : (int)blockIdx.x;
     if (s.skip && s.skip[c]) return;
     const int lane = threadIdx.x;
     const int e = s.cut_elems[c];
@@ -119,7 +117,7 @@ __global__ __launch_bounds__(64) void k_cut_elasticity(DevView s) {
     __shared__ T sgw[64 * GPAD];
     __shared__ T sg[64 * GPAD];
 
-    const int c = blockIdx.x;
+    const int c = s.cut_order ? s.cut_order[blockIdx.x] : (int)blockIdx.x;
     if (s.skip && s.skip[c]) return;
     const int lane = threadIdx.x;
     const int e = s.cut_elems[c];
@@ -282,10 +280,15 @@ __device__ constexpr int lattice_sorted_pos(int b) {
     return DIM == 3 ? (b == 2 ? 3 : (b == 3 ? 2 : (b == 6 ? 7 : (b == 7 ? 6 : b)))) : (b == 2 ? 3 : (b == 3 ? 2 : b));
 }
 
-// rows per block; a block has ROWS * NF threads and stages its contiguous CSR range in LDS
-template <int DIM, int NF>
-__host__ __device__ constexpr int stencil_rows() {
-    return NF == 1 ? 128 : (DIM == 3 ? 64 : 128);
+// Rows per block (the autotuner's stencil variants); a block has ROWS * NF threads (<= 256) and
+// stages its contiguous CSR range in LDS.  Variant 1 is the default.
+template <int NF>
+__host__ __device__ constexpr int stencil_rows(int variant) {
+    return NF == 3 ? (variant == 0 ? 32 : 64) : (variant == 0 ? 64 : (variant == 1 ? 128 : 256));
+}
+template <int NF>
+__host__ __device__ constexpr int stencil_variants() {
+    return NF == 1 ? 3 : 2;
 }
 
 template <int DIM, int NF, int AL, int BE>
@@ -410,13 +413,12 @@ __device__ inline void stencil_dispatch_be(const DevView& s, const double* sM, i
     }
 }
 
-template <int DIM, int NF>
+template <int DIM, int NF, int ROWS>
 __global__ __launch_bounds__(256) void k_stencil(DevView s, const double* u, double* vals, double* rhs) {
     constexpr int NB = DIM == 3 ? 8 : 4;
     constexpr int NG = DIM * NB;
     constexpr bool DIFF = NF == 1;
     constexpr int NM = DIFF ? NB * NB : NG * NG;
-    constexpr int ROWS = stencil_rows<DIM, NF>();
     constexpr int NS27 = DIM == 3 ? 27 : 9;
     __shared__ double sM[NM];
     __shared__ double stage[ROWS * NF * NS27];
@@ -459,35 +461,9 @@ __global__ __launch_bounds__(256) void k_stencil(DevView s, const double* u, dou
 // ================================================================================================
 using namespace mgpu;
 
-struct mgpu_set {
-    SetHost h;
-    DevView v{};
-    hipStream_t stream = nullptr;
-    int precision = MGPU_PREC_FP64;
-    int quadrature = MGPU_QUAD_SUBDIVISION;
-    int64_t stats[5] = {0, 0, 0, 0, 0};
-    int64_t n_ip_cut = 0;
-    std::vector<void*> allocs;
-    double* d_u_tmp = nullptr;
-    double* d_vals_tmp = nullptr;
-    double* d_rhs_tmp = nullptr;
-    // learned cut-cell quadrature
-    std::vector<double> vphi_cut;      // host copy of the cut elements' nodal level sets
-    LqView lv{};
-    const mgpu_mlp* mlp = nullptr;
-    bool lq_ready = false;
-};
-
 namespace {
 
 thread_local std::string g_last_error;
-
-#define HIP_TRY(expr)                                                                              \
-    do {                                                                                           \
-        hipError_t _e = (expr);                                                                    \
-        if (_e != hipSuccess)                                                                      \
-            return mgpu::set_error(MGPU_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e)); \
-    } while (0)
 
 template <class T>
 mgpu_status upload(mgpu_set* S, const T* src, size_t n, T** dst) {
@@ -522,31 +498,48 @@ void launch_subdivision(mgpu_set* S, const int32_t* skip) {
     }
 }
 
-// ev (optional): [0] start, [1] after the cut-element integrands, [2] after the scatter,
-// [3] after the learned moments/fit (learned quadrature only)
+template <int DIM, int NF, int ROWS>
+void launch_stencil_rows(mgpu_set* S, const double* u, double* vals, double* rhs) {
+    const dim3 grid((unsigned)((S->v.N + ROWS - 1) / ROWS), (unsigned)NF);
+    hipLaunchKernelGGL((k_stencil<DIM, NF, ROWS>), grid, dim3(ROWS * NF), 0, S->stream, S->v, u, vals, rhs);
+}
+
+template <int DIM, int NF>
+void launch_stencil(mgpu_set* S, int variant, const double* u, double* vals, double* rhs) {
+    switch (variant) {
+        case 0: launch_stencil_rows<DIM, NF, stencil_rows<NF>(0)>(S, u, vals, rhs); break;
+        case 2: if constexpr (stencil_variants<NF>() > 2) { launch_stencil_rows<DIM, NF, stencil_rows<NF>(2)>(S, u, vals, rhs); break; }
+                [[fallthrough]];
+        default: launch_stencil_rows<DIM, NF, stencil_rows<NF>(1)>(S, u, vals, rhs); break;
+    }
+}
+
+}  // namespace
+
+namespace mgpu {
 mgpu_status launch_assemble(mgpu_set* S, const double* u, double* vals, double* rhs, hipEvent_t* ev) {
     const SetHost& h = S->h;
     if (h.n_rows == 0) return MGPU_OK;
     const int dim = h.dim, nf = h.nf;
     if (ev) HIP_TRY(hipEventRecord(ev[0], S->stream));
+    const bool learned = S->v.n_cut > 0 && S->quadrature == MGPU_QUAD_LEARNED;
+    if (learned) {
+        lq_launch_inputs(S->lv, S->stream);
+        mlp_forward_launch(S->mlp, S->lv.x_in, S->lv.n_cut, S->lv.w_pred, S->stream);
+        lq_launch_fit(S->v, S->lv, S->stream);
+        HIP_TRY(hipGetLastError());
+    }
+    if (ev) HIP_TRY(hipEventRecord(ev[1], S->stream));
     if (S->v.n_cut > 0) {
-        if (S->quadrature == MGPU_QUAD_LEARNED) {
-            lq_launch_inputs(S->lv, S->stream);
-            mlp_forward_launch(S->mlp, S->lv.x_in, S->lv.n_cut, S->lv.w_pred, S->stream);
-            lq_launch_fit(S->v, S->lv, S->stream);
-            HIP_TRY(hipGetLastError());
-            if (ev) HIP_TRY(hipEventRecord(ev[3], S->stream));
+        if (learned) {
             lq_launch_element(S->v, S->lv, nf, S->stream);
             launch_subdivision(S, S->lv.flag);
         } else {
-            if (ev) HIP_TRY(hipEventRecord(ev[3], S->stream));
             launch_subdivision(S, nullptr);
         }
         HIP_TRY(hipGetLastError());
-    } else if (ev) {
-        HIP_TRY(hipEventRecord(ev[3], S->stream));
     }
-    if (ev) HIP_TRY(hipEventRecord(ev[1], S->stream));
+    if (ev) HIP_TRY(hipEventRecord(ev[2], S->stream));
     {
         const int64_t n_nodes = (int64_t)(S->v.nx + 1) * (S->v.ny + 1) * (dim == 3 ? S->v.nz + 1 : 1);
         dim3 gp((unsigned)(1 + (n_nodes + 255) / 256)), bp(256);
@@ -555,21 +548,21 @@ mgpu_status launch_assemble(mgpu_set* S, const double* u, double* vals, double* 
         else if (dim == 2 && nf == 1) hipLaunchKernelGGL((k_prep<2, 1>), gp, bp, 0, S->stream, S->v, u);
         else hipLaunchKernelGGL((k_prep<2, 2>), gp, bp, 0, S->stream, S->v, u);
         HIP_TRY(hipGetLastError());
-        auto grid = [&](int rows) { return dim3((unsigned)((S->v.N + rows - 1) / rows), (unsigned)nf); };
-        if (dim == 3 && nf == 1)
-            hipLaunchKernelGGL((k_stencil<3, 1>), grid(stencil_rows<3, 1>()), dim3(stencil_rows<3, 1>() * 1), 0, S->stream, S->v, u, vals, rhs);
-        else if (dim == 3 && nf == 3)
-            hipLaunchKernelGGL((k_stencil<3, 3>), grid(stencil_rows<3, 3>()), dim3(stencil_rows<3, 3>() * 3), 0, S->stream, S->v, u, vals, rhs);
-        else if (dim == 2 && nf == 1)
-            hipLaunchKernelGGL((k_stencil<2, 1>), grid(stencil_rows<2, 1>()), dim3(stencil_rows<2, 1>() * 1), 0, S->stream, S->v, u, vals, rhs);
-        else
-            hipLaunchKernelGGL((k_stencil<2, 2>), grid(stencil_rows<2, 2>()), dim3(stencil_rows<2, 2>() * 2), 0, S->stream, S->v, u, vals, rhs);
+        if (ev) HIP_TRY(hipEventRecord(ev[3], S->stream));
+        const int var = S->stencil_variant < 0 ? 1 : S->stencil_variant;
+        if (dim == 3 && nf == 1) launch_stencil<3, 1>(S, var, u, vals, rhs);
+        else if (dim == 3 && nf == 3) launch_stencil<3, 3>(S, var, u, vals, rhs);
+        else if (dim == 2 && nf == 1) launch_stencil<2, 1>(S, var, u, vals, rhs);
+        else launch_stencil<2, 2>(S, var, u, vals, rhs);
         HIP_TRY(hipGetLastError());
     }
-    if (ev) HIP_TRY(hipEventRecord(ev[2], S->stream));
+    if (ev) HIP_TRY(hipEventRecord(ev[4], S->stream));
     return MGPU_OK;
 }
 
+}  // namespace mgpu
+
+namespace {
 // Buffers and constant tables of the learned quadrature (lattice hex8 sets).
 mgpu_status lq_prepare(mgpu_set* S) {
     if (S->lq_ready) return MGPU_OK;
@@ -617,6 +610,22 @@ mgpu_status lq_prepare(mgpu_set* S) {
     return MGPU_OK;
 }
 
+}  // namespace
+
+namespace mgpu {
+int stencil_shape(int nf, int variant, int* n_variants) {
+    if (nf == 1) {
+        *n_variants = stencil_variants<1>();
+        return stencil_rows<1>(variant);
+    }
+    if (nf == 3) {
+        *n_variants = stencil_variants<3>();
+        return stencil_rows<3>(variant);
+    }
+    *n_variants = stencil_variants<2>();
+    return stencil_rows<2>(variant);
+}
+
 mgpu_status check_err_flag(mgpu_set* S) {
     int32_t flag = 0;
     HIP_TRY(hipMemcpyAsync(&flag, S->v.err_flag, sizeof(int32_t), hipMemcpyDeviceToHost, S->stream));
@@ -625,9 +634,6 @@ mgpu_status check_err_flag(mgpu_set* S) {
     return MGPU_OK;
 }
 
-}  // namespace
-
-namespace mgpu {
 mgpu_status set_error(mgpu_status code, const std::string& msg) {
     g_last_error = msg;
     return code;
@@ -679,6 +685,15 @@ extern "C" mgpu_status mgpu_set_create(const mgpu_set_desc* desc, mgpu_set** out
         HIP_TRY(hipMalloc(&p, (size_t)v.n_cut * h.na * h.na * sizeof(double)));
         S->allocs.push_back(p);
         v.ke_cut = static_cast<double*>(p);
+        // longest-first processing order of the cut elements (sub-cell count descending, stable)
+        std::vector<int32_t> order(v.n_cut);
+        for (int32_t c = 0; c < v.n_cut; ++c) order[c] = c;
+        std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
+            const int32_t ex = h.cut_elems[x], ey = h.cut_elems[y];
+            return desc->ig_ptr[ex + 1] - desc->ig_ptr[ex] > desc->ig_ptr[ey + 1] - desc->ig_ptr[ey];
+        });
+        if ((rc = upload(S.get(), order.data(), order.size(), const_cast<int32_t**>(&S->d_cut_order))) != MGPU_OK)
+            return rc;
     }
     {
         void* p = nullptr;
@@ -834,23 +849,22 @@ extern "C" mgpu_status mgpu_set_time_assemble(mgpu_set* set, const double* u, do
     MGPU_GUARD_BEGIN
     if (!set || iters < 1) return set_error(MGPU_ERR_INVALID_ARG, "bad argument");
     if (set->quadrature == MGPU_QUAD_LEARNED && !set->mlp) return set_error(MGPU_ERR_STATE, "no quadrature model");
-    hipEvent_t ev[4];
+    hipEvent_t ev[5];
     for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
     double t[4] = {0, 0, 0, 0};
     hipEvent_t t0, t1;
     HIP_TRY(hipEventCreate(&t0));
     HIP_TRY(hipEventCreate(&t1));
-    // per-kernel segments, one call at a time
+    // per-stage segments, one call at a time
     for (int it = 0; it < iters; ++it) {
         mgpu_status rc = launch_assemble(set, u, vals, rhs, ev);
         if (rc != MGPU_OK) return rc;
-        HIP_TRY(hipEventSynchronize(ev[2]));
-        float a = 0.f, b = 0.f, c = 0.f, d = 0.f;
-        HIP_TRY(hipEventElapsedTime(&a, ev[0], ev[1]));
-        HIP_TRY(hipEventElapsedTime(&b, ev[1], ev[2]));
-        HIP_TRY(hipEventElapsedTime(&c, ev[0], ev[3]));
-        HIP_TRY(hipEventElapsedTime(&d, ev[3], ev[1]));
-        t[0] += a; t[1] += b; t[2] += c; t[3] += d;
+        HIP_TRY(hipEventSynchronize(ev[4]));
+        for (int k = 0; k < 4; ++k) {
+            float a = 0.f;
+            HIP_TRY(hipEventElapsedTime(&a, ev[k], ev[k + 1]));
+            t[k] += a;
+        }
     }
     // back-to-back calls for the per-call mean
     HIP_TRY(hipEventRecord(t0, set->stream));

@@ -33,6 +33,7 @@ struct DevView {
                                 // elasticity [4] {flag, D_nn, D_nn', D_shear}
     double* u_node;             // per lattice node: u of its adofs [n_nodes][nf] (0 if unused)
     const int32_t* skip;        // [n_cut] nonzero: element integrated elsewhere (learned), or NULL
+    const int32_t* cut_order;   // [n_cut] cut element processed by block b (longest first), or NULL
 };
 
 // Voigt index of the symmetric pair (i, j): 3D [xx,yy,zz,yz,xz,xy]

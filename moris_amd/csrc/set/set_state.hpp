@@ -1,0 +1,54 @@
+// set_state.hpp -- host-side state of an mgpu_set and the launch sequence shared by the single-set
+// entry points (assemble.hip) and the captured batches (batch.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../setup/internal.hpp"
+#include "../learned/lq_view.hpp"
+#include "set_device.hpp"
+#include "set_internal.hpp"
+
+#define HIP_TRY(expr)                                                                              \
+    do {                                                                                           \
+        hipError_t _e = (expr);                                                                    \
+        if (_e != hipSuccess)                                                                      \
+            return mgpu::set_error(MGPU_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+    } while (0)
+
+struct mgpu_set {
+    mgpu::SetHost h;
+    mgpu::DevView v{};
+    hipStream_t stream = nullptr;
+    int precision = MGPU_PREC_FP64;
+    int quadrature = MGPU_QUAD_SUBDIVISION;
+    int64_t stats[5] = {0, 0, 0, 0, 0};
+    int64_t n_ip_cut = 0;
+    std::vector<void*> allocs;
+    double* d_u_tmp = nullptr;
+    double* d_vals_tmp = nullptr;
+    double* d_rhs_tmp = nullptr;
+    // launch shape (autotuned or set by mgpu_set_tuning)
+    int stencil_variant = -1;               // index into the stencil row-block sizes, -1 default
+    int cut_order = 0;                      // 1: cut elements longest-first
+    const int32_t* d_cut_order = nullptr;   // device permutation (built on first use)
+    // learned cut-cell quadrature
+    std::vector<double> vphi_cut;      // host copy of the cut elements' nodal level sets
+    mgpu::LqView lv{};
+    const mgpu_mlp* mlp = nullptr;
+    bool lq_ready = false;
+};
+
+namespace mgpu {
+// Enqueue one assemble call of the set on S->stream (no synchronisation, no allocation, no memset:
+// safe to capture into a graph).  ev (optional, 5 events): [0] start, [1] after the learned
+// inputs/network/moment fit, [2] after the cut-element integrands, [3] after the cell records,
+// [4] after the stencil scatter.
+mgpu_status launch_assemble(mgpu_set* S, const double* u, double* vals, double* rhs, hipEvent_t* ev);
+// Synchronise the set's stream and turn the kernels' row-length flag into a status.
+mgpu_status check_err_flag(mgpu_set* S);
+// Rows per workgroup of stencil variant `variant` for nf fields; *n_variants: number of variants.
+int stencil_shape(int nf, int variant, int* n_variants);
+}  // namespace mgpu

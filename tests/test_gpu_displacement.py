@@ -1,0 +1,98 @@
+"""Displacement error of the GPU assembly against the oracle's K.
+
+Linear elasticity (E = 1, nu = 0.3), left face x = 0 clamped, unit traction (1, 0[, 0]) on the
+right face x = 1 as consistent nodal loads of the lattice faces there (the right face is uncut
+material in these cases).  K u = f is solved with a sparse direct solver in FP64 for K from the
+oracle, from the plain FP64 GPU path and from the mixed-precision (+ learned quadrature in 3D)
+GPU path; the displacement errors are held to 1e-10 (FP64) and 1e-5 (mixed / learned).
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+import moris_amd as M
+from helpers import adof_coords, spec_circle2
+from moris_amd import learned as L
+from moris_amd import workloads as W
+from oracle import py as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _loads_and_free(P, dim, n):
+    nodes, X = adof_coords(P.arrays(), dim, n)
+    N = len(nodes)
+    h = 1.0 / n[0]
+    f = np.zeros(dim * N)
+    right = np.isclose(X[:, 0], 1.0)
+    # consistent loads of a uniform unit x-traction on the lattice faces of x = 1
+    if dim == 2:
+        w = np.where(np.isclose(X[right, 1], 0.0) | np.isclose(X[right, 1], 1.0), 0.5 * h, h)
+    else:
+        edge_y = np.isclose(X[right, 1], 0.0) | np.isclose(X[right, 1], 1.0)
+        edge_z = np.isclose(X[right, 2], 0.0) | np.isclose(X[right, 2], 1.0)
+        w = h * h * np.where(edge_y, 0.5, 1.0) * np.where(edge_z, 0.5, 1.0)
+    f[np.flatnonzero(right)] = w
+    fixed_nodes = np.flatnonzero(np.isclose(X[:, 0], 0.0))
+    fixed = np.concatenate([c * N + fixed_nodes for c in range(dim)])
+    free = np.setdiff1d(np.arange(dim * N), fixed)
+    return f, free
+
+
+def _solve(rp, ci, vals, f, free):
+    n = len(rp) - 1
+    K = sp.csr_matrix((vals, ci, rp), shape=(n, n))
+    Kf = K[free][:, free].tocsc()
+    u = np.zeros(n)
+    u[free] = spla.spsolve(Kf, f[free])
+    return u
+
+
+def _errors(spec, keep, dim, n, modes):
+    P = M.Problem(spec, keep)
+    S = M.Set(P.desc, keepalive=P)
+    rp, ci = S.pattern()
+    f, free = _loads_and_free(P, dim, n)
+    u0 = np.zeros(S.n_rows)
+    vo, _ = O.OracleProblem(spec).assemble(u0)
+    u_ref = _solve(rp, ci, vo, f, free)
+    out = {}
+    for name, setup in modes.items():
+        setup(S)
+        vals, _ = S.compute_jacobian_and_residual(u0)
+        u = _solve(rp, ci, vals, f, free)
+        out[name] = np.linalg.norm(u - u_ref) / np.linalg.norm(u_ref)
+    return out
+
+
+def test_displacement_error_2d_plane_stress_circle_hole():
+    n = (64, 64)
+    spec, keep = spec_circle2(64)
+    e = _errors(spec, keep, 2, n, {
+        "fp64": lambda S: S.options(M.PREC_FP64, M.QUAD_SUBDIVISION),
+        "mixed": lambda S: S.options(M.PREC_MIXED, M.QUAD_SUBDIVISION),
+    })
+    print("2D displacement errors", e)
+    assert e["fp64"] <= 1e-10
+    assert e["mixed"] <= 1e-5
+
+
+def test_displacement_error_3d_sphere_array_learned():
+    n = 16
+    spec, keep = W.elasticity_spec(n, W.sphere_array(0, per_axis=2, radius=0.2))
+    net = L.MLP(seed=12)
+
+    def learned(S):
+        L.attach(S, net, tol=1e-7)
+        S.options(M.PREC_MIXED, M.QUAD_LEARNED)
+
+    e = _errors(spec, keep, 3, (n, n, n), {
+        "fp64": lambda S: S.options(M.PREC_FP64, M.QUAD_SUBDIVISION),
+        "mixed": lambda S: S.options(M.PREC_MIXED, M.QUAD_SUBDIVISION),
+        "mixed+learned": learned,
+    })
+    print("3D displacement errors", e)
+    assert e["fp64"] <= 1e-10
+    assert e["mixed"] <= 1e-5
+    assert e["mixed+learned"] <= 1e-5
