@@ -87,6 +87,10 @@ typedef struct mgpu_problem_spec {
     double  E, nu;             /* elasticity                                                     */
     double  k_mu, k_sigma;     /* diffusion: k_e = exp(k_mu + k_sigma * N(0,1)) per element     */
     uint64_t seed;             /* splitmix64 seed of the per-element conductivity                */
+    int32_t bspline_order;     /* 1 (or 0): T rows of weight 1 per Lagrange node (p = 1);
+                                  2: quadratic B-splines of the lattice evaluated at the linear
+                                  Lagrange nodes, 2^dim entries of weight 1/2^dim per row
+                                  (HMR T-matrix at level 0, cl_HMR_T_Matrix_Base.cpp:306-457)  */
 } mgpu_problem_spec;
 
 /* Uniform lattice the elements of a set sit on (HMR level 0).  n[0] == 0: no lattice. */
@@ -202,12 +206,15 @@ mgpu_status mgpu_set_time_assemble(mgpu_set* set, const double* u_adof, double* 
                                    double kernel_ms[4]);
 
 /* ---- launch shapes and the autotuner (no counterpart in the reference) -------------------------
-   A set's launch shape is the stencil row-block variant (rows per workgroup, see
-   mgpu_set_tuning_variants) and the order of the cut-element workgroups (0 natural, 1 longest
-   first by sub-cell count).  Results are bit-identical for every shape.                        */
+   A set's launch shape is the stencil variant and the order of the cut-element workgroups
+   (0 natural, 1 longest first by sub-cell count).  Stencil variants differ in rows per workgroup
+   and in how a row reaches HBM: the first half of the variants stage the workgroup's CSR range
+   in LDS and write it coalesced, the second half (same row counts) store each row directly.
+   Results are bit-identical for every shape.                                                   */
 mgpu_status mgpu_set_tuning(mgpu_set* set, int32_t stencil_variant, int32_t cut_order);
 mgpu_status mgpu_set_get_tuning(const mgpu_set* set, int32_t* stencil_variant, int32_t* cut_order);
-/* Number of stencil variants of the set and the rows per workgroup of each (rows[] optional, >= 3). */
+/* Number of stencil variants of the set and the rows per workgroup of each (rows[] optional,
+   at least 6 entries).                                                                         */
 mgpu_status mgpu_set_tuning_variants(const mgpu_set* set, int32_t* n_variants, int32_t* rows);
 /* Time every launch shape with `iters` assemble calls on the given device buffers and keep the
    fastest.  The choice is cached per process by set shape (dimension, fields, rows, cut count

@@ -4,9 +4,10 @@
  * These entry points have no counterpart in the reference: MORIS integrates cut cells only by XTK
  * subdivision (auto rules TRI_7 / TET_11 / TET_35, projects/FEM/INT/src/CORE/cl_FEM_Set.cpp:2983-3100).
  * They add a small network that predicts weights on the HEX_3x3x3 Gauss points of a hex8 IP cell
- * from its 8 nodal level-set values, a moment fit that makes the rule reproduce the tri-quadratic
- * moments (degree <= 4) of the element's material region, and a per-element fallback to the
- * subdivision when the fit residual exceeds the tolerance.  Same conventions as moris_gpu.h.
+ * from its 8 nodal level-set values, a moment fit that makes the rule reproduce the moments of
+ * degree <= 4 (a, b, c <= 2) of the element's material region, and a per-element fallback to the
+ * subdivision when the fit residual or the weights' cancellation exceeds its bound.  Same
+ * conventions as moris_gpu.h.
  */
 #ifndef MORIS_GPU_LEARNED_H
 #define MORIS_GPU_LEARNED_H
@@ -54,11 +55,13 @@ mgpu_status mgpu_mlp_gradients(mgpu_mlp* m, const float* x, const float* t, int6
 mgpu_status mgpu_mlp_train(mgpu_mlp* m, const float* x, const float* t, int64_t n, int32_t epochs, int32_t batch,
                            float lr, uint64_t seed, float* history);
 
-/* Attach a network to a 3D hex8 set (the set descriptor must carry vertex_phi) with the fit
-   tolerance on ||V w - m|| / ||m|| and the number of fit iterations; then
+/* Attach a network to a 3D hex8 set (the set descriptor must carry vertex_phi); then
    mgpu_set_options(set, precision, MGPU_QUAD_LEARNED) selects it.  The network must outlive the
-   set's assemble calls.                                                                       */
-mgpu_status mgpu_set_quadrature_model(mgpu_set* set, const mgpu_mlp* mlp, double tol, int32_t iters);
+   set's assemble calls.  An element keeps its learned rule when the FP32 weights w reproduce the
+   moments to ||V w - m|| / ||m|| <= tol and sum |w| <= kappa * volume (kappa >= 1 bounds the
+   cancellation, and with it the FP32 rounding of the 27-point products relative to K_e); iters > 0
+   alternates the fit with the nonnegative orthant.  Typical: tol 1e-6, iters 0, kappa 4.       */
+mgpu_status mgpu_set_quadrature_model(mgpu_set* set, const mgpu_mlp* mlp, double tol, int32_t iters, double kappa);
 /* Last learned assemble: w_pred[n_cut][27] (network output / 8), w_fit[n_cut][27] (parametric
    weights used), flags[n_cut] (1 learned, 0 subdivision fallback).  Any pointer may be NULL.  */
 mgpu_status mgpu_set_lq_result(const mgpu_set* set, float* w_pred, float* w_fit, int32_t* flags);

@@ -50,6 +50,7 @@ class ProblemSpec(ctypes.Structure):
         ("k_mu", ctypes.c_double),
         ("k_sigma", ctypes.c_double),
         ("seed", ctypes.c_uint64),
+        ("bspline_order", ctypes.c_int32),
     ]
 
 
@@ -156,7 +157,7 @@ def _ptr(a, ctype):
 
 def make_spec(dim, n, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), iwg=IWG_SPATIALDIFF_BULK, spheres=None,
               ls_kind=None, gyroid_scale=1.0, plane=(0.0, 0.0, 0.0, 0.0), material_phase=1, E=1.0, nu=0.3,
-              k_mu=0.0, k_sigma=0.0, seed=20261202):
+              k_mu=0.0, k_sigma=0.0, seed=20261202, bspline_order=1):
     """Build a ProblemSpec; returns (spec, keepalive) -- keep the second value alive with the spec."""
     s = ProblemSpec()
     s.dim = dim
@@ -178,6 +179,7 @@ def make_spec(dim, n, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), iwg=IWG_SPATIALDIF
     s.material_phase = material_phase
     s.E, s.nu, s.k_mu, s.k_sigma = E, nu, k_mu, k_sigma
     s.seed = seed
+    s.bspline_order = bspline_order
     return s, keep
 
 
@@ -213,12 +215,16 @@ class Problem:
             return np.ctypeslib.as_array(p, shape=(n,)).astype(dt, copy=True)
 
         ig_ptr = view(d.ig_ptr, ne + 1, np.int32)
+        t_ptr = view(d.t_ptr, ne * nb + 1, np.int32)
+        n_t = int(t_ptr[-1]) if ne else 0
         return {
+            "t_ptr": t_ptr,
+            "t_weight": view(d.t_weight, n_t, np.float64),
             "elem_cell": view(d.elem_cell, ne, np.int32),
             "ig_ptr": ig_ptr,
             "ig_xi": view(d.ig_xi, int(ig_ptr[-1]) * nv * dim, np.float64),
             "material": view(d.material, ne * nm, np.float64),
-            "t_adof": view(d.t_adof, ne * nb, np.int32),
+            "t_adof": view(d.t_adof, n_t, np.int32),
             "ip_box": view(d.ip_box, ne * 2 * dim, np.float64),
             "vertex_phi": view(d.vertex_phi, ne * nb, np.float64),
         }
@@ -331,9 +337,9 @@ class Set:
         return per.value, list(km)
 
     def tuning_variants(self):
-        """Rows per workgroup of each stencil variant."""
+        """Rows per workgroup of each stencil variant (second half: direct stores, no LDS staging)."""
         n = ctypes.c_int32()
-        rows = (ctypes.c_int32 * 3)()
+        rows = (ctypes.c_int32 * 8)()
         _check(load_library().mgpu_set_tuning_variants(self._h, ctypes.byref(n), rows), "mgpu_set_tuning_variants")
         return list(rows)[: n.value]
 

@@ -129,7 +129,11 @@ void lq_build_operators(LqOperators* op) {
 
 double lq_fit_nonnegative(const LqOperators& op, const double* m, const double* w0, int iters, double* w) {
     double r[kLqMoments];
-    for (int q = 0; q < kLqPoints; ++q) w[q] = w0[q];
+    // prior scaled to the volume moment m[0] (moment 0 is the monomial 1)
+    double sw = 0.0;
+    for (int q = 0; q < kLqPoints; ++q) sw += w0[q];
+    const double scale = sw > 0.0 && m[0] > 0.0 ? m[0] / sw : 1.0;
+    for (int q = 0; q < kLqPoints; ++q) w[q] = w0[q] * scale;
     for (int it = 0; it <= iters; ++it) {
         if (it > 0)
             for (int q = 0; q < kLqPoints; ++q) w[q] = std::max(w[q], 0.0);

@@ -7,8 +7,11 @@
 //                    (TET_11 on the XTK sub-tets, FP64 sub-cell Jacobians), then the moment fit
 //                    started from the MLP prior: the minimum-norm correction onto the moment
 //                    constraints (iters = 0), optionally alternated `iters` times with the
-//                    nonnegative orthant; the residual ||V w - m|| / ||m|| of the final weights
-//                    decides learned (<= tol) or fallback to subdivision
+//                    nonnegative orthant; the prior is first scaled to the element's volume.
+//                    The final weights are rounded to FP32 as the element kernel uses them; the
+//                    element stays learned if their residual ||V w - m|| / ||m|| <= tol and
+//                    sum |w| <= kappa * volume (bounded cancellation), else it falls back to
+//                    subdivision
 //   k_lq_element     one wavefront per accepted element: M = sum_q w_q g_q g_q^T on FP32 MFMA
 //                    (v_mfma_f32_16x16x4f32, FP32 accumulation) over the 27 fixed points, then
 //                    B^T D B and the projection to sorted adof order in FP64
@@ -79,6 +82,15 @@ __global__ __launch_bounds__(64) void k_lq_fit(DevView s, LqView lv) {
     }
     if (lane < kLqPoints) w[lane] = 8.0 * (double)lv.w_pred[(size_t)c * kLqPoints + lane];
     __syncthreads();
+    // prior scaled to the element's volume moment m[0]
+    {
+        double sw = lane < kLqPoints ? w[lane] : 0.0;
+        for (int off = 32; off > 0; off >>= 1) sw += __shfl_xor(sw, off);
+        const double m0 = m[0];
+        __syncthreads();
+        if (sw > 0.0 && m0 > 0.0 && lane < kLqPoints) w[lane] *= m0 / sw;
+        __syncthreads();
+    }
     for (int it = 0; it <= lv.iters; ++it) {
         if (it > 0 && lane < kLqPoints) w[lane] = fmax(w[lane], 0.0);
         __syncthreads();
@@ -95,9 +107,10 @@ __global__ __launch_bounds__(64) void k_lq_fit(DevView s, LqView lv) {
         }
         __syncthreads();
     }
-    if (lv.iters > 0 && lane < kLqPoints) w[lane] = fmax(w[lane], 0.0);
+    // the weights as the element kernel uses them (FP32), then the residual of exactly those
+    if (lane < kLqPoints) w[lane] = (double)(float)(lv.iters > 0 ? fmax(w[lane], 0.0) : w[lane]);
     __syncthreads();
-    double rk = 0.0, mk = 0.0;
+    double rk = 0.0, mk = 0.0, sa = lane < kLqPoints ? fabs(w[lane]) : 0.0;
     if (lane < kLqMoments) {
         double v = m[lane];
         for (int q = 0; q < kLqPoints; ++q) v -= lv.Vt[q * kLqMoments + lane] * w[q];
@@ -107,8 +120,11 @@ __global__ __launch_bounds__(64) void k_lq_fit(DevView s, LqView lv) {
     for (int off = 32; off > 0; off >>= 1) {
         rk += __shfl_xor(rk, off);
         mk += __shfl_xor(mk, off);
+        sa += __shfl_xor(sa, off);
     }
-    const bool ok = mk > 0.0 && sqrt(rk / mk) <= lv.tol;
+    // accepted: moments reproduced to tol, and little cancellation among the weights
+    // (sum |w| <= kappa * volume bounds the FP32 rounding of the 27-point products relative to K_e)
+    const bool ok = mk > 0.0 && m[0] > 0.0 && sqrt(rk / mk) <= lv.tol && sa <= lv.kappa * m[0];
     if (lane < kLqPoints) lv.w_fit[(size_t)c * kLqPoints + lane] = (float)w[lane];
     if (lane == 0) lv.flag[c] = ok ? 1 : 0;
 }

@@ -14,7 +14,8 @@ namespace mgpu {
 struct LqView {
     int32_t n_cut;
     int32_t iters;
-    double tol;
+    double tol;                     // moment residual tolerance
+    double kappa;                   // bound on sum |w| / volume
     double detJ_ip;                 // prod(h / 2) of the lattice cell
     const double* vphi;             // [n_cut][8] nodal level-set values of the cut elements
     float* x_in;                    // [n_cut][8]

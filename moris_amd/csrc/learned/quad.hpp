@@ -82,8 +82,9 @@ struct LqOperators {
 };
 void lq_build_operators(LqOperators* op);
 
-// Minimum-norm fit of weights to moments m starting from w0 (iters = 0), optionally alternated
-// `iters` times with the nonnegative orthant; returns the relative residual ||V w - m|| / ||m||.
+// Minimum-norm fit of weights to moments m starting from w0 scaled to the volume moment m[0]
+// (iters = 0), optionally alternated `iters` times with the nonnegative orthant; returns the
+// relative residual ||V w - m|| / ||m|| (FP64 weights; the device fit also rounds them to FP32).
 double lq_fit_nonnegative(const LqOperators& op, const double* m, const double* w0, int iters, double* w);
 
 // Physical gradients dN_a/dx_i of the hex8 basis at the 27 points of a cell of size h:

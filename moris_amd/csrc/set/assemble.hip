@@ -280,15 +280,26 @@ __device__ constexpr int lattice_sorted_pos(int b) {
     return DIM == 3 ? (b == 2 ? 3 : (b == 3 ? 2 : (b == 6 ? 7 : (b == 7 ? 6 : b)))) : (b == 2 ? 3 : (b == 3 ? 2 : b));
 }
 
-// Rows per block (the autotuner's stencil variants); a block has ROWS * NF threads (<= 256) and
-// stages its contiguous CSR range in LDS.  Variant 1 is the default.
+// Stencil variants (the autotuner's choices): rows per block (a block has ROWS * NF threads,
+// <= 256) and whether the block stages its contiguous CSR range in LDS for coalesced writes or
+// each thread stores its row directly.  Variants 0 .. n_rows_choices - 1 stage, the rest store
+// directly with the same row counts; variant 1 is the default.
 template <int NF>
-__host__ __device__ constexpr int stencil_rows(int variant) {
-    return NF == 3 ? (variant == 0 ? 32 : 64) : (variant == 0 ? 64 : (variant == 1 ? 128 : 256));
+__host__ __device__ constexpr int stencil_row_choices() {
+    return NF == 1 ? 3 : 2;
 }
 template <int NF>
 __host__ __device__ constexpr int stencil_variants() {
-    return NF == 1 ? 3 : 2;
+    return 2 * stencil_row_choices<NF>();
+}
+template <int NF>
+__host__ __device__ constexpr int stencil_rows(int variant) {
+    const int v = variant % stencil_row_choices<NF>();
+    return NF == 3 ? (v == 0 ? 32 : 64) : (v == 0 ? 64 : (v == 1 ? 128 : 256));
+}
+template <int NF>
+__host__ __device__ constexpr bool stencil_direct(int variant) {
+    return variant >= stencil_row_choices<NF>();
 }
 
 template <int DIM, int NF, int AL, int BE>
@@ -413,7 +424,7 @@ __device__ inline void stencil_dispatch_be(const DevView& s, const double* sM, i
     }
 }
 
-template <int DIM, int NF, int ROWS>
+template <int DIM, int NF, int ROWS, bool DIRECT>
 __global__ __launch_bounds__(256) void k_stencil(DevView s, const double* u, double* vals, double* rhs) {
     constexpr int NB = DIM == 3 ? 8 : 4;
     constexpr int NG = DIM * NB;
@@ -421,7 +432,7 @@ __global__ __launch_bounds__(256) void k_stencil(DevView s, const double* u, dou
     constexpr int NM = DIFF ? NB * NB : NG * NG;
     constexpr int NS27 = DIM == 3 ? 27 : 9;
     __shared__ double sM[NM];
-    __shared__ double stage[ROWS * NF * NS27];
+    __shared__ double stage[DIRECT ? 1 : ROWS * NF * NS27];
     __shared__ double rpart[NF * ROWS];
     for (int idx = threadIdx.x; idx < NM; idx += blockDim.x) sM[idx] = s.m_ref[idx];
     const int al = blockIdx.y;
@@ -435,10 +446,13 @@ __global__ __launch_bounds__(256) void k_stencil(DevView s, const double* u, dou
     const int r = r0 + rr;
     if (r < r1) {
         double* rp = rpart + be * ROWS + rr;
+        // direct: each thread stores its row segment into vals; staged: into the block's LDS copy
+        double* out = DIRECT ? vals : stage;
+        const int out_base = DIRECT ? 0 : base;
         switch (al) {
-            case 0: stencil_dispatch_be<DIM, NF, 0>(s, sM, r, u, stage, base, rp, be); break;
-            case 1: if constexpr (NF > 1) stencil_dispatch_be<DIM, NF, 1>(s, sM, r, u, stage, base, rp, be); break;
-            case 2: if constexpr (NF > 2) stencil_dispatch_be<DIM, NF, 2>(s, sM, r, u, stage, base, rp, be); break;
+            case 0: stencil_dispatch_be<DIM, NF, 0>(s, sM, r, u, out, out_base, rp, be); break;
+            case 1: if constexpr (NF > 1) stencil_dispatch_be<DIM, NF, 1>(s, sM, r, u, out, out_base, rp, be); break;
+            case 2: if constexpr (NF > 2) stencil_dispatch_be<DIM, NF, 2>(s, sM, r, u, out, out_base, rp, be); break;
             default: break;
         }
     }
@@ -449,9 +463,11 @@ __global__ __launch_bounds__(256) void k_stencil(DevView s, const double* u, dou
         for (int f = 0; f < NF; ++f) R += rpart[f * ROWS + rr];
         rhs[al * s.N + r] = R;
     }
-    const int n = end - base;
-    if (n > ROWS * NF * NS27) return;   // flagged by the row-length check
-    for (int p = threadIdx.x; p < n; p += ROWS * NF) vals[base + p] = stage[p];
+    if constexpr (!DIRECT) {
+        const int n = end - base;
+        if (n > ROWS * NF * NS27) return;   // flagged by the row-length check
+        for (int p = threadIdx.x; p < n; p += ROWS * NF) vals[base + p] = stage[p];
+    }
 }
 
 }  // namespace mgpu
@@ -498,19 +514,25 @@ void launch_subdivision(mgpu_set* S, const int32_t* skip) {
     }
 }
 
-template <int DIM, int NF, int ROWS>
-void launch_stencil_rows(mgpu_set* S, const double* u, double* vals, double* rhs) {
+template <int DIM, int NF, int V>
+void launch_stencil_variant(mgpu_set* S, const double* u, double* vals, double* rhs) {
+    constexpr int ROWS = stencil_rows<NF>(V);
     const dim3 grid((unsigned)((S->v.N + ROWS - 1) / ROWS), (unsigned)NF);
-    hipLaunchKernelGGL((k_stencil<DIM, NF, ROWS>), grid, dim3(ROWS * NF), 0, S->stream, S->v, u, vals, rhs);
+    hipLaunchKernelGGL((k_stencil<DIM, NF, ROWS, stencil_direct<NF>(V)>), grid, dim3(ROWS * NF), 0, S->stream, S->v,
+                       u, vals, rhs);
 }
 
 template <int DIM, int NF>
 void launch_stencil(mgpu_set* S, int variant, const double* u, double* vals, double* rhs) {
     switch (variant) {
-        case 0: launch_stencil_rows<DIM, NF, stencil_rows<NF>(0)>(S, u, vals, rhs); break;
-        case 2: if constexpr (stencil_variants<NF>() > 2) { launch_stencil_rows<DIM, NF, stencil_rows<NF>(2)>(S, u, vals, rhs); break; }
+        case 0: launch_stencil_variant<DIM, NF, 0>(S, u, vals, rhs); break;
+        case 2: launch_stencil_variant<DIM, NF, 2>(S, u, vals, rhs); break;
+        case 3: launch_stencil_variant<DIM, NF, 3>(S, u, vals, rhs); break;
+        case 4: if constexpr (stencil_variants<NF>() > 4) { launch_stencil_variant<DIM, NF, 4>(S, u, vals, rhs); break; }
                 [[fallthrough]];
-        default: launch_stencil_rows<DIM, NF, stencil_rows<NF>(1)>(S, u, vals, rhs); break;
+        case 5: if constexpr (stencil_variants<NF>() > 5) { launch_stencil_variant<DIM, NF, 5>(S, u, vals, rhs); break; }
+                [[fallthrough]];
+        default: launch_stencil_variant<DIM, NF, 1>(S, u, vals, rhs); break;
     }
 }
 
@@ -573,7 +595,8 @@ mgpu_status lq_prepare(mgpu_set* S) {
         return set_error(MGPU_ERR_INVALID_ARG, "learned quadrature needs vertex_phi in the set descriptor");
     LqView& lv = S->lv;
     lv.n_cut = nc;
-    if (lv.tol == 0.0) lv.tol = 1e-7;
+    if (lv.tol == 0.0) lv.tol = 1e-6;
+    if (lv.kappa == 0.0) lv.kappa = 4.0;
     lv.detJ_ip = 0.125 * h.lat.h[0] * h.lat.h[1] * h.lat.h[2];
     auto alloc = [&](size_t bytes, void** p) -> mgpu_status {
         HIP_TRY(hipMalloc(p, std::max<size_t>(bytes, 8)));
@@ -685,6 +708,11 @@ extern "C" mgpu_status mgpu_set_create(const mgpu_set_desc* desc, mgpu_set** out
         HIP_TRY(hipMalloc(&p, (size_t)v.n_cut * h.na * h.na * sizeof(double)));
         S->allocs.push_back(p);
         v.ke_cut = static_cast<double*>(p);
+        S->cut_nip.resize(v.n_cut);
+        for (int32_t c = 0; c < v.n_cut; ++c) {
+            const int32_t ce = h.cut_elems[c];
+            S->cut_nip[c] = (desc->ig_ptr[ce + 1] - desc->ig_ptr[ce]) * (h.dim == 3 ? kTet11Points : kTri7Points);
+        }
         // longest-first processing order of the cut elements (sub-cell count descending, stable)
         std::vector<int32_t> order(v.n_cut);
         for (int32_t c = 0; c < v.n_cut; ++c) order[c] = c;
@@ -811,23 +839,25 @@ extern "C" mgpu_status mgpu_set_stats(const mgpu_set* set, int64_t stats[5]) {
         stats[1] = set->v.n_cut - acc;
         stats[2] = acc;
         stats[3] = set->v.n_cut - acc;
-        stats[4] = (int64_t)(set->h.n_el - set->v.n_cut) * 8 + acc * kLqPoints;   // + fallback points (below)
-        const int32_t* ip = nullptr;
-        (void)ip;
+        int64_t n_ip = (int64_t)(set->h.n_el - set->v.n_cut) * 8 + acc * kLqPoints;
+        for (size_t c = 0; c < f.size(); ++c)
+            if (f[c] == 0) n_ip += set->cut_nip[c];
+        stats[4] = n_ip;
     }
     return MGPU_OK;
 }
 
-extern "C" mgpu_status mgpu_set_quadrature_model(mgpu_set* set, const mgpu_mlp* mlp, double tol, int32_t iters) {
+extern "C" mgpu_status mgpu_set_quadrature_model(mgpu_set* set, const mgpu_mlp* mlp, double tol, int32_t iters,
+                                                 double kappa) {
     MGPU_GUARD_BEGIN
     if (!set || !mlp) return set_error(MGPU_ERR_INVALID_ARG, "null argument");
-    if (!(tol > 0.0) || iters < 0) return set_error(MGPU_ERR_INVALID_ARG, "bad tolerance / iteration count");
-    set->lv.tol = tol;
-    set->lv.iters = iters;
+    if (!(tol > 0.0) || iters < 0 || !(kappa >= 1.0))
+        return set_error(MGPU_ERR_INVALID_ARG, "bad tolerance / iteration count / cancellation bound");
     mgpu_status rc = lq_prepare(set);
     if (rc != MGPU_OK) return rc;
     set->lv.tol = tol;
     set->lv.iters = iters;
+    set->lv.kappa = kappa;
     set->mlp = mlp;
     return MGPU_OK;
     MGPU_GUARD_END

@@ -47,11 +47,11 @@ void apply_tuning(mgpu_set* S, int variant, int order) {
 extern "C" mgpu_status mgpu_set_tuning_variants(const mgpu_set* set, int32_t* n_variants, int32_t* rows) {
     if (!set) return set_error(MGPU_ERR_INVALID_ARG, "null set");
     int n = 0;
-    for (int v = 0; v < 3; ++v) {
+    stencil_shape(set->h.nf, 0, &n);
+    for (int v = 0; v < n; ++v) {
         int nv = 0;
         const int r = stencil_shape(set->h.nf, v, &nv);
-        n = nv;
-        if (rows && v < nv) rows[v] = r;
+        if (rows) rows[v] = r;
     }
     if (n_variants) *n_variants = n;
     return MGPU_OK;

@@ -26,6 +26,7 @@ struct mgpu_set {
     int quadrature = MGPU_QUAD_SUBDIVISION;
     int64_t stats[5] = {0, 0, 0, 0, 0};
     int64_t n_ip_cut = 0;
+    std::vector<int32_t> cut_nip;           // integration points of each cut element's sub-cells
     std::vector<void*> allocs;
     double* d_u_tmp = nullptr;
     double* d_vals_tmp = nullptr;

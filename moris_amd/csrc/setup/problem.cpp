@@ -9,7 +9,9 @@
 //   adof ids                       Dof_Manager::set_owned_adofs_ids_by_type,
 //                                  cl_MSI_Dof_Manager.cpp:703-745: by dof type, then ascending
 //                                  basis index over the bases some element uses
-//   T rows of p=1 B-splines        one entry of weight 1 per Lagrange node (HMR level 0)
+//   T rows                         p = 1: one entry of weight 1 per Lagrange node; p = 2: the
+//                                  lattice's quadratic B-splines at the node (HMR level 0,
+//                                  T_Matrix_Base::evaluate, cl_HMR_T_Matrix_Base.cpp:306-457)
 #include <algorithm>
 #include <cstring>
 #include <string>
@@ -350,42 +352,96 @@ extern "C" mgpu_status mgpu_problem_build(const mgpu_problem_spec* spec, mgpu_pr
     }
     p->ig_ptr[(size_t)n_el] = (int32_t)ig;
 
-    // 4. adof numbering over used lattice nodes, ascending node index
-    std::vector<int32_t> node_adof((size_t)p->n_nodes, -1);
-    for (int64_t ee = 0; ee < n_el; ++ee) {
-        const int64_t c = p->elem_cell[(size_t)ee];
-        const int i = (int)(c % p->n[0]);
-        const int j = (int)((c / p->n[0]) % p->n[1]);
-        const int k = p->dim == 3 ? (int)(c / ((int64_t)p->n[0] * p->n[1])) : 0;
-        for (int v = 0; v < p->nb; ++v) {
-            const int64_t nid = p->dim == 3 ? node_index(*p, i + kHex8Off[v][0], j + kHex8Off[v][1], k + kHex8Off[v][2])
-                                            : node_index(*p, i + kQuad4Off[v][0], j + kQuad4Off[v][1], 0);
-            node_adof[(size_t)nid] = 0;
+    // 4./5. adof numbering and T rows
+    const int order = spec->bspline_order <= 1 ? 1 : spec->bspline_order;
+    if (order > 2) return mgpu::set_error(MGPU_ERR_UNSUPPORTED, "bspline_order must be 1 or 2");
+    if (order == 1) {
+        // p = 1: one adof per used lattice node, ascending node index; T rows of weight 1
+        std::vector<int32_t> node_adof((size_t)p->n_nodes, -1);
+        for (int64_t ee = 0; ee < n_el; ++ee) {
+            const int64_t c = p->elem_cell[(size_t)ee];
+            const int i = (int)(c % p->n[0]);
+            const int j = (int)((c / p->n[0]) % p->n[1]);
+            const int k = p->dim == 3 ? (int)(c / ((int64_t)p->n[0] * p->n[1])) : 0;
+            for (int v = 0; v < p->nb; ++v) {
+                const int64_t nid = p->dim == 3 ? node_index(*p, i + kHex8Off[v][0], j + kHex8Off[v][1], k + kHex8Off[v][2])
+                                                : node_index(*p, i + kQuad4Off[v][0], j + kQuad4Off[v][1], 0);
+                node_adof[(size_t)nid] = 0;
+            }
         }
-    }
-    int32_t next = 0;
-    for (int64_t nid = 0; nid < p->n_nodes; ++nid)
-        if (node_adof[(size_t)nid] == 0) node_adof[(size_t)nid] = next++;
-    p->n_adofs_per_type = next;
-
-    // 5. T rows: one entry of weight 1 per IP-cell vertex
-    p->t_ptr.resize((size_t)n_el * p->nb + 1);
-    p->t_adof.resize((size_t)n_el * p->nb);
-    p->t_weight.assign((size_t)n_el * p->nb, 1.0);
-    for (int64_t ee = 0; ee < n_el; ++ee) {
-        const int64_t c = p->elem_cell[(size_t)ee];
-        const int i = (int)(c % p->n[0]);
-        const int j = (int)((c / p->n[0]) % p->n[1]);
-        const int k = p->dim == 3 ? (int)(c / ((int64_t)p->n[0] * p->n[1])) : 0;
-        for (int v = 0; v < p->nb; ++v) {
-            const int64_t nid = p->dim == 3 ? node_index(*p, i + kHex8Off[v][0], j + kHex8Off[v][1], k + kHex8Off[v][2])
-                                            : node_index(*p, i + kQuad4Off[v][0], j + kQuad4Off[v][1], 0);
-            const size_t r = (size_t)ee * p->nb + v;
-            p->t_ptr[r] = (int32_t)r;
-            p->t_adof[r] = node_adof[(size_t)nid];
+        int32_t next = 0;
+        for (int64_t nid = 0; nid < p->n_nodes; ++nid)
+            if (node_adof[(size_t)nid] == 0) node_adof[(size_t)nid] = next++;
+        p->n_adofs_per_type = next;
+        p->t_ptr.resize((size_t)n_el * p->nb + 1);
+        p->t_adof.resize((size_t)n_el * p->nb);
+        p->t_weight.assign((size_t)n_el * p->nb, 1.0);
+        for (int64_t ee = 0; ee < n_el; ++ee) {
+            const int64_t c = p->elem_cell[(size_t)ee];
+            const int i = (int)(c % p->n[0]);
+            const int j = (int)((c / p->n[0]) % p->n[1]);
+            const int k = p->dim == 3 ? (int)(c / ((int64_t)p->n[0] * p->n[1])) : 0;
+            for (int v = 0; v < p->nb; ++v) {
+                const int64_t nid = p->dim == 3 ? node_index(*p, i + kHex8Off[v][0], j + kHex8Off[v][1], k + kHex8Off[v][2])
+                                                : node_index(*p, i + kQuad4Off[v][0], j + kQuad4Off[v][1], 0);
+                const size_t r = (size_t)ee * p->nb + v;
+                p->t_ptr[r] = (int32_t)r;
+                p->t_adof[r] = node_adof[(size_t)nid];
+            }
         }
+        p->t_ptr[(size_t)n_el * p->nb] = (int32_t)((size_t)n_el * p->nb);
+    } else {
+        // p = 2: uniform quadratic B-splines on the lattice, n + 2 per axis (spline s covers the
+        // cells s - 2 .. s); at lattice node i the splines s = i and s = i + 1 take the value 1/2
+        // each, so a node's row holds 2^dim splines of weight 2^-dim.  Adofs: used splines in
+        // ascending spline index (x fastest).
+        const int64_t sx = p->n[0] + 2, sy = p->n[1] + 2, sz = p->dim == 3 ? p->n[2] + 2 : 1;
+        if (sx * sy * sz >= INT32_MAX) return mgpu::set_error(MGPU_ERR_INVALID_ARG, "too many splines");
+        const int nsp = p->dim == 3 ? 8 : 4;
+        const double w = p->dim == 3 ? 0.125 : 0.25;
+        auto node_splines = [&](int i, int j, int k, int64_t* out) {
+            int q = 0;
+            for (int c = 0; c < (p->dim == 3 ? 2 : 1); ++c)
+                for (int b = 0; b < 2; ++b)
+                    for (int a = 0; a < 2; ++a) out[q++] = (i + a) + sx * ((j + b) + sy * (int64_t)(k + c));
+        };
+        std::vector<int32_t> spline_adof((size_t)(sx * sy * sz), -1);
+        int64_t sp[8];
+        for (int64_t ee = 0; ee < n_el; ++ee) {
+            const int64_t c = p->elem_cell[(size_t)ee];
+            const int i = (int)(c % p->n[0]);
+            const int j = (int)((c / p->n[0]) % p->n[1]);
+            const int k = p->dim == 3 ? (int)(c / ((int64_t)p->n[0] * p->n[1])) : 0;
+            for (int v = 0; v < p->nb; ++v) {
+                const int* off = p->dim == 3 ? kHex8Off[v] : kQuad4Off[v];
+                node_splines(i + off[0], j + off[1], p->dim == 3 ? k + off[2] : 0, sp);
+                for (int q = 0; q < nsp; ++q) spline_adof[(size_t)sp[q]] = 0;
+            }
+        }
+        int32_t next = 0;
+        for (auto& a : spline_adof)
+            if (a == 0) a = next++;
+        p->n_adofs_per_type = next;
+        const size_t n_rows = (size_t)n_el * p->nb;
+        if (n_rows * nsp >= (size_t)INT32_MAX) return mgpu::set_error(MGPU_ERR_INVALID_ARG, "too many T entries");
+        p->t_ptr.resize(n_rows + 1);
+        p->t_adof.resize(n_rows * nsp);
+        p->t_weight.assign(n_rows * nsp, w);
+        for (int64_t ee = 0; ee < n_el; ++ee) {
+            const int64_t c = p->elem_cell[(size_t)ee];
+            const int i = (int)(c % p->n[0]);
+            const int j = (int)((c / p->n[0]) % p->n[1]);
+            const int k = p->dim == 3 ? (int)(c / ((int64_t)p->n[0] * p->n[1])) : 0;
+            for (int v = 0; v < p->nb; ++v) {
+                const int* off = p->dim == 3 ? kHex8Off[v] : kQuad4Off[v];
+                node_splines(i + off[0], j + off[1], p->dim == 3 ? k + off[2] : 0, sp);
+                const size_t r = (size_t)ee * p->nb + v;
+                p->t_ptr[r] = (int32_t)(r * nsp);
+                for (int q = 0; q < nsp; ++q) p->t_adof[r * nsp + q] = spline_adof[(size_t)sp[q]];
+            }
+        }
+        p->t_ptr[n_rows] = (int32_t)(n_rows * nsp);
     }
-    p->t_ptr[(size_t)n_el * p->nb] = (int32_t)((size_t)n_el * p->nb);
 
     *out = p.release();
     return MGPU_OK;

@@ -26,7 +26,7 @@ def _lib():
         lib.mgpu_mlp_gradients.argtypes = [_vp, _vp, _vp, ctypes.c_int64, _vp, _vp]
         lib.mgpu_mlp_train.argtypes = [_vp, _vp, _vp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, ctypes.c_float,
                                        ctypes.c_uint64, _vp]
-        lib.mgpu_set_quadrature_model.argtypes = [_vp, _vp, ctypes.c_double, ctypes.c_int32]
+        lib.mgpu_set_quadrature_model.argtypes = [_vp, _vp, ctypes.c_double, ctypes.c_int32, ctypes.c_double]
         lib.mgpu_set_lq_result.argtypes = [_vp, _vp, _vp, _vp]
         for n in ("mgpu_lq_dataset", "mgpu_lq_operators", "mgpu_lq_fit", "mgpu_mlp_create", "mgpu_mlp_destroy",
                   "mgpu_mlp_num_params", "mgpu_mlp_get_params", "mgpu_mlp_set_params", "mgpu_mlp_forward",
@@ -139,9 +139,10 @@ class MLP:
             pass
 
 
-def attach(set_, mlp, tol=1e-7, iters=0):
-    """Attach a network to a Set (keeps a reference so it outlives the set's calls)."""
-    _check(_lib().mgpu_set_quadrature_model(set_._h, mlp._h, tol, iters), "mgpu_set_quadrature_model")
+def attach(set_, mlp, tol=1e-6, iters=0, kappa=4.0):
+    """Attach a network to a Set (keeps a reference so it outlives the set's calls).  Elements
+    stay learned when their FP32 weights reproduce the moments to tol and sum |w| <= kappa * volume."""
+    _check(_lib().mgpu_set_quadrature_model(set_._h, mlp._h, tol, iters, kappa), "mgpu_set_quadrature_model")
     set_._mlp = mlp
 
 

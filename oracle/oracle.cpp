@@ -482,6 +482,66 @@ void decompose(OracleProblem& p, std::map<int64_t, std::vector<double>>& cell_su
     }
 }
 
+// Uniform quadratic B-spline on knots 0, 1, 2, 3 (Cox-de Boor, degree 2), zero outside [0, 3].
+double bspline2(double t) {
+    if (t <= 0.0 || t >= 3.0) return 0.0;
+    if (t < 1.0) return 0.5 * t * t;
+    if (t < 2.0) return 0.5 * (-2.0 * t * t + 6.0 * t - 3.0);
+    return 0.5 * (3.0 - t) * (3.0 - t);
+}
+
+// T rows of p = 2 B-splines at the linear Lagrange nodes of each element (HMR level 0 T-matrix,
+// cl_HMR_T_Matrix_Base.cpp:306-457, B-spline values at the Lagrange nodes; entries with
+// |T| <= 1e-12 dropped as there).  Spline s of an axis starts at knot s - 2 (so n + 2 splines
+// cover n cells); adofs are the used splines in ascending index, x fastest.
+void bspline2_rows(OracleProblem& p, const std::vector<char>& used_nodes) {
+    const int64_t ns[3] = {p.n[0] + 2, p.n[1] + 2, p.dim == 3 ? p.n[2] + 2 : 1};
+    auto node_ijk = [&](int64_t nid, int* ijk) {
+        ijk[0] = (int)(nid % (p.n[0] + 1));
+        ijk[1] = (int)((nid / (p.n[0] + 1)) % (p.n[1] + 1));
+        ijk[2] = p.dim == 3 ? (int)(nid / ((int64_t)(p.n[0] + 1) * (p.n[1] + 1))) : 0;
+    };
+    // per node: (spline index, weight) of every spline non-zero there
+    auto row = [&](int64_t nid, std::vector<std::pair<int64_t, double>>& out) {
+        int ijk[3];
+        node_ijk(nid, ijk);
+        std::vector<std::pair<int64_t, double>> ax[3];
+        for (int d = 0; d < 3; ++d) {
+            if (d >= p.dim) { ax[d].push_back({0, 1.0}); continue; }
+            for (int64_t sp = 0; sp < ns[d]; ++sp) {
+                const double v = bspline2((double)ijk[d] - (double)(sp - 2));
+                if (std::fabs(v) > 1e-12) ax[d].push_back({sp, v});
+            }
+        }
+        out.clear();
+        for (auto& c : ax[2])
+            for (auto& b : ax[1])
+                for (auto& a : ax[0]) out.push_back({a.first + ns[0] * (b.first + ns[1] * c.first), a.second * b.second * c.second});
+        std::sort(out.begin(), out.end());
+    };
+    std::map<int64_t, int32_t> adof;
+    std::vector<std::pair<int64_t, double>> r;
+    for (int64_t nid = 0; nid < p.n_nodes; ++nid) {
+        if (!used_nodes[nid]) continue;
+        row(nid, r);
+        for (auto& q : r) adof[q.first] = 0;
+    }
+    p.N = 0;
+    for (auto& kv : adof) kv.second = p.N++;
+    for (size_t e = 0; e < p.elem_cell.size(); ++e) {
+        int64_t nodes[8];
+        cell_nodes(p, p.elem_cell[e], nodes);
+        for (int v = 0; v < p.nb; ++v) {
+            p.t_ptr.push_back((int32_t)p.t_adof.size());
+            row(nodes[v], r);
+            for (auto& q : r) {
+                p.t_adof.push_back(adof.at(q.first));
+                p.t_weight.push_back(q.second);
+            }
+        }
+    }
+}
+
 void build(OracleProblem& p, const mgpu_problem_spec& s) {
     p.spec = s;
     p.dim = s.dim;
@@ -541,17 +601,22 @@ void build(OracleProblem& p, const mgpu_problem_spec& s) {
         for (int d = 0; d < p.dim; ++d) p.ip_box.push_back(p.h[d]);
         for (int v = 0; v < p.nb; ++v) used[nodes[v]] = 1;
     }
-    std::vector<int32_t> adof(p.n_nodes, -1);
-    for (int64_t nid = 0; nid < p.n_nodes; ++nid)
-        if (used[nid]) adof[nid] = p.N++;
-    for (size_t e = 0; e < p.elem_cell.size(); ++e) {
-        int64_t nodes[8];
-        cell_nodes(p, p.elem_cell[e], nodes);
-        for (int v = 0; v < p.nb; ++v) {
-            p.t_ptr.push_back((int32_t)p.t_adof.size());
-            p.t_adof.push_back(adof[nodes[v]]);
-            p.t_weight.push_back(1.0);
+    const int order = s.bspline_order <= 1 ? 1 : s.bspline_order;
+    if (order == 1) {
+        std::vector<int32_t> adof(p.n_nodes, -1);
+        for (int64_t nid = 0; nid < p.n_nodes; ++nid)
+            if (used[nid]) adof[nid] = p.N++;
+        for (size_t e = 0; e < p.elem_cell.size(); ++e) {
+            int64_t nodes[8];
+            cell_nodes(p, p.elem_cell[e], nodes);
+            for (int v = 0; v < p.nb; ++v) {
+                p.t_ptr.push_back((int32_t)p.t_adof.size());
+                p.t_adof.push_back(adof[nodes[v]]);
+                p.t_weight.push_back(1.0);
+            }
         }
+    } else {
+        bspline2_rows(p, used);
     }
     p.t_ptr.push_back((int32_t)p.t_adof.size());
 }
@@ -873,12 +938,15 @@ void oracle_counts(void* h, int64_t* c) {
     c[3] = p.N;
     c[4] = (int64_t)p.row_ptr.size() - 1;
     c[5] = (int64_t)p.col_idx.size();
+    c[6] = (int64_t)p.t_adof.size();
 }
 
 // copies of the set-up arrays (sizes from oracle_counts)
 void oracle_get_setup(void* h, int32_t* elem_cell, int32_t* ig_ptr, double* ig_xi, double* material,
-                      int32_t* t_adof, int32_t* row_ptr, int32_t* col_idx) {
+                      int32_t* t_adof, int32_t* row_ptr, int32_t* col_idx, int32_t* t_ptr, double* t_weight) {
     const OracleProblem& p = *static_cast<OracleProblem*>(h);
+    if (t_ptr) std::copy(p.t_ptr.begin(), p.t_ptr.end(), t_ptr);
+    if (t_weight) std::copy(p.t_weight.begin(), p.t_weight.end(), t_weight);
     if (elem_cell) std::copy(p.elem_cell.begin(), p.elem_cell.end(), elem_cell);
     if (ig_ptr) std::copy(p.ig_ptr.begin(), p.ig_ptr.end(), ig_ptr);
     if (ig_xi) std::copy(p.ig_xi.begin(), p.ig_xi.end(), ig_xi);

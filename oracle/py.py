@@ -30,7 +30,7 @@ def load():
     lib.oracle_build.restype = vp
     lib.oracle_destroy.argtypes = [vp]
     lib.oracle_counts.argtypes = [vp, vp]
-    lib.oracle_get_setup.argtypes = [vp] * 8
+    lib.oracle_get_setup.argtypes = [vp] * 10
     lib.oracle_assemble.argtypes = [vp, vp, vp, vp]
     lib.oracle_assemble.restype = ctypes.c_int
     lib.oracle_assemble_desc.argtypes = [vp, vp, vp, ctypes.c_int32, vp, vp, vp, ctypes.c_int64, ctypes.c_int64,
@@ -52,9 +52,10 @@ class OracleProblem:
         self._h = lib.oracle_build(ctypes.byref(spec))
         if not self._h:
             raise RuntimeError("oracle_build: " + lib.oracle_last_error().decode())
-        c = np.zeros(6, dtype=np.int64)
+        c = np.zeros(7, dtype=np.int64)
         lib.oracle_counts(self._h, _p(c))
-        self.n_elements, self.n_cut, self.n_ig, self.n_adofs_per_type, self.n_rows, self.nnz = [int(x) for x in c]
+        (self.n_elements, self.n_cut, self.n_ig, self.n_adofs_per_type, self.n_rows, self.nnz,
+         self.n_t) = [int(x) for x in c]
         self.dim = spec.dim
         self.iwg = spec.iwg
 
@@ -67,13 +68,15 @@ class OracleProblem:
             "ig_ptr": np.zeros(self.n_elements + 1, np.int32),
             "ig_xi": np.zeros(self.n_ig * (dim + 1) * dim, np.float64),
             "material": np.zeros(self.n_elements * nm, np.float64),
-            "t_adof": np.zeros(self.n_elements * nb, np.int32),
+            "t_adof": np.zeros(self.n_t, np.int32),
             "row_ptr": np.zeros(self.n_rows + 1, np.int32),
             "col_idx": np.zeros(self.nnz, np.int32),
+            "t_ptr": np.zeros(self.n_elements * nb + 1, np.int32),
+            "t_weight": np.zeros(self.n_t, np.float64),
         }
         load().oracle_get_setup(self._h, *[_p(out[k]) for k in
                                            ("elem_cell", "ig_ptr", "ig_xi", "material", "t_adof", "row_ptr",
-                                            "col_idx")])
+                                            "col_idx", "t_ptr", "t_weight")])
         return out
 
     def assemble(self, u):

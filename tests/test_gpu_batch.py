@@ -25,7 +25,7 @@ def _set(spec_keep, quad=None, net=None, prec=M.PREC_FP64):
     P = M.Problem(spec, keep)
     S = M.Set(P.desc, keepalive=P)
     if net is not None:
-        L.attach(S, net, tol=1e-7)
+        L.attach(S, net)
     S.options(prec, quad if quad is not None else M.QUAD_SUBDIVISION)
     return spec, S
 

@@ -19,6 +19,10 @@ CASES = {
     "plane_2d": lambda: M.make_spec(2, (9, 7), ls_kind=M.LS_PLANE, plane=[0.6, 0.8, 0.0, -0.55]),
     "no_levelset": lambda: M.make_spec(3, (3, 4, 5), iwg=M.IWG_STRUC_LINEAR_BULK),
     "single_cell_cut": lambda: M.make_spec(3, (1, 1, 1), spheres=[[0.2, 0.3, 0.1, 0.55]]),
+    "p2_sphere_10_elast": lambda: M.make_spec(3, (10, 10, 10), iwg=M.IWG_STRUC_LINEAR_BULK,
+                                              spheres=[[0.5, 0.5, 0.5, 0.3]], bspline_order=2),
+    "p2_circle_16_diff": lambda: M.make_spec(2, (16, 12), spheres=[[0.5, 0.5, 0.0, 0.3]], bspline_order=2,
+                                             material_phase=0, k_sigma=0.3),
 }
 
 
@@ -29,7 +33,7 @@ def test_setup_bit_exact(case):
     Q = O.OracleProblem(spec)
     a, b = P.arrays(), Q.setup()
     assert (P.n_elements, P.n_cut, P.n_ig, P.n_adofs_per_type) == (Q.n_elements, Q.n_cut, Q.n_ig, Q.n_adofs_per_type)
-    for k in ("elem_cell", "ig_ptr", "ig_xi", "material", "t_adof"):
+    for k in ("elem_cell", "ig_ptr", "ig_xi", "material", "t_ptr", "t_adof", "t_weight"):
         assert a[k].shape == b[k].shape, k
         assert np.array_equal(a[k].view(np.uint8), b[k].view(np.uint8)), k
 
