@@ -111,7 +111,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--variants", type=int, default=32)
     ap.add_argument("--n", type=int, default=64)
-    ap.add_argument("--branches", type=int, default=0, help="graph branches (0: autotuned)")
+    ap.add_argument("--branches", type=int, default=1,
+                    help="parallel graph branches (0: autotuned); with 1 every kernel runs alone, so its "
+                         "duration under a profiler matches the per-stage event timings")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:

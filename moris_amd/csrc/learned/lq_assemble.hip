@@ -3,8 +3,10 @@
 // Per cut element (hex8 IP cell):
 //   k_lq_inputs      8 nodal level-set values / max |phi| -> MLP input
 //   (MLP forward)    8 -> 64 -> 64 -> 27 on BF16 MFMA (mlp.hip): a prior for the 27 weights
-//   k_lq_fit         one wavefront per element: the 23 moments of the material sub-cells in FP64
-//                    (TET_11 on the XTK sub-tets, FP64 sub-cell Jacobians), then the moment fit
+//   k_lq_fit         one wavefront per element, one lane per XTK sub-tet: the 23 moments of the
+//                    material sub-cells in FP64 (TET_11 on the sub-tet, its FP64 Jacobian taken
+//                    once for its 11 points, butterfly reduction across the wavefront), then the
+//                    moment fit
 //                    started from the MLP prior: the minimum-norm correction onto the moment
 //                    constraints (iters = 0), optionally alternated `iters` times with the
 //                    nonnegative orthant; the prior is first scaled to the element's volume.
@@ -33,26 +35,7 @@ __global__ void k_lq_inputs(LqView lv) {
     for (int v = 0; v < 8; ++v) lv.x_in[8 * c + v] = (float)(lv.vphi[8 * c + v] * inv);
 }
 
-// Parametric weight of point ip of the cut element's sub-cells and its image in the IP cell.
-__device__ inline double cut_point_param(const DevView& s, int ig0, int ip, double* xip) {
-    const int sc = ig0 + ip / kTet11Points;
-    const int q = ip % kTet11Points;
-    const double* X = s.ig_xi + (size_t)sc * 12;
-    const double N[4] = {kTet11Xi[q][0], kTet11Xi[q][1], 1.0 - kTet11Xi[q][0] - kTet11Xi[q][1] - kTet11Xi[q][2],
-                         kTet11Xi[q][2]};
-    for (int d = 0; d < 3; ++d) xip[d] = N[0] * X[d] + N[1] * X[3 + d] + N[2] * X[6 + d] + N[3] * X[9 + d];
-    double J[3][3];
-    const int rr[3] = {0, 1, 3};
-    for (int r = 0; r < 3; ++r)
-        for (int d = 0; d < 3; ++d) J[r][d] = X[rr[r] * 3 + d] - X[6 + d];
-    const double detJ = (J[0][0] * (J[1][1] * J[2][2] - J[2][1] * J[1][2]) -
-                         J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
-                         J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0])) / 6.0;
-    return detJ < kDetJLowerLimit ? 0.0 : kTet11W[q] * detJ;
-}
-
 __global__ __launch_bounds__(64) void k_lq_fit(DevView s, LqView lv) {
-    __shared__ double red[kLqMoments * 65];
     __shared__ double m[kLqMoments];
     __shared__ double w[32];
     __shared__ double r[32];
@@ -60,24 +43,52 @@ __global__ __launch_bounds__(64) void k_lq_fit(DevView s, LqView lv) {
     const int lane = threadIdx.x;
     const int e = s.cut_elems[c];
     const int ig0 = s.ig_ptr[e];
-    const int nip = (s.ig_ptr[e + 1] - ig0) * kTet11Points;
     double acc[kLqMoments];
 #pragma unroll
     for (int k = 0; k < kLqMoments; ++k) acc[k] = 0.0;
-    for (int ip = lane; ip < nip; ip += 64) {
-        double xip[3];
-        const double wq = cut_point_param(s, ig0, ip, xip);
-        double p[kLqMoments];
-        lq_monomials(xip, p);
+    const int nsub = s.ig_ptr[e + 1] - ig0;
+    for (int t = lane; t < nsub; t += 64) {
+        const double2* src2 = reinterpret_cast<const double2*>(s.ig_xi + (size_t)(ig0 + t) * 12);
+        double X[12];
 #pragma unroll
-        for (int k = 0; k < kLqMoments; ++k) acc[k] += wq * p[k];
+        for (int h = 0; h < 6; ++h) {
+            const double2 v = src2[h];
+            X[2 * h] = v.x;
+            X[2 * h + 1] = v.y;
+        }
+        double J[3][3];
+        const int rr[3] = {0, 1, 3};
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int d = 0; d < 3; ++d) J[r][d] = X[rr[r] * 3 + d] - X[6 + d];
+        const double detJ = (J[0][0] * (J[1][1] * J[2][2] - J[2][1] * J[1][2]) -
+                             J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                             J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0])) / 6.0;
+        if (detJ < kDetJLowerLimit) continue;
+#pragma unroll
+        for (int q = 0; q < kTet11Points; ++q) {
+            const double N[4] = {kTet11Xi[q][0], kTet11Xi[q][1], 1.0 - kTet11Xi[q][0] - kTet11Xi[q][1] - kTet11Xi[q][2],
+                                 kTet11Xi[q][2]};
+            double xip[3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) xip[d] = N[0] * X[d] + N[1] * X[3 + d] + N[2] * X[6 + d] + N[3] * X[9 + d];
+            const double wq = kTet11W[q] * detJ;
+            double p[kLqMoments];
+            lq_monomials(xip, p);
+#pragma unroll
+            for (int k = 0; k < kLqMoments; ++k) acc[k] += wq * p[k];
+        }
     }
+    // wavefront butterfly reduction of the 23 moments (no LDS buffer: keeps occupancy VGPR-bound)
 #pragma unroll
-    for (int k = 0; k < kLqMoments; ++k) red[k * 65 + lane] = acc[k];
-    __syncthreads();
+    for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+        for (int k = 0; k < kLqMoments; ++k) acc[k] += __shfl_xor(acc[k], off);
     if (lane < kLqMoments) {
         double v = 0.0;
-        for (int l = 0; l < 64; ++l) v += red[lane * 65 + l];
+#pragma unroll
+        for (int k = 0; k < kLqMoments; ++k) v = lane == k ? acc[k] : v;
         m[lane] = v;
     }
     if (lane < kLqPoints) w[lane] = 8.0 * (double)lv.w_pred[(size_t)c * kLqPoints + lane];
@@ -159,13 +170,12 @@ __global__ __launch_bounds__(64) void k_lq_element(DevView s, LqView lv) {
         }
     __syncthreads();
     const int e = s.cut_elems[c];
-    const uint8_t* vp = s.vert_pos + (size_t)e * NB;
     double* ke = s.ke_cut + (size_t)c * NA * NA;
     const int a = lane / NB, b = lane % NB;
     if (NF == 1) {
         double v = 0.0;
         for (int i = 0; i < 3; ++i) v += (double)Ms[(i * NB + a) * 33 + (i * NB + b)];
-        ke[vp[a] * NB + vp[b]] = s.material[e] * v;
+        ke[a * NB + b] = s.material[e] * v;
     } else {
         double D[6][6];
         elastic_D<3>(s.material[2 * e], s.material[2 * e + 1], D);
@@ -176,7 +186,7 @@ __global__ __launch_bounds__(64) void k_lq_element(DevView s, LqView lv) {
                     for (int j = 0; j < 3; ++j)
                         if (d_nonzero<3>(voigt<3>(al, i), voigt<3>(be, j)))
                             v += D[voigt<3>(al, i)][voigt<3>(be, j)] * (double)Ms[(i * NB + a) * 33 + (j * NB + b)];
-                ke[(al * NB + vp[a]) * NA + (be * NB + vp[b])] = v;
+                ke[(al * NB + a) * NA + (be * NB + b)] = v;
             }
     }
 }

@@ -87,7 +87,6 @@ __global__ __launch_bounds__(64) void k_cut_diffusion(DevView s) {
 #pragma unroll
     for (int t = 0; t < NSYM; ++t) red[t * 65 + lane] = (double)acc[t];
     __syncthreads();
-    const uint8_t* vp = s.vert_pos + (size_t)e * NB;
     double* ke = s.ke_cut + (size_t)c * NB * NB;
     const double k = s.material[e];
     if (lane < NSYM) {
@@ -97,8 +96,8 @@ __global__ __launch_bounds__(64) void k_cut_diffusion(DevView s) {
         int a = 0, t = lane;
         while (t >= NB - a) { t -= NB - a; ++a; }
         const int b = a + t;
-        ke[vp[a] * NB + vp[b]] = k * v;
-        ke[vp[b] * NB + vp[a]] = k * v;
+        ke[a * NB + b] = k * v;
+        ke[b * NB + a] = k * v;
     }
 }
 
@@ -172,7 +171,6 @@ __global__ __launch_bounds__(64) void k_cut_elasticity(DevView s) {
     constexpr int NS = DIM == 3 ? 6 : 3;
     double D[NS][NS];
     elastic_D<DIM>(s.material[2 * e], s.material[2 * e + 1], D);
-    const uint8_t* vp = s.vert_pos + (size_t)e * NB;
     double* ke = s.ke_cut + (size_t)c * NA * NA;
 #pragma unroll
     for (int al = 0; al < DIM; ++al)
@@ -185,7 +183,7 @@ __global__ __launch_bounds__(64) void k_cut_elasticity(DevView s) {
                 for (int j = 0; j < DIM; ++j)
                     if (d_nonzero<DIM>(voigt<DIM>(al, i), voigt<DIM>(be, j)))
                         v += D[voigt<DIM>(al, i)][voigt<DIM>(be, j)] * (double)M[i][j];
-            ke[(al * NB + vp[a]) * NA + (be * NB + vp[b])] = v;
+            ke[(al * NB + a) * NA + (be * NB + b)] = v;
         }
 }
 
@@ -205,40 +203,14 @@ __global__ __launch_bounds__(256) void k_prep(DevView s, const double* u) {
     constexpr bool DIFF = NF == 1;
     constexpr int NM = DIFF ? NB * NB : NG * NG;
     constexpr int NQ = DIM == 3 ? kHex2Points : kQuad2Points;
-    if (blockIdx.x == 0) {
-        double detJ = 1.0;
-        for (int d = 0; d < DIM; ++d) detJ *= 0.5 * s.h[d];
-        for (int idx = threadIdx.x; idx < NM; idx += blockDim.x) {
-            double v = 0.0;
-            for (int q = 0; q < NQ; ++q) {
-                double dN[DIM][NB];
-                double xq[DIM];
-                for (int d = 0; d < DIM; ++d) xq[d] = DIM == 3 ? kHex2Xi[q][d] : kQuad2Xi[q][d];
-                lagrange_dNdxi<DIM>(xq, dN);
-                const double w = (DIM == 3 ? kHex2W[q] : kQuad2W[q]) * detJ;
-                if (DIFF) {
-                    const int a = idx / NB, b = idx % NB;
-                    double t = 0.0;
-                    for (int i = 0; i < DIM; ++i) t += (dN[i][a] * (2.0 / s.h[i])) * (dN[i][b] * (2.0 / s.h[i]));
-                    v += w * t;
-                } else {
-                    const int ra = idx / NG, cb = idx % NG;
-                    const int i = ra / NB, a = ra % NB, jj = cb / NB, b = cb % NB;
-                    v += w * (dN[i][a] * (2.0 / s.h[i])) * (dN[jj][b] * (2.0 / s.h[jj]));
-                }
-            }
-            s.m_ref[idx] = v;
-        }
-        return;
-    }
-    const int64_t c = (int64_t)(blockIdx.x - 1) * blockDim.x + threadIdx.x;
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t n_cells = (int64_t)s.nx * s.ny * s.nz;
     const int64_t n_nodes = (int64_t)(s.nx + 1) * (s.ny + 1) * (DIM == 3 ? s.nz + 1 : 1);
     if (c < n_nodes) {
         // u gathered per lattice node so that rows read it without the node -> adof hop
         const int a = s.node_adof[c];
 #pragma unroll
-        for (int f = 0; f < NF; ++f) s.u_node[(size_t)NF * c + f] = a >= 0 ? u[(size_t)f * s.N + a] : 0.0;
+        for (int f = 0; f < NF; ++f) s.u_node[(size_t)f * n_nodes + c] = a >= 0 ? u[(size_t)f * s.N + a] : 0.0;
     }
     if (c >= n_cells) return;
     const int e = s.cell_elem[c];
@@ -303,7 +275,7 @@ __host__ __device__ constexpr bool stencil_direct(int variant) {
 }
 
 template <int DIM, int NF, int AL, int BE>
-__device__ inline void stencil_row(const DevView& s, const double* sM, int r, const double* u,
+__device__ inline void stencil_row(const DevView& s, const double (*sc)[3], int r, const double* u,
                                    double* stage, int stage_base, double* r_part) {
     constexpr int NB = DIM == 3 ? 8 : 4;
     constexpr int NG = DIM * NB;
@@ -316,85 +288,143 @@ __device__ inline void stencil_row(const DevView& s, const double* sM, int r, co
     const int j = (node / (nx + 1)) % (ny + 1);
     const int k = DIM == 3 ? node / ((nx + 1) * (ny + 1)) : 0;
 
-    double acc[NS27];
+    constexpr int NC = DIM == 3 ? 8 : 4;
+    // the row's CSR range and the records of its (up to NC) cells, all requested before any is
+    // used: out-of-lattice cells read cell 0 and are masked afterwards
+    const int row = AL * s.N + r;
+    const int start = s.row_ptr[row];
+    const int next = s.row_ptr[row + 1];
+    bool inside[NC];
+    double recf[NC];
+    double4 rec4[NC];
 #pragma unroll
-    for (int q = 0; q < NS27; ++q) acc[q] = 0.0;
-    uint32_t mask = 0;
+    for (int cc = 0; cc < NC; ++cc) {
+        const int ci = i + ((cc & 1) ? 0 : -1);
+        const int cj = j + ((cc & 2) ? 0 : -1);
+        const int ck = DIM == 3 ? k + ((cc & 4) ? 0 : -1) : 0;
+        inside[cc] = !(ci < 0 || ci >= nx || cj < 0 || cj >= ny || (DIM == 3 && (ck < 0 || ck >= nz)));
+        const int cell = inside[cc] ? ci + nx * (cj + ny * ck) : 0;
+        if (DIFF) recf[cc] = s.cell_rec[cell];
+        else rec4[cc] = *reinterpret_cast<const double4*>(s.cell_rec + 4 * (size_t)cell);
+    }
 
+    // stencil mask: the neighbours of the row node in its material cells
+    uint32_t mask = 0;
 #pragma unroll
-    for (int cc = 0; cc < (DIM == 3 ? 8 : 4); ++cc) {
+    for (int cc = 0; cc < NC; ++cc) {
         const int cx = (cc & 1) ? 0 : -1;
         const int cy = (cc & 2) ? 0 : -1;
         const int cz = DIM == 3 ? ((cc & 4) ? 0 : -1) : 0;
-        const int ci = i + cx, cj = j + cy, ck = k + cz;
-        if (ci < 0 || ci >= nx || cj < 0 || cj >= ny || (DIM == 3 && (ck < 0 || ck >= nz))) continue;
-        const int cell = ci + nx * (cj + ny * ck);
-        // local vertex of the row node in this cell
-        constexpr int a_tab[8] = {6, 7, 5, 4, 2, 3, 1, 0};   // vertex with offset (-cx,-cy,-cz)
-        constexpr int a_tab2[4] = {2, 3, 1, 0};
-        const int a_loc = DIM == 3 ? a_tab[cc] : a_tab2[cc];
-        double flag, c0 = 0.0, c1 = 0.0, cs = 0.0;
-        if (DIFF) {
-            flag = s.cell_rec[cell];
-            if (flag == 0.0) continue;
-        } else {
-            const double4 rec = *reinterpret_cast<const double4*>(s.cell_rec + 4 * (size_t)cell);
-            flag = rec.x;
-            if (flag == 0.0) continue;
-            c0 = rec.y;
-            c1 = rec.z;
-            cs = rec.w;
-        }
+        const double flag = !inside[cc] ? 0.0 : (DIFF ? recf[cc] : rec4[cc].x);
+        if (flag == 0.0) continue;
 #pragma unroll
         for (int b = 0; b < NB; ++b) {
             const int* off = DIM == 3 ? kHex8Off[b] : kQuad4Off[b];
             const int q = (cx + off[0] + 1) + 3 * (cy + off[1] + 1) + (DIM == 3 ? 9 * (cz + off[2] + 1) : 0);
             mask |= 1u << q;
         }
+    }
+    // u at the 27 neighbours, requested before the row is summed and unconditionally (a
+    // neighbour outside the stencil reads the row's own node), so that no load waits behind a
+    // branch; the mask selects the use
+    const int64_t n_nodes = (int64_t)(nx + 1) * (ny + 1) * (DIM == 3 ? nz + 1 : 1);
+    double un[NS27];
+#pragma unroll
+    for (int q = 0; q < NS27; ++q) {
+        const int dx = q % 3 - 1, dy = (q / 3) % 3 - 1, dz = DIM == 3 ? q / 9 - 1 : 0;
+        const int nbr = (mask & (1u << q)) ? node + dx + (nx + 1) * (dy + (ny + 1) * dz) : node;
+        un[q] = s.u_node[(size_t)BE * n_nodes + nbr];
+    }
+
+    double acc[NS27];
+#pragma unroll
+    for (int q = 0; q < NS27; ++q) acc[q] = 0.0;
+
+    // the (AL, a_loc; BE, .) segment of a cut cell's stored K_e: NB contiguous values in vertex
+    // order, 16-byte aligned
+    auto ke_seg = [&](int cc, double flag) -> const double2* {
+        constexpr int a_tab[8] = {6, 7, 5, 4, 2, 3, 1, 0};
+        constexpr int a_tab2[4] = {2, 3, 1, 0};
+        const int a_loc = DIM == 3 ? a_tab[cc] : a_tab2[cc];
+        const int cut = (int)(-flag) - 1;
+        return reinterpret_cast<const double2*>(s.ke_cut + (size_t)cut * NA * NA + (size_t)(AL * NB + a_loc) * NA +
+                                                BE * NB);
+    };
+
+#pragma unroll
+    for (int cc = 0; cc < NC; ++cc) {
+        const int cx = (cc & 1) ? 0 : -1;
+        const int cy = (cc & 2) ? 0 : -1;
+        const int cz = DIM == 3 ? ((cc & 4) ? 0 : -1) : 0;
+        // local vertex of the row node in this cell
+        constexpr int a_tab[8] = {6, 7, 5, 4, 2, 3, 1, 0};   // vertex with offset (-cx,-cy,-cz)
+        constexpr int a_tab2[4] = {2, 3, 1, 0};
+        const int a_loc = DIM == 3 ? a_tab[cc] : a_tab2[cc];
+        const double flag = !inside[cc] ? 0.0 : (DIFF ? recf[cc] : rec4[cc].x);
+        if (flag == 0.0) continue;
+        double c0 = 0.0, c1 = 0.0, cs = 0.0;
+        if (!DIFF) {
+            c0 = rec4[cc].y;
+            c1 = rec4[cc].z;
+            cs = rec4[cc].w;
+        }
         if (flag < 0.0) {
-            const int cut = (int)(-flag) - 1;
-            const double* ke = s.ke_cut + (size_t)cut * NA * NA + (size_t)(AL * NB + lattice_sorted_pos<DIM>(a_loc)) * NA;
+            const double2* kp = ke_seg(cc, flag);
+            double kv[NB];
+#pragma unroll
+            for (int h2 = 0; h2 < NB / 2; ++h2) {
+                const double2 t = kp[h2];
+                kv[2 * h2] = t.x;
+                kv[2 * h2 + 1] = t.y;
+            }
 #pragma unroll
             for (int b = 0; b < NB; ++b) {
                 const int* off = DIM == 3 ? kHex8Off[b] : kQuad4Off[b];
                 const int q = (cx + off[0] + 1) + 3 * (cy + off[1] + 1) + (DIM == 3 ? 9 * (cz + off[2] + 1) : 0);
-                acc[q] += ke[BE * NB + lattice_sorted_pos<DIM>(b)];
+                acc[q] += kv[b];
             }
         } else if (DIFF) {
+            // k * sum_i M[(i,a),(i,b)] from the reference-cell integrals (compile-time constants)
+            double ks[DIM];
+#pragma unroll
+            for (int ii = 0; ii < DIM; ++ii) ks[ii] = flag * sc[ii][ii];
 #pragma unroll
             for (int b = 0; b < NB; ++b) {
                 const int* off = DIM == 3 ? kHex8Off[b] : kQuad4Off[b];
                 const int q = (cx + off[0] + 1) + 3 * (cy + off[1] + 1) + (DIM == 3 ? 9 * (cz + off[2] + 1) : 0);
-                acc[q] += flag * sM[a_loc * NB + b];
+                double v = 0.0;
+#pragma unroll
+                for (int ii = 0; ii < DIM; ++ii) v += ks[ii] * kRefGrad<DIM>.v[ii][a_loc][ii][b];
+                acc[q] += v;
             }
         } else {
             // isotropic B^T D B block (AL, BE):
             //   D[AL][BE] M[(AL,a),(BE,b)] + D_s (delta_{AL,BE} sum_{i!=AL} M[(i,a),(i,b)]
             //                                      + (1 - delta_{AL,BE}) M[(BE,a),(AL,b)])
-            const double cab = AL == BE ? c0 : c1;
+            // with M = G * scale from the reference-cell integrals (compile-time constants)
+            const double cab = (AL == BE ? c0 : c1) * sc[AL][BE];
+            double css[DIM];
+#pragma unroll
+            for (int ii = 0; ii < DIM; ++ii) css[ii] = cs * (AL == BE ? sc[ii][ii] : sc[BE][AL]);
 #pragma unroll
             for (int b = 0; b < NB; ++b) {
                 const int* off = DIM == 3 ? kHex8Off[b] : kQuad4Off[b];
                 const int q = (cx + off[0] + 1) + 3 * (cy + off[1] + 1) + (DIM == 3 ? 9 * (cz + off[2] + 1) : 0);
-                double v = cab * sM[(AL * NB + a_loc) * NG + (BE * NB + b)];
+                double v = cab * kRefGrad<DIM>.v[AL][a_loc][BE][b];
                 if (AL == BE) {
-                    double t = 0.0;
 #pragma unroll
                     for (int ii = 0; ii < DIM; ++ii)
-                        if (ii != AL) t += sM[(ii * NB + a_loc) * NG + (ii * NB + b)];
-                    v += cs * t;
+                        if (ii != AL) v += css[ii] * kRefGrad<DIM>.v[ii][a_loc][ii][b];
                 } else {
-                    v += cs * sM[(BE * NB + a_loc) * NG + (AL * NB + b)];
+                    v += css[0] * kRefGrad<DIM>.v[BE][a_loc][AL][b];
                 }
                 acc[q] += v;
             }
         }
     }
 
-    const int row = AL * s.N + r;
-    const int start = s.row_ptr[row];
     const int cnt = __popc(mask);
-    if (s.row_ptr[row + 1] - start != NF * cnt) {
+    if (next - start != NF * cnt) {
         atomicOr(s.err_flag, 1);
         *r_part = 0.0;
         return;
@@ -404,22 +434,20 @@ __device__ inline void stencil_row(const DevView& s, const double* sM, int r, co
 #pragma unroll
     for (int q = 0; q < NS27; ++q) {
         if (mask & (1u << q)) {
-            const int dx = q % 3 - 1, dy = (q / 3) % 3 - 1, dz = DIM == 3 ? q / 9 - 1 : 0;
-            const int nbr = node + dx + (nx + 1) * (dy + (ny + 1) * dz);
             stage[idx++] = acc[q];
-            R += acc[q] * s.u_node[(size_t)NF * nbr + BE];
+            R += acc[q] * un[q];
         }
     }
     *r_part = R;
 }
 
 template <int DIM, int NF, int AL>
-__device__ inline void stencil_dispatch_be(const DevView& s, const double* sM, int r, const double* u,
+__device__ inline void stencil_dispatch_be(const DevView& s, const double (*sc)[3], int r, const double* u,
                                            double* stage, int base, double* rp, int be) {
     switch (be) {
-        case 0: stencil_row<DIM, NF, AL, 0>(s, sM, r, u, stage, base, rp); break;
-        case 1: if constexpr (NF > 1) stencil_row<DIM, NF, AL, 1>(s, sM, r, u, stage, base, rp); break;
-        case 2: if constexpr (NF > 2) stencil_row<DIM, NF, AL, 2>(s, sM, r, u, stage, base, rp); break;
+        case 0: stencil_row<DIM, NF, AL, 0>(s, sc, r, u, stage, base, rp); break;
+        case 1: if constexpr (NF > 1) stencil_row<DIM, NF, AL, 1>(s, sc, r, u, stage, base, rp); break;
+        case 2: if constexpr (NF > 2) stencil_row<DIM, NF, AL, 2>(s, sc, r, u, stage, base, rp); break;
         default: break;
     }
 }
@@ -429,18 +457,22 @@ __global__ __launch_bounds__(256) void k_stencil(DevView s, const double* u, dou
     constexpr int NB = DIM == 3 ? 8 : 4;
     constexpr int NG = DIM * NB;
     constexpr bool DIFF = NF == 1;
-    constexpr int NM = DIFF ? NB * NB : NG * NG;
     constexpr int NS27 = DIM == 3 ? 27 : 9;
-    __shared__ double sM[NM];
     __shared__ double stage[DIRECT ? 1 : ROWS * NF * NS27];
     __shared__ double rpart[NF * ROWS];
-    for (int idx = threadIdx.x; idx < NM; idx += blockDim.x) sM[idx] = s.m_ref[idx];
+    // scale of the reference-cell integrals: M[(i,a),(j,b)] = G[i][a][j][b] * sc[i][j]
+    double sc[3][3];
+    {
+        double vol = 1.0;
+        for (int d = 0; d < DIM; ++d) vol *= 0.5 * s.h[d];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) sc[i][j] = i < DIM && j < DIM ? 4.0 / (s.h[i] * s.h[j]) * vol : 0.0;
+    }
     const int al = blockIdx.y;
     const int r0 = blockIdx.x * ROWS;
     const int r1 = min(r0 + ROWS, s.N);
     const int base = s.row_ptr[al * s.N + r0];
     const int end = s.row_ptr[al * s.N + r1];
-    __syncthreads();
     const int be = threadIdx.x / ROWS;
     const int rr = threadIdx.x % ROWS;
     const int r = r0 + rr;
@@ -450,9 +482,9 @@ __global__ __launch_bounds__(256) void k_stencil(DevView s, const double* u, dou
         double* out = DIRECT ? vals : stage;
         const int out_base = DIRECT ? 0 : base;
         switch (al) {
-            case 0: stencil_dispatch_be<DIM, NF, 0>(s, sM, r, u, out, out_base, rp, be); break;
-            case 1: if constexpr (NF > 1) stencil_dispatch_be<DIM, NF, 1>(s, sM, r, u, out, out_base, rp, be); break;
-            case 2: if constexpr (NF > 2) stencil_dispatch_be<DIM, NF, 2>(s, sM, r, u, out, out_base, rp, be); break;
+            case 0: stencil_dispatch_be<DIM, NF, 0>(s, sc, r, u, out, out_base, rp, be); break;
+            case 1: if constexpr (NF > 1) stencil_dispatch_be<DIM, NF, 1>(s, sc, r, u, out, out_base, rp, be); break;
+            case 2: if constexpr (NF > 2) stencil_dispatch_be<DIM, NF, 2>(s, sc, r, u, out, out_base, rp, be); break;
             default: break;
         }
     }
@@ -564,7 +596,7 @@ mgpu_status launch_assemble(mgpu_set* S, const double* u, double* vals, double* 
     if (ev) HIP_TRY(hipEventRecord(ev[2], S->stream));
     {
         const int64_t n_nodes = (int64_t)(S->v.nx + 1) * (S->v.ny + 1) * (dim == 3 ? S->v.nz + 1 : 1);
-        dim3 gp((unsigned)(1 + (n_nodes + 255) / 256)), bp(256);
+        dim3 gp((unsigned)((n_nodes + 255) / 256)), bp(256);
         if (dim == 3 && nf == 1) hipLaunchKernelGGL((k_prep<3, 1>), gp, bp, 0, S->stream, S->v, u);
         else if (dim == 3 && nf == 3) hipLaunchKernelGGL((k_prep<3, 3>), gp, bp, 0, S->stream, S->v, u);
         else if (dim == 2 && nf == 1) hipLaunchKernelGGL((k_prep<2, 1>), gp, bp, 0, S->stream, S->v, u);
@@ -697,7 +729,6 @@ extern "C" mgpu_status mgpu_set_create(const mgpu_set_desc* desc, mgpu_set** out
     if ((rc = upload(S.get(), h.cut_elems.data(), h.cut_elems.size(), const_cast<int32_t**>(&v.cut_elems))) != MGPU_OK) return rc;
     if ((rc = upload(S.get(), desc->ig_ptr, (size_t)h.n_el + 1, const_cast<int32_t**>(&v.ig_ptr))) != MGPU_OK) return rc;
     if ((rc = upload(S.get(), desc->ig_xi, n_ig * nv_ig * h.dim, const_cast<double**>(&v.ig_xi))) != MGPU_OK) return rc;
-    if ((rc = upload(S.get(), h.vert_pos.data(), h.vert_pos.size(), const_cast<uint8_t**>(&v.vert_pos))) != MGPU_OK) return rc;
     if (desc->vertex_phi && h.dim == 3) {
         S->vphi_cut.resize(h.cut_elems.size() * 8);
         for (size_t c = 0; c < h.cut_elems.size(); ++c)
@@ -729,10 +760,6 @@ extern "C" mgpu_status mgpu_set_create(const mgpu_set_desc* desc, mgpu_set** out
         S->allocs.push_back(p);
         HIP_TRY(hipMemset(p, 0, sizeof(int32_t)));
         v.err_flag = static_cast<int32_t*>(p);
-        const size_t nm_ref = (size_t)(h.nf == 1 ? h.nb * h.nb : (h.dim * h.nb) * (h.dim * h.nb));
-        HIP_TRY(hipMalloc(&p, nm_ref * sizeof(double)));
-        S->allocs.push_back(p);
-        v.m_ref = static_cast<double*>(p);
         const size_t n_cells = (size_t)v.nx * v.ny * v.nz;
         HIP_TRY(hipMalloc(&p, std::max<size_t>(n_cells, 1) * (h.nf == 1 ? 1 : 4) * sizeof(double)));
         S->allocs.push_back(p);

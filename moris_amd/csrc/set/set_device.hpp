@@ -25,16 +25,53 @@ struct DevView {
     const int32_t* cut_elems;   // [n_cut]
     const int32_t* ig_ptr;      // [n_el + 1]
     const double* ig_xi;        // [n_ig][nv][dim]
-    const uint8_t* vert_pos;    // [n_el][nb]
-    double* ke_cut;             // [n_cut][na][na]
+    double* ke_cut;             // [n_cut][na][na], rows/columns (field, vertex) in vertex order
     int32_t* err_flag;          // [1]
-    double* m_ref;              // reference moments of an uncut lattice cell
     double* cell_rec;           // per lattice cell: diffusion [1] {k | -(cut+1) | 0 void},
                                 // elasticity [4] {flag, D_nn, D_nn', D_shear}
-    double* u_node;             // per lattice node: u of its adofs [n_nodes][nf] (0 if unused)
+    double* u_node;             // per lattice node: u of its adofs, planar [nf][n_nodes] (0 if unused)
     const int32_t* skip;        // [n_cut] nonzero: element integrated elsewhere (learned), or NULL
     const int32_t* cut_order;   // [n_cut] cut element processed by block b (longest first), or NULL
 };
+
+// ------------------------------------------------------------------------------------------------
+// Integrals over the reference cell [-1,1]^DIM of dN_a/dxi_i * dN_b/dxi_j for the trilinear /
+// bilinear Lagrange basis N_a = prod_d (1 + xi_d s_ad) / 2^DIM (s_ad = +-1, the vertex signs of
+// the HEX8 / QUAD4 vertex order).  The integrand factorises over the axes; per axis d the factor
+// is 2 s_ad s_bd (d = i = j), 2 s_ad (d = i != j), 2 s_bd (d = j != i) or 2 + 2/3 s_ad s_bd
+// (otherwise), times 4^-DIM overall.  HEX_2x2x2 / QUAD_2x2 integrate these products (degree <= 2
+// per variable) exactly, so on an axis-aligned cell of size h the moments of an uncut element are
+//   M[(i,a),(j,b)] = G[i][a][j][b] * 4 / (h_i h_j) * prod_d h_d / 2.
+// ------------------------------------------------------------------------------------------------
+__host__ __device__ constexpr int lagrange_vertex_sign(int dim, int a, int d) {
+    // HEX8: 0 (-,-,-) 1 (+,-,-) 2 (+,+,-) 3 (-,+,-) 4..7 the same at z = +1; QUAD4: 0..3 as z = -1
+    return d == 0 ? ((a % 4 == 1 || a % 4 == 2) ? 1 : -1)
+                  : d == 1 ? ((a % 4 == 2 || a % 4 == 3) ? 1 : -1) : (dim == 3 && a >= 4 ? 1 : -1);
+}
+
+template <int DIM>
+struct RefGradIntegrals {
+    static constexpr int NB = DIM == 3 ? 8 : 4;
+    double v[DIM][NB][DIM][NB];
+    constexpr RefGradIntegrals() : v{} {
+        for (int i = 0; i < DIM; ++i)
+            for (int a = 0; a < NB; ++a)
+                for (int j = 0; j < DIM; ++j)
+                    for (int b = 0; b < NB; ++b) {
+                        double p = 1.0;
+                        for (int d = 0; d < DIM; ++d) {
+                            const double sa = lagrange_vertex_sign(DIM, a, d), sb = lagrange_vertex_sign(DIM, b, d);
+                            if (d == i && d == j) p *= 2.0 * sa * sb;
+                            else if (d == i) p *= 2.0 * sa;
+                            else if (d == j) p *= 2.0 * sb;
+                            else p *= 2.0 + (2.0 / 3.0) * sa * sb;
+                        }
+                        v[i][a][j][b] = p / (DIM == 3 ? 64.0 : 16.0);
+                    }
+    }
+};
+template <int DIM>
+constexpr RefGradIntegrals<DIM> kRefGrad{};
 
 // Voigt index of the symmetric pair (i, j): 3D [xx,yy,zz,yz,xz,xy]
 // (CM_Struc_Linear::eval_strain_3d, cl_FEM_CM_Struc_Linear.cpp:448-453); 2D [xx,yy,xy].
