@@ -197,13 +197,15 @@ mgpu_status mgpu_memcpy_htod(void* dst, const void* src, size_t bytes);
 mgpu_status mgpu_memcpy_dtoh(void* dst, const void* src, size_t bytes);
 mgpu_status mgpu_synchronize(void);
 /* Time `iters` back-to-back mgpu_set_assemble calls with HIP events on the set's stream.
-   per_call_ms receives the mean; kernel_ms[4] the mean per-call durations, from events around
-   each stage of separate calls: [0] learned inputs + network + moment fit, [1] cut-element
-   integrands (learned element kernel and subdivision kernel), [2] cell records (k_prep),
-   [3] stencil scatter (k_stencil); 0 where a stage does not run.                               */
+   per_call_ms receives the mean; kernel_ms[5] the mean per-call durations, from events around
+   each stage of separate calls: [0] learned inputs + network (k_lq_inputs, k_mlp_forward),
+   [1] moment fit (k_lq_fit), [2] cut-element integrands (k_lq_element and the subdivision
+   kernel), [3] cell records (k_prep), [4] stencil scatter (k_stencil); 0 where a stage does not
+   run.                                                                                          */
 mgpu_status mgpu_set_time_assemble(mgpu_set* set, const double* u_adof, double* csr_vals,
                                    double* rhs, int32_t iters, double* per_call_ms,
-                                   double kernel_ms[4]);
+                                   double kernel_ms[5]);
+
 
 /* ---- launch shapes and the autotuner (no counterpart in the reference) -------------------------
    A set's launch shape is the stencil variant and the order of the cut-element workgroups

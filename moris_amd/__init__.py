@@ -331,7 +331,7 @@ class Set:
 
     def time_assemble(self, u_buf, vals_buf, rhs_buf, iters):
         per = ctypes.c_double()
-        km = (ctypes.c_double * 4)()
+        km = (ctypes.c_double * 5)()
         _check(load_library().mgpu_set_time_assemble(self._h, u_buf.ptr, vals_buf.ptr, rhs_buf.ptr, iters,
                                                      ctypes.byref(per), km), "mgpu_set_time_assemble")
         return per.value, list(km)

@@ -580,10 +580,13 @@ mgpu_status launch_assemble(mgpu_set* S, const double* u, double* vals, double* 
     if (learned) {
         lq_launch_inputs(S->lv, S->stream);
         mlp_forward_launch(S->mlp, S->lv.x_in, S->lv.n_cut, S->lv.w_pred, S->stream);
+        if (ev) HIP_TRY(hipEventRecord(ev[1], S->stream));
         lq_launch_fit(S->v, S->lv, S->stream);
         HIP_TRY(hipGetLastError());
+    } else if (ev) {
+        HIP_TRY(hipEventRecord(ev[1], S->stream));
     }
-    if (ev) HIP_TRY(hipEventRecord(ev[1], S->stream));
+    if (ev) HIP_TRY(hipEventRecord(ev[2], S->stream));
     if (S->v.n_cut > 0) {
         if (learned) {
             lq_launch_element(S->v, S->lv, nf, S->stream);
@@ -593,7 +596,7 @@ mgpu_status launch_assemble(mgpu_set* S, const double* u, double* vals, double* 
         }
         HIP_TRY(hipGetLastError());
     }
-    if (ev) HIP_TRY(hipEventRecord(ev[2], S->stream));
+    if (ev) HIP_TRY(hipEventRecord(ev[3], S->stream));
     {
         const int64_t n_nodes = (int64_t)(S->v.nx + 1) * (S->v.ny + 1) * (dim == 3 ? S->v.nz + 1 : 1);
         dim3 gp((unsigned)((n_nodes + 255) / 256)), bp(256);
@@ -602,7 +605,7 @@ mgpu_status launch_assemble(mgpu_set* S, const double* u, double* vals, double* 
         else if (dim == 2 && nf == 1) hipLaunchKernelGGL((k_prep<2, 1>), gp, bp, 0, S->stream, S->v, u);
         else hipLaunchKernelGGL((k_prep<2, 2>), gp, bp, 0, S->stream, S->v, u);
         HIP_TRY(hipGetLastError());
-        if (ev) HIP_TRY(hipEventRecord(ev[3], S->stream));
+        if (ev) HIP_TRY(hipEventRecord(ev[4], S->stream));
         const int var = S->stencil_variant < 0 ? 1 : S->stencil_variant;
         if (dim == 3 && nf == 1) launch_stencil<3, 1>(S, var, u, vals, rhs);
         else if (dim == 3 && nf == 3) launch_stencil<3, 3>(S, var, u, vals, rhs);
@@ -610,7 +613,7 @@ mgpu_status launch_assemble(mgpu_set* S, const double* u, double* vals, double* 
         else launch_stencil<2, 2>(S, var, u, vals, rhs);
         HIP_TRY(hipGetLastError());
     }
-    if (ev) HIP_TRY(hipEventRecord(ev[4], S->stream));
+    if (ev) HIP_TRY(hipEventRecord(ev[5], S->stream));
     return MGPU_OK;
 }
 
@@ -902,13 +905,13 @@ extern "C" mgpu_status mgpu_set_lq_result(const mgpu_set* set, float* w_pred, fl
 }
 
 extern "C" mgpu_status mgpu_set_time_assemble(mgpu_set* set, const double* u, double* vals, double* rhs,
-                                              int32_t iters, double* per_call_ms, double kernel_ms[4]) {
+                                              int32_t iters, double* per_call_ms, double kernel_ms[5]) {
     MGPU_GUARD_BEGIN
     if (!set || iters < 1) return set_error(MGPU_ERR_INVALID_ARG, "bad argument");
     if (set->quadrature == MGPU_QUAD_LEARNED && !set->mlp) return set_error(MGPU_ERR_STATE, "no quadrature model");
-    hipEvent_t ev[5];
+    hipEvent_t ev[6];
     for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
-    double t[4] = {0, 0, 0, 0};
+    double t[5] = {0, 0, 0, 0, 0};
     hipEvent_t t0, t1;
     HIP_TRY(hipEventCreate(&t0));
     HIP_TRY(hipEventCreate(&t1));
@@ -916,8 +919,8 @@ extern "C" mgpu_status mgpu_set_time_assemble(mgpu_set* set, const double* u, do
     for (int it = 0; it < iters; ++it) {
         mgpu_status rc = launch_assemble(set, u, vals, rhs, ev);
         if (rc != MGPU_OK) return rc;
-        HIP_TRY(hipEventSynchronize(ev[4]));
-        for (int k = 0; k < 4; ++k) {
+        HIP_TRY(hipEventSynchronize(ev[5]));
+        for (int k = 0; k < 5; ++k) {
             float a = 0.f;
             HIP_TRY(hipEventElapsedTime(&a, ev[k], ev[k + 1]));
             t[k] += a;
@@ -935,7 +938,7 @@ extern "C" mgpu_status mgpu_set_time_assemble(mgpu_set* set, const double* u, do
     HIP_TRY(hipEventElapsedTime(&total, t0, t1));
     if (per_call_ms) *per_call_ms = total / iters;
     if (kernel_ms)
-        for (int k = 0; k < 4; ++k) kernel_ms[k] = t[k] / iters;
+        for (int k = 0; k < 5; ++k) kernel_ms[k] = t[k] / iters;
     for (auto& e : ev) (void)hipEventDestroy(e);
     (void)hipEventDestroy(t0);
     (void)hipEventDestroy(t1);

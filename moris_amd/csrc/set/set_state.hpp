@@ -44,9 +44,9 @@ struct mgpu_set {
 
 namespace mgpu {
 // Enqueue one assemble call of the set on S->stream (no synchronisation, no allocation, no memset:
-// safe to capture into a graph).  ev (optional, 5 events): [0] start, [1] after the learned
-// inputs/network/moment fit, [2] after the cut-element integrands, [3] after the cell records,
-// [4] after the stencil scatter.
+// safe to capture into a graph).  ev (optional, 6 events): [0] start, [1] after the learned
+// inputs/network, [2] after the moment fit, [3] after the cut-element integrands, [4] after the
+// cell records, [5] after the stencil scatter.
 mgpu_status launch_assemble(mgpu_set* S, const double* u, double* vals, double* rhs, hipEvent_t* ev);
 // Synchronise the set's stream and turn the kernels' row-length flag into a status.
 mgpu_status check_err_flag(mgpu_set* S);
