@@ -27,6 +27,7 @@ HOST_SOURCES = [
 HIP_SOURCES = [
     "set/assemble.hip",
     "set/batch.hip",
+    "set/generic.hip",
     "learned/lq_assemble.hip",
     "learned/mlp.hip",
 ]

@@ -574,6 +574,7 @@ namespace mgpu {
 mgpu_status launch_assemble(mgpu_set* S, const double* u, double* vals, double* rhs, hipEvent_t* ev) {
     const SetHost& h = S->h;
     if (h.n_rows == 0) return MGPU_OK;
+    if (h.generic) return launch_generic(S, u, vals, rhs, ev);
     const int dim = h.dim, nf = h.nf;
     if (ev) HIP_TRY(hipEventRecord(ev[0], S->stream));
     const bool learned = S->v.n_cut > 0 && S->quadrature == MGPU_QUAD_LEARNED;
@@ -626,6 +627,8 @@ mgpu_status lq_prepare(mgpu_set* S) {
     const SetHost& h = S->h;
     const int nc = S->v.n_cut;
     if (h.dim != 3) return set_error(MGPU_ERR_UNSUPPORTED, "learned quadrature is implemented for 3D hex8 sets");
+    if (h.generic)
+        return set_error(MGPU_ERR_UNSUPPORTED, "learned quadrature is implemented for lattice sets with nodal T rows");
     if (nc > 0 && S->vphi_cut.empty())
         return set_error(MGPU_ERR_INVALID_ARG, "learned quadrature needs vertex_phi in the set descriptor");
     LqView& lv = S->lv;
@@ -685,6 +688,10 @@ int stencil_shape(int nf, int variant, int* n_variants) {
 }
 
 mgpu_status check_err_flag(mgpu_set* S) {
+    if (!S->v.err_flag) {   // generic path: no row-length check
+        HIP_TRY(hipStreamSynchronize(S->stream));
+        return MGPU_OK;
+    }
     int32_t flag = 0;
     HIP_TRY(hipMemcpyAsync(&flag, S->v.err_flag, sizeof(int32_t), hipMemcpyDeviceToHost, S->stream));
     HIP_TRY(hipStreamSynchronize(S->stream));
@@ -712,6 +719,16 @@ extern "C" mgpu_status mgpu_set_create(const mgpu_set_desc* desc, mgpu_set** out
     if (rc != MGPU_OK) return set_error(rc, err);
     SetHost& h = S->h;
     DevView& v = S->v;
+    if (h.generic) {
+        v.n_el = h.n_el;
+        v.n_cut = (int32_t)h.cut_elems.size();
+        v.N = h.n_adofs_type;
+        v.n_rows = h.n_rows;
+        S->n_ip_cut = (int64_t)desc->ig_ptr[h.n_el] * (h.dim == 3 ? kTet11Points : kTri7Points);
+        if ((rc = generic_upload(S.get(), *desc)) != MGPU_OK) return rc;
+        *out = S.release();
+        return MGPU_OK;
+    }
     v.n_el = h.n_el;
     v.n_cut = (int32_t)h.cut_elems.size();
     v.N = h.n_adofs_type;
@@ -815,6 +832,15 @@ extern "C" mgpu_status mgpu_pattern_get(const mgpu_set* set, int32_t* n_rows, in
 
 extern "C" mgpu_status mgpu_element_topology(const mgpu_set* set, int32_t* n_per_element, const int32_t** ids) {
     if (!set) return set_error(MGPU_ERR_INVALID_ARG, "null set");
+    if (set->h.generic) {
+        const auto& na_t = set->h.na_t;
+        for (size_t e = 1; e < na_t.size(); ++e)
+            if (na_t[e] != na_t[0])
+                return set_error(MGPU_ERR_UNSUPPORTED, "elements of the set have topologies of different sizes");
+        if (n_per_element) *n_per_element = na_t.empty() ? 0 : set->h.nf * na_t[0];
+        if (ids) *ids = set->h.topo.data();
+        return MGPU_OK;
+    }
     if (n_per_element) *n_per_element = set->h.na;
     if (ids) *ids = set->h.topo.data();
     return MGPU_OK;

@@ -1,0 +1,288 @@
+// generic.hip -- the generic element path of an mgpu_set: any T rows (B-spline p = 2, THB-like
+// rows with several weighted entries) and sets without a lattice.
+//
+//   k_generic_element   one wavefront per element.  The moments
+//                         M[(i,a),(j,b)] = sum_ip w* dN_a/dx_i dN_b/dx_j
+//                       over the element's integration points (HEX_2x2x2 / QUAD_2x2 on the IP box
+//                       when uncut, TET_11 / TRI_7 on the XTK sub-cells when cut; w* = w detJ,
+//                       points with detJ < -1e-6 skipped, cl_FEM_Element_Bulk.cpp:258-355) are
+//                       staged 64 points at a time in LDS; lane (a, b) owns the DIM x DIM block.
+//                       K_p = B^T D B from M (IWG compute_jacobian,
+//                       cl_FEM_IWG_Isotropic_Struc_Linear_Bulk.cpp:200, cl_FEM_IWG_Diffusion_Bulk.cpp
+//                       :280), then K_e = T^T K_p T with the element's dense T built from its rows
+//                       (Equation_Object::build_PADofMap / get_egn_obj_jacobian_and_residual,
+//                       cl_MSI_Equation_Object.cpp:426-493, :789-842), in FP64 through LDS, written
+//                       to the element's slot of the K_e pool.
+//   k_generic_gather    one thread per CSR value: the sum of its K_e entries in ascending element
+//                       order (deterministic SumIntoGlobalValues, cl_Sparse_Matrix_EpetraFECrs.cpp
+//                       :103-133).
+//   k_generic_rhs       one thread per row: R = K u (R_e = K_e u_e for these linear IWGs).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "../common/fem_tables.hpp"
+#include "set_state.hpp"
+
+namespace mgpu {
+
+namespace {
+
+template <class T>
+mgpu_status gupload(mgpu_set* S, const T* src, size_t n, T** dst) {
+    *dst = nullptr;
+    if (n == 0) return MGPU_OK;
+    void* p = nullptr;
+    HIP_TRY(hipMalloc(&p, n * sizeof(T)));
+    S->allocs.push_back(p);
+    HIP_TRY(hipMemcpy(p, src, n * sizeof(T), hipMemcpyHostToDevice));
+    *dst = static_cast<T*>(p);
+    return MGPU_OK;
+}
+
+// Integration point ip of element e: w* and the physical gradients g[i][a] of the IP cell.
+template <int DIM>
+__device__ inline double generic_point(const GenView& g, int e, bool cut, int ip, const double* hh,
+                                       double (*grad)[DIM == 3 ? 8 : 4]) {
+    constexpr int NB = DIM == 3 ? 8 : 4;
+    double xi[DIM];
+    double w;
+    if (!cut) {
+        // IG cell = IP cell: HEX_2x2x2 / QUAD_2x2, detJ = prod(h / 2)
+        for (int d = 0; d < DIM; ++d) xi[d] = DIM == 3 ? kHex2Xi[ip][d] : kQuad2Xi[ip][d];
+        double detJ = 1.0;
+        for (int d = 0; d < DIM; ++d) detJ *= 0.5 * hh[d];
+        w = (DIM == 3 ? kHex2W[ip] : kQuad2W[ip]) * detJ;
+    } else {
+        constexpr int NV = DIM + 1;
+        constexpr int NQ = DIM == 3 ? kTet11Points : kTri7Points;
+        const int sc = g.ig_ptr[e] + ip / NQ;
+        const int q = ip % NQ;
+        const double* X = g.ig_xi + (size_t)sc * NV * DIM;
+        double N[NV];
+        if constexpr (DIM == 3) {
+            N[0] = kTet11Xi[q][0];
+            N[1] = kTet11Xi[q][1];
+            N[2] = 1.0 - kTet11Xi[q][0] - kTet11Xi[q][1] - kTet11Xi[q][2];
+            N[3] = kTet11Xi[q][2];
+        } else {
+            N[0] = kTri7Xi[q][0];
+            N[1] = kTri7Xi[q][1];
+            N[2] = 1.0 - kTri7Xi[q][0] - kTri7Xi[q][1];
+        }
+        for (int d = 0; d < DIM; ++d) {
+            double v = 0.0;
+            for (int k = 0; k < NV; ++k) v += N[k] * X[k * DIM + d];
+            xi[d] = v;
+        }
+        double detJ;
+        if constexpr (DIM == 3) {
+            double J[3][3];
+            const int rr[3] = {0, 1, 3};
+            for (int r = 0; r < 3; ++r)
+                for (int d = 0; d < 3; ++d) J[r][d] = (X[rr[r] * 3 + d] - X[2 * 3 + d]) * (0.5 * hh[d]);
+            detJ = (J[0][0] * (J[1][1] * J[2][2] - J[2][1] * J[1][2]) -
+                    J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                    J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0])) / 6.0;
+        } else {
+            double J[2][2];
+            for (int r = 0; r < 2; ++r)
+                for (int d = 0; d < 2; ++d) J[r][d] = (X[r * 2 + d] - X[2 * 2 + d]) * (0.5 * hh[d]);
+            detJ = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) / 2.0;
+        }
+        w = detJ < kDetJLowerLimit ? 0.0 : (DIM == 3 ? kTet11W[q] : kTri7W[q]) * detJ;
+    }
+    lagrange_dNdxi<DIM>(xi, grad);
+    for (int i = 0; i < DIM; ++i)
+        for (int a = 0; a < NB; ++a) grad[i][a] *= 2.0 / hh[i];
+    return w;
+}
+
+constexpr int kGenLds = 24 * 24 + 8 * kGenericMaxAdofsPerType + 24 * 3 * kGenericMaxAdofsPerType;
+
+template <int DIM, int NF, typename T>
+__global__ __launch_bounds__(64) void k_generic_element(GenView g) {
+    constexpr int NB = DIM == 3 ? 8 : 4;
+    constexpr int NG = DIM * NB;
+    constexpr int NP = NF * NB;
+    constexpr int GP = NG + 1;
+    static_assert(2 * 64 * GP * sizeof(T) <= kGenLds * sizeof(double), "LDS phases");
+    __shared__ double lds[kGenLds];
+    const int e = blockIdx.x;
+    const int lane = threadIdx.x;
+    double hh[DIM];
+    for (int d = 0; d < DIM; ++d) hh[d] = g.ip_box[(size_t)e * 2 * DIM + DIM + d];
+    const bool cut = g.ig_ptr[e + 1] > g.ig_ptr[e];
+    const int npts = cut ? (g.ig_ptr[e + 1] - g.ig_ptr[e]) * (DIM == 3 ? kTet11Points : kTri7Points)
+                         : (DIM == 3 ? kHex2Points : kQuad2Points);
+
+    // phase 1: moments, lane (a, b) owns M[(i,a),(j,b)], i, j < DIM
+    T* sgw = reinterpret_cast<T*>(lds);
+    T* sg = sgw + 64 * GP;
+    const int a = (lane / NB) % NB, b = lane % NB;
+    const bool owner = lane < NB * NB;
+    T M[DIM][DIM];
+    for (int i = 0; i < DIM; ++i)
+        for (int j = 0; j < DIM; ++j) M[i][j] = T(0);
+    for (int p0 = 0; p0 < npts; p0 += 64) {
+        const int ip = p0 + lane;
+        if (ip < npts) {
+            double grad[DIM][NB];
+            const double w = generic_point<DIM>(g, e, cut, ip, hh, grad);
+            for (int i = 0; i < DIM; ++i)
+                for (int c = 0; c < NB; ++c) {
+                    sgw[lane * GP + i * NB + c] = (T)(w * grad[i][c]);
+                    sg[lane * GP + i * NB + c] = (T)grad[i][c];
+                }
+        }
+        __syncthreads();
+        const int np = min(64, npts - p0);
+        if (owner)
+            for (int p = 0; p < np; ++p)
+                for (int i = 0; i < DIM; ++i)
+                    for (int j = 0; j < DIM; ++j) M[i][j] += sgw[p * GP + i * NB + a] * sg[p * GP + j * NB + b];
+        __syncthreads();
+    }
+
+    // phase 2: K_p (NP x NP) from M and the material
+    double* Kp = lds;
+    double* Tt = Kp + NP * NP;                        // [NB][na_t]
+    double* W = Tt + NB * kGenericMaxAdofsPerType;     // [NP][NF * na_t]
+    const int na_t = g.na_t[e];
+    const int NA = NF * na_t;
+    if (owner) {
+        if (NF == 1) {
+            double v = 0.0;
+            for (int i = 0; i < DIM; ++i) v += (double)M[i][i];
+            Kp[a * NP + b] = g.material[e] * v;
+        } else {
+            constexpr int NS = DIM == 3 ? 6 : 3;
+            double D[NS][NS];
+            elastic_D<DIM>(g.material[2 * e], g.material[2 * e + 1], D);
+            for (int al = 0; al < DIM; ++al)
+                for (int be = 0; be < DIM; ++be) {
+                    double v = 0.0;
+                    for (int i = 0; i < DIM; ++i)
+                        for (int j = 0; j < DIM; ++j)
+                            if (d_nonzero<DIM>(voigt<DIM>(al, i), voigt<DIM>(be, j)))
+                                v += D[voigt<DIM>(al, i)][voigt<DIM>(be, j)] * (double)M[i][j];
+                    Kp[(al * NB + a) * NP + be * NB + b] = v;
+                }
+        }
+    }
+    // phase 3: dense element-local T (per dof type; the same rows serve every type)
+    for (int idx = lane; idx < NB * na_t; idx += 64) Tt[idx] = 0.0;
+    __syncthreads();
+    if (lane < NB)
+        for (int q = g.t_ptr[e * NB + lane]; q < g.t_ptr[e * NB + lane + 1]; ++q) Tt[lane * na_t + g.t_col[q]] += g.t_w[q];
+    __syncthreads();
+    // phase 4: W = K_p T (block diagonal T over dof types)
+    for (int idx = lane; idx < NP * NA; idx += 64) {
+        const int r = idx / NA, cy = idx % NA;
+        const int gg = cy / na_t, y = cy % na_t;
+        double v = 0.0;
+        for (int b2 = 0; b2 < NB; ++b2) v += Kp[r * NP + gg * NB + b2] * Tt[b2 * na_t + y];
+        W[idx] = v;
+    }
+    __syncthreads();
+    // phase 5: K_e = T^T W
+    double* ke = g.ke + g.ke_off[e];
+    for (int idx = lane; idx < NA * NA; idx += 64) {
+        const int X = idx / NA, Y = idx % NA;
+        const int f = X / na_t, x = X % na_t;
+        double v = 0.0;
+        for (int b1 = 0; b1 < NB; ++b1) v += Tt[b1 * na_t + x] * W[(f * NB + b1) * NA + Y];
+        ke[idx] = v;
+    }
+}
+
+__global__ void k_generic_gather(GenView g, double* vals) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= g.nnz) return;
+    double v = 0.0;
+    for (int64_t k = g.gather_ptr[p]; k < g.gather_ptr[p + 1]; ++k) v += g.ke[g.gather_src[k]];
+    vals[p] = v;
+}
+
+__global__ void k_generic_rhs(GenView g, const double* vals, const double* u, double* rhs) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= g.n_rows) return;
+    double v = 0.0;
+    for (int p = g.row_ptr[r]; p < g.row_ptr[r + 1]; ++p) v += vals[p] * u[g.col_idx[p]];
+    rhs[r] = v;
+}
+
+}  // namespace
+
+mgpu_status generic_upload(mgpu_set* S, const mgpu_set_desc& d) {
+    const SetHost& h = S->h;
+    GenView& g = S->g;
+    g.n_el = h.n_el;
+    g.N = h.n_adofs_type;
+    g.n_rows = h.n_rows;
+    g.nnz = (int64_t)h.col_idx.size();
+    const size_t n_ig = (size_t)d.ig_ptr[h.n_el];
+    const size_t n_t = (size_t)d.t_ptr[(size_t)h.n_el * h.nb];
+    mgpu_status rc;
+    if ((rc = gupload(S, d.ip_box, (size_t)h.n_el * 2 * h.dim, const_cast<double**>(&g.ip_box))) ||
+        (rc = gupload(S, d.material, (size_t)h.n_el * h.nm, const_cast<double**>(&g.material))) ||
+        (rc = gupload(S, d.ig_ptr, (size_t)h.n_el + 1, const_cast<int32_t**>(&g.ig_ptr))) ||
+        (rc = gupload(S, d.ig_xi, n_ig * (h.dim + 1) * h.dim, const_cast<double**>(&g.ig_xi))) ||
+        (rc = gupload(S, d.t_ptr, (size_t)h.n_el * h.nb + 1, const_cast<int32_t**>(&g.t_ptr))) ||
+        (rc = gupload(S, h.t_col.data(), n_t, const_cast<int32_t**>(&g.t_col))) ||
+        (rc = gupload(S, d.t_weight, n_t, const_cast<double**>(&g.t_w))) ||
+        (rc = gupload(S, h.na_t.data(), h.na_t.size(), const_cast<int32_t**>(&g.na_t))) ||
+        (rc = gupload(S, h.ke_off.data(), h.ke_off.size(), const_cast<int64_t**>(&g.ke_off))) ||
+        (rc = gupload(S, h.gather_ptr.data(), h.gather_ptr.size(), const_cast<int64_t**>(&g.gather_ptr))) ||
+        (rc = gupload(S, h.gather_src.data(), h.gather_src.size(), const_cast<int64_t**>(&g.gather_src))) ||
+        (rc = gupload(S, h.row_ptr.data(), h.row_ptr.size(), const_cast<int32_t**>(&g.row_ptr))) ||
+        (rc = gupload(S, h.col_idx.data(), h.col_idx.size(), const_cast<int32_t**>(&g.col_idx))))
+        return rc;
+    const size_t n_ke = h.ke_off.empty() ? 0 : (size_t)h.ke_off.back();
+    if (n_ke > 0) {
+        void* p = nullptr;
+        HIP_TRY(hipMalloc(&p, n_ke * sizeof(double)));
+        S->allocs.push_back(p);
+        g.ke = static_cast<double*>(p);
+    }
+    return MGPU_OK;
+}
+
+mgpu_status launch_generic(mgpu_set* S, const double* u, double* vals, double* rhs, hipEvent_t* ev) {
+    const SetHost& h = S->h;
+    if (ev)
+        for (int k = 0; k < 3; ++k) HIP_TRY(hipEventRecord(ev[k], S->stream));
+    if (h.n_el > 0) {
+        const bool f32 = S->precision == MGPU_PREC_MIXED;
+        const dim3 gr((unsigned)h.n_el), bl(64);
+        if (h.dim == 3 && h.nf == 1) {
+            if (f32) hipLaunchKernelGGL((k_generic_element<3, 1, float>), gr, bl, 0, S->stream, S->g);
+            else hipLaunchKernelGGL((k_generic_element<3, 1, double>), gr, bl, 0, S->stream, S->g);
+        } else if (h.dim == 3) {
+            if (f32) hipLaunchKernelGGL((k_generic_element<3, 3, float>), gr, bl, 0, S->stream, S->g);
+            else hipLaunchKernelGGL((k_generic_element<3, 3, double>), gr, bl, 0, S->stream, S->g);
+        } else if (h.nf == 1) {
+            if (f32) hipLaunchKernelGGL((k_generic_element<2, 1, float>), gr, bl, 0, S->stream, S->g);
+            else hipLaunchKernelGGL((k_generic_element<2, 1, double>), gr, bl, 0, S->stream, S->g);
+        } else {
+            if (f32) hipLaunchKernelGGL((k_generic_element<2, 2, float>), gr, bl, 0, S->stream, S->g);
+            else hipLaunchKernelGGL((k_generic_element<2, 2, double>), gr, bl, 0, S->stream, S->g);
+        }
+        HIP_TRY(hipGetLastError());
+    }
+    if (ev) HIP_TRY(hipEventRecord(ev[3], S->stream));
+    if (S->g.nnz > 0)
+        hipLaunchKernelGGL(k_generic_gather, dim3((unsigned)((S->g.nnz + 255) / 256)), dim3(256), 0, S->stream, S->g,
+                           vals);
+    HIP_TRY(hipGetLastError());
+    if (ev) HIP_TRY(hipEventRecord(ev[4], S->stream));
+    if (h.n_rows > 0)
+        hipLaunchKernelGGL(k_generic_rhs, dim3((unsigned)((h.n_rows + 255) / 256)), dim3(256), 0, S->stream, S->g,
+                           vals, u, rhs);
+    HIP_TRY(hipGetLastError());
+    if (ev) HIP_TRY(hipEventRecord(ev[5], S->stream));
+    return MGPU_OK;
+}
+
+}  // namespace mgpu

@@ -73,6 +73,28 @@ struct RefGradIntegrals {
 template <int DIM>
 constexpr RefGradIntegrals<DIM> kRefGrad{};
 
+// ------------------------------------------------------------------------------------------------
+// device-side view of a set on the generic element path (any T rows, no lattice needed)
+// ------------------------------------------------------------------------------------------------
+struct GenView {
+    int32_t n_el, N, n_rows;
+    int64_t nnz;
+    const double* ip_box;        // [n_el][2 dim] lower corner, extent
+    const double* material;      // [n_el][nm]
+    const int32_t* ig_ptr;       // [n_el + 1]
+    const double* ig_xi;         // [n_ig][nv][dim]
+    const int32_t* t_ptr;        // [n_el nb + 1]
+    const int32_t* t_col;        // element-local column of each T entry
+    const double* t_w;           // weight of each T entry
+    const int32_t* na_t;         // [n_el] adofs per dof type
+    const int64_t* ke_off;       // [n_el + 1]
+    double* ke;                  // element matrices, (nf na_t)^2 per element, row-major
+    const int64_t* gather_ptr;   // [nnz + 1]
+    const int64_t* gather_src;   // K_e entries summed into each CSR value
+    const int32_t* row_ptr;      // [n_rows + 1]
+    const int32_t* col_idx;      // [nnz]
+};
+
 // Voigt index of the symmetric pair (i, j): 3D [xx,yy,zz,yz,xz,xy]
 // (CM_Struc_Linear::eval_strain_3d, cl_FEM_CM_Struc_Linear.cpp:448-453); 2D [xx,yy,xy].
 template <int DIM>

@@ -35,6 +35,122 @@ void parallel_for(int64_t n, F f) {
 }
 }  // namespace
 
+// Generic path: element topology of any T rows (sorted unique adofs per type), element-local T
+// columns, the CSR pattern as the union of the topologies (Sparse_Matrix_EpetraFECrs::build_graph,
+// cl_Sparse_Matrix_EpetraFECrs.cpp:195-240) and, per CSR value, the list of element-matrix
+// entries summed into it in ascending element order (the deterministic replacement of
+// SumIntoGlobalValues, :103-133).
+mgpu_status build_generic(const mgpu_set_desc& d, SetHost& h, std::string* err) {
+    const int nb = h.nb, nf = h.nf;
+    const int32_t N = h.n_adofs_type;
+    h.na_t.assign((size_t)h.n_el, 0);
+    h.t_col.assign((size_t)d.t_ptr[(size_t)h.n_el * nb], 0);
+    // per element: sorted unique type-local adofs
+    std::vector<std::vector<int32_t>> loc((size_t)h.n_el);
+    std::atomic<bool> too_big{false};
+    parallel_for(h.n_el, [&](int64_t e) {
+        std::vector<int32_t>& u = loc[(size_t)e];
+        for (int b = 0; b < nb; ++b)
+            for (int32_t q = d.t_ptr[e * nb + b]; q < d.t_ptr[e * nb + b + 1]; ++q) u.push_back(d.t_adof[q]);
+        std::sort(u.begin(), u.end());
+        u.erase(std::unique(u.begin(), u.end()), u.end());
+        if (u.size() > (size_t)kGenericMaxAdofsPerType) too_big.store(true);
+        h.na_t[(size_t)e] = (int32_t)u.size();
+        for (int b = 0; b < nb; ++b)
+            for (int32_t q = d.t_ptr[e * nb + b]; q < d.t_ptr[e * nb + b + 1]; ++q)
+                h.t_col[(size_t)q] = (int32_t)(std::lower_bound(u.begin(), u.end(), d.t_adof[q]) - u.begin());
+    });
+    if (too_big.load()) {
+        *err = "an element has more than " + std::to_string(kGenericMaxAdofsPerType) + " adofs per dof type";
+        return MGPU_ERR_UNSUPPORTED;
+    }
+    h.topo_ptr.assign((size_t)h.n_el + 1, 0);
+    h.ke_off.assign((size_t)h.n_el + 1, 0);
+    for (int32_t e = 0; e < h.n_el; ++e) {
+        const int64_t na = (int64_t)nf * h.na_t[(size_t)e];
+        h.topo_ptr[(size_t)e + 1] = h.topo_ptr[(size_t)e] + na;
+        h.ke_off[(size_t)e + 1] = h.ke_off[(size_t)e] + na * na;
+        h.na_t_max = std::max(h.na_t_max, h.na_t[(size_t)e]);
+    }
+    h.topo.resize((size_t)h.topo_ptr[(size_t)h.n_el]);
+    parallel_for(h.n_el, [&](int64_t e) {
+        int32_t* t = h.topo.data() + h.topo_ptr[(size_t)e];
+        for (int f = 0; f < nf; ++f)
+            for (int32_t a : loc[(size_t)e]) *t++ = a + f * N;
+    });
+    // adof -> elements
+    std::vector<int64_t> a_ptr((size_t)N + 1, 0);
+    for (int32_t e = 0; e < h.n_el; ++e)
+        for (int32_t a : loc[(size_t)e]) a_ptr[(size_t)a + 1]++;
+    for (int32_t a = 0; a < N; ++a) a_ptr[(size_t)a + 1] += a_ptr[(size_t)a];
+    std::vector<int32_t> a_el((size_t)a_ptr[(size_t)N]);
+    {
+        std::vector<int64_t> fill(a_ptr.begin(), a_ptr.end() - 1);
+        for (int32_t e = 0; e < h.n_el; ++e)
+            for (int32_t a : loc[(size_t)e]) a_el[(size_t)fill[(size_t)a]++] = e;
+    }
+    // neighbours of each type-local adof
+    std::vector<std::vector<int32_t>> nbr((size_t)N);
+    parallel_for(N, [&](int64_t a) {
+        std::vector<int32_t>& v = nbr[(size_t)a];
+        for (int64_t q = a_ptr[(size_t)a]; q < a_ptr[(size_t)a + 1]; ++q) {
+            const auto& u = loc[(size_t)a_el[(size_t)q]];
+            v.insert(v.end(), u.begin(), u.end());
+        }
+        std::sort(v.begin(), v.end());
+        v.erase(std::unique(v.begin(), v.end()), v.end());
+    });
+    h.row_ptr.assign((size_t)h.n_rows + 1, 0);
+    int64_t nnz = 0;
+    for (int f = 0; f < nf; ++f)
+        for (int32_t a = 0; a < N; ++a) {
+            nnz += (int64_t)nf * (int64_t)nbr[(size_t)a].size();
+            if (nnz >= INT32_MAX) { *err = "nnz exceeds int32"; return MGPU_ERR_INVALID_ARG; }
+            h.row_ptr[(size_t)f * N + a + 1] = (int32_t)nnz;
+        }
+    h.col_idx.resize((size_t)nnz);
+    parallel_for(N, [&](int64_t a) {
+        const auto& v = nbr[(size_t)a];
+        for (int f = 0; f < nf; ++f) {
+            int32_t* dst = h.col_idx.data() + h.row_ptr[(size_t)f * N + a];
+            for (int g = 0; g < nf; ++g)
+                for (int32_t c : v) *dst++ = c + g * N;
+        }
+    });
+    // gather lists: K_e entry (x, y) of element e goes to CSR value (topo[x], topo[y])
+    const int64_t n_src = h.ke_off[(size_t)h.n_el];
+    if (n_src >= ((int64_t)1 << 40)) { *err = "element matrices too large"; return MGPU_ERR_INVALID_ARG; }
+    std::vector<int64_t> pos((size_t)n_src);
+    parallel_for(h.n_el, [&](int64_t e) {
+        const int32_t* t = h.topo.data() + h.topo_ptr[(size_t)e];
+        const int64_t na = h.topo_ptr[(size_t)e + 1] - h.topo_ptr[(size_t)e];
+        for (int64_t x = 0; x < na; ++x) {
+            const int32_t row = t[x];
+            const int32_t* c0 = h.col_idx.data() + h.row_ptr[(size_t)row];
+            const int32_t* c1 = h.col_idx.data() + h.row_ptr[(size_t)row + 1];
+            for (int64_t y = 0; y < na; ++y)
+                pos[(size_t)(h.ke_off[(size_t)e] + x * na + y)] = (int64_t)(std::lower_bound(c0, c1, t[y]) - h.col_idx.data());
+        }
+    });
+    h.gather_ptr.assign((size_t)nnz + 1, 0);
+    for (int64_t k = 0; k < n_src; ++k) h.gather_ptr[(size_t)pos[(size_t)k] + 1]++;
+    for (int64_t p = 0; p < nnz; ++p) h.gather_ptr[(size_t)p + 1] += h.gather_ptr[(size_t)p];
+    h.gather_src.resize((size_t)n_src);
+    {
+        std::vector<int64_t> fill(h.gather_ptr.begin(), h.gather_ptr.end() - 1);
+        for (int64_t k = 0; k < n_src; ++k) h.gather_src[(size_t)fill[(size_t)pos[(size_t)k]]++] = k;
+    }
+    // cut elements
+    h.elem_cut.assign((size_t)h.n_el, -1);
+    h.cut_elems.clear();
+    for (int32_t e = 0; e < h.n_el; ++e)
+        if (d.ig_ptr[e + 1] > d.ig_ptr[e]) {
+            h.elem_cut[(size_t)e] = (int32_t)h.cut_elems.size();
+            h.cut_elems.push_back(e);
+        }
+    return MGPU_OK;
+}
+
 mgpu_status build_set_host(const mgpu_set_desc& d, SetHost& h, std::string* err) {
     if (d.spatial_dim != 2 && d.spatial_dim != 3) { *err = "spatial_dim must be 2 or 3"; return MGPU_ERR_INVALID_ARG; }
     if (d.iwg != MGPU_IWG_SPATIALDIFF_BULK && d.iwg != MGPU_IWG_STRUC_LINEAR_BULK) {
@@ -80,13 +196,15 @@ mgpu_status build_set_host(const mgpu_set_desc& d, SetHost& h, std::string* err)
         if (d.ig_ptr[e + 1] < d.ig_ptr[e]) { *err = "ig_ptr not monotone"; return MGPU_ERR_INVALID_ARG; }
     if (d.ig_ptr[h.n_el] > 0 && !d.ig_xi) { *err = "ig_xi missing"; return MGPU_ERR_INVALID_ARG; }
 
-    // the kernels implement p=1 nodal T rows (one adof of weight 1 per basis); refuse the rest
+    // lattice stencil path: nodal T rows (one adof of weight 1 per basis) on a lattice; anything
+    // else (B-spline p > 1, THB rows, sets without a lattice) takes the generic element path
     bool nodal = true;
     for (int64_t r = 0; r < n_trows && nodal; ++r)
         nodal = (d.t_ptr[r + 1] - d.t_ptr[r] == 1) && d.t_weight[d.t_ptr[r]] == 1.0;
-    if (!nodal) {
-        *err = "T rows with more than one entry or weight != 1 (B-spline p>1, THB) are not implemented";
-        return MGPU_ERR_UNSUPPORTED;
+    const bool has_lattice = d.lattice.n[0] > 0 && d.elem_cell != nullptr;
+    if (!nodal || !has_lattice) {
+        h.generic = true;
+        return build_generic(d, h, err);
     }
 
     // element topology: sorted unique adofs per type; vertex -> sorted position

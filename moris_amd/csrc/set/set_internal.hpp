@@ -11,6 +11,10 @@
 
 namespace mgpu {
 
+// Largest element topology per dof type of the generic path (element-local T and the T^T K T
+// products are staged in LDS at this size).
+constexpr int kGenericMaxAdofsPerType = 64;
+
 // Host-side products of set creation (pure C++, no HIP).
 struct SetHost {
     int dim = 3;
@@ -22,7 +26,8 @@ struct SetHost {
     int32_t n_el = 0;
     int32_t n_adofs_type = 0;
     int32_t n_rows = 0;    // nf * n_adofs_type
-    std::vector<int32_t> topo;       // [n_el][na] sorted unique per type, global ids
+    std::vector<int32_t> topo;       // lattice path [n_el][na]; generic path by topo_ptr; sorted
+                                     // unique per type, global ids, types concatenated
     std::vector<int32_t> row_ptr;    // [n_rows + 1]
     std::vector<int32_t> col_idx;    // [nnz]
     // element vertex b -> position of its adof in the sorted per-type topology
@@ -37,6 +42,16 @@ struct SetHost {
     // cut elements
     std::vector<int32_t> cut_elems;  // element ids with sub-cells
     std::vector<int32_t> elem_cut;   // [n_el] cut index or -1
+    // generic path (T rows with several entries, or no lattice): element-local T and the gather
+    // lists of the CSR values
+    bool generic = false;
+    int32_t na_t_max = 0;                // largest per-type element topology
+    std::vector<int32_t> na_t;           // [n_el] adofs per dof type of each element
+    std::vector<int64_t> topo_ptr;       // [n_el + 1] into topo (nf * na_t per element)
+    std::vector<int64_t> ke_off;         // [n_el + 1] K_e offsets, (nf * na_t)^2 per element
+    std::vector<int32_t> t_col;          // [n T entries] element-local column of each T entry
+    std::vector<int64_t> gather_ptr;     // [nnz + 1]
+    std::vector<int64_t> gather_src;     // [sum of K_e sizes] K_e entry summed into each value
 };
 
 // Build topology, pattern and the lattice maps from a descriptor.  Returns MGPU_OK or an error

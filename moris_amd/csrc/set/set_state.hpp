@@ -31,6 +31,8 @@ struct mgpu_set {
     double* d_u_tmp = nullptr;
     double* d_vals_tmp = nullptr;
     double* d_rhs_tmp = nullptr;
+    // generic element path (T rows with several entries, or no lattice)
+    mgpu::GenView g{};
     // launch shape (autotuned or set by mgpu_set_tuning)
     int stencil_variant = -1;               // index into the stencil row-block sizes, -1 default
     int cut_order = 0;                      // 1: cut elements longest-first
@@ -50,6 +52,10 @@ namespace mgpu {
 mgpu_status launch_assemble(mgpu_set* S, const double* u, double* vals, double* rhs, hipEvent_t* ev);
 // Synchronise the set's stream and turn the kernels' row-length flag into a status.
 mgpu_status check_err_flag(mgpu_set* S);
+// Generic element path: device buffers of the set (at creation) and one assemble call (events as
+// launch_assemble; [1], [2] coincide with [0]).
+mgpu_status generic_upload(mgpu_set* S, const mgpu_set_desc& d);
+mgpu_status launch_generic(mgpu_set* S, const double* u, double* vals, double* rhs, hipEvent_t* ev);
 // Rows per workgroup of stencil variant `variant` for nf fields; *n_variants: number of variants.
 int stencil_shape(int nf, int variant, int* n_variants);
 }  // namespace mgpu

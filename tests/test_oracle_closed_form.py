@@ -244,3 +244,72 @@ def test_graph_is_union_of_element_topologies():
         rp.append(len(ci))
     assert np.array_equal(np.array(rp), s["row_ptr"])
     assert np.array_equal(np.array(ci), s["col_idx"])
+
+
+def _greville_coords(setup, dim, n):
+    """Coordinates of the adofs of a quadratic B-spline set (bspline_order 2): the used splines in
+    ascending spline index, each at its Greville abscissa lo + h (s - 1/2) per axis (spline s of an
+    axis starts at knot s - 2, so its middle knots are s - 1 and s)."""
+    ec = setup["elem_cell"].astype(np.int64)
+    nx, ny = n[0], n[1]
+    ijk = [ec % nx, (ec // nx) % ny, ec // (nx * ny) if dim == 3 else np.zeros_like(ec)]
+    ns = [nn + 2 for nn in n] + [1] * (3 - dim)
+    ids = []
+    for off in product(range(3), repeat=dim):
+        s = [ijk[d] + off[d] for d in range(dim)] + [np.zeros_like(ec)] * (3 - dim)
+        ids.append(s[0] + ns[0] * (s[1] + ns[1] * s[2]))
+    used = np.unique(np.concatenate(ids))
+    sx, sy, sz = used % ns[0], (used // ns[0]) % ns[1], used // (ns[0] * ns[1])
+    h = [1.0 / nn for nn in n]
+    X = np.stack([(sx - 0.5) * h[0], (sy - 0.5) * h[1]] + ([(sz - 0.5) * h[2]] if dim == 3 else []), axis=1)
+    return X
+
+
+@pytest.mark.parametrize("dim,iwg", [(3, 0), (3, 1), (2, 0), (2, 1)])
+def test_quadratic_bspline_affine_energy_on_planar_cut(dim, iwg):
+    """p = 2 B-spline T rows (dense T^T K T in the oracle): Greville coefficients reproduce an affine
+    field, whose energy on the cut box is the closed form over the half-space volume."""
+    n = 5 if dim == 3 else 9
+    nrm0 = [0.3, 0.5, 0.8][:dim]
+    d = -0.45 * sum(nrm0) / np.linalg.norm(nrm0)
+    nrm = np.asarray(nrm0) / np.linalg.norm(nrm0)
+    plane = list(nrm) + [0.0] * (3 - dim) + [d]
+    spec, keep = M.make_spec(dim, (n,) * dim, iwg=iwg, ls_kind=M.LS_PLANE, plane=plane, bspline_order=2)
+    P = O.OracleProblem(spec)
+    s = P.setup()
+    assert np.allclose(np.add.reduceat(s["t_weight"], s["t_ptr"][:-1]), 1.0)   # partition of unity
+    V = _halfspace_volume(nrm, d)
+    X = _greville_coords(s, dim, (n,) * dim)
+    N = P.n_adofs_per_type
+    assert X.shape[0] == N
+    rp, ci = s["row_ptr"], s["col_idx"]
+    E, nu = 1.0, 0.3
+    if iwg == 0:
+        g = np.array([0.7, -0.4, 0.2][:dim])
+        u = X @ g
+        vals, _ = P.assemble(u)
+        energy = u @ csr_matvec(rp, ci, vals, u)
+        assert abs(energy - g @ g * V) < 1e-11 * g @ g * V
+    else:
+        u = np.zeros(dim * N)
+        u[:N] = 0.01 * X[:, 1]          # engineering shear gamma_xy = 0.01
+        expect = 0.01 ** 2 * E / (2 * (1 + nu)) * V
+        vals, _ = P.assemble(u)
+        energy = u @ csr_matvec(rp, ci, vals, u)
+        assert abs(energy - expect) < 1e-11 * expect, (energy, expect)
+
+
+def test_quadratic_bspline_rigid_rotation_null_space():
+    n = 4
+    spec, keep = M.make_spec(3, (n, n, n), iwg=1, spheres=[[0.5, 0.5, 0.5, 0.3]], bspline_order=2)
+    P = O.OracleProblem(spec)
+    s = P.setup()
+    X = _greville_coords(s, 3, (n, n, n))
+    N = P.n_adofs_per_type
+    rp, ci = s["row_ptr"], s["col_idx"]
+    vals, _ = P.assemble(np.zeros(3 * N))
+    w = np.array([0.3, -0.2, 0.5])
+    rot = np.cross(w, X - 0.5)
+    for u in (np.concatenate([rot[:, 0], rot[:, 1], rot[:, 2]]), np.concatenate([np.ones(N), np.zeros(2 * N)])):
+        Ku = csr_matvec(rp, ci, vals, u)
+        assert np.linalg.norm(Ku) < 1e-12 * np.abs(vals).max() * np.linalg.norm(u) * np.sqrt(len(u))
