@@ -45,13 +45,14 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def stencil_bytes(spec, S, n_elements, n_cut, nf=3, nb=8):
+def stencil_bytes(spec, S, n_elements, n_cut, ke_bytes, nf=3, nb=8):
     """Algorithmic HBM bytes of one k_stencil launch: K values and R written once, u read once,
-    per-element material, the stored cut-element K_e, lattice maps and row offsets."""
+    per-element material, the stored cut-element K_e (FP32 on the mixed path, FP64 on the plain
+    path), lattice maps and row offsets."""
     na = nf * nb
     n_nodes = (spec.n[0] + 1) * (spec.n[1] + 1) * (spec.n[2] + 1)
     n_cells = spec.n[0] * spec.n[1] * spec.n[2]
-    return (8 * S.nnz + 8 * S.n_rows + 8 * S.n_rows + 16 * n_elements + 8 * na * na * n_cut
+    return (8 * S.nnz + 8 * S.n_rows + 8 * S.n_rows + 16 * n_elements + ke_bytes * na * na * n_cut
             + 4 * n_cells + 4 * n_nodes + 4 * (S.n_rows + 1))
 
 
@@ -192,7 +193,7 @@ def main():
     # [1] k_lq_fit, [2] cut-element integrands, [3] k_prep, [4] k_stencil) and its roofline
     P0, _ = problems[0]
     S0 = sets[0]
-    st_bytes = stencil_bytes(P0.spec, S0, P0.n_elements, P0.n_cut)
+    st_bytes = stencil_bytes(P0.spec, S0, P0.n_elements, P0.n_cut, ke_bytes=4)
     fl0 = L.result(S0)[2]
     n_learned0 = int(fl0.sum())
     ig = np.diff(P0.arrays()["ig_ptr"])

@@ -143,7 +143,7 @@ __global__ __launch_bounds__(64) void k_lq_fit(DevView s, LqView lv) {
 typedef float floatx4_t __attribute__((ext_vector_type(4)));
 
 // K_e of an accepted element from its 27 weights.  NF = 1 diffusion, 3 elasticity (3D hex8).
-template <int NF>
+template <int NF, typename KT>
 __global__ __launch_bounds__(64) void k_lq_element(DevView s, LqView lv) {
     constexpr int NB = 8, NG = 24, NA = NF * NB;
     const int c = blockIdx.x;
@@ -170,12 +170,12 @@ __global__ __launch_bounds__(64) void k_lq_element(DevView s, LqView lv) {
         }
     __syncthreads();
     const int e = s.cut_elems[c];
-    double* ke = s.ke_cut + (size_t)c * NA * NA;
+    KT* ke = ke_store<KT>(s) + (size_t)c * NA * NA;
     const int a = lane / NB, b = lane % NB;
     if (NF == 1) {
         double v = 0.0;
         for (int i = 0; i < 3; ++i) v += (double)Ms[(i * NB + a) * 33 + (i * NB + b)];
-        ke[a * NB + b] = s.material[e] * v;
+        ke[a * NB + b] = (KT)(s.material[e] * v);
     } else {
         double D[6][6];
         elastic_D<3>(s.material[2 * e], s.material[2 * e + 1], D);
@@ -186,7 +186,7 @@ __global__ __launch_bounds__(64) void k_lq_element(DevView s, LqView lv) {
                     for (int j = 0; j < 3; ++j)
                         if (d_nonzero<3>(voigt<3>(al, i), voigt<3>(be, j)))
                             v += D[voigt<3>(al, i)][voigt<3>(be, j)] * (double)Ms[(i * NB + a) * 33 + (j * NB + b)];
-                ke[(al * NB + a) * NA + (be * NB + b)] = v;
+                ke[(al * NB + a) * NA + (be * NB + b)] = (KT)v;
             }
     }
 }
@@ -198,10 +198,15 @@ void lq_launch_inputs(const LqView& lv, hipStream_t st) {
 void lq_launch_fit(const DevView& s, const LqView& lv, hipStream_t st) {
     if (lv.n_cut > 0) hipLaunchKernelGGL(k_lq_fit, dim3(lv.n_cut), dim3(64), 0, st, s, lv);
 }
-void lq_launch_element(const DevView& s, const LqView& lv, int nf, hipStream_t st) {
+void lq_launch_element(const DevView& s, const LqView& lv, int nf, bool f32_store, hipStream_t st) {
     if (lv.n_cut == 0) return;
-    if (nf == 1) hipLaunchKernelGGL(k_lq_element<1>, dim3(lv.n_cut), dim3(64), 0, st, s, lv);
-    else hipLaunchKernelGGL(k_lq_element<3>, dim3(lv.n_cut), dim3(64), 0, st, s, lv);
+    if (f32_store) {
+        if (nf == 1) hipLaunchKernelGGL((k_lq_element<1, float>), dim3(lv.n_cut), dim3(64), 0, st, s, lv);
+        else hipLaunchKernelGGL((k_lq_element<3, float>), dim3(lv.n_cut), dim3(64), 0, st, s, lv);
+    } else {
+        if (nf == 1) hipLaunchKernelGGL((k_lq_element<1, double>), dim3(lv.n_cut), dim3(64), 0, st, s, lv);
+        else hipLaunchKernelGGL((k_lq_element<3, double>), dim3(lv.n_cut), dim3(64), 0, st, s, lv);
+    }
 }
 
 }  // namespace mgpu

@@ -29,7 +29,7 @@ struct LqView {
 
 void lq_launch_inputs(const LqView& lv, hipStream_t st);
 void lq_launch_fit(const DevView& s, const LqView& lv, hipStream_t st);
-void lq_launch_element(const DevView& s, const LqView& lv, int nf, hipStream_t st);
+void lq_launch_element(const DevView& s, const LqView& lv, int nf, bool f32_store, hipStream_t st);
 void mlp_forward_launch(const mgpu_mlp* m, const float* x, int64_t n, float* y, hipStream_t st);
 
 }  // namespace mgpu

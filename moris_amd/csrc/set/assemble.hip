@@ -87,7 +87,7 @@ __global__ __launch_bounds__(64) void k_cut_diffusion(DevView s) {
 #pragma unroll
     for (int t = 0; t < NSYM; ++t) red[t * 65 + lane] = (double)acc[t];
     __syncthreads();
-    double* ke = s.ke_cut + (size_t)c * NB * NB;
+    T* ke = ke_store<T>(s) + (size_t)c * NB * NB;
     const double k = s.material[e];
     if (lane < NSYM) {
         double v = 0.0;
@@ -96,8 +96,8 @@ __global__ __launch_bounds__(64) void k_cut_diffusion(DevView s) {
         int a = 0, t = lane;
         while (t >= NB - a) { t -= NB - a; ++a; }
         const int b = a + t;
-        ke[a * NB + b] = k * v;
-        ke[b * NB + a] = k * v;
+        ke[a * NB + b] = (T)(k * v);
+        ke[b * NB + a] = (T)(k * v);
     }
 }
 
@@ -171,7 +171,7 @@ __global__ __launch_bounds__(64) void k_cut_elasticity(DevView s) {
     constexpr int NS = DIM == 3 ? 6 : 3;
     double D[NS][NS];
     elastic_D<DIM>(s.material[2 * e], s.material[2 * e + 1], D);
-    double* ke = s.ke_cut + (size_t)c * NA * NA;
+    T* ke = ke_store<T>(s) + (size_t)c * NA * NA;
 #pragma unroll
     for (int al = 0; al < DIM; ++al)
 #pragma unroll
@@ -183,7 +183,7 @@ __global__ __launch_bounds__(64) void k_cut_elasticity(DevView s) {
                 for (int j = 0; j < DIM; ++j)
                     if (d_nonzero<DIM>(voigt<DIM>(al, i), voigt<DIM>(be, j)))
                         v += D[voigt<DIM>(al, i)][voigt<DIM>(be, j)] * (double)M[i][j];
-            ke[(al * NB + a) * NA + (be * NB + b)] = v;
+            ke[(al * NB + a) * NA + (be * NB + b)] = (T)v;
         }
 }
 
@@ -274,7 +274,7 @@ __host__ __device__ constexpr bool stencil_direct(int variant) {
     return variant >= stencil_row_choices<NF>();
 }
 
-template <int DIM, int NF, int AL, int BE>
+template <int DIM, int NF, int AL, int BE, typename KT>
 __device__ inline void stencil_row(const DevView& s, const double (*sc)[3], int r, const double* u,
                                    double* stage, int stage_base, double* r_part) {
     constexpr int NB = DIM == 3 ? 8 : 4;
@@ -341,14 +341,32 @@ __device__ inline void stencil_row(const DevView& s, const double (*sc)[3], int 
     for (int q = 0; q < NS27; ++q) acc[q] = 0.0;
 
     // the (AL, a_loc; BE, .) segment of a cut cell's stored K_e: NB contiguous values in vertex
-    // order, 16-byte aligned
-    auto ke_seg = [&](int cc, double flag) -> const double2* {
+    // order, 16-byte aligned, read as 16-byte vectors
+    auto ke_seg = [&](int cc, double flag, double* kv) {
         constexpr int a_tab[8] = {6, 7, 5, 4, 2, 3, 1, 0};
         constexpr int a_tab2[4] = {2, 3, 1, 0};
         const int a_loc = DIM == 3 ? a_tab[cc] : a_tab2[cc];
         const int cut = (int)(-flag) - 1;
-        return reinterpret_cast<const double2*>(s.ke_cut + (size_t)cut * NA * NA + (size_t)(AL * NB + a_loc) * NA +
-                                                BE * NB);
+        const KT* base = ke_store<KT>(s) + (size_t)cut * NA * NA + (size_t)(AL * NB + a_loc) * NA + BE * NB;
+        if constexpr (sizeof(KT) == 8) {
+            const double2* p2 = reinterpret_cast<const double2*>(base);
+#pragma unroll
+            for (int h2 = 0; h2 < NB / 2; ++h2) {
+                const double2 t = p2[h2];
+                kv[2 * h2] = t.x;
+                kv[2 * h2 + 1] = t.y;
+            }
+        } else {
+            const float4* p4 = reinterpret_cast<const float4*>(base);
+#pragma unroll
+            for (int h4 = 0; h4 < NB / 4; ++h4) {
+                const float4 t = p4[h4];
+                kv[4 * h4] = t.x;
+                kv[4 * h4 + 1] = t.y;
+                kv[4 * h4 + 2] = t.z;
+                kv[4 * h4 + 3] = t.w;
+            }
+        }
     };
 
 #pragma unroll
@@ -369,14 +387,8 @@ __device__ inline void stencil_row(const DevView& s, const double (*sc)[3], int 
             cs = rec4[cc].w;
         }
         if (flag < 0.0) {
-            const double2* kp = ke_seg(cc, flag);
             double kv[NB];
-#pragma unroll
-            for (int h2 = 0; h2 < NB / 2; ++h2) {
-                const double2 t = kp[h2];
-                kv[2 * h2] = t.x;
-                kv[2 * h2 + 1] = t.y;
-            }
+            ke_seg(cc, flag, kv);
 #pragma unroll
             for (int b = 0; b < NB; ++b) {
                 const int* off = DIM == 3 ? kHex8Off[b] : kQuad4Off[b];
@@ -441,18 +453,18 @@ __device__ inline void stencil_row(const DevView& s, const double (*sc)[3], int 
     *r_part = R;
 }
 
-template <int DIM, int NF, int AL>
+template <int DIM, int NF, int AL, typename KT>
 __device__ inline void stencil_dispatch_be(const DevView& s, const double (*sc)[3], int r, const double* u,
                                            double* stage, int base, double* rp, int be) {
     switch (be) {
-        case 0: stencil_row<DIM, NF, AL, 0>(s, sc, r, u, stage, base, rp); break;
-        case 1: if constexpr (NF > 1) stencil_row<DIM, NF, AL, 1>(s, sc, r, u, stage, base, rp); break;
-        case 2: if constexpr (NF > 2) stencil_row<DIM, NF, AL, 2>(s, sc, r, u, stage, base, rp); break;
+        case 0: stencil_row<DIM, NF, AL, 0, KT>(s, sc, r, u, stage, base, rp); break;
+        case 1: if constexpr (NF > 1) stencil_row<DIM, NF, AL, 1, KT>(s, sc, r, u, stage, base, rp); break;
+        case 2: if constexpr (NF > 2) stencil_row<DIM, NF, AL, 2, KT>(s, sc, r, u, stage, base, rp); break;
         default: break;
     }
 }
 
-template <int DIM, int NF, int ROWS, bool DIRECT>
+template <int DIM, int NF, int ROWS, bool DIRECT, typename KT>
 __global__ __launch_bounds__(256) void k_stencil(DevView s, const double* u, double* vals, double* rhs) {
     constexpr int NB = DIM == 3 ? 8 : 4;
     constexpr int NG = DIM * NB;
@@ -482,9 +494,9 @@ __global__ __launch_bounds__(256) void k_stencil(DevView s, const double* u, dou
         double* out = DIRECT ? vals : stage;
         const int out_base = DIRECT ? 0 : base;
         switch (al) {
-            case 0: stencil_dispatch_be<DIM, NF, 0>(s, sc, r, u, out, out_base, rp, be); break;
-            case 1: if constexpr (NF > 1) stencil_dispatch_be<DIM, NF, 1>(s, sc, r, u, out, out_base, rp, be); break;
-            case 2: if constexpr (NF > 2) stencil_dispatch_be<DIM, NF, 2>(s, sc, r, u, out, out_base, rp, be); break;
+            case 0: stencil_dispatch_be<DIM, NF, 0, KT>(s, sc, r, u, out, out_base, rp, be); break;
+            case 1: if constexpr (NF > 1) stencil_dispatch_be<DIM, NF, 1, KT>(s, sc, r, u, out, out_base, rp, be); break;
+            case 2: if constexpr (NF > 2) stencil_dispatch_be<DIM, NF, 2, KT>(s, sc, r, u, out, out_base, rp, be); break;
             default: break;
         }
     }
@@ -550,8 +562,12 @@ template <int DIM, int NF, int V>
 void launch_stencil_variant(mgpu_set* S, const double* u, double* vals, double* rhs) {
     constexpr int ROWS = stencil_rows<NF>(V);
     const dim3 grid((unsigned)((S->v.N + ROWS - 1) / ROWS), (unsigned)NF);
-    hipLaunchKernelGGL((k_stencil<DIM, NF, ROWS, stencil_direct<NF>(V)>), grid, dim3(ROWS * NF), 0, S->stream, S->v,
-                       u, vals, rhs);
+    if (S->precision == MGPU_PREC_MIXED)
+        hipLaunchKernelGGL((k_stencil<DIM, NF, ROWS, stencil_direct<NF>(V), float>), grid, dim3(ROWS * NF), 0, S->stream,
+                           S->v, u, vals, rhs);
+    else
+        hipLaunchKernelGGL((k_stencil<DIM, NF, ROWS, stencil_direct<NF>(V), double>), grid, dim3(ROWS * NF), 0,
+                           S->stream, S->v, u, vals, rhs);
 }
 
 template <int DIM, int NF>
@@ -590,7 +606,7 @@ mgpu_status launch_assemble(mgpu_set* S, const double* u, double* vals, double* 
     if (ev) HIP_TRY(hipEventRecord(ev[2], S->stream));
     if (S->v.n_cut > 0) {
         if (learned) {
-            lq_launch_element(S->v, S->lv, nf, S->stream);
+            lq_launch_element(S->v, S->lv, nf, S->precision == MGPU_PREC_MIXED, S->stream);
             launch_subdivision(S, S->lv.flag);
         } else {
             launch_subdivision(S, nullptr);
@@ -759,6 +775,9 @@ extern "C" mgpu_status mgpu_set_create(const mgpu_set_desc* desc, mgpu_set** out
         HIP_TRY(hipMalloc(&p, (size_t)v.n_cut * h.na * h.na * sizeof(double)));
         S->allocs.push_back(p);
         v.ke_cut = static_cast<double*>(p);
+        HIP_TRY(hipMalloc(&p, (size_t)v.n_cut * h.na * h.na * sizeof(float)));
+        S->allocs.push_back(p);
+        v.ke_cut_f = static_cast<float*>(p);
         S->cut_nip.resize(v.n_cut);
         for (int32_t c = 0; c < v.n_cut; ++c) {
             const int32_t ce = h.cut_elems[c];

@@ -26,6 +26,7 @@ struct DevView {
     const int32_t* ig_ptr;      // [n_el + 1]
     const double* ig_xi;        // [n_ig][nv][dim]
     double* ke_cut;             // [n_cut][na][na], rows/columns (field, vertex) in vertex order
+    float* ke_cut_f;            // the same in FP32 (mixed precision: FP32 integrands)
     int32_t* err_flag;          // [1]
     double* cell_rec;           // per lattice cell: diffusion [1] {k | -(cut+1) | 0 void},
                                 // elasticity [4] {flag, D_nn, D_nn', D_shear}
@@ -94,6 +95,13 @@ struct GenView {
     const int32_t* row_ptr;      // [n_rows + 1]
     const int32_t* col_idx;      // [nnz]
 };
+
+// Storage of the cut-element matrices: FP64 on the plain path, FP32 with the FP32 integrands.
+template <typename KT>
+__device__ inline KT* ke_store(const DevView& s) {
+    if constexpr (sizeof(KT) == 4) return s.ke_cut_f;
+    else return s.ke_cut;
+}
 
 // Voigt index of the symmetric pair (i, j): 3D [xx,yy,zz,yz,xz,xy]
 // (CM_Struc_Linear::eval_strain_3d, cl_FEM_CM_Struc_Linear.cpp:448-453); 2D [xx,yy,xy].
