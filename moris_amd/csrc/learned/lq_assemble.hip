@@ -3,11 +3,11 @@
 // Per cut element (hex8 IP cell):
 //   k_lq_inputs      8 nodal level-set values / max |phi| -> MLP input
 //   (MLP forward)    8 -> 64 -> 64 -> 27 on BF16 MFMA (mlp.hip): a prior for the 27 weights
-//   k_lq_fit         one wavefront per element, one lane per XTK sub-tet: the 23 moments of the
+//   k_lq_moments     one wavefront per element, one lane per XTK sub-tet: the 23 moments of the
 //                    material sub-cells in FP64 (TET_11 on the sub-tet, its FP64 Jacobian taken
-//                    once for its 11 points, butterfly reduction across the wavefront), then the
-//                    moment fit
-//                    started from the MLP prior: the minimum-norm correction onto the moment
+//                    once for its 11 points, butterfly reduction across the wavefront)
+//   k_lq_project     one wavefront per 16 elements, FP64 on MFMA (v_mfma_f64_16x16x4f64): the
+//                    moment fit started from the MLP prior: the minimum-norm correction onto the moment
 //                    constraints (iters = 0), optionally alternated `iters` times with the
 //                    nonnegative orthant; the prior is first scaled to the element's volume.
 //                    The final weights are rounded to FP32 as the element kernel uses them; the
@@ -16,7 +16,7 @@
 //                    subdivision
 //   k_lq_element     one wavefront per accepted element: M = sum_q w_q g_q g_q^T on FP32 MFMA
 //                    (v_mfma_f32_16x16x4f32, FP32 accumulation) over the 27 fixed points, then
-//                    B^T D B and the projection to sorted adof order in FP64
+//                    B^T D B in FP64, stored in vertex order
 // Elements over tolerance run the subdivision kernels (assemble.hip) unchanged.
 #include <hip/hip_runtime.h>
 
@@ -35,10 +35,7 @@ __global__ void k_lq_inputs(LqView lv) {
     for (int v = 0; v < 8; ++v) lv.x_in[8 * c + v] = (float)(lv.vphi[8 * c + v] * inv);
 }
 
-__global__ __launch_bounds__(64) void k_lq_fit(DevView s, LqView lv) {
-    __shared__ double m[kLqMoments];
-    __shared__ double w[32];
-    __shared__ double r[32];
+__global__ __launch_bounds__(64) void k_lq_moments(DevView s, LqView lv) {
     const int c = s.cut_order ? s.cut_order[blockIdx.x] : (int)blockIdx.x;
     const int lane = threadIdx.x;
     const int e = s.cut_elems[c];
@@ -85,59 +82,110 @@ __global__ __launch_bounds__(64) void k_lq_fit(DevView s, LqView lv) {
     for (int off = 32; off > 0; off >>= 1)
 #pragma unroll
         for (int k = 0; k < kLqMoments; ++k) acc[k] += __shfl_xor(acc[k], off);
-    if (lane < kLqMoments) {
+    if (lane < kLqMomentsPad) {
         double v = 0.0;
 #pragma unroll
         for (int k = 0; k < kLqMoments; ++k) v = lane == k ? acc[k] : v;
-        m[lane] = v;
+        lv.mom[(size_t)c * kLqMomentsPad + lane] = v;
     }
-    if (lane < kLqPoints) w[lane] = 8.0 * (double)lv.w_pred[(size_t)c * kLqPoints + lane];
-    __syncthreads();
-    // prior scaled to the element's volume moment m[0]
-    {
-        double sw = lane < kLqPoints ? w[lane] : 0.0;
-        for (int off = 32; off > 0; off >>= 1) sw += __shfl_xor(sw, off);
-        const double m0 = m[0];
-        __syncthreads();
-        if (sw > 0.0 && m0 > 0.0 && lane < kLqPoints) w[lane] *= m0 / sw;
-        __syncthreads();
-    }
-    for (int it = 0; it <= lv.iters; ++it) {
-        if (it > 0 && lane < kLqPoints) w[lane] = fmax(w[lane], 0.0);
-        __syncthreads();
-        if (lane < kLqMoments) {
-            double v = m[lane];
-            for (int q = 0; q < kLqPoints; ++q) v -= lv.Vt[q * kLqMoments + lane] * w[q];
-            r[lane] = v;
+}
+
+typedef double doublex4_t __attribute__((ext_vector_type(4)));
+
+// Rs = Ms - V Ws for the 16 columns of the block (rows 0..23, V zero-padded to 24 x 32), on
+// v_mfma_f64_16x16x4f64: A = V (row lane % 16 of the tile, k = lane / 16), B = Ws (k = lane / 16,
+// column lane % 16); result register j of a lane holds row lane / 16 + 4 j of column lane % 16.
+__device__ inline void lq_residual(const LqView& lv, const double* Ws, const double* Ms, double* Rs, int lane) {
+    const int col = lane % 16, grp = lane / 16;
+    for (int mt = 0; mt < 2; ++mt) {
+        doublex4_t acc = {0.0, 0.0, 0.0, 0.0};
+        const int row = 16 * mt + col;
+        for (int ks = 0; ks < 8; ++ks) {
+            const int k = 4 * ks + grp;
+            const double a = (row < kLqMoments && k < kLqPoints) ? lv.Vt[k * kLqMoments + row] : 0.0;
+            const double b = Ws[k * 16 + col];
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
         }
+        for (int j = 0; j < 4; ++j) {
+            const int r = 16 * mt + grp + 4 * j;
+            if (r < kLqMomentsPad) Rs[r * 16 + col] = Ms[r * 16 + col] - acc[j];
+        }
+    }
+}
+
+// The moment fit of 16 cut elements per wavefront (columns of Ws / Ms), in FP64 on MFMA:
+// prior scaled to the volume moment, w += P (m - V w) (iters + 1 times, alternated with the
+// nonnegative orthant when iters > 0), weights rounded to FP32, residual of those, decision.
+__global__ __launch_bounds__(64) void k_lq_project(LqView lv) {
+    __shared__ double Ws[32 * 16];
+    __shared__ double Ms[kLqMomentsPad * 16];
+    __shared__ double Rs[kLqMomentsPad * 16];
+    const int lane = threadIdx.x;
+    const int col = lane % 16, grp = lane / 16;
+    const int c0 = blockIdx.x * 16;
+    for (int idx = lane; idx < kLqMomentsPad * 16; idx += 64) {
+        const int r = idx / 16, ce = c0 + idx % 16;
+        Ms[idx] = ce < lv.n_cut ? lv.mom[(size_t)ce * kLqMomentsPad + r] : 0.0;
+    }
+    for (int idx = lane; idx < 32 * 16; idx += 64) {
+        const int r = idx / 16, ce = c0 + idx % 16;
+        Ws[idx] = (ce < lv.n_cut && r < kLqPoints) ? 8.0 * (double)lv.w_pred[(size_t)ce * kLqPoints + r] : 0.0;
+    }
+    __syncthreads();
+    if (lane < 16) {
+        double sw = 0.0;
+        for (int r = 0; r < kLqPoints; ++r) sw += Ws[r * 16 + lane];
+        const double m0 = Ms[lane];
+        if (sw > 0.0 && m0 > 0.0) {
+            const double f = m0 / sw;
+            for (int r = 0; r < kLqPoints; ++r) Ws[r * 16 + lane] *= f;
+        }
+    }
+    __syncthreads();
+    for (int it = 0; it <= lv.iters; ++it) {
+        if (it > 0) {
+            for (int idx = lane; idx < 32 * 16; idx += 64) Ws[idx] = fmax(Ws[idx], 0.0);
+            __syncthreads();
+        }
+        lq_residual(lv, Ws, Ms, Rs, lane);
         __syncthreads();
-        if (lane < kLqPoints) {
-            double v = 0.0;
-            for (int k = 0; k < kLqMoments; ++k) v += lv.Pt[k * kLqPoints + lane] * r[k];
-            w[lane] += v;
+        // Ws += P Rs (P zero-padded to 32 x 24)
+        for (int mt = 0; mt < 2; ++mt) {
+            doublex4_t acc = {0.0, 0.0, 0.0, 0.0};
+            const int row = 16 * mt + col;
+            for (int ks = 0; ks < kLqMomentsPad / 4; ++ks) {
+                const int k = 4 * ks + grp;
+                const double a = (row < kLqPoints && k < kLqMoments) ? lv.Pt[k * kLqPoints + row] : 0.0;
+                const double b = Rs[k * 16 + col];
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+            }
+            for (int j = 0; j < 4; ++j) Ws[(16 * mt + grp + 4 * j) * 16 + col] += acc[j];
         }
         __syncthreads();
     }
     // the weights as the element kernel uses them (FP32), then the residual of exactly those
-    if (lane < kLqPoints) w[lane] = (double)(float)(lv.iters > 0 ? fmax(w[lane], 0.0) : w[lane]);
+    for (int idx = lane; idx < 32 * 16; idx += 64) {
+        const double v = lv.iters > 0 ? fmax(Ws[idx], 0.0) : Ws[idx];
+        Ws[idx] = (double)(float)v;
+    }
     __syncthreads();
-    double rk = 0.0, mk = 0.0, sa = lane < kLqPoints ? fabs(w[lane]) : 0.0;
-    if (lane < kLqMoments) {
-        double v = m[lane];
-        for (int q = 0; q < kLqPoints; ++q) v -= lv.Vt[q * kLqMoments + lane] * w[q];
-        rk = v * v;
-        mk = m[lane] * m[lane];
+    lq_residual(lv, Ws, Ms, Rs, lane);
+    __syncthreads();
+    const int c = c0 + lane;
+    if (lane < 16 && c < lv.n_cut) {
+        double rk = 0.0, mk = 0.0, sa = 0.0;
+        for (int r = 0; r < kLqMoments; ++r) {
+            rk += Rs[r * 16 + lane] * Rs[r * 16 + lane];
+            mk += Ms[r * 16 + lane] * Ms[r * 16 + lane];
+        }
+        for (int r = 0; r < kLqPoints; ++r) sa += fabs(Ws[r * 16 + lane]);
+        const double m0 = Ms[lane];
+        // accepted: moments reproduced to tol, and little cancellation among the weights
+        // (sum |w| <= kappa * volume bounds the FP32 rounding of the 27-point products relative to K_e)
+        const bool ok = mk > 0.0 && m0 > 0.0 && sqrt(rk / mk) <= lv.tol && sa <= lv.kappa * m0;
+        for (int r = 0; r < kLqPoints; ++r) lv.w_fit[(size_t)c * kLqPoints + r] = (float)Ws[r * 16 + lane];
+        lv.flag[c] = ok ? 1 : 0;
     }
-    for (int off = 32; off > 0; off >>= 1) {
-        rk += __shfl_xor(rk, off);
-        mk += __shfl_xor(mk, off);
-        sa += __shfl_xor(sa, off);
-    }
-    // accepted: moments reproduced to tol, and little cancellation among the weights
-    // (sum |w| <= kappa * volume bounds the FP32 rounding of the 27-point products relative to K_e)
-    const bool ok = mk > 0.0 && m[0] > 0.0 && sqrt(rk / mk) <= lv.tol && sa <= lv.kappa * m[0];
-    if (lane < kLqPoints) lv.w_fit[(size_t)c * kLqPoints + lane] = (float)w[lane];
-    if (lane == 0) lv.flag[c] = ok ? 1 : 0;
 }
 
 typedef float floatx4_t __attribute__((ext_vector_type(4)));
@@ -196,7 +244,10 @@ void lq_launch_inputs(const LqView& lv, hipStream_t st) {
     if (lv.n_cut > 0) hipLaunchKernelGGL(k_lq_inputs, dim3((lv.n_cut + 255) / 256), dim3(256), 0, st, lv);
 }
 void lq_launch_fit(const DevView& s, const LqView& lv, hipStream_t st) {
-    if (lv.n_cut > 0) hipLaunchKernelGGL(k_lq_fit, dim3(lv.n_cut), dim3(64), 0, st, s, lv);
+    if (lv.n_cut > 0) {
+        hipLaunchKernelGGL(k_lq_moments, dim3(lv.n_cut), dim3(64), 0, st, s, lv);
+        hipLaunchKernelGGL(k_lq_project, dim3((lv.n_cut + 15) / 16), dim3(64), 0, st, lv);
+    }
 }
 void lq_launch_element(const DevView& s, const LqView& lv, int nf, bool f32_store, hipStream_t st) {
     if (lv.n_cut == 0) return;

@@ -20,6 +20,7 @@ struct LqView {
     const double* vphi;             // [n_cut][8] nodal level-set values of the cut elements
     float* x_in;                    // [n_cut][8]
     float* w_pred;                  // [n_cut][27] MLP output (weights / 8)
+    double* mom;                    // [n_cut][24] moments of the material sub-cells (23 + padding)
     float* w_fit;                   // [n_cut][27] fitted parametric weights
     int32_t* flag;                  // [n_cut] 1 learned, 0 fallback
     const double* Vt;               // [27][23]  V^T

@@ -16,6 +16,7 @@ namespace mgpu {
 
 constexpr int kLqPoints = 27;       // HEX_3x3x3
 constexpr int kLqMoments = 23;      // monomials of the moment space
+constexpr int kLqMomentsPad = 24;   // stored per element (zero-padded for the 16 x 16 x 4 MFMA tiles)
 constexpr int kLqInputs = 8;        // nodal level-set values of the hex8 IP cell
 constexpr int kLqHidden = 64;
 // MLP parameter layout (FP32): W1[8][64], b1[64], W2[64][64], b2[64], W3[64][27], b3[27]

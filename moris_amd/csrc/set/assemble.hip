@@ -664,6 +664,8 @@ mgpu_status lq_prepare(mgpu_set* S) {
     lv.x_in = static_cast<float*>(p);
     if ((rc = alloc((size_t)nc * kLqPoints * sizeof(float), &p)) != MGPU_OK) return rc;
     lv.w_pred = static_cast<float*>(p);
+    if ((rc = alloc((size_t)nc * kLqMomentsPad * sizeof(double), &p)) != MGPU_OK) return rc;
+    lv.mom = static_cast<double*>(p);
     if ((rc = alloc((size_t)nc * kLqPoints * sizeof(float), &p)) != MGPU_OK) return rc;
     lv.w_fit = static_cast<float*>(p);
     if ((rc = alloc((size_t)nc * sizeof(int32_t), &p)) != MGPU_OK) return rc;
