@@ -5,7 +5,7 @@
 //   (MLP forward)    8 -> 64 -> 64 -> 27 on BF16 MFMA (mlp.hip): a prior for the 27 weights
 //   k_lq_moments     one wavefront per element, one lane per XTK sub-tet: the 23 moments of the
 //                    material sub-cells in FP64 (TET_11 on the sub-tet, its FP64 Jacobian taken
-//                    once for its 11 points, butterfly reduction across the wavefront)
+//                    once for its 11 points, two-level reduction through LDS)
 //   k_lq_project     one wavefront per 16 elements, FP64 on MFMA (v_mfma_f64_16x16x4f64): the
 //                    moment fit started from the MLP prior: the minimum-norm correction onto the moment
 //                    constraints (iters = 0), optionally alternated `iters` times with the
@@ -77,15 +77,22 @@ __global__ __launch_bounds__(64) void k_lq_moments(DevView s, LqView lv) {
             for (int k = 0; k < kLqMoments; ++k) acc[k] += wq * p[k];
         }
     }
-    // wavefront butterfly reduction of the 23 moments (no LDS buffer: keeps occupancy VGPR-bound)
+    // two-level reduction through LDS: 24 x 4 partial sums of 16 lanes, then 4 per moment
+    __shared__ double red[kLqMomentsPad * 65];
+    __shared__ double part[kLqMomentsPad * 4];
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1)
-#pragma unroll
-        for (int k = 0; k < kLqMoments; ++k) acc[k] += __shfl_xor(acc[k], off);
-    if (lane < kLqMomentsPad) {
+    for (int k = 0; k < kLqMoments; ++k) red[k * 65 + lane] = acc[k];
+    __syncthreads();
+    for (int t = lane; t < kLqMoments * 4; t += 64) {
+        const int k = t / 4, q = t % 4;
         double v = 0.0;
 #pragma unroll
-        for (int k = 0; k < kLqMoments; ++k) v = lane == k ? acc[k] : v;
+        for (int l = 0; l < 16; ++l) v += red[k * 65 + 16 * q + l];
+        part[t] = v;
+    }
+    __syncthreads();
+    if (lane < kLqMomentsPad) {
+        const double v = lane < kLqMoments ? (part[4 * lane] + part[4 * lane + 1]) + (part[4 * lane + 2] + part[4 * lane + 3]) : 0.0;
         lv.mom[(size_t)c * kLqMomentsPad + lane] = v;
     }
 }
