@@ -12,6 +12,8 @@
 //   C (16 x 16, M x N): col l%16, rows 4*(l/16) + 0..3
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -169,17 +171,63 @@ __device__ inline void load_inputs(TileLds& L, const float* x, int64_t s0, int64
     __syncthreads();
 }
 
-// forward only: y[n][27]
+// Inference workspace of one 16-sample tile (no backward buffers: small enough for high occupancy).
+struct InferLds {
+    unsigned short xs[16 * KI];
+    unsigned short h1s[16 * H];
+    unsigned short h2s[16 * H];
+    float yo[16 * NO];
+};
+
+// Forward of one tile for inference (same arithmetic as tile_forward: BF16 operands, FP32
+// accumulation, SiLU on the FP32 pre-activation).
+__device__ inline void tile_infer(const MlpDev& d, InferLds& L, int lane) {
+    const float* P = d.params;
+    for (int nt = 0; nt < H / 16; ++nt) {
+        floatx4 c = {0.f, 0.f, 0.f, 0.f};
+        c = mfma16(frag(L.xs, KI, 0, 0, lane), frag(d.w1t, KI, nt * 16, 0, lane), c);
+        const int col = nt * 16 + lane % 16;
+        for (int j = 0; j < 4; ++j) L.h1s[(4 * (lane / 16) + j) * H + col] = f2bf(silu(c[j] + P[kLqOffB1 + col]));
+    }
+    __syncthreads();
+    for (int nt = 0; nt < H / 16; ++nt) {
+        floatx4 c = {0.f, 0.f, 0.f, 0.f};
+        for (int ks = 0; ks < H / 16; ++ks)
+            c = mfma16(frag(L.h1s, H, 0, ks * 16, lane), frag(d.w2t, H, nt * 16, ks * 16, lane), c);
+        const int col = nt * 16 + lane % 16;
+        for (int j = 0; j < 4; ++j) L.h2s[(4 * (lane / 16) + j) * H + col] = f2bf(silu(c[j] + P[kLqOffB2 + col]));
+    }
+    __syncthreads();
+    for (int nt = 0; nt < NO / 16; ++nt) {
+        floatx4 c = {0.f, 0.f, 0.f, 0.f};
+        for (int ks = 0; ks < H / 16; ++ks)
+            c = mfma16(frag(L.h2s, H, 0, ks * 16, lane), frag(d.w3t, H, nt * 16, ks * 16, lane), c);
+        const int col = nt * 16 + lane % 16;
+        for (int j = 0; j < 4; ++j) L.yo[(4 * (lane / 16) + j) * NO + col] = col < kLqPoints ? c[j] + P[kLqOffB3 + col] : 0.0f;
+    }
+    __syncthreads();
+}
+
+// forward only: y[n][27]; each wavefront loops over 16-sample tiles
 __global__ __launch_bounds__(64) void k_mlp_forward(MlpDev d, const float* x, int64_t n, float* y) {
-    __shared__ TileLds L;
-    __shared__ float yo[16 * NO];
+    __shared__ InferLds L;
     const int lane = threadIdx.x;
-    const int64_t s0 = (int64_t)blockIdx.x * 16;
-    load_inputs(L, x, s0, n, lane);
-    tile_forward(d, L, lane, yo);
-    for (int idx = lane; idx < 16 * kLqPoints; idx += 64) {
-        const int s = idx / kLqPoints, o = idx % kLqPoints;
-        if (s0 + s < n) y[(s0 + s) * kLqPoints + o] = yo[s * NO + o];
+    const int64_t n_tiles = (n + 15) / 16;
+    for (int64_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
+        const int64_t s0 = tile * 16;
+        for (int idx = lane; idx < 16 * KI; idx += 64) {
+            const int sm = idx / KI, i = idx % KI;
+            float v = 0.0f;
+            if (i < kLqInputs && s0 + sm < n) v = x[(s0 + sm) * kLqInputs + i];
+            L.xs[sm * KI + i] = f2bf(v);
+        }
+        __syncthreads();
+        tile_infer(d, L, lane);
+        for (int idx = lane; idx < 16 * kLqPoints; idx += 64) {
+            const int sm = idx / kLqPoints, o = idx % kLqPoints;
+            if (s0 + sm < n) y[(s0 + sm) * kLqPoints + o] = L.yo[sm * NO + o];
+        }
+        __syncthreads();
     }
 }
 
@@ -359,8 +407,11 @@ namespace mgpu {
 // Device view of a model's weights, used by the assembly's inference kernel.
 const void* mlp_device_view(const mgpu_mlp* m) { return &m->d; }
 void mlp_forward_launch(const mgpu_mlp* m, const float* x, int64_t n, float* y, hipStream_t st) {
-    if (n > 0)
-        hipLaunchKernelGGL(k_mlp_forward, dim3((unsigned)((n + 15) / 16)), dim3(64), 0, st, m->d, x, n, y);
+    if (n > 0) {
+        const int64_t n_tiles = (n + 15) / 16;
+        const unsigned grid = (unsigned)std::min<int64_t>(n_tiles, 2048);
+        hipLaunchKernelGGL(k_mlp_forward, dim3(grid), dim3(64), 0, st, m->d, x, n, y);
+    }
 }
 }  // namespace mgpu
 
