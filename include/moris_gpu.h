@@ -196,8 +196,8 @@ mgpu_status mgpu_free(void* ptr);
 mgpu_status mgpu_memcpy_htod(void* dst, const void* src, size_t bytes);
 mgpu_status mgpu_memcpy_dtoh(void* dst, const void* src, size_t bytes);
 mgpu_status mgpu_synchronize(void);
-/* Time `iters` back-to-back mgpu_set_assemble calls with HIP events on the set's stream.
-   per_call_ms receives the mean; kernel_ms[5] the mean per-call durations, from events around
+/* Time `iters` back-to-back mgpu_set_assemble calls with HIP events on the set's stream, after
+   one untimed call.  per_call_ms receives the mean; kernel_ms[5] the mean per-call durations, from events around
    each stage of separate calls: [0] learned inputs + network (k_lq_inputs, k_mlp_forward),
    [1] moment fit (k_lq_moments, k_lq_project), [2] cut-element integrands (k_lq_element and the subdivision
    kernel), [3] cell records (k_prep), [4] stencil scatter (k_stencil); 0 where a stage does not

@@ -20,6 +20,9 @@
 // Elements over tolerance run the subdivision kernels (assemble.hip) unchanged.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <type_traits>
+
 #include "../set/set_device.hpp"
 #include "lq_view.hpp"
 #include "quad.hpp"
@@ -201,48 +204,80 @@ typedef float floatx4_t __attribute__((ext_vector_type(4)));
 template <int NF, typename KT>
 __global__ __launch_bounds__(64) void k_lq_element(DevView s, LqView lv) {
     constexpr int NB = 8, NG = 24, NA = NF * NB;
-    const int c = blockIdx.x;
-    if (lv.flag[c] == 0) return;
     __shared__ float gq[kLqPoints * NG];
     __shared__ float wq[32];
     __shared__ float Ms[32 * 33];
+    __shared__ __align__(16) KT Ks[NA * NA];
     const int lane = threadIdx.x;
+    // the 27-point gradient table once per wavefront; elements in a grid-stride loop whose flag,
+    // element id and weights are loaded one element ahead, under the current element's arithmetic
     for (int idx = lane; idx < kLqPoints * NG; idx += 64) gq[idx] = lv.g27[idx];
-    if (lane < 32) wq[lane] = lane < kLqPoints ? (float)(lv.w_fit[(size_t)c * kLqPoints + lane] * lv.detJ_ip) : 0.0f;
-    __syncthreads();
-    // M (24 x 24, padded 32 x 32) = sum_q (w_q g_q) g_q^T, K = 28 points in steps of 4
-    for (int mt = 0; mt < 2; ++mt)
-        for (int nt = 0; nt < 2; ++nt) {
-            floatx4_t acc = {0.f, 0.f, 0.f, 0.f};
-            const int mrow = mt * 16 + lane % 16, ncol = nt * 16 + lane % 16;
-            for (int ks = 0; ks < 7; ++ks) {
-                const int q = ks * 4 + lane / 16;
-                const float a = (q < kLqPoints && mrow < NG) ? wq[q] * gq[q * NG + mrow] : 0.0f;
-                const float b = (q < kLqPoints && ncol < NG) ? gq[q * NG + ncol] : 0.0f;
-                acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
-            }
-            for (int j = 0; j < 4; ++j) Ms[(mt * 16 + 4 * (lane / 16) + j) * 33 + ncol] = acc[j];
+    int c = blockIdx.x, f = 0, e = 0;
+    float w = 0.0f;
+    if (c < lv.n_cut) {
+        f = lv.flag[c];
+        e = s.cut_elems[c];
+        if (lane < kLqPoints) w = (float)(lv.w_fit[(size_t)c * kLqPoints + lane] * lv.detJ_ip);
+    }
+    while (c < lv.n_cut) {
+        const double m0 = s.material[NF == 1 ? e : 2 * e];
+        const double m1 = NF == 1 ? 0.0 : s.material[2 * e + 1];
+        const int cn = c + (int)gridDim.x;
+        int fn = 0, en = 0;
+        float wn = 0.0f;
+        if (cn < lv.n_cut) {
+            fn = lv.flag[cn];
+            en = s.cut_elems[cn];
+            if (lane < kLqPoints) wn = (float)(lv.w_fit[(size_t)cn * kLqPoints + lane] * lv.detJ_ip);
         }
-    __syncthreads();
-    const int e = s.cut_elems[c];
-    KT* ke = ke_store<KT>(s) + (size_t)c * NA * NA;
-    const int a = lane / NB, b = lane % NB;
-    if (NF == 1) {
-        double v = 0.0;
-        for (int i = 0; i < 3; ++i) v += (double)Ms[(i * NB + a) * 33 + (i * NB + b)];
-        ke[a * NB + b] = (KT)(s.material[e] * v);
-    } else {
-        double D[6][6];
-        elastic_D<3>(s.material[2 * e], s.material[2 * e + 1], D);
-        for (int al = 0; al < 3; ++al)
-            for (int be = 0; be < 3; ++be) {
+        if (f != 0) {   // uniform across the wavefront
+            __syncthreads();
+            if (lane < 32) wq[lane] = w;
+            __syncthreads();
+            // M (24 x 24, padded 32 x 32) = sum_q (w_q g_q) g_q^T, K = 28 points in steps of 4
+            for (int mt = 0; mt < 2; ++mt)
+                for (int nt = 0; nt < 2; ++nt) {
+                    floatx4_t acc = {0.f, 0.f, 0.f, 0.f};
+                    const int mrow = mt * 16 + lane % 16, ncol = nt * 16 + lane % 16;
+                    for (int ks = 0; ks < 7; ++ks) {
+                        const int q = ks * 4 + lane / 16;
+                        const float a = (q < kLqPoints && mrow < NG) ? wq[q] * gq[q * NG + mrow] : 0.0f;
+                        const float b = (q < kLqPoints && ncol < NG) ? gq[q * NG + ncol] : 0.0f;
+                        acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
+                    }
+                    for (int j = 0; j < 4; ++j) Ms[(mt * 16 + 4 * (lane / 16) + j) * 33 + ncol] = acc[j];
+                }
+            __syncthreads();
+            const int a = lane / NB, b = lane % NB;
+            if (NF == 1) {
                 double v = 0.0;
-                for (int i = 0; i < 3; ++i)
-                    for (int j = 0; j < 3; ++j)
-                        if (d_nonzero<3>(voigt<3>(al, i), voigt<3>(be, j)))
-                            v += D[voigt<3>(al, i)][voigt<3>(be, j)] * (double)Ms[(i * NB + a) * 33 + (j * NB + b)];
-                ke[(al * NB + a) * NA + (be * NB + b)] = (KT)v;
+                for (int i = 0; i < 3; ++i) v += (double)Ms[(i * NB + a) * 33 + (i * NB + b)];
+                Ks[a * NB + b] = (KT)(m0 * v);
+            } else {
+                double D[6][6];
+                elastic_D<3>(m0, m1, D);
+                for (int al = 0; al < 3; ++al)
+                    for (int be = 0; be < 3; ++be) {
+                        double v = 0.0;
+                        for (int i = 0; i < 3; ++i)
+                            for (int j = 0; j < 3; ++j)
+                                if (d_nonzero<3>(voigt<3>(al, i), voigt<3>(be, j)))
+                                    v += D[voigt<3>(al, i)][voigt<3>(be, j)] * (double)Ms[(i * NB + a) * 33 + (j * NB + b)];
+                        Ks[(al * NB + a) * NA + (be * NB + b)] = (KT)v;
+                    }
             }
+            __syncthreads();
+            // the element matrix leaves in 16-byte vectors (NA * NA * sizeof(KT) is a multiple of 16)
+            using V4 = typename std::conditional<sizeof(KT) == 4, float4, double2>::type;
+            constexpr int NV = NA * NA * (int)sizeof(KT) / 16;
+            const V4* src = reinterpret_cast<const V4*>(Ks);
+            V4* dst = reinterpret_cast<V4*>(ke_store<KT>(s) + (size_t)c * NA * NA);
+            for (int t = lane; t < NV; t += 64) dst[t] = src[t];
+        }
+        c = cn;
+        f = fn;
+        e = en;
+        w = wn;
     }
 }
 
@@ -258,12 +293,13 @@ void lq_launch_fit(const DevView& s, const LqView& lv, hipStream_t st) {
 }
 void lq_launch_element(const DevView& s, const LqView& lv, int nf, bool f32_store, hipStream_t st) {
     if (lv.n_cut == 0) return;
+    const dim3 grid((unsigned)std::min(lv.n_cut, 4096));
     if (f32_store) {
-        if (nf == 1) hipLaunchKernelGGL((k_lq_element<1, float>), dim3(lv.n_cut), dim3(64), 0, st, s, lv);
-        else hipLaunchKernelGGL((k_lq_element<3, float>), dim3(lv.n_cut), dim3(64), 0, st, s, lv);
+        if (nf == 1) hipLaunchKernelGGL((k_lq_element<1, float>), grid, dim3(64), 0, st, s, lv);
+        else hipLaunchKernelGGL((k_lq_element<3, float>), grid, dim3(64), 0, st, s, lv);
     } else {
-        if (nf == 1) hipLaunchKernelGGL((k_lq_element<1, double>), dim3(lv.n_cut), dim3(64), 0, st, s, lv);
-        else hipLaunchKernelGGL((k_lq_element<3, double>), dim3(lv.n_cut), dim3(64), 0, st, s, lv);
+        if (nf == 1) hipLaunchKernelGGL((k_lq_element<1, double>), grid, dim3(64), 0, st, s, lv);
+        else hipLaunchKernelGGL((k_lq_element<3, double>), grid, dim3(64), 0, st, s, lv);
     }
 }
 
