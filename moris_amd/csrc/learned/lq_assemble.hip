@@ -3,9 +3,10 @@
 // Per cut element (hex8 IP cell):
 //   k_lq_inputs      8 nodal level-set values / max |phi| -> MLP input
 //   (MLP forward)    8 -> 64 -> 64 -> 27 on BF16 MFMA (mlp.hip): a prior for the 27 weights
-//   k_lq_moments     one wavefront per element, one lane per XTK sub-tet: the 23 moments of the
-//                    material sub-cells in FP64 (TET_11 on the sub-tet, its FP64 Jacobian taken
-//                    once for its 11 points, two-level reduction through LDS)
+//   k_lq_moments     one lane per XTK sub-tet, elements packed into wavefronts by 16-lane
+//                    quarters (a 48-lane element beside a 16-lane one, two 32-lane ones, ...):
+//                    the 23 moments of the material sub-cells in FP64 (TET_11 on the sub-tet,
+//                    its FP64 Jacobian taken once for its 11 points, reduction through LDS)
 //   k_lq_project     one wavefront per 16 elements, FP64 on MFMA (v_mfma_f64_16x16x4f64): the
 //                    moment fit started from the MLP prior: the minimum-norm correction onto the moment
 //                    constraints (iters = 0), optionally alternated `iters` times with the
@@ -39,15 +40,31 @@ __global__ void k_lq_inputs(LqView lv) {
 }
 
 __global__ __launch_bounds__(64) void k_lq_moments(DevView s, LqView lv) {
-    const int c = s.cut_order ? s.cut_order[blockIdx.x] : (int)blockIdx.x;
+    // the wavefront's elements: each owns a run of 16-lane quarters (a slot) and loops over its
+    // sub-tets with the slot's width as stride
     const int lane = threadIdx.x;
-    const int e = s.cut_elems[c];
-    const int ig0 = s.ig_ptr[e];
+    int own[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) own[k] = lv.mom_owner[4 * blockIdx.x + k];
+    // (selects, not indexing, so that own[] stays in registers)
+    auto own_at = [&](int k) { return k == 0 ? own[0] : (k == 1 ? own[1] : (k == 2 ? own[2] : own[3])); };
+    const int quarter = lane >> 4;
+    const int c = own_at(quarter);
+    int q0 = 0, q1 = 3;   // the slot of the lane's quarter: last start <= quarter, first end >= quarter
+#pragma unroll
+    for (int k = 1; k < 4; ++k)
+        if (k <= quarter && own[k] != own[k - 1]) q0 = k;
+#pragma unroll
+    for (int k = 2; k >= 0; --k)
+        if (k >= quarter && own[k] != own[k + 1]) q1 = k;
+    const int width = 16 * (q1 - q0 + 1);
     double acc[kLqMoments];
 #pragma unroll
     for (int k = 0; k < kLqMoments; ++k) acc[k] = 0.0;
-    const int nsub = s.ig_ptr[e + 1] - ig0;
-    for (int t = lane; t < nsub; t += 64) {
+    const int e = c >= 0 ? s.cut_elems[c] : 0;
+    const int ig0 = s.ig_ptr[e];
+    const int nsub = c >= 0 ? s.ig_ptr[e + 1] - ig0 : 0;
+    for (int t = lane - 16 * q0; t < nsub; t += width) {
         const double2* src2 = reinterpret_cast<const double2*>(s.ig_xi + (size_t)(ig0 + t) * 12);
         double X[12];
 #pragma unroll
@@ -80,23 +97,35 @@ __global__ __launch_bounds__(64) void k_lq_moments(DevView s, LqView lv) {
             for (int k = 0; k < kLqMoments; ++k) acc[k] += wq * p[k];
         }
     }
-    // two-level reduction through LDS: 24 x 4 partial sums of 16 lanes, then 4 per moment
-    __shared__ double red[kLqMomentsPad * 65];
+    // two-level reduction through LDS in three chunks of 8 moments (4 KB, so that LDS does not
+    // bound the occupancy): 8 x 4 partial sums of 16 lanes, then 4 per moment
+    __shared__ double red[8 * 65];
     __shared__ double part[kLqMomentsPad * 4];
 #pragma unroll
-    for (int k = 0; k < kLqMoments; ++k) red[k * 65 + lane] = acc[k];
-    __syncthreads();
-    for (int t = lane; t < kLqMoments * 4; t += 64) {
-        const int k = t / 4, q = t % 4;
-        double v = 0.0;
+    for (int ch = 0; ch < 3; ++ch) {
+        if (ch > 0) __syncthreads();
 #pragma unroll
-        for (int l = 0; l < 16; ++l) v += red[k * 65 + 16 * q + l];
-        part[t] = v;
+        for (int k = 0; k < 8; ++k) red[k * 65 + lane] = (8 * ch + k < kLqMoments) ? acc[8 * ch + k] : 0.0;
+        __syncthreads();
+        if (lane < 32) {
+            const int k = lane / 4, q = lane % 4;
+            double v = 0.0;
+#pragma unroll
+            for (int l = 0; l < 16; ++l) v += red[k * 65 + 16 * q + l];
+            part[(8 * ch + k) * 4 + q] = v;
+        }
     }
     __syncthreads();
-    if (lane < kLqMomentsPad) {
-        const double v = lane < kLqMoments ? (part[4 * lane] + part[4 * lane + 1]) + (part[4 * lane + 2] + part[4 * lane + 3]) : 0.0;
-        lv.mom[(size_t)c * kLqMomentsPad + lane] = v;
+    // each slot sums its quarters in order: (slot start quarter, moment) pairs over the lanes
+    for (int t = lane; t < 4 * kLqMomentsPad; t += 64) {
+        const int qs = t / kLqMomentsPad, k = t % kLqMomentsPad;
+        const int cs = own_at(qs);
+        if (cs < 0 || (qs > 0 && own_at(qs - 1) == cs)) continue;
+        double v = 0.0;
+#pragma unroll
+        for (int qq = 0; qq < 4; ++qq)
+            if (qq >= qs && own[qq] == cs) v += part[k * 4 + qq];
+        lv.mom[(size_t)cs * kLqMomentsPad + k] = k < kLqMoments ? v : 0.0;
     }
 }
 
@@ -287,7 +316,7 @@ void lq_launch_inputs(const LqView& lv, hipStream_t st) {
 }
 void lq_launch_fit(const DevView& s, const LqView& lv, hipStream_t st) {
     if (lv.n_cut > 0) {
-        hipLaunchKernelGGL(k_lq_moments, dim3(lv.n_cut), dim3(64), 0, st, s, lv);
+        hipLaunchKernelGGL(k_lq_moments, dim3(lv.n_mom_waves), dim3(64), 0, st, s, lv);
         hipLaunchKernelGGL(k_lq_project, dim3((lv.n_cut + 15) / 16), dim3(64), 0, st, lv);
     }
 }

@@ -670,6 +670,36 @@ mgpu_status lq_prepare(mgpu_set* S) {
     lv.w_fit = static_cast<float*>(p);
     if ((rc = alloc((size_t)nc * sizeof(int32_t), &p)) != MGPU_OK) return rc;
     lv.flag = static_cast<int32_t*>(p);
+    // work list of the moment kernel: each cut element takes ceil(sub-tets / 16) 16-lane quarters
+    // of a wavefront (at most 4; a 4-quarter element loops); 4-quarter elements go alone,
+    // 3-quarter ones beside a 1-quarter one, 2-quarter ones in pairs, 1-quarter ones in fours
+    {
+        std::vector<int32_t> byq[5];
+        for (int32_t c = 0; c < nc; ++c) {
+            const int nsub = S->cut_nip[c] / kTet11Points;
+            byq[std::min(4, std::max(1, (nsub + 15) / 16))].push_back(c);
+        }
+        std::vector<int32_t> owner;
+        size_t i1 = 0;
+        auto take1 = [&]() { return i1 < byq[1].size() ? byq[1][i1++] : -1; };
+        for (const int32_t c : byq[4]) owner.insert(owner.end(), {c, c, c, c});
+        for (const int32_t c : byq[3]) owner.insert(owner.end(), {c, c, c, take1()});
+        for (size_t k = 0; k < byq[2].size(); k += 2) {
+            const int32_t a = byq[2][k];
+            if (k + 1 < byq[2].size()) {
+                owner.insert(owner.end(), {a, a, byq[2][k + 1], byq[2][k + 1]});
+            } else {
+                const int32_t x = take1(), y = take1();
+                owner.insert(owner.end(), {a, a, x, y});
+            }
+        }
+        while (i1 < byq[1].size()) {
+            const int32_t a = take1(), b = take1(), c = take1(), d = take1();
+            owner.insert(owner.end(), {a, b, c, d});
+        }
+        lv.n_mom_waves = (int32_t)(owner.size() / 4);
+        if ((rc = upload(S, owner.data(), owner.size(), const_cast<int32_t**>(&lv.mom_owner))) != MGPU_OK) return rc;
+    }
     LqOperators op;
     lq_build_operators(&op);
     std::vector<double> Vt(kLqPoints * kLqMoments), Pt(kLqMoments * kLqPoints);
