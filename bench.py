@@ -45,6 +45,12 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+# HBM bytes per launch of k_stencil<3,3,64,false,float> on variant 0 of the 64^3 workload, measured
+# with two rocprofv3 --pmc runs (FETCH_SIZE scaled x2.0, WRITE_SIZE x1.0 by the copy-kernel
+# calibration): 177,003 KiB x 2 + 430,946 KiB.
+STENCIL_TRAFFIC_BYTES = (177003 * 2 + 430946) * 1024
+
+
 def stencil_bytes(spec, S, n_elements, n_cut, ke_bytes, nf=3, nb=8):
     """Algorithmic HBM bytes of one k_stencil launch: K values and R written once, u read once,
     per-element material, the stored cut-element K_e (FP32 on the mixed path, FP64 on the plain
@@ -220,7 +226,8 @@ def main():
     if dom == "k_stencil":
         gbs = kernels["k_stencil"]["GB/s"]
         roof = {"bound": "hbm", "kernel": dom, "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": gbs / HBM_PEAK_GBS, "traffic": None}
+                "frac": gbs / HBM_PEAK_GBS, "traffic": STENCIL_TRAFFIC_BYTES if args.n == 64 else None,
+                "traffic_source": "rocprofv3 --pmc FETCH_SIZE (x2.0) + WRITE_SIZE (x1.0) per launch, variant 0"}
     elif dom == "k_lq_moments+k_lq_project":
         tf = kernels[dom]["TFLOP/s"]
         roof = {"bound": "mfma", "kernel": dom, "achieved": tf, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",

@@ -49,6 +49,9 @@ typedef int32_t mgpu_status;
 #define MGPU_GEOM_HEX8   1
 #define MGPU_GEOM_TRI3   2
 #define MGPU_GEOM_TET4   3
+#define MGPU_GEOM_HEX27  4   /* quadratic Lagrange hex (cl_MTK_Interpolation_Function_Lagrange_Hex27.hpp
+                                node order); IP cell of the generic path, HEX_3x3x3 uncut, TET_35 on
+                                its TET4 sub-cells (cl_FEM_Set.cpp:3048,3091)                      */
 
 /* Arithmetic of the element integrands. */
 #define MGPU_PREC_FP64   0   /* plain path: everything FP64                                        */
@@ -88,9 +91,12 @@ typedef struct mgpu_problem_spec {
     double  k_mu, k_sigma;     /* diffusion: k_e = exp(k_mu + k_sigma * N(0,1)) per element     */
     uint64_t seed;             /* splitmix64 seed of the per-element conductivity                */
     int32_t bspline_order;     /* 1 (or 0): T rows of weight 1 per Lagrange node (p = 1);
-                                  2: quadratic B-splines of the lattice evaluated at the linear
-                                  Lagrange nodes, 2^dim entries of weight 1/2^dim per row
-                                  (HMR T-matrix at level 0, cl_HMR_T_Matrix_Base.cpp:306-457)  */
+                                  2: quadratic B-splines of the lattice evaluated at the Lagrange
+                                  nodes (HMR T-matrix at level 0, cl_HMR_T_Matrix_Base.cpp:306-457):
+                                  2 splines of weight 1/2 per axis at a lattice node, 3 of weights
+                                  1/8, 3/4, 1/8 at a cell-centre coordinate                     */
+    int32_t lagrange_order;    /* 1 (or 0): HEX8 / QUAD4 IP cells; 2: HEX27 IP cells (3D only),
+                                  whose 27 nodes sit on the half-step grid of the lattice        */
 } mgpu_problem_spec;
 
 /* Uniform lattice the elements of a set sit on (HMR level 0).  n[0] == 0: no lattice. */
@@ -108,10 +114,10 @@ typedef struct mgpu_lattice {
 typedef struct mgpu_set_desc {
     int32_t spatial_dim;       /* 2 | 3                                                          */
     int32_t iwg;               /* MGPU_IWG_*                                                     */
-    int32_t ip_geometry;       /* MGPU_GEOM_QUAD4 | MGPU_GEOM_HEX8 (IP cell, Lagrange linear)   */
+    int32_t ip_geometry;       /* MGPU_GEOM_QUAD4 | MGPU_GEOM_HEX8 | MGPU_GEOM_HEX27 (IP cell)    */
     int32_t ig_geometry;       /* MGPU_GEOM_TRI3 | MGPU_GEOM_TET4 (cut sub-cells)              */
     int32_t n_elements;        /* equation objects                                               */
-    int32_t n_bases;           /* IP-cell bases per field (4 | 8)                                */
+    int32_t n_bases;           /* IP-cell bases per field (4 | 8 | 27)                           */
     int32_t n_adofs_per_type;  /* adofs of one dof type; total adofs = n_fields * this           */
 
     /* IP cell: axis-aligned box, [n_elements][2*dim] = lower corner then extent.                */
@@ -127,17 +133,18 @@ typedef struct mgpu_set_desc {
     const double*  t_weight;
 
     /* Cut sub-cells (IG cells of the element's material subphase).  ig_ptr[e] == ig_ptr[e+1]
-       means uncut: the IP cell itself is integrated with the auto rule (QUAD_2x2 / HEX_2x2x2).
+       means uncut: the IP cell itself is integrated with the auto rule (QUAD_2x2 / HEX_2x2x2 /
+       HEX_3x3x3 for HEX27); cut sub-cells take TRI_7 / TET_11, or TET_35 under HEX27.
        ig_xi holds every IG vertex's parametric coordinate wrt the IP cell,
        [n_ig][n_vertices(ig_geometry)][spatial_dim]
        (Cluster::get_primary_cell_local_coords_on_side_wrt_interp_cell).                      */
     const int32_t* ig_ptr;     /* [n_elements + 1]                                               */
     const double*  ig_xi;
 
-    /* Lattice hint: elem_cell[e] = lattice cell (i + n0*(j + n1*k)).  The implemented path is the
-       owner-computes stencil on a lattice with nodal T rows (one entry, weight 1, p = 1);
-       mgpu_set_create rejects a set without a lattice, or with T rows of more than one entry
-       (p = 2 / THB), with MGPU_ERR_UNSUPPORTED.                                                */
+    /* Lattice hint: elem_cell[e] = lattice cell (i + n0*(j + n1*k)).  HEX8 / QUAD4 sets on a
+       lattice with nodal T rows (one entry of weight 1 per basis) take the owner-computes stencil
+       path; every other set (no lattice, T rows of several entries such as p = 2 B-splines, HEX27
+       IP cells) takes the generic element path (K_e per element, deterministic gather).        */
     mgpu_lattice   lattice;
     const int32_t* elem_cell;  /* [n_elements] or NULL                                          */
 

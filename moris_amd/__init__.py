@@ -23,7 +23,7 @@ LIB_PATH = os.path.join(_HERE, "lib", "libmoris_gpu.so")
 MGPU_OK = 0
 IWG_SPATIALDIFF_BULK = 0
 IWG_STRUC_LINEAR_BULK = 1
-GEOM_QUAD4, GEOM_HEX8, GEOM_TRI3, GEOM_TET4 = 0, 1, 2, 3
+GEOM_QUAD4, GEOM_HEX8, GEOM_TRI3, GEOM_TET4, GEOM_HEX27 = 0, 1, 2, 3, 4
 PREC_FP64, PREC_MIXED = 0, 1
 QUAD_SUBDIVISION, QUAD_LEARNED = 0, 1
 LS_NONE, LS_SPHERES, LS_GYROID, LS_PLANE = 0, 1, 2, 3
@@ -51,6 +51,7 @@ class ProblemSpec(ctypes.Structure):
         ("k_sigma", ctypes.c_double),
         ("seed", ctypes.c_uint64),
         ("bspline_order", ctypes.c_int32),
+        ("lagrange_order", ctypes.c_int32),
     ]
 
 
@@ -157,7 +158,7 @@ def _ptr(a, ctype):
 
 def make_spec(dim, n, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), iwg=IWG_SPATIALDIFF_BULK, spheres=None,
               ls_kind=None, gyroid_scale=1.0, plane=(0.0, 0.0, 0.0, 0.0), material_phase=1, E=1.0, nu=0.3,
-              k_mu=0.0, k_sigma=0.0, seed=20261202, bspline_order=1):
+              k_mu=0.0, k_sigma=0.0, seed=20261202, bspline_order=1, lagrange_order=1):
     """Build a ProblemSpec; returns (spec, keepalive) -- keep the second value alive with the spec."""
     s = ProblemSpec()
     s.dim = dim
@@ -180,6 +181,7 @@ def make_spec(dim, n, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), iwg=IWG_SPATIALDIF
     s.E, s.nu, s.k_mu, s.k_sigma = E, nu, k_mu, k_sigma
     s.seed = seed
     s.bspline_order = bspline_order
+    s.lagrange_order = lagrange_order
     return s, keep
 
 

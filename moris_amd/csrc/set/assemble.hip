@@ -772,7 +772,7 @@ extern "C" mgpu_status mgpu_set_create(const mgpu_set_desc* desc, mgpu_set** out
         v.n_cut = (int32_t)h.cut_elems.size();
         v.N = h.n_adofs_type;
         v.n_rows = h.n_rows;
-        S->n_ip_cut = (int64_t)desc->ig_ptr[h.n_el] * (h.dim == 3 ? kTet11Points : kTri7Points);
+        S->n_ip_cut = (int64_t)desc->ig_ptr[h.n_el] * (h.nb == 27 ? kTet35Points : (h.dim == 3 ? kTet11Points : kTri7Points));
         if ((rc = generic_upload(S.get(), *desc)) != MGPU_OK) return rc;
         *out = S.release();
         return MGPU_OK;
@@ -908,7 +908,8 @@ extern "C" mgpu_status mgpu_set_assemble(mgpu_set* set, const double* u, double*
     set->stats[1] = set->v.n_cut;
     set->stats[2] = 0;
     set->stats[3] = 0;
-    set->stats[4] = (int64_t)(set->h.n_el - set->v.n_cut) * (set->h.dim == 3 ? 8 : 4) + set->n_ip_cut;
+    const int uncut_ip = set->h.nb == 27 ? kHex3Points : (set->h.dim == 3 ? kHex2Points : kQuad2Points);
+    set->stats[4] = (int64_t)(set->h.n_el - set->v.n_cut) * uncut_ip + set->n_ip_cut;
     return MGPU_OK;
     MGPU_GUARD_END
 }

@@ -13,6 +13,9 @@
 //                       (Equation_Object::build_PADofMap / get_egn_obj_jacobian_and_residual,
 //                       cl_MSI_Equation_Object.cpp:426-493, :789-842), in FP64 through LDS, written
 //                       to the element's slot of the K_e pool.
+//   k_generic_hex27     HEX27 IP cells (HEX_3x3x3 / TET_35): one block of four wavefronts per
+//                       element, the moments on MFMA (FP64, or FP32 per 32-point chunk summed in
+//                       FP64), K_p and K_e = T^T K_p T in FP64 through LDS.
 //   k_generic_gather    one thread per CSR value: the sum of its K_e entries in ascending element
 //                       order (deterministic SumIntoGlobalValues, cl_Sparse_Matrix_EpetraFECrs.cpp
 //                       :103-133).
@@ -197,6 +200,228 @@ __global__ __launch_bounds__(64) void k_generic_element(GenView g) {
     }
 }
 
+// HEX27 IP cells: one block of 4 wavefronts per element.  Points (HEX_3x3x3 uncut, TET_35 on the
+// sub-tets, w* = w detJ, detJ < -1e-6 skipped) are staged 32 at a time: the 81 physical
+// gradients dN_a/dx_i of each point and the same times w* go to LDS, and the moments
+//   M[(i,a),(j,b)] = sum_ip w* dN_a/dx_i dN_b/dx_j      (81 x 81, padded to 96 x 96)
+// accumulate on MFMA, each wavefront owning a 3 x 3 block of 16 x 16 tiles: FP64
+// (v_mfma_f64_16x16x4f64) on the plain path; on the mixed path FP32 products
+// (v_mfma_f32_16x16x4f32) summed per 32-point chunk and added into FP64 registers.  Then K_p from
+// M and the material in LDS, and K_e = T^T K_p T per (dof type, dof type) block in FP64.
+constexpr int kH27Chunk = 32;                    // points per staging round
+constexpr int kH27Cols = 96;                     // 81 gradient columns padded to 6 tiles
+constexpr int kH27PtStride = 20;                 // w*, then l[3][3], dl[3][3] of the point
+constexpr int kH27Lds = 81 * 81 + 2 * 27 * 27;   // doubles: K_p, then T and W of one block
+static_assert(2 * kH27Chunk * kH27Cols + kH27Chunk * kH27PtStride <= kH27Lds, "staging fits the K_p region");
+
+typedef double doublex4_t __attribute__((ext_vector_type(4)));
+typedef float floatx4_t __attribute__((ext_vector_type(4)));
+
+template <int NF, typename T>
+__global__ __launch_bounds__(256) void k_generic_hex27(GenView g) {
+    constexpr int NB = 27, NG = 81, NP = NF * NB;
+    __shared__ double lds[kH27Lds];
+    const int e = blockIdx.x;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    double hh[3];
+    for (int d = 0; d < 3; ++d) hh[d] = g.ip_box[(size_t)e * 6 + 3 + d];
+    const int ig0 = g.ig_ptr[e], nsub = g.ig_ptr[e + 1] - ig0;
+    const bool cut = nsub > 0;
+    const int npts = cut ? nsub * kTet35Points : kHex3Points;
+
+    T* sGw = reinterpret_cast<T*>(lds);
+    T* sG = sGw + kH27Chunk * kH27Cols;
+    double* pts = lds + 2 * kH27Chunk * kH27Cols;
+    const int mt0 = 3 * (wave >> 1), nt0 = 3 * (wave & 1);   // the wavefront's tile block
+    const int col16 = lane & 15, kq = lane >> 4;
+    doublex4_t acc[3][3];
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+#pragma unroll
+        for (int n = 0; n < 3; ++n) acc[m][n] = doublex4_t{0.0, 0.0, 0.0, 0.0};
+
+    for (int p0 = 0; p0 < npts; p0 += kH27Chunk) {
+        // the chunk's points: w* and the 1D quadratic Lagrange values / derivatives per axis
+        if (tid < kH27Chunk) {
+            const int ip = p0 + tid;
+            double xi[3] = {0.0, 0.0, 0.0}, w = 0.0;
+            if (ip < npts) {
+                if (!cut) {
+                    double detJ = 1.0;
+                    for (int d = 0; d < 3; ++d) {
+                        xi[d] = kHex3Xi[ip][d];
+                        detJ *= 0.5 * hh[d];
+                    }
+                    w = kHex3W[ip] * detJ;
+                } else {
+                    const int t = ip / kTet35Points, q = ip - kTet35Points * t;
+                    const double* X = g.ig_xi + (size_t)(ig0 + t) * 12;
+                    const double N[4] = {kTet35Xi[q][0], kTet35Xi[q][1],
+                                         1.0 - kTet35Xi[q][0] - kTet35Xi[q][1] - kTet35Xi[q][2], kTet35Xi[q][2]};
+                    for (int d = 0; d < 3; ++d) xi[d] = N[0] * X[d] + N[1] * X[3 + d] + N[2] * X[6 + d] + N[3] * X[9 + d];
+                    double J[3][3];
+                    const int rr[3] = {0, 1, 3};
+                    for (int r = 0; r < 3; ++r)
+                        for (int d = 0; d < 3; ++d) J[r][d] = (X[rr[r] * 3 + d] - X[2 * 3 + d]) * (0.5 * hh[d]);
+                    const double detJ = (J[0][0] * (J[1][1] * J[2][2] - J[2][1] * J[1][2]) -
+                                         J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                                         J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0])) / 6.0;
+                    w = detJ < kDetJLowerLimit ? 0.0 : kTet35W[q] * detJ;
+                }
+            }
+            double* pp = pts + tid * kH27PtStride;
+            pp[0] = w;
+            for (int d = 0; d < 3; ++d) lagrange_quadratic_1d(xi[d], pp + 1 + 3 * d, pp + 10 + 3 * d);
+        }
+        __syncthreads();
+        // physical gradients: column (i, a) of point p is dl_i(xi_i) l_j(xi_j) l_k(xi_k) * 2 / h_i
+        for (int idx = tid; idx < kH27Chunk * kH27Cols; idx += 256) {
+            const int p = idx / kH27Cols, col = idx - kH27Cols * (idx / kH27Cols);
+            const double* pp = pts + p * kH27PtStride;
+            double v = 0.0;
+            if (col < NG) {
+                const int i = col / NB, a = col - NB * (col / NB);
+                v = 2.0 / hh[i];
+#pragma unroll
+                for (int d = 0; d < 3; ++d) v *= pp[(d == i ? 10 : 1) + 3 * d + kHex27Half[a][d]];
+            }
+            sG[idx] = (T)v;
+            sGw[idx] = (T)(v * pp[0]);
+        }
+        __syncthreads();
+        // M += Gw^T G over the chunk's points (A: row lane % 16 of the tile, k = lane / 16;
+        // B: k = lane / 16, column lane % 16)
+        if constexpr (sizeof(T) == 8) {
+#pragma unroll 2
+            for (int ks = 0; ks < kH27Chunk / 4; ++ks) {
+                const int p = 4 * ks + kq;
+                double a[3], b[3];
+#pragma unroll
+                for (int m = 0; m < 3; ++m) a[m] = sGw[p * kH27Cols + (mt0 + m) * 16 + col16];
+#pragma unroll
+                for (int n = 0; n < 3; ++n) b[n] = sG[p * kH27Cols + (nt0 + n) * 16 + col16];
+#pragma unroll
+                for (int m = 0; m < 3; ++m)
+#pragma unroll
+                    for (int n = 0; n < 3; ++n) acc[m][n] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[m], b[n], acc[m][n], 0, 0, 0);
+            }
+        } else {
+            floatx4_t accf[3][3];
+#pragma unroll
+            for (int m = 0; m < 3; ++m)
+#pragma unroll
+                for (int n = 0; n < 3; ++n) accf[m][n] = floatx4_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 2
+            for (int ks = 0; ks < kH27Chunk / 4; ++ks) {
+                const int p = 4 * ks + kq;
+                float a[3], b[3];
+#pragma unroll
+                for (int m = 0; m < 3; ++m) a[m] = sGw[p * kH27Cols + (mt0 + m) * 16 + col16];
+#pragma unroll
+                for (int n = 0; n < 3; ++n) b[n] = sG[p * kH27Cols + (nt0 + n) * 16 + col16];
+#pragma unroll
+                for (int m = 0; m < 3; ++m)
+#pragma unroll
+                    for (int n = 0; n < 3; ++n) accf[m][n] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[m], b[n], accf[m][n], 0, 0, 0);
+            }
+#pragma unroll
+            for (int m = 0; m < 3; ++m)
+#pragma unroll
+                for (int n = 0; n < 3; ++n)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) acc[m][n][j] += (double)accf[m][n][j];
+        }
+        __syncthreads();
+    }
+
+    // M to LDS (rows, columns < 81).  FP64 MFMA: register j of lane l holds row l / 16 + 4 j;
+    // FP32 MFMA: row 4 (l / 16) + j; column l % 16 in both.
+    double* Ms = lds;
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+#pragma unroll
+        for (int n = 0; n < 3; ++n)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int r = (mt0 + m) * 16 + (sizeof(T) == 8 ? kq + 4 * j : 4 * kq + j);
+                const int c = (nt0 + n) * 16 + col16;
+                if (r < NG && c < NG) Ms[r * NG + c] = acc[m][n][j];
+            }
+    __syncthreads();
+
+    // K_p from M: thread pairs (a, b) read their 9 moments, then write their K_p entries
+    double kv[3][9];
+#pragma unroll
+    for (int s2 = 0; s2 < 3; ++s2) {
+        const int pr = tid + 256 * s2;
+        if (pr < NB * NB) {
+            const int a = pr / NB, b = pr - NB * (pr / NB);
+            double Mv[3][3];
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) Mv[i][j] = Ms[(i * NB + a) * NG + j * NB + b];
+            if (NF == 1) {
+                kv[s2][0] = g.material[e] * (Mv[0][0] + Mv[1][1] + Mv[2][2]);
+            } else {
+                double D[6][6];
+                elastic_D<3>(g.material[2 * e], g.material[2 * e + 1], D);
+                for (int al = 0; al < 3; ++al)
+                    for (int be = 0; be < 3; ++be) {
+                        double v = 0.0;
+                        for (int i = 0; i < 3; ++i)
+                            for (int j = 0; j < 3; ++j)
+                                if (d_nonzero<3>(voigt<3>(al, i), voigt<3>(be, j)))
+                                    v += D[voigt<3>(al, i)][voigt<3>(be, j)] * Mv[i][j];
+                        kv[s2][al * 3 + be] = v;
+                    }
+            }
+        }
+    }
+    __syncthreads();
+    double* Kp = lds;
+#pragma unroll
+    for (int s2 = 0; s2 < 3; ++s2) {
+        const int pr = tid + 256 * s2;
+        if (pr < NB * NB) {
+            const int a = pr / NB, b = pr - NB * (pr / NB);
+            if (NF == 1) {
+                Kp[a * NP + b] = kv[s2][0];
+            } else {
+                for (int al = 0; al < 3; ++al)
+                    for (int be = 0; be < 3; ++be) Kp[(al * NB + a) * NP + be * NB + b] = kv[s2][al * 3 + be];
+            }
+        }
+    }
+    // dense element-local T (27 x na_t, the same for every dof type)
+    double* Tt = lds + NG * NG;
+    double* W = Tt + NB * NB;
+    const int na_t = g.na_t[e];
+    const int NA = NF * na_t;
+    for (int idx = tid; idx < NB * na_t; idx += 256) Tt[idx] = 0.0;
+    __syncthreads();
+    if (tid < NB)
+        for (int q = g.t_ptr[e * NB + tid]; q < g.t_ptr[e * NB + tid + 1]; ++q) Tt[tid * na_t + g.t_col[q]] += g.t_w[q];
+    __syncthreads();
+    // K_e = T^T K_p T per (f, h) block: W = K_p[f][h] T, then T^T W
+    double* ke = g.ke + g.ke_off[e];
+    for (int f = 0; f < NF; ++f)
+        for (int hb = 0; hb < NF; ++hb) {
+            for (int idx = tid; idx < NB * na_t; idx += 256) {
+                const int a = idx / na_t, y = idx - na_t * (idx / na_t);
+                double v = 0.0;
+                for (int b = 0; b < NB; ++b) v += Kp[(f * NB + a) * NP + hb * NB + b] * Tt[b * na_t + y];
+                W[idx] = v;
+            }
+            __syncthreads();
+            for (int idx = tid; idx < na_t * na_t; idx += 256) {
+                const int x = idx / na_t, y = idx - na_t * (idx / na_t);
+                double v = 0.0;
+                for (int a = 0; a < NB; ++a) v += Tt[a * na_t + x] * W[a * na_t + y];
+                ke[(size_t)(f * na_t + x) * NA + hb * na_t + y] = v;
+            }
+            __syncthreads();
+        }
+}
+
 __global__ void k_generic_gather(GenView g, double* vals) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= g.nnz) return;
@@ -256,7 +481,16 @@ mgpu_status launch_generic(mgpu_set* S, const double* u, double* vals, double* r
     if (h.n_el > 0) {
         const bool f32 = S->precision == MGPU_PREC_MIXED;
         const dim3 gr((unsigned)h.n_el), bl(64);
-        if (h.dim == 3 && h.nf == 1) {
+        if (h.nb == 27) {
+            const dim3 b27(256);
+            if (h.nf == 1) {
+                if (f32) hipLaunchKernelGGL((k_generic_hex27<1, float>), gr, b27, 0, S->stream, S->g);
+                else hipLaunchKernelGGL((k_generic_hex27<1, double>), gr, b27, 0, S->stream, S->g);
+            } else {
+                if (f32) hipLaunchKernelGGL((k_generic_hex27<3, float>), gr, b27, 0, S->stream, S->g);
+                else hipLaunchKernelGGL((k_generic_hex27<3, double>), gr, b27, 0, S->stream, S->g);
+            }
+        } else if (h.dim == 3 && h.nf == 1) {
             if (f32) hipLaunchKernelGGL((k_generic_element<3, 1, float>), gr, bl, 0, S->stream, S->g);
             else hipLaunchKernelGGL((k_generic_element<3, 1, double>), gr, bl, 0, S->stream, S->g);
         } else if (h.dim == 3) {

@@ -159,12 +159,13 @@ mgpu_status build_set_host(const mgpu_set_desc& d, SetHost& h, std::string* err)
     }
     const int want_ip = d.spatial_dim == 3 ? MGPU_GEOM_HEX8 : MGPU_GEOM_QUAD4;
     const int want_ig = d.spatial_dim == 3 ? MGPU_GEOM_TET4 : MGPU_GEOM_TRI3;
-    if (d.ip_geometry != want_ip || d.ig_geometry != want_ig) {
-        *err = "ip/ig geometry must be HEX8/TET4 (3D) or QUAD4/TRI3 (2D)";
+    const bool hex27 = d.spatial_dim == 3 && d.ip_geometry == MGPU_GEOM_HEX27;
+    if ((d.ip_geometry != want_ip && !hex27) || d.ig_geometry != want_ig) {
+        *err = "ip/ig geometry must be HEX8/TET4 or HEX27/TET4 (3D), QUAD4/TRI3 (2D)";
         return MGPU_ERR_UNSUPPORTED;
     }
-    const int nb_expect = d.spatial_dim == 3 ? 8 : 4;
-    if (d.n_bases != nb_expect) { *err = "n_bases must be 8 (HEX8) or 4 (QUAD4)"; return MGPU_ERR_INVALID_ARG; }
+    const int nb_expect = hex27 ? 27 : (d.spatial_dim == 3 ? 8 : 4);
+    if (d.n_bases != nb_expect) { *err = "n_bases must be 27 (HEX27), 8 (HEX8) or 4 (QUAD4)"; return MGPU_ERR_INVALID_ARG; }
     if (d.n_elements < 0 || d.n_adofs_per_type < 0) { *err = "negative sizes"; return MGPU_ERR_INVALID_ARG; }
     if (d.n_elements > 0 && (!d.ip_box || !d.material || !d.t_ptr || !d.t_adof || !d.t_weight || !d.ig_ptr)) {
         *err = "descriptor array missing";
@@ -202,9 +203,15 @@ mgpu_status build_set_host(const mgpu_set_desc& d, SetHost& h, std::string* err)
     for (int64_t r = 0; r < n_trows && nodal; ++r)
         nodal = (d.t_ptr[r + 1] - d.t_ptr[r] == 1) && d.t_weight[d.t_ptr[r]] == 1.0;
     const bool has_lattice = d.lattice.n[0] > 0 && d.elem_cell != nullptr;
-    if (!nodal || !has_lattice) {
+    if (!nodal || !has_lattice || hex27) {
         h.generic = true;
-        return build_generic(d, h, err);
+        const mgpu_status rc = build_generic(d, h, err);
+        // the HEX27 kernel stages T and T^T K_p T blocks of 27 x na_t in LDS beside K_p
+        if (rc == MGPU_OK && hex27 && h.na_t_max > kHex27MaxAdofsPerType) {
+            *err = "a HEX27 element has more than " + std::to_string(kHex27MaxAdofsPerType) + " adofs per dof type";
+            return MGPU_ERR_UNSUPPORTED;
+        }
+        return rc;
     }
 
     // element topology: sorted unique adofs per type; vertex -> sorted position

@@ -14,6 +14,8 @@ namespace mgpu {
 // Largest element topology per dof type of the generic path (element-local T and the T^T K T
 // products are staged in LDS at this size).
 constexpr int kGenericMaxAdofsPerType = 64;
+// The same for HEX27 IP cells (27 bases): level-0 quadratic B-splines or Lagrange nodes give 27.
+constexpr int kHex27MaxAdofsPerType = 27;
 
 // Host-side products of set creation (pure C++, no HIP).
 struct SetHost {

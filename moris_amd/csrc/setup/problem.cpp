@@ -12,6 +12,9 @@
 //   T rows                         p = 1: one entry of weight 1 per Lagrange node; p = 2: the
 //                                  lattice's quadratic B-splines at the node (HMR level 0,
 //                                  T_Matrix_Base::evaluate, cl_HMR_T_Matrix_Base.cpp:306-457)
+//   HEX27 IP cells                 lagrange_order 2: 27 nodes per cell on the half-step grid
+//                                  (cl_MTK_Interpolation_Function_Lagrange_Hex27.hpp node order);
+//                                  the cut is that of the cell's box, as for HEX8
 #include <algorithm>
 #include <cstring>
 #include <string>
@@ -33,7 +36,8 @@ struct mgpu_problem {
     int n[3] = {1, 1, 1};
     double lo[3] = {0, 0, 0};
     double h[3] = {1, 1, 1};
-    int nb = 8;
+    int nb = 8;      // vertices of the IP cell's box (the cut geometry)
+    int nb_ip = 8;   // IP-cell bases: nb, or 27 for HEX27
     int64_t n_cells = 0, n_nodes = 0, n_faces = 0;
     int64_t n_cut = 0, n_void = 0;
 
@@ -251,6 +255,10 @@ extern "C" mgpu_status mgpu_problem_build(const mgpu_problem_spec* spec, mgpu_pr
     p->ls.gyroid_scale = spec->gyroid_scale;
     for (int d = 0; d < 4; ++d) p->ls.plane[d] = spec->plane[d];
     p->nb = p->dim == 3 ? 8 : 4;
+    const int lag = spec->lagrange_order <= 1 ? 1 : spec->lagrange_order;
+    if (lag > 2 || (lag == 2 && p->dim != 3))
+        return mgpu::set_error(MGPU_ERR_UNSUPPORTED, "lagrange_order 2 (HEX27) is implemented in 3D only");
+    p->nb_ip = lag == 2 ? 27 : p->nb;
     p->n_cells = (int64_t)p->n[0] * p->n[1] * p->n[2];
     p->n_nodes = p->dim == 3 ? (int64_t)(p->n[0] + 1) * (p->n[1] + 1) * (p->n[2] + 1)
                              : (int64_t)(p->n[0] + 1) * (p->n[1] + 1);
@@ -309,7 +317,7 @@ extern "C" mgpu_status mgpu_problem_build(const mgpu_problem_spec* spec, mgpu_pr
     p->cell_elem.assign((size_t)p->n_cells, -1);
     p->elem_cell.resize((size_t)n_el);
     p->ip_box.resize((size_t)n_el * d2);
-    p->vertex_phi.resize((size_t)n_el * p->nb);
+    p->vertex_phi.resize((size_t)n_el * p->nb_ip);
     p->ig_ptr.resize((size_t)n_el + 1);
     p->ig_xi.resize((size_t)n_ig * nv_ig * p->dim);
     const bool diff = spec->iwg == MGPU_IWG_SPATIALDIFF_BULK;
@@ -328,10 +336,19 @@ extern "C" mgpu_status mgpu_problem_build(const mgpu_problem_spec* spec, mgpu_pr
                 p->ip_box[(size_t)e * d2 + d] = p->lo[d] + (double)ijk[d] * p->h[d];
                 p->ip_box[(size_t)e * d2 + p->dim + d] = p->h[d];
             }
-            for (int v = 0; v < p->nb; ++v) {
-                const int64_t nid = p->dim == 3 ? node_index(*p, i + kHex8Off[v][0], j + kHex8Off[v][1], k + kHex8Off[v][2])
-                                                : node_index(*p, i + kQuad4Off[v][0], j + kQuad4Off[v][1], 0);
-                p->vertex_phi[(size_t)e * p->nb + v] = p->node_phi[(size_t)nid];
+            if (p->nb_ip == 27) {
+                // level set at the 27 HEX27 nodes (the cut itself uses the 8 corner values)
+                for (int v = 0; v < 27; ++v) {
+                    double x[3];
+                    for (int d = 0; d < 3; ++d) x[d] = p->lo[d] + ((double)ijk[d] + 0.5 * kHex27Half[v][d]) * p->h[d];
+                    p->vertex_phi[(size_t)e * 27 + v] = p->ls(x);
+                }
+            } else {
+                for (int v = 0; v < p->nb; ++v) {
+                    const int64_t nid = p->dim == 3 ? node_index(*p, i + kHex8Off[v][0], j + kHex8Off[v][1], k + kHex8Off[v][2])
+                                                    : node_index(*p, i + kQuad4Off[v][0], j + kQuad4Off[v][1], 0);
+                    p->vertex_phi[(size_t)e * p->nb + v] = p->node_phi[(size_t)nid];
+                }
             }
             if (diff) {
                 p->material[(size_t)e] = spec->k_sigma == 0.0
@@ -355,17 +372,31 @@ extern "C" mgpu_status mgpu_problem_build(const mgpu_problem_spec* spec, mgpu_pr
     // 4./5. adof numbering and T rows
     const int order = spec->bspline_order <= 1 ? 1 : spec->bspline_order;
     if (order > 2) return mgpu::set_error(MGPU_ERR_UNSUPPORTED, "bspline_order must be 1 or 2");
-    if (order == 1) {
+    // position of basis v of the element in cell (i, j, k) on the half-step grid of the lattice
+    auto basis_half = [&](int i, int j, int k, int v, int* g) {
+        const int ijk[3] = {i, j, k};
+        for (int d = 0; d < 3; ++d) {
+            if (d >= p->dim) { g[d] = 0; continue; }
+            const int off = p->nb_ip == 27 ? kHex27Half[v][d] : 2 * (p->dim == 3 ? kHex8Off[v][d] : kQuad4Off[v][d]);
+            g[d] = 2 * ijk[d] + off;
+        }
+    };
+    auto elem_ijk = [&](int64_t ee, int* ijk) {
+        const int64_t c = p->elem_cell[(size_t)ee];
+        ijk[0] = (int)(c % p->n[0]);
+        ijk[1] = (int)((c / p->n[0]) % p->n[1]);
+        ijk[2] = p->dim == 3 ? (int)(c / ((int64_t)p->n[0] * p->n[1])) : 0;
+    };
+    const size_t n_rows = (size_t)n_el * p->nb_ip;
+    if (order == 1 && p->nb_ip != 27) {
         // p = 1: one adof per used lattice node, ascending node index; T rows of weight 1
         std::vector<int32_t> node_adof((size_t)p->n_nodes, -1);
         for (int64_t ee = 0; ee < n_el; ++ee) {
-            const int64_t c = p->elem_cell[(size_t)ee];
-            const int i = (int)(c % p->n[0]);
-            const int j = (int)((c / p->n[0]) % p->n[1]);
-            const int k = p->dim == 3 ? (int)(c / ((int64_t)p->n[0] * p->n[1])) : 0;
+            int ijk[3];
+            elem_ijk(ee, ijk);
             for (int v = 0; v < p->nb; ++v) {
-                const int64_t nid = p->dim == 3 ? node_index(*p, i + kHex8Off[v][0], j + kHex8Off[v][1], k + kHex8Off[v][2])
-                                                : node_index(*p, i + kQuad4Off[v][0], j + kQuad4Off[v][1], 0);
+                const int64_t nid = p->dim == 3 ? node_index(*p, ijk[0] + kHex8Off[v][0], ijk[1] + kHex8Off[v][1], ijk[2] + kHex8Off[v][2])
+                                                : node_index(*p, ijk[0] + kQuad4Off[v][0], ijk[1] + kQuad4Off[v][1], 0);
                 node_adof[(size_t)nid] = 0;
             }
         }
@@ -373,74 +404,118 @@ extern "C" mgpu_status mgpu_problem_build(const mgpu_problem_spec* spec, mgpu_pr
         for (int64_t nid = 0; nid < p->n_nodes; ++nid)
             if (node_adof[(size_t)nid] == 0) node_adof[(size_t)nid] = next++;
         p->n_adofs_per_type = next;
-        p->t_ptr.resize((size_t)n_el * p->nb + 1);
-        p->t_adof.resize((size_t)n_el * p->nb);
-        p->t_weight.assign((size_t)n_el * p->nb, 1.0);
+        p->t_ptr.resize(n_rows + 1);
+        p->t_adof.resize(n_rows);
+        p->t_weight.assign(n_rows, 1.0);
         for (int64_t ee = 0; ee < n_el; ++ee) {
-            const int64_t c = p->elem_cell[(size_t)ee];
-            const int i = (int)(c % p->n[0]);
-            const int j = (int)((c / p->n[0]) % p->n[1]);
-            const int k = p->dim == 3 ? (int)(c / ((int64_t)p->n[0] * p->n[1])) : 0;
+            int ijk[3];
+            elem_ijk(ee, ijk);
             for (int v = 0; v < p->nb; ++v) {
-                const int64_t nid = p->dim == 3 ? node_index(*p, i + kHex8Off[v][0], j + kHex8Off[v][1], k + kHex8Off[v][2])
-                                                : node_index(*p, i + kQuad4Off[v][0], j + kQuad4Off[v][1], 0);
+                const int64_t nid = p->dim == 3 ? node_index(*p, ijk[0] + kHex8Off[v][0], ijk[1] + kHex8Off[v][1], ijk[2] + kHex8Off[v][2])
+                                                : node_index(*p, ijk[0] + kQuad4Off[v][0], ijk[1] + kQuad4Off[v][1], 0);
                 const size_t r = (size_t)ee * p->nb + v;
                 p->t_ptr[r] = (int32_t)r;
                 p->t_adof[r] = node_adof[(size_t)nid];
             }
         }
-        p->t_ptr[(size_t)n_el * p->nb] = (int32_t)((size_t)n_el * p->nb);
-    } else {
-        // p = 2: uniform quadratic B-splines on the lattice, n + 2 per axis (spline s covers the
-        // cells s - 2 .. s); at lattice node i the splines s = i and s = i + 1 take the value 1/2
-        // each, so a node's row holds 2^dim splines of weight 2^-dim.  Adofs: used splines in
-        // ascending spline index (x fastest).
-        const int64_t sx = p->n[0] + 2, sy = p->n[1] + 2, sz = p->dim == 3 ? p->n[2] + 2 : 1;
-        if (sx * sy * sz >= INT32_MAX) return mgpu::set_error(MGPU_ERR_INVALID_ARG, "too many splines");
-        const int nsp = p->dim == 3 ? 8 : 4;
-        const double w = p->dim == 3 ? 0.125 : 0.25;
-        auto node_splines = [&](int i, int j, int k, int64_t* out) {
-            int q = 0;
-            for (int c = 0; c < (p->dim == 3 ? 2 : 1); ++c)
-                for (int b = 0; b < 2; ++b)
-                    for (int a = 0; a < 2; ++a) out[q++] = (i + a) + sx * ((j + b) + sy * (int64_t)(k + c));
-        };
-        std::vector<int32_t> spline_adof((size_t)(sx * sy * sz), -1);
-        int64_t sp[8];
+        p->t_ptr[n_rows] = (int32_t)n_rows;
+    } else if (order == 1) {
+        // HEX27 Lagrange mesh with nodal T rows: one adof per used node of the half-step grid,
+        // ascending node index (x fastest), weight 1
+        const int64_t gx = 2 * (int64_t)p->n[0] + 1, gy = 2 * (int64_t)p->n[1] + 1, gz = 2 * (int64_t)p->n[2] + 1;
+        if (gx * gy * gz >= INT32_MAX) return mgpu::set_error(MGPU_ERR_INVALID_ARG, "too many HEX27 nodes");
+        std::vector<int32_t> node_adof((size_t)(gx * gy * gz), -1);
+        auto gid = [&](const int* g) { return (size_t)(g[0] + gx * (g[1] + gy * (int64_t)g[2])); };
+        int g[3], ijk[3];
         for (int64_t ee = 0; ee < n_el; ++ee) {
-            const int64_t c = p->elem_cell[(size_t)ee];
-            const int i = (int)(c % p->n[0]);
-            const int j = (int)((c / p->n[0]) % p->n[1]);
-            const int k = p->dim == 3 ? (int)(c / ((int64_t)p->n[0] * p->n[1])) : 0;
-            for (int v = 0; v < p->nb; ++v) {
-                const int* off = p->dim == 3 ? kHex8Off[v] : kQuad4Off[v];
-                node_splines(i + off[0], j + off[1], p->dim == 3 ? k + off[2] : 0, sp);
-                for (int q = 0; q < nsp; ++q) spline_adof[(size_t)sp[q]] = 0;
+            elem_ijk(ee, ijk);
+            for (int v = 0; v < 27; ++v) {
+                basis_half(ijk[0], ijk[1], ijk[2], v, g);
+                node_adof[gid(g)] = 0;
             }
         }
+        int32_t next = 0;
+        for (auto& a : node_adof)
+            if (a == 0) a = next++;
+        p->n_adofs_per_type = next;
+        p->t_ptr.resize(n_rows + 1);
+        p->t_adof.resize(n_rows);
+        p->t_weight.assign(n_rows, 1.0);
+        for (int64_t ee = 0; ee < n_el; ++ee) {
+            elem_ijk(ee, ijk);
+            for (int v = 0; v < 27; ++v) {
+                basis_half(ijk[0], ijk[1], ijk[2], v, g);
+                const size_t r = (size_t)ee * 27 + v;
+                p->t_ptr[r] = (int32_t)r;
+                p->t_adof[r] = node_adof[gid(g)];
+            }
+        }
+        p->t_ptr[n_rows] = (int32_t)n_rows;
+    } else {
+        // p = 2: uniform quadratic B-splines on the lattice, n + 2 per axis (spline s covers the
+        // cells s - 2 .. s).  Per axis, a lattice node m (even half-step coordinate 2m) carries
+        // the splines m, m + 1 with value 1/2 each; the centre of cell m (odd coordinate 2m + 1)
+        // carries the splines m, m + 1, m + 2 with 1/8, 3/4, 1/8.  A row is the tensor product,
+        // x fastest.  Adofs: used splines in ascending spline index (x fastest).
+        const int64_t sx = p->n[0] + 2, sy = p->n[1] + 2, sz = p->dim == 3 ? p->n[2] + 2 : 1;
+        if (sx * sy * sz >= INT32_MAX) return mgpu::set_error(MGPU_ERR_INVALID_ARG, "too many splines");
+        struct AxisRow { int n; int s[3]; double w[3]; };
+        auto axis_row = [&](int d, int g, AxisRow& a) {
+            if (d >= p->dim) { a.n = 1; a.s[0] = 0; a.w[0] = 1.0; return; }
+            const int m = g / 2;
+            if ((g & 1) == 0) {
+                a.n = 2; a.s[0] = m; a.s[1] = m + 1; a.w[0] = a.w[1] = 0.5;
+            } else {
+                a.n = 3; a.s[0] = m; a.s[1] = m + 1; a.s[2] = m + 2; a.w[0] = 0.125; a.w[1] = 0.75; a.w[2] = 0.125;
+            }
+        };
+        // row of basis v of element ee: (spline, weight) pairs
+        auto row = [&](int64_t ee, int v, int64_t* sp, double* w) {
+            int ijk[3], g[3];
+            elem_ijk(ee, ijk);
+            basis_half(ijk[0], ijk[1], ijk[2], v, g);
+            AxisRow ax[3];
+            for (int d = 0; d < 3; ++d) axis_row(d, g[d], ax[d]);
+            int q = 0;
+            for (int c = 0; c < ax[2].n; ++c)
+                for (int b = 0; b < ax[1].n; ++b)
+                    for (int a = 0; a < ax[0].n; ++a) {
+                        sp[q] = ax[0].s[a] + sx * (ax[1].s[b] + sy * (int64_t)ax[2].s[c]);
+                        w[q] = ax[0].w[a] * ax[1].w[b] * ax[2].w[c];
+                        ++q;
+                    }
+            return q;
+        };
+        std::vector<int32_t> spline_adof((size_t)(sx * sy * sz), -1);
+        int64_t sp[27];
+        double wt[27];
+        size_t n_entries = 0;
+        for (int64_t ee = 0; ee < n_el; ++ee)
+            for (int v = 0; v < p->nb_ip; ++v) {
+                const int nq = row(ee, v, sp, wt);
+                n_entries += (size_t)nq;
+                for (int q = 0; q < nq; ++q) spline_adof[(size_t)sp[q]] = 0;
+            }
         int32_t next = 0;
         for (auto& a : spline_adof)
             if (a == 0) a = next++;
         p->n_adofs_per_type = next;
-        const size_t n_rows = (size_t)n_el * p->nb;
-        if (n_rows * nsp >= (size_t)INT32_MAX) return mgpu::set_error(MGPU_ERR_INVALID_ARG, "too many T entries");
+        if (n_entries >= (size_t)INT32_MAX) return mgpu::set_error(MGPU_ERR_INVALID_ARG, "too many T entries");
         p->t_ptr.resize(n_rows + 1);
-        p->t_adof.resize(n_rows * nsp);
-        p->t_weight.assign(n_rows * nsp, w);
-        for (int64_t ee = 0; ee < n_el; ++ee) {
-            const int64_t c = p->elem_cell[(size_t)ee];
-            const int i = (int)(c % p->n[0]);
-            const int j = (int)((c / p->n[0]) % p->n[1]);
-            const int k = p->dim == 3 ? (int)(c / ((int64_t)p->n[0] * p->n[1])) : 0;
-            for (int v = 0; v < p->nb; ++v) {
-                const int* off = p->dim == 3 ? kHex8Off[v] : kQuad4Off[v];
-                node_splines(i + off[0], j + off[1], p->dim == 3 ? k + off[2] : 0, sp);
-                const size_t r = (size_t)ee * p->nb + v;
-                p->t_ptr[r] = (int32_t)(r * nsp);
-                for (int q = 0; q < nsp; ++q) p->t_adof[r * nsp + q] = spline_adof[(size_t)sp[q]];
+        p->t_adof.resize(n_entries);
+        p->t_weight.resize(n_entries);
+        size_t pos = 0;
+        for (int64_t ee = 0; ee < n_el; ++ee)
+            for (int v = 0; v < p->nb_ip; ++v) {
+                const int nq = row(ee, v, sp, wt);
+                p->t_ptr[(size_t)ee * p->nb_ip + v] = (int32_t)pos;
+                for (int q = 0; q < nq; ++q) {
+                    p->t_adof[pos] = spline_adof[(size_t)sp[q]];
+                    p->t_weight[pos] = wt[q];
+                    ++pos;
+                }
             }
-        }
-        p->t_ptr[n_rows] = (int32_t)(n_rows * nsp);
+        p->t_ptr[n_rows] = (int32_t)pos;
     }
 
     *out = p.release();
@@ -453,10 +528,10 @@ extern "C" mgpu_status mgpu_problem_desc(const mgpu_problem* p, mgpu_set_desc* d
     std::memset(d, 0, sizeof(*d));
     d->spatial_dim = p->dim;
     d->iwg = p->spec.iwg;
-    d->ip_geometry = p->dim == 3 ? MGPU_GEOM_HEX8 : MGPU_GEOM_QUAD4;
+    d->ip_geometry = p->nb_ip == 27 ? MGPU_GEOM_HEX27 : (p->dim == 3 ? MGPU_GEOM_HEX8 : MGPU_GEOM_QUAD4);
     d->ig_geometry = p->dim == 3 ? MGPU_GEOM_TET4 : MGPU_GEOM_TRI3;
     d->n_elements = (int32_t)p->elem_cell.size();
-    d->n_bases = p->nb;
+    d->n_bases = p->nb_ip;
     d->n_adofs_per_type = p->n_adofs_per_type;
     d->ip_box = p->ip_box.data();
     d->material = p->material.data();

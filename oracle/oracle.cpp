@@ -29,7 +29,10 @@
 //   CSR graph                          SOL/SOL_CORE/src/cl_Sparse_Matrix_EpetraFECrs.cpp:195-240
 //   SumIntoGlobalValues                cl_Sparse_Matrix_EpetraFECrs.cpp:103-133
 //   quadrature tables                  MTK/src/integration/cl_MTK_Integration_Coeffs_{Quad_2x2,
-//                                      Hex_2x2x2,Tri_7,Tet_11}.hpp
+//                                      Hex_2x2x2,Hex_3x3x3,Tri_7,Tet_11,Tet_35}.hpp
+//   HEX27 shape functions              MTK/src/interpolation/cl_MTK_Interpolation_Function_Lagrange_Hex27.hpp
+//                                      (node order of get_param_coords; tensor-product form)
+//   auto rules of quadratic IP cells   FEM/INT/src/CORE/cl_FEM_Set.cpp:3022-3091
 #include <algorithm>
 #include <array>
 #include <cmath>
@@ -113,6 +116,95 @@ void shape_dN(int dim, const double* xi, double dN[3][8]) {
             dN[1][a] = 0.25 * sy * (1.0 + sx * xi[0]);
         }
     }
+}
+
+// HEX_3x3x3 (cl_MTK_Integration_Coeffs_Hex_3x3x3.hpp): the tensor Gauss rule; the weight of a point
+// is the reference literal of its class (number of zero coordinates: 0, 1, 2, 3).
+const double G3 = 0.774596669241483;
+const double kHex3Wclass[4] = {0.171467764060357, 0.274348422496571, 0.438957475994513, 0.702331961591221};
+void hex3_rule(double (*xi)[3], double* w) {
+    const double g1[3] = {-G3, 0.0, G3};
+    int q = 0;
+    for (int c = 0; c < 3; ++c)
+        for (int b = 0; b < 3; ++b)
+            for (int a = 0; a < 3; ++a) {
+                xi[q][0] = g1[a];
+                xi[q][1] = g1[b];
+                xi[q][2] = g1[c];
+                w[q] = kHex3Wclass[(a == 1) + (b == 1) + (c == 1)];
+                ++q;
+            }
+}
+
+// TET_35 (cl_MTK_Integration_Coeffs_Tet_35.hpp): five orbits of barycentric tuples under
+// permutation; a point is the first three barycentric coordinates of a distinct permutation.
+// Weights as in the reference file (sum 1, detJ = volume).
+void tet35_rule(double (*xi)[3], double* w) {
+    struct Orbit { double l[4]; double w; };
+    const Orbit orb[5] = {
+        {{0.9197896733368800, 0.0267367755543735, 0.0267367755543735, 0.0267367755543735}, 0.0021900463965388},
+        {{0.1740356302468940, 0.7477598884818090, 0.0391022406356488, 0.0391022406356488}, 0.0143395670177665},
+        {{0.4547545999844830, 0.4547545999844830, 0.0452454000155172, 0.0452454000155172}, 0.0250305395686746},
+        {{0.5031186450145980, 0.2232010379623150, 0.2232010379623150, 0.0504792790607720}, 0.0479839333057554},
+        {{0.2500000000000000, 0.2500000000000000, 0.2500000000000000, 0.2500000000000000}, 0.0931745731195340}};
+    int q = 0;
+    for (const Orbit& o : orb) {
+        std::array<double, 4> l = {o.l[0], o.l[1], o.l[2], o.l[3]};
+        std::sort(l.begin(), l.end());
+        do {
+            xi[q][0] = l[0];
+            xi[q][1] = l[1];
+            xi[q][2] = l[2];
+            w[q] = o.w;
+            ++q;
+        } while (std::next_permutation(l.begin(), l.end()));
+    }
+    if (q != 35) throw std::runtime_error("TET_35 orbit count");
+}
+
+// HEX27 (cl_MTK_Interpolation_Function_Lagrange_Hex27.hpp): parametric node coordinates in the
+// reference's node order, and the functions as tensor products of the 1D quadratic Lagrange
+// polynomials through -1, 0, 1.
+const int kHex27V[27][3] = {{-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {-1, 1, -1}, {-1, -1, 1}, {1, -1, 1}, {1, 1, 1},
+                            {-1, 1, 1},   {0, -1, -1}, {1, 0, -1}, {0, 1, -1}, {-1, 0, -1}, {-1, -1, 0}, {1, -1, 0},
+                            {1, 1, 0},    {-1, 1, 0},  {0, -1, 1}, {1, 0, 1},  {0, 1, 1},   {-1, 0, 1},  {0, 0, 0},
+                            {0, 0, -1},   {0, 0, 1},   {-1, 0, 0}, {1, 0, 0},  {0, -1, 0},  {0, 1, 0}};
+double lag2(int node, double s) {   // 1D quadratic Lagrange function of node -1 / 0 / 1
+    return node == 0 ? (1.0 - s) * (1.0 + s) : 0.5 * s * (s + (double)node);
+}
+double dlag2(int node, double s) {
+    return node == 0 ? -2.0 * s : s + 0.5 * (double)node;
+}
+void hex27_N(const double* xi, double* N) {
+    for (int a = 0; a < 27; ++a) N[a] = lag2(kHex27V[a][0], xi[0]) * lag2(kHex27V[a][1], xi[1]) * lag2(kHex27V[a][2], xi[2]);
+}
+void hex27_dN(const double* xi, double dN[3][27]) {
+    for (int a = 0; a < 27; ++a)
+        for (int i = 0; i < 3; ++i) {
+            double v = 1.0;
+            for (int d = 0; d < 3; ++d) v *= d == i ? dlag2(kHex27V[a][d], xi[d]) : lag2(kHex27V[a][d], xi[d]);
+            dN[i][a] = v;
+        }
+}
+
+// IP-cell bases of nb = 4, 8 or 27
+void basis_N(int dim, int nb, const double* xi, double* N) {
+    if (nb == 27) hex27_N(xi, N);
+    else shape_N(dim, xi, N);
+}
+void basis_dN(int dim, int nb, const double* xi, double dN[3][27]) {
+    if (nb == 27) {
+        hex27_dN(xi, dN);
+        return;
+    }
+    double d8[3][8];
+    shape_dN(dim, xi, d8);
+    for (int i = 0; i < dim; ++i)
+        for (int a = 0; a < nb; ++a) dN[i][a] = d8[i][a];
+}
+// parametric coordinate of basis a of an nb-basis cell (for its nodal position)
+void basis_xi(int dim, int nb, int a, double* xi) {
+    for (int d = 0; d < dim; ++d) xi[d] = nb == 27 ? kHex27V[a][d] : (dim == 3 ? kHexV[a][d] : kQuadV[a][d]);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -270,6 +362,7 @@ struct OracleProblem {
     mgpu_problem_spec spec{};
     LS ls;
     int dim = 3, nb = 8, nf = 1, nm = 1;
+    int nb_ip = 8;   // IP-cell bases: nb (the box vertices), or 27 for HEX27
     int n[3] = {1, 1, 1};
     double lo[3] = {0, 0, 0}, h[3] = {1, 1, 1};
     int64_t n_cells = 0, n_nodes = 0;
@@ -490,26 +583,29 @@ double bspline2(double t) {
     return 0.5 * (3.0 - t) * (3.0 - t);
 }
 
-// T rows of p = 2 B-splines at the linear Lagrange nodes of each element (HMR level 0 T-matrix,
+// Lattice coordinates (in cell units) of basis a of element e: the Lagrange node of the IP cell.
+void basis_pos(const OracleProblem& p, size_t e, int a, double* x) {
+    const int64_t c = p.elem_cell[e];
+    const int ijk[3] = {(int)(c % p.n[0]), (int)((c / p.n[0]) % p.n[1]),
+                        p.dim == 3 ? (int)(c / ((int64_t)p.n[0] * p.n[1])) : 0};
+    double xi[3] = {0.0, 0.0, 0.0};
+    basis_xi(p.dim, p.nb_ip, a, xi);
+    for (int d = 0; d < 3; ++d) x[d] = d < p.dim ? (double)ijk[d] + 0.5 * (xi[d] + 1.0) : 0.0;
+}
+
+// T rows of p = 2 B-splines at the Lagrange nodes of each element (HMR level 0 T-matrix,
 // cl_HMR_T_Matrix_Base.cpp:306-457, B-spline values at the Lagrange nodes; entries with
 // |T| <= 1e-12 dropped as there).  Spline s of an axis starts at knot s - 2 (so n + 2 splines
 // cover n cells); adofs are the used splines in ascending index, x fastest.
-void bspline2_rows(OracleProblem& p, const std::vector<char>& used_nodes) {
+void bspline2_rows(OracleProblem& p) {
     const int64_t ns[3] = {p.n[0] + 2, p.n[1] + 2, p.dim == 3 ? p.n[2] + 2 : 1};
-    auto node_ijk = [&](int64_t nid, int* ijk) {
-        ijk[0] = (int)(nid % (p.n[0] + 1));
-        ijk[1] = (int)((nid / (p.n[0] + 1)) % (p.n[1] + 1));
-        ijk[2] = p.dim == 3 ? (int)(nid / ((int64_t)(p.n[0] + 1) * (p.n[1] + 1))) : 0;
-    };
-    // per node: (spline index, weight) of every spline non-zero there
-    auto row = [&](int64_t nid, std::vector<std::pair<int64_t, double>>& out) {
-        int ijk[3];
-        node_ijk(nid, ijk);
+    // per node position: (spline index, weight) of every spline non-zero there
+    auto row = [&](const double* x, std::vector<std::pair<int64_t, double>>& out) {
         std::vector<std::pair<int64_t, double>> ax[3];
         for (int d = 0; d < 3; ++d) {
             if (d >= p.dim) { ax[d].push_back({0, 1.0}); continue; }
             for (int64_t sp = 0; sp < ns[d]; ++sp) {
-                const double v = bspline2((double)ijk[d] - (double)(sp - 2));
+                const double v = bspline2(x[d] - (double)(sp - 2));
                 if (std::fabs(v) > 1e-12) ax[d].push_back({sp, v});
             }
         }
@@ -521,31 +617,34 @@ void bspline2_rows(OracleProblem& p, const std::vector<char>& used_nodes) {
     };
     std::map<int64_t, int32_t> adof;
     std::vector<std::pair<int64_t, double>> r;
-    for (int64_t nid = 0; nid < p.n_nodes; ++nid) {
-        if (!used_nodes[nid]) continue;
-        row(nid, r);
-        for (auto& q : r) adof[q.first] = 0;
-    }
+    double x[3];
+    for (size_t e = 0; e < p.elem_cell.size(); ++e)
+        for (int v = 0; v < p.nb_ip; ++v) {
+            basis_pos(p, e, v, x);
+            row(x, r);
+            for (auto& q : r) adof[q.first] = 0;
+        }
     p.N = 0;
     for (auto& kv : adof) kv.second = p.N++;
-    for (size_t e = 0; e < p.elem_cell.size(); ++e) {
-        int64_t nodes[8];
-        cell_nodes(p, p.elem_cell[e], nodes);
-        for (int v = 0; v < p.nb; ++v) {
+    for (size_t e = 0; e < p.elem_cell.size(); ++e)
+        for (int v = 0; v < p.nb_ip; ++v) {
+            basis_pos(p, e, v, x);
             p.t_ptr.push_back((int32_t)p.t_adof.size());
-            row(nodes[v], r);
+            row(x, r);
             for (auto& q : r) {
                 p.t_adof.push_back(adof.at(q.first));
                 p.t_weight.push_back(q.second);
             }
         }
-    }
 }
 
 void build(OracleProblem& p, const mgpu_problem_spec& s) {
     p.spec = s;
     p.dim = s.dim;
     p.nb = s.dim == 3 ? 8 : 4;
+    const int lag = s.lagrange_order <= 1 ? 1 : s.lagrange_order;
+    if (lag > 2 || (lag == 2 && s.dim != 3)) throw std::runtime_error("lagrange_order 2 (HEX27) is 3D only");
+    p.nb_ip = lag == 2 ? 27 : p.nb;
     p.nf = s.iwg == MGPU_IWG_STRUC_LINEAR_BULK ? s.dim : 1;
     p.nm = s.iwg == MGPU_IWG_STRUC_LINEAR_BULK ? 2 : 1;
     for (int d = 0; d < 3; ++d) {
@@ -602,7 +701,7 @@ void build(OracleProblem& p, const mgpu_problem_spec& s) {
         for (int v = 0; v < p.nb; ++v) used[nodes[v]] = 1;
     }
     const int order = s.bspline_order <= 1 ? 1 : s.bspline_order;
-    if (order == 1) {
+    if (order == 1 && p.nb_ip != 27) {
         std::vector<int32_t> adof(p.n_nodes, -1);
         for (int64_t nid = 0; nid < p.n_nodes; ++nid)
             if (used[nid]) adof[nid] = p.N++;
@@ -615,8 +714,29 @@ void build(OracleProblem& p, const mgpu_problem_spec& s) {
                 p.t_weight.push_back(1.0);
             }
         }
+    } else if (order == 1) {
+        // HEX27 Lagrange nodes on the half-step grid: used nodes in ascending index, x fastest
+        const int64_t gx = 2 * (int64_t)p.n[0] + 1, gy = 2 * (int64_t)p.n[1] + 1;
+        auto key = [&](const double* x) {
+            return std::llround(2.0 * x[0]) + gx * (std::llround(2.0 * x[1]) + gy * std::llround(2.0 * x[2]));
+        };
+        std::map<int64_t, int32_t> adof;
+        double x[3];
+        for (size_t e = 0; e < p.elem_cell.size(); ++e)
+            for (int v = 0; v < 27; ++v) {
+                basis_pos(p, e, v, x);
+                adof[key(x)] = 0;
+            }
+        for (auto& kv : adof) kv.second = p.N++;
+        for (size_t e = 0; e < p.elem_cell.size(); ++e)
+            for (int v = 0; v < 27; ++v) {
+                basis_pos(p, e, v, x);
+                p.t_ptr.push_back((int32_t)p.t_adof.size());
+                p.t_adof.push_back(adof.at(key(x)));
+                p.t_weight.push_back(1.0);
+            }
     } else {
-        bspline2_rows(p, used);
+        bspline2_rows(p);
     }
     p.t_ptr.push_back((int32_t)p.t_adof.size());
 }
@@ -685,12 +805,20 @@ void element_assemble(const mgpu_set_desc& d, int64_t e, const double* u, const 
     s.Rp.assign(np, 0.0);
     // IP cell geometry: vertices of the box
     const double* box = d.ip_box + (size_t)e * 2 * D;
-    double X[8][3];
-    for (int v = 0; v < nb; ++v)
-        for (int q = 0; q < D; ++q) {
-            const int sgn = D == 3 ? kHexV[v][q] : kQuadV[v][q];
-            X[v][q] = box[q] + (sgn > 0 ? box[D + q] : 0.0);
-        }
+    double X[27][3];
+    for (int v = 0; v < nb; ++v) {
+        double xv[3];
+        basis_xi(D, nb, v, xv);
+        for (int q = 0; q < D; ++q) X[v][q] = box[q] + 0.5 * (xv[q] + 1.0) * box[D + q];
+    }
+    // rules: HEX_2x2x2 / QUAD_2x2 and TET_11 / TRI_7 under linear IP cells; HEX_3x3x3 and TET_35
+    // under HEX27 (cl_FEM_Set.cpp:3022-3091)
+    struct Rules {
+        double h3[27][3], h3w[27], t35[35][3], t35w[35];
+        Rules() { hex3_rule(h3, h3w); tet35_rule(t35, t35w); }
+    };
+    static const Rules rules;
+    const bool quad = nb == 27;
     // material
     const int ns = elast ? (D == 3 ? 6 : 3) : D;
     double Dm[6][6] = {{0}};
@@ -718,15 +846,15 @@ void element_assemble(const mgpu_set_desc& d, int64_t e, const double* u, const 
     std::vector<double> B((size_t)ns * np), DB((size_t)ns * np);
     for (int dom = 0; dom < n_dom; ++dom) {
         int nq;
-        if (!cut) nq = D == 3 ? 8 : 4;
-        else nq = D == 3 ? 11 : 7;
+        if (!cut) nq = quad ? 27 : (D == 3 ? 8 : 4);
+        else nq = quad ? 35 : (D == 3 ? 11 : 7);
         const double* Xh = cut ? d.ig_xi + (size_t)(ig0 + dom) * nv * D : nullptr;
         // IG physical vertex coordinates (interpolated in the IP cell)
         double Xig[4][3];
         if (cut)
             for (int v = 0; v < nv; ++v) {
-                double Nv[8];
-                shape_N(D, Xh + v * D, Nv);
+                double Nv[27];
+                basis_N(D, nb, Xh + v * D, Nv);
                 for (int q = 0; q < D; ++q) {
                     Xig[v][q] = 0.0;
                     for (int a = 0; a < nb; ++a) Xig[v][q] += Nv[a] * X[a][q];
@@ -736,11 +864,11 @@ void element_assemble(const mgpu_set_desc& d, int64_t e, const double* u, const 
             double xip[3] = {0, 0, 0};
             double w, detJ;
             if (!cut) {
-                for (int q = 0; q < D; ++q) xip[q] = D == 3 ? kHex2[iq][q] : kQuad2[iq][q];
-                w = 1.0;
+                for (int q = 0; q < D; ++q) xip[q] = quad ? rules.h3[iq][q] : (D == 3 ? kHex2[iq][q] : kQuad2[iq][q]);
+                w = quad ? rules.h3w[iq] : 1.0;
                 // IG cell = IP cell: detJ of the hex / quad from physical coordinates
-                double dN[3][8];
-                shape_dN(D, xip, dN);
+                double dN[3][27];
+                basis_dN(D, nb, xip, dN);
                 double Jt[3][3] = {{0}};
                 for (int r = 0; r < D; ++r)
                     for (int q = 0; q < D; ++q)
@@ -753,9 +881,9 @@ void element_assemble(const mgpu_set_desc& d, int64_t e, const double* u, const 
                 double xg[3];
                 double Ng[4];
                 if (D == 3) {
-                    for (int q = 0; q < 3; ++q) xg[q] = kTet11P[iq][q];
+                    for (int q = 0; q < 3; ++q) xg[q] = quad ? rules.t35[iq][q] : kTet11P[iq][q];
                     Ng[0] = xg[0]; Ng[1] = xg[1]; Ng[2] = 1.0 - xg[0] - xg[1] - xg[2]; Ng[3] = xg[2];
-                    w = tet11_weight(iq);
+                    w = quad ? rules.t35w[iq] : tet11_weight(iq);
                     // dN/dxi of TET4: rows (1,0,-1,0), (0,1,-1,0), (0,0,-1,1)
                     const double dNt[3][4] = {{1, 0, -1, 0}, {0, 1, -1, 0}, {0, 0, -1, 1}};
                     double Jt[3][3] = {{0}};
@@ -785,13 +913,13 @@ void element_assemble(const mgpu_set_desc& d, int64_t e, const double* u, const 
             const double ws = w * detJ;
             if (n_ip) ++*n_ip;
             // IP cell Jacobian at xi_IP and its rectangular (diagonal) inverse
-            double dN[3][8];
-            shape_dN(D, xip, dN);
+            double dN[3][27];
+            basis_dN(D, nb, xip, dN);
             double Jt[3][3] = {{0}};
             for (int r = 0; r < D; ++r)
                 for (int q = 0; q < D; ++q)
                     for (int a = 0; a < nb; ++a) Jt[r][q] += dN[r][a] * X[a][q];
-            double dNdx[3][8];
+            double dNdx[3][27];
             for (int r = 0; r < D; ++r)
                 for (int a = 0; a < nb; ++a) dNdx[r][a] = dN[r][a] / Jt[r][r];
             // B (test strain) and flux
@@ -889,10 +1017,10 @@ mgpu_set_desc desc_of(const OracleProblem& p) {
     std::memset(&d, 0, sizeof(d));
     d.spatial_dim = p.dim;
     d.iwg = p.spec.iwg;
-    d.ip_geometry = p.dim == 3 ? MGPU_GEOM_HEX8 : MGPU_GEOM_QUAD4;
+    d.ip_geometry = p.nb_ip == 27 ? MGPU_GEOM_HEX27 : (p.dim == 3 ? MGPU_GEOM_HEX8 : MGPU_GEOM_QUAD4);
     d.ig_geometry = p.dim == 3 ? MGPU_GEOM_TET4 : MGPU_GEOM_TRI3;
     d.n_elements = (int32_t)p.elem_cell.size();
-    d.n_bases = p.nb;
+    d.n_bases = p.nb_ip;
     d.n_adofs_per_type = p.N;
     d.ip_box = p.ip_box.data();
     d.material = p.material.data();
@@ -1045,10 +1173,37 @@ void oracle_shape(int dim, int n, const double* xi, double* N, double* dN) {
     }
 }
 
-// Quadrature rule: 0 QUAD_2x2, 1 HEX_2x2x2, 2 TRI_7, 3 TET_11.  Writes points [n][dim] and
-// weights [n]; returns n.
+// HEX27 shape functions at n points xi[n][3]: N[n][27], dN[n][3][27].
+void oracle_shape_hex27(int n, const double* xi, double* N, double* dN) {
+    for (int k = 0; k < n; ++k) {
+        if (N) hex27_N(xi + 3 * k, N + (size_t)k * 27);
+        if (dN) {
+            double d[3][27];
+            hex27_dN(xi + 3 * k, d);
+            for (int i = 0; i < 3; ++i)
+                for (int a = 0; a < 27; ++a) dN[((size_t)k * 3 + i) * 27 + a] = d[i][a];
+        }
+    }
+}
+
+// Quadrature rule: 0 QUAD_2x2, 1 HEX_2x2x2, 2 TRI_7, 3 TET_11, 4 HEX_3x3x3, 5 TET_35.  Writes
+// points [n][dim] and weights [n]; returns n.
 int oracle_rule(int rule, double* xi, double* w) {
     switch (rule) {
+        case 4: {
+            double x[27][3];
+            hex3_rule(x, w);
+            for (int q = 0; q < 27; ++q)
+                for (int d = 0; d < 3; ++d) xi[3 * q + d] = x[q][d];
+            return 27;
+        }
+        case 5: {
+            double x[35][3];
+            tet35_rule(x, w);
+            for (int q = 0; q < 35; ++q)
+                for (int d = 0; d < 3; ++d) xi[3 * q + d] = x[q][d];
+            return 35;
+        }
         case 0:
             for (int q = 0; q < 4; ++q) { xi[2 * q] = kQuad2[q][0]; xi[2 * q + 1] = kQuad2[q][1]; w[q] = 1.0; }
             return 4;

@@ -58,10 +58,11 @@ class OracleProblem:
          self.n_t) = [int(x) for x in c]
         self.dim = spec.dim
         self.iwg = spec.iwg
+        self.nb = 27 if getattr(spec, "lagrange_order", 1) == 2 else (8 if spec.dim == 3 else 4)
 
     def setup(self):
         dim = self.dim
-        nb = 8 if dim == 3 else 4
+        nb = self.nb
         nm = 2 if self.iwg == 1 else 1
         out = {
             "elem_cell": np.zeros(self.n_elements, np.int32),
@@ -114,12 +115,18 @@ def assemble_desc(desc, row_ptr, col_idx, u, e0=0, e1=None, n_threads=1):
     return vals, rhs, nip
 
 
-def shape(dim, xi):
-    """Lagrange QUAD4/HEX8 N [n][nb] and dN/dxi [n][dim][nb] at points xi [n][dim]."""
+def shape(dim, xi, nb=None):
+    """Lagrange QUAD4/HEX8 (or HEX27 with nb=27) N [n][nb] and dN/dxi [n][dim][nb] at points xi [n][dim]."""
     lib = load()
-    lib.oracle_shape.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
     xi = np.ascontiguousarray(xi, dtype=np.float64).reshape(-1, dim)
     n = xi.shape[0]
+    if nb == 27:
+        lib.oracle_shape_hex27.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        N = np.zeros((n, 27))
+        dN = np.zeros((n, 3, 27))
+        lib.oracle_shape_hex27(n, _p(xi), _p(N), _p(dN))
+        return N, dN
+    lib.oracle_shape.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
     nb = 8 if dim == 3 else 4
     N = np.zeros((n, nb))
     dN = np.zeros((n, dim, nb))
@@ -127,7 +134,8 @@ def shape(dim, xi):
     return N, dN
 
 
-RULES = {"QUAD_2x2": (0, 2), "HEX_2x2x2": (1, 3), "TRI_7": (2, 2), "TET_11": (3, 3)}
+RULES = {"QUAD_2x2": (0, 2), "HEX_2x2x2": (1, 3), "TRI_7": (2, 2), "TET_11": (3, 3), "HEX_3x3x3": (4, 3),
+         "TET_35": (5, 3)}
 
 
 def rule(name):
@@ -149,9 +157,10 @@ def element(problem, e, u):
                                    ctypes.c_void_p, ctypes.c_void_p]
     lib.oracle_element.restype = ctypes.c_int
     u = np.ascontiguousarray(u, dtype=np.float64)
-    ids = np.zeros(64, np.int32)
-    Ke = np.zeros(64 * 64)
-    Re = np.zeros(64)
+    nmax = 3 * 64   # fields x adofs per type of the largest element
+    ids = np.zeros(nmax, np.int32)
+    Ke = np.zeros(nmax * nmax)
+    Re = np.zeros(nmax)
     na = lib.oracle_element(problem._h, e, _p(u), _p(ids), _p(Ke), _p(Re))
     if na < 0:
         raise RuntimeError("oracle_element: " + lib.oracle_last_error().decode())

@@ -35,7 +35,8 @@ def _test_polynomial(name):
     return ndim, coeffs, expo, integral
 
 
-@pytest.mark.parametrize("fixture,rule", [("integrate_quad_2x2", "QUAD_2x2"), ("integrate_hex_2x2x2", "HEX_2x2x2")])
+@pytest.mark.parametrize("fixture,rule", [("integrate_quad_2x2", "QUAD_2x2"), ("integrate_hex_2x2x2", "HEX_2x2x2"),
+                                          ("integrate_hex_3x3x3", "HEX_3x3x3")])
 def test_gauss_rules_integrate_reference_polynomials(fixture, rule):
     ndim, coeffs, expo, integral = _test_polynomial(fixture)
     xi, w = O.rule(rule)
@@ -50,7 +51,7 @@ def _simplex_monomials(dim, max_deg):
     return [(a, b, c) for a in range(max_deg + 1) for b in range(max_deg + 1 - a) for c in range(max_deg + 1 - a - b)]
 
 
-@pytest.mark.parametrize("rule,dim,degree", [("TRI_7", 2, 5), ("TET_11", 3, 4)])
+@pytest.mark.parametrize("rule,dim,degree", [("TRI_7", 2, 5), ("TET_11", 3, 4), ("TET_35", 3, 6)])
 def test_simplex_rules_exact_to_their_degree(rule, dim, degree):
     xi, w = O.rule(rule)
     vol = 0.5 if dim == 2 else 1.0 / 6.0
@@ -70,7 +71,22 @@ def test_simplex_rules_exact_to_their_degree(rule, dim, degree):
     assert max(errs) > 1e-8
 
 
-def test_hex27_fixture_is_readable():
-    # quadratic Lagrange fixtures ship for the HEX27 IP cell (p=2 B-splines); format check only
-    phi = load_bin("lagrange_hex27_phi")
-    assert phi.shape == (1, 343)
+def test_hex27_shape_functions_match_reference_fixtures():
+    """HEX27 against projects/FEM/INT/test/data/lagrange_hex27_{phihat,phi,dphidxi}.bin (the fixtures of
+    cl_FEM_Interpolation_Function_Lagrange_Hex27.cpp, 1e-12), plus N_a(node_b) = delta_ab."""
+    xi = load_bin("points_3d").T
+    phihat = load_bin("lagrange_hex27_phihat")[:, 0]
+    phi = load_bin("lagrange_hex27_phi")[0]
+    dphi = load_bin("lagrange_hex27_dphidxi")
+    N, dN = O.shape(3, xi, nb=27)
+    assert np.all(np.abs(N.sum(axis=1) - 1.0) < 1e-12)
+    assert np.all(np.abs(dN.sum(axis=2)) < 1e-12)
+    assert np.max(np.abs(N @ phihat - phi)) < 1e-12
+    assert np.max(np.abs(np.einsum("kdb,b->dk", dN, phihat) - dphi)) < 1e-12
+    # node order of get_param_coords: corners, edge midpoints, centre, face centres
+    nodes = np.array([[-1, -1, -1], [1, -1, -1], [1, 1, -1], [-1, 1, -1], [-1, -1, 1], [1, -1, 1], [1, 1, 1],
+                      [-1, 1, 1], [0, -1, -1], [1, 0, -1], [0, 1, -1], [-1, 0, -1], [-1, -1, 0], [1, -1, 0],
+                      [1, 1, 0], [-1, 1, 0], [0, -1, 1], [1, 0, 1], [0, 1, 1], [-1, 0, 1], [0, 0, 0],
+                      [0, 0, -1], [0, 0, 1], [-1, 0, 0], [1, 0, 0], [0, -1, 0], [0, 1, 0]], dtype=float)
+    Nn, _ = O.shape(3, nodes, nb=27)
+    assert np.max(np.abs(Nn - np.eye(27))) < 1e-15

@@ -5,6 +5,8 @@ values, R = K u.
 * quadratic B-spline T rows (bspline_order 2: every Lagrange node carries 2^dim splines of weight
   2^-dim) against the oracle's dense T^T K T: CSR pattern and element topology bit-exact, K and R
   within 1e-12 (FP64) and 1e-6 (mixed precision);
+* HEX27 IP cells (lagrange_order 2: HEX_3x3x3 on whole cells, TET_35 on XTK sub-tets, p = 2
+  B-spline rows of 8 to 27 entries or nodal rows) to the same bars;
 * a nodal set with its lattice hint removed runs the generic path and reproduces the lattice
   stencil path to 1e-12;
 * graph replays of generic sets are bit-identical to eager calls; learned quadrature is refused.
@@ -31,6 +33,16 @@ CASES = {
     "p2_plane_2d_diffusion_phase0": lambda: M.make_spec(2, (9, 7), ls_kind=M.LS_PLANE, plane=[0.6, 0.8, 0.0, -0.55],
                                                         material_phase=0, bspline_order=2),
     "p2_uncut_elasticity": lambda: M.make_spec(3, (3, 4, 2), iwg=M.IWG_STRUC_LINEAR_BULK, bspline_order=2),
+    # HEX27 IP cells (HEX_3x3x3 uncut, TET_35 on the sub-tets): p = 2 B-splines and nodal Lagrange rows
+    "hex27_p2_gyroid_6_elasticity": lambda: M.make_spec(3, (6, 6, 6), iwg=M.IWG_STRUC_LINEAR_BULK,
+                                                        ls_kind=M.LS_GYROID, gyroid_scale=2 * np.pi * 1.5,
+                                                        bspline_order=2, lagrange_order=2),
+    "hex27_p2_sphere_6_diffusion": lambda: M.make_spec(3, (6, 5, 4), spheres=[[0.5, 0.5, 0.5, 0.3]], k_sigma=0.5,
+                                                       bspline_order=2, lagrange_order=2),
+    "hex27_lagrange_sphere_5_elasticity": lambda: M.make_spec(3, (5, 5, 5), iwg=M.IWG_STRUC_LINEAR_BULK,
+                                                              spheres=[[0.4, 0.5, 0.6, 0.35]], lagrange_order=2),
+    "hex27_p2_uncut_elasticity": lambda: M.make_spec(3, (3, 4, 2), iwg=M.IWG_STRUC_LINEAR_BULK, bspline_order=2,
+                                                     lagrange_order=2),
 }
 
 
@@ -53,7 +65,7 @@ def test_generic_path_matches_oracle(case):
     s = Q.setup()
     rp, ci = S.pattern()
     assert np.array_equal(rp, s["row_ptr"]) and np.array_equal(ci, s["col_idx"])
-    nb = 8 if spec.dim == 3 else 4
+    nb = P.desc.n_bases
     nf = spec.dim if spec.iwg == M.IWG_STRUC_LINEAR_BULK else 1
     assert np.array_equal(S.topology(), _oracle_topology(s, nb, nf, Q.n_adofs_per_type))
     u = np.random.default_rng(SEED + 11).uniform(-1.0, 1.0, S.n_rows)

@@ -313,3 +313,97 @@ def test_quadratic_bspline_rigid_rotation_null_space():
     for u in (np.concatenate([rot[:, 0], rot[:, 1], rot[:, 2]]), np.concatenate([np.ones(N), np.zeros(2 * N)])):
         Ku = csr_matvec(rp, ci, vals, u)
         assert np.linalg.norm(Ku) < 1e-12 * np.abs(vals).max() * np.linalg.norm(u) * np.sqrt(len(u))
+
+
+def _hex27_node_coords(setup, n):
+    """Coordinates of the adofs of a nodal HEX27 set (lagrange_order 2, bspline_order 1): the used
+    nodes of the half-step grid in ascending index (x fastest)."""
+    ec = setup["elem_cell"].astype(np.int64)
+    ijk = [ec % n[0], (ec // n[0]) % n[1], ec // (n[0] * n[1])]
+    g = [2 * nn + 1 for nn in n]
+    ids = []
+    for off in product(range(3), repeat=3):
+        q = [2 * ijk[d] + off[d] for d in range(3)]
+        ids.append(q[0] + g[0] * (q[1] + g[1] * q[2]))
+    used = np.unique(np.concatenate(ids))
+    gx, gy, gz = used % g[0], (used // g[0]) % g[1], used // (g[0] * g[1])
+    return np.stack([0.5 * gx / n[0], 0.5 * gy / n[1], 0.5 * gz / n[2]], axis=1)
+
+
+@pytest.mark.parametrize("iwg", [0, 1])
+def test_hex27_quadratic_field_energy_on_planar_cut(iwg):
+    """HEX27 Lagrange cells reproduce a quadratic field; its energy over the material part of a
+    planar cut z < 0.45 (through the cell layer 0.4 < z < 0.6: sub-tets integrated with TET_35,
+    whole cells with HEX_3x3x3) is the closed-form integral of a degree-2 integrand."""
+    n = 5
+    zc = 0.45
+    spec, keep = M.make_spec(3, (n, n, n), iwg=iwg, ls_kind=M.LS_PLANE, plane=[0.0, 0.0, -1.0, zc], lagrange_order=2)
+    P = O.OracleProblem(spec)
+    s = P.setup()
+    X = _hex27_node_coords(s, (n, n, n))
+    N = P.n_adofs_per_type
+    assert X.shape[0] == N and P.n_cut == n * n
+    rp, ci = s["row_ptr"], s["col_idx"]
+    x, y, z = X[:, 0], X[:, 1], X[:, 2]
+    if iwg == 0:
+        u = x ** 2 + 0.5 * y * z                       # grad u = (2x, z/2, y/2)
+        expect = 4.0 / 3.0 * zc + 0.25 * zc ** 3 / 3.0 + 0.25 / 3.0 * zc
+    else:
+        E, nu = 1.0, 0.3
+        u = np.zeros(3 * N)
+        u[:N] = 0.01 * y ** 2                          # gamma_xy = 0.02 y, every other strain 0
+        expect = E / (2 * (1 + nu)) * 4e-4 / 3.0 * zc
+    vals, _ = P.assemble(u)
+    energy = u @ csr_matvec(rp, ci, vals, u)
+    assert abs(energy - expect) < 1e-12 * expect, (energy, expect)
+
+
+@pytest.mark.parametrize("iwg", [0, 1])
+def test_hex27_quadratic_bspline_affine_energy_on_planar_cut(iwg):
+    """HEX27 IP cells with p = 2 B-spline T rows: Greville coefficients reproduce an affine field
+    (the B-spline field is triquadratic per cell, which HEX27 interpolates exactly)."""
+    n = 4
+    nrm0 = [0.3, 0.5, 0.8]
+    d = -0.45 * sum(nrm0) / np.linalg.norm(nrm0)
+    nrm = np.asarray(nrm0) / np.linalg.norm(nrm0)
+    spec, keep = M.make_spec(3, (n, n, n), iwg=iwg, ls_kind=M.LS_PLANE, plane=list(nrm) + [d], bspline_order=2,
+                             lagrange_order=2)
+    P = O.OracleProblem(spec)
+    s = P.setup()
+    V = _halfspace_volume(nrm, d)
+    X = _greville_coords(s, 3, (n, n, n))
+    N = P.n_adofs_per_type
+    assert X.shape[0] == N
+    rp, ci = s["row_ptr"], s["col_idx"]
+    if iwg == 0:
+        g = np.array([0.7, -0.4, 0.2])
+        u = X @ g
+        expect = g @ g * V
+    else:
+        E, nu = 1.0, 0.3
+        u = np.zeros(3 * N)
+        u[:N] = 0.01 * X[:, 1]
+        expect = 0.01 ** 2 * E / (2 * (1 + nu)) * V
+    vals, _ = P.assemble(u)
+    energy = u @ csr_matvec(rp, ci, vals, u)
+    assert abs(energy - expect) < 1e-11 * expect, (energy, expect)
+
+
+@pytest.mark.parametrize("bspline_order", [1, 2])
+def test_hex27_rigid_motions_null_space(bspline_order):
+    n = 3
+    spec, keep = M.make_spec(3, (n, n, n), iwg=1, spheres=[[0.5, 0.5, 0.5, 0.3]], bspline_order=bspline_order,
+                             lagrange_order=2)
+    P = O.OracleProblem(spec)
+    s = P.setup()
+    X = _greville_coords(s, 3, (n, n, n)) if bspline_order == 2 else _hex27_node_coords(s, (n, n, n))
+    N = P.n_adofs_per_type
+    rp, ci = s["row_ptr"], s["col_idx"]
+    vals, _ = P.assemble(np.zeros(3 * N))
+    K = csr_to_scipy(rp, ci, vals, 3 * N)
+    assert abs(K - K.T).max() < 1e-13 * np.abs(vals).max()
+    w = np.array([0.3, -0.2, 0.5])
+    rot = np.cross(w, X - 0.5)
+    for u in (np.concatenate([rot[:, 0], rot[:, 1], rot[:, 2]]), np.concatenate([np.ones(N), np.zeros(2 * N)])):
+        Ku = csr_matvec(rp, ci, vals, u)
+        assert np.linalg.norm(Ku) < 1e-12 * np.abs(vals).max() * np.linalg.norm(u) * np.sqrt(len(u))

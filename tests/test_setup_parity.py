@@ -23,6 +23,13 @@ CASES = {
                                               spheres=[[0.5, 0.5, 0.5, 0.3]], bspline_order=2),
     "p2_circle_16_diff": lambda: M.make_spec(2, (16, 12), spheres=[[0.5, 0.5, 0.0, 0.3]], bspline_order=2,
                                              material_phase=0, k_sigma=0.3),
+    # HEX27 IP cells: p = 2 B-splines at the 27 Lagrange nodes, and the nodal (Lagrange) HEX27 mesh
+    "hex27_p2_gyroid_8_elast": lambda: M.make_spec(3, (8, 8, 8), iwg=M.IWG_STRUC_LINEAR_BULK, ls_kind=M.LS_GYROID,
+                                                   gyroid_scale=2 * np.pi * 2.0, bspline_order=2, lagrange_order=2),
+    "hex27_p2_sphere_6_diff": lambda: M.make_spec(3, (6, 5, 4), spheres=[[0.5, 0.5, 0.5, 0.3]], k_sigma=0.5,
+                                                  bspline_order=2, lagrange_order=2),
+    "hex27_lagrange_sphere_5_elast": lambda: M.make_spec(3, (5, 5, 5), iwg=M.IWG_STRUC_LINEAR_BULK,
+                                                         spheres=[[0.4, 0.5, 0.6, 0.35]], lagrange_order=2),
 }
 
 
