@@ -17,6 +17,7 @@ The learned path's K and R are compared with the FP64 path's for every variant (
 matches the CPU restatement to 1e-12 in the test suite).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--variants 32] [--n 64] [--no-cpu]
+       python bench.py --workload hex27 [--n 32]   (the quadratic HEX27 configuration, its own line)
 """
 import argparse
 import json
@@ -79,7 +80,7 @@ def fit_flops_of(n_points, n_cut, n_moments=23, n_points_rule=27):
     return n_points * (2 * 12 + 2 * n_moments) + n_cut * 3 * 2 * n_moments * n_points_rule
 
 
-def cpu_baseline(P, S, u, budget_s=20.0):
+def cpu_baseline(P, S, u, budget_s=20.0, probe_n=4000, rule="TET_11"):
     """Oracle (FP64 C++ restatement of the reference algorithm) on the host cores, on a strided
     sample of one variant's elements sized to the time budget."""
     from oracle import py as O
@@ -87,7 +88,7 @@ def cpu_baseline(P, S, u, budget_s=20.0):
     rp, ci = S.pattern()
     ne = P.n_elements
     elems = np.arange(ne, dtype=np.int64)
-    probe = elems[:: max(1, ne // 4000)]
+    probe = elems[:: max(1, ne // probe_n)]
     t0 = time.perf_counter()
     O.assemble_list(P.desc, rp, ci, u, probe, n_threads=cores)
     per_elem = (time.perf_counter() - t0) / len(probe)
@@ -103,8 +104,59 @@ def cpu_baseline(P, S, u, budget_s=20.0):
     cut_frac = float(np.mean(np.diff(P.arrays()["ig_ptr"])[sample] > 0))
     return {"value": len(sample) / best, "unit": "elements/s", "cores": cores, "kind": "port",
             "sample": "every %d-th element of variant 0 (%d of %d elements, cut fraction %.3f), FP64 subdivision "
-                      "(TET_11), best of %d runs, %d worker threads with private CSR arrays + row-wise reduction"
-                      % (stride, len(sample), ne, cut_frac, len(runs), cores)}
+                      "(%s), best of %d runs, %d worker threads with private CSR arrays + row-wise reduction"
+                      % (stride, len(sample), ne, cut_frac, rule, len(runs), cores)}
+
+
+def run_hex27(args):
+    """HEX27 IP cells with p = 2 B-splines (the quadratic configuration): 3D elasticity on an n^3
+    lattice of the unit cube cut by a gyroid of period 4 cells, HEX_3x3x3 on whole cells and TET_35
+    on the XTK sub-tets, generic element path.  One JSON line: FP64 and mixed-precision assembly
+    rates, the moment kernel's MFMA rate, mixed-vs-FP64 error, and the oracle on a sample."""
+    n = args.n if args.n_given else 32
+    spec, keep = M.make_spec(3, (n, n, n), iwg=M.IWG_STRUC_LINEAR_BULK, ls_kind=M.LS_GYROID,
+                             gyroid_scale=2 * np.pi * n / 4.0, bspline_order=2, lagrange_order=2)
+    t0 = time.time()
+    P = M.Problem(spec, keep)
+    S = M.Set(P.desc, keepalive=P)
+    log("hex27 setup %.1fs: %d^3, elements %d, cut %d, sub-tets %d, rows %d, nnz %d" % (
+        time.time() - t0, n, P.n_elements, P.n_cut, P.n_ig, S.n_rows, S.nnz))
+    u = np.random.default_rng(SEED + 27).uniform(-1.0, 1.0, S.n_rows)
+    ub, vb, rb = M.DeviceBuffer.from_array(u), M.DeviceBuffer(8 * S.nnz), M.DeviceBuffer(8 * S.n_rows)
+    ig = np.diff(P.arrays()["ig_ptr"])
+    npts = np.where(ig > 0, ig * 35, 27)
+    mfma_flops = float(np.sum(np.ceil(npts / 32.0)) * 36 * 8 * 2 * 16 * 16 * 4)
+    native, _, _ = elasticity_flops(int(npts.sum()), P.n_elements, nb=27)
+    out = {"metric": "elements/s for K+R assembly, HEX27 p=2 elasticity (gyroid cut), generic path",
+           "unit": "elements/s", "n_gpus": 1, "steps": args.steps, "higher_is_better": True,
+           "data": "synthetic", "config": {"workload": "%d^3 HEX27 lattice, p=2 B-splines, gyroid period 4 cells, "
+                                                       "HEX_3x3x3 / TET_35" % n,
+                                           "elements": P.n_elements, "cut_elements": P.n_cut,
+                                           "ig_subcells": P.n_ig, "points": int(npts.sum()), "nnz": S.nnz}}
+    res = {}
+    for prec, name in ((M.PREC_FP64, "fp64"), (M.PREC_MIXED, "mixed")):
+        S.options(prec, M.QUAD_SUBDIVISION)
+        per, st = S.time_assemble(ub, vb, rb, max(3, args.steps))
+        res[name] = (vb.to_array(np.float64, S.nnz), rb.to_array(np.float64, S.n_rows))
+        out[name] = {"value": P.n_elements / (per * 1e-3), "ms_per_call": per,
+                     "ms_element_kernel": st[2], "ms_gather": st[3], "ms_rhs": st[4],
+                     "element_kernel_mfma_TFLOP/s": mfma_flops / (st[2] * 1e-3) / 1e12,
+                     "element_kernel_native_TFLOP/s": native / (st[2] * 1e-3) / 1e12}
+        log("hex27 %s: %.3f ms per call, stages %s" % (name, per, ["%.4f" % x for x in st]))
+    out["value"] = out["mixed"]["value"]
+    out["dtype"] = "f32+f64"
+    out["max_rel_err_K_mixed_vs_fp64"] = float(np.linalg.norm(res["mixed"][0] - res["fp64"][0]) /
+                                               np.linalg.norm(res["fp64"][0]))
+    out["mfma_flops_per_call"] = mfma_flops
+    out["native_flops_per_call"] = native
+    out["roofline"] = {"bound": "mfma", "kernel": "k_generic_hex27<3,double>",
+                       "achieved": out["fp64"]["element_kernel_mfma_TFLOP/s"], "peak": FP64_PEAK_TFLOPS,
+                       "unit": "TFLOP/s", "frac": out["fp64"]["element_kernel_mfma_TFLOP/s"] / FP64_PEAK_TFLOPS,
+                       "traffic": None}
+    if not args.no_cpu:
+        log("hex27 cpu baseline ...")
+        out["cpu_baseline"] = cpu_baseline(P, S, u, budget_s=15.0, probe_n=200, rule="TET_35")
+    print(json.dumps(out), flush=True)
 
 
 def timed_replays(B, steps, warmup):
@@ -129,13 +181,19 @@ def main():
                     help="parallel graph branches (0: autotuned); with 1 every kernel runs alone, so its "
                          "duration under a profiler matches the per-stage event timings")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", choices=("variants", "hex27"), default="variants",
+                    help="variants: the headline batch (default); hex27: the quadratic HEX27 configuration")
     args = ap.parse_args()
+    args.n_given = "--n" in sys.argv
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; --gpus %d is not supported" % args.gpus, file=sys.stderr)
         sys.exit(2)
     if M.device_count() < 1:
         print("bench.py: no GPU visible", file=sys.stderr)
         sys.exit(1)
+    if args.workload == "hex27":
+        run_hex27(args)
+        return
 
     # network: the committed checkpoint, else trained here with the library's kernels
     net = train_lq.load()
