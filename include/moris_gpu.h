@@ -97,6 +97,11 @@ typedef struct mgpu_problem_spec {
                                   1/8, 3/4, 1/8 at a cell-centre coordinate                     */
     int32_t lagrange_order;    /* 1 (or 0): HEX8 / QUAD4 IP cells; 2: HEX27 IP cells (3D only),
                                   whose 27 nodes sit on the half-step grid of the lattice        */
+    int32_t refine_levels;     /* 0: level-0 lattice only; 1 (3D, bspline_order 2): one THB level:
+                                  every level-0 cell the level set intersects is split into 8
+                                  level-1 cells; truncated hierarchical quadratic B-splines
+                                  (cl_HMR_T_Matrix_Base.cpp:127-258); such sets have no lattice
+                                  hint and take the generic path                                */
 } mgpu_problem_spec;
 
 /* Uniform lattice the elements of a set sit on (HMR level 0).  n[0] == 0: no lattice. */
@@ -181,6 +186,11 @@ mgpu_status mgpu_pattern_get(const mgpu_set* set, int32_t* n_rows, int64_t* nnz,
 /* Element topology (cl_MSI_Equation_Object.cpp:505-554): sorted unique adof ids per dof type,
    concatenated over types.  ids: [n_elements][n_fields*n_bases] (host, owned by the handle).  */
 mgpu_status mgpu_element_topology(const mgpu_set* set, int32_t* n_per_element, const int32_t** ids);
+/* The same for sets whose elements differ in size (e.g. truncated hierarchical rows), one call
+   per set instead of get_element_topology per object (cl_MSI_Solver_Interface.hpp:369-385):
+   element e's ids are ids[offsets[e] .. offsets[e+1]); offsets [n_elements + 1] (host arrays
+   owned by the handle).                                                                         */
+mgpu_status mgpu_element_topology_csr(const mgpu_set* set, const int64_t** offsets, const int32_t** ids);
 
 /* K+R assembly of all elements of the set into caller-owned DEVICE arrays:
    csr_vals[nnz] and rhs[n_rows] are overwritten (zeroed then summed, cl_DLA_Linear_Problem.cpp

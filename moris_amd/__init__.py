@@ -52,6 +52,7 @@ class ProblemSpec(ctypes.Structure):
         ("seed", ctypes.c_uint64),
         ("bspline_order", ctypes.c_int32),
         ("lagrange_order", ctypes.c_int32),
+        ("refine_levels", ctypes.c_int32),
     ]
 
 
@@ -85,7 +86,7 @@ class SetDesc(ctypes.Structure):
 EXPORTED = [
     "mgpu_problem_build", "mgpu_problem_desc", "mgpu_problem_counts", "mgpu_problem_destroy",
     "mgpu_set_create", "mgpu_set_destroy", "mgpu_set_options", "mgpu_set_stream", "mgpu_pattern_get",
-    "mgpu_element_topology", "mgpu_set_assemble", "mgpu_set_assemble_host", "mgpu_set_stats",
+    "mgpu_element_topology", "mgpu_element_topology_csr", "mgpu_set_assemble", "mgpu_set_assemble_host", "mgpu_set_stats",
     "mgpu_device_count", "mgpu_malloc", "mgpu_free", "mgpu_memcpy_htod", "mgpu_memcpy_dtoh",
     "mgpu_synchronize", "mgpu_set_time_assemble", "mgpu_last_error_string",
     "mgpu_set_tuning", "mgpu_set_get_tuning", "mgpu_set_tuning_variants", "mgpu_set_autotune",
@@ -120,6 +121,8 @@ def load_library(path=LIB_PATH):
     lib.mgpu_pattern_get.argtypes = [vp, _c_i32p, ctypes.POINTER(ctypes.c_int64),
                                      ctypes.POINTER(_c_i32p), ctypes.POINTER(_c_i32p)]
     lib.mgpu_element_topology.argtypes = [vp, _c_i32p, ctypes.POINTER(_c_i32p)]
+    lib.mgpu_element_topology_csr.argtypes = [vp, ctypes.POINTER(ctypes.POINTER(ctypes.c_int64)),
+                                              ctypes.POINTER(_c_i32p)]
     lib.mgpu_set_assemble.argtypes = [vp, vp, vp, vp]
     lib.mgpu_set_assemble_host.argtypes = [vp, _c_f64p, _c_f64p, _c_f64p]
     lib.mgpu_set_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
@@ -158,7 +161,7 @@ def _ptr(a, ctype):
 
 def make_spec(dim, n, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), iwg=IWG_SPATIALDIFF_BULK, spheres=None,
               ls_kind=None, gyroid_scale=1.0, plane=(0.0, 0.0, 0.0, 0.0), material_phase=1, E=1.0, nu=0.3,
-              k_mu=0.0, k_sigma=0.0, seed=20261202, bspline_order=1, lagrange_order=1):
+              k_mu=0.0, k_sigma=0.0, seed=20261202, bspline_order=1, lagrange_order=1, refine_levels=0):
     """Build a ProblemSpec; returns (spec, keepalive) -- keep the second value alive with the spec."""
     s = ProblemSpec()
     s.dim = dim
@@ -182,6 +185,7 @@ def make_spec(dim, n, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), iwg=IWG_SPATIALDIF
     s.seed = seed
     s.bspline_order = bspline_order
     s.lagrange_order = lagrange_order
+    s.refine_levels = refine_levels
     return s, keep
 
 
@@ -316,6 +320,18 @@ class Set:
         if ne == 0:
             return np.zeros((0, n.value), dtype=np.int32)
         return np.ctypeslib.as_array(p, shape=(ne * n.value,)).reshape(ne, n.value).copy()
+
+    def topology_lists(self):
+        """Element topologies as a list of arrays (elements may differ in size)."""
+        off = ctypes.POINTER(ctypes.c_int64)()
+        p = _c_i32p()
+        _check(load_library().mgpu_element_topology_csr(self._h, ctypes.byref(off), ctypes.byref(p)), "topology_csr")
+        ne = self.desc.n_elements
+        if ne == 0:
+            return []
+        o = np.ctypeslib.as_array(off, shape=(ne + 1,)).copy()
+        ids = np.ctypeslib.as_array(p, shape=(int(o[-1]),)).copy() if o[-1] > 0 else np.zeros(0, np.int32)
+        return [ids[o[e]:o[e + 1]] for e in range(ne)]
 
     def compute_jacobian_and_residual(self, u_adof):
         """K values (CSR, pattern()) and R for adof vector u (host arrays)."""

@@ -897,6 +897,13 @@ extern "C" mgpu_status mgpu_element_topology(const mgpu_set* set, int32_t* n_per
     return MGPU_OK;
 }
 
+extern "C" mgpu_status mgpu_element_topology_csr(const mgpu_set* set, const int64_t** offsets, const int32_t** ids) {
+    if (!set) return set_error(MGPU_ERR_INVALID_ARG, "null set");
+    if (offsets) *offsets = set->h.topo_ptr.data();
+    if (ids) *ids = set->h.topo.data();
+    return MGPU_OK;
+}
+
 extern "C" mgpu_status mgpu_set_assemble(mgpu_set* set, const double* u, double* vals, double* rhs) {
     MGPU_GUARD_BEGIN
     if (!set) return set_error(MGPU_ERR_INVALID_ARG, "null set");

@@ -207,11 +207,13 @@ __global__ __launch_bounds__(64) void k_generic_element(GenView g) {
 // accumulate on MFMA, each wavefront owning a 3 x 3 block of 16 x 16 tiles: FP64
 // (v_mfma_f64_16x16x4f64) on the plain path; on the mixed path FP32 products
 // (v_mfma_f32_16x16x4f32) summed per 32-point chunk and added into FP64 registers.  Then K_p from
-// M and the material in LDS, and K_e = T^T K_p T per (dof type, dof type) block in FP64.
+// M and the material in LDS, and K_e = T^T K_p T per (dof type, dof type) block in FP64.  The LDS
+// (80 KB: K_p, T and one block of K_p T for up to 64 adofs per type, e.g. truncated hierarchical
+// rows) uses gfx950's 160 KB per workgroup.
 constexpr int kH27Chunk = 32;                    // points per staging round
 constexpr int kH27Cols = 96;                     // 81 gradient columns padded to 6 tiles
 constexpr int kH27PtStride = 20;                 // w*, then l[3][3], dl[3][3] of the point
-constexpr int kH27Lds = 81 * 81 + 2 * 27 * 27;   // doubles: K_p, then T and W of one block
+constexpr int kH27Lds = 81 * 81 + 2 * 27 * kHex27MaxAdofsPerType;   // doubles: K_p, T, W of one block (80 KB)
 static_assert(2 * kH27Chunk * kH27Cols + kH27Chunk * kH27PtStride <= kH27Lds, "staging fits the K_p region");
 
 typedef double doublex4_t __attribute__((ext_vector_type(4)));
@@ -393,7 +395,7 @@ __global__ __launch_bounds__(256) void k_generic_hex27(GenView g) {
     }
     // dense element-local T (27 x na_t, the same for every dof type)
     double* Tt = lds + NG * NG;
-    double* W = Tt + NB * NB;
+    double* W = Tt + NB * kHex27MaxAdofsPerType;
     const int na_t = g.na_t[e];
     const int NA = NF * na_t;
     for (int idx = tid; idx < NB * na_t; idx += 256) Tt[idx] = 0.0;

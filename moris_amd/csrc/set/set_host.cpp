@@ -231,6 +231,8 @@ mgpu_status build_set_host(const mgpu_set_desc& d, SetHost& h, std::string* err)
             for (int b = 0; b < nb; ++b) h.topo[(size_t)e * na + f * nb + b] = s[b] + f * N;
     });
     if (dup.load()) { *err = "an element references one adof from two bases"; return MGPU_ERR_UNSUPPORTED; }
+    h.topo_ptr.resize((size_t)h.n_el + 1);
+    for (int64_t e = 0; e <= h.n_el; ++e) h.topo_ptr[(size_t)e] = e * na;
 
     // adof -> elements (counting sort)
     std::vector<int32_t> a_ptr((size_t)N + 1, 0), a_el;

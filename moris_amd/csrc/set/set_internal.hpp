@@ -14,8 +14,9 @@ namespace mgpu {
 // Largest element topology per dof type of the generic path (element-local T and the T^T K T
 // products are staged in LDS at this size).
 constexpr int kGenericMaxAdofsPerType = 64;
-// The same for HEX27 IP cells (27 bases): level-0 quadratic B-splines or Lagrange nodes give 27.
-constexpr int kHex27MaxAdofsPerType = 27;
+// The same for HEX27 IP cells (27 bases): level-0 quadratic B-splines or Lagrange nodes give 27,
+// truncated hierarchical rows up to about 40.
+constexpr int kHex27MaxAdofsPerType = 64;
 
 // Host-side products of set creation (pure C++, no HIP).
 struct SetHost {
@@ -49,7 +50,7 @@ struct SetHost {
     bool generic = false;
     int32_t na_t_max = 0;                // largest per-type element topology
     std::vector<int32_t> na_t;           // [n_el] adofs per dof type of each element
-    std::vector<int64_t> topo_ptr;       // [n_el + 1] into topo (nf * na_t per element)
+    std::vector<int64_t> topo_ptr;       // [n_el + 1] into topo (nf * na_t per element; both paths)
     std::vector<int64_t> ke_off;         // [n_el + 1] K_e offsets, (nf * na_t)^2 per element
     std::vector<int32_t> t_col;          // [n T entries] element-local column of each T entry
     std::vector<int64_t> gather_ptr;     // [nnz + 1]

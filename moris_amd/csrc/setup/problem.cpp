@@ -16,6 +16,7 @@
 //                                  (cl_MTK_Interpolation_Function_Lagrange_Hex27.hpp node order);
 //                                  the cut is that of the cell's box, as for HEX8
 #include <algorithm>
+#include <memory>
 #include <cstring>
 #include <string>
 #include <thread>
@@ -45,6 +46,9 @@ struct mgpu_problem {
     std::vector<int32_t> cell_elem;
 
     std::vector<int32_t> elem_cell;
+    std::vector<int8_t> elem_level;      // 0, or 1 for a THB level-1 cell
+    std::vector<int64_t> elem_lcell;     // cell index on the element's level lattice
+    std::vector<char> refined;           // THB: level-0 cells split into 8 level-1 cells
     std::vector<double> ip_box, material, vertex_phi;
     std::vector<int32_t> t_ptr, t_adof;
     std::vector<double> t_weight;
@@ -288,8 +292,57 @@ extern "C" mgpu_status mgpu_problem_build(const mgpu_problem_spec* spec, mgpu_pr
             });
         for (auto& t : th) t.join();
     }
-    // 2. cut every cell (chunks in lattice order)
-    std::vector<std::vector<CellOut>> parts(T);
+    // THB: one refinement level on the intersected level-0 cells, cut on the level-1 lattice
+    if (spec->refine_levels < 0 || spec->refine_levels > 1)
+        return mgpu::set_error(MGPU_ERR_UNSUPPORTED, "refine_levels must be 0 or 1");
+    const bool thb = spec->refine_levels == 1;
+    if (thb && (p->dim != 3 || spec->bspline_order != 2))
+        return mgpu::set_error(MGPU_ERR_UNSUPPORTED, "THB refinement is implemented for 3D sets with bspline_order 2");
+    std::unique_ptr<mgpu_problem> fine;
+    if (thb) {
+        p->refined.assign((size_t)p->n_cells, 0);
+        for (int64_t c = 0; c < p->n_cells; ++c)
+            p->refined[(size_t)c] = cell_intersected(*p, (int)(c % p->n[0]), (int)((c / p->n[0]) % p->n[1]),
+                                                     (int)(c / ((int64_t)p->n[0] * p->n[1])));
+        fine = std::make_unique<mgpu_problem>();
+        mgpu_problem& f = *fine;
+        f.spec = p->spec;
+        f.ls = p->ls;   // spheres stay owned by p
+        f.dim = 3;
+        f.nb = p->nb;
+        f.nb_ip = p->nb_ip;
+        for (int d = 0; d < 3; ++d) {
+            f.n[d] = 2 * p->n[d];
+            f.lo[d] = p->lo[d];
+            f.h[d] = 0.5 * p->h[d];
+        }
+        f.n_cells = (int64_t)f.n[0] * f.n[1] * f.n[2];
+        f.n_nodes = (int64_t)(f.n[0] + 1) * (f.n[1] + 1) * (f.n[2] + 1);
+        f.n_faces = (int64_t)(f.n[0] + 1) * f.n[1] * f.n[2] + (int64_t)f.n[0] * (f.n[1] + 1) * f.n[2] +
+                    (int64_t)f.n[0] * f.n[1] * (f.n[2] + 1);
+        if (f.n_nodes + f.n_faces + f.n_cells >= (int64_t)UINT32_MAX)
+            return mgpu::set_error(MGPU_ERR_INVALID_ARG, "level-1 grid too large for 32-bit ids");
+        f.node_phi.resize((size_t)f.n_nodes);
+        std::vector<std::thread> th;
+        for (int t = 0; t < T; ++t)
+            th.emplace_back([&, t]() {
+                const int64_t a = f.n_nodes * t / T, b = f.n_nodes * (t + 1) / T;
+                for (int64_t nid = a; nid < b; ++nid) {
+                    double x[3];
+                    node_coords(f, (int)(nid % (f.n[0] + 1)), (int)((nid / (f.n[0] + 1)) % (f.n[1] + 1)),
+                                (int)(nid / ((int64_t)(f.n[0] + 1) * (f.n[1] + 1))), x);
+                    f.node_phi[(size_t)nid] = f.ls(x);
+                }
+            });
+        for (auto& t : th) t.join();
+    }
+    // 2. cut every leaf cell (chunks of level-0 cells in lattice order; a refined cell contributes
+    // its 8 level-1 cells, x fastest)
+    struct LeafOut {
+        int level;
+        CellOut co;
+    };
+    std::vector<std::vector<LeafOut>> parts(T);
     std::vector<int64_t> voids(T, 0);
     std::vector<int> rcs(T, MGPU_OK);
     std::vector<std::string> errs(T);
@@ -298,7 +351,30 @@ extern "C" mgpu_status mgpu_problem_build(const mgpu_problem_spec* spec, mgpu_pr
         for (int t = 0; t < T; ++t)
             th.emplace_back([&, t]() {
                 const int64_t a = p->n_cells * t / T, b = p->n_cells * (t + 1) / T;
-                rcs[t] = process_cells(*p, a, b, parts[t], voids[t], errs[t]);
+                std::vector<CellOut> tmp;
+                if (!thb) {
+                    rcs[t] = process_cells(*p, a, b, tmp, voids[t], errs[t]);
+                    for (auto& co : tmp) parts[t].push_back(LeafOut{0, std::move(co)});
+                    return;
+                }
+                for (int64_t c = a; c < b && rcs[t] == MGPU_OK; ++c) {
+                    tmp.clear();
+                    if (!p->refined[(size_t)c]) {
+                        rcs[t] = process_cells(*p, c, c + 1, tmp, voids[t], errs[t]);
+                        for (auto& co : tmp) parts[t].push_back(LeafOut{0, std::move(co)});
+                        continue;
+                    }
+                    const int i = (int)(c % p->n[0]), j = (int)((c / p->n[0]) % p->n[1]),
+                              k = (int)(c / ((int64_t)p->n[0] * p->n[1]));
+                    for (int cz = 0; cz < 2 && rcs[t] == MGPU_OK; ++cz)
+                        for (int cy = 0; cy < 2 && rcs[t] == MGPU_OK; ++cy)
+                            for (int cx = 0; cx < 2 && rcs[t] == MGPU_OK; ++cx) {
+                                const int64_t fc = (2 * i + cx) + (int64_t)fine->n[0] *
+                                                   ((2 * j + cy) + (int64_t)fine->n[1] * (2 * k + cz));
+                                rcs[t] = process_cells(*fine, fc, fc + 1, tmp, voids[t], errs[t]);
+                            }
+                    for (auto& co : tmp) parts[t].push_back(LeafOut{1, std::move(co)});
+                }
             });
         for (auto& t : th) t.join();
     }
@@ -310,12 +386,14 @@ extern "C" mgpu_status mgpu_problem_build(const mgpu_problem_spec* spec, mgpu_pr
     for (int t = 0; t < T; ++t) {
         n_el += (int64_t)parts[t].size();
         p->n_void += voids[t];
-        for (const auto& co : parts[t]) n_ig += co.n_sub;
+        for (const auto& lo : parts[t]) n_ig += lo.co.n_sub;
     }
     const int d2 = 2 * p->dim;
     const int nv_ig = p->dim == 3 ? 4 : 3;
     p->cell_elem.assign((size_t)p->n_cells, -1);
     p->elem_cell.resize((size_t)n_el);
+    p->elem_level.resize((size_t)n_el);
+    p->elem_lcell.resize((size_t)n_el);
     p->ip_box.resize((size_t)n_el * d2);
     p->vertex_phi.resize((size_t)n_el * p->nb_ip);
     p->ig_ptr.resize((size_t)n_el + 1);
@@ -324,36 +402,44 @@ extern "C" mgpu_status mgpu_problem_build(const mgpu_problem_spec* spec, mgpu_pr
     p->material.resize((size_t)n_el * (diff ? 1 : 2));
     int64_t e = 0, ig = 0;
     for (int t = 0; t < T; ++t) {
-        for (const auto& co : parts[t]) {
+        for (const auto& lo : parts[t]) {
+            const CellOut& co = lo.co;
+            const mgpu_problem& L = lo.level == 0 ? *p : *fine;
             const int64_t c = co.cell;
-            const int i = (int)(c % p->n[0]);
-            const int j = (int)((c / p->n[0]) % p->n[1]);
-            const int k = p->dim == 3 ? (int)(c / ((int64_t)p->n[0] * p->n[1])) : 0;
+            const int i = (int)(c % L.n[0]);
+            const int j = (int)((c / L.n[0]) % L.n[1]);
+            const int k = p->dim == 3 ? (int)(c / ((int64_t)L.n[0] * L.n[1])) : 0;
             const int ijk[3] = {i, j, k};
-            p->cell_elem[(size_t)c] = (int32_t)e;
-            p->elem_cell[(size_t)e] = (int32_t)c;
+            // level-0 cell of the element (its parent for a level-1 cell)
+            const int64_t c0 = lo.level == 0 ? c : (i / 2) + (int64_t)p->n[0] * ((j / 2) + (int64_t)p->n[1] * (k / 2));
+            if (lo.level == 0) p->cell_elem[(size_t)c] = (int32_t)e;
+            p->elem_cell[(size_t)e] = (int32_t)c0;
+            p->elem_level[(size_t)e] = (int8_t)lo.level;
+            p->elem_lcell[(size_t)e] = c;
             for (int d = 0; d < p->dim; ++d) {
-                p->ip_box[(size_t)e * d2 + d] = p->lo[d] + (double)ijk[d] * p->h[d];
-                p->ip_box[(size_t)e * d2 + p->dim + d] = p->h[d];
+                p->ip_box[(size_t)e * d2 + d] = L.lo[d] + (double)ijk[d] * L.h[d];
+                p->ip_box[(size_t)e * d2 + p->dim + d] = L.h[d];
             }
             if (p->nb_ip == 27) {
                 // level set at the 27 HEX27 nodes (the cut itself uses the 8 corner values)
                 for (int v = 0; v < 27; ++v) {
                     double x[3];
-                    for (int d = 0; d < 3; ++d) x[d] = p->lo[d] + ((double)ijk[d] + 0.5 * kHex27Half[v][d]) * p->h[d];
+                    for (int d = 0; d < 3; ++d) x[d] = L.lo[d] + ((double)ijk[d] + 0.5 * kHex27Half[v][d]) * L.h[d];
                     p->vertex_phi[(size_t)e * 27 + v] = p->ls(x);
                 }
             } else {
                 for (int v = 0; v < p->nb; ++v) {
-                    const int64_t nid = p->dim == 3 ? node_index(*p, i + kHex8Off[v][0], j + kHex8Off[v][1], k + kHex8Off[v][2])
-                                                    : node_index(*p, i + kQuad4Off[v][0], j + kQuad4Off[v][1], 0);
-                    p->vertex_phi[(size_t)e * p->nb + v] = p->node_phi[(size_t)nid];
+                    const int64_t nid = p->dim == 3 ? node_index(L, i + kHex8Off[v][0], j + kHex8Off[v][1], k + kHex8Off[v][2])
+                                                    : node_index(L, i + kQuad4Off[v][0], j + kQuad4Off[v][1], 0);
+                    p->vertex_phi[(size_t)e * p->nb + v] = L.node_phi[(size_t)nid];
                 }
             }
             if (diff) {
+                // per-element conductivity seeded by the cell id (level-1 cells: n_cells + fine id)
+                const uint64_t sid = lo.level == 0 ? (uint64_t)c : (uint64_t)(p->n_cells + c);
                 p->material[(size_t)e] = spec->k_sigma == 0.0
                                              ? std::exp(spec->k_mu)
-                                             : std::exp(spec->k_mu + spec->k_sigma * normal_at(spec->seed, (uint64_t)c));
+                                             : std::exp(spec->k_mu + spec->k_sigma * normal_at(spec->seed, sid));
             } else {
                 p->material[(size_t)e * 2] = spec->E;
                 p->material[(size_t)e * 2 + 1] = spec->nu;
@@ -388,7 +474,143 @@ extern "C" mgpu_status mgpu_problem_build(const mgpu_problem_spec* spec, mgpu_pr
         ijk[2] = p->dim == 3 ? (int)(c / ((int64_t)p->n[0] * p->n[1])) : 0;
     };
     const size_t n_rows = (size_t)n_el * p->nb_ip;
-    if (order == 1 && p->nb_ip != 27) {
+    if (thb) {
+        // Truncated hierarchical quadratic B-splines, two levels (T_Matrix_Base::
+        // calculate_truncated_t_matrix, cl_HMR_T_Matrix_Base.cpp:127-258, applied at the Lagrange
+        // nodes as in evaluate, :306-457).  Level-0 spline s (n + 2 per axis, support cells s - 2 .. s)
+        // is active unless every cell of its support is refined; level-1 spline t (2n + 2 per axis,
+        // support level-1 cells t - 2 .. t) is active if every cell of its support lies in a refined
+        // level-0 cell.  On a level-1 cell an active level-0 spline is replaced by its truncation:
+        // the sum of its inactive level-1 children t = 2 s - 2 + k (k = 0..3 per axis, two-scale
+        // weights 1/4, 3/4, 3/4, 1/4).  On an unrefined cell every level-0 spline is active and no
+        // level-1 spline is non-zero.  Adofs: used level-0 splines in ascending index, then used
+        // level-1 splines in ascending index; each row sorted by adof.
+        const int64_t n0[3] = {p->n[0], p->n[1], p->n[2]};
+        const int64_t s0[3] = {n0[0] + 2, n0[1] + 2, n0[2] + 2};
+        const int64_t s1[3] = {2 * n0[0] + 2, 2 * n0[1] + 2, 2 * n0[2] + 2};
+        if (s1[0] * s1[1] * s1[2] >= INT32_MAX) return mgpu::set_error(MGPU_ERR_INVALID_ARG, "too many splines");
+        auto refined0 = [&](int64_t i, int64_t j, int64_t k) { return p->refined[(size_t)(i + n0[0] * (j + n0[1] * k))] != 0; };
+        std::vector<char> act0((size_t)(s0[0] * s0[1] * s0[2]), 0), act1((size_t)(s1[0] * s1[1] * s1[2]), 0);
+        for (int64_t c = 0; c < s0[2]; ++c)
+            for (int64_t b = 0; b < s0[1]; ++b)
+                for (int64_t a = 0; a < s0[0]; ++a) {
+                    bool any_unrefined = false;
+                    for (int64_t k = std::max<int64_t>(0, c - 2); k <= std::min(n0[2] - 1, c) && !any_unrefined; ++k)
+                        for (int64_t j = std::max<int64_t>(0, b - 2); j <= std::min(n0[1] - 1, b) && !any_unrefined; ++j)
+                            for (int64_t i = std::max<int64_t>(0, a - 2); i <= std::min(n0[0] - 1, a); ++i)
+                                if (!refined0(i, j, k)) { any_unrefined = true; break; }
+                    act0[(size_t)(a + s0[0] * (b + s0[1] * c))] = any_unrefined;
+                }
+        for (int64_t c = 0; c < s1[2]; ++c)
+            for (int64_t b = 0; b < s1[1]; ++b)
+                for (int64_t a = 0; a < s1[0]; ++a) {
+                    bool all_refined = true;
+                    for (int64_t k = std::max<int64_t>(0, c - 2); k <= std::min(2 * n0[2] - 1, c) && all_refined; ++k)
+                        for (int64_t j = std::max<int64_t>(0, b - 2); j <= std::min(2 * n0[1] - 1, b) && all_refined; ++j)
+                            for (int64_t i = std::max<int64_t>(0, a - 2); i <= std::min(2 * n0[0] - 1, a); ++i)
+                                if (!refined0(i / 2, j / 2, k / 2)) { all_refined = false; break; }
+                    act1[(size_t)(a + s1[0] * (b + s1[1] * c))] = all_refined;
+                }
+        // per-axis splines of a lattice with its half-step coordinate g: (index, value)
+        struct AxisRow { int n; int64_t s[3]; double w[3]; };
+        auto axis_row = [](int64_t g, AxisRow& r) {
+            const int64_t m = g / 2;
+            if ((g & 1) == 0) {
+                r.n = 2; r.s[0] = m; r.s[1] = m + 1; r.w[0] = r.w[1] = 0.5;
+            } else {
+                r.n = 3; r.s[0] = m; r.s[1] = m + 1; r.s[2] = m + 2; r.w[0] = 0.125; r.w[1] = 0.75; r.w[2] = 0.125;
+            }
+        };
+        const double two_scale[4] = {0.25, 0.75, 0.75, 0.25};
+        // row of basis v of element ee: (level, spline index, weight), |weight| > 1e-12
+        struct Ent { int level; int64_t sp; double w; };
+        auto row = [&](int64_t ee, int v, std::vector<Ent>& out) {
+            out.clear();
+            const int lvl = p->elem_level[(size_t)ee];
+            const int64_t nl[3] = {lvl ? 2 * n0[0] : n0[0], lvl ? 2 * n0[1] : n0[1], lvl ? 2 * n0[2] : n0[2]};
+            const int64_t c = p->elem_lcell[(size_t)ee];
+            const int64_t ijk[3] = {c % nl[0], (c / nl[0]) % nl[1], c / (nl[0] * nl[1])};
+            AxisRow ax[3];
+            for (int d = 0; d < 3; ++d) {
+                const int off = p->nb_ip == 27 ? kHex27Half[v][d] : 2 * kHex8Off[v][d];
+                axis_row(2 * ijk[d] + off, ax[d]);
+            }
+            if (lvl == 0) {
+                for (int zc = 0; zc < ax[2].n; ++zc)
+                    for (int yc = 0; yc < ax[1].n; ++yc)
+                        for (int xc = 0; xc < ax[0].n; ++xc)
+                            out.push_back({0, ax[0].s[xc] + s0[0] * (ax[1].s[yc] + s0[1] * ax[2].s[zc]),
+                                           ax[0].w[xc] * ax[1].w[yc] * ax[2].w[zc]});
+                return;
+            }
+            // level-1 splines at the node
+            for (int zc = 0; zc < ax[2].n; ++zc)
+                for (int yc = 0; yc < ax[1].n; ++yc)
+                    for (int xc = 0; xc < ax[0].n; ++xc) {
+                        const int64_t t = ax[0].s[xc] + s1[0] * (ax[1].s[yc] + s1[1] * ax[2].s[zc]);
+                        if (act1[(size_t)t]) out.push_back({1, t, ax[0].w[xc] * ax[1].w[yc] * ax[2].w[zc]});
+                    }
+            // truncated active level-0 splines of the parent cell
+            const int64_t P[3] = {ijk[0] / 2, ijk[1] / 2, ijk[2] / 2};
+            for (int64_t sc = P[2]; sc <= P[2] + 2; ++sc)
+                for (int64_t sb = P[1]; sb <= P[1] + 2; ++sb)
+                    for (int64_t sa = P[0]; sa <= P[0] + 2; ++sa) {
+                        const int64_t s = sa + s0[0] * (sb + s0[1] * sc);
+                        if (!act0[(size_t)s]) continue;
+                        const int64_t sd[3] = {sa, sb, sc};
+                        double val = 0.0;
+                        for (int zc = 0; zc < ax[2].n; ++zc)
+                            for (int yc = 0; yc < ax[1].n; ++yc)
+                                for (int xc = 0; xc < ax[0].n; ++xc) {
+                                    const int64_t td[3] = {ax[0].s[xc], ax[1].s[yc], ax[2].s[zc]};
+                                    double w = 1.0;
+                                    for (int d = 0; d < 3 && w != 0.0; ++d) {
+                                        const int64_t kk = td[d] - (2 * sd[d] - 2);
+                                        w = (kk >= 0 && kk <= 3) ? w * two_scale[kk] : 0.0;
+                                    }
+                                    if (w == 0.0) continue;
+                                    if (act1[(size_t)(td[0] + s1[0] * (td[1] + s1[1] * td[2]))]) continue;
+                                    val += w * (ax[0].w[xc] * ax[1].w[yc] * ax[2].w[zc]);
+                                }
+                        if (std::fabs(val) > 1e-12) out.push_back({0, s, val});
+                    }
+        };
+        std::vector<int32_t> id0((size_t)(s0[0] * s0[1] * s0[2]), -1), id1((size_t)(s1[0] * s1[1] * s1[2]), -1);
+        std::vector<Ent> r;
+        size_t n_entries = 0;
+        for (int64_t ee = 0; ee < n_el; ++ee)
+            for (int v = 0; v < p->nb_ip; ++v) {
+                row(ee, v, r);
+                n_entries += r.size();
+                for (const Ent& q : r) (q.level == 0 ? id0 : id1)[(size_t)q.sp] = 0;
+            }
+        int32_t next = 0;
+        for (auto& a : id0)
+            if (a == 0) a = next++;
+        for (auto& a : id1)
+            if (a == 0) a = next++;
+        p->n_adofs_per_type = next;
+        if (n_entries >= (size_t)INT32_MAX) return mgpu::set_error(MGPU_ERR_INVALID_ARG, "too many T entries");
+        p->t_ptr.resize(n_rows + 1);
+        p->t_adof.resize(n_entries);
+        p->t_weight.resize(n_entries);
+        size_t pos = 0;
+        std::vector<std::pair<int32_t, double>> sorted;
+        for (int64_t ee = 0; ee < n_el; ++ee)
+            for (int v = 0; v < p->nb_ip; ++v) {
+                row(ee, v, r);
+                sorted.clear();
+                for (const Ent& q : r) sorted.push_back({(q.level == 0 ? id0 : id1)[(size_t)q.sp], q.w});
+                std::sort(sorted.begin(), sorted.end());
+                p->t_ptr[(size_t)ee * p->nb_ip + v] = (int32_t)pos;
+                for (const auto& q : sorted) {
+                    p->t_adof[pos] = q.first;
+                    p->t_weight[pos] = q.second;
+                    ++pos;
+                }
+            }
+        p->t_ptr[n_rows] = (int32_t)pos;
+    } else if (order == 1 && p->nb_ip != 27) {
         // p = 1: one adof per used lattice node, ascending node index; T rows of weight 1
         std::vector<int32_t> node_adof((size_t)p->n_nodes, -1);
         for (int64_t ee = 0; ee < n_el; ++ee) {
@@ -545,6 +767,8 @@ extern "C" mgpu_status mgpu_problem_desc(const mgpu_problem* p, mgpu_set_desc* d
         d->lattice.lo[k] = p->lo[k];
         d->lattice.h[k] = p->h[k];
     }
+    // a THB set's elements are not the cells of one lattice: no lattice hint (generic path)
+    if (!p->refined.empty()) d->lattice.n[0] = 0;
     d->elem_cell = p->elem_cell.data();
     d->vertex_phi = p->vertex_phi.data();
     return MGPU_OK;

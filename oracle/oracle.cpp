@@ -363,6 +363,11 @@ struct OracleProblem {
     LS ls;
     int dim = 3, nb = 8, nf = 1, nm = 1;
     int nb_ip = 8;   // IP-cell bases: nb (the box vertices), or 27 for HEX27
+    // THB (one refinement level): refined level-0 cells, each element's level and its cell on
+    // that level's lattice
+    std::vector<char> refined;
+    std::vector<int8_t> elem_level;
+    std::vector<int64_t> elem_lcell;
     int n[3] = {1, 1, 1};
     double lo[3] = {0, 0, 0}, h[3] = {1, 1, 1};
     int64_t n_cells = 0, n_nodes = 0;
@@ -638,6 +643,120 @@ void bspline2_rows(OracleProblem& p) {
         }
 }
 
+// THB T rows (two levels), in the reference's flow: per element the B-spline T-matrix of the
+// element's bases -- its own level's 27 splines (identity columns for the active ones) and, on a
+// level-1 element, the parent's active level-0 splines truncated to their inactive level-1
+// children among the element's splines (calculate_truncated_t_matrix,
+// cl_HMR_T_Matrix_Base.cpp:127-258, two-scale weights 1/4, 3/4, 3/4, 1/4 per axis) -- multiplied by
+// the Lagrange matrix L (the element's 27 splines at its Lagrange nodes), T = L B, entries with
+// |T| <= 1e-12 dropped (evaluate, :306-457).  Activity: a level-0 spline is active unless all
+// level-0 cells of its support are refined; a level-1 spline is active if all level-1 cells of its
+// support lie in refined level-0 cells.  Adofs: used level-0 splines ascending, then level-1.
+void thb_rows(OracleProblem& p) {
+    const int64_t n0[3] = {p.n[0], p.n[1], p.n[2]};
+    const int64_t ns0[3] = {n0[0] + 2, n0[1] + 2, n0[2] + 2};
+    const int64_t ns1[3] = {2 * n0[0] + 2, 2 * n0[1] + 2, 2 * n0[2] + 2};
+    auto is_refined = [&](int64_t i, int64_t j, int64_t k) { return p.refined[i + n0[0] * (j + n0[1] * k)] != 0; };
+    auto active = [&](int level, const int64_t* t) {
+        // support cells of spline t on its level: t - 2 .. t per axis, inside the lattice
+        const int64_t nl[3] = {level ? 2 * n0[0] : n0[0], level ? 2 * n0[1] : n0[1], level ? 2 * n0[2] : n0[2]};
+        int n_cells = 0, n_refined = 0;
+        for (int64_t k = t[2] - 2; k <= t[2]; ++k)
+            for (int64_t j = t[1] - 2; j <= t[1]; ++j)
+                for (int64_t i = t[0] - 2; i <= t[0]; ++i) {
+                    if (i < 0 || j < 0 || k < 0 || i >= nl[0] || j >= nl[1] || k >= nl[2]) continue;
+                    ++n_cells;
+                    const bool r = level ? is_refined(i / 2, j / 2, k / 2) : is_refined(i, j, k);
+                    n_refined += r;
+                }
+        return level == 0 ? n_refined < n_cells : n_refined == n_cells;
+    };
+    const double tw[4] = {0.25, 0.75, 0.75, 0.25};
+    // per element: rows of (level, global spline index, weight)
+    struct E3 { int level; int64_t sp; double w; };
+    std::vector<std::vector<std::vector<E3>>> rows(p.elem_cell.size());
+    std::map<int64_t, int32_t> used0, used1;
+    for (size_t e = 0; e < p.elem_cell.size(); ++e) {
+        const int lvl = p.elem_level[e];
+        const int64_t nl[3] = {lvl ? 2 * n0[0] : n0[0], lvl ? 2 * n0[1] : n0[1], lvl ? 2 * n0[2] : n0[2]};
+        const int64_t c = p.elem_lcell[e];
+        const int64_t ijk[3] = {c % nl[0], (c / nl[0]) % nl[1], c / (nl[0] * nl[1])};
+        // the element's splines (local j = a + 3 b + 9 c -> spline ijk + (a, b, c))
+        int64_t own[27][3];
+        for (int j = 0; j < 27; ++j) {
+            own[j][0] = ijk[0] + j % 3;
+            own[j][1] = ijk[1] + (j / 3) % 3;
+            own[j][2] = ijk[2] + j / 9;
+        }
+        // B: columns (level, spline) -> 27 coefficients on the element's splines
+        std::vector<std::pair<std::pair<int, int64_t>, std::array<double, 27>>> B;
+        const int64_t* ns = lvl ? ns1 : ns0;
+        for (int j = 0; j < 27; ++j)
+            if (active(lvl, own[j])) {
+                std::array<double, 27> col{};
+                col[j] = 1.0;
+                B.push_back({{lvl, own[j][0] + ns[0] * (own[j][1] + ns[1] * own[j][2])}, col});
+            }
+        if (lvl == 1) {
+            const int64_t P[3] = {ijk[0] / 2, ijk[1] / 2, ijk[2] / 2};
+            for (int jp = 0; jp < 27; ++jp) {
+                const int64_t sp[3] = {P[0] + jp % 3, P[1] + (jp / 3) % 3, P[2] + jp / 9};
+                if (!active(0, sp)) continue;
+                std::array<double, 27> col{};
+                // children 2 s - 2 + k, k = 0..3 per axis, that are inactive and among the element's splines
+                for (int kz = 0; kz < 4; ++kz)
+                    for (int ky = 0; ky < 4; ++ky)
+                        for (int kx = 0; kx < 4; ++kx) {
+                            const int64_t ch[3] = {2 * sp[0] - 2 + kx, 2 * sp[1] - 2 + ky, 2 * sp[2] - 2 + kz};
+                            if (ch[0] < 0 || ch[1] < 0 || ch[2] < 0 || ch[0] >= ns1[0] || ch[1] >= ns1[1] || ch[2] >= ns1[2]) continue;
+                            if (active(1, ch)) continue;
+                            for (int j = 0; j < 27; ++j)
+                                if (own[j][0] == ch[0] && own[j][1] == ch[1] && own[j][2] == ch[2]) {
+                                    col[j] += tw[kx] * tw[ky] * tw[kz];
+                                    break;
+                                }
+                        }
+                bool nonzero = false;
+                for (double v : col) nonzero = nonzero || std::fabs(v) > 1e-12;
+                if (nonzero) B.push_back({{0, sp[0] + ns0[0] * (sp[1] + ns0[1] * sp[2])}, col});
+            }
+        }
+        // L: the element's splines at its Lagrange nodes; T = L B
+        rows[e].resize(p.nb_ip);
+        for (int v = 0; v < p.nb_ip; ++v) {
+            double xi[3];
+            basis_xi(3, p.nb_ip, v, xi);
+            double L[27];
+            for (int j = 0; j < 27; ++j) {
+                L[j] = 1.0;
+                for (int d = 0; d < 3; ++d) L[j] *= bspline2((double)ijk[d] + 0.5 * (xi[d] + 1.0) - (double)(own[j][d] - 2));
+            }
+            for (const auto& col : B) {
+                double t = 0.0;
+                for (int j = 0; j < 27; ++j) t += L[j] * col.second[j];
+                if (std::fabs(t) > 1e-12) {
+                    rows[e][v].push_back({col.first.first, col.first.second, t});
+                    (col.first.first == 0 ? used0 : used1)[col.first.second] = 0;
+                }
+            }
+        }
+    }
+    p.N = 0;
+    for (auto& kv : used0) kv.second = p.N++;
+    for (auto& kv : used1) kv.second = p.N++;
+    for (size_t e = 0; e < p.elem_cell.size(); ++e)
+        for (int v = 0; v < p.nb_ip; ++v) {
+            std::vector<std::pair<int32_t, double>> r;
+            for (const E3& q : rows[e][v]) r.push_back({(q.level == 0 ? used0 : used1).at(q.sp), q.w});
+            std::sort(r.begin(), r.end());
+            p.t_ptr.push_back((int32_t)p.t_adof.size());
+            for (auto& q : r) {
+                p.t_adof.push_back(q.first);
+                p.t_weight.push_back(q.second);
+            }
+        }
+}
+
 void build(OracleProblem& p, const mgpu_problem_spec& s) {
     p.spec = s;
     p.dim = s.dim;
@@ -669,9 +788,82 @@ void build(OracleProblem& p, const mgpu_problem_spec& s) {
     std::map<int64_t, std::vector<double>> cut;
     decompose(p, cut);
     const int nv = p.dim + 1;
+    // THB: the intersected level-0 cells are refined; their level-1 cells are cut on the level-1
+    // lattice (all of whose intersected cells are decomposed, as for level 0)
+    if (s.refine_levels < 0 || s.refine_levels > 1) throw std::runtime_error("refine_levels must be 0 or 1");
+    const bool thb = s.refine_levels == 1;
+    if (thb && (s.dim != 3 || s.bspline_order != 2)) throw std::runtime_error("THB needs 3D and bspline_order 2");
+    OracleProblem fine;
+    std::map<int64_t, std::vector<double>> cut1;
+    if (thb) {
+        p.refined.assign(p.n_cells, 0);
+        for (int64_t c = 0; c < p.n_cells; ++c) p.refined[c] = intersected(p, c);
+        fine.spec = p.spec;
+        fine.ls = p.ls;
+        fine.dim = 3;
+        fine.nb = p.nb;
+        fine.nb_ip = p.nb_ip;
+        for (int d = 0; d < 3; ++d) {
+            fine.n[d] = 2 * p.n[d];
+            fine.lo[d] = p.lo[d];
+            fine.h[d] = 0.5 * p.h[d];
+        }
+        fine.n_cells = (int64_t)fine.n[0] * fine.n[1] * fine.n[2];
+        fine.n_nodes = (int64_t)(fine.n[0] + 1) * (fine.n[1] + 1) * (fine.n[2] + 1);
+        fine.node_phi.resize(fine.n_nodes);
+        for (int64_t nid = 0; nid < fine.n_nodes; ++nid) {
+            double x[3];
+            node_x(fine, nid, x);
+            fine.node_phi[nid] = fine.ls.eval(x);
+        }
+        decompose(fine, cut1);
+    }
     p.ig_ptr.push_back(0);
     std::vector<char> used(p.n_nodes, 0);
+    // one element: level, its cell on that level's lattice, its level-0 cell, its sub-cells
+    auto add_element = [&](int level, int64_t lc, int64_t c0, const std::vector<double>* sub) {
+        const OracleProblem& L = level == 0 ? p : fine;
+        p.elem_cell.push_back((int32_t)c0);
+        p.elem_level.push_back((int8_t)level);
+        p.elem_lcell.push_back(lc);
+        const uint64_t sid = level == 0 ? (uint64_t)lc : (uint64_t)(p.n_cells + lc);
+        if (s.iwg == MGPU_IWG_SPATIALDIFF_BULK) {
+            p.material.push_back(s.k_sigma == 0.0 ? std::exp(s.k_mu) : std::exp(s.k_mu + s.k_sigma * gauss_draw(s.seed, sid)));
+        } else {
+            p.material.push_back(s.E);
+            p.material.push_back(s.nu);
+        }
+        if (sub) {
+            p.ig_xi.insert(p.ig_xi.end(), sub->begin(), sub->end());
+            ++p.n_cut;
+        }
+        p.ig_ptr.push_back((int32_t)(p.ig_xi.size() / (nv * p.dim)));
+        const int i = (int)(lc % L.n[0]);
+        const int j = (int)((lc / L.n[0]) % L.n[1]);
+        const int k = p.dim == 3 ? (int)(lc / ((int64_t)L.n[0] * L.n[1])) : 0;
+        const int ijk[3] = {i, j, k};
+        for (int d = 0; d < p.dim; ++d) p.ip_box.push_back(L.lo[d] + (double)ijk[d] * L.h[d]);
+        for (int d = 0; d < p.dim; ++d) p.ip_box.push_back(L.h[d]);
+    };
     for (int64_t c = 0; c < p.n_cells; ++c) {
+        if (thb && p.refined[c]) {
+            const int64_t i = c % p.n[0], j = (c / p.n[0]) % p.n[1], k = c / ((int64_t)p.n[0] * p.n[1]);
+            for (int cz = 0; cz < 2; ++cz)
+                for (int cy = 0; cy < 2; ++cy)
+                    for (int cx = 0; cx < 2; ++cx) {
+                        const int64_t fc = (2 * i + cx) + (int64_t)fine.n[0] * ((2 * j + cy) + (int64_t)fine.n[1] * (2 * k + cz));
+                        auto it = cut1.find(fc);
+                        int64_t nodes[8];
+                        cell_nodes(fine, fc, nodes);
+                        if (it == cut1.end()) {
+                            if (region(fine.node_phi[nodes[0]]) != s.material_phase) continue;
+                            add_element(1, fc, c, nullptr);
+                        } else if (!it->second.empty()) {
+                            add_element(1, fc, c, &it->second);
+                        }
+                    }
+            continue;
+        }
         auto it = cut.find(c);
         int64_t nodes[8];
         cell_nodes(p, c, nodes);
@@ -680,24 +872,7 @@ void build(OracleProblem& p, const mgpu_problem_spec& s) {
         } else if (it->second.empty()) {
             continue;
         }
-        p.elem_cell.push_back((int32_t)c);
-        if (s.iwg == MGPU_IWG_SPATIALDIFF_BULK) {
-            p.material.push_back(s.k_sigma == 0.0 ? std::exp(s.k_mu) : std::exp(s.k_mu + s.k_sigma * gauss_draw(s.seed, (uint64_t)c)));
-        } else {
-            p.material.push_back(s.E);
-            p.material.push_back(s.nu);
-        }
-        if (it != cut.end()) {
-            p.ig_xi.insert(p.ig_xi.end(), it->second.begin(), it->second.end());
-            ++p.n_cut;
-        }
-        p.ig_ptr.push_back((int32_t)(p.ig_xi.size() / (nv * p.dim)));
-        const int i = (int)(c % p.n[0]);
-        const int j = (int)((c / p.n[0]) % p.n[1]);
-        const int k = p.dim == 3 ? (int)(c / ((int64_t)p.n[0] * p.n[1])) : 0;
-        const int ijk[3] = {i, j, k};
-        for (int d = 0; d < p.dim; ++d) p.ip_box.push_back(p.lo[d] + (double)ijk[d] * p.h[d]);
-        for (int d = 0; d < p.dim; ++d) p.ip_box.push_back(p.h[d]);
+        add_element(0, c, c, it == cut.end() ? nullptr : &it->second);
         for (int v = 0; v < p.nb; ++v) used[nodes[v]] = 1;
     }
     const int order = s.bspline_order <= 1 ? 1 : s.bspline_order;
@@ -735,6 +910,8 @@ void build(OracleProblem& p, const mgpu_problem_spec& s) {
                 p.t_adof.push_back(adof.at(key(x)));
                 p.t_weight.push_back(1.0);
             }
+    } else if (thb) {
+        thb_rows(p);
     } else {
         bspline2_rows(p);
     }
@@ -1082,6 +1259,12 @@ void oracle_get_setup(void* h, int32_t* elem_cell, int32_t* ig_ptr, double* ig_x
     if (t_adof) std::copy(p.t_adof.begin(), p.t_adof.end(), t_adof);
     if (row_ptr) std::copy(p.row_ptr.begin(), p.row_ptr.end(), row_ptr);
     if (col_idx) std::copy(p.col_idx.begin(), p.col_idx.end(), col_idx);
+}
+
+// IP boxes of the elements: [n_elements][2 dim], lower corner then extent.
+void oracle_get_ip_box(void* h, double* ip_box) {
+    const OracleProblem& p = *static_cast<OracleProblem*>(h);
+    std::copy(p.ip_box.begin(), p.ip_box.end(), ip_box);
 }
 
 // Assemble the oracle's own problem.  vals[nnz], rhs[rows] are zeroed then summed.

@@ -78,6 +78,10 @@ class OracleProblem:
         load().oracle_get_setup(self._h, *[_p(out[k]) for k in
                                            ("elem_cell", "ig_ptr", "ig_xi", "material", "t_adof", "row_ptr",
                                             "col_idx", "t_ptr", "t_weight")])
+        out["ip_box"] = np.zeros(self.n_elements * 2 * dim, np.float64)
+        lib = load()
+        lib.oracle_get_ip_box.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        lib.oracle_get_ip_box(self._h, _p(out["ip_box"]))
         return out
 
     def assemble(self, u):

@@ -7,6 +7,8 @@ values, R = K u.
   within 1e-12 (FP64) and 1e-6 (mixed precision);
 * HEX27 IP cells (lagrange_order 2: HEX_3x3x3 on whole cells, TET_35 on XTK sub-tets, p = 2
   B-spline rows of 8 to 27 entries or nodal rows) to the same bars;
+* one THB refinement level (refine_levels 1) on HEX8 and HEX27 cells, up to 38 adofs per type and
+  element, to the same bars;
 * a nodal set with its lattice hint removed runs the generic path and reproduces the lattice
   stencil path to 1e-12;
 * graph replays of generic sets are bit-identical to eager calls; learned quadrature is refused.
@@ -43,6 +45,17 @@ CASES = {
                                                               spheres=[[0.4, 0.5, 0.6, 0.35]], lagrange_order=2),
     "hex27_p2_uncut_elasticity": lambda: M.make_spec(3, (3, 4, 2), iwg=M.IWG_STRUC_LINEAR_BULK, bspline_order=2,
                                                      lagrange_order=2),
+    # one THB level (truncated hierarchical quadratic B-splines, up to 38 adofs per type and element)
+    "thb_sphere_6_hex8_elasticity": lambda: M.make_spec(3, (6, 6, 6), iwg=M.IWG_STRUC_LINEAR_BULK,
+                                                        spheres=[[0.5, 0.5, 0.5, 0.3]], bspline_order=2,
+                                                        refine_levels=1),
+    "thb_plane_6_hex8_elasticity": lambda: M.make_spec(3, (6, 6, 6), iwg=M.IWG_STRUC_LINEAR_BULK, ls_kind=M.LS_PLANE,
+                                                       plane=[0.3, 0.5, 0.8, -0.6], bspline_order=2, refine_levels=1),
+    "thb_plane_5_hex27_elasticity": lambda: M.make_spec(3, (5, 5, 5), iwg=M.IWG_STRUC_LINEAR_BULK, ls_kind=M.LS_PLANE,
+                                                        plane=[0.3, 0.5, 0.8, -0.6], bspline_order=2,
+                                                        lagrange_order=2, refine_levels=1),
+    "thb_sphere_4_hex27_diffusion": lambda: M.make_spec(3, (4, 4, 4), spheres=[[0.5, 0.5, 0.5, 0.3]], k_sigma=0.4,
+                                                        bspline_order=2, lagrange_order=2, refine_levels=1),
 }
 
 
@@ -52,7 +65,7 @@ def _oracle_topology(s, nb, nf, N):
         q0, q1 = s["t_ptr"][e * nb], s["t_ptr"][(e + 1) * nb]
         u = np.unique(s["t_adof"][q0:q1])
         rows.append(np.concatenate([u + f * N for f in range(nf)]))
-    return np.array(rows)
+    return rows
 
 
 @pytest.mark.parametrize("case", sorted(CASES))
@@ -67,7 +80,8 @@ def test_generic_path_matches_oracle(case):
     assert np.array_equal(rp, s["row_ptr"]) and np.array_equal(ci, s["col_idx"])
     nb = P.desc.n_bases
     nf = spec.dim if spec.iwg == M.IWG_STRUC_LINEAR_BULK else 1
-    assert np.array_equal(S.topology(), _oracle_topology(s, nb, nf, Q.n_adofs_per_type))
+    topo, want = S.topology_lists(), _oracle_topology(s, nb, nf, Q.n_adofs_per_type)
+    assert len(topo) == len(want) and all(np.array_equal(a, b) for a, b in zip(topo, want))
     u = np.random.default_rng(SEED + 11).uniform(-1.0, 1.0, S.n_rows)
     vo, Ro = Q.assemble(u)
     vals, R = S.compute_jacobian_and_residual(u)

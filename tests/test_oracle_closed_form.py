@@ -407,3 +407,80 @@ def test_hex27_rigid_motions_null_space(bspline_order):
     for u in (np.concatenate([rot[:, 0], rot[:, 1], rot[:, 2]]), np.concatenate([np.ones(N), np.zeros(2 * N)])):
         Ku = csr_matvec(rp, ci, vals, u)
         assert np.linalg.norm(Ku) < 1e-12 * np.abs(vals).max() * np.linalg.norm(u) * np.sqrt(len(u))
+
+
+def _node_coords(setup, nb):
+    """Physical coordinates of every (element, basis) Lagrange node, in T-row order."""
+    box = setup["ip_box"].reshape(-1, 6)
+    if nb == 27:
+        off = np.array([[0, 0, 0], [2, 0, 0], [2, 2, 0], [0, 2, 0], [0, 0, 2], [2, 0, 2], [2, 2, 2], [0, 2, 2],
+                        [1, 0, 0], [2, 1, 0], [1, 2, 0], [0, 1, 0], [0, 0, 1], [2, 0, 1], [2, 2, 1], [0, 2, 1],
+                        [1, 0, 2], [2, 1, 2], [1, 2, 2], [0, 1, 2], [1, 1, 1], [1, 1, 0], [1, 1, 2], [0, 1, 1],
+                        [2, 1, 1], [1, 0, 1], [1, 2, 1]], dtype=float)
+    else:
+        off = 2.0 * np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0], [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]])
+    return (box[:, None, :3] + 0.5 * off[None, :, :] * box[:, None, 3:]).reshape(-1, 3)
+
+
+def _t_dense(setup, n_adofs):
+    tp, ta, tw = setup["t_ptr"], setup["t_adof"], setup["t_weight"]
+    T = np.zeros((len(tp) - 1, n_adofs))
+    for r in range(len(tp) - 1):
+        for q in range(tp[r], tp[r + 1]):
+            T[r, ta[q]] += tw[q]
+    return T
+
+
+@pytest.mark.parametrize("iwg", [0, 1])
+def test_thb_truncated_basis_affine_energy_on_planar_cut(iwg):
+    """One THB level on the cells the plane intersects, HEX27 cells: the truncated hierarchical
+    quadratic B-splines are a partition of unity and reproduce linear fields (their coefficients are
+    each level's Greville abscissae, which truncation preserves); the energy of an affine field over
+    the cut box is then the closed form over the half-space volume."""
+    n = 4
+    nrm0 = [0.3, 0.5, 0.8]
+    d = -0.45 * sum(nrm0) / np.linalg.norm(nrm0)
+    nrm = np.asarray(nrm0) / np.linalg.norm(nrm0)
+    spec, keep = M.make_spec(3, (n, n, n), iwg=iwg, ls_kind=M.LS_PLANE, plane=list(nrm) + [d], bspline_order=2,
+                             lagrange_order=2, refine_levels=1)
+    P = O.OracleProblem(spec)
+    s = P.setup()
+    N = P.n_adofs_per_type
+    T = _t_dense(s, N)
+    X = _node_coords(s, 27)
+    assert np.abs(T.sum(axis=1) - 1.0).max() < 1e-14                      # partition of unity
+    # adof coordinates (the truncated functions are independent on the box, not necessarily on its
+    # material cells: near the refinement boundary more than 27 are non-zero on one cell)
+    G = np.linalg.lstsq(T, X, rcond=None)[0]
+    assert np.abs(T @ G - X).max() < 1e-13                                 # linear reproduction
+    h = s["ip_box"].reshape(-1, 6)[:, 3]
+    assert (h < 0.75 / n).sum() > 0 and (h > 0.75 / n).sum() > 0           # both levels present
+    V = _halfspace_volume(nrm, d)
+    rp, ci = s["row_ptr"], s["col_idx"]
+    if iwg == 0:
+        g = np.array([0.7, -0.4, 0.2])
+        u = G @ g
+        expect = g @ g * V
+    else:
+        u = np.zeros(3 * N)
+        u[:N] = 0.01 * G[:, 1]
+        expect = 0.01 ** 2 * 1.0 / (2 * 1.3) * V
+    vals, _ = P.assemble(u)
+    energy = u @ csr_matvec(rp, ci, vals, u)
+    assert abs(energy - expect) < 1e-11 * expect, (energy, expect)
+
+
+def test_thb_rigid_motions_null_space():
+    n = 4
+    spec, keep = M.make_spec(3, (n, n, n), iwg=1, spheres=[[0.5, 0.5, 0.5, 0.3]], bspline_order=2,
+                             lagrange_order=2, refine_levels=1)
+    P = O.OracleProblem(spec)
+    s = P.setup()
+    N = P.n_adofs_per_type
+    G = np.linalg.lstsq(_t_dense(s, N), _node_coords(s, 27), rcond=None)[0]
+    rp, ci = s["row_ptr"], s["col_idx"]
+    vals, _ = P.assemble(np.zeros(3 * N))
+    rot = np.cross(np.array([0.3, -0.2, 0.5]), G - 0.5)
+    for u in (np.concatenate([rot[:, 0], rot[:, 1], rot[:, 2]]), np.concatenate([np.ones(N), np.zeros(2 * N)])):
+        Ku = csr_matvec(rp, ci, vals, u)
+        assert np.linalg.norm(Ku) < 1e-12 * np.abs(vals).max() * np.linalg.norm(u) * np.sqrt(len(u))

@@ -30,6 +30,14 @@ CASES = {
                                                   bspline_order=2, lagrange_order=2),
     "hex27_lagrange_sphere_5_elast": lambda: M.make_spec(3, (5, 5, 5), iwg=M.IWG_STRUC_LINEAR_BULK,
                                                          spheres=[[0.4, 0.5, 0.6, 0.35]], lagrange_order=2),
+    # one THB level on the intersected cells (truncated hierarchical quadratic B-splines)
+    "thb_sphere_6_hex8_elast": lambda: M.make_spec(3, (6, 6, 6), iwg=M.IWG_STRUC_LINEAR_BULK,
+                                                   spheres=[[0.5, 0.5, 0.5, 0.3]], bspline_order=2, refine_levels=1),
+    "thb_plane_5_hex27_elast": lambda: M.make_spec(3, (5, 5, 5), iwg=M.IWG_STRUC_LINEAR_BULK, ls_kind=M.LS_PLANE,
+                                                   plane=[0.3, 0.5, 0.8, -0.6], bspline_order=2, lagrange_order=2,
+                                                   refine_levels=1),
+    "thb_gyroid_6_diff": lambda: M.make_spec(3, (6, 6, 6), ls_kind=M.LS_GYROID, gyroid_scale=2 * np.pi, k_sigma=0.4,
+                                             bspline_order=2, refine_levels=1),
 }
 
 
