@@ -17,7 +17,8 @@ The learned path's K and R are compared with the FP64 path's for every variant (
 matches the CPU restatement to 1e-12 in the test suite).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--variants 32] [--n 64] [--no-cpu]
-       python bench.py --workload hex27 [--n 32]   (the quadratic HEX27 configuration, its own line)
+       python bench.py --workload hex27 [--n 32] [--refine 1]   (the quadratic HEX27 configuration,
+                                                                    optionally with one THB level; its own line)
 """
 import argparse
 import json
@@ -113,9 +114,10 @@ def run_hex27(args):
     lattice of the unit cube cut by a gyroid of period 4 cells, HEX_3x3x3 on whole cells and TET_35
     on the XTK sub-tets, generic element path.  One JSON line: FP64 and mixed-precision assembly
     rates, the moment kernel's MFMA rate, mixed-vs-FP64 error, and the oracle on a sample."""
-    n = args.n if args.n_given else 32
+    n = args.n if args.n_given else (16 if args.refine else 32)
     spec, keep = M.make_spec(3, (n, n, n), iwg=M.IWG_STRUC_LINEAR_BULK, ls_kind=M.LS_GYROID,
-                             gyroid_scale=2 * np.pi * n / 4.0, bspline_order=2, lagrange_order=2)
+                             gyroid_scale=2 * np.pi * n / 4.0, bspline_order=2, lagrange_order=2,
+                             refine_levels=args.refine)
     t0 = time.time()
     P = M.Problem(spec, keep)
     S = M.Set(P.desc, keepalive=P)
@@ -129,8 +131,9 @@ def run_hex27(args):
     native, _, _ = elasticity_flops(int(npts.sum()), P.n_elements, nb=27)
     out = {"metric": "elements/s for K+R assembly, HEX27 p=2 elasticity (gyroid cut), generic path",
            "unit": "elements/s", "n_gpus": 1, "steps": args.steps, "higher_is_better": True,
-           "data": "synthetic", "config": {"workload": "%d^3 HEX27 lattice, p=2 B-splines, gyroid period 4 cells, "
-                                                       "HEX_3x3x3 / TET_35" % n,
+           "data": "synthetic", "config": {"workload": "%d^3 HEX27 lattice, p=2 B-splines%s, gyroid period 4 cells, "
+                                                       "HEX_3x3x3 / TET_35" % (n, ", one THB level on the intersected "
+                                                                                  "cells" if args.refine else ""),
                                            "elements": P.n_elements, "cut_elements": P.n_cut,
                                            "ig_subcells": P.n_ig, "points": int(npts.sum()), "nnz": S.nnz}}
     res = {}
@@ -181,6 +184,8 @@ def main():
                     help="parallel graph branches (0: autotuned); with 1 every kernel runs alone, so its "
                          "duration under a profiler matches the per-stage event timings")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--refine", type=int, default=0, choices=(0, 1),
+                    help="hex27 workload: one THB level on the intersected cells")
     ap.add_argument("--workload", choices=("variants", "hex27"), default="variants",
                     help="variants: the headline batch (default); hex27: the quadratic HEX27 configuration")
     args = ap.parse_args()
