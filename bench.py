@@ -19,6 +19,7 @@ matches the CPU restatement to 1e-12 in the test suite).
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--variants 32] [--n 64] [--no-cpu]
        python bench.py --workload hex27 [--n 32] [--refine 1]   (the quadratic HEX27 configuration,
                                                                     optionally with one THB level; its own line)
+       python bench.py --workload c1|c2 [--n N]                 (the 2D circle / 3D sphere-diffusion sets)
 """
 import argparse
 import json
@@ -53,14 +54,15 @@ def log(*a):
 STENCIL_TRAFFIC_BYTES = (177003 * 2 + 430946) * 1024
 
 
-def stencil_bytes(spec, S, n_elements, n_cut, ke_bytes, nf=3, nb=8):
+def stencil_bytes(spec, S, n_elements, n_cut, ke_bytes, nf=3, nb=8, dim=3):
     """Algorithmic HBM bytes of one k_stencil launch: K values and R written once, u read once,
-    per-element material, the stored cut-element K_e (FP32 on the mixed path, FP64 on the plain
-    path), lattice maps and row offsets."""
+    per-element material (8 B per parameter), the stored cut-element K_e (FP32 on the mixed path,
+    FP64 on the plain path), lattice maps and row offsets."""
     na = nf * nb
-    n_nodes = (spec.n[0] + 1) * (spec.n[1] + 1) * (spec.n[2] + 1)
-    n_cells = spec.n[0] * spec.n[1] * spec.n[2]
-    return (8 * S.nnz + 8 * S.n_rows + 8 * S.n_rows + 16 * n_elements + ke_bytes * na * na * n_cut
+    n_nodes = (spec.n[0] + 1) * (spec.n[1] + 1) * ((spec.n[2] + 1) if dim == 3 else 1)
+    n_cells = spec.n[0] * spec.n[1] * (spec.n[2] if dim == 3 else 1)
+    n_mat = 1 if nf == 1 else 2
+    return (8 * S.nnz + 8 * S.n_rows + 8 * S.n_rows + 8 * n_mat * n_elements + ke_bytes * na * na * n_cut
             + 4 * n_cells + 4 * n_nodes + 4 * (S.n_rows + 1))
 
 
@@ -162,6 +164,55 @@ def run_hex27(args):
     print(json.dumps(out), flush=True)
 
 
+def run_lattice(args):
+    """One lattice set (stencil path) at a parity configuration: c1 = 64 x 64 QUAD4 plane-stress
+    elasticity around a circular hole, c2 = 128^3 HEX8 diffusion around a spherical hole with a
+    log-normal conductivity.  FP64 and mixed-precision subdivision; one JSON line."""
+    if args.workload == "c1":
+        n = args.n if args.n_given else 64
+        spec, keep = W.circle_2d_spec(n)
+        dim, nf, nb, rule = 2, 2, 4, "TRI_7"
+    else:
+        n = args.n if args.n_given else 128
+        spec, keep = W.sphere_diffusion_spec(n)
+        dim, nf, nb, rule = 3, 1, 8, "TET_11"
+    t0 = time.time()
+    P = M.Problem(spec, keep)
+    S = M.Set(P.desc, keepalive=P)
+    log("%s setup %.1fs: elements %d, cut %d, sub-cells %d, rows %d, nnz %d" % (
+        args.workload, time.time() - t0, P.n_elements, P.n_cut, P.n_ig, S.n_rows, S.nnz))
+    u = np.random.default_rng(SEED + 29).uniform(-1.0, 1.0, S.n_rows)
+    ub, vb, rb = M.DeviceBuffer.from_array(u), M.DeviceBuffer(8 * S.nnz), M.DeviceBuffer(8 * S.n_rows)
+    out = {"metric": "elements/s for K+R assembly, %s" % ("2D QUAD4 plane-stress elasticity (circle cut)"
+                                                         if dim == 2 else "3D HEX8 diffusion (sphere cut)"),
+           "unit": "elements/s", "n_gpus": 1, "steps": args.steps, "higher_is_better": True, "data": "synthetic",
+           "config": {"workload": "%s, %s lattice of the unit %s" % (args.workload, "x".join([str(n)] * dim),
+                                                                    "square" if dim == 2 else "cube"),
+                      "elements": P.n_elements, "cut_elements": P.n_cut, "ig_subcells": P.n_ig, "nnz": S.nnz}}
+    res = {}
+    for prec, name in ((M.PREC_FP64, "fp64"), (M.PREC_MIXED, "mixed")):
+        S.options(prec, M.QUAD_SUBDIVISION)
+        S.autotune(ub, vb, rb, iters=3)
+        per, st = S.time_assemble(ub, vb, rb, max(5, args.steps))
+        res[name] = vb.to_array(np.float64, S.nnz)
+        st_bytes = stencil_bytes(spec, S, P.n_elements, P.n_cut, ke_bytes=8 if prec == M.PREC_FP64 else 4, nf=nf, nb=nb,
+                                 dim=dim)
+        out[name] = {"value": P.n_elements / (per * 1e-3), "ms_per_call": per, "stage_ms": st,
+                     "k_stencil_GB/s": st_bytes / (st[4] * 1e-3) / 1e9, "k_stencil_bytes": st_bytes,
+                     "tuning": list(S.tuning())}
+        log("%s %s: %.4f ms per call, stages %s" % (args.workload, name, per, ["%.4f" % x for x in st]))
+    out["value"] = out["fp64"]["value"]
+    out["dtype"] = "f64"
+    out["max_rel_err_K_mixed_vs_fp64"] = float(np.linalg.norm(res["mixed"] - res["fp64"]) / np.linalg.norm(res["fp64"]))
+    gbs = out["fp64"]["k_stencil_GB/s"]
+    out["roofline"] = {"bound": "hbm", "kernel": "k_stencil (FP64 path)", "achieved": gbs, "peak": HBM_PEAK_GBS,
+                       "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS, "traffic": None}
+    if not args.no_cpu:
+        log("%s cpu baseline ..." % args.workload)
+        out["cpu_baseline"] = cpu_baseline(P, S, u, budget_s=15.0, rule=rule)
+    print(json.dumps(out), flush=True)
+
+
 def timed_replays(B, steps, warmup):
     for _ in range(warmup):
         B.launch()
@@ -186,8 +237,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--refine", type=int, default=0, choices=(0, 1),
                     help="hex27 workload: one THB level on the intersected cells")
-    ap.add_argument("--workload", choices=("variants", "hex27"), default="variants",
-                    help="variants: the headline batch (default); hex27: the quadratic HEX27 configuration")
+    ap.add_argument("--workload", choices=("variants", "hex27", "c1", "c2"), default="variants",
+                    help="variants: the headline batch (default); hex27: the quadratic HEX27 configuration; "
+                         "c1: 64x64 QUAD4 plane-stress elasticity; c2: 128^3 HEX8 diffusion")
     args = ap.parse_args()
     args.n_given = "--n" in sys.argv
     if args.gpus != 1:
@@ -198,6 +250,9 @@ def main():
         sys.exit(1)
     if args.workload == "hex27":
         run_hex27(args)
+        return
+    if args.workload in ("c1", "c2"):
+        run_lattice(args)
         return
 
     # network: the committed checkpoint, else trained here with the library's kernels

@@ -1,4 +1,5 @@
-"""Seeded level-set inputs of the assembly workloads (sphere arrays and random sphere sets).
+"""Seeded level-set inputs of the assembly workloads (sphere arrays, random sphere sets, the 2D
+circle and the 3D sphere diffusion cases).
 
 Random numbers come from splitmix64 so that every host produces the same spheres.
 """
@@ -45,3 +46,16 @@ def elasticity_spec(n, spheres, material_phase=1):
     """3D linear elasticity (E = 1, nu = 0.3) on an n^3 hex8 lattice of the unit cube; the material
     is the region outside the spheres (material_phase 1, level set positive)."""
     return make_spec(3, (n, n, n), iwg=IWG_STRUC_LINEAR_BULK, spheres=spheres, material_phase=material_phase)
+
+
+def circle_2d_spec(n=64):
+    """Unit square, n x n QUAD4, circle hole r = 0.25 at the centre (material outside), plane stress
+    E = 1, nu = 0.3, p = 1 (TRI_7 on the cut sub-triangles)."""
+    return make_spec(2, (n, n), iwg=IWG_STRUC_LINEAR_BULK, spheres=[[0.5, 0.5, 0.0, 0.25]], seed=BASE_SEED)
+
+
+def sphere_diffusion_spec(n=128):
+    """Unit cube, n^3 HEX8 diffusion, sphere |x - c| - 0.3 (c the centre; material outside),
+    k_e = exp(N(0, 0.5^2)) per element (seeded), p = 1 (TET_11 on the cut sub-tets)."""
+    return make_spec(3, (n, n, n), spheres=[[0.5, 0.5, 0.5, 0.3]], k_sigma=0.5, seed=BASE_SEED + 2)
+
