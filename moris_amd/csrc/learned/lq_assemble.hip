@@ -15,7 +15,8 @@
 //                    element stays learned if their residual ||V w - m|| / ||m|| <= tol and
 //                    sum |w| <= kappa * volume (bounded cancellation), else it falls back to
 //                    subdivision
-//   k_lq_element     one wavefront per accepted element: M = sum_q w_q g_q g_q^T on FP32 MFMA
+//   k_lq_element     blocks of 4 wavefronts sharing the 27-point table, a wavefront per accepted
+//                    element at a time: M = sum_q w_q g_q g_q^T on FP32 MFMA
 //                    (v_mfma_f32_16x16x4f32, FP32 accumulation) over the 27 fixed points, then
 //                    B^T D B in FP64, stored in vertex order
 // Elements over tolerance run the subdivision kernels (assemble.hip) unchanged.
@@ -230,28 +231,41 @@ __global__ __launch_bounds__(64) void k_lq_project(LqView lv) {
 typedef float floatx4_t __attribute__((ext_vector_type(4)));
 
 // K_e of an accepted element from its 27 weights.  NF = 1 diffusion, 3 elasticity (3D hex8).
+// Blocks of kLqElemWaves wavefronts share the 27-point gradient table; each wavefront walks its own
+// elements (block base + wavefront, stepping by the grid's wavefront count) with the flag, element
+// id and weights of its next element loaded under the current one's arithmetic.  The block's
+// iteration count is uniform, so block barriers order the LDS phases; a wavefront whose element is
+// rejected or past the end idles through an iteration.  Each wavefront's matrix buffer holds M,
+// then the element matrix that leaves in 16-byte vectors.
+constexpr int kLqElemWaves = 4;
+
 template <int NF, typename KT>
-__global__ __launch_bounds__(64) void k_lq_element(DevView s, LqView lv) {
-    constexpr int NB = 8, NG = 24, NA = NF * NB;
+__global__ __launch_bounds__(64 * kLqElemWaves) void k_lq_element(DevView s, LqView lv) {
+    constexpr int NB = 8, NG = 24, NA = NF * NB, NW = kLqElemWaves;
+    constexpr int MBUF = (32 * 33 > NA * NA * (int)sizeof(KT) / 4) ? 32 * 33 : NA * NA * (int)sizeof(KT) / 4;
     __shared__ float gq[kLqPoints * NG];
-    __shared__ float wq[32];
-    __shared__ float Ms[32 * 33];
-    __shared__ __align__(16) KT Ks[NA * NA];
-    const int lane = threadIdx.x;
-    // the 27-point gradient table once per wavefront; elements in a grid-stride loop whose flag,
-    // element id and weights are loaded one element ahead, under the current element's arithmetic
-    for (int idx = lane; idx < kLqPoints * NG; idx += 64) gq[idx] = lv.g27[idx];
-    int c = blockIdx.x, f = 0, e = 0;
+    __shared__ float wq[NW][32];
+    __shared__ __align__(16) float mbuf[NW][MBUF];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    float* Ms = mbuf[wave];
+    KT* Ks = reinterpret_cast<KT*>(mbuf[wave]);
+    for (int idx = tid; idx < kLqPoints * NG; idx += 64 * NW) gq[idx] = lv.g27[idx];
+    const int stride = (int)gridDim.x * NW;
+    int c = (int)blockIdx.x * NW + wave, f = 0, e = 0;
     float w = 0.0f;
     if (c < lv.n_cut) {
         f = lv.flag[c];
         e = s.cut_elems[c];
         if (lane < kLqPoints) w = (float)(lv.w_fit[(size_t)c * kLqPoints + lane] * lv.detJ_ip);
     }
-    while (c < lv.n_cut) {
-        const double m0 = s.material[NF == 1 ? e : 2 * e];
-        const double m1 = NF == 1 ? 0.0 : s.material[2 * e + 1];
-        const int cn = c + (int)gridDim.x;
+    for (int base = (int)blockIdx.x * NW; base < lv.n_cut; base += stride) {
+        const bool act = c < lv.n_cut && f != 0;   // uniform across the wavefront
+        double m0 = 0.0, m1 = 0.0;
+        if (act) {
+            m0 = s.material[NF == 1 ? e : 2 * e];
+            if (NF != 1) m1 = s.material[2 * e + 1];
+        }
+        const int cn = c + stride;
         int fn = 0, en = 0;
         float wn = 0.0f;
         if (cn < lv.n_cut) {
@@ -259,10 +273,10 @@ __global__ __launch_bounds__(64) void k_lq_element(DevView s, LqView lv) {
             en = s.cut_elems[cn];
             if (lane < kLqPoints) wn = (float)(lv.w_fit[(size_t)cn * kLqPoints + lane] * lv.detJ_ip);
         }
-        if (f != 0) {   // uniform across the wavefront
-            __syncthreads();
-            if (lane < 32) wq[lane] = w;
-            __syncthreads();
+        __syncthreads();
+        if (act && lane < 32) wq[wave][lane] = w;
+        __syncthreads();
+        if (act) {
             // M (24 x 24, padded 32 x 32) = sum_q (w_q g_q) g_q^T, K = 28 points in steps of 4
             for (int mt = 0; mt < 2; ++mt)
                 for (int nt = 0; nt < 2; ++nt) {
@@ -270,18 +284,22 @@ __global__ __launch_bounds__(64) void k_lq_element(DevView s, LqView lv) {
                     const int mrow = mt * 16 + lane % 16, ncol = nt * 16 + lane % 16;
                     for (int ks = 0; ks < 7; ++ks) {
                         const int q = ks * 4 + lane / 16;
-                        const float a = (q < kLqPoints && mrow < NG) ? wq[q] * gq[q * NG + mrow] : 0.0f;
+                        const float a = (q < kLqPoints && mrow < NG) ? wq[wave][q] * gq[q * NG + mrow] : 0.0f;
                         const float b = (q < kLqPoints && ncol < NG) ? gq[q * NG + ncol] : 0.0f;
                         acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
                     }
                     for (int j = 0; j < 4; ++j) Ms[(mt * 16 + 4 * (lane / 16) + j) * 33 + ncol] = acc[j];
                 }
-            __syncthreads();
-            const int a = lane / NB, b = lane % NB;
+        }
+        __syncthreads();
+        // B^T D B in FP64 from M: lane (a, b) owns the NF x NF block (a, b)
+        const int a = lane / NB, b = lane % NB;
+        KT kv[NF * NF];
+        if (act) {
             if (NF == 1) {
                 double v = 0.0;
                 for (int i = 0; i < 3; ++i) v += (double)Ms[(i * NB + a) * 33 + (i * NB + b)];
-                Ks[a * NB + b] = (KT)(m0 * v);
+                kv[0] = (KT)(m0 * v);
             } else {
                 double D[6][6];
                 elastic_D<3>(m0, m1, D);
@@ -292,10 +310,21 @@ __global__ __launch_bounds__(64) void k_lq_element(DevView s, LqView lv) {
                             for (int j = 0; j < 3; ++j)
                                 if (d_nonzero<3>(voigt<3>(al, i), voigt<3>(be, j)))
                                     v += D[voigt<3>(al, i)][voigt<3>(be, j)] * (double)Ms[(i * NB + a) * 33 + (j * NB + b)];
-                        Ks[(al * NB + a) * NA + (be * NB + b)] = (KT)v;
+                        kv[al * 3 + be] = (KT)v;
                     }
             }
-            __syncthreads();
+        }
+        __syncthreads();
+        if (act) {
+            if (NF == 1) {
+                Ks[a * NB + b] = kv[0];
+            } else {
+                for (int al = 0; al < 3; ++al)
+                    for (int be = 0; be < 3; ++be) Ks[(al * NB + a) * NA + (be * NB + b)] = kv[al * 3 + be];
+            }
+        }
+        __syncthreads();
+        if (act) {
             // the element matrix leaves in 16-byte vectors (NA * NA * sizeof(KT) is a multiple of 16)
             using V4 = typename std::conditional<sizeof(KT) == 4, float4, double2>::type;
             constexpr int NV = NA * NA * (int)sizeof(KT) / 16;
@@ -322,13 +351,13 @@ void lq_launch_fit(const DevView& s, const LqView& lv, hipStream_t st) {
 }
 void lq_launch_element(const DevView& s, const LqView& lv, int nf, bool f32_store, hipStream_t st) {
     if (lv.n_cut == 0) return;
-    const dim3 grid((unsigned)std::min(lv.n_cut, 4096));
+    const dim3 grid((unsigned)std::min((lv.n_cut + kLqElemWaves - 1) / kLqElemWaves, 2048)), block(64 * kLqElemWaves);
     if (f32_store) {
-        if (nf == 1) hipLaunchKernelGGL((k_lq_element<1, float>), grid, dim3(64), 0, st, s, lv);
-        else hipLaunchKernelGGL((k_lq_element<3, float>), grid, dim3(64), 0, st, s, lv);
+        if (nf == 1) hipLaunchKernelGGL((k_lq_element<1, float>), grid, block, 0, st, s, lv);
+        else hipLaunchKernelGGL((k_lq_element<3, float>), grid, block, 0, st, s, lv);
     } else {
-        if (nf == 1) hipLaunchKernelGGL((k_lq_element<1, double>), grid, dim3(64), 0, st, s, lv);
-        else hipLaunchKernelGGL((k_lq_element<3, double>), grid, dim3(64), 0, st, s, lv);
+        if (nf == 1) hipLaunchKernelGGL((k_lq_element<1, double>), grid, block, 0, st, s, lv);
+        else hipLaunchKernelGGL((k_lq_element<3, double>), grid, block, 0, st, s, lv);
     }
 }
 
