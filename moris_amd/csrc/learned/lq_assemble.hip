@@ -44,9 +44,14 @@ __global__ __launch_bounds__(64) void k_lq_moments(DevView s, LqView lv) {
     // the wavefront's elements: each owns a run of 16-lane quarters (a slot) and loops over its
     // sub-tets with the slot's width as stride
     const int lane = threadIdx.x;
-    int own[4];
+    // the wavefront's record: cut element, first sub-cell and sub-cell count of each quarter
+    int own[4], sub0[4], nsubq[4];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) own[k] = lv.mom_owner[4 * blockIdx.x + k];
+    for (int k = 0; k < 4; ++k) {
+        own[k] = lv.mom_owner[12 * blockIdx.x + k];
+        sub0[k] = lv.mom_owner[12 * blockIdx.x + 4 + k];
+        nsubq[k] = lv.mom_owner[12 * blockIdx.x + 8 + k];
+    }
     // (selects, not indexing, so that own[] stays in registers)
     auto own_at = [&](int k) { return k == 0 ? own[0] : (k == 1 ? own[1] : (k == 2 ? own[2] : own[3])); };
     const int quarter = lane >> 4;
@@ -62,9 +67,8 @@ __global__ __launch_bounds__(64) void k_lq_moments(DevView s, LqView lv) {
     double acc[kLqMoments];
 #pragma unroll
     for (int k = 0; k < kLqMoments; ++k) acc[k] = 0.0;
-    const int e = c >= 0 ? s.cut_elems[c] : 0;
-    const int ig0 = s.ig_ptr[e];
-    const int nsub = c >= 0 ? s.ig_ptr[e + 1] - ig0 : 0;
+    const int ig0 = (quarter == 0 ? sub0[0] : (quarter == 1 ? sub0[1] : (quarter == 2 ? sub0[2] : sub0[3])));
+    const int nsub = c < 0 ? 0 : (quarter == 0 ? nsubq[0] : (quarter == 1 ? nsubq[1] : (quarter == 2 ? nsubq[2] : nsubq[3])));
     for (int t = lane - 16 * q0; t < nsub; t += width) {
         const double2* src2 = reinterpret_cast<const double2*>(s.ig_xi + (size_t)(ig0 + t) * 12);
         double X[12];

@@ -697,8 +697,20 @@ mgpu_status lq_prepare(mgpu_set* S) {
             const int32_t a = take1(), b = take1(), c = take1(), d = take1();
             owner.insert(owner.end(), {a, b, c, d});
         }
+        // records of 12: the 4 owners, then each quarter's first sub-cell and sub-cell count (so
+        // that the kernel's first dependent load is the sub-cell coordinates)
         lv.n_mom_waves = (int32_t)(owner.size() / 4);
-        if ((rc = upload(S, owner.data(), owner.size(), const_cast<int32_t**>(&lv.mom_owner))) != MGPU_OK) return rc;
+        std::vector<int32_t> rec((size_t)lv.n_mom_waves * 12, 0);
+        for (int32_t w = 0; w < lv.n_mom_waves; ++w)
+            for (int k = 0; k < 4; ++k) {
+                const int32_t c = owner[(size_t)w * 4 + k];
+                rec[(size_t)w * 12 + k] = c;
+                if (c >= 0) {
+                    rec[(size_t)w * 12 + 4 + k] = S->cut_ig0[(size_t)c];
+                    rec[(size_t)w * 12 + 8 + k] = S->cut_nip[(size_t)c] / kTet11Points;
+                }
+            }
+        if ((rc = upload(S, rec.data(), rec.size(), const_cast<int32_t**>(&lv.mom_owner))) != MGPU_OK) return rc;
     }
     LqOperators op;
     lq_build_operators(&op);
@@ -811,9 +823,11 @@ extern "C" mgpu_status mgpu_set_create(const mgpu_set_desc* desc, mgpu_set** out
         S->allocs.push_back(p);
         v.ke_cut_f = static_cast<float*>(p);
         S->cut_nip.resize(v.n_cut);
+        S->cut_ig0.resize(v.n_cut);
         for (int32_t c = 0; c < v.n_cut; ++c) {
             const int32_t ce = h.cut_elems[c];
             S->cut_nip[c] = (desc->ig_ptr[ce + 1] - desc->ig_ptr[ce]) * (h.dim == 3 ? kTet11Points : kTri7Points);
+            S->cut_ig0[c] = desc->ig_ptr[ce];
         }
         // longest-first processing order of the cut elements (sub-cell count descending, stable)
         std::vector<int32_t> order(v.n_cut);

@@ -27,6 +27,7 @@ struct mgpu_set {
     int64_t stats[5] = {0, 0, 0, 0, 0};
     int64_t n_ip_cut = 0;
     std::vector<int32_t> cut_nip;           // integration points of each cut element's sub-cells
+    std::vector<int32_t> cut_ig0;           // first sub-cell of each cut element
     std::vector<void*> allocs;
     double* d_u_tmp = nullptr;
     double* d_vals_tmp = nullptr;
