@@ -204,9 +204,10 @@ __global__ __launch_bounds__(64) void k_generic_element(GenView g) {
 // sub-tets, w* = w detJ, detJ < -1e-6 skipped) are staged 32 at a time: the 81 physical
 // gradients dN_a/dx_i of each point and the same times w* go to LDS, and the moments
 //   M[(i,a),(j,b)] = sum_ip w* dN_a/dx_i dN_b/dx_j      (81 x 81, padded to 96 x 96)
-// accumulate on MFMA, each wavefront owning a 3 x 3 block of 16 x 16 tiles: FP64
-// (v_mfma_f64_16x16x4f64) on the plain path; on the mixed path FP32 products
-// (v_mfma_f32_16x16x4f32) summed per 32-point chunk and added into FP64 registers.  Then K_p from
+// accumulate on MFMA: FP64 (v_mfma_f64_16x16x4f64) on the plain path; on the mixed path FP32
+// products (v_mfma_f32_16x16x4f32) summed per 32-point chunk and added into FP64 registers.  M
+// being symmetric, the four wavefronts compute its 21 upper tiles of 16 x 16 (6, 5, 5, 5) and
+// mirror them.  Then K_p from
 // M and the material in LDS, and K_e = T^T K_p T per (dof type, dof type) block in FP64.  The LDS
 // (80 KB: K_p, T and one block of K_p T for up to 64 adofs per type, e.g. truncated hierarchical
 // rows) uses gfx950's 160 KB per workgroup.
@@ -218,6 +219,78 @@ static_assert(2 * kH27Chunk * kH27Cols + kH27Chunk * kH27PtStride <= kH27Lds, "s
 
 typedef double doublex4_t __attribute__((ext_vector_type(4)));
 typedef float floatx4_t __attribute__((ext_vector_type(4)));
+
+// M is symmetric: only the 21 tiles (mt <= nt) of the 6 x 6 tile grid are computed, the rest
+// mirrored.  Wavefront W owns H27Tiles<W> (rows sharing an A operand, columns a B operand):
+// W0 row 0; W1 row 1; W2 row 2 and (5,5); W3 rows 3 and 4.
+template <int W> struct H27Tiles;
+template <> struct H27Tiles<0> {
+    static constexpr int n = 6;
+    static constexpr int mt[6] = {0, 0, 0, 0, 0, 0}, nt[6] = {0, 1, 2, 3, 4, 5};
+};
+template <> struct H27Tiles<1> {
+    static constexpr int n = 5;
+    static constexpr int mt[6] = {1, 1, 1, 1, 1, 0}, nt[6] = {1, 2, 3, 4, 5, 0};
+};
+template <> struct H27Tiles<2> {
+    static constexpr int n = 5;
+    static constexpr int mt[6] = {2, 2, 2, 2, 5, 0}, nt[6] = {2, 3, 4, 5, 5, 0};
+};
+template <> struct H27Tiles<3> {
+    static constexpr int n = 5;
+    static constexpr int mt[6] = {3, 3, 3, 4, 4, 0}, nt[6] = {3, 4, 5, 4, 5, 0};
+};
+
+// M tiles += Gw^T G over one 32-point chunk (A: row lane % 16 of the tile, k = lane / 16; B: k =
+// lane / 16, column lane % 16): FP64 MFMA, or FP32 MFMA summed into the FP64 accumulators.
+template <int W, typename T>
+__device__ inline void h27_chunk(const T* sGw, const T* sG, int col16, int kq, doublex4_t* acc) {
+    using TL = H27Tiles<W>;
+    if constexpr (sizeof(T) == 8) {
+#pragma unroll 2
+        for (int ks = 0; ks < kH27Chunk / 4; ++ks) {
+            const int p = 4 * ks + kq;
+#pragma unroll
+            for (int t = 0; t < TL::n; ++t)
+                acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(sGw[p * kH27Cols + TL::mt[t] * 16 + col16],
+                                                             sG[p * kH27Cols + TL::nt[t] * 16 + col16], acc[t], 0, 0, 0);
+        }
+    } else {
+        floatx4_t accf[6];
+#pragma unroll
+        for (int t = 0; t < 6; ++t) accf[t] = floatx4_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 2
+        for (int ks = 0; ks < kH27Chunk / 4; ++ks) {
+            const int p = 4 * ks + kq;
+#pragma unroll
+            for (int t = 0; t < TL::n; ++t)
+                accf[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(sGw[p * kH27Cols + TL::mt[t] * 16 + col16],
+                                                              sG[p * kH27Cols + TL::nt[t] * 16 + col16], accf[t], 0, 0, 0);
+        }
+#pragma unroll
+        for (int t = 0; t < TL::n; ++t)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc[t][j] += (double)accf[t][j];
+    }
+}
+
+// The wavefront's tiles to M in LDS (rows, columns < 81) and their mirror images.  FP64 MFMA:
+// register j of lane l holds row l / 16 + 4 j; FP32 MFMA: row 4 (l / 16) + j; column l % 16.
+template <int W, typename T>
+__device__ inline void h27_store(double* Ms, const doublex4_t* acc, int col16, int kq) {
+    using TL = H27Tiles<W>;
+#pragma unroll
+    for (int t = 0; t < TL::n; ++t)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int r = TL::mt[t] * 16 + (sizeof(T) == 8 ? kq + 4 * j : 4 * kq + j);
+            const int c = TL::nt[t] * 16 + col16;
+            if (r < 81 && c < 81) {
+                Ms[r * 81 + c] = acc[t][j];
+                if (TL::mt[t] != TL::nt[t]) Ms[c * 81 + r] = acc[t][j];
+            }
+        }
+}
 
 template <int NF, typename T>
 __global__ __launch_bounds__(256) void k_generic_hex27(GenView g) {
@@ -234,13 +307,10 @@ __global__ __launch_bounds__(256) void k_generic_hex27(GenView g) {
     T* sGw = reinterpret_cast<T*>(lds);
     T* sG = sGw + kH27Chunk * kH27Cols;
     double* pts = lds + 2 * kH27Chunk * kH27Cols;
-    const int mt0 = 3 * (wave >> 1), nt0 = 3 * (wave & 1);   // the wavefront's tile block
     const int col16 = lane & 15, kq = lane >> 4;
-    doublex4_t acc[3][3];
+    doublex4_t acc[6];
 #pragma unroll
-    for (int m = 0; m < 3; ++m)
-#pragma unroll
-        for (int n = 0; n < 3; ++n) acc[m][n] = doublex4_t{0.0, 0.0, 0.0, 0.0};
+    for (int t = 0; t < 6; ++t) acc[t] = doublex4_t{0.0, 0.0, 0.0, 0.0};
 
     for (int p0 = 0; p0 < npts; p0 += kH27Chunk) {
         // the chunk's points: w* and the 1D quadratic Lagrange values / derivatives per axis
@@ -291,64 +361,23 @@ __global__ __launch_bounds__(256) void k_generic_hex27(GenView g) {
             sGw[idx] = (T)(v * pp[0]);
         }
         __syncthreads();
-        // M += Gw^T G over the chunk's points (A: row lane % 16 of the tile, k = lane / 16;
-        // B: k = lane / 16, column lane % 16)
-        if constexpr (sizeof(T) == 8) {
-#pragma unroll 2
-            for (int ks = 0; ks < kH27Chunk / 4; ++ks) {
-                const int p = 4 * ks + kq;
-                double a[3], b[3];
-#pragma unroll
-                for (int m = 0; m < 3; ++m) a[m] = sGw[p * kH27Cols + (mt0 + m) * 16 + col16];
-#pragma unroll
-                for (int n = 0; n < 3; ++n) b[n] = sG[p * kH27Cols + (nt0 + n) * 16 + col16];
-#pragma unroll
-                for (int m = 0; m < 3; ++m)
-#pragma unroll
-                    for (int n = 0; n < 3; ++n) acc[m][n] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[m], b[n], acc[m][n], 0, 0, 0);
-            }
-        } else {
-            floatx4_t accf[3][3];
-#pragma unroll
-            for (int m = 0; m < 3; ++m)
-#pragma unroll
-                for (int n = 0; n < 3; ++n) accf[m][n] = floatx4_t{0.f, 0.f, 0.f, 0.f};
-#pragma unroll 2
-            for (int ks = 0; ks < kH27Chunk / 4; ++ks) {
-                const int p = 4 * ks + kq;
-                float a[3], b[3];
-#pragma unroll
-                for (int m = 0; m < 3; ++m) a[m] = sGw[p * kH27Cols + (mt0 + m) * 16 + col16];
-#pragma unroll
-                for (int n = 0; n < 3; ++n) b[n] = sG[p * kH27Cols + (nt0 + n) * 16 + col16];
-#pragma unroll
-                for (int m = 0; m < 3; ++m)
-#pragma unroll
-                    for (int n = 0; n < 3; ++n) accf[m][n] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[m], b[n], accf[m][n], 0, 0, 0);
-            }
-#pragma unroll
-            for (int m = 0; m < 3; ++m)
-#pragma unroll
-                for (int n = 0; n < 3; ++n)
-#pragma unroll
-                    for (int j = 0; j < 4; ++j) acc[m][n][j] += (double)accf[m][n][j];
+        switch (wave) {
+            case 0: h27_chunk<0, T>(sGw, sG, col16, kq, acc); break;
+            case 1: h27_chunk<1, T>(sGw, sG, col16, kq, acc); break;
+            case 2: h27_chunk<2, T>(sGw, sG, col16, kq, acc); break;
+            default: h27_chunk<3, T>(sGw, sG, col16, kq, acc); break;
         }
         __syncthreads();
     }
 
-    // M to LDS (rows, columns < 81).  FP64 MFMA: register j of lane l holds row l / 16 + 4 j;
-    // FP32 MFMA: row 4 (l / 16) + j; column l % 16 in both.
+    // M to LDS: the wavefront's tiles and their mirror images
     double* Ms = lds;
-#pragma unroll
-    for (int m = 0; m < 3; ++m)
-#pragma unroll
-        for (int n = 0; n < 3; ++n)
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int r = (mt0 + m) * 16 + (sizeof(T) == 8 ? kq + 4 * j : 4 * kq + j);
-                const int c = (nt0 + n) * 16 + col16;
-                if (r < NG && c < NG) Ms[r * NG + c] = acc[m][n][j];
-            }
+    switch (wave) {
+        case 0: h27_store<0, T>(Ms, acc, col16, kq); break;
+        case 1: h27_store<1, T>(Ms, acc, col16, kq); break;
+        case 2: h27_store<2, T>(Ms, acc, col16, kq); break;
+        default: h27_store<3, T>(Ms, acc, col16, kq); break;
+    }
     __syncthreads();
 
     // K_p from M: thread pairs (a, b) read their 9 moments, then write their K_p entries
