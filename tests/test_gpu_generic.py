@@ -141,3 +141,33 @@ def test_generic_batch_replays_and_learned_refused():
     B.close()
     with pytest.raises(M.MgpuError, match="learned quadrature"):
         L.attach(sets[0], L.MLP(seed=1))
+
+
+def test_quadratic_thb_gyroid_size_independent_properties():
+    """The quadratic configuration at a size the oracle does not reach in seconds (12^3 base, one THB
+    level on the intersected cells, HEX27 cells, gyroid of period 4 cells, elasticity): K is
+    symmetric, rigid translations (partition of unity) are in its null space, u^T K u >= 0, R = K u,
+    and the mixed path stays within 1e-6 of FP64."""
+    n = 12
+    spec, keep = M.make_spec(3, (n, n, n), iwg=M.IWG_STRUC_LINEAR_BULK, ls_kind=M.LS_GYROID,
+                             gyroid_scale=2 * np.pi * n / 4.0, bspline_order=2, lagrange_order=2, refine_levels=1)
+    P = M.Problem(spec, keep)
+    S = M.Set(P.desc, keepalive=P)
+    print("elements", P.n_elements, "cut", P.n_cut, "rows", S.n_rows, "nnz", S.nnz, flush=True)
+    rp, ci = S.pattern()
+    N = S.n_rows // 3
+    u = np.random.default_rng(SEED + 41).uniform(-1.0, 1.0, S.n_rows)
+    vals, R = S.compute_jacobian_and_residual(u)
+    import scipy.sparse as sp
+    K = sp.csr_matrix((vals, ci, rp), shape=(S.n_rows, S.n_rows))
+    kmax = np.abs(vals).max()
+    assert abs(K - K.T).max() <= 1e-13 * kmax
+    for f in range(3):
+        t = np.zeros(S.n_rows)
+        t[f * N:(f + 1) * N] = 1.0
+        assert np.abs(K @ t).max() <= 1e-12 * kmax * np.sqrt(S.n_rows)
+    assert u @ (K @ u) >= -1e-12 * kmax * S.n_rows
+    assert rel(R, K @ u) <= 1e-12
+    S.options(M.PREC_MIXED, M.QUAD_SUBDIVISION)
+    vm, Rm = S.compute_jacobian_and_residual(u)
+    assert rel(vm, vals) <= 1e-6 and rel(Rm, R) <= 1e-6
