@@ -510,7 +510,17 @@ __global__ __launch_bounds__(256) void k_stencil(DevView s, const double* u, dou
     if constexpr (!DIRECT) {
         const int n = end - base;
         if (n > ROWS * NF * NS27) return;   // flagged by the row-length check
-        for (int p = threadIdx.x; p < n; p += ROWS * NF) vals[base + p] = stage[p];
+        // the block's contiguous CSR range in 16-byte stores (one value peeled at either end
+        // where the range is not 16-byte aligned)
+        double* dst = vals + base;
+        const int head = (reinterpret_cast<uintptr_t>(dst) & 15) ? 1 : 0;
+        const int n2 = (n - head) / 2;
+        if (threadIdx.x == 0) {
+            if (head && n > 0) dst[0] = stage[0];
+            if (((n - head) & 1) && n > head) dst[n - 1] = stage[n - 1];
+        }
+        double2* dst2 = reinterpret_cast<double2*>(dst + head);
+        for (int p = threadIdx.x; p < n2; p += ROWS * NF) dst2[p] = make_double2(stage[head + 2 * p], stage[head + 2 * p + 1]);
     }
 }
 
