@@ -409,7 +409,8 @@ const void* mlp_device_view(const mgpu_mlp* m) { return &m->d; }
 void mlp_forward_launch(const mgpu_mlp* m, const float* x, int64_t n, float* y, hipStream_t st) {
     if (n > 0) {
         const int64_t n_tiles = (n + 15) / 16;
-        const unsigned grid = (unsigned)std::min<int64_t>(n_tiles, 2048);
+        // one tile per block: a 6.6 KB block leaves room for every tile of a set resident at once
+        const unsigned grid = (unsigned)std::min<int64_t>(n_tiles, 1 << 20);
         hipLaunchKernelGGL(k_mlp_forward, dim3(grid), dim3(64), 0, st, m->d, x, n, y);
     }
 }
