@@ -122,9 +122,16 @@ def run_hex27(args):
                              refine_levels=args.refine)
     t0 = time.time()
     P = M.Problem(spec, keep)
+    log("hex27 problem %.1fs: %d^3, elements %d, cut %d, sub-tets %d" % (time.time() - t0, n, P.n_elements, P.n_cut,
+                                                                         P.n_ig))
     S = M.Set(P.desc, keepalive=P)
-    log("hex27 setup %.1fs: %d^3, elements %d, cut %d, sub-tets %d, rows %d, nnz %d" % (
-        time.time() - t0, n, P.n_elements, P.n_cut, P.n_ig, S.n_rows, S.nnz))
+    # sets whose element matrices exceed 2^28 entries assemble in scatter mode (no K_e pool);
+    # MGPU_GENERIC_SCATTER=1 / =0 forces it on / off
+    ke_entries = int(np.sum(np.diff(S.topology_offsets()).astype(np.int64) ** 2))
+    force = os.environ.get("MGPU_GENERIC_SCATTER", "")[:1]
+    mode = "scatter" if force == "1" or (ke_entries > 2 ** 28 and force != "0") else "gather"
+    log("hex27 setup %.1fs: %d^3, elements %d, cut %d, sub-tets %d, rows %d, nnz %d, %s mode" % (
+        time.time() - t0, n, P.n_elements, P.n_cut, P.n_ig, S.n_rows, S.nnz, mode))
     u = np.random.default_rng(SEED + 27).uniform(-1.0, 1.0, S.n_rows)
     ub, vb, rb = M.DeviceBuffer.from_array(u), M.DeviceBuffer(8 * S.nnz), M.DeviceBuffer(8 * S.n_rows)
     ig = np.diff(P.arrays()["ig_ptr"])
@@ -138,7 +145,8 @@ def run_hex27(args):
                                                        "HEX_3x3x3 / TET_35" % (n, ", one THB level on the intersected "
                                                                                   "cells" if args.refine else ""),
                                            "elements": P.n_elements, "cut_elements": P.n_cut,
-                                           "ig_subcells": P.n_ig, "points": int(npts.sum()), "nnz": S.nnz}}
+                                           "ig_subcells": P.n_ig, "points": int(npts.sum()), "nnz": S.nnz,
+                                           "assembly": mode, "element_matrix_entries": ke_entries}}
     res = {}
     for prec, name in ((M.PREC_FP64, "fp64"), (M.PREC_MIXED, "mixed")):
         S.options(prec, M.QUAD_SUBDIVISION)

@@ -321,6 +321,16 @@ class Set:
             return np.zeros((0, n.value), dtype=np.int32)
         return np.ctypeslib.as_array(p, shape=(ne * n.value,)).reshape(ne, n.value).copy()
 
+    def topology_offsets(self):
+        """Offsets of the element topologies (int64, n_elements + 1)."""
+        off = ctypes.POINTER(ctypes.c_int64)()
+        p = _c_i32p()
+        _check(load_library().mgpu_element_topology_csr(self._h, ctypes.byref(off), ctypes.byref(p)), "topology_csr")
+        ne = self.desc.n_elements
+        if ne == 0:
+            return np.zeros(1, np.int64)
+        return np.ctypeslib.as_array(off, shape=(ne + 1,)).copy()
+
     def topology_lists(self):
         """Element topologies as a list of arrays (elements may differ in size)."""
         off = ctypes.POINTER(ctypes.c_int64)()

@@ -15,11 +15,13 @@
 //                       to the element's slot of the K_e pool.
 //   k_generic_hex27     HEX27 IP cells (HEX_3x3x3 / TET_35): one block of four wavefronts per
 //                       element, the moments on MFMA (FP64, or FP32 per 32-point chunk summed in
-//                       FP64), K_p and K_e = T^T K_p T in FP64 through LDS.
+//                       FP64), K_p and K_e = T^T K_p T in FP64 through LDS.  In scatter mode
+//                       (sets too large for a K_e pool) one launch per colour class, each
+//                       element adding its K_e into the CSR values after k_generic_zero.
 //   k_generic_gather    one thread per CSR value: the sum of its K_e entries in ascending element
 //                       order (deterministic SumIntoGlobalValues, cl_Sparse_Matrix_EpetraFECrs.cpp
 //                       :103-133).
-//   k_generic_rhs       one thread per row: R = K u (R_e = K_e u_e for these linear IWGs).
+//   k_generic_rhs       one wavefront per row: R = K u (R_e = K_e u_e for these linear IWGs).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -292,11 +294,12 @@ __device__ inline void h27_store(double* Ms, const doublex4_t* acc, int col16, i
         }
 }
 
-template <int NF, typename T>
-__global__ __launch_bounds__(256) void k_generic_hex27(GenView g) {
+template <int NF, typename T, bool SCATTER>
+__global__ __launch_bounds__(256) void k_generic_hex27(GenView g, int c0, double* vals) {
     constexpr int NB = 27, NG = 81, NP = NF * NB;
     __shared__ double lds[kH27Lds];
-    const int e = blockIdx.x;
+    // scatter mode: block b of a launch takes element b of the colour starting at c0
+    const int e = SCATTER ? g.colour_el[c0 + blockIdx.x] : blockIdx.x;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     double hh[3];
     for (int d = 0; d < 3; ++d) hh[d] = g.ip_box[(size_t)e * 6 + 3 + d];
@@ -432,8 +435,13 @@ __global__ __launch_bounds__(256) void k_generic_hex27(GenView g) {
     if (tid < NB)
         for (int q = g.t_ptr[e * NB + tid]; q < g.t_ptr[e * NB + tid + 1]; ++q) Tt[tid * na_t + g.t_col[q]] += g.t_w[q];
     __syncthreads();
-    // K_e = T^T K_p T per (f, h) block: W = K_p[f][h] T, then T^T W
-    double* ke = g.ke + g.ke_off[e];
+    // K_e = T^T K_p T per (f, h) block: W = K_p[f][h] T, then T^T W; stored to the element's
+    // slot of the pool, or (scatter mode) added into the CSR values: entry (x, y) of block (f, h)
+    // sits in row topo[f na_t + x], field block h of that row, at the rank of adof y among the
+    // row adof's neighbours.  No other element of the colour touches the row.
+    double* ke = SCATTER ? nullptr : g.ke + g.ke_off[e];
+    const int32_t* rows = SCATTER ? g.topo + g.topo_ptr[e] : nullptr;
+    const uint16_t* rk = SCATTER ? g.rank + g.rank_off[e] : nullptr;
     for (int f = 0; f < NF; ++f)
         for (int hb = 0; hb < NF; ++hb) {
             for (int idx = tid; idx < NB * na_t; idx += 256) {
@@ -447,10 +455,21 @@ __global__ __launch_bounds__(256) void k_generic_hex27(GenView g) {
                 const int x = idx / na_t, y = idx - na_t * (idx / na_t);
                 double v = 0.0;
                 for (int a = 0; a < NB; ++a) v += Tt[a * na_t + x] * W[a * na_t + y];
-                ke[(size_t)(f * na_t + x) * NA + hb * na_t + y] = v;
+                if constexpr (SCATTER) {
+                    const int32_t row = rows[f * na_t + x];
+                    const int32_t r0 = g.row_ptr[row], len = (g.row_ptr[row + 1] - r0) / NF;
+                    vals[r0 + hb * len + rk[idx]] += v;
+                } else {
+                    ke[(size_t)(f * na_t + x) * NA + hb * na_t + y] = v;
+                }
             }
             __syncthreads();
         }
+}
+
+__global__ void k_generic_zero(double* vals, int64_t n) {
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x)
+        vals[p] = 0.0;
 }
 
 __global__ void k_generic_gather(GenView g, double* vals) {
@@ -461,12 +480,15 @@ __global__ void k_generic_gather(GenView g, double* vals) {
     vals[p] = v;
 }
 
-__global__ void k_generic_rhs(GenView g, const double* vals, const double* u, double* rhs) {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+// one wavefront per row: lanes stride over the row's values, then a butterfly reduction (a fixed
+// order, so R is the same on every call)
+__global__ __launch_bounds__(256) void k_generic_rhs(GenView g, const double* vals, const double* u, double* rhs) {
+    const int r = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (r >= g.n_rows) return;
     double v = 0.0;
-    for (int p = g.row_ptr[r]; p < g.row_ptr[r + 1]; ++p) v += vals[p] * u[g.col_idx[p]];
-    rhs[r] = v;
+    for (int p = g.row_ptr[r] + lane; p < g.row_ptr[r + 1]; p += 64) v += vals[p] * u[g.col_idx[p]];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    if (lane == 0) rhs[r] = v;
 }
 
 }  // namespace
@@ -495,6 +517,15 @@ mgpu_status generic_upload(mgpu_set* S, const mgpu_set_desc& d) {
         (rc = gupload(S, h.row_ptr.data(), h.row_ptr.size(), const_cast<int32_t**>(&g.row_ptr))) ||
         (rc = gupload(S, h.col_idx.data(), h.col_idx.size(), const_cast<int32_t**>(&g.col_idx))))
         return rc;
+    if (h.scatter) {
+        if ((rc = gupload(S, h.topo.data(), h.topo.size(), const_cast<int32_t**>(&g.topo))) ||
+            (rc = gupload(S, h.topo_ptr.data(), h.topo_ptr.size(), const_cast<int64_t**>(&g.topo_ptr))) ||
+            (rc = gupload(S, h.colour_el.data(), h.colour_el.size(), const_cast<int32_t**>(&g.colour_el))) ||
+            (rc = gupload(S, h.rank_off.data(), h.rank_off.size(), const_cast<int64_t**>(&g.rank_off))) ||
+            (rc = gupload(S, h.rank.data(), h.rank.size(), const_cast<uint16_t**>(&g.rank))))
+            return rc;
+        return MGPU_OK;   // no K_e pool
+    }
     const size_t n_ke = h.ke_off.empty() ? 0 : (size_t)h.ke_off.back();
     if (n_ke > 0) {
         void* p = nullptr;
@@ -505,6 +536,20 @@ mgpu_status generic_upload(mgpu_set* S, const mgpu_set_desc& d) {
     return MGPU_OK;
 }
 
+namespace {
+template <bool SCATTER>
+void launch_hex27(mgpu_set* S, bool f32, dim3 gr, int c0, double* vals) {
+    const dim3 bl(256);
+    if (S->h.nf == 1) {
+        if (f32) hipLaunchKernelGGL((k_generic_hex27<1, float, SCATTER>), gr, bl, 0, S->stream, S->g, c0, vals);
+        else hipLaunchKernelGGL((k_generic_hex27<1, double, SCATTER>), gr, bl, 0, S->stream, S->g, c0, vals);
+    } else {
+        if (f32) hipLaunchKernelGGL((k_generic_hex27<3, float, SCATTER>), gr, bl, 0, S->stream, S->g, c0, vals);
+        else hipLaunchKernelGGL((k_generic_hex27<3, double, SCATTER>), gr, bl, 0, S->stream, S->g, c0, vals);
+    }
+}
+}  // namespace
+
 mgpu_status launch_generic(mgpu_set* S, const double* u, double* vals, double* rhs, hipEvent_t* ev) {
     const SetHost& h = S->h;
     if (ev)
@@ -512,15 +557,17 @@ mgpu_status launch_generic(mgpu_set* S, const double* u, double* vals, double* r
     if (h.n_el > 0) {
         const bool f32 = S->precision == MGPU_PREC_MIXED;
         const dim3 gr((unsigned)h.n_el), bl(64);
-        if (h.nb == 27) {
-            const dim3 b27(256);
-            if (h.nf == 1) {
-                if (f32) hipLaunchKernelGGL((k_generic_hex27<1, float>), gr, b27, 0, S->stream, S->g);
-                else hipLaunchKernelGGL((k_generic_hex27<1, double>), gr, b27, 0, S->stream, S->g);
-            } else {
-                if (f32) hipLaunchKernelGGL((k_generic_hex27<3, float>), gr, b27, 0, S->stream, S->g);
-                else hipLaunchKernelGGL((k_generic_hex27<3, double>), gr, b27, 0, S->stream, S->g);
+        if (h.nb == 27 && h.scatter) {
+            // zero the values, then one launch per colour
+            if (S->g.nnz > 0)
+                hipLaunchKernelGGL(k_generic_zero, dim3((unsigned)std::min<int64_t>((S->g.nnz + 255) / 256, 65536)),
+                                   dim3(256), 0, S->stream, vals, S->g.nnz);
+            for (size_t c = 0; c + 1 < h.colour_ptr.size(); ++c) {
+                const int32_t c0 = h.colour_ptr[c], cnt = h.colour_ptr[c + 1] - c0;
+                if (cnt > 0) launch_hex27<true>(S, f32, dim3((unsigned)cnt), c0, vals);
             }
+        } else if (h.nb == 27) {
+            launch_hex27<false>(S, f32, gr, 0, vals);
         } else if (h.dim == 3 && h.nf == 1) {
             if (f32) hipLaunchKernelGGL((k_generic_element<3, 1, float>), gr, bl, 0, S->stream, S->g);
             else hipLaunchKernelGGL((k_generic_element<3, 1, double>), gr, bl, 0, S->stream, S->g);
@@ -537,13 +584,13 @@ mgpu_status launch_generic(mgpu_set* S, const double* u, double* vals, double* r
         HIP_TRY(hipGetLastError());
     }
     if (ev) HIP_TRY(hipEventRecord(ev[3], S->stream));
-    if (S->g.nnz > 0)
+    if (S->g.nnz > 0 && !h.scatter)
         hipLaunchKernelGGL(k_generic_gather, dim3((unsigned)((S->g.nnz + 255) / 256)), dim3(256), 0, S->stream, S->g,
                            vals);
     HIP_TRY(hipGetLastError());
     if (ev) HIP_TRY(hipEventRecord(ev[4], S->stream));
     if (h.n_rows > 0)
-        hipLaunchKernelGGL(k_generic_rhs, dim3((unsigned)((h.n_rows + 255) / 256)), dim3(256), 0, S->stream, S->g,
+        hipLaunchKernelGGL(k_generic_rhs, dim3((unsigned)((h.n_rows + 3) / 4)), dim3(256), 0, S->stream, S->g,
                            vals, u, rhs);
     HIP_TRY(hipGetLastError());
     if (ev) HIP_TRY(hipEventRecord(ev[5], S->stream));

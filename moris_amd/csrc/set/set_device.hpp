@@ -94,6 +94,12 @@ struct GenView {
     const int64_t* gather_src;   // K_e entries summed into each CSR value
     const int32_t* row_ptr;      // [n_rows + 1]
     const int32_t* col_idx;      // [nnz]
+    // scatter mode (HEX27 sets without a K_e pool)
+    const int32_t* topo;         // rows of each element, nf * na_t from topo_ptr[e]
+    const int64_t* topo_ptr;     // [n_el + 1]
+    const int32_t* colour_el;    // [n_el] elements by colour
+    const int64_t* rank_off;     // [n_el + 1]
+    const uint16_t* rank;        // [na_t][na_t] per element: column of adof y among adof x's neighbours
 };
 
 // Storage of the cut-element matrices: FP64 on the plain path, FP32 with the FP32 integrands.

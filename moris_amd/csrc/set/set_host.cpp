@@ -7,6 +7,7 @@
 //                      (union over elements of topology x topology), rows/cols ascending
 #include <algorithm>
 #include <atomic>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
 #include <thread>
@@ -39,7 +40,9 @@ void parallel_for(int64_t n, F f) {
 // columns, the CSR pattern as the union of the topologies (Sparse_Matrix_EpetraFECrs::build_graph,
 // cl_Sparse_Matrix_EpetraFECrs.cpp:195-240) and, per CSR value, the list of element-matrix
 // entries summed into it in ascending element order (the deterministic replacement of
-// SumIntoGlobalValues, :103-133).
+// SumIntoGlobalValues, :103-133).  Large HEX27 sets take the scatter mode instead: elements in
+// colour classes that share no adof, each adding its K_e into the values, one colour after the
+// other (a fixed order of summation again), with per-element tables of column positions.
 mgpu_status build_generic(const mgpu_set_desc& d, SetHost& h, std::string* err) {
     const int nb = h.nb, nf = h.nf;
     const int32_t N = h.n_adofs_type;
@@ -117,8 +120,79 @@ mgpu_status build_generic(const mgpu_set_desc& d, SetHost& h, std::string* err) 
                 for (int32_t c : v) *dst++ = c + g * N;
         }
     });
-    // gather lists: K_e entry (x, y) of element e goes to CSR value (topo[x], topo[y])
+    // cut elements
+    h.elem_cut.assign((size_t)h.n_el, -1);
+    h.cut_elems.clear();
+    for (int32_t e = 0; e < h.n_el; ++e)
+        if (d.ig_ptr[e + 1] > d.ig_ptr[e]) {
+            h.elem_cut[(size_t)e] = (int32_t)h.cut_elems.size();
+            h.cut_elems.push_back(e);
+        }
     const int64_t n_src = h.ke_off[(size_t)h.n_el];
+    h.scatter = nb == 27 && n_src > kGatherMaxEntries;
+    if (nb == 27)
+        if (const char* env = std::getenv("MGPU_GENERIC_SCATTER")) {
+            if (env[0] == '1') h.scatter = true;
+            if (env[0] == '0') h.scatter = false;
+        }
+    if (h.scatter) {
+        // greedy colouring in element order: the smallest colour that no element sharing an
+        // adof with e already holds
+        std::vector<int32_t> colour((size_t)h.n_el, -1), seen;
+        int32_t n_col = 0;
+        for (int32_t e = 0; e < h.n_el; ++e) {
+            for (int32_t a : loc[(size_t)e])
+                for (int64_t q = a_ptr[(size_t)a]; q < a_ptr[(size_t)a + 1]; ++q) {
+                    const int32_t c = colour[(size_t)a_el[(size_t)q]];
+                    if (c >= 0) seen[(size_t)c] = e;
+                }
+            int32_t c = 0;
+            while (c < n_col && seen[(size_t)c] == e) ++c;
+            if (c == n_col) {
+                ++n_col;
+                seen.push_back(-1);
+            }
+            colour[(size_t)e] = c;
+        }
+        h.colour_ptr.assign((size_t)n_col + 1, 0);
+        for (int32_t e = 0; e < h.n_el; ++e) h.colour_ptr[(size_t)colour[(size_t)e] + 1]++;
+        for (int32_t c = 0; c < n_col; ++c) h.colour_ptr[(size_t)c + 1] += h.colour_ptr[(size_t)c];
+        h.colour_el.resize((size_t)h.n_el);
+        {
+            std::vector<int32_t> fill(h.colour_ptr.begin(), h.colour_ptr.end() - 1);
+            for (int32_t e = 0; e < h.n_el; ++e) h.colour_el[(size_t)fill[(size_t)colour[(size_t)e]]++] = e;
+        }
+        // within a colour the elements with most sub-cells first, so that a launch does not end
+        // on one long element
+        for (int32_t c = 0; c < n_col; ++c)
+            std::stable_sort(h.colour_el.begin() + h.colour_ptr[(size_t)c], h.colour_el.begin() + h.colour_ptr[(size_t)c + 1],
+                             [&](int32_t x, int32_t y) {
+                                 return d.ig_ptr[x + 1] - d.ig_ptr[x] > d.ig_ptr[y + 1] - d.ig_ptr[y];
+                             });
+        // rank tables: both the element's adofs and each adof's neighbours are ascending, so
+        // one merge per row finds every column
+        for (int32_t a = 0; a < N; ++a)
+            if (nbr[(size_t)a].size() > 65535) { *err = "an adof has more than 65535 neighbours"; return MGPU_ERR_UNSUPPORTED; }
+        h.rank_off.assign((size_t)h.n_el + 1, 0);
+        for (int32_t e = 0; e < h.n_el; ++e)
+            h.rank_off[(size_t)e + 1] = h.rank_off[(size_t)e] + (int64_t)h.na_t[(size_t)e] * h.na_t[(size_t)e];
+        h.rank.resize((size_t)h.rank_off[(size_t)h.n_el]);
+        parallel_for(h.n_el, [&](int64_t e) {
+            const auto& u = loc[(size_t)e];
+            const int na = (int)u.size();
+            uint16_t* rk = h.rank.data() + h.rank_off[(size_t)e];
+            for (int x = 0; x < na; ++x) {
+                const auto& v = nbr[(size_t)u[(size_t)x]];
+                size_t p = 0;
+                for (int y = 0; y < na; ++y) {
+                    while (v[p] < u[(size_t)y]) ++p;
+                    rk[x * na + y] = (uint16_t)p;
+                }
+            }
+        });
+        return MGPU_OK;
+    }
+    // gather lists: K_e entry (x, y) of element e goes to CSR value (topo[x], topo[y])
     if (n_src >= ((int64_t)1 << 40)) { *err = "element matrices too large"; return MGPU_ERR_INVALID_ARG; }
     std::vector<int64_t> pos((size_t)n_src);
     parallel_for(h.n_el, [&](int64_t e) {
@@ -140,14 +214,6 @@ mgpu_status build_generic(const mgpu_set_desc& d, SetHost& h, std::string* err) 
         std::vector<int64_t> fill(h.gather_ptr.begin(), h.gather_ptr.end() - 1);
         for (int64_t k = 0; k < n_src; ++k) h.gather_src[(size_t)fill[(size_t)pos[(size_t)k]]++] = k;
     }
-    // cut elements
-    h.elem_cut.assign((size_t)h.n_el, -1);
-    h.cut_elems.clear();
-    for (int32_t e = 0; e < h.n_el; ++e)
-        if (d.ig_ptr[e + 1] > d.ig_ptr[e]) {
-            h.elem_cut[(size_t)e] = (int32_t)h.cut_elems.size();
-            h.cut_elems.push_back(e);
-        }
     return MGPU_OK;
 }
 

@@ -17,6 +17,9 @@ constexpr int kGenericMaxAdofsPerType = 64;
 // The same for HEX27 IP cells (27 bases): level-0 quadratic B-splines or Lagrange nodes give 27,
 // truncated hierarchical rows up to about 40.
 constexpr int kHex27MaxAdofsPerType = 64;
+// HEX27 sets whose element matrices hold more entries than this (a K_e pool of 2 GiB and gather
+// lists of as much again) take the scatter mode; MGPU_GENERIC_SCATTER=1 / =0 forces it on / off.
+constexpr int64_t kGatherMaxEntries = (int64_t)1 << 28;
 
 // Host-side products of set creation (pure C++, no HIP).
 struct SetHost {
@@ -55,6 +58,13 @@ struct SetHost {
     std::vector<int32_t> t_col;          // [n T entries] element-local column of each T entry
     std::vector<int64_t> gather_ptr;     // [nnz + 1]
     std::vector<int64_t> gather_src;     // [sum of K_e sizes] K_e entry summed into each value
+    // scatter mode of the HEX27 kernel (no K_e pool, no gather lists): elements coloured so that
+    // no two of one colour share an adof; each element adds its K_e into the CSR values
+    bool scatter = false;
+    std::vector<int32_t> colour_ptr;     // [n_colours + 1] into colour_el
+    std::vector<int32_t> colour_el;      // [n_el] elements by colour, most sub-cells first within one
+    std::vector<int64_t> rank_off;       // [n_el + 1] na_t^2 per element
+    std::vector<uint16_t> rank;          // [x][y]: position of adof y in the sorted neighbours of adof x
 };
 
 // Build topology, pattern and the lattice maps from a descriptor.  Returns MGPU_OK or an error
