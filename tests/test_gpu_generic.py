@@ -237,3 +237,34 @@ def test_hex27_scatter_mode_graph_replays(monkeypatch):
             assert np.array_equal(vb.to_array(np.float64, S.nnz), ve)
             assert np.array_equal(rb.to_array(np.float64, S.n_rows), re)
     B.close()
+
+
+def test_thb_scatter_mode_chosen_by_size_properties():
+    """A THB set large enough (20^3 base, one level, about 3.3e8 element-matrix entries) to take the
+    scatter mode without being forced: symmetric K, rigid translations in its null space,
+    u^T K u >= 0, R = K u, mixed within 1e-6 of FP64."""
+    n = 20
+    spec, keep = M.make_spec(3, (n, n, n), iwg=M.IWG_STRUC_LINEAR_BULK, ls_kind=M.LS_GYROID,
+                             gyroid_scale=2 * np.pi * n / 4.0, bspline_order=2, lagrange_order=2, refine_levels=1)
+    P = M.Problem(spec, keep)
+    S = M.Set(P.desc, keepalive=P)
+    entries = int(np.sum(np.diff(S.topology_offsets()).astype(np.int64) ** 2))
+    print("elements", P.n_elements, "rows", S.n_rows, "nnz", S.nnz, "K_e entries", entries, flush=True)
+    assert entries > 2 ** 28
+    rp, ci = S.pattern()
+    N = S.n_rows // 3
+    u = np.random.default_rng(SEED + 43).uniform(-1.0, 1.0, S.n_rows)
+    vals, R = S.compute_jacobian_and_residual(u)
+    import scipy.sparse as sp
+    K = sp.csr_matrix((vals, ci, rp), shape=(S.n_rows, S.n_rows))
+    kmax = np.abs(vals).max()
+    assert abs(K - K.T).max() <= 1e-13 * kmax
+    for f in range(3):
+        t = np.zeros(S.n_rows)
+        t[f * N:(f + 1) * N] = 1.0
+        assert np.abs(K @ t).max() <= 1e-12 * kmax * np.sqrt(S.n_rows)
+    assert u @ (K @ u) >= -1e-12 * kmax * S.n_rows
+    assert rel(R, K @ u) <= 1e-12
+    S.options(M.PREC_MIXED, M.QUAD_SUBDIVISION)
+    vm, Rm = S.compute_jacobian_and_residual(u)
+    assert rel(vm, vals) <= 1e-6 and rel(Rm, R) <= 1e-6
