@@ -274,9 +274,12 @@ __host__ __device__ constexpr bool stencil_direct(int variant) {
     return variant >= stencil_row_choices<NF>();
 }
 
+// The (AL, BE) block of row r: its values in acc[q] for the stencil neighbours q set in mask, in
+// ascending q from CSR index idx0; its part of R in *r_part.  False (and the error flag set) if the
+// row's pattern does not match the stencil.
 template <int DIM, int NF, int AL, int BE, typename KT>
-__device__ inline void stencil_row(const DevView& s, const double (*sc)[3], int r, const double* u,
-                                   double* stage, int stage_base, double* r_part) {
+__device__ inline bool stencil_row(const DevView& s, const double (*sc)[3], int r, double* acc, uint32_t& mask_out,
+                                   int& idx0, double* r_part) {
     constexpr int NB = DIM == 3 ? 8 : 4;
     constexpr int NG = DIM * NB;
     constexpr int NA = NF * NB;
@@ -308,6 +311,27 @@ __device__ inline void stencil_row(const DevView& s, const double (*sc)[3], int 
         else rec4[cc] = *reinterpret_cast<const double4*>(s.cell_rec + 4 * (size_t)cell);
     }
 
+    // u at the 27 neighbours.  Where the registers allow (FP32 K_e, or diffusion) they are
+    // requested together with the cell records, the address not waiting for the stencil mask (a
+    // neighbour beyond the lattice is clamped into the node array and never used); with FP64 K_e
+    // elasticity that would cost a wavefront per SIMD, so there they follow the mask (a neighbour
+    // outside the stencil reads the row's own node).  The mask selects the use.
+    // (node indices are int32: the lattice has fewer than 2^31 nodes)
+    constexpr bool kEarlyU = sizeof(KT) == 4 || DIFF;
+    const int n_nodes = (nx + 1) * (ny + 1) * (DIM == 3 ? nz + 1 : 1);
+    const double* ub = s.u_node + (size_t)BE * n_nodes;
+    double un[NS27];
+    auto load_u = [&](uint32_t m) {
+#pragma unroll
+        for (int q = 0; q < NS27; ++q) {
+            const int dx = q % 3 - 1, dy = (q / 3) % 3 - 1, dz = DIM == 3 ? q / 9 - 1 : 0;
+            const int nb = node + dx + (nx + 1) * (dy + (ny + 1) * dz);
+            const int at = kEarlyU ? min(max(nb, 0), n_nodes - 1) : ((m & (1u << q)) ? nb : node);
+            un[q] = ub[at];
+        }
+    };
+    if constexpr (kEarlyU) load_u(0u);
+
     // stencil mask: the neighbours of the row node in its material cells
     uint32_t mask = 0;
 #pragma unroll
@@ -324,19 +348,8 @@ __device__ inline void stencil_row(const DevView& s, const double (*sc)[3], int 
             mask |= 1u << q;
         }
     }
-    // u at the 27 neighbours, requested before the row is summed and unconditionally (a
-    // neighbour outside the stencil reads the row's own node), so that no load waits behind a
-    // branch; the mask selects the use
-    const int64_t n_nodes = (int64_t)(nx + 1) * (ny + 1) * (DIM == 3 ? nz + 1 : 1);
-    double un[NS27];
-#pragma unroll
-    for (int q = 0; q < NS27; ++q) {
-        const int dx = q % 3 - 1, dy = (q / 3) % 3 - 1, dz = DIM == 3 ? q / 9 - 1 : 0;
-        const int nbr = (mask & (1u << q)) ? node + dx + (nx + 1) * (dy + (ny + 1) * dz) : node;
-        un[q] = s.u_node[(size_t)BE * n_nodes + nbr];
-    }
+    if constexpr (!kEarlyU) load_u(mask);
 
-    double acc[NS27];
 #pragma unroll
     for (int q = 0; q < NS27; ++q) acc[q] = 0.0;
 
@@ -439,29 +452,35 @@ __device__ inline void stencil_row(const DevView& s, const double (*sc)[3], int 
     if (next - start != NF * cnt) {
         atomicOr(s.err_flag, 1);
         *r_part = 0.0;
-        return;
+        return false;
     }
     double R = 0.0;
-    int idx = start - stage_base + BE * cnt;
 #pragma unroll
-    for (int q = 0; q < NS27; ++q) {
-        if (mask & (1u << q)) {
-            stage[idx++] = acc[q];
-            R += acc[q] * un[q];
-        }
-    }
+    for (int q = 0; q < NS27; ++q)
+        if (mask & (1u << q)) R += acc[q] * un[q];
     *r_part = R;
+    mask_out = mask;
+    idx0 = start + BE * cnt;
+    return true;
 }
 
 template <int DIM, int NF, int AL, typename KT>
-__device__ inline void stencil_dispatch_be(const DevView& s, const double (*sc)[3], int r, const double* u,
-                                           double* stage, int base, double* rp, int be) {
+__device__ inline bool stencil_dispatch_be(const DevView& s, const double (*sc)[3], int r, double* acc,
+                                           uint32_t& mask, int& idx0, double* rp, int be) {
     switch (be) {
-        case 0: stencil_row<DIM, NF, AL, 0, KT>(s, sc, r, u, stage, base, rp); break;
-        case 1: if constexpr (NF > 1) stencil_row<DIM, NF, AL, 1, KT>(s, sc, r, u, stage, base, rp); break;
-        case 2: if constexpr (NF > 2) stencil_row<DIM, NF, AL, 2, KT>(s, sc, r, u, stage, base, rp); break;
+        case 0: return stencil_row<DIM, NF, AL, 0, KT>(s, sc, r, acc, mask, idx0, rp);
+        case 1: if constexpr (NF > 1) return stencil_row<DIM, NF, AL, 1, KT>(s, sc, r, acc, mask, idx0, rp); break;
+        case 2: if constexpr (NF > 2) return stencil_row<DIM, NF, AL, 2, KT>(s, sc, r, acc, mask, idx0, rp); break;
         default: break;
     }
+    return false;
+}
+
+// Rows per staging round of a staged stencil block: the block's CSR range is written in rounds of
+// this many rows, each a contiguous range, so the LDS copy holds one round only.
+template <int NF>
+__host__ __device__ constexpr int stencil_round_rows(int rows) {
+    return rows < (NF == 3 ? 32 : 64) ? rows : (NF == 3 ? 32 : 64);
 }
 
 template <int DIM, int NF, int ROWS, bool DIRECT, typename KT>
@@ -470,7 +489,9 @@ __global__ __launch_bounds__(256) void k_stencil(DevView s, const double* u, dou
     constexpr int NG = DIM * NB;
     constexpr bool DIFF = NF == 1;
     constexpr int NS27 = DIM == 3 ? 27 : 9;
-    __shared__ double stage[DIRECT ? 1 : ROWS * NF * NS27];
+    constexpr int RR = stencil_round_rows<NF>(ROWS);
+    static_assert(ROWS % RR == 0, "whole staging rounds");
+    __shared__ double stage[DIRECT ? 1 : RR * NF * NS27];
     __shared__ double rpart[NF * ROWS];
     // scale of the reference-cell integrals: M[(i,a),(j,b)] = G[i][a][j][b] * sc[i][j]
     double sc[3][3];
@@ -483,21 +504,29 @@ __global__ __launch_bounds__(256) void k_stencil(DevView s, const double* u, dou
     const int al = blockIdx.y;
     const int r0 = blockIdx.x * ROWS;
     const int r1 = min(r0 + ROWS, s.N);
-    const int base = s.row_ptr[al * s.N + r0];
-    const int end = s.row_ptr[al * s.N + r1];
     const int be = threadIdx.x / ROWS;
     const int rr = threadIdx.x % ROWS;
     const int r = r0 + rr;
+    double acc[NS27];
+    uint32_t mask = 0;
+    int idx0 = 0;
+    bool ok = false;
     if (r < r1) {
         double* rp = rpart + be * ROWS + rr;
-        // direct: each thread stores its row segment into vals; staged: into the block's LDS copy
-        double* out = DIRECT ? vals : stage;
-        const int out_base = DIRECT ? 0 : base;
         switch (al) {
-            case 0: stencil_dispatch_be<DIM, NF, 0, KT>(s, sc, r, u, out, out_base, rp, be); break;
-            case 1: if constexpr (NF > 1) stencil_dispatch_be<DIM, NF, 1, KT>(s, sc, r, u, out, out_base, rp, be); break;
-            case 2: if constexpr (NF > 2) stencil_dispatch_be<DIM, NF, 2, KT>(s, sc, r, u, out, out_base, rp, be); break;
+            case 0: ok = stencil_dispatch_be<DIM, NF, 0, KT>(s, sc, r, acc, mask, idx0, rp, be); break;
+            case 1: if constexpr (NF > 1) ok = stencil_dispatch_be<DIM, NF, 1, KT>(s, sc, r, acc, mask, idx0, rp, be); break;
+            case 2: if constexpr (NF > 2) ok = stencil_dispatch_be<DIM, NF, 2, KT>(s, sc, r, acc, mask, idx0, rp, be); break;
             default: break;
+        }
+    }
+    if constexpr (DIRECT) {
+        // each thread stores its row segment
+        if (ok) {
+            int idx = idx0;
+#pragma unroll
+            for (int q = 0; q < NS27; ++q)
+                if (mask & (1u << q)) vals[idx++] = acc[q];
         }
     }
     __syncthreads();
@@ -508,19 +537,34 @@ __global__ __launch_bounds__(256) void k_stencil(DevView s, const double* u, dou
         rhs[al * s.N + r] = R;
     }
     if constexpr (!DIRECT) {
-        const int n = end - base;
-        if (n > ROWS * NF * NS27) return;   // flagged by the row-length check
-        // the block's contiguous CSR range in 16-byte stores (one value peeled at either end
-        // where the range is not 16-byte aligned)
-        double* dst = vals + base;
-        const int head = (reinterpret_cast<uintptr_t>(dst) & 15) ? 1 : 0;
-        const int n2 = (n - head) / 2;
-        if (threadIdx.x == 0) {
-            if (head && n > 0) dst[0] = stage[0];
-            if (((n - head) & 1) && n > head) dst[n - 1] = stage[n - 1];
+        // rounds of RR rows: their threads put their values in the block's LDS copy, then the
+        // round's contiguous CSR range goes out in 16-byte stores (one value peeled at either end
+        // where the range is not 16-byte aligned).  Every bound here is uniform over the block.
+        for (int h = 0; h * RR < r1 - r0; ++h) {
+            const int ra = r0 + h * RR, rb = min(ra + RR, r1);
+            const int base = s.row_ptr[al * s.N + ra];
+            const int n = s.row_ptr[al * s.N + rb] - base;
+            if (ok && rr >= h * RR && rr < (h + 1) * RR) {
+                int idx = idx0 - base;
+#pragma unroll
+                for (int q = 0; q < NS27; ++q)
+                    if (mask & (1u << q)) stage[idx++] = acc[q];
+            }
+            __syncthreads();
+            if (n <= RR * NF * NS27) {   // a longer range is flagged by the row-length check
+                double* dst = vals + base;
+                const int head = (reinterpret_cast<uintptr_t>(dst) & 15) ? 1 : 0;
+                const int n2 = (n - head) / 2;
+                if (threadIdx.x == 0) {
+                    if (head && n > 0) dst[0] = stage[0];
+                    if (((n - head) & 1) && n > head) dst[n - 1] = stage[n - 1];
+                }
+                double2* dst2 = reinterpret_cast<double2*>(dst + head);
+                for (int p = threadIdx.x; p < n2; p += ROWS * NF)
+                    dst2[p] = make_double2(stage[head + 2 * p], stage[head + 2 * p + 1]);
+            }
+            __syncthreads();
         }
-        double2* dst2 = reinterpret_cast<double2*>(dst + head);
-        for (int p = threadIdx.x; p < n2; p += ROWS * NF) dst2[p] = make_double2(stage[head + 2 * p], stage[head + 2 * p + 1]);
     }
 }
 

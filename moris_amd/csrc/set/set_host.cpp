@@ -385,6 +385,8 @@ mgpu_status build_set_host(const mgpu_set_desc& d, SetHost& h, std::string* err)
     const int64_t nx = h.lat.n[0], ny = h.lat.n[1], nz = h.dim == 3 ? h.lat.n[2] : 1;
     h.n_cells = nx * ny * nz;
     h.n_nodes = h.dim == 3 ? (nx + 1) * (ny + 1) * (nz + 1) : (nx + 1) * (ny + 1);
+    // the stencil kernel indexes lattice nodes in int32
+    if (h.n_nodes >= INT32_MAX) { *err = "lattice has 2^31 or more nodes"; return MGPU_ERR_UNSUPPORTED; }
     h.cell_elem.assign((size_t)h.n_cells, -1);
     h.node_adof.assign((size_t)h.n_nodes, -1);
     for (int32_t e = 0; e < h.n_el; ++e) {
