@@ -324,7 +324,7 @@ __global__ __launch_bounds__(64 * kLqElemWaves) void k_lq_element(DevView s, LqV
                 Ks[a * NB + b] = kv[0];
             } else {
                 for (int al = 0; al < 3; ++al)
-                    for (int be = 0; be < 3; ++be) Ks[(al * NB + a) * NA + (be * NB + b)] = kv[al * 3 + be];
+                    for (int be = 0; be < 3; ++be) Ks[(a * 3 + al) * NA + (be * NB + b)] = kv[al * 3 + be];
             }
         }
         __syncthreads();

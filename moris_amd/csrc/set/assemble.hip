@@ -183,7 +183,7 @@ __global__ __launch_bounds__(64) void k_cut_elasticity(DevView s) {
                 for (int j = 0; j < DIM; ++j)
                     if (d_nonzero<DIM>(voigt<DIM>(al, i), voigt<DIM>(be, j)))
                         v += D[voigt<DIM>(al, i)][voigt<DIM>(be, j)] * (double)M[i][j];
-            ke[(al * NB + a) * NA + (be * NB + b)] = (T)v;
+            ke[(a * DIM + al) * NA + (be * NB + b)] = (T)v;
         }
 }
 
@@ -360,7 +360,7 @@ __device__ inline bool stencil_row(const DevView& s, const double (*sc)[3], int 
         constexpr int a_tab2[4] = {2, 3, 1, 0};
         const int a_loc = DIM == 3 ? a_tab[cc] : a_tab2[cc];
         const int cut = (int)(-flag) - 1;
-        const KT* base = ke_store<KT>(s) + (size_t)cut * NA * NA + (size_t)(AL * NB + a_loc) * NA + BE * NB;
+        const KT* base = ke_store<KT>(s) + (size_t)cut * NA * NA + (size_t)(a_loc * NF + AL) * NA + BE * NB;
         if constexpr (sizeof(KT) == 8) {
             const double2* p2 = reinterpret_cast<const double2*>(base);
 #pragma unroll
@@ -501,8 +501,15 @@ __global__ __launch_bounds__(256) void k_stencil(DevView s, const double* u, dou
         for (int i = 0; i < 3; ++i)
             for (int j = 0; j < 3; ++j) sc[i][j] = i < DIM && j < DIM ? 4.0 / (s.h[i] * s.h[j]) * vol : 0.0;
     }
-    const int al = blockIdx.y;
-    const int r0 = blockIdx.x * ROWS;
+    // block -> (row block, row field AL): the NF blocks of one row block are 8 IDs apart, so that
+    // the dispatcher's round robin over the 8 XCDs puts them on one XCD at about the same time,
+    // sharing in its L2 the cell records, u and the node-major rows of the stored cut K_e
+    const int n_rb = (s.N + ROWS - 1) / ROWS;
+    const int g8 = blockIdx.x >> 3;
+    const int al = g8 % NF;
+    const int rb = (g8 / NF) * 8 + (blockIdx.x & 7);
+    if (rb >= n_rb) return;   // padding of the last group of 8 (the whole block, before any barrier)
+    const int r0 = rb * ROWS;
     const int r1 = min(r0 + ROWS, s.N);
     const int be = threadIdx.x / ROWS;
     const int rr = threadIdx.x % ROWS;
@@ -615,7 +622,8 @@ void launch_subdivision(mgpu_set* S, const int32_t* skip) {
 template <int DIM, int NF, int V>
 void launch_stencil_variant(mgpu_set* S, const double* u, double* vals, double* rhs) {
     constexpr int ROWS = stencil_rows<NF>(V);
-    const dim3 grid((unsigned)((S->v.N + ROWS - 1) / ROWS), (unsigned)NF);
+    const int n_rb = (S->v.N + ROWS - 1) / ROWS;
+    const dim3 grid((unsigned)(((n_rb + 7) / 8) * 8 * NF));
     if (S->precision == MGPU_PREC_MIXED)
         hipLaunchKernelGGL((k_stencil<DIM, NF, ROWS, stencil_direct<NF>(V), float>), grid, dim3(ROWS * NF), 0, S->stream,
                            S->v, u, vals, rhs);

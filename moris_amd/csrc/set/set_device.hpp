@@ -25,7 +25,7 @@ struct DevView {
     const int32_t* cut_elems;   // [n_cut]
     const int32_t* ig_ptr;      // [n_el + 1]
     const double* ig_xi;        // [n_ig][nv][dim]
-    double* ke_cut;             // [n_cut][na][na], rows/columns (field, vertex) in vertex order
+    double* ke_cut;             // [n_cut][na][na]: rows (vertex, field) node-major, columns (field, vertex)
     float* ke_cut_f;            // the same in FP32 (mixed precision: FP32 integrands)
     int32_t* err_flag;          // [1]
     double* cell_rec;           // per lattice cell: diffusion [1] {k | -(cut+1) | 0 void},
