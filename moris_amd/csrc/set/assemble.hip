@@ -562,13 +562,18 @@ __global__ __launch_bounds__(256) void k_stencil(DevView s, const double* u, dou
                 double* dst = vals + base;
                 const int head = (reinterpret_cast<uintptr_t>(dst) & 15) ? 1 : 0;
                 const int n2 = (n - head) / 2;
+                // nontemporal: the values (an output nothing reads back here) stream out without
+                // displacing from L2 and the memory-side cache the cell records, u and the cut
+                // K_e that the next blocks read (the cut K_e, written by the kernel before, is
+                // meant to be found there, so its own stores stay ordinary)
                 if (threadIdx.x == 0) {
-                    if (head && n > 0) dst[0] = stage[0];
-                    if (((n - head) & 1) && n > head) dst[n - 1] = stage[n - 1];
+                    if (head && n > 0) __builtin_nontemporal_store(stage[0], dst);
+                    if (((n - head) & 1) && n > head) __builtin_nontemporal_store(stage[n - 1], dst + n - 1);
                 }
-                double2* dst2 = reinterpret_cast<double2*>(dst + head);
+                typedef double pair_t __attribute__((ext_vector_type(2)));
+                pair_t* dst2 = reinterpret_cast<pair_t*>(dst + head);
                 for (int p = threadIdx.x; p < n2; p += ROWS * NF)
-                    dst2[p] = make_double2(stage[head + 2 * p], stage[head + 2 * p + 1]);
+                    __builtin_nontemporal_store(pair_t{stage[head + 2 * p], stage[head + 2 * p + 1]}, dst2 + p);
             }
             __syncthreads();
         }

@@ -469,7 +469,7 @@ __global__ __launch_bounds__(256) void k_generic_hex27(GenView g, int c0, double
 
 __global__ void k_generic_zero(double* vals, int64_t n) {
     for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x)
-        vals[p] = 0.0;
+        __builtin_nontemporal_store(0.0, vals + p);   // an output: nothing reads it back here
 }
 
 __global__ void k_generic_gather(GenView g, double* vals) {
@@ -477,7 +477,7 @@ __global__ void k_generic_gather(GenView g, double* vals) {
     if (p >= g.nnz) return;
     double v = 0.0;
     for (int64_t k = g.gather_ptr[p]; k < g.gather_ptr[p + 1]; ++k) v += g.ke[g.gather_src[k]];
-    vals[p] = v;
+    __builtin_nontemporal_store(v, vals + p);   // an output: nothing reads it back here
 }
 
 // one wavefront per row: lanes stride over the row's values, then a butterfly reduction (a fixed
