@@ -311,6 +311,19 @@ __global__ __launch_bounds__(256) void k_generic_hex27(GenView g, int c0, double
     T* sG = sGw + kH27Chunk * kH27Cols;
     double* pts = lds + 2 * kH27Chunk * kH27Cols;
     const int col16 = lane & 15, kq = lane >> 4;
+    // the gradient column of this thread in the staging phase: (i, a) = (gcol / 27, gcol % 27), and
+    // the offsets of dl_i(xi_i), l(xi_j), l(xi_k) in a point record
+    const int gcol = tid % kH27Cols;
+    const bool gcol_ok = gcol < NG;
+    int goff0 = 0, goff1 = 0, goff2 = 0;
+    double gscale = 0.0;
+    if (gcol_ok) {
+        const int i = gcol / NB, a = gcol - NB * (gcol / NB);
+        goff0 = (i == 0 ? 10 : 1) + kHex27Half[a][0];
+        goff1 = (i == 1 ? 10 : 1) + 3 + kHex27Half[a][1];
+        goff2 = (i == 2 ? 10 : 1) + 6 + kHex27Half[a][2];
+        gscale = 2.0 / hh[i];
+    }
     doublex4_t acc[6];
 #pragma unroll
     for (int t = 0; t < 6; ++t) acc[t] = doublex4_t{0.0, 0.0, 0.0, 0.0};
@@ -349,19 +362,16 @@ __global__ __launch_bounds__(256) void k_generic_hex27(GenView g, int c0, double
             for (int d = 0; d < 3; ++d) lagrange_quadratic_1d(xi[d], pp + 1 + 3 * d, pp + 10 + 3 * d);
         }
         __syncthreads();
-        // physical gradients: column (i, a) of point p is dl_i(xi_i) l_j(xi_j) l_k(xi_k) * 2 / h_i
-        for (int idx = tid; idx < kH27Chunk * kH27Cols; idx += 256) {
-            const int p = idx / kH27Cols, col = idx - kH27Cols * (idx / kH27Cols);
-            const double* pp = pts + p * kH27PtStride;
-            double v = 0.0;
-            if (col < NG) {
-                const int i = col / NB, a = col - NB * (col / NB);
-                v = 2.0 / hh[i];
-#pragma unroll
-                for (int d = 0; d < 3; ++d) v *= pp[(d == i ? 10 : 1) + 3 * d + kHex27Half[a][d]];
+        // physical gradients: column (i, a) of point p is dl_i(xi_i) l_j(xi_j) l_k(xi_k) * 2 / h_i;
+        // thread tid < 192 owns column tid % 96 for every other point (its record offsets and
+        // scale set once per element)
+        if (tid < 2 * kH27Cols) {
+            for (int p = tid / kH27Cols; p < kH27Chunk; p += 2) {
+                const double* pp = pts + p * kH27PtStride;
+                const double v = gcol_ok ? ((gscale * pp[goff0]) * pp[goff1]) * pp[goff2] : 0.0;
+                sG[p * kH27Cols + gcol] = (T)v;
+                sGw[p * kH27Cols + gcol] = (T)(v * pp[0]);
             }
-            sG[idx] = (T)v;
-            sGw[idx] = (T)(v * pp[0]);
         }
         __syncthreads();
         switch (wave) {
