@@ -214,6 +214,7 @@ __global__ __launch_bounds__(64) void k_generic_element(GenView g) {
 // (80 KB: K_p, T and one block of K_p T for up to 64 adofs per type, e.g. truncated hierarchical
 // rows) uses gfx950's 160 KB per workgroup.
 constexpr int kH27Chunk = 32;                    // points per staging round
+static_assert((kH27Chunk & (kH27Chunk - 1)) == 0 && 4 * kH27Chunk <= 256, "4 threads per point of a chunk");
 constexpr int kH27Cols = 96;                     // 81 gradient columns padded to 6 tiles
 constexpr int kH27PtStride = 20;                 // w*, then l[3][3], dl[3][3] of the point
 constexpr int kH27Lds = 81 * 81 + 2 * 27 * kHex27MaxAdofsPerType;   // doubles: K_p, T, W of one block (80 KB)
@@ -329,37 +330,48 @@ __global__ __launch_bounds__(256) void k_generic_hex27(GenView g, int c0, double
     for (int t = 0; t < 6; ++t) acc[t] = doublex4_t{0.0, 0.0, 0.0, 0.0};
 
     for (int p0 = 0; p0 < npts; p0 += kH27Chunk) {
-        // the chunk's points: w* and the 1D quadratic Lagrange values / derivatives per axis
-        if (tid < kH27Chunk) {
-            const int ip = p0 + tid;
-            double xi[3] = {0.0, 0.0, 0.0}, w = 0.0;
-            if (ip < npts) {
-                if (!cut) {
-                    double detJ = 1.0;
-                    for (int d = 0; d < 3; ++d) {
-                        xi[d] = kHex3Xi[ip][d];
-                        detJ *= 0.5 * hh[d];
+        // the chunk's points: w* and the 1D quadratic Lagrange values / derivatives per axis, by 4
+        // threads per point (one per axis, one for the weight)
+        if (tid < 4 * kH27Chunk) {
+            const int pl = tid & (kH27Chunk - 1), role = tid / kH27Chunk;
+            const int ip = p0 + pl;
+            double* pp = pts + pl * kH27PtStride;
+            const int t = ip / kTet35Points, q = ip - kTet35Points * t;
+            if (role < 3) {
+                const int d = role;
+                double xd = 0.0;
+                if (ip < npts) {
+                    if (!cut) {
+                        xd = kHex3Xi[ip][d];
+                    } else {
+                        const double* X = g.ig_xi + (size_t)(ig0 + t) * 12;
+                        const double N[4] = {kTet35Xi[q][0], kTet35Xi[q][1],
+                                             1.0 - kTet35Xi[q][0] - kTet35Xi[q][1] - kTet35Xi[q][2], kTet35Xi[q][2]};
+                        xd = N[0] * X[d] + N[1] * X[3 + d] + N[2] * X[6 + d] + N[3] * X[9 + d];
                     }
-                    w = kHex3W[ip] * detJ;
-                } else {
-                    const int t = ip / kTet35Points, q = ip - kTet35Points * t;
-                    const double* X = g.ig_xi + (size_t)(ig0 + t) * 12;
-                    const double N[4] = {kTet35Xi[q][0], kTet35Xi[q][1],
-                                         1.0 - kTet35Xi[q][0] - kTet35Xi[q][1] - kTet35Xi[q][2], kTet35Xi[q][2]};
-                    for (int d = 0; d < 3; ++d) xi[d] = N[0] * X[d] + N[1] * X[3 + d] + N[2] * X[6 + d] + N[3] * X[9 + d];
-                    double J[3][3];
-                    const int rr[3] = {0, 1, 3};
-                    for (int r = 0; r < 3; ++r)
-                        for (int d = 0; d < 3; ++d) J[r][d] = (X[rr[r] * 3 + d] - X[2 * 3 + d]) * (0.5 * hh[d]);
-                    const double detJ = (J[0][0] * (J[1][1] * J[2][2] - J[2][1] * J[1][2]) -
-                                         J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
-                                         J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0])) / 6.0;
-                    w = detJ < kDetJLowerLimit ? 0.0 : kTet35W[q] * detJ;
                 }
+                lagrange_quadratic_1d(xd, pp + 1 + 3 * d, pp + 10 + 3 * d);
+            } else {
+                double w = 0.0;
+                if (ip < npts) {
+                    if (!cut) {
+                        double detJ = 1.0;
+                        for (int d = 0; d < 3; ++d) detJ *= 0.5 * hh[d];
+                        w = kHex3W[ip] * detJ;
+                    } else {
+                        const double* X = g.ig_xi + (size_t)(ig0 + t) * 12;
+                        double J[3][3];
+                        const int rr[3] = {0, 1, 3};
+                        for (int r = 0; r < 3; ++r)
+                            for (int d = 0; d < 3; ++d) J[r][d] = (X[rr[r] * 3 + d] - X[2 * 3 + d]) * (0.5 * hh[d]);
+                        const double detJ = (J[0][0] * (J[1][1] * J[2][2] - J[2][1] * J[1][2]) -
+                                             J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                                             J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0])) / 6.0;
+                        w = detJ < kDetJLowerLimit ? 0.0 : kTet35W[q] * detJ;
+                    }
+                }
+                pp[0] = w;
             }
-            double* pp = pts + tid * kH27PtStride;
-            pp[0] = w;
-            for (int d = 0; d < 3; ++d) lagrange_quadratic_1d(xi[d], pp + 1 + 3 * d, pp + 10 + 3 * d);
         }
         __syncthreads();
         // physical gradients: column (i, a) of point p is dl_i(xi_i) l_j(xi_j) l_k(xi_k) * 2 / h_i;
