@@ -203,8 +203,10 @@ __global__ __launch_bounds__(64) void k_generic_element(GenView g) {
 }
 
 // HEX27 IP cells: one block of 4 wavefronts per element.  Points (HEX_3x3x3 uncut, TET_35 on the
-// sub-tets, w* = w detJ, detJ < -1e-6 skipped) are staged 32 at a time: the 81 physical
-// gradients dN_a/dx_i of each point and the same times w* go to LDS, and the moments
+// sub-tets, w* = w detJ, detJ < -1e-6 skipped) are staged 32 at a time as records of w* and the
+// 1D quadratic Lagrange values and derivatives per axis (two LDS buffers, one barrier per chunk);
+// each lane forms its MFMA operands, the physical gradients dN_a/dx_i (times w* for A), from them
+// in registers, and the moments
 //   M[(i,a),(j,b)] = sum_ip w* dN_a/dx_i dN_b/dx_j      (81 x 81, padded to 96 x 96)
 // accumulate on MFMA: FP64 (v_mfma_f64_16x16x4f64) on the plain path; on the mixed path FP32
 // products (v_mfma_f32_16x16x4f32) summed per 32-point chunk and added into FP64 registers.  M
@@ -215,10 +217,9 @@ __global__ __launch_bounds__(64) void k_generic_element(GenView g) {
 // rows) uses gfx950's 160 KB per workgroup.
 constexpr int kH27Chunk = 32;                    // points per staging round
 static_assert((kH27Chunk & (kH27Chunk - 1)) == 0 && 4 * kH27Chunk <= 256, "4 threads per point of a chunk");
-constexpr int kH27Cols = 96;                     // 81 gradient columns padded to 6 tiles
 constexpr int kH27PtStride = 20;                 // w*, then l[3][3], dl[3][3] of the point
 constexpr int kH27Lds = 81 * 81 + 2 * 27 * kHex27MaxAdofsPerType;   // doubles: K_p, T, W of one block (80 KB)
-static_assert(2 * kH27Chunk * kH27Cols + kH27Chunk * kH27PtStride <= kH27Lds, "staging fits the K_p region");
+static_assert(2 * kH27Chunk * kH27PtStride <= 81 * 81, "two point-record buffers fit the K_p region");
 
 typedef double doublex4_t __attribute__((ext_vector_type(4)));
 typedef float floatx4_t __attribute__((ext_vector_type(4)));
@@ -245,31 +246,38 @@ template <> struct H27Tiles<3> {
 };
 
 // M tiles += Gw^T G over one 32-point chunk (A: row lane % 16 of the tile, k = lane / 16; B: k =
-// lane / 16, column lane % 16): FP64 MFMA, or FP32 MFMA summed into the FP64 accumulators.
+// lane / 16, column lane % 16), FP64 MFMA or FP32 MFMA summed into the FP64 accumulators, straight
+// from the chunk's point records: the lane forms the operands of its tiles in registers (column (i, a) of point p is ((2/h_i dl_i) l_j) l_k, times w* for A), so that no
+// staged gradient table and no barrier sit between the point records and the MFMA work.  go / gs:
+// the lane's record offsets (packed 3 x 8 bits) and scale for column group cg (columns cg*16 + lane
+// % 16; zero beyond column 80).
 template <int W, typename T>
-__device__ inline void h27_chunk(const T* sGw, const T* sG, int col16, int kq, doublex4_t* acc) {
+__device__ inline void h27_chunk_direct(const double* pts, const int* go, const double* gs, int kq, doublex4_t* acc) {
     using TL = H27Tiles<W>;
-    if constexpr (sizeof(T) == 8) {
-#pragma unroll 2
-        for (int ks = 0; ks < kH27Chunk / 4; ++ks) {
-            const int p = 4 * ks + kq;
-#pragma unroll
-            for (int t = 0; t < TL::n; ++t)
-                acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(sGw[p * kH27Cols + TL::mt[t] * 16 + col16],
-                                                             sG[p * kH27Cols + TL::nt[t] * 16 + col16], acc[t], 0, 0, 0);
-        }
-    } else {
-        floatx4_t accf[6];
+    constexpr int lo = W < 3 ? W : 3;   // the lowest column group of the wavefront's tiles
+    [[maybe_unused]] floatx4_t accf[6];
+    if constexpr (sizeof(T) == 4) {
 #pragma unroll
         for (int t = 0; t < 6; ++t) accf[t] = floatx4_t{0.f, 0.f, 0.f, 0.f};
+    }
 #pragma unroll 2
-        for (int ks = 0; ks < kH27Chunk / 4; ++ks) {
-            const int p = 4 * ks + kq;
+    for (int ks = 0; ks < kH27Chunk / 4; ++ks) {
+        const double* pp = pts + (4 * ks + kq) * kH27PtStride;
+        const double w = pp[0];
+        double v[6];
 #pragma unroll
-            for (int t = 0; t < TL::n; ++t)
-                accf[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(sGw[p * kH27Cols + TL::mt[t] * 16 + col16],
-                                                              sG[p * kH27Cols + TL::nt[t] * 16 + col16], accf[t], 0, 0, 0);
+        for (int cg = lo; cg < 6; ++cg) {
+            const int o = go[cg];
+            v[cg] = ((gs[cg] * pp[o & 255]) * pp[(o >> 8) & 255]) * pp[o >> 16];
         }
+#pragma unroll
+        for (int t = 0; t < TL::n; ++t) {
+            const T a = (T)(v[TL::mt[t]] * w), b = (T)v[TL::nt[t]];
+            if constexpr (sizeof(T) == 8) acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[t], 0, 0, 0);
+            else accf[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, accf[t], 0, 0, 0);
+        }
+    }
+    if constexpr (sizeof(T) == 4) {
 #pragma unroll
         for (int t = 0; t < TL::n; ++t)
 #pragma unroll
@@ -308,28 +316,31 @@ __global__ __launch_bounds__(256) void k_generic_hex27(GenView g, int c0, double
     const bool cut = nsub > 0;
     const int npts = cut ? nsub * kTet35Points : kHex3Points;
 
-    T* sGw = reinterpret_cast<T*>(lds);
-    T* sG = sGw + kH27Chunk * kH27Cols;
-    double* pts = lds + 2 * kH27Chunk * kH27Cols;
     const int col16 = lane & 15, kq = lane >> 4;
-    // the gradient column of this thread in the staging phase: (i, a) = (gcol / 27, gcol % 27), and
-    // the offsets of dl_i(xi_i), l(xi_j), l(xi_k) in a point record
-    const int gcol = tid % kH27Cols;
-    const bool gcol_ok = gcol < NG;
-    int goff0 = 0, goff1 = 0, goff2 = 0;
-    double gscale = 0.0;
-    if (gcol_ok) {
-        const int i = gcol / NB, a = gcol - NB * (gcol / NB);
-        goff0 = (i == 0 ? 10 : 1) + kHex27Half[a][0];
-        goff1 = (i == 1 ? 10 : 1) + 3 + kHex27Half[a][1];
-        goff2 = (i == 2 ? 10 : 1) + 6 + kHex27Half[a][2];
-        gscale = 2.0 / hh[i];
+    // the lane's operand columns cg * 16 + lane % 16, cg < 6: (i, a) = (col / 27, col % 27), the
+    // record offsets of dl_i(xi_i), l(xi_j), l(xi_k) and 2 / h_i (zero beyond column 80)
+    int go[6];
+    double gs[6];
+#pragma unroll
+    for (int cg = 0; cg < 6; ++cg) {
+        const int col = cg * 16 + col16;
+        go[cg] = 0;
+        gs[cg] = 0.0;
+        if (col < NG) {
+            const int i = col / NB, a = col - NB * (col / NB);
+            go[cg] = ((i == 0 ? 10 : 1) + kHex27Half[a][0]) | (((i == 1 ? 10 : 1) + 3 + kHex27Half[a][1]) << 8) |
+                     (((i == 2 ? 10 : 1) + 6 + kHex27Half[a][2]) << 16);
+            gs[cg] = 2.0 / hh[i];
+        }
     }
     doublex4_t acc[6];
 #pragma unroll
     for (int t = 0; t < 6; ++t) acc[t] = doublex4_t{0.0, 0.0, 0.0, 0.0};
 
-    for (int p0 = 0; p0 < npts; p0 += kH27Chunk) {
+    // point records in two buffers: a chunk's records are written while the other wavefronts may
+    // still read the previous chunk's, and the barrier of a chunk orders the one before it
+    for (int p0 = 0, buf = 0; p0 < npts; p0 += kH27Chunk, buf ^= 1) {
+        double* pts = lds + buf * kH27Chunk * kH27PtStride;
         // the chunk's points: w* and the 1D quadratic Lagrange values / derivatives per axis, by 4
         // threads per point (one per axis, one for the weight)
         if (tid < 4 * kH27Chunk) {
@@ -374,26 +385,14 @@ __global__ __launch_bounds__(256) void k_generic_hex27(GenView g, int c0, double
             }
         }
         __syncthreads();
-        // physical gradients: column (i, a) of point p is dl_i(xi_i) l_j(xi_j) l_k(xi_k) * 2 / h_i;
-        // thread tid < 192 owns column tid % 96 for every other point (its record offsets and
-        // scale set once per element)
-        if (tid < 2 * kH27Cols) {
-            for (int p = tid / kH27Cols; p < kH27Chunk; p += 2) {
-                const double* pp = pts + p * kH27PtStride;
-                const double v = gcol_ok ? ((gscale * pp[goff0]) * pp[goff1]) * pp[goff2] : 0.0;
-                sG[p * kH27Cols + gcol] = (T)v;
-                sGw[p * kH27Cols + gcol] = (T)(v * pp[0]);
-            }
-        }
-        __syncthreads();
         switch (wave) {
-            case 0: h27_chunk<0, T>(sGw, sG, col16, kq, acc); break;
-            case 1: h27_chunk<1, T>(sGw, sG, col16, kq, acc); break;
-            case 2: h27_chunk<2, T>(sGw, sG, col16, kq, acc); break;
-            default: h27_chunk<3, T>(sGw, sG, col16, kq, acc); break;
+            case 0: h27_chunk_direct<0, T>(pts, go, gs, kq, acc); break;
+            case 1: h27_chunk_direct<1, T>(pts, go, gs, kq, acc); break;
+            case 2: h27_chunk_direct<2, T>(pts, go, gs, kq, acc); break;
+            default: h27_chunk_direct<3, T>(pts, go, gs, kq, acc); break;
         }
-        __syncthreads();
     }
+    __syncthreads();   // the last chunk's records are read before M overwrites them
 
     // M to LDS: the wavefront's tiles and their mirror images
     double* Ms = lds;
