@@ -456,32 +456,74 @@ __global__ __launch_bounds__(256) void k_generic_hex27(GenView g, int c0, double
     if (tid < NB)
         for (int q = g.t_ptr[e * NB + tid]; q < g.t_ptr[e * NB + tid + 1]; ++q) Tt[tid * na_t + g.t_col[q]] += g.t_w[q];
     __syncthreads();
-    // K_e = T^T K_p T per (f, h) block: W = K_p[f][h] T, then T^T W; stored to the element's
-    // slot of the pool, or (scatter mode) added into the CSR values: entry (x, y) of block (f, h)
-    // sits in row topo[f na_t + x], field block h of that row, at the rank of adof y among the
-    // row adof's neighbours.  No other element of the colour touches the row.
+    // K_e = T^T K_p T per (f, h) block on FP64 MFMA (16 x 16 tiles, k in steps of 4, operands
+    // from LDS; A: row lane % 16, k = lane / 16; B: k = lane / 16, column lane % 16; result
+    // register j of a lane: row lane / 16 + 4 j, column lane % 16): W = K_p[f][h] T (27 x na_t),
+    // then T^T W.  The wavefronts take the tiles in turn.  K_e is stored to the element's slot of
+    // the pool, or (scatter mode) added into the CSR values: entry (x, y) of block (f, h) sits in
+    // row topo[f na_t + x], field block h of that row, at the rank of adof y among the row adof's
+    // neighbours.  No other element of the colour touches the row.
     double* ke = SCATTER ? nullptr : g.ke + g.ke_off[e];
     const int32_t* rows = SCATTER ? g.topo + g.topo_ptr[e] : nullptr;
     const uint16_t* rk = SCATTER ? g.rank + g.rank_off[e] : nullptr;
+    const int n_nt = (na_t + 15) / 16;   // 16-column tiles of T, W and K_e
     for (int f = 0; f < NF; ++f)
         for (int hb = 0; hb < NF; ++hb) {
-            for (int idx = tid; idx < NB * na_t; idx += 256) {
-                const int a = idx / na_t, y = idx - na_t * (idx / na_t);
-                double v = 0.0;
-                for (int b = 0; b < NB; ++b) v += Kp[(f * NB + a) * NP + hb * NB + b] * Tt[b * na_t + y];
-                W[idx] = v;
+            for (int tile = wave; tile < 2 * n_nt; tile += 4) {
+                const int mt = tile / n_nt, nt = tile - n_nt * (tile / n_nt);
+                const int arow = mt * 16 + col16, bcol = nt * 16 + col16;   // a; y
+                doublex4_t c4 = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int ks = 0; ks < (NB + 3) / 4; ++ks) {
+                    const int k = 4 * ks + kq;   // b
+                    const double av = (arow < NB && k < NB) ? Kp[(f * NB + arow) * NP + hb * NB + k] : 0.0;
+                    const double bv = (k < NB && bcol < na_t) ? Tt[k * na_t + bcol] : 0.0;
+                    c4 = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, c4, 0, 0, 0);
+                }
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int r = mt * 16 + kq + 4 * j;
+                    if (r < NB && bcol < na_t) W[r * na_t + bcol] = c4[j];
+                }
             }
             __syncthreads();
-            for (int idx = tid; idx < na_t * na_t; idx += 256) {
-                const int x = idx / na_t, y = idx - na_t * (idx / na_t);
-                double v = 0.0;
-                for (int a = 0; a < NB; ++a) v += Tt[a * na_t + x] * W[a * na_t + y];
+            for (int tile = wave; tile < n_nt * n_nt; tile += 4) {
+                const int mt = tile / n_nt, nt = tile - n_nt * (tile / n_nt);
+                const int xrow = mt * 16 + col16, ycol = nt * 16 + col16;   // x; y
+                // scatter mode: the tile's CSR positions and old values are requested before its
+                // MFMA work, which hides their chain of dependent loads
+                [[maybe_unused]] int pos[4];
+                [[maybe_unused]] double old[4];
                 if constexpr (SCATTER) {
-                    const int32_t row = rows[f * na_t + x];
-                    const int32_t r0 = g.row_ptr[row], len = (g.row_ptr[row + 1] - r0) / NF;
-                    vals[r0 + hb * len + rk[idx]] += v;
-                } else {
-                    ke[(size_t)(f * na_t + x) * NA + hb * na_t + y] = v;
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        const int x = mt * 16 + kq + 4 * j;
+                        pos[j] = -1;
+                        old[j] = 0.0;
+                        if (x < na_t && ycol < na_t) {
+                            const int32_t row = rows[f * na_t + x];
+                            const int32_t r0 = g.row_ptr[row], len = (g.row_ptr[row + 1] - r0) / NF;
+                            pos[j] = r0 + hb * len + rk[x * na_t + ycol];
+                            old[j] = vals[pos[j]];
+                        }
+                    }
+                }
+                doublex4_t c4 = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int ks = 0; ks < (NB + 3) / 4; ++ks) {
+                    const int k = 4 * ks + kq;   // a
+                    const double av = (k < NB && xrow < na_t) ? Tt[k * na_t + xrow] : 0.0;
+                    const double bv = (k < NB && ycol < na_t) ? W[k * na_t + ycol] : 0.0;
+                    c4 = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, c4, 0, 0, 0);
+                }
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int x = mt * 16 + kq + 4 * j;
+                    if constexpr (SCATTER) {
+                        if (pos[j] >= 0) vals[pos[j]] = old[j] + c4[j];
+                    } else if (x < na_t && ycol < na_t) {
+                        ke[(size_t)(f * na_t + x) * NA + hb * na_t + ycol] = c4[j];
+                    }
                 }
             }
             __syncthreads();
