@@ -95,11 +95,22 @@ __global__ __launch_bounds__(64) void k_lq_moments(DevView s, LqView lv) {
             double xip[3];
 #pragma unroll
             for (int d = 0; d < 3; ++d) xip[d] = N[0] * X[d] + N[1] * X[3 + d] + N[2] * X[6 + d] + N[3] * X[9 + d];
+            // the 23 monomials x^a y^b z^c (a + b + c <= 4, a fastest) times w*, as
+            // (w* x^a) (y^b z^c): six products per point before the 23 multiply-adds
             const double wq = kTet11W[q] * detJ;
-            double p[kLqMoments];
-            lq_monomials(xip, p);
+            const double wx[3] = {wq, wq * xip[0], wq * (xip[0] * xip[0])};
+            const double yv[3] = {1.0, xip[1], xip[1] * xip[1]};
+            const double zv[3] = {1.0, xip[2], xip[2] * xip[2]};
+            int k = 0;
 #pragma unroll
-            for (int k = 0; k < kLqMoments; ++k) acc[k] += wq * p[k];
+            for (int cc = 0; cc <= 2; ++cc)
+#pragma unroll
+                for (int bb = 0; bb <= 2; ++bb) {
+                    const double yz = yv[bb] * zv[cc];
+#pragma unroll
+                    for (int aa = 0; aa <= 2; ++aa)
+                        if (aa + bb + cc <= 4) acc[k++] += wx[aa] * yz;
+                }
         }
     }
     // two-level reduction through LDS in three chunks of 8 moments (4 KB, so that LDS does not
