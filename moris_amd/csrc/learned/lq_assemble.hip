@@ -90,11 +90,12 @@ __global__ __launch_bounds__(64) void k_lq_moments(DevView s, LqView lv) {
         if (detJ < kDetJLowerLimit) continue;
 #pragma unroll
         for (int q = 0; q < kTet11Points; ++q) {
-            const double N[4] = {kTet11Xi[q][0], kTet11Xi[q][1], 1.0 - kTet11Xi[q][0] - kTet11Xi[q][1] - kTet11Xi[q][2],
-                                 kTet11Xi[q][2]};
+            // the point: vertex 2 plus the rule's coordinates along the edges J (the same affine
+            // map as the vertex weights N = (xi0, xi1, 1 - xi0 - xi1 - xi2, xi2))
             double xip[3];
 #pragma unroll
-            for (int d = 0; d < 3; ++d) xip[d] = N[0] * X[d] + N[1] * X[3 + d] + N[2] * X[6 + d] + N[3] * X[9 + d];
+            for (int d = 0; d < 3; ++d)
+                xip[d] = X[6 + d] + kTet11Xi[q][0] * J[0][d] + kTet11Xi[q][1] * J[1][d] + kTet11Xi[q][2] * J[2][d];
             // the 23 monomials x^a y^b z^c (a + b + c <= 4, a fastest) times w*, as
             // (w* x^a) (y^b z^c): six products per point before the 23 multiply-adds
             const double wq = kTet11W[q] * detJ;
