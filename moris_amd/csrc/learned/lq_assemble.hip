@@ -148,19 +148,43 @@ __global__ __launch_bounds__(64) void k_lq_moments(DevView s, LqView lv) {
 
 typedef double doublex4_t __attribute__((ext_vector_type(4)));
 
+// The lane's fragments of the fit operators, loaded once per wavefront: V (A operand of the
+// residual, rows = moments, k = points) and P (A operand of the correction, rows = points, k =
+// moments), zero-padded (A: row lane % 16 of tile mt, k = 4 ks + lane / 16).
+struct LqFrags {
+    double v[2][8];
+    double p[2][kLqMomentsPad / 4];
+};
+__device__ inline void lq_load_frags(const LqView& lv, LqFrags& fr, int lane) {
+    const int col = lane % 16, grp = lane / 16;
+#pragma unroll
+    for (int mt = 0; mt < 2; ++mt) {
+        const int row = 16 * mt + col;
+#pragma unroll
+        for (int ks = 0; ks < 8; ++ks) {
+            const int k = 4 * ks + grp;
+            fr.v[mt][ks] = (row < kLqMoments && k < kLqPoints) ? lv.Vt[k * kLqMoments + row] : 0.0;
+        }
+#pragma unroll
+        for (int ks = 0; ks < kLqMomentsPad / 4; ++ks) {
+            const int k = 4 * ks + grp;
+            fr.p[mt][ks] = (row < kLqPoints && k < kLqMoments) ? lv.Pt[k * kLqPoints + row] : 0.0;
+        }
+    }
+}
+
 // Rs = Ms - V Ws for the 16 columns of the block (rows 0..23, V zero-padded to 24 x 32), on
 // v_mfma_f64_16x16x4f64: A = V (row lane % 16 of the tile, k = lane / 16), B = Ws (k = lane / 16,
 // column lane % 16); result register j of a lane holds row lane / 16 + 4 j of column lane % 16.
-__device__ inline void lq_residual(const LqView& lv, const double* Ws, const double* Ms, double* Rs, int lane) {
+__device__ inline void lq_residual(const LqFrags& fr, const double* Ws, const double* Ms, double* Rs, int lane) {
     const int col = lane % 16, grp = lane / 16;
+#pragma unroll
     for (int mt = 0; mt < 2; ++mt) {
         doublex4_t acc = {0.0, 0.0, 0.0, 0.0};
-        const int row = 16 * mt + col;
+#pragma unroll
         for (int ks = 0; ks < 8; ++ks) {
             const int k = 4 * ks + grp;
-            const double a = (row < kLqMoments && k < kLqPoints) ? lv.Vt[k * kLqMoments + row] : 0.0;
-            const double b = Ws[k * 16 + col];
-            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(fr.v[mt][ks], Ws[k * 16 + col], acc, 0, 0, 0);
         }
         for (int j = 0; j < 4; ++j) {
             const int r = 16 * mt + grp + 4 * j;
@@ -179,6 +203,8 @@ __global__ __launch_bounds__(64) void k_lq_project(LqView lv) {
     const int lane = threadIdx.x;
     const int col = lane % 16, grp = lane / 16;
     const int c0 = blockIdx.x * 16;
+    LqFrags fr;
+    lq_load_frags(lv, fr, lane);   // requested together with the moments and weights below
     for (int idx = lane; idx < kLqMomentsPad * 16; idx += 64) {
         const int r = idx / 16, ce = c0 + idx % 16;
         Ms[idx] = ce < lv.n_cut ? lv.mom[(size_t)ce * kLqMomentsPad + r] : 0.0;
@@ -203,17 +229,16 @@ __global__ __launch_bounds__(64) void k_lq_project(LqView lv) {
             for (int idx = lane; idx < 32 * 16; idx += 64) Ws[idx] = fmax(Ws[idx], 0.0);
             __syncthreads();
         }
-        lq_residual(lv, Ws, Ms, Rs, lane);
+        lq_residual(fr, Ws, Ms, Rs, lane);
         __syncthreads();
         // Ws += P Rs (P zero-padded to 32 x 24)
         for (int mt = 0; mt < 2; ++mt) {
             doublex4_t acc = {0.0, 0.0, 0.0, 0.0};
             const int row = 16 * mt + col;
+#pragma unroll
             for (int ks = 0; ks < kLqMomentsPad / 4; ++ks) {
                 const int k = 4 * ks + grp;
-                const double a = (row < kLqPoints && k < kLqMoments) ? lv.Pt[k * kLqPoints + row] : 0.0;
-                const double b = Rs[k * 16 + col];
-                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(fr.p[mt][ks], Rs[k * 16 + col], acc, 0, 0, 0);
             }
             for (int j = 0; j < 4; ++j) Ws[(16 * mt + grp + 4 * j) * 16 + col] += acc[j];
         }
@@ -225,7 +250,7 @@ __global__ __launch_bounds__(64) void k_lq_project(LqView lv) {
         Ws[idx] = (double)(float)v;
     }
     __syncthreads();
-    lq_residual(lv, Ws, Ms, Rs, lane);
+    lq_residual(fr, Ws, Ms, Rs, lane);
     __syncthreads();
     const int c = c0 + lane;
     if (lane < 16 && c < lv.n_cut) {
@@ -239,8 +264,13 @@ __global__ __launch_bounds__(64) void k_lq_project(LqView lv) {
         // accepted: moments reproduced to tol, and little cancellation among the weights
         // (sum |w| <= kappa * volume bounds the FP32 rounding of the 27-point products relative to K_e)
         const bool ok = mk > 0.0 && m0 > 0.0 && sqrt(rk / mk) <= lv.tol && sa <= lv.kappa * m0;
-        for (int r = 0; r < kLqPoints; ++r) lv.w_fit[(size_t)c * kLqPoints + r] = (float)Ws[r * 16 + lane];
         lv.flag[c] = ok ? 1 : 0;
+    }
+    // the block's 16 x 27 weights are one contiguous range: all lanes store it in order
+    const int n_el = min(16, lv.n_cut - c0);
+    for (int idx = lane; idx < n_el * kLqPoints; idx += 64) {
+        const int ce = idx / kLqPoints, r = idx - kLqPoints * ce;
+        lv.w_fit[(size_t)c0 * kLqPoints + idx] = (float)Ws[r * 16 + ce];
     }
 }
 
