@@ -50,8 +50,8 @@ def log(*a):
 
 # HBM bytes per launch of k_stencil<3,3,64,false,float> on variant 0 of the 64^3 workload, measured
 # with two rocprofv3 --pmc runs (FETCH_SIZE scaled x2.0, WRITE_SIZE x1.0 by the copy-kernel
-# calibration): 97,549 KiB x 2 + 445,993 KiB.
-STENCIL_TRAFFIC_BYTES = (97549 * 2 + 445993) * 1024
+# calibration): 97,555 KiB x 2 + 445,915 KiB.
+STENCIL_TRAFFIC_BYTES = (97555 * 2 + 445915) * 1024
 
 
 def stencil_bytes(spec, S, n_elements, n_cut, ke_bytes, nf=3, nb=8, dim=3):
