@@ -1,8 +1,8 @@
 // lq_assemble.hip -- learned cut-cell quadrature inside the assembly.
 //
 // Per cut element (hex8 IP cell):
-//   k_lq_inputs      8 nodal level-set values / max |phi| -> MLP input
-//   (MLP forward)    8 -> 64 -> 64 -> 27 on BF16 MFMA (mlp.hip): a prior for the 27 weights
+//   (MLP forward)    8 -> 64 -> 64 -> 27 on BF16 MFMA (mlp.hip), its inputs the 8 nodal level-set
+//                    values over their largest magnitude: a prior for the 27 weights
 //   k_lq_moments     one lane per XTK sub-tet, elements packed into wavefronts by 16-lane
 //                    quarters (a 48-lane element beside a 16-lane one, two 32-lane ones, ...):
 //                    the 23 moments of the material sub-cells in FP64 (TET_11 on the sub-tet,
@@ -30,15 +30,6 @@
 #include "quad.hpp"
 
 namespace mgpu {
-
-__global__ void k_lq_inputs(LqView lv) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= lv.n_cut) return;
-    double m = 0.0;
-    for (int v = 0; v < 8; ++v) m = fmax(m, fabs(lv.vphi[8 * c + v]));
-    const double inv = m > 0.0 ? 1.0 / m : 0.0;
-    for (int v = 0; v < 8; ++v) lv.x_in[8 * c + v] = (float)(lv.vphi[8 * c + v] * inv);
-}
 
 __global__ __launch_bounds__(64) void k_lq_moments(DevView s, LqView lv) {
     // the wavefront's elements: each owns a run of 16-lane quarters (a slot) and loops over its
@@ -323,9 +314,10 @@ __global__ __launch_bounds__(64 * kLqElemWaves) void k_lq_element(DevView s, LqV
         if (act && lane < 32) wq[wave][lane] = w;
         __syncthreads();
         if (act) {
-            // M (24 x 24, padded 32 x 32) = sum_q (w_q g_q) g_q^T, K = 28 points in steps of 4
+            // M (24 x 24, padded 32 x 32) = sum_q (w_q g_q) g_q^T, K = 28 points in steps of 4;
+            // M is symmetric: the lower-left tile is not formed (read from the upper-right one)
             for (int mt = 0; mt < 2; ++mt)
-                for (int nt = 0; nt < 2; ++nt) {
+                for (int nt = mt; nt < 2; ++nt) {
                     floatx4_t acc = {0.f, 0.f, 0.f, 0.f};
                     const int mrow = mt * 16 + lane % 16, ncol = nt * 16 + lane % 16;
                     for (int ks = 0; ks < 7; ++ks) {
@@ -339,12 +331,13 @@ __global__ __launch_bounds__(64 * kLqElemWaves) void k_lq_element(DevView s, LqV
         }
         __syncthreads();
         // B^T D B in FP64 from M: lane (a, b) owns the NF x NF block (a, b)
+        auto m_at = [&](int r, int c2) { return (r >= 16 && c2 < 16) ? Ms[c2 * 33 + r] : Ms[r * 33 + c2]; };
         const int a = lane / NB, b = lane % NB;
         KT kv[NF * NF];
         if (act) {
             if (NF == 1) {
                 double v = 0.0;
-                for (int i = 0; i < 3; ++i) v += (double)Ms[(i * NB + a) * 33 + (i * NB + b)];
+                for (int i = 0; i < 3; ++i) v += (double)m_at(i * NB + a, i * NB + b);
                 kv[0] = (KT)(m0 * v);
             } else {
                 double D[6][6];
@@ -355,7 +348,7 @@ __global__ __launch_bounds__(64 * kLqElemWaves) void k_lq_element(DevView s, LqV
                         for (int i = 0; i < 3; ++i)
                             for (int j = 0; j < 3; ++j)
                                 if (d_nonzero<3>(voigt<3>(al, i), voigt<3>(be, j)))
-                                    v += D[voigt<3>(al, i)][voigt<3>(be, j)] * (double)Ms[(i * NB + a) * 33 + (j * NB + b)];
+                                    v += D[voigt<3>(al, i)][voigt<3>(be, j)] * (double)m_at(i * NB + a, j * NB + b);
                         kv[al * 3 + be] = (KT)v;
                     }
             }
@@ -386,9 +379,6 @@ __global__ __launch_bounds__(64 * kLqElemWaves) void k_lq_element(DevView s, LqV
 }
 
 // Launch the learned part of an assembly on stream st (after the MLP forward has written w_pred).
-void lq_launch_inputs(const LqView& lv, hipStream_t st) {
-    if (lv.n_cut > 0) hipLaunchKernelGGL(k_lq_inputs, dim3((lv.n_cut + 255) / 256), dim3(256), 0, st, lv);
-}
 void lq_launch_fit(const DevView& s, const LqView& lv, hipStream_t st) {
     if (lv.n_cut > 0) {
         hipLaunchKernelGGL(k_lq_moments, dim3(lv.n_mom_waves), dim3(64), 0, st, s, lv);

@@ -18,7 +18,6 @@ struct LqView {
     double kappa;                   // bound on sum |w| / volume
     double detJ_ip;                 // prod(h / 2) of the lattice cell
     const double* vphi;             // [n_cut][8] nodal level-set values of the cut elements
-    float* x_in;                    // [n_cut][8]
     float* w_pred;                  // [n_cut][27] MLP output (weights / 8)
     double* mom;                    // [n_cut][24] moments of the material sub-cells (23 + padding)
     int32_t n_mom_waves;            // wavefronts of the moment kernel
@@ -31,9 +30,10 @@ struct LqView {
     const float* g27;               // [27][NG] physical gradients at the 27 points
 };
 
-void lq_launch_inputs(const LqView& lv, hipStream_t st);
 void lq_launch_fit(const DevView& s, const LqView& lv, hipStream_t st);
 void lq_launch_element(const DevView& s, const LqView& lv, int nf, bool f32_store, hipStream_t st);
 void mlp_forward_launch(const mgpu_mlp* m, const float* x, int64_t n, float* y, hipStream_t st);
+// the same with the inputs formed from the cells' 8 nodal level-set values (vphi[n][8])
+void mlp_forward_phi_launch(const mgpu_mlp* m, const double* vphi, int64_t n, float* y, hipStream_t st);
 
 }  // namespace mgpu

@@ -208,8 +208,12 @@ __device__ inline void tile_infer(const MlpDev& d, InferLds& L, int lane) {
     __syncthreads();
 }
 
-// forward only: y[n][27]; each wavefront loops over 16-sample tiles
-__global__ __launch_bounds__(64) void k_mlp_forward(MlpDev d, const float* x, int64_t n, float* y) {
+// forward only: y[n][27]; each wavefront loops over 16-sample tiles.  PHI: the inputs are formed
+// from the 8 nodal level-set values of each cell, scaled by their largest magnitude (FP64, then
+// FP32), instead of being read from x.
+template <bool PHI>
+__global__ __launch_bounds__(64) void k_mlp_forward(MlpDev d, const float* x, const double* vphi, int64_t n,
+                                                    float* y) {
     __shared__ InferLds L;
     const int lane = threadIdx.x;
     const int64_t n_tiles = (n + 15) / 16;
@@ -218,7 +222,17 @@ __global__ __launch_bounds__(64) void k_mlp_forward(MlpDev d, const float* x, in
         for (int idx = lane; idx < 16 * KI; idx += 64) {
             const int sm = idx / KI, i = idx % KI;
             float v = 0.0f;
-            if (i < kLqInputs && s0 + sm < n) v = x[(s0 + sm) * kLqInputs + i];
+            if (i < kLqInputs && s0 + sm < n) {
+                if constexpr (PHI) {
+                    const double* ph = vphi + (s0 + sm) * kLqInputs;
+                    double m = 0.0;
+                    for (int k = 0; k < kLqInputs; ++k) m = fmax(m, fabs(ph[k]));
+                    const double inv = m > 0.0 ? 1.0 / m : 0.0;
+                    v = (float)(ph[i] * inv);
+                } else {
+                    v = x[(s0 + sm) * kLqInputs + i];
+                }
+            }
             L.xs[sm * KI + i] = f2bf(v);
         }
         __syncthreads();
@@ -411,7 +425,14 @@ void mlp_forward_launch(const mgpu_mlp* m, const float* x, int64_t n, float* y, 
         const int64_t n_tiles = (n + 15) / 16;
         // one tile per block: a 6.6 KB block leaves room for every tile of a set resident at once
         const unsigned grid = (unsigned)std::min<int64_t>(n_tiles, 1 << 20);
-        hipLaunchKernelGGL(k_mlp_forward, dim3(grid), dim3(64), 0, st, m->d, x, n, y);
+        hipLaunchKernelGGL(k_mlp_forward<false>, dim3(grid), dim3(64), 0, st, m->d, x, nullptr, n, y);
+    }
+}
+void mlp_forward_phi_launch(const mgpu_mlp* m, const double* vphi, int64_t n, float* y, hipStream_t st) {
+    if (n > 0) {
+        const int64_t n_tiles = (n + 15) / 16;
+        const unsigned grid = (unsigned)std::min<int64_t>(n_tiles, 1 << 20);
+        hipLaunchKernelGGL(k_mlp_forward<true>, dim3(grid), dim3(64), 0, st, m->d, nullptr, vphi, n, y);
     }
 }
 }  // namespace mgpu

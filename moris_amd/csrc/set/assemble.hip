@@ -662,8 +662,7 @@ mgpu_status launch_assemble(mgpu_set* S, const double* u, double* vals, double* 
     if (ev) HIP_TRY(hipEventRecord(ev[0], S->stream));
     const bool learned = S->v.n_cut > 0 && S->quadrature == MGPU_QUAD_LEARNED;
     if (learned) {
-        lq_launch_inputs(S->lv, S->stream);
-        mlp_forward_launch(S->mlp, S->lv.x_in, S->lv.n_cut, S->lv.w_pred, S->stream);
+        mlp_forward_phi_launch(S->mlp, S->lv.vphi, S->lv.n_cut, S->lv.w_pred, S->stream);
         if (ev) HIP_TRY(hipEventRecord(ev[1], S->stream));
         lq_launch_fit(S->v, S->lv, S->stream);
         HIP_TRY(hipGetLastError());
@@ -727,8 +726,6 @@ mgpu_status lq_prepare(mgpu_set* S) {
     mgpu_status rc;
     void* p;
     if ((rc = upload(S, S->vphi_cut.data(), S->vphi_cut.size(), const_cast<double**>(&lv.vphi))) != MGPU_OK) return rc;
-    if ((rc = alloc((size_t)nc * 8 * sizeof(float), &p)) != MGPU_OK) return rc;
-    lv.x_in = static_cast<float*>(p);
     if ((rc = alloc((size_t)nc * kLqPoints * sizeof(float), &p)) != MGPU_OK) return rc;
     lv.w_pred = static_cast<float*>(p);
     if ((rc = alloc((size_t)nc * kLqMomentsPad * sizeof(double), &p)) != MGPU_OK) return rc;
