@@ -347,7 +347,7 @@ def main():
                                    "algorithmic_flops": cut_flops_l,
                                    "TFLOP/s": cut_flops_l / (st_l[2] * 1e-3) / 1e12 if st_l[2] > 0 else None,
                                    "points": 27 * n_learned0 + fallback_ip},
-        "k_lq_inputs+k_mlp_forward": {"ms": st_l[0]},
+        "k_mlp_forward": {"ms": st_l[0]},
         "k_prep": {"ms": st_l[3]},
         "cut_integrands_fp64": {"ms": st64[2], "kernels": "k_cut_elasticity<3,double>", "algorithmic_flops": cut_flops_64,
                                 "TFLOP/s": cut_flops_64 / (st64[2] * 1e-3) / 1e12 if st64[2] > 0 else None,
