@@ -204,7 +204,8 @@ __global__ __launch_bounds__(64) void k_generic_element(GenView g) {
 
 // HEX27 IP cells: one block of 4 wavefronts per element.  Points (HEX_3x3x3 uncut, TET_35 on the
 // sub-tets, w* = w detJ, detJ < -1e-6 skipped) are staged 32 at a time as records of w* and the
-// 1D quadratic Lagrange values and derivatives per axis (two LDS buffers, one barrier per chunk);
+// 1D quadratic Lagrange values and derivatives per axis, in the MFMA operand type (two LDS buffers,
+// one barrier per chunk);
 // each lane forms its MFMA operands, the physical gradients dN_a/dx_i (times w* for A), from them
 // in registers, and the moments
 //   M[(i,a),(j,b)] = sum_ip w* dN_a/dx_i dN_b/dx_j      (81 x 81, padded to 96 x 96)
@@ -252,7 +253,7 @@ template <> struct H27Tiles<3> {
 // the lane's record offsets (packed 3 x 8 bits) and scale for column group cg (columns cg*16 + lane
 // % 16; zero beyond column 80).
 template <int W, typename T>
-__device__ inline void h27_chunk_direct(const double* pts, const int* go, const double* gs, int kq, doublex4_t* acc) {
+__device__ inline void h27_chunk_direct(const T* pts, const int* go, const T* gs, int kq, doublex4_t* acc) {
     using TL = H27Tiles<W>;
     constexpr int lo = W < 3 ? W : 3;   // the lowest column group of the wavefront's tiles
     [[maybe_unused]] floatx4_t accf[6];
@@ -262,9 +263,9 @@ __device__ inline void h27_chunk_direct(const double* pts, const int* go, const 
     }
 #pragma unroll 2
     for (int ks = 0; ks < kH27Chunk / 4; ++ks) {
-        const double* pp = pts + (4 * ks + kq) * kH27PtStride;
-        const double w = pp[0];
-        double v[6];
+        const T* pp = pts + (4 * ks + kq) * kH27PtStride;
+        const T w = pp[0];
+        T v[6];
 #pragma unroll
         for (int cg = lo; cg < 6; ++cg) {
             const int o = go[cg];
@@ -272,7 +273,7 @@ __device__ inline void h27_chunk_direct(const double* pts, const int* go, const 
         }
 #pragma unroll
         for (int t = 0; t < TL::n; ++t) {
-            const T a = (T)(v[TL::mt[t]] * w), b = (T)v[TL::nt[t]];
+            const T a = v[TL::mt[t]] * w, b = v[TL::nt[t]];
             if constexpr (sizeof(T) == 8) acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[t], 0, 0, 0);
             else accf[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, accf[t], 0, 0, 0);
         }
@@ -320,17 +321,17 @@ __global__ __launch_bounds__(256) void k_generic_hex27(GenView g, int c0, double
     // the lane's operand columns cg * 16 + lane % 16, cg < 6: (i, a) = (col / 27, col % 27), the
     // record offsets of dl_i(xi_i), l(xi_j), l(xi_k) and 2 / h_i (zero beyond column 80)
     int go[6];
-    double gs[6];
+    T gs[6];   // the records and operands are in T (FP64, or FP32 on the mixed path)
 #pragma unroll
     for (int cg = 0; cg < 6; ++cg) {
         const int col = cg * 16 + col16;
         go[cg] = 0;
-        gs[cg] = 0.0;
+        gs[cg] = T(0);
         if (col < NG) {
             const int i = col / NB, a = col - NB * (col / NB);
             go[cg] = ((i == 0 ? 10 : 1) + kHex27Half[a][0]) | (((i == 1 ? 10 : 1) + 3 + kHex27Half[a][1]) << 8) |
                      (((i == 2 ? 10 : 1) + 6 + kHex27Half[a][2]) << 16);
-            gs[cg] = 2.0 / hh[i];
+            gs[cg] = (T)(2.0 / hh[i]);
         }
     }
     doublex4_t acc[6];
@@ -340,13 +341,13 @@ __global__ __launch_bounds__(256) void k_generic_hex27(GenView g, int c0, double
     // point records in two buffers: a chunk's records are written while the other wavefronts may
     // still read the previous chunk's, and the barrier of a chunk orders the one before it
     for (int p0 = 0, buf = 0; p0 < npts; p0 += kH27Chunk, buf ^= 1) {
-        double* pts = lds + buf * kH27Chunk * kH27PtStride;
+        T* pts = reinterpret_cast<T*>(lds) + buf * kH27Chunk * kH27PtStride;
         // the chunk's points: w* and the 1D quadratic Lagrange values / derivatives per axis, by 4
         // threads per point (one per axis, one for the weight)
         if (tid < 4 * kH27Chunk) {
             const int pl = tid & (kH27Chunk - 1), role = tid / kH27Chunk;
             const int ip = p0 + pl;
-            double* pp = pts + pl * kH27PtStride;
+            T* pp = pts + pl * kH27PtStride;
             const int t = ip / kTet35Points, q = ip - kTet35Points * t;
             if (role < 3) {
                 const int d = role;
@@ -361,7 +362,13 @@ __global__ __launch_bounds__(256) void k_generic_hex27(GenView g, int c0, double
                         xd = N[0] * X[d] + N[1] * X[3 + d] + N[2] * X[6 + d] + N[3] * X[9 + d];
                     }
                 }
-                lagrange_quadratic_1d(xd, pp + 1 + 3 * d, pp + 10 + 3 * d);
+                double l[3], dl[3];
+                lagrange_quadratic_1d(xd, l, dl);
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    pp[1 + 3 * d + k] = (T)l[k];
+                    pp[10 + 3 * d + k] = (T)dl[k];
+                }
             } else {
                 double w = 0.0;
                 if (ip < npts) {
@@ -381,7 +388,7 @@ __global__ __launch_bounds__(256) void k_generic_hex27(GenView g, int c0, double
                         w = detJ < kDetJLowerLimit ? 0.0 : kTet35W[q] * detJ;
                     }
                 }
-                pp[0] = w;
+                pp[0] = (T)w;
             }
         }
         __syncthreads();
