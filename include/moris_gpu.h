@@ -215,7 +215,7 @@ mgpu_status mgpu_memcpy_dtoh(void* dst, const void* src, size_t bytes);
 mgpu_status mgpu_synchronize(void);
 /* Time `iters` back-to-back mgpu_set_assemble calls with HIP events on the set's stream, after
    one untimed call.  per_call_ms receives the mean; kernel_ms[5] the mean per-call durations, from events around
-   each stage of separate calls: [0] learned inputs + network (k_lq_inputs, k_mlp_forward),
+   each stage of separate calls: [0] network (k_mlp_forward, inputs formed from the level set),
    [1] moment fit (k_lq_moments, k_lq_project), [2] cut-element integrands (k_lq_element and the subdivision
    kernel), [3] cell records (k_prep), [4] stencil scatter (k_stencil); 0 where a stage does not
    run.                                                                                          */

@@ -15,10 +15,12 @@
 //                   cl_FEM_IWG_Isotropic_Struc_Linear_Bulk.cpp:200, cl_FEM_IWG_Diffusion_Bulk.cpp
 //                   :280) is projected with the element's T rows into sorted adof order
 //                   (cl_MSI_Equation_Object.cpp:810-841) and stored.
+//   k_prep          u gathered per lattice node, one material record per lattice cell.
 //   k_stencil       owner-computes scatter: one thread per CSR row.  The row sums the K_e rows of
 //                   the (up to 8 / 4) lattice cells around its node -- uncut cells from the
-//                   block-shared reference moments (HEX_2x2x2 / QUAD_2x2 on the IP cell), cut cells
-//                   from k_cut_moments -- and writes every value of the row exactly once; the row
+//                   compile-time reference moments (HEX_2x2x2 / QUAD_2x2 on the IP cell) and the
+//                   cell record, cut cells from the stored K_e -- and writes every value of the
+//                   row exactly once; the row
 //                   of R is the same row applied to u (R = sum_e K_e u_e, linear IWGs).  Replaces
 //                   Epetra_FECrsMatrix::SumIntoGlobalValues (cl_Sparse_Matrix_EpetraFECrs.cpp
 //                   :103-133); deterministic, no atomics.
@@ -188,13 +190,12 @@ __global__ __launch_bounds__(64) void k_cut_elasticity(DevView s) {
 }
 
 // ------------------------------------------------------------------------------------------------
-// k_prep: per-call constants of the stencil kernel.  Block 0 computes the reference moments of an
-// uncut lattice cell (IG cell = IP cell, HEX_2x2x2 / QUAD_2x2, detJ = prod(h/2), dN/dx = dN/dxi 2/h):
-//   diffusion   S[a][b]        = sum_q w* g_a . g_b
-//   elasticity  M[(i,a),(j,b)] = sum_q w* g_ia g_jb
-// the other blocks write one record per lattice cell (void / uncut material / cut index) with the
-// material folded in: k (CM_Diffusion_Linear_Isotropic) or the three distinct entries of the
-// isotropic D (CM_Struc_Linear_Isotropic::full_3d / full_plane_stress).
+// k_prep: per-call inputs of the stencil kernel, one thread per lattice node / cell.  Each node
+// gathers u of its adofs into the planar u_node (0 where the node has no adof); each cell writes
+// one record (void / uncut material / cut index) with the material folded in: k
+// (CM_Diffusion_Linear_Isotropic) or the three distinct entries of the isotropic D
+// (CM_Struc_Linear_Isotropic::full_3d / full_plane_stress).  The reference moments of an uncut
+// cell are compile-time constants (kRefGrad), scaled by the stencil itself.
 // ------------------------------------------------------------------------------------------------
 template <int DIM, int NF>
 __global__ __launch_bounds__(256) void k_prep(DevView s, const double* u) {
