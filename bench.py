@@ -52,6 +52,13 @@ def log(*a):
 # with two rocprofv3 --pmc runs (FETCH_SIZE scaled x2.0, WRITE_SIZE x1.0 by the copy-kernel
 # calibration): 97,555 KiB x 2 + 445,915 KiB.
 STENCIL_TRAFFIC_BYTES = (97555 * 2 + 445915) * 1024
+# The same for k_generic_hex27<3,double,false> on the 32^3 HEX27 gyroid set (gather mode):
+# 109,644 KiB fetched, 1,470,336 KiB written (the K_e pool).
+HEX27_32_TRAFFIC_BYTES = (109644 * 2 + 1470336) * 1024
+# The same for the FP64 k_stencil<3,1,ROWS,DIRECT,double> of the 128^3 diffusion set, per stencil
+# variant (the autotuner's choice): (FETCH_SIZE, WRITE_SIZE) KiB per launch.
+C2_STENCIL_TRAFFIC_KIB = {0: (100537, 428126), 1: (94277, 433217), 2: (68008, 426869),
+                          3: (102682, 471827), 4: (95728, 464089), 5: (68942, 459620)}
 
 
 def stencil_bytes(spec, S, n_elements, n_cut, ke_bytes, nf=3, nb=8, dim=3):
@@ -170,7 +177,10 @@ def run_hex27(args):
     out["roofline"] = {"bound": "mfma", "kernel": "k_generic_hex27<3,double>",
                        "achieved": out["fp64"]["element_kernel_mfma_TFLOP/s"], "peak": FP64_PEAK_TFLOPS,
                        "unit": "TFLOP/s", "frac": out["fp64"]["element_kernel_mfma_TFLOP/s"] / FP64_PEAK_TFLOPS,
-                       "traffic": None}
+                       "traffic": HEX27_32_TRAFFIC_BYTES if n == 32 and not args.refine and mode == "gather"
+                       else None}
+    if out["roofline"]["traffic"] is not None:
+        out["roofline"]["traffic_source"] = "rocprofv3 --pmc FETCH_SIZE (x2.0) + WRITE_SIZE (x1.0) per launch"
     if not args.no_cpu:
         log("hex27 cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(P, S, u, budget_s=15.0, probe_n=200, rule="TET_35")
@@ -218,8 +228,15 @@ def run_lattice(args):
     out["dtype"] = "f64"
     out["max_rel_err_K_mixed_vs_fp64"] = float(np.linalg.norm(res["mixed"] - res["fp64"]) / np.linalg.norm(res["fp64"]))
     gbs = out["fp64"]["k_stencil_GB/s"]
+    traffic = None
+    if args.workload == "c2" and n == 128:
+        f_kib, w_kib = C2_STENCIL_TRAFFIC_KIB[out["fp64"]["tuning"][0]]
+        traffic = (f_kib * 2 + w_kib) * 1024
     out["roofline"] = {"bound": "hbm", "kernel": "k_stencil (FP64 path)", "achieved": gbs, "peak": HBM_PEAK_GBS,
-                       "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS, "traffic": None}
+                       "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS, "traffic": traffic}
+    if traffic is not None:
+        out["roofline"]["traffic_source"] = ("rocprofv3 --pmc FETCH_SIZE (x2.0) + WRITE_SIZE (x1.0) per launch of "
+                                             "the tuned stencil variant")
     if not args.no_cpu:
         log("%s cpu baseline ..." % args.workload)
         out["cpu_baseline"] = cpu_baseline(P, S, u, budget_s=15.0, rule=rule)
