@@ -113,7 +113,7 @@ def cpu_baseline(P, S, u, budget_s=20.0, probe_n=4000, rule="TET_11"):
     best = min(runs)
     cut_frac = float(np.mean(np.diff(P.arrays()["ig_ptr"])[sample] > 0))
     return {"value": len(sample) / best, "unit": "elements/s", "cores": cores, "kind": "port",
-            "sample": "every %d-th element of variant 0 (%d of %d elements, cut fraction %.3f), FP64 subdivision "
+            "sample": "one element in %d of variant 0 (%d of %d elements, cut fraction %.3f), FP64 subdivision "
                       "(%s), best of %d runs, %d worker threads with private CSR arrays + row-wise reduction"
                       % (stride, len(sample), ne, cut_frac, rule, len(runs), cores)}
 
