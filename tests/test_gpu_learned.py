@@ -36,6 +36,28 @@ def test_mlp_forward_matches_cpu_restatement():
     assert rel(y, y_64) < 3e-2
 
 
+def test_network_inputs_formed_from_the_level_set_match_explicit_inputs():
+    """The assembly forms each cut element's network input inside the forward kernel: its 8
+    nodal level-set values times 1 / max |phi| in FP64, then FP32.  The same inputs formed on the
+    host and given to the forward pass run the same BF16 arithmetic: identical predictions."""
+    spec, keep = _elastic_sphere(12)
+    P = M.Problem(spec, keep)
+    S = M.Set(P.desc, keepalive=P)
+    net = L.MLP(seed=6)
+    L.attach(S, net)
+    S.options(M.PREC_MIXED, M.QUAD_LEARNED)
+    S.compute_jacobian_and_residual(np.random.default_rng(3).uniform(-1.0, 1.0, S.n_rows))
+    wp, _, _ = L.result(S)
+    a = P.arrays()
+    cut = np.flatnonzero(np.diff(a["ig_ptr"]) > 0)
+    assert len(cut) == S.n_cut > 16
+    ph = a["vertex_phi"].reshape(-1, 8)[cut]
+    m = np.abs(ph).max(axis=1)
+    inv = np.where(m > 0.0, 1.0 / np.where(m > 0.0, m, 1.0), 0.0)
+    x = (ph * inv[:, None]).astype(np.float32)
+    np.testing.assert_array_equal(wp, net.forward(x))
+
+
 def test_mlp_gradients_match_cpu_restatement():
     net = L.MLP(seed=4)
     rng = np.random.default_rng(1)
