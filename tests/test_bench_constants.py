@@ -48,3 +48,12 @@ def test_profiles_name_their_commands():
         with open(os.path.join(ROOT, "profiles", name)) as f:
             text = f.read()
         assert re.search(r"rocprofv3 |mgpu_hbm_calibrate", text), name
+
+
+def test_thb48_scatter_traffic_matches_its_profile():
+    f, w = _row("k_generic_hex27_scatter_elasticity_p2_thb_gyroid_48_hbm_traffic.md",
+                "k_generic_hex27<3,double,true>, 42 launches (FP64)")
+    assert bench.THB48_SCATTER_TRAFFIC_BYTES == (2 * f + w) * 1024
+    # the zero fill writes the 949,141,152 FP64 values once (0.11 % over, measured), to within 0.5 %
+    _, wz = _row("k_generic_hex27_scatter_elasticity_p2_thb_gyroid_48_hbm_traffic.md", "k_generic_zero")
+    assert abs(wz * 1024 - 949141152 * 8) < 5e-3 * 949141152 * 8
