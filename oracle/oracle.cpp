@@ -431,6 +431,98 @@ bool intersected(const OracleProblem& p, int64_t c) {
     return false;
 }
 
+// Node hierarchy of one IG cell of the regular subdivision (tet in 3D, tri in 2D; cl_XTK_Node_
+// Hierarchy_Interface.cpp:160-287, 672-968): g, f, x are its vertices' global ids, level-set values
+// and parametric coordinates.  Appends the vertex coordinates of its sub-cells of phase `phase`
+// (region() of their non-interface vertices) to sub.
+void nh_cell(int D, const uint32_t* g, const double* f, const std::array<double, 3>* x, int phase,
+             std::vector<double>& sub) {
+    const int nv = D + 1;
+    // intersected edges: (ordinal, key, parametric point)
+    struct IE { int ord; uint64_t key; double x[3]; };
+    std::vector<IE> ies;
+    const int ne = D == 3 ? 6 : 3;
+    for (int e = 0; e < ne; ++e) {
+        const int l0 = D == 3 ? kTetE[e][0] : kTriE[e][0];
+        const int l1 = D == 3 ? kTetE[e][1] : kTriE[e][1];
+        const double f0 = f[l0], f1 = f[l1];
+        // an endpoint on the interface: no general intersection on this edge
+        if (region(f0) == 2 || region(f1) == 2) continue;
+        if (region(f0) == region(f1)) continue;
+        // parent nodes ordered by global id
+        const int la = g[l0] < g[l1] ? l0 : l1;
+        const int lb = g[l0] < g[l1] ? l1 : l0;
+        const double fa = f[la], fb = f[lb];
+        const double loc = (2.0 * 0.0 - fb - fa) / (fb - fa);
+        if (!(std::fabs(loc) <= 1.0)) continue;
+        IE ie;
+        ie.ord = e;
+        ie.key = ((uint64_t)g[la] << 32) | g[lb];
+        for (int d = 0; d < 3; ++d) ie.x[d] = 0.5 * (1.0 - loc) * x[la][d] + 0.5 * (1.0 + loc) * x[lb][d];
+        ies.push_back(ie);
+    }
+    auto phase_of = [&](int local) { return region(f[local]); };
+    if (ies.empty()) {
+        int ph = -1;
+        for (int q = 0; q < nv && ph < 0; ++q)
+            if (phase_of(q) != 2) ph = phase_of(q);
+        if (ph == phase)
+            for (int q = 0; q < nv; ++q)
+                for (int d = 0; d < D; ++d) sub.push_back(x[q][d]);
+        return;
+    }
+    Tmpl tm;
+    int nodeord[4] = {0, 1, 2, 3};
+    std::vector<IE> sorted = ies;
+    if (D == 3) {
+        std::stable_sort(sorted.begin(), sorted.end(), [](const IE& a, const IE& b) { return a.key < b.key; });
+        int perm = 0;
+        if (sorted.size() == 1) {
+            perm = 10000 + sorted[0].ord;
+        } else if (sorted.size() == 2) {
+            perm = 10000 + 100 * (sorted[0].ord + 1) + 10 * (sorted[1].ord + 1);
+        } else if (sorted.size() == 3) {
+            const int L = sorted[0].ord, M = sorted[1].ord, H = sorted[2].ord;
+            const int n0 = tet_shared(L, M);
+            auto other = [&](int e) { return kTetE[e][0] == n0 ? kTetE[e][1] : kTetE[e][0]; };
+            nodeord[0] = n0; nodeord[1] = other(L); nodeord[2] = other(M); nodeord[3] = other(H);
+            perm = L + 10 * M + 100 * H;
+        } else if (sorted.size() == 4) {
+            const int L = sorted[0].ord, ML = sorted[1].ord, MH = sorted[2].ord, H = sorted[3].ord;
+            perm = L + 10 * ML + 100 * MH + 1000 * H;
+            const int HL = tet_shared(L, H), HMH = tet_shared(MH, H), HML = tet_shared(ML, H);
+            const int LML = tet_shared(L, ML), LMH = tet_shared(L, MH), MLMH = tet_shared(ML, MH);
+            if (HL < 0) { nodeord[0] = HMH; nodeord[1] = HML; nodeord[2] = LMH; nodeord[3] = LML; }
+            else if (HMH < 0) { nodeord[0] = HL; nodeord[1] = HML; nodeord[2] = LMH; nodeord[3] = MLMH; }
+            else { nodeord[0] = HL; nodeord[1] = HMH; nodeord[2] = LML; nodeord[3] = MLMH; }
+        } else {
+            throw std::runtime_error("unexpected intersected edge count");
+        }
+        if (!load_3d_template(perm, tm)) throw std::runtime_error("template not found");
+    } else {
+        std::stable_sort(sorted.begin(), sorted.end(), [](const IE& a, const IE& b) { return a.ord < b.ord; });
+        int id;
+        if (sorted.size() == 1) id = 10 + sorted[0].ord;
+        else if (sorted.size() == 2) id = sorted[0].ord + sorted[1].ord;
+        else throw std::runtime_error("unexpected 2D cut");
+        if (!load_2d_template(id, tm)) throw std::runtime_error("template not found");
+    }
+    for (int cidx = 0; cidx < tm.n; ++cidx) {
+        int ph = -1;
+        for (int q = 0; q < nv && ph < 0; ++q)
+            if (tm.c[cidx][q] < nv && phase_of(nodeord[tm.c[cidx][q]]) != 2) ph = phase_of(nodeord[tm.c[cidx][q]]);
+        if (ph != phase) continue;
+        for (int q = 0; q < nv; ++q) {
+            const int o = tm.c[cidx][q];
+            if (o < nv) {
+                for (int d = 0; d < D; ++d) sub.push_back(x[nodeord[o]][d]);
+            } else {
+                for (int d = 0; d < D; ++d) sub.push_back(sorted[o - nv].x[d]);
+            }
+        }
+    }
+}
+
 // XTK decomposition of all intersected cells; fills per-cell material sub-cells.
 void decompose(OracleProblem& p, std::map<int64_t, std::vector<double>>& cell_sub) {
     const int D = p.dim;
@@ -488,93 +580,16 @@ void decompose(OracleProblem& p, std::map<int64_t, std::vector<double>>& cell_su
         const int nv = D + 1;
         std::vector<double> sub;
         for (int t = 0; t < ncell; ++t) {
-            int tv[4];
-            for (int q = 0; q < nv; ++q) tv[q] = D == 3 ? kR24[t][q] : kR4[t][q];
-            // intersected edges: (ordinal, key, parametric point)
-            struct IE { int ord; uint64_t key; double x[3]; };
-            std::vector<IE> ies;
-            const int ne = D == 3 ? 6 : 3;
-            for (int e = 0; e < ne; ++e) {
-                const int l0 = tv[D == 3 ? kTetE[e][0] : kTriE[e][0]];
-                const int l1 = tv[D == 3 ? kTetE[e][1] : kTriE[e][1]];
-                const double f0 = vphi[gid[l0]], f1 = vphi[gid[l1]];
-                // an endpoint on the interface: no general intersection on this edge
-                if (region(f0) == 2 || region(f1) == 2) continue;
-                if (region(f0) == region(f1)) continue;
-                // parent nodes ordered by global id
-                const int la = gid[l0] < gid[l1] ? l0 : l1;
-                const int lb = gid[l0] < gid[l1] ? l1 : l0;
-                const double fa = vphi[gid[la]], fb = vphi[gid[lb]];
-                const double loc = (2.0 * 0.0 - fb - fa) / (fb - fa);
-                if (!(std::fabs(loc) <= 1.0)) continue;
-                IE ie;
-                ie.ord = e;
-                ie.key = ((uint64_t)gid[la] << 32) | gid[lb];
-                for (int d = 0; d < 3; ++d) ie.x[d] = 0.5 * (1.0 - loc) * xi[la][d] + 0.5 * (1.0 + loc) * xi[lb][d];
-                ies.push_back(ie);
+            uint32_t tg[4];
+            double tf[4];
+            std::array<double, 3> tx[4];
+            for (int q = 0; q < nv; ++q) {
+                const int lv = D == 3 ? kR24[t][q] : kR4[t][q];
+                tg[q] = gid[lv];
+                tf[q] = vphi[gid[lv]];
+                tx[q] = xi[lv];
             }
-            auto tet_vertex_xi = [&](int local_in_tet) { return xi[tv[local_in_tet]]; };
-            auto phase_of = [&](int local_in_tet) { return region(vphi[gid[tv[local_in_tet]]]); };
-            if (ies.empty()) {
-                int ph = -1;
-                for (int q = 0; q < nv && ph < 0; ++q)
-                    if (phase_of(q) != 2) ph = phase_of(q);
-                if (ph == p.spec.material_phase)
-                    for (int q = 0; q < nv; ++q)
-                        for (int d = 0; d < D; ++d) sub.push_back(tet_vertex_xi(q)[d]);
-                continue;
-            }
-            Tmpl tm;
-            int nodeord[4] = {0, 1, 2, 3};
-            std::vector<IE> sorted = ies;
-            if (D == 3) {
-                std::stable_sort(sorted.begin(), sorted.end(), [](const IE& a, const IE& b) { return a.key < b.key; });
-                int perm = 0;
-                if (sorted.size() == 1) {
-                    perm = 10000 + sorted[0].ord;
-                } else if (sorted.size() == 2) {
-                    perm = 10000 + 100 * (sorted[0].ord + 1) + 10 * (sorted[1].ord + 1);
-                } else if (sorted.size() == 3) {
-                    const int L = sorted[0].ord, M = sorted[1].ord, H = sorted[2].ord;
-                    const int n0 = tet_shared(L, M);
-                    auto other = [&](int e) { return kTetE[e][0] == n0 ? kTetE[e][1] : kTetE[e][0]; };
-                    nodeord[0] = n0; nodeord[1] = other(L); nodeord[2] = other(M); nodeord[3] = other(H);
-                    perm = L + 10 * M + 100 * H;
-                } else if (sorted.size() == 4) {
-                    const int L = sorted[0].ord, ML = sorted[1].ord, MH = sorted[2].ord, H = sorted[3].ord;
-                    perm = L + 10 * ML + 100 * MH + 1000 * H;
-                    const int HL = tet_shared(L, H), HMH = tet_shared(MH, H), HML = tet_shared(ML, H);
-                    const int LML = tet_shared(L, ML), LMH = tet_shared(L, MH), MLMH = tet_shared(ML, MH);
-                    if (HL < 0) { nodeord[0] = HMH; nodeord[1] = HML; nodeord[2] = LMH; nodeord[3] = LML; }
-                    else if (HMH < 0) { nodeord[0] = HL; nodeord[1] = HML; nodeord[2] = LMH; nodeord[3] = MLMH; }
-                    else { nodeord[0] = HL; nodeord[1] = HMH; nodeord[2] = LML; nodeord[3] = MLMH; }
-                } else {
-                    throw std::runtime_error("unexpected intersected edge count");
-                }
-                if (!load_3d_template(perm, tm)) throw std::runtime_error("template not found");
-            } else {
-                std::stable_sort(sorted.begin(), sorted.end(), [](const IE& a, const IE& b) { return a.ord < b.ord; });
-                int id;
-                if (sorted.size() == 1) id = 10 + sorted[0].ord;
-                else if (sorted.size() == 2) id = sorted[0].ord + sorted[1].ord;
-                else throw std::runtime_error("unexpected 2D cut");
-                if (!load_2d_template(id, tm)) throw std::runtime_error("template not found");
-            }
-            for (int cidx = 0; cidx < tm.n; ++cidx) {
-                int ph = -1;
-                for (int q = 0; q < nv && ph < 0; ++q)
-                    if (tm.c[cidx][q] < nv && phase_of(nodeord[tm.c[cidx][q]]) != 2) ph = phase_of(nodeord[tm.c[cidx][q]]);
-                if (ph != p.spec.material_phase) continue;
-                for (int q = 0; q < nv; ++q) {
-                    const int o = tm.c[cidx][q];
-                    if (o < nv) {
-                        auto a = tet_vertex_xi(nodeord[o]);
-                        for (int d = 0; d < D; ++d) sub.push_back(a[d]);
-                    } else {
-                        for (int d = 0; d < D; ++d) sub.push_back(sorted[o - nv].x[d]);
-                    }
-                }
-            }
+            nh_cell(D, tg, tf, tx, p.spec.material_phase, sub);
         }
         cell_sub[c] = std::move(sub);
     }
@@ -1371,6 +1386,29 @@ void oracle_shape_hex27(int n, const double* xi, double* N, double* dN) {
 
 // Quadrature rule: 0 QUAD_2x2, 1 HEX_2x2x2, 2 TRI_7, 3 TET_11, 4 HEX_3x3x3, 5 TET_35.  Writes
 // points [n][dim] and weights [n]; returns n.
+// Node hierarchy of one IG cell (nh_cell): vertex global ids g[dim+1], level-set values f[dim+1],
+// coordinates x[dim+1][3]; writes the sub-cells of phase `phase` as [n][dim+1][3] into out (at most
+// max_cells) and returns n, or -1 on error.
+int oracle_nh_cell(int dim, const uint32_t* g, const double* f, const double* x, int phase, double* out,
+                   int max_cells) {
+    try {
+        std::array<double, 3> xa[4];
+        for (int q = 0; q <= dim; ++q) xa[q] = {x[3 * q], x[3 * q + 1], x[3 * q + 2]};
+        std::vector<double> sub;
+        nh_cell(dim, g, f, xa, phase, sub);
+        const int nv = dim + 1;
+        const int n = (int)(sub.size() / (nv * dim));
+        if (n > max_cells) throw std::runtime_error("too many sub-cells");
+        for (int c = 0; c < n; ++c)
+            for (int q = 0; q < nv; ++q)
+                for (int d = 0; d < 3; ++d) out[(c * nv + q) * 3 + d] = d < dim ? sub[(c * nv + q) * dim + d] : 0.0;
+        return n;
+    } catch (const std::exception& ex) {
+        g_err = ex.what();
+        return -1;
+    }
+}
+
 int oracle_rule(int rule, double* xi, double* w) {
     switch (rule) {
         case 4: {

@@ -190,3 +190,20 @@ def assemble_list(desc, row_ptr, col_idx, u, elems, n_threads=1):
     if nip < 0:
         raise RuntimeError("oracle_assemble_list: " + lib.oracle_last_error().decode())
     return vals, rhs, nip
+
+
+def nh_cell(dim, gid, phi, x, phase):
+    """Node hierarchy of one IG cell (tet / tri): sub-cells of `phase` as [n][dim+1][3]."""
+    lib = load()
+    lib.oracle_nh_cell.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                   ctypes.c_void_p, ctypes.c_int]
+    lib.oracle_nh_cell.restype = ctypes.c_int
+    g = np.ascontiguousarray(gid, dtype=np.uint32)
+    f = np.ascontiguousarray(phi, dtype=np.float64)
+    xx = np.zeros((dim + 1, 3))
+    xx[:, :dim] = np.asarray(x, dtype=np.float64)[:, :dim]
+    out = np.zeros(16 * (dim + 1) * 3)
+    n = lib.oracle_nh_cell(dim, _p(g), _p(f), _p(xx), phase, _p(out), 16)
+    if n < 0:
+        raise RuntimeError("oracle_nh_cell: " + lib.oracle_last_error().decode())
+    return out[: n * (dim + 1) * 3].reshape(n, dim + 1, 3)

@@ -484,3 +484,51 @@ def test_thb_rigid_motions_null_space():
     for u in (np.concatenate([rot[:, 0], rot[:, 1], rot[:, 2]]), np.concatenate([np.ones(N), np.zeros(2 * N)])):
         Ku = csr_matvec(rp, ci, vals, u)
         assert np.linalg.norm(Ku) < 1e-12 * np.abs(vals).max() * np.linalg.norm(u) * np.sqrt(len(u))
+
+
+@pytest.mark.parametrize("lagrange_order", [1, 2])
+def test_quadratic_bspline_rows_reproduce_quadratics(lagrange_order):
+    """p = 2 T rows at the Lagrange nodes reproduce quadratic fields.  The coefficient of x^2 on the
+    uniform quadratic B-spline with knots t, t+h, t+2h, t+3h is the product of its two interior knots
+    (the blossom of x^2), (t+h)(t+2h); that of xy the product of the two axes' Greville abscissae.
+    Partition of unity and linear reproduction leave the weights at a cell-centre node as the family
+    (a, 1-2a, a); only this check fixes a = 1/8 (and 1/2, 1/2 at a lattice node)."""
+    n = 4
+    spec, keep = M.make_spec(3, (n, n, n), iwg=1, spheres=[[0.5, 0.5, 0.5, 0.3]], bspline_order=2,
+                             lagrange_order=lagrange_order)
+    P = O.OracleProblem(spec)
+    s = P.setup()
+    nb = 27 if lagrange_order == 2 else 8
+    N = P.n_adofs_per_type
+    T = _t_dense(s, N)
+    Xn = _node_coords(s, nb)                    # every (element, basis) node, T-row order
+    G = _greville_coords(s, 3, (n, n, n))       # adof Greville abscissae (s - 1/2) h
+    h = 1.0 / n
+    knot_lo = G - 1.5 * h                       # first knot t of each spline
+    c2 = (knot_lo + h) * (knot_lo + 2 * h)      # interior-knot products per axis
+    for d in range(3):
+        assert np.abs(T @ c2[:, d] - Xn[:, d] ** 2).max() < 1e-14
+    assert np.abs(T @ (G[:, 0] * G[:, 1]) - Xn[:, 0] * Xn[:, 1]).max() < 1e-14
+    if lagrange_order == 2:
+        # the three non-zero weights of a cell-centre coordinate, per axis
+        w = np.sort(np.unique(np.round(s["t_weight"], 15)))
+        assert np.any(np.isclose(w, 0.125 ** 3)) and np.any(np.isclose(w, 0.75 ** 3))
+
+
+def test_thb_truncated_basis_reproduces_quadratics():
+    """The truncated hierarchical space contains the level-0 quadratic splines, hence every quadratic
+    polynomial: least-squares coefficients reproduce x^2, y z and x y at every Lagrange node of both
+    levels to round-off.  A wrong two-scale weight of the truncation breaks this while keeping the
+    partition of unity."""
+    n = 4
+    spec, keep = M.make_spec(3, (n, n, n), iwg=1, spheres=[[0.5, 0.5, 0.5, 0.3]], bspline_order=2,
+                             lagrange_order=2, refine_levels=1)
+    P = O.OracleProblem(spec)
+    s = P.setup()
+    T = _t_dense(s, P.n_adofs_per_type)
+    X = _node_coords(s, 27)
+    h = s["ip_box"].reshape(-1, 6)[:, 3]
+    assert (h < 0.75 / n).sum() > 0 and (h > 0.75 / n).sum() > 0
+    for f in (X[:, 0] ** 2, X[:, 1] * X[:, 2], X[:, 0] * X[:, 1], X[:, 2] ** 2):
+        c = np.linalg.lstsq(T, f, rcond=None)[0]
+        assert np.abs(T @ c - f).max() < 1e-13
