@@ -196,7 +196,10 @@ mgpu_status mgpu_element_topology_csr(const mgpu_set* set, const int64_t** offse
    csr_vals[nnz] and rhs[n_rows] are overwritten (zeroed then summed, cl_DLA_Linear_Problem.cpp
    :131-138 mat_put_scalar(0) + fill_matrix_and_RHS); u_adof[n_rows] is the adof vector
    (compute_my_pdof_values, cl_MSI_Equation_Object.cpp:1100-1131).  Asynchronous on the set's
-   stream.                                                                                       */
+   stream: a kernel-side failure (a stencil row whose length disagrees with the pattern, whose
+   values are then left unwritten) is reported by the next synchronising call on the set
+   (mgpu_set_assemble_host, mgpu_set_time_assemble, mgpu_set_autotune, mgpu_batch_synchronize),
+   once; the flag is cleared when it is reported.                                               */
 mgpu_status mgpu_set_assemble(mgpu_set* set, const double* u_adof, double* csr_vals, double* rhs);
 /* Same with HOST arrays; copies in and out, synchronous. */
 mgpu_status mgpu_set_assemble_host(mgpu_set* set, const double* u_adof, double* csr_vals, double* rhs);
@@ -205,6 +208,11 @@ mgpu_status mgpu_set_assemble_host(mgpu_set* set, const double* u_adof, double* 
    [2] cut elements integrated by learned quadrature, [3] learned elements rejected by the moment
    check, [4] integration points evaluated.                                                      */
 mgpu_status mgpu_set_stats(const mgpu_set* set, int64_t stats[5]);
+/* The same statistics and more, the first n_stats of: [0]..[4] as mgpu_set_stats, [5] integration
+   points skipped because their sub-cell has detJ < -1e-6 (Element_Bulk::compute_jacobian_and_
+   residual, cl_FEM_Element_Bulk.cpp:291 -- a reported statistic, not an error), [6] such
+   sub-cells.  Entries beyond [6] are set to 0.                                                */
+mgpu_status mgpu_set_statistics(const mgpu_set* set, int64_t* stats, int32_t n_stats);
 
 /* Device helpers (so callers need no HIP headers). */
 mgpu_status mgpu_device_count(int32_t* n);

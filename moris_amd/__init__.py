@@ -87,6 +87,7 @@ EXPORTED = [
     "mgpu_problem_build", "mgpu_problem_desc", "mgpu_problem_counts", "mgpu_problem_destroy",
     "mgpu_set_create", "mgpu_set_destroy", "mgpu_set_options", "mgpu_set_stream", "mgpu_pattern_get",
     "mgpu_element_topology", "mgpu_element_topology_csr", "mgpu_set_assemble", "mgpu_set_assemble_host", "mgpu_set_stats",
+    "mgpu_set_statistics",
     "mgpu_device_count", "mgpu_malloc", "mgpu_free", "mgpu_memcpy_htod", "mgpu_memcpy_dtoh",
     "mgpu_synchronize", "mgpu_set_time_assemble", "mgpu_last_error_string",
     "mgpu_set_tuning", "mgpu_set_get_tuning", "mgpu_set_tuning_variants", "mgpu_set_autotune",
@@ -126,6 +127,7 @@ def load_library(path=LIB_PATH):
     lib.mgpu_set_assemble.argtypes = [vp, vp, vp, vp]
     lib.mgpu_set_assemble_host.argtypes = [vp, _c_f64p, _c_f64p, _c_f64p]
     lib.mgpu_set_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
+    lib.mgpu_set_statistics.argtypes = [vp, ctypes.POINTER(ctypes.c_int64), ctypes.c_int32]
     lib.mgpu_device_count.argtypes = [_c_i32p]
     lib.mgpu_malloc.argtypes = [ctypes.POINTER(vp), ctypes.c_size_t]
     lib.mgpu_free.argtypes = [vp]
@@ -245,6 +247,23 @@ class Problem:
             self.close()
         except Exception:
             pass
+
+
+def desc_from_arrays(desc, arrays):
+    """A copy of set descriptor `desc` whose arrays are taken from `arrays` (numpy, the keys of
+    Problem.arrays()); returns (SetDesc, keepalive) -- keep the second value alive with the first."""
+    d = SetDesc()
+    ctypes.pointer(d)[0] = desc
+    keep = {}
+    ctypes_of = {"t_ptr": ctypes.c_int32, "t_adof": ctypes.c_int32, "ig_ptr": ctypes.c_int32,
+                 "elem_cell": ctypes.c_int32, "t_weight": ctypes.c_double, "ig_xi": ctypes.c_double,
+                 "material": ctypes.c_double, "ip_box": ctypes.c_double, "vertex_phi": ctypes.c_double}
+    for k, a in arrays.items():
+        ct = ctypes_of[k]
+        a = np.ascontiguousarray(a, dtype=np.int32 if ct is ctypes.c_int32 else np.float64)
+        keep[k] = a
+        setattr(d, k, _ptr(a, ct) if a.size else None)
+    return d, keep
 
 
 class DeviceBuffer:
@@ -395,6 +414,12 @@ class Set:
     def stats(self):
         s = (ctypes.c_int64 * 5)()
         _check(load_library().mgpu_set_stats(self._h, s), "mgpu_set_stats")
+        return list(s)
+
+    def statistics(self):
+        """mgpu_set_statistics: stats() plus [5] points and [6] sub-cells skipped for detJ < -1e-6."""
+        s = (ctypes.c_int64 * 7)()
+        _check(load_library().mgpu_set_statistics(self._h, s, 7), "mgpu_set_statistics")
         return list(s)
 
     def close(self):

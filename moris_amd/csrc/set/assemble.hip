@@ -820,7 +820,13 @@ mgpu_status check_err_flag(mgpu_set* S) {
     int32_t flag = 0;
     HIP_TRY(hipMemcpyAsync(&flag, S->v.err_flag, sizeof(int32_t), hipMemcpyDeviceToHost, S->stream));
     HIP_TRY(hipStreamSynchronize(S->stream));
-    if (flag != 0) return mgpu::set_error(MGPU_ERR_STATE, "kernel row length disagrees with the CSR pattern");
+    if (flag != 0) {
+        // reported once: the flag is cleared so that the next clean call succeeds
+        static const int32_t zero = 0;
+        HIP_TRY(hipMemcpy(S->v.err_flag, &zero, sizeof(int32_t), hipMemcpyHostToDevice));
+        return mgpu::set_error(MGPU_ERR_STATE, "kernel row length disagrees with the CSR pattern (rows of that call "
+                                               "were left unwritten)");
+    }
     return MGPU_OK;
 }
 
@@ -836,12 +842,13 @@ extern "C" mgpu_status mgpu_set_create(const mgpu_set_desc* desc, mgpu_set** out
     MGPU_GUARD_BEGIN
     if (!desc || !out) return set_error(MGPU_ERR_INVALID_ARG, "null argument");
     *out = nullptr;
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return set_error(MGPU_ERR_NO_DEVICE, "no HIP device visible");
+    // the descriptor is validated before any device is touched
     auto S = std::make_unique<mgpu_set>();
     std::string err;
     mgpu_status rc = build_set_host(*desc, S->h, &err);
     if (rc != MGPU_OK) return set_error(rc, err);
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return set_error(MGPU_ERR_NO_DEVICE, "no HIP device visible");
     SetHost& h = S->h;
     DevView& v = S->v;
     if (h.generic) {
@@ -924,12 +931,7 @@ extern "C" mgpu_status mgpu_set_create(const mgpu_set_desc* desc, mgpu_set** out
 }
 
 extern "C" mgpu_status mgpu_set_destroy(mgpu_set* set) {
-    if (!set) return MGPU_OK;
-    for (void* p : set->allocs) (void)hipFree(p);
-    if (set->d_u_tmp) (void)hipFree(set->d_u_tmp);
-    if (set->d_vals_tmp) (void)hipFree(set->d_vals_tmp);
-    if (set->d_rhs_tmp) (void)hipFree(set->d_rhs_tmp);
-    delete set;
+    delete set;   // ~mgpu_set frees the device buffers
     return MGPU_OK;
 }
 
@@ -1041,6 +1043,20 @@ extern "C" mgpu_status mgpu_set_stats(const mgpu_set* set, int64_t stats[5]) {
     return MGPU_OK;
 }
 
+extern "C" mgpu_status mgpu_set_statistics(const mgpu_set* set, int64_t* stats, int32_t n_stats) {
+    if (!set || (!stats && n_stats > 0) || n_stats < 0) return set_error(MGPU_ERR_INVALID_ARG, "bad argument");
+    int64_t all[7];
+    mgpu_status rc = mgpu_set_stats(set, all);
+    if (rc != MGPU_OK) return rc;
+    // points of sub-cells with detJ < -1e-6 are skipped in every mode; the learned path skips
+    // them only on its fallback elements (its accepted elements use 27 points of the IP cell)
+    all[5] = set->h.n_skip_points;
+    all[6] = set->h.n_skip_subcells;
+    for (int32_t i = 0; i < n_stats && i < 7; ++i) stats[i] = all[i];
+    for (int32_t i = 7; i < n_stats; ++i) stats[i] = 0;
+    return MGPU_OK;
+}
+
 extern "C" mgpu_status mgpu_set_quadrature_model(mgpu_set* set, const mgpu_mlp* mlp, double tol, int32_t iters,
                                                  double kappa) {
     MGPU_GUARD_BEGIN
@@ -1073,12 +1089,18 @@ extern "C" mgpu_status mgpu_set_time_assemble(mgpu_set* set, const double* u, do
     MGPU_GUARD_BEGIN
     if (!set || iters < 1) return set_error(MGPU_ERR_INVALID_ARG, "bad argument");
     if (set->quadrature == MGPU_QUAD_LEARNED && !set->mlp) return set_error(MGPU_ERR_STATE, "no quadrature model");
-    hipEvent_t ev[6];
-    for (auto& e : ev) HIP_TRY(hipEventCreate(&e));
+    // events released on every return path
+    struct Events {
+        hipEvent_t e[8] = {};
+        ~Events() {
+            for (hipEvent_t x : e)
+                if (x) (void)hipEventDestroy(x);
+        }
+    } evs;
+    for (auto& e : evs.e) HIP_TRY(hipEventCreate(&e));
+    hipEvent_t* ev = evs.e;
+    hipEvent_t t0 = evs.e[6], t1 = evs.e[7];
     double t[5] = {0, 0, 0, 0, 0};
-    hipEvent_t t0, t1;
-    HIP_TRY(hipEventCreate(&t0));
-    HIP_TRY(hipEventCreate(&t1));
     // one untimed call first: the first launch of each kernel in a process loads its code object
     {
         mgpu_status rc = launch_assemble(set, u, vals, rhs, nullptr);
@@ -1109,9 +1131,6 @@ extern "C" mgpu_status mgpu_set_time_assemble(mgpu_set* set, const double* u, do
     if (per_call_ms) *per_call_ms = total / iters;
     if (kernel_ms)
         for (int k = 0; k < 5; ++k) kernel_ms[k] = t[k] / iters;
-    for (auto& e : ev) (void)hipEventDestroy(e);
-    (void)hipEventDestroy(t0);
-    (void)hipEventDestroy(t1);
     return check_err_flag(set);
     MGPU_GUARD_END
 }

@@ -6,6 +6,7 @@
 //   CSR pattern        Sparse_Matrix_EpetraFECrs::build_graph, cl_Sparse_Matrix_EpetraFECrs.cpp:195-240
 //                      (union over elements of topology x topology), rows/cols ascending
 #include <algorithm>
+#include <cmath>
 #include <atomic>
 #include <cstdlib>
 #include <cstring>
@@ -262,6 +263,50 @@ mgpu_status build_set_host(const mgpu_set_desc& d, SetHost& h, std::string* err)
     for (int32_t e = 0; e < h.n_el; ++e)
         if (d.ig_ptr[e + 1] < d.ig_ptr[e]) { *err = "ig_ptr not monotone"; return MGPU_ERR_INVALID_ARG; }
     if (d.ig_ptr[h.n_el] > 0 && !d.ig_xi) { *err = "ig_xi missing"; return MGPU_ERR_INVALID_ARG; }
+    // material: k > 0 (CM_Diffusion_Linear_Isotropic), E > 0 and -1 < nu < 1/2
+    // (CM_Struc_Linear_Isotropic); the kernels encode void and cut cells beside these values
+    for (int32_t e = 0; e < h.n_el; ++e) {
+        if (h.nm == 1) {
+            const double k = d.material[e];
+            if (!(std::isfinite(k) && k > 0.0)) { *err = "conductivity must be finite and positive"; return MGPU_ERR_INVALID_ARG; }
+        } else {
+            const double E = d.material[2 * (size_t)e], nu = d.material[2 * (size_t)e + 1];
+            if (!(std::isfinite(E) && E > 0.0 && nu > -1.0 && nu < 0.5)) {
+                *err = "elasticity needs finite E > 0 and -1 < nu < 0.5";
+                return MGPU_ERR_INVALID_ARG;
+            }
+        }
+    }
+    // sub-cells the kernels skip (detJ < -1e-6, as in cl_FEM_Element_Bulk.cpp:291): counted once
+    // here with the kernels' formula (rows X_r - X_2 of the vertices scaled by half the cell extent)
+    {
+        const int D = h.dim, nv = D + 1;
+        const int npts = hex27 ? kTet35Points : (D == 3 ? kTet11Points : kTri7Points);
+        for (int32_t e = 0; e < h.n_el; ++e) {
+            const double* ext = d.ip_box + (size_t)e * 2 * D + D;
+            for (int32_t s = d.ig_ptr[e]; s < d.ig_ptr[e + 1]; ++s) {
+                const double* X = d.ig_xi + (size_t)s * nv * D;
+                double detJ;
+                if (D == 3) {
+                    double J[3][3];
+                    const int rr[3] = {0, 1, 3};
+                    for (int r = 0; r < 3; ++r)
+                        for (int c = 0; c < 3; ++c) J[r][c] = (X[rr[r] * 3 + c] - X[2 * 3 + c]) * (0.5 * ext[c]);
+                    detJ = (J[0][0] * (J[1][1] * J[2][2] - J[2][1] * J[1][2]) - J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                            J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0])) / 6.0;
+                } else {
+                    double J[2][2];
+                    for (int r = 0; r < 2; ++r)
+                        for (int c = 0; c < 2; ++c) J[r][c] = (X[r * 2 + c] - X[2 * 2 + c]) * (0.5 * ext[c]);
+                    detJ = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) / 2.0;
+                }
+                if (detJ < kDetJLowerLimit) {
+                    ++h.n_skip_subcells;
+                    h.n_skip_points += npts;
+                }
+            }
+        }
+    }
 
     // lattice stencil path: nodal T rows (one adof of weight 1 per basis) on a lattice; anything
     // else (B-spline p > 1, THB rows, sets without a lattice) takes the generic element path

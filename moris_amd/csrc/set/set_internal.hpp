@@ -32,6 +32,10 @@ struct SetHost {
     int32_t n_el = 0;
     int32_t n_adofs_type = 0;
     int32_t n_rows = 0;    // nf * n_adofs_type
+    // IG sub-cells whose detJ is below -1e-6 and the integration points skipped on them
+    // (Element_Bulk::compute_jacobian_and_residual, cl_FEM_Element_Bulk.cpp:291)
+    int64_t n_skip_subcells = 0;
+    int64_t n_skip_points = 0;
     std::vector<int32_t> topo;       // lattice path [n_el][na]; generic path by topo_ptr; sorted
                                      // unique per type, global ids, types concatenated
     std::vector<int32_t> row_ptr;    // [n_rows + 1]

@@ -43,6 +43,17 @@ struct mgpu_set {
     mgpu::LqView lv{};
     const mgpu_mlp* mlp = nullptr;
     bool lq_ready = false;
+
+    mgpu_set() = default;
+    mgpu_set(const mgpu_set&) = delete;
+    mgpu_set& operator=(const mgpu_set&) = delete;
+    // device buffers go with the handle, also when mgpu_set_create fails half-way
+    ~mgpu_set() {
+        for (void* p : allocs) (void)hipFree(p);
+        if (d_u_tmp) (void)hipFree(d_u_tmp);
+        if (d_vals_tmp) (void)hipFree(d_vals_tmp);
+        if (d_rhs_tmp) (void)hipFree(d_rhs_tmp);
+    }
 };
 
 namespace mgpu {

@@ -34,3 +34,33 @@ def test_error_string_after_bad_call():
     rc = lib.mgpu_problem_build(None, ctypes.byref(out))
     assert rc == 1
     assert b"null" in lib.mgpu_last_error_string()
+
+
+def _create_status(desc):
+    lib = M.load_library()
+    out = ctypes.c_void_p()
+    rc = lib.mgpu_set_create(ctypes.byref(desc), ctypes.byref(out))
+    if rc == 0:
+        lib.mgpu_set_destroy(out)
+    return rc, lib.mgpu_last_error_string().decode()
+
+
+def test_create_rejects_bad_material_before_touching_a_device():
+    """Non-positive or non-finite k, E <= 0 and nu outside (-1, 1/2) are refused by mgpu_set_create
+    with MGPU_ERR_INVALID_ARG (the stencil encodes void and cut cells beside these values), before
+    any device is needed."""
+    for iwg, bad in ((M.IWG_SPATIALDIFF_BULK, [0.0, -0.5, float("nan"), float("inf")]),
+                     (M.IWG_STRUC_LINEAR_BULK, [(0.0, 0.3), (-1.0, 0.3), (1.0, 0.5), (1.0, -1.0), (float("nan"), 0.3)])):
+        spec, keep = M.make_spec(3, (4, 4, 4), iwg=iwg, spheres=[[0.5, 0.5, 0.5, 0.3]])
+        P = M.Problem(spec, keep)
+        a = P.arrays()
+        for b in bad:
+            m = a["material"].copy()
+            if iwg == M.IWG_SPATIALDIFF_BULK:
+                m[3] = b
+            else:
+                m[6:8] = b
+            d, k = M.desc_from_arrays(P.desc, dict(a, material=m))
+            rc, msg = _create_status(d)
+            assert rc == 1, (iwg, b, rc, msg)
+            assert ("conductivity" in msg) if iwg == M.IWG_SPATIALDIFF_BULK else ("nu" in msg)
