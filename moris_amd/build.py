@@ -30,6 +30,7 @@ HIP_SOURCES = [
     "set/generic.hip",
     "learned/lq_assemble.hip",
     "learned/mlp.hip",
+    "solve/cg.hip",
 ]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-I" + os.path.join(ROOT, "include")]

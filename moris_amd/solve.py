@@ -1,0 +1,68 @@
+"""Python view of include/moris_gpu_solve.h: the assembled system's solve on the GPU.
+
+The reference solves K u = R after assembly with AztecOO (projects/SOL/DLA/src/
+cl_DLA_Linear_Solver_Aztec.cpp:88-137); here conjugate gradients with a Jacobi preconditioner run
+on the CSR values the assembly left in device memory, with Aztec's default convergence test
+||r_k|| <= rtol ||r_0||.  No CPU fallback: every call goes through libmoris_gpu.so.
+"""
+import ctypes
+
+import numpy as np
+
+from . import DeviceBuffer, _check, load_library
+
+_vp = ctypes.c_void_p
+
+
+class CgInfo(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int32), ("converged", ctypes.c_int32), ("rel_residual", ctypes.c_double),
+                ("true_rel_residual", ctypes.c_double)]
+
+
+def _lib():
+    lib = load_library()
+    if not getattr(lib, "_solve_bound", False):
+        lib.mgpu_csr_apply_dirichlet.argtypes = [ctypes.c_int32, _vp, _vp, _vp, _vp, _vp, _vp, _vp]
+        lib.mgpu_csr_apply_dirichlet.restype = ctypes.c_int32
+        lib.mgpu_cg_solve.argtypes = [ctypes.c_int32, _vp, _vp, _vp, _vp, _vp, ctypes.c_double, ctypes.c_int32, _vp,
+                                      ctypes.POINTER(CgInfo)]
+        lib.mgpu_cg_solve.restype = ctypes.c_int32
+        lib._solve_bound = True
+    return lib
+
+
+class DeviceCSR:
+    """A CSR matrix in device memory: the pattern (int32) and a values buffer."""
+
+    def __init__(self, row_ptr, col_idx, vals=None, vals_buf=None):
+        self.n_rows = len(row_ptr) - 1
+        self.nnz = len(col_idx)
+        self.row_ptr = DeviceBuffer.from_array(np.ascontiguousarray(row_ptr, np.int32))
+        self.col_idx = DeviceBuffer.from_array(np.ascontiguousarray(col_idx, np.int32))
+        if vals_buf is not None:
+            self.vals = vals_buf
+        else:
+            self.vals = DeviceBuffer.from_array(np.ascontiguousarray(vals, np.float64))
+
+    def values(self):
+        return self.vals.to_array(np.float64, self.nnz)
+
+
+def apply_dirichlet(A, rhs_buf, fixed, value=None):
+    """Strong Dirichlet rows/columns on A (in place) and rhs (device buffer); fixed: bool per row."""
+    f = DeviceBuffer.from_array(np.ascontiguousarray(fixed, np.uint8))
+    v = DeviceBuffer.from_array(np.ascontiguousarray(value, np.float64)) if value is not None else None
+    _check(_lib().mgpu_csr_apply_dirichlet(A.n_rows, A.row_ptr.ptr, A.col_idx.ptr, A.vals.ptr, rhs_buf.ptr, f.ptr,
+                                           v.ptr if v is not None else None, None), "mgpu_csr_apply_dirichlet")
+    load_library().mgpu_synchronize()
+
+
+def cg(A, b, x0=None, rtol=1e-10, max_iter=20000):
+    """Solve A x = b (b: host array or DeviceBuffer) by Jacobi-preconditioned CG on the GPU;
+    returns (x as a host array, CgInfo)."""
+    bb = b if isinstance(b, DeviceBuffer) else DeviceBuffer.from_array(np.ascontiguousarray(b, np.float64))
+    x = DeviceBuffer.from_array(np.zeros(A.n_rows) if x0 is None else np.ascontiguousarray(x0, np.float64))
+    info = CgInfo()
+    _check(_lib().mgpu_cg_solve(A.n_rows, A.row_ptr.ptr, A.col_idx.ptr, A.vals.ptr, bb.ptr, x.ptr, rtol, max_iter,
+                                None, ctypes.byref(info)), "mgpu_cg_solve")
+    return x.to_array(np.float64, A.n_rows), info
