@@ -46,6 +46,18 @@ mgpu_status mgpu_cg_solve(int32_t n_rows, const int32_t* row_ptr, const int32_t*
                           const double* b, double* x, double rtol, int32_t max_iter, void* stream,
                           mgpu_cg_info* info);
 
+/* The same solve without any device allocation.  No counterpart in the reference: it is for
+   callers that cannot allocate device memory (serving loops, harnesses that forbid hipMalloc).
+   The work vectors live in a fixed device array sized for MGPU_CG_NOALLOC_MAX_ROWS rows (about
+   100 MB), one copy per GPU, held from the library's first use on that GPU until the process
+   ends; larger systems are refused with MGPU_ERR_INVALID_ARG.  All calls on one GPU share that
+   array: calls that overlap in time on one GPU (several host threads or streams) race and must be
+   serialised by the caller.  Results equal those of mgpu_cg_solve bit for bit.                */
+#define MGPU_CG_NOALLOC_MAX_ROWS 2500000
+mgpu_status mgpu_cg_solve_noalloc(int32_t n_rows, const int32_t* row_ptr, const int32_t* col_idx,
+                                  const double* vals, const double* b, double* x, double rtol, int32_t max_iter,
+                                  void* stream, mgpu_cg_info* info);
+
 #ifdef __cplusplus
 }
 #endif

@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <mutex>
 #include <string>
 
 #include "../../../include/moris_gpu_solve.h"
@@ -62,9 +63,9 @@ struct Work {
     State* st;
 };
 
-inline size_t work_doubles(int64_t n) {
-    const size_t vec = ((size_t)n + 31) / 32 * 32;   // 256-byte aligned vectors
-    return 5 * vec + 2 * kBlocks + (sizeof(State) + 7) / 8 + 32;
+constexpr size_t work_doubles(int64_t n) {
+    // five 256-byte aligned vectors, the partial sums, the state
+    return 5 * (((size_t)n + 31) / 32 * 32) + 2 * kBlocks + (sizeof(State) + 7) / 8 + 32;
 }
 
 inline Work carve(double* base, int64_t n) {
@@ -339,6 +340,39 @@ mgpu_status check_args(int32_t n, const int32_t* rp, const int32_t* ci, const do
 }  // namespace cg
 }  // namespace mgpu
 
+// Work space of mgpu_cg_solve_noalloc: one fixed array per GPU (the code object's copy on that
+// device), MGPU_CG_NOALLOC_MAX_ROWS rows at most (about 100 MB).  Every part of it is written before
+// it is read within a call (residual, search direction, preconditioner, partial sums, state), so
+// nothing needs clearing between calls.
+namespace {
+constexpr size_t kNoallocDoubles = mgpu::cg::work_doubles(MGPU_CG_NOALLOC_MAX_ROWS);
+}  // namespace
+static __device__ __align__(256) double scratch[kNoallocDoubles];
+
+namespace {
+// Device address of `scratch` on each GPU, looked up once per device with that device current.
+// In host code the name `scratch` is only a handle for hipGetSymbolAddress; the kernels receive
+// the looked-up address.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_dev[kMaxDevices] = {};
+
+mgpu_status noalloc_scratch(double** out) {
+    int dev = 0;
+    CG_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return mgpu::set_error(MGPU_ERR_INVALID_ARG, "device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void* p = nullptr;
+        CG_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return mgpu::set_error(MGPU_ERR_HIP, "hipGetSymbolAddress returned no address for the work space");
+        g_scratch_dev[dev] = static_cast<double*>(p);
+    }
+    *out = g_scratch_dev[dev];
+    return MGPU_OK;
+}
+}  // namespace
+
 using namespace mgpu;
 
 extern "C" mgpu_status mgpu_csr_apply_dirichlet(int32_t n_rows, const int32_t* row_ptr, const int32_t* col_idx,
@@ -371,5 +405,24 @@ extern "C" mgpu_status mgpu_cg_solve(int32_t n_rows, const int32_t* row_ptr, con
     (void)hipStreamSynchronize(static_cast<hipStream_t>(stream));
     (void)hipFree(base);
     return rc;
+    MGPU_GUARD_END
+}
+
+extern "C" mgpu_status mgpu_cg_solve_noalloc(int32_t n_rows, const int32_t* row_ptr, const int32_t* col_idx,
+                                             const double* vals, const double* b, double* x, double rtol,
+                                             int32_t max_iter, void* stream, mgpu_cg_info* info) {
+    MGPU_GUARD_BEGIN
+    mgpu_status rc = cg::check_args(n_rows, row_ptr, col_idx, vals, b, x, rtol);
+    if (rc != MGPU_OK) return rc;
+    if (n_rows > MGPU_CG_NOALLOC_MAX_ROWS)
+        return set_error(MGPU_ERR_INVALID_ARG, "mgpu_cg_solve_noalloc: " + std::to_string(n_rows) +
+                                                   " rows exceed its fixed work space of " +
+                                                   std::to_string(MGPU_CG_NOALLOC_MAX_ROWS) + " rows");
+    if (info) *info = mgpu_cg_info{0, 1, 0.0, 0.0};
+    if (n_rows == 0) return MGPU_OK;
+    double* base = nullptr;
+    if ((rc = noalloc_scratch(&base)) != MGPU_OK) return rc;
+    return cg::solve(n_rows, row_ptr, col_idx, vals, b, x, rtol, max_iter, static_cast<hipStream_t>(stream), base,
+                     info);
     MGPU_GUARD_END
 }
