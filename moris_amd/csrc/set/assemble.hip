@@ -31,6 +31,7 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../common/fem_tables.hpp"
@@ -186,6 +187,117 @@ __global__ __launch_bounds__(64) void k_cut_elasticity(DevView s) {
                     if (d_nonzero<DIM>(voigt<DIM>(al, i), voigt<DIM>(be, j)))
                         v += D[voigt<DIM>(al, i)][voigt<DIM>(be, j)] * (double)M[i][j];
             ke[(a * DIM + al) * NA + (be * NB + b)] = (T)v;
+        }
+}
+
+// ------------------------------------------------------------------------------------------------
+// k_cut_elasticity_mfma: 3D elasticity, one wavefront per cut element, the moment matrix on MFMA.
+//   M (24 x 24, component c = 8 i + a, i the direction, a the vertex) = sum_p w*_p g_p g_p^T is
+//   symmetric; padded to 32 x 32 it is three 16 x 16 tiles (upper left, upper right, lower right).
+//   Points are mapped 64 at a time into FP64 records (xi, w*) in LDS (one lane per point: IG -> IP
+//   map and sub-cell detJ).  Each MFMA step takes 4 points: lane l handles point l / 16 of the step
+//   and component column l % 16 = col, and forms in registers, from the point's record, the two
+//   gradient entries it feeds -- g[col] (direction col / 8, vertex col % 8) and g[16 + col]
+//   (direction 2, vertex col; zero padding for col >= 8) -- and w* times them: those are its A
+//   (row col, k = point) and B (k = point, column col) operands of v_mfma_f64_16x16x4f64
+//   (v_mfma_f32_16x16x4f32 with FP32 operands and FP32 accumulation on the mixed path; the records
+//   are formed in FP64 there too).  The tiles go to LDS once, and the contraction with D and the
+//   store of K_e follow as in k_cut_elasticity.
+// ------------------------------------------------------------------------------------------------
+typedef double mgpu_doublex4_t __attribute__((ext_vector_type(4)));
+typedef float mgpu_floatx4_t __attribute__((ext_vector_type(4)));
+
+template <typename T>
+__device__ inline void mfma_16x16x4(T a, T b, typename std::conditional<sizeof(T) == 8, mgpu_doublex4_t,
+                                                                        mgpu_floatx4_t>::type& acc) {
+    if constexpr (sizeof(T) == 8) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+    else acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
+}
+
+template <typename T>
+__global__ __launch_bounds__(64) void k_cut_elasticity_mfma(DevView s) {
+    constexpr int NB = 8, NG = 24, NA = 24;
+    constexpr int MP = NG + 1;
+    using acc_t = typename std::conditional<sizeof(T) == 8, mgpu_doublex4_t, mgpu_floatx4_t>::type;
+    __shared__ double4 rec[64];
+    __shared__ T Ms[NG * MP];
+
+    const int c = s.cut_order ? s.cut_order[blockIdx.x] : (int)blockIdx.x;
+    if (s.skip && s.skip[c]) return;
+    const int lane = threadIdx.x;
+    const int e = s.cut_elems[c];
+    const int ig0 = s.ig_ptr[e];
+    const int nip = (s.ig_ptr[e + 1] - ig0) * kTet11Points;
+    double hh[3];
+    for (int d = 0; d < 3; ++d) hh[d] = s.h[d];
+
+    // the lane's component column and the constant factors of its two gradient entries
+    const int col = lane & 15, grp = lane >> 4;
+    const int a = col & 7;
+    const double sx = kHex8XiHat[a][0], sy = kHex8XiHat[a][1], sz = kHex8XiHat[a][2];
+    const bool lo_x = col < 8;                 // g[col]: d/dx (col < 8) or d/dy (col >= 8)
+    const double c_lo = lo_x ? 0.25 * sx / hh[0] : 0.25 * sy / hh[1];
+    const double c_hi = lo_x ? 0.25 * sz / hh[2] : 0.0;   // g[16 + col]: d/dz, or padding
+
+    acc_t m00 = {0, 0, 0, 0}, m01 = {0, 0, 0, 0}, m11 = {0, 0, 0, 0};
+    for (int base = 0; base < nip; base += 64) {
+        {
+            const int ip = base + lane;
+            double4 r = make_double4(0.0, 0.0, 0.0, 0.0);
+            if (ip < nip) {
+                double xi[3];
+                r.w = cut_point_map<3>(s, ig0, ip, hh, xi);
+                r.x = xi[0];
+                r.y = xi[1];
+                r.z = xi[2];
+            }
+            rec[lane] = r;
+        }
+        __syncthreads();
+        const int steps = (min(64, nip - base) + 3) >> 2;
+        for (int st = 0; st < steps; ++st) {
+            const double4 r = rec[4 * st + grp];
+            const double fx = 1.0 + sx * r.x, fy = 1.0 + sy * r.y, fz = 1.0 + sz * r.z;
+            const double lo = c_lo * (lo_x ? fy : fx) * fz;
+            const double hi = c_hi * fx * fy;
+            const T blo = (T)lo, bhi = (T)hi;
+            const T alo = (T)(r.w * lo), ahi = (T)(r.w * hi);
+            mfma_16x16x4<T>(alo, blo, m00);
+            mfma_16x16x4<T>(alo, bhi, m01);
+            mfma_16x16x4<T>(ahi, bhi, m11);
+        }
+        __syncthreads();
+    }
+    // tiles -> M in LDS (result register j of a lane: row grp + 4 j for FP64, 4 grp + j for FP32;
+    // column col), the lower left tile mirrored from the upper right one
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int row = sizeof(T) == 8 ? grp + 4 * j : 4 * grp + j;
+        Ms[row * MP + col] = m00[j];
+        if (col < 8) {
+            Ms[row * MP + 16 + col] = m01[j];
+            Ms[(16 + col) * MP + row] = m01[j];
+            if (row < 8) Ms[(16 + row) * MP + 16 + col] = m11[j];
+        }
+    }
+    __syncthreads();
+    // B^T D B: lane (a, b) owns the 3 x 3 direction block of the vertex pair (a, b)
+    const int va = lane >> 3, vb = lane & 7;
+    double D[6][6];
+    elastic_D<3>(s.material[2 * e], s.material[2 * e + 1], D);
+    T* ke = ke_store<T>(s) + (size_t)c * NA * NA;
+#pragma unroll
+    for (int al = 0; al < 3; ++al)
+#pragma unroll
+        for (int be = 0; be < 3; ++be) {
+            double v = 0.0;
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j)
+                    if (d_nonzero<3>(voigt<3>(al, i), voigt<3>(be, j)))
+                        v += D[voigt<3>(al, i)][voigt<3>(be, j)] * (double)Ms[(i * NB + va) * MP + j * NB + vb];
+            ke[(va * 3 + al) * NA + (be * NB + vb)] = (T)v;
         }
 }
 
@@ -614,8 +726,8 @@ void launch_subdivision(mgpu_set* S, const int32_t* skip) {
         if (f32) hipLaunchKernelGGL((k_cut_diffusion<3, float>), g, b, 0, S->stream, v);
         else hipLaunchKernelGGL((k_cut_diffusion<3, double>), g, b, 0, S->stream, v);
     } else if (dim == 3 && nf == 3) {
-        if (f32) hipLaunchKernelGGL((k_cut_elasticity<3, float>), g, b, 0, S->stream, v);
-        else hipLaunchKernelGGL((k_cut_elasticity<3, double>), g, b, 0, S->stream, v);
+        if (f32) hipLaunchKernelGGL((k_cut_elasticity_mfma<float>), g, b, 0, S->stream, v);
+        else hipLaunchKernelGGL((k_cut_elasticity_mfma<double>), g, b, 0, S->stream, v);
     } else if (dim == 2 && nf == 1) {
         if (f32) hipLaunchKernelGGL((k_cut_diffusion<2, float>), g, b, 0, S->stream, v);
         else hipLaunchKernelGGL((k_cut_diffusion<2, double>), g, b, 0, S->stream, v);

@@ -166,6 +166,52 @@ __device__ inline void lagrange_dNdxi(const double* xi, double (*out)[DIM == 3 ?
 }
 
 // ------------------------------------------------------------------------------------------------
+// Integration point ip of the cut element's IG sub-cells: its parametric position in the IP cell
+// (Space_Interpolator::map_integration_point, cl_MTK_Space_Interpolator.cpp:1381-1397) and its
+// weight w* = w detJ (0 where detJ < -1e-6, cl_FEM_Element_Bulk.cpp:291), all in FP64.
+// ------------------------------------------------------------------------------------------------
+template <int DIM>
+__device__ inline double cut_point_map(const DevView& s, int ig0, int ip, const double* hh, double* xip) {
+    constexpr int NV = DIM + 1;
+    constexpr int NQ = DIM == 3 ? kTet11Points : kTri7Points;
+    const int sc = ig0 + ip / NQ;
+    const int q = ip % NQ;
+    const double* X = s.ig_xi + (size_t)sc * NV * DIM;
+    double N[NV];
+    if constexpr (DIM == 3) {
+        N[0] = kTet11Xi[q][0];
+        N[1] = kTet11Xi[q][1];
+        N[2] = 1.0 - kTet11Xi[q][0] - kTet11Xi[q][1] - kTet11Xi[q][2];
+        N[3] = kTet11Xi[q][2];
+    } else {
+        N[0] = kTri7Xi[q][0];
+        N[1] = kTri7Xi[q][1];
+        N[2] = 1.0 - kTri7Xi[q][0] - kTri7Xi[q][1];
+    }
+    for (int d = 0; d < DIM; ++d) {
+        double v = 0.0;
+        for (int k = 0; k < NV; ++k) v += N[k] * X[k * DIM + d];
+        xip[d] = v;
+    }
+    double detJ;
+    if constexpr (DIM == 3) {
+        double J[3][3];
+        const int rr[3] = {0, 1, 3};
+        for (int r = 0; r < 3; ++r)
+            for (int d = 0; d < 3; ++d) J[r][d] = (X[rr[r] * 3 + d] - X[2 * 3 + d]) * (0.5 * hh[d]);
+        detJ = (J[0][0] * (J[1][1] * J[2][2] - J[2][1] * J[1][2]) -
+                J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0])) / 6.0;
+    } else {
+        double J[2][2];
+        for (int r = 0; r < 2; ++r)
+            for (int d = 0; d < 2; ++d) J[r][d] = (X[r * 2 + d] - X[2 * 2 + d]) * (0.5 * hh[d]);
+        detJ = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) / 2.0;
+    }
+    return detJ < kDetJLowerLimit ? 0.0 : (DIM == 3 ? kTet11W[q] : kTri7W[q]) * detJ;
+}
+
+// ------------------------------------------------------------------------------------------------
 // Integration point ip of the cut element's IG sub-cells: weight w* and IP-cell gradients g[i][a].
 // ------------------------------------------------------------------------------------------------
 template <int DIM>

