@@ -28,6 +28,7 @@ HIP_SOURCES = [
     "set/assemble.hip",
     "set/batch.hip",
     "set/generic.hip",
+    "set/sideset.hip",
     "learned/lq_assemble.hip",
     "learned/mlp.hip",
     "solve/cg.hip",

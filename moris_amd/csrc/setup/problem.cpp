@@ -789,3 +789,176 @@ extern "C" mgpu_status mgpu_problem_destroy(mgpu_problem* p) {
     delete p;
     return MGPU_OK;
 }
+
+// ------------------------------------------------------------------------------------------------
+// Side facets of a problem (the reference's side sets of the XTK integration mesh: the domain
+// boundary faces of the material cells, and the interface facets between the material sub-cells
+// and the void).  A face of a material sub-cell is on the interface when it lies on no face of
+// its cell and no other material sub-cell of that cell has the same vertices: the material
+// sub-cells tile the cell's material region, so every other face is shared or on the cell's
+// boundary.  Interface normals point out of the material sub-cell that owns the face.
+// ------------------------------------------------------------------------------------------------
+#include "../../../include/moris_gpu_sideset.h"
+
+#include <array>
+#include <cmath>
+#include <map>
+
+struct mgpu_facets {
+    int32_t dim = 3, nv = 3;
+    std::vector<int32_t> elem;
+    std::vector<double> xi;      // [n][nv][dim]
+    std::vector<double> normal;  // [n][dim]
+};
+
+namespace {
+constexpr double kFaceTol = 1e-12;   // a vertex lies on a cell face when |xi_d -+ 1| <= this
+
+bool on_cell_face(const double* const* v, int nvf, int dim, int d, double s) {
+    for (int k = 0; k < nvf; ++k)
+        if (std::fabs(v[k][d] - s) > kFaceTol) return false;
+    (void)dim;
+    return true;
+}
+}  // namespace
+
+extern "C" mgpu_status mgpu_problem_facets(const mgpu_problem* p, int32_t where, int32_t geom, mgpu_facets** out) {
+    MGPU_GUARD_BEGIN
+    if (!p || !out) return mgpu::set_error(MGPU_ERR_INVALID_ARG, "null argument");
+    *out = nullptr;
+    const int dim = p->dim;
+    if (where < MGPU_SIDE_INTERFACE || where > 2 * dim) return mgpu::set_error(MGPU_ERR_INVALID_ARG, "bad facet location");
+    const bool ok_geom = dim == 3 ? (geom == MGPU_GEOM_TRI3 || geom == MGPU_GEOM_QUAD4) : geom == MGPU_GEOM_LINE2;
+    if (!ok_geom) return mgpu::set_error(MGPU_ERR_INVALID_ARG, "side geometry must be TRI3 / QUAD4 in 3D, LINE2 in 2D");
+    if (p->nb_ip != p->nb) return mgpu::set_error(MGPU_ERR_UNSUPPORTED, "side facets are implemented for HEX8 / QUAD4 IP cells");
+    if (!p->refined.empty() && where != MGPU_SIDE_INTERFACE)
+        return mgpu::set_error(MGPU_ERR_UNSUPPORTED, "boundary facets are implemented for level-0 lattices");
+    auto F = std::make_unique<mgpu_facets>();
+    F->dim = dim;
+    const int nv_ig = dim + 1;                                 // sub-cell vertices
+    F->nv = geom == MGPU_GEOM_QUAD4 ? 4 : dim;                 // facet vertices
+    const int bd = where > 0 ? (where - 1) / 2 : -1;           // boundary axis
+    const double bs = where > 0 ? ((where - 1) % 2 ? 1.0 : -1.0) : 0.0;
+    const bool want_uncut = geom == MGPU_GEOM_QUAD4 || (dim == 2 && where != MGPU_SIDE_INTERFACE);
+    const bool want_cut = geom != MGPU_GEOM_QUAD4;
+    const size_t n_el = p->elem_cell.size();
+    for (size_t e = 0; e < n_el; ++e) {
+        const double* box = p->ip_box.data() + e * 2 * dim;
+        auto phys = [&](const double* xi, double* x) {
+            for (int d = 0; d < dim; ++d) x[d] = box[d] + 0.5 * (xi[d] + 1.0) * box[dim + d];
+        };
+        bool at_boundary = false;
+        if (where > 0) {
+            const int64_t c = p->elem_cell[e];
+            const int64_t ijk[3] = {c % p->n[0], (c / p->n[0]) % p->n[1], dim == 3 ? c / ((int64_t)p->n[0] * p->n[1]) : 0};
+            at_boundary = bs < 0 ? ijk[bd] == 0 : ijk[bd] == p->n[bd] - 1;
+        }
+        const int32_t s0 = p->ig_ptr[e], s1 = p->ig_ptr[e + 1];
+        if (s0 == s1) {
+            if (!(want_uncut && at_boundary)) continue;
+            // the whole face (edge) of the box: the other axes in ascending order, cyclic vertices
+            int ax[2] = {0, 0}, na = 0;
+            for (int d = 0; d < dim; ++d)
+                if (d != bd) ax[na++] = d;
+            const double cyc[4][2] = {{-1, -1}, {1, -1}, {1, 1}, {-1, 1}};
+            F->elem.push_back((int32_t)e);
+            for (int k = 0; k < F->nv; ++k) {
+                double v[3] = {0, 0, 0};
+                v[bd] = bs;
+                if (dim == 3) {
+                    v[ax[0]] = cyc[k][0];
+                    v[ax[1]] = cyc[k][1];
+                } else {
+                    v[ax[0]] = k == 0 ? -1.0 : 1.0;
+                }
+                for (int d = 0; d < dim; ++d) F->xi.push_back(v[d]);
+            }
+            for (int d = 0; d < dim; ++d) F->normal.push_back(d == bd ? bs : 0.0);
+            continue;
+        }
+        if (!want_cut) continue;
+        if (where > 0 && !at_boundary) continue;
+        // faces of the material sub-cells: count of each vertex set within the element
+        std::map<std::array<double, 9>, int> count;
+        auto face_key = [&](const double* X, int omit) {
+            std::array<std::array<double, 3>, 3> vs{};
+            int m = 0;
+            for (int k = 0; k < nv_ig; ++k)
+                if (k != omit) {
+                    for (int d = 0; d < dim; ++d) vs[m][d] = X[k * dim + d];
+                    ++m;
+                }
+            std::sort(vs.begin(), vs.begin() + m);
+            std::array<double, 9> key{};
+            for (int a = 0; a < m; ++a)
+                for (int d = 0; d < 3; ++d) key[a * 3 + d] = vs[a][d];
+            return key;
+        };
+        if (where == MGPU_SIDE_INTERFACE)
+            for (int32_t s = s0; s < s1; ++s)
+                for (int k = 0; k < nv_ig; ++k) ++count[face_key(p->ig_xi.data() + (size_t)s * nv_ig * dim, k)];
+        for (int32_t s = s0; s < s1; ++s) {
+            const double* X = p->ig_xi.data() + (size_t)s * nv_ig * dim;
+            for (int k = 0; k < nv_ig; ++k) {
+                const double* v[3];
+                int m = 0;
+                for (int q = 0; q < nv_ig; ++q)
+                    if (q != k) v[m++] = X + q * dim;
+                double nrm[3] = {0, 0, 0};
+                if (where > 0) {
+                    if (!on_cell_face(v, m, dim, bd, bs)) continue;
+                    nrm[bd] = bs;
+                } else {
+                    bool on_face = false;
+                    for (int d = 0; d < dim && !on_face; ++d)
+                        on_face = on_cell_face(v, m, dim, d, -1.0) || on_cell_face(v, m, dim, d, 1.0);
+                    if (on_face || count[face_key(X, k)] != 1) continue;
+                    double x[3][3] = {}, xo[3] = {0, 0, 0};
+                    for (int a = 0; a < m; ++a) phys(v[a], x[a]);
+                    phys(X + k * dim, xo);
+                    if (dim == 3) {
+                        const double a1[3] = {x[1][0] - x[0][0], x[1][1] - x[0][1], x[1][2] - x[0][2]};
+                        const double a2[3] = {x[2][0] - x[0][0], x[2][1] - x[0][1], x[2][2] - x[0][2]};
+                        nrm[0] = a1[1] * a2[2] - a1[2] * a2[1];
+                        nrm[1] = a1[2] * a2[0] - a1[0] * a2[2];
+                        nrm[2] = a1[0] * a2[1] - a1[1] * a2[0];
+                    } else {
+                        nrm[0] = x[1][1] - x[0][1];
+                        nrm[1] = -(x[1][0] - x[0][0]);
+                    }
+                    double dot = 0.0, len = 0.0;
+                    for (int d = 0; d < dim; ++d) {
+                        dot += nrm[d] * (xo[d] - x[0][d]);
+                        len += nrm[d] * nrm[d];
+                    }
+                    len = std::sqrt(len);
+                    if (!(len > 0.0)) continue;   // degenerate face
+                    const double sgn = dot > 0.0 ? -1.0 : 1.0;
+                    for (int d = 0; d < dim; ++d) nrm[d] *= sgn / len;
+                }
+                F->elem.push_back((int32_t)e);
+                for (int a = 0; a < m; ++a)
+                    for (int d = 0; d < dim; ++d) F->xi.push_back(v[a][d]);
+                for (int d = 0; d < dim; ++d) F->normal.push_back(nrm[d]);
+            }
+        }
+    }
+    *out = F.release();
+    return MGPU_OK;
+    MGPU_GUARD_END
+}
+
+extern "C" mgpu_status mgpu_facets_get(const mgpu_facets* f, int32_t* n, const int32_t** elem, const double** xi,
+                                       const double** normal) {
+    if (!f) return mgpu::set_error(MGPU_ERR_INVALID_ARG, "null facets");
+    if (n) *n = (int32_t)f->elem.size();
+    if (elem) *elem = f->elem.data();
+    if (xi) *xi = f->xi.data();
+    if (normal) *normal = f->normal.data();
+    return MGPU_OK;
+}
+
+extern "C" mgpu_status mgpu_facets_destroy(mgpu_facets* f) {
+    delete f;
+    return MGPU_OK;
+}

@@ -1,0 +1,125 @@
+"""Python view of include/moris_gpu_sideset.h: side-set IWGs (Nitsche Dirichlet, Neumann) on the
+facets of a bulk set, assembled on the GPU into the bulk set's CSR values and residual
+(Element_Sideset::compute_jacobian_and_residual, projects/FEM/INT/src/ELEM/cl_FEM_Element_Sideset.cpp
+:245-330).  No CPU fallback: every call goes through libmoris_gpu.so.
+"""
+import ctypes
+
+import numpy as np
+
+from . import DeviceBuffer, Set, _check, load_library
+
+_vp = ctypes.c_void_p
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_f64p = ctypes.POINTER(ctypes.c_double)
+
+IWG_SPATIALDIFF_DIRICHLET_SYMMETRIC_NITSCHE = 2
+IWG_SPATIALDIFF_DIRICHLET_UNSYMMETRIC_NITSCHE = 3
+IWG_SPATIALDIFF_NEUMANN = 4
+IWG_STRUC_LINEAR_DIRICHLET_SYMMETRIC_NITSCHE = 5
+IWG_STRUC_LINEAR_DIRICHLET_UNSYMMETRIC_NITSCHE = 6
+IWG_STRUC_LINEAR_NEUMANN = 7
+GEOM_TRI3, GEOM_QUAD4, GEOM_LINE2 = 2, 0, 5
+SIDE_INTERFACE, SIDE_XMIN, SIDE_XMAX, SIDE_YMIN, SIDE_YMAX, SIDE_ZMIN, SIDE_ZMAX = range(7)
+N_VERTS = {GEOM_TRI3: 3, GEOM_QUAD4: 4, GEOM_LINE2: 2}
+
+
+class SideSetDesc(ctypes.Structure):
+    _fields_ = [("iwg", ctypes.c_int32), ("side_geometry", ctypes.c_int32), ("n_facets", ctypes.c_int32),
+                ("facet_elem", _i32p), ("facet_xi", _f64p), ("facet_normal", _f64p),
+                ("nitsche_alpha", ctypes.c_double), ("value", ctypes.c_double * 3)]
+
+
+def _lib():
+    lib = load_library()
+    if not getattr(lib, "_side_bound", False):
+        lib.mgpu_sideset_create.argtypes = [_vp, ctypes.POINTER(SideSetDesc), ctypes.POINTER(_vp)]
+        lib.mgpu_sideset_destroy.argtypes = [_vp]
+        lib.mgpu_sideset_assemble.argtypes = [_vp, _vp, _vp, _vp]
+        lib.mgpu_problem_facets.argtypes = [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(_vp)]
+        lib.mgpu_facets_get.argtypes = [_vp, _i32p, ctypes.POINTER(_i32p), ctypes.POINTER(_f64p), ctypes.POINTER(_f64p)]
+        lib.mgpu_facets_destroy.argtypes = [_vp]
+        for n in ("mgpu_sideset_create", "mgpu_sideset_destroy", "mgpu_sideset_assemble", "mgpu_problem_facets",
+                  "mgpu_facets_get", "mgpu_facets_destroy"):
+            getattr(lib, n).restype = ctypes.c_int32
+        lib._side_bound = True
+    return lib
+
+
+def facets(problem, where, geometry):
+    """Facets of a Problem: (elem [n], xi [n][nv][dim], normal [n][dim]) as numpy copies."""
+    lib = _lib()
+    h = _vp()
+    _check(lib.mgpu_problem_facets(problem._h, where, geometry, ctypes.byref(h)), "mgpu_problem_facets")
+    try:
+        n = ctypes.c_int32()
+        e, x, nr = _i32p(), _f64p(), _f64p()
+        _check(lib.mgpu_facets_get(h, ctypes.byref(n), ctypes.byref(e), ctypes.byref(x), ctypes.byref(nr)),
+               "mgpu_facets_get")
+        dim = problem.dim
+        nv = N_VERTS[geometry]
+        if n.value == 0:
+            return np.zeros(0, np.int32), np.zeros((0, nv, dim)), np.zeros((0, dim))
+        elem = np.ctypeslib.as_array(e, shape=(n.value,)).copy()
+        xi = np.ctypeslib.as_array(x, shape=(n.value * nv * dim,)).reshape(n.value, nv, dim).copy()
+        normal = np.ctypeslib.as_array(nr, shape=(n.value * dim,)).reshape(n.value, dim).copy()
+        return elem, xi, normal
+    finally:
+        lib.mgpu_facets_destroy(h)
+
+
+def make_desc(iwg, geometry, elem, xi, normal, alpha=0.0, value=(0.0, 0.0, 0.0)):
+    """(SideSetDesc, keepalive) from facet arrays."""
+    d = SideSetDesc()
+    keep = [np.ascontiguousarray(elem, np.int32), np.ascontiguousarray(xi, np.float64),
+            np.ascontiguousarray(normal, np.float64)]
+    d.iwg, d.side_geometry, d.n_facets = iwg, geometry, len(keep[0])
+    d.facet_elem = keep[0].ctypes.data_as(_i32p) if len(keep[0]) else None
+    d.facet_xi = keep[1].ctypes.data_as(_f64p) if len(keep[0]) else None
+    d.facet_normal = keep[2].ctypes.data_as(_f64p) if len(keep[0]) else None
+    d.nitsche_alpha = alpha
+    v = list(value) + [0.0] * (3 - len(value))
+    d.value[:] = v[:3]
+    return d, keep
+
+
+class SideSet:
+    """A side set on a bulk Set: its K and R are added into the bulk set's CSR values and R."""
+
+    def __init__(self, bulk, iwg, geometry, elem, xi, normal, alpha=0.0, value=(0.0, 0.0, 0.0)):
+        if not isinstance(bulk, Set):
+            raise TypeError("bulk must be a moris_amd.Set")
+        self.bulk = bulk
+        self.desc, self._keep = make_desc(iwg, geometry, elem, xi, normal, alpha, value)
+        self.n_facets = self.desc.n_facets
+        self._h = _vp()
+        _check(_lib().mgpu_sideset_create(bulk._h, ctypes.byref(self.desc), ctypes.byref(self._h)),
+               "mgpu_sideset_create")
+
+    def assemble_device(self, u_buf, vals_buf, rhs_buf):
+        _check(_lib().mgpu_sideset_assemble(self._h, u_buf.ptr, vals_buf.ptr, rhs_buf.ptr), "mgpu_sideset_assemble")
+
+    def close(self):
+        if self._h:
+            _lib().mgpu_sideset_destroy(self._h)
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def assemble(bulk, side_sets, u):
+    """Bulk set plus side sets for host u: (K values, R) as host arrays (fill_matrix_and_RHS over
+    the bulk and side sets, cl_DLA_Solver_Interface.cpp:317-365)."""
+    u = np.ascontiguousarray(u, np.float64)
+    ub = DeviceBuffer.from_array(u)
+    vb = DeviceBuffer(8 * bulk.nnz)
+    rb = DeviceBuffer(8 * bulk.n_rows)
+    bulk.assemble_device(ub, vb, rb)
+    for ss in side_sets:
+        ss.assemble_device(ub, vb, rb)
+    load_library().mgpu_synchronize()
+    return vb.to_array(np.float64, bulk.nnz), rb.to_array(np.float64, bulk.n_rows)

@@ -46,6 +46,7 @@
 #include <vector>
 
 #include "../include/moris_gpu.h"
+#include "../include/moris_gpu_sideset.h"
 
 namespace {
 
@@ -1202,6 +1203,363 @@ void element_assemble(const mgpu_set_desc& d, int64_t e, const double* u, const 
     }
 }
 
+// ------------------------------------------------------------------------------------------------
+// side sets (Element_Sideset::compute_jacobian_and_residual, cl_FEM_Element_Sideset.cpp:245-330,
+// with IWG_*_Dirichlet(_Nitsche) / IWG_*_Neumann, cl_FEM_IWG_Isotropic_Struc_Linear_Dirichlet.cpp
+// :52-240, cl_FEM_IWG_Diffusion_Dirichlet_Nitsche.cpp, cl_FEM_IWG_Isotropic_Struc_Linear_Neumann.cpp
+// :38-85; SP_Dirichlet_Nitsche, cl_FEM_SP_Dirichlet_Nitsche.cpp:75-115; side cluster length
+// measure, cl_FEM_Cluster.cpp:858-919; flatten_normal, cl_FEM_CM_Struc_Linear.cpp:836-861)
+// ------------------------------------------------------------------------------------------------
+const double kBar3P[3] = {-0.7745966692414834, 0.000000000000000, 0.7745966692414834};
+const double kBar3Wt[3] = {5.555555555555556e-01, 8.888888888888888e-01, 5.555555555555556e-01};
+
+// side rule points: shape values of the facet vertices (phi[q][k]), weights, and the physical
+// Jacobian measure at each point
+int side_points(int D, int nvs, const double (*Xf)[3], double (*phi)[4], double* w, double* detJ) {
+    auto cross_norm = [](const double* a, const double* b) {
+        const double c0 = a[1] * b[2] - a[2] * b[1], c1 = a[2] * b[0] - a[0] * b[2], c2 = a[0] * b[1] - a[1] * b[0];
+        return std::sqrt(c0 * c0 + c1 * c1 + c2 * c2);
+    };
+    if (nvs == 3) {   // TRI_7 on a tet face: space Jacobian rows x1 - x3, x2 - x3, detJ = |cross| / 2
+        for (int q = 0; q < 7; ++q) {
+            phi[q][0] = kTri7P[q][0];
+            phi[q][1] = kTri7P[q][1];
+            phi[q][2] = 1.0 - kTri7P[q][0] - kTri7P[q][1];
+            w[q] = kTri7W[q];
+            double a[3], b[3];
+            for (int k = 0; k < 3; ++k) {
+                a[k] = Xf[0][k] - Xf[2][k];
+                b[k] = Xf[1][k] - Xf[2][k];
+            }
+            detJ[q] = cross_norm(a, b) / 2.0;
+        }
+        return 7;
+    }
+    if (nvs == 4) {   // QUAD_2x2 on a hex face: bilinear map, detJ = |dX/ds x dX/dt|
+        for (int q = 0; q < 4; ++q) {
+            const double s = kQuad2[q][0], t = kQuad2[q][1];
+            const double Nq[4] = {0.25 * (1 - s) * (1 - t), 0.25 * (1 + s) * (1 - t), 0.25 * (1 + s) * (1 + t),
+                                  0.25 * (1 - s) * (1 + t)};
+            const double ds[4] = {-0.25 * (1 - t), 0.25 * (1 - t), 0.25 * (1 + t), -0.25 * (1 + t)};
+            const double dt[4] = {-0.25 * (1 - s), -0.25 * (1 + s), 0.25 * (1 + s), 0.25 * (1 - s)};
+            double a[3] = {0, 0, 0}, b[3] = {0, 0, 0};
+            for (int k = 0; k < 4; ++k) {
+                phi[q][k] = Nq[k];
+                for (int c = 0; c < 3; ++c) {
+                    a[c] += ds[k] * Xf[k][c];
+                    b[c] += dt[k] * Xf[k][c];
+                }
+            }
+            w[q] = 1.0;
+            detJ[q] = cross_norm(a, b);
+        }
+        return 4;
+    }
+    for (int q = 0; q < 3; ++q) {   // BAR_3 on a 2D side: detJ = |dX/ds|
+        phi[q][0] = 0.5 * (1.0 - kBar3P[q]);
+        phi[q][1] = 0.5 * (1.0 + kBar3P[q]);
+        w[q] = kBar3Wt[q];
+        double l2 = 0.0;
+        for (int c = 0; c < D; ++c) l2 += 0.25 * (Xf[1][c] - Xf[0][c]) * (Xf[1][c] - Xf[0][c]);
+        detJ[q] = std::sqrt(l2);
+    }
+    return 3;
+}
+
+// Side set on descriptor d: facets (element, vertices in IP parametric coordinates, normal),
+// added into vals / rhs with dense N, B, D, n_flat and T per facet.
+void side_assemble(const mgpu_set_desc& d, int iwg, int nvs, int64_t nfc, const int32_t* fe, const double* fx,
+                   const double* fnrm, double alpha, const double* value, const double* u, const int32_t* row_ptr,
+                   const int32_t* col_idx, double* vals, double* rhs) {
+    const int D = d.spatial_dim, nb = d.n_bases;
+    const bool elast = d.iwg == MGPU_IWG_STRUC_LINEAR_BULK;
+    const int nf = elast ? D : 1, np = nf * nb, ns = elast ? (D == 3 ? 6 : 3) : D;
+    const bool neumann = iwg == MGPU_IWG_SPATIALDIFF_NEUMANN || iwg == MGPU_IWG_STRUC_LINEAR_NEUMANN;
+    const double beta = (iwg == MGPU_IWG_SPATIALDIFF_DIRICHLET_SYMMETRIC_NITSCHE || iwg == MGPU_IWG_STRUC_LINEAR_DIRICHLET_SYMMETRIC_NITSCHE) ? -1.0 : 1.0;
+    // cluster measure per element (sum of the side measures of its facets)
+    std::map<int32_t, double> cluster;
+    auto facet_phys = [&](int64_t k, double (*Xf)[3]) {
+        const int32_t e = fe[k];
+        const double* box = d.ip_box + (size_t)e * 2 * D;
+        for (int v = 0; v < nvs; ++v)
+            for (int c = 0; c < 3; ++c)
+                Xf[v][c] = c < D ? box[c] + 0.5 * (fx[((size_t)k * nvs + v) * D + c] + 1.0) * box[D + c] : 0.0;
+    };
+    for (int64_t k = 0; k < nfc; ++k) {
+        double Xf[4][3], phi[7][4], w[7], dJ[7];
+        facet_phys(k, Xf);
+        const int nq = side_points(D, nvs, Xf, phi, w, dJ);
+        double m = 0.0;
+        for (int q = 0; q < nq; ++q) m += w[q] * dJ[q];
+        cluster[fe[k]] += m;
+    }
+    Scratch s;
+    for (int64_t k = 0; k < nfc; ++k) {
+        const int32_t e = fe[k];
+        element_topology(d, nf, e, s.ids);
+        const int na = (int)s.ids.size(), na_t = na / nf;
+        s.T.assign((size_t)np * na, 0.0);
+        for (int b = 0; b < nb; ++b)
+            for (int q = d.t_ptr[e * nb + b]; q < d.t_ptr[e * nb + b + 1]; ++q) {
+                const int col = (int)(std::lower_bound(s.ids.begin(), s.ids.begin() + na_t, d.t_adof[q]) - s.ids.begin());
+                for (int f = 0; f < nf; ++f) s.T[(size_t)(f * nb + b) * na + f * na_t + col] += d.t_weight[q];
+            }
+        s.up.assign(np, 0.0);
+        for (int i = 0; i < np; ++i)
+            for (int a = 0; a < na; ++a) s.up[i] += s.T[(size_t)i * na + a] * u[s.ids[a]];
+        s.Kp.assign((size_t)np * np, 0.0);
+        s.Rp.assign(np, 0.0);
+        // material and the Nitsche parameter of the side cluster
+        double Dm[6][6] = {{0}}, kcond = 0.0, Emod = 0.0;
+        if (elast) {
+            const double E = d.material[2 * e], nu = d.material[2 * e + 1];
+            Emod = E;
+            if (D == 3) {
+                const double pre = E / (1.0 + nu) / (1.0 - 2.0 * nu);
+                for (int i = 0; i < 3; ++i)
+                    for (int j = 0; j < 3; ++j) Dm[i][j] = i == j ? pre * (1.0 - nu) : pre * nu;
+                for (int i = 3; i < 6; ++i) Dm[i][i] = pre * (1.0 - 2.0 * nu) / 2.0;
+            } else {
+                const double pre = E / (1.0 - nu * nu);
+                Dm[0][0] = pre; Dm[1][1] = pre; Dm[0][1] = pre * nu; Dm[1][0] = pre * nu;
+                Dm[2][2] = pre * 0.5 * (1.0 - nu);
+            }
+        } else {
+            kcond = d.material[e];
+        }
+        const double A = cluster[e];
+        const double hlen = D == 3 ? std::pow(4.0 * A / 3.141592653589793, 0.5) : A;
+        const double gamma = alpha * (elast ? Emod : kcond) / hlen;
+        // flattened normal (n_f x n_s): traction = n_flat sigma
+        const double* nv = fnrm + (size_t)k * D;
+        double nfl[3][6] = {{0}};
+        if (elast && D == 3) {
+            nfl[0][0] = nv[0]; nfl[1][1] = nv[1]; nfl[2][2] = nv[2];
+            nfl[0][4] = nv[2]; nfl[0][5] = nv[1]; nfl[1][3] = nv[2];
+            nfl[1][5] = nv[0]; nfl[2][3] = nv[1]; nfl[2][4] = nv[0];
+        } else if (elast) {
+            nfl[0][0] = nv[0]; nfl[0][2] = nv[1]; nfl[1][1] = nv[1]; nfl[1][2] = nv[0];
+        } else {
+            for (int c = 0; c < D; ++c) nfl[0][c] = nv[c];
+        }
+        // IP cell vertices
+        const double* box = d.ip_box + (size_t)e * 2 * D;
+        double X[8][3];
+        for (int v = 0; v < nb; ++v) {
+            double xv[3];
+            basis_xi(D, nb, v, xv);
+            for (int c = 0; c < D; ++c) X[v][c] = box[c] + 0.5 * (xv[c] + 1.0) * box[D + c];
+        }
+        double Xf[4][3], phi[7][4], w[7], dJ[7];
+        facet_phys(k, Xf);
+        const int nq = side_points(D, nvs, Xf, phi, w, dJ);
+        std::vector<double> Nm((size_t)nf * np), B((size_t)ns * np), Tt((size_t)nf * np);
+        for (int q = 0; q < nq; ++q) {
+            if (dJ[q] < -1.0e-6) continue;
+            const double ws = w[q] * dJ[q];
+            // point in the IP cell (parametric), N and dN/dx there
+            double xip[3] = {0, 0, 0};
+            for (int c = 0; c < D; ++c)
+                for (int v = 0; v < nvs; ++v) xip[c] += phi[q][v] * fx[((size_t)k * nvs + v) * D + c];
+            double Nb[27], dN[3][27];
+            basis_N(D, nb, xip, Nb);
+            basis_dN(D, nb, xip, dN);
+            double Jt[3][3] = {{0}};
+            for (int r = 0; r < D; ++r)
+                for (int c = 0; c < D; ++c)
+                    for (int a = 0; a < nb; ++a) Jt[r][c] += dN[r][a] * X[a][c];
+            double dNdx[3][27];
+            for (int r = 0; r < D; ++r)
+                for (int a = 0; a < nb; ++a) dNdx[r][a] = dN[r][a] / Jt[r][r];
+            std::fill(Nm.begin(), Nm.end(), 0.0);
+            std::fill(B.begin(), B.end(), 0.0);
+            for (int f = 0; f < nf; ++f)
+                for (int a = 0; a < nb; ++a) Nm[(size_t)f * np + f * nb + a] = Nb[a];
+            if (elast && D == 3) {
+                for (int a = 0; a < nb; ++a) {
+                    B[0 * np + a] = dNdx[0][a]; B[4 * np + a] = dNdx[2][a]; B[5 * np + a] = dNdx[1][a];
+                    B[1 * np + nb + a] = dNdx[1][a]; B[3 * np + nb + a] = dNdx[2][a]; B[5 * np + nb + a] = dNdx[0][a];
+                    B[2 * np + 2 * nb + a] = dNdx[2][a]; B[3 * np + 2 * nb + a] = dNdx[1][a];
+                    B[4 * np + 2 * nb + a] = dNdx[0][a];
+                }
+            } else if (elast) {
+                for (int a = 0; a < nb; ++a) {
+                    B[0 * np + a] = dNdx[0][a]; B[2 * np + a] = dNdx[1][a];
+                    B[1 * np + nb + a] = dNdx[1][a]; B[2 * np + nb + a] = dNdx[0][a];
+                }
+            } else {
+                for (int i = 0; i < D; ++i)
+                    for (int a = 0; a < nb; ++a) B[(size_t)i * np + a] = dNdx[i][a];
+            }
+            // T = n_flat D B (diffusion: k n^T B)
+            for (int f = 0; f < nf; ++f)
+                for (int c = 0; c < np; ++c) {
+                    double v = 0.0;
+                    for (int i = 0; i < ns; ++i) {
+                        double db = 0.0;
+                        if (elast)
+                            for (int j = 0; j < ns; ++j) db += Dm[i][j] * B[(size_t)j * np + c];
+                        else
+                            db = kcond * B[(size_t)i * np + c];
+                        v += nfl[f][i] * db;
+                    }
+                    Tt[(size_t)f * np + c] = v;
+                }
+            // jump u - u_D and traction at the point
+            double uh[3] = {0, 0, 0}, tr[3] = {0, 0, 0};
+            for (int f = 0; f < nf; ++f)
+                for (int c = 0; c < np; ++c) {
+                    uh[f] += Nm[(size_t)f * np + c] * s.up[c];
+                    tr[f] += Tt[(size_t)f * np + c] * s.up[c];
+                }
+            for (int r = 0; r < np; ++r) {
+                if (neumann) {
+                    double v = 0.0;
+                    for (int f = 0; f < nf; ++f) v += Nm[(size_t)f * np + r] * value[f];
+                    s.Rp[r] -= ws * v;
+                    continue;
+                }
+                double rv = 0.0;
+                for (int f = 0; f < nf; ++f) {
+                    const double jump = uh[f] - value[f];
+                    rv += -Nm[(size_t)f * np + r] * tr[f] + beta * Tt[(size_t)f * np + r] * jump +
+                          gamma * Nm[(size_t)f * np + r] * jump;
+                }
+                s.Rp[r] += ws * rv;
+                for (int c = 0; c < np; ++c) {
+                    double kv = 0.0;
+                    for (int f = 0; f < nf; ++f)
+                        kv += -Nm[(size_t)f * np + r] * Tt[(size_t)f * np + c] + beta * Tt[(size_t)f * np + r] * Nm[(size_t)f * np + c] +
+                              gamma * Nm[(size_t)f * np + r] * Nm[(size_t)f * np + c];
+                    s.Kp[(size_t)r * np + c] += ws * kv;
+                }
+            }
+        }
+        // K_e = T^T K_p T, R_e = T^T R_p, scatter by row search
+        std::vector<double> KT((size_t)np * na, 0.0);
+        for (int i = 0; i < np; ++i)
+            for (int a = 0; a < na; ++a) {
+                double v = 0.0;
+                for (int j = 0; j < np; ++j) v += s.Kp[(size_t)i * np + j] * s.T[(size_t)j * na + a];
+                KT[(size_t)i * na + a] = v;
+            }
+        for (int a = 0; a < na; ++a) {
+            const int32_t row = s.ids[a];
+            const int32_t* c0 = col_idx + row_ptr[row];
+            const int32_t* c1 = col_idx + row_ptr[row + 1];
+            for (int b = 0; b < na; ++b) {
+                double v = 0.0;
+                for (int i = 0; i < np; ++i) v += s.T[(size_t)i * na + a] * KT[(size_t)i * na + b];
+                const int32_t* it = std::lower_bound(c0, c1, s.ids[b]);
+                if (it == c1 || *it != s.ids[b]) throw std::runtime_error("column not in graph");
+                vals[it - col_idx] += v;
+            }
+            double r = 0.0;
+            for (int i = 0; i < np; ++i) r += s.T[(size_t)i * na + a] * s.Rp[i];
+            rhs[row] += r;
+        }
+    }
+}
+
+// Facets of the oracle's own problem: domain boundary faces of material cells and interface faces
+// of material sub-cells (on no cell face, shared with no other material sub-cell of the cell).
+void problem_facets(const OracleProblem& p, int where, int nvs, std::vector<int32_t>& fe, std::vector<double>& fx,
+                    std::vector<double>& fn) {
+    const int D = p.dim, nv = D + 1;
+    const int bd = where > 0 ? (where - 1) / 2 : -1;
+    const double bs = where > 0 ? ((where - 1) % 2 ? 1.0 : -1.0) : 0.0;
+    const bool uncut_faces = nvs == 4 || (D == 2 && where > 0);
+    const bool cut_faces = nvs != 4;
+    auto on_face = [&](const std::vector<const double*>& vs, int dd, double sg) {
+        for (const double* v : vs)
+            if (std::fabs(v[dd] - sg) > 1e-12) return false;
+        return true;
+    };
+    for (size_t e = 0; e < p.elem_cell.size(); ++e) {
+        const int64_t c = p.elem_cell[e];
+        const int64_t ijk[3] = {c % p.n[0], (c / p.n[0]) % p.n[1], D == 3 ? c / ((int64_t)p.n[0] * p.n[1]) : 0};
+        const bool at_b = where > 0 && (bs < 0 ? ijk[bd] == 0 : ijk[bd] == p.n[bd] - 1);
+        const double* box = p.ip_box.data() + e * 2 * D;
+        const int s0 = p.ig_ptr[e], s1 = p.ig_ptr[e + 1];
+        if (s0 == s1) {
+            if (!uncut_faces || !at_b) continue;
+            int o[2], m = 0;
+            for (int dd = 0; dd < D; ++dd)
+                if (dd != bd) o[m++] = dd;
+            const double cyc[4][2] = {{-1, -1}, {1, -1}, {1, 1}, {-1, 1}};
+            fe.push_back((int32_t)e);
+            for (int k = 0; k < (D == 3 ? 4 : 2); ++k) {
+                double v[3] = {0, 0, 0};
+                v[bd] = bs;
+                if (D == 3) { v[o[0]] = cyc[k][0]; v[o[1]] = cyc[k][1]; }
+                else v[o[0]] = k == 0 ? -1.0 : 1.0;
+                for (int dd = 0; dd < D; ++dd) fx.push_back(v[dd]);
+            }
+            for (int dd = 0; dd < D; ++dd) fn.push_back(dd == bd ? bs : 0.0);
+            continue;
+        }
+        if (!cut_faces || (where > 0 && !at_b)) continue;
+        const double* X0 = p.ig_xi.data() + (size_t)s0 * nv * D;
+        const int nsub = s1 - s0;
+        // sorted vertex list of face (sub-cell s, omitted vertex k)
+        auto face_sorted = [&](int sc, int k) {
+            std::vector<std::vector<double>> vs;
+            for (int q = 0; q < nv; ++q)
+                if (q != k) vs.push_back(std::vector<double>(X0 + (sc * nv + q) * D, X0 + (sc * nv + q) * D + D));
+            std::sort(vs.begin(), vs.end());
+            return vs;
+        };
+        for (int sc = 0; sc < nsub; ++sc)
+            for (int k = 0; k < nv; ++k) {
+                std::vector<const double*> vs;
+                for (int q = 0; q < nv; ++q)
+                    if (q != k) vs.push_back(X0 + (sc * nv + q) * D);
+                double nrm[3] = {0, 0, 0};
+                if (where > 0) {
+                    if (!on_face(vs, bd, bs)) continue;
+                    nrm[bd] = bs;
+                } else {
+                    bool cf = false;
+                    for (int dd = 0; dd < D; ++dd) cf = cf || on_face(vs, dd, -1.0) || on_face(vs, dd, 1.0);
+                    if (cf) continue;
+                    const auto me = face_sorted(sc, k);
+                    bool shared = false;
+                    for (int s2 = 0; s2 < nsub && !shared; ++s2)
+                        for (int k2 = 0; k2 < nv && !shared; ++k2)
+                            if (!(s2 == sc && k2 == k) && face_sorted(s2, k2) == me) shared = true;
+                    if (shared) continue;
+                    // outward normal of the sub-cell (physical coordinates)
+                    double x[3][3] = {{0}}, xo[3] = {0, 0, 0};
+                    for (int a = 0; a < (int)vs.size(); ++a)
+                        for (int dd = 0; dd < D; ++dd) x[a][dd] = box[dd] + 0.5 * (vs[a][dd] + 1.0) * box[D + dd];
+                    for (int dd = 0; dd < D; ++dd) xo[dd] = box[dd] + 0.5 * (X0[(sc * nv + k) * D + dd] + 1.0) * box[D + dd];
+                    if (D == 3) {
+                        const double a1[3] = {x[1][0] - x[0][0], x[1][1] - x[0][1], x[1][2] - x[0][2]};
+                        const double a2[3] = {x[2][0] - x[0][0], x[2][1] - x[0][1], x[2][2] - x[0][2]};
+                        nrm[0] = a1[1] * a2[2] - a1[2] * a2[1];
+                        nrm[1] = a1[2] * a2[0] - a1[0] * a2[2];
+                        nrm[2] = a1[0] * a2[1] - a1[1] * a2[0];
+                    } else {
+                        nrm[0] = x[1][1] - x[0][1];
+                        nrm[1] = x[0][0] - x[1][0];
+                    }
+                    double dot = 0.0, len = 0.0;
+                    for (int dd = 0; dd < D; ++dd) {
+                        dot += nrm[dd] * (xo[dd] - x[0][dd]);
+                        len += nrm[dd] * nrm[dd];
+                    }
+                    len = std::sqrt(len);
+                    if (!(len > 0.0)) continue;
+                    for (int dd = 0; dd < D; ++dd) nrm[dd] = (dot > 0.0 ? -nrm[dd] : nrm[dd]) / len;
+                }
+                fe.push_back((int32_t)e);
+                for (const double* v : vs)
+                    for (int dd = 0; dd < D; ++dd) fx.push_back(v[dd]);
+                for (int dd = 0; dd < D; ++dd) fn.push_back(nrm[dd]);
+            }
+    }
+}
+
 thread_local std::string g_err;
 
 mgpu_set_desc desc_of(const OracleProblem& p) {
@@ -1406,6 +1764,40 @@ int oracle_nh_cell(int dim, const uint32_t* g, const double* f, const double* x,
     } catch (const std::exception& ex) {
         g_err = ex.what();
         return -1;
+    }
+}
+
+// Facets of the oracle's problem (problem_facets): returns their count n; elem[n], xi[n][nvs][dim],
+// normal[n][dim] are written when the arrays are given (max facets), or -1 on error.
+int64_t oracle_facets(void* h, int where, int nvs, int32_t* elem, double* xi, double* normal, int64_t max) {
+    try {
+        const OracleProblem& p = *static_cast<OracleProblem*>(h);
+        std::vector<int32_t> fe;
+        std::vector<double> fx, fn;
+        problem_facets(p, where, nvs, fe, fx, fn);
+        const int64_t n = (int64_t)fe.size();
+        if (elem && n <= max) {
+            std::copy(fe.begin(), fe.end(), elem);
+            std::copy(fx.begin(), fx.end(), xi);
+            std::copy(fn.begin(), fn.end(), normal);
+        }
+        return n;
+    } catch (const std::exception& ex) {
+        g_err = ex.what();
+        return -1;
+    }
+}
+
+// Side set on an external descriptor and graph (side_assemble), ADDED into vals / rhs.
+int oracle_side_assemble(const mgpu_set_desc* d, const int32_t* row_ptr, const int32_t* col_idx, int iwg, int nvs,
+                         int64_t n_facets, const int32_t* elem, const double* xi, const double* normal, double alpha,
+                         const double* value, const double* u, double* vals, double* rhs) {
+    try {
+        side_assemble(*d, iwg, nvs, n_facets, elem, xi, normal, alpha, value, u, row_ptr, col_idx, vals, rhs);
+        return 0;
+    } catch (const std::exception& ex) {
+        g_err = ex.what();
+        return 1;
     }
 }
 

@@ -207,3 +207,41 @@ def nh_cell(dim, gid, phi, x, phase):
     if n < 0:
         raise RuntimeError("oracle_nh_cell: " + lib.oracle_last_error().decode())
     return out[: n * (dim + 1) * 3].reshape(n, dim + 1, 3)
+
+
+def facets(problem, where, nvs):
+    """Facets of an OracleProblem (its own extraction): (elem, xi [n][nvs][dim], normal [n][dim])."""
+    lib = load()
+    lib.oracle_facets.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                  ctypes.c_void_p, ctypes.c_int64]
+    lib.oracle_facets.restype = ctypes.c_int64
+    n = lib.oracle_facets(problem._h, where, nvs, None, None, None, 0)
+    if n < 0:
+        raise RuntimeError("oracle_facets: " + lib.oracle_last_error().decode())
+    dim = problem.dim
+    elem = np.zeros(max(n, 1), np.int32)
+    xi = np.zeros(max(n, 1) * nvs * dim)
+    nr = np.zeros(max(n, 1) * dim)
+    lib.oracle_facets(problem._h, where, nvs, _p(elem), _p(xi), _p(nr), n)
+    return elem[:n].copy(), xi[: n * nvs * dim].reshape(n, nvs, dim), nr[: n * dim].reshape(n, dim)
+
+
+def side_assemble(desc, row_ptr, col_idx, iwg, nvs, elem, xi, normal, alpha, value, u, vals, rhs):
+    """Reference algorithm of a side set on descriptor `desc`, ADDED into vals / rhs (numpy, in place)."""
+    lib = load()
+    lib.oracle_side_assemble.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                         ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                         ctypes.c_void_p]
+    lib.oracle_side_assemble.restype = ctypes.c_int
+    row_ptr = np.ascontiguousarray(row_ptr, np.int32)
+    col_idx = np.ascontiguousarray(col_idx, np.int32)
+    elem = np.ascontiguousarray(elem, np.int32)
+    xi = np.ascontiguousarray(xi, np.float64)
+    normal = np.ascontiguousarray(normal, np.float64)
+    value = np.ascontiguousarray(list(value) + [0.0] * (3 - len(value)), np.float64)
+    u = np.ascontiguousarray(u, np.float64)
+    assert vals.dtype == np.float64 and rhs.dtype == np.float64 and vals.flags.c_contiguous and rhs.flags.c_contiguous
+    if lib.oracle_side_assemble(ctypes.byref(desc), _p(row_ptr), _p(col_idx), iwg, nvs, len(elem), _p(elem), _p(xi),
+                                _p(normal), alpha, _p(value), _p(u), _p(vals), _p(rhs)) != 0:
+        raise RuntimeError("oracle_side_assemble: " + lib.oracle_last_error().decode())

@@ -1,10 +1,12 @@
 """Shared test helpers: workload specs of the configurations, lattice geometry, CSR utilities."""
+import ctypes
 import os
 import struct
 
 import numpy as np
 
 import moris_amd as M
+from oracle import py as O
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 SEED = 20261202
@@ -72,3 +74,38 @@ def csr_to_scipy(row_ptr, col_idx, vals, n):
 def rel(a, b):
     nb = np.linalg.norm(b)
     return np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)
+
+
+# facet vertex counts of the side geometries (MGPU_GEOM_TRI3, MGPU_GEOM_QUAD4, MGPU_GEOM_LINE2)
+NV = {2: 3, 0: 4, 5: 2}
+
+
+def oracle_side_system(spec, sides, u):
+    """Oracle bulk assembly plus side sets [(iwg, where, geom, alpha, value)] at u."""
+    Q = O.OracleProblem(spec)
+    vals, R = Q.assemble(u)
+    s = Q.setup()
+    desc = oracle_desc(Q, s)
+    for iwg, where, geom, alpha, value in sides:
+        e, x, n = O.facets(Q, where, NV[geom])
+        O.side_assemble(desc[0], s["row_ptr"], s["col_idx"], iwg, NV[geom], e, x, n, alpha, value, u, vals, R)
+    return Q, s, vals, R
+
+
+def oracle_desc(Q, s):
+    """A set descriptor over the oracle's own set-up arrays (for the side assembly)."""
+    d = M.SetDesc()
+    dim = Q.dim
+    d.spatial_dim = dim
+    d.iwg = Q.iwg
+    d.ip_geometry = M.GEOM_HEX8 if dim == 3 else M.GEOM_QUAD4
+    d.ig_geometry = M.GEOM_TET4 if dim == 3 else M.GEOM_TRI3
+    d.n_elements = Q.n_elements
+    d.n_bases = Q.nb
+    d.n_adofs_per_type = Q.n_adofs_per_type
+    keep = {k: np.ascontiguousarray(s[k]) for k in ("ip_box", "material", "t_ptr", "t_adof", "t_weight", "ig_ptr",
+                                                      "ig_xi", "elem_cell")}
+    for k, a in keep.items():
+        ct = ctypes.c_int32 if a.dtype == np.int32 else ctypes.c_double
+        setattr(d, k, a.ctypes.data_as(ctypes.POINTER(ct)))
+    return d, keep
