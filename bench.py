@@ -93,11 +93,37 @@ def fit_flops_of(n_points, n_cut, n_moments=23, n_points_rule=27):
     return n_points * (2 * 12 + 2 * n_moments) + n_cut * 3 * 2 * n_moments * n_points_rule
 
 
-def cpu_baseline(P, S, u, budget_s=20.0, probe_n=4000, rule="TET_11"):
-    """Oracle (FP64 C++ restatement of the reference algorithm) on the host cores, on a strided
-    sample of one variant's elements sized to the time budget."""
+def host_cpu_info():
+    """The host the CPU baseline runs on: CPU model, os.cpu_count(), the CPUs this process may run
+    on, OMP_NUM_THREADS, and the worker count used (the CPUs granted to this job: the affinity set,
+    capped by OMP_NUM_THREADS where the job's share is set that way)."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    try:
+        omp = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    except ValueError:
+        omp = 0
+    cores = min(aff, omp) if omp > 0 else aff
+    return {"cpu_model": model, "nproc": os.cpu_count(), "affinity_cpus": aff, "omp_num_threads": omp or None,
+            "cores": max(1, cores)}
+
+
+def cpu_baseline(P, S, u, budget_s=20.0, probe_n=4000, rule="TET_11", budget_1core_s=8.0):
+    """Oracle (FP64 C++ restatement of the reference algorithm, its -O3 -march=x86-64-v3 build) on
+    the host cores granted to this job, on a strided sample of one variant's elements sized to the
+    time budget; then the same on one core on a smaller sample."""
     from oracle import py as O
-    cores = min(os.cpu_count() or 1, 16)
+    O.use_bench_build()
+    host = host_cpu_info()
+    cores = host["cores"]
     rp, ci = S.pattern()
     ne = P.n_elements
     elems = np.arange(ne, dtype=np.int64)
@@ -105,20 +131,30 @@ def cpu_baseline(P, S, u, budget_s=20.0, probe_n=4000, rule="TET_11"):
     t0 = time.perf_counter()
     O.assemble_list(P.desc, rp, ci, u, probe, n_threads=cores)
     per_elem = (time.perf_counter() - t0) / len(probe)
-    stride = max(1, int(np.ceil(ne * per_elem / (budget_s / 2))))
-    sample = elems[::stride]
-    runs = []
-    t_end = time.perf_counter() + budget_s
-    while not runs or (time.perf_counter() < t_end and len(runs) < 3):
-        t0 = time.perf_counter()
-        O.assemble_list(P.desc, rp, ci, u, sample, n_threads=cores)
-        runs.append(time.perf_counter() - t0)
-    best = min(runs)
-    cut_frac = float(np.mean(np.diff(P.arrays()["ig_ptr"])[sample] > 0))
-    return {"value": len(sample) / best, "unit": "elements/s", "cores": cores, "kind": "port",
-            "sample": "one element in %d of variant 0 (%d of %d elements, cut fraction %.3f), FP64 subdivision "
-                      "(%s), best of %d runs, %d worker threads with private CSR arrays + row-wise reduction"
-                      % (stride, len(sample), ne, cut_frac, rule, len(runs), cores)}
+
+    def timed(n_threads, budget):
+        stride = max(1, int(np.ceil(ne * per_elem * (cores / n_threads) / (budget / 2))))
+        sample = elems[::stride]
+        runs = []
+        t_end = time.perf_counter() + budget
+        while not runs or (time.perf_counter() < t_end and len(runs) < 3):
+            t1 = time.perf_counter()
+            O.assemble_list(P.desc, rp, ci, u, sample, n_threads=n_threads)
+            runs.append(time.perf_counter() - t1)
+        cut_frac = float(np.mean(np.diff(P.arrays()["ig_ptr"])[sample] > 0))
+        return len(sample) / min(runs), ("one element in %d of variant 0 (%d of %d elements, cut fraction %.3f), "
+                                         "FP64 subdivision (%s), best of %d runs, %d worker thread%s%s" % (
+                                             stride, len(sample), ne, cut_frac, rule, len(runs), n_threads,
+                                             "s" if n_threads > 1 else "",
+                                             " with private CSR arrays + row-wise reduction" if n_threads > 1 else ""))
+
+    value, sample = timed(cores, budget_s)
+    value1, sample1 = timed(1, budget_1core_s)
+    out = {"value": value, "unit": "elements/s", "cores": cores, "kind": "port", "sample": sample,
+           "value_1core": value1, "sample_1core": sample1,
+           "build": "oracle/liboracle_bench.so: g++ -O3 -march=x86-64-v3 (built off the GPU host)"}
+    out.update({k: v for k, v in host.items() if k != "cores"})
+    return out
 
 
 def run_hex27(args):
@@ -250,6 +286,17 @@ def run_lattice(args):
     print(json.dumps(out), flush=True)
 
 
+def gauss_prior_network():
+    """A network whose output is the HEX_3x3x3 Gauss weights / 8 for every input (all weights zero,
+    last bias the Gauss weights): the moment fit then starts from the volume-scaled Gauss rule."""
+    _, _, g = L.operators()
+    net = L.MLP(seed=1)
+    p = np.zeros(net.n_params, np.float32)
+    p[-27:] = (g / 8.0).astype(np.float32)
+    net.params = p
+    return net
+
+
 def timed_replays(B, steps, warmup):
     for _ in range(warmup):
         B.launch()
@@ -346,6 +393,20 @@ def main():
         err_k.append(float(np.linalg.norm(v - v0) / np.linalg.norm(v0)))
         err_r.append(float(np.linalg.norm(r - r0) / np.linalg.norm(r0)))
         accepted += int(L.result(S)[2].sum())
+    fl0 = L.result(sets[0])[2].copy()
+    # the same path with the network replaced by the volume-scaled Gauss prior (a constant network:
+    # HEX_3x3x3 weights, scaled to each element's volume moment by the fit), to show what the trained
+    # network changes: acceptance, time, error
+    gauss = gauss_prior_network()
+    for S in sets:
+        L.attach(S, gauss)
+    sec_g, _, _, _, st_g = run_mode(M.PREC_MIXED, M.QUAD_LEARNED, "mixed + Gauss prior")
+    err_kg, acc_g = [], 0
+    for S, (_, vb, rb), (v0, r0) in zip(sets, bufs, ref):
+        err_kg.append(float(np.linalg.norm(vb.to_array(np.float64, S.nnz) - v0) / np.linalg.norm(v0)))
+        acc_g += int(L.result(S)[2].sum())
+    for S in sets:
+        L.attach(S, net)
     del ref
 
     value = n_el / sec
@@ -355,27 +416,42 @@ def main():
     P0, _ = problems[0]
     S0 = sets[0]
     st_bytes = stencil_bytes(P0.spec, S0, P0.n_elements, P0.n_cut, ke_bytes=4)
-    fl0 = L.result(S0)[2]
     n_learned0 = int(fl0.sum())
     ig = np.diff(P0.arrays()["ig_ptr"])
     fallback_ip = int(ig[ig > 0][fl0 == 0].sum()) * 11
     cut_flops_l, f_ip, f_t = elasticity_flops(27 * n_learned0 + fallback_ip, P0.n_cut)
     cut_flops_64, _, _ = elasticity_flops(P0.n_ig * 11, P0.n_cut)
     fit_flops = fit_flops_of(P0.n_ig * 11, P0.n_cut)
+    # executed MFMA work: the cut-element kernel issues 3 tiles (16 x 16 x 4) per step of 4 points
+    # (ceil(points / 4) steps per element); the learned element kernel 3 tiles x 7 steps per element
+    steps = np.ceil(ig[ig > 0] * 11 / 4.0)
+    mfma_64 = float(steps.sum()) * 3 * 2048
+    mfma_l = float(n_learned0) * 3 * 7 * 2048 + float(steps[fl0 == 0].sum()) * 3 * 2048
     kernels = {
         "k_stencil": {"ms": st_l[4], "bound": "hbm", "algorithmic_bytes": st_bytes,
                       "GB/s": st_bytes / (st_l[4] * 1e-3) / 1e9},
-        "k_lq_moments+k_lq_project": {"ms": st_l[1], "bound": "fp64", "algorithmic_flops": fit_flops,
-                     "TFLOP/s": fit_flops / (st_l[1] * 1e-3) / 1e12 if st_l[1] > 0 else None},
-        "cut_integrands_learned": {"ms": st_l[2], "kernels": "k_lq_element + k_cut_elasticity<3,float> (fallback)",
-                                   "algorithmic_flops": cut_flops_l,
-                                   "TFLOP/s": cut_flops_l / (st_l[2] * 1e-3) / 1e12 if st_l[2] > 0 else None,
-                                   "points": 27 * n_learned0 + fallback_ip},
+        "k_lq_moments+k_lq_project": {"ms": st_l[1], "bound": "fp64 (VALU moments, MFMA fit)",
+                                      "algorithmic_flops": fit_flops,
+                                      "TFLOP/s": fit_flops / (st_l[1] * 1e-3) / 1e12 if st_l[1] > 0 else None},
+        "cut_integrands_learned": {"ms": st_l[2], "kernels": "k_lq_element<3,float> + k_cut_elasticity_mfma<float> "
+                                                             "(fallback)",
+                                   "executed_mfma_flops": mfma_l,
+                                   "TFLOP/s": mfma_l / (st_l[2] * 1e-3) / 1e12 if st_l[2] > 0 else None,
+                                   "peak_TFLOP/s": FP32_PEAK_TFLOPS,
+                                   "points": 27 * n_learned0 + fallback_ip,
+                                   "reference_shape_flops": cut_flops_l},
         "k_mlp_forward": {"ms": st_l[0]},
         "k_prep": {"ms": st_l[3]},
-        "cut_integrands_fp64": {"ms": st64[2], "kernels": "k_cut_elasticity<3,double>", "algorithmic_flops": cut_flops_64,
-                                "TFLOP/s": cut_flops_64 / (st64[2] * 1e-3) / 1e12 if st64[2] > 0 else None,
-                                "points": P0.n_ig * 11, "flops_per_point": f_ip, "flops_T_per_element": f_t},
+        "cut_integrands_fp64": {"ms": st64[2], "kernels": "k_cut_elasticity_mfma<double>",
+                                "executed_mfma_flops": mfma_64,
+                                "TFLOP/s": mfma_64 / (st64[2] * 1e-3) / 1e12 if st64[2] > 0 else None,
+                                "peak_TFLOP/s": FP64_PEAK_TFLOPS,
+                                "frac": mfma_64 / (st64[2] * 1e-3) / 1e12 / FP64_PEAK_TFLOPS if st64[2] > 0 else None,
+                                "points": P0.n_ig * 11, "reference_shape_flops": cut_flops_64,
+                                "reference_shape_flops_per_point": f_ip, "reference_shape_flops_T_per_element": f_t},
+        "note": "TFLOP/s figures are executed work (MFMA instructions issued) per kernel time; "
+                "reference_shape_flops count the reference algorithm's dense per-point products and T projection "
+                "and are not a utilisation figure",
     }
     dom = max(("k_stencil", st_l[4]), ("k_lq_moments+k_lq_project", st_l[1]), ("cut_integrands_learned", st_l[2]), key=lambda kv: kv[1])[0]
     if dom == "k_stencil":
@@ -414,6 +490,9 @@ def main():
                        "tuning_variant0": list(tune64), "stage_ms_variant0": st64},
         "speedup_vs_fp64_gpu": value / value64,
         "learned_accepted": accepted / max(n_cut, 1),
+        "gauss_prior_leg": {"value": n_el / sec_g, "ms_per_step": 1e3 * sec_g, "learned_accepted": acc_g / max(n_cut, 1),
+                            "max_rel_err_K_vs_fp64": max(err_kg), "stage_ms_variant0": st_g,
+                            "note": "network replaced by the volume-scaled HEX_3x3x3 Gauss prior, same fit and checks"},
         "max_rel_err_K_vs_fp64": max(err_k),
         "max_rel_err_R_vs_fp64": max(err_r),
         "graph_branches": nbl,

@@ -14,7 +14,19 @@ _lib = None
 
 
 def build():
-    subprocess.run(["make", "-s", "-C", _HERE, "-B", "liboracle.so"], check=True)
+    subprocess.run(["make", "-s", "-C", _HERE, "-B", "all"], check=True)
+
+
+BENCH_LIB_PATH = os.path.join(_HERE, "liboracle_bench.so")
+
+
+def use_bench_build():
+    """Load the -O3 -march=x86-64-v3 build (CPU baseline timing) instead of the checker build;
+    must be called before any other use of the oracle in the process."""
+    global LIB_PATH
+    if _lib is not None and LIB_PATH != BENCH_LIB_PATH:
+        raise RuntimeError("the oracle checker build is already loaded")
+    LIB_PATH = BENCH_LIB_PATH
 
 
 def load():
