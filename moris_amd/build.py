@@ -32,6 +32,7 @@ HIP_SOURCES = [
     "learned/lq_assemble.hip",
     "learned/mlp.hip",
     "solve/cg.hip",
+    "setup/gpu_setup.hip",
 ]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-I" + os.path.join(ROOT, "include")]

@@ -5,10 +5,12 @@
 // Derived vertices (face / cell-centre nodes of the regular subdivision) are evaluated at their own
 // coordinates: use_multilinear_interpolation defaults to false
 // (projects/PRM/src/fn_PRM_GEN_Parameters.hpp:207, cl_GEN_Design_Field.cpp:112-140).
+// Host and device share these functions (the GPU set-up evaluates the same expressions).
 #pragma once
 #include <cmath>
 #include <cstdint>
 #include "../../../include/moris_gpu.h"
+#include "fem_tables.hpp"
 
 namespace mgpu {
 
@@ -20,7 +22,7 @@ struct LevelSet {
     double gyroid_scale = 1.0;
     double plane[4] = {0, 0, 0, 0};
 
-    double operator()(const double* x) const {
+    MGPU_HD double operator()(const double* x) const {
         switch (kind) {
             case MGPU_LS_SPHERES: {
                 double best = INFINITY;
@@ -50,17 +52,17 @@ struct LevelSet {
 };
 
 // splitmix64 (Steele, Lea, Flood 2014) as a counter-based generator: value idx of stream seed.
-inline uint64_t splitmix64_at(uint64_t seed, uint64_t idx) {
+MGPU_HD inline uint64_t splitmix64_at(uint64_t seed, uint64_t idx) {
     uint64_t z = seed + (idx + 1) * 0x9E3779B97F4A7C15ULL;
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
     z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
     return z ^ (z >> 31);
 }
-inline double uniform01_at(uint64_t seed, uint64_t idx) {
+MGPU_HD inline double uniform01_at(uint64_t seed, uint64_t idx) {
     return (double)(splitmix64_at(seed, idx) >> 11) * (1.0 / 9007199254740992.0);
 }
 // Standard normal of lattice cell `cell` (Box-Muller on draws 2*cell, 2*cell+1).
-inline double normal_at(uint64_t seed, uint64_t cell) {
+MGPU_HD inline double normal_at(uint64_t seed, uint64_t cell) {
     const double u1 = 1.0 - uniform01_at(seed, 2 * cell);   // (0, 1]
     const double u2 = uniform01_at(seed, 2 * cell + 1);
     return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
