@@ -308,6 +308,38 @@ def timed_replays(B, steps, warmup):
     return (time.perf_counter() - t0) / steps
 
 
+def gpu_setup_leg(problems, sets, args, host_problem_s, host_set_s):
+    """Per-variant set-up (level set, XTK cut, numbering, T rows, topology, CSR pattern) on the GPU
+    beside the host set-up of the same variants; variant 0's arrays and pattern compared."""
+    from moris_amd import gpu_setup as G
+    dev_ms, wall_s = [], 0.0
+    for k, (P, _) in enumerate(problems):
+        t1 = time.perf_counter()
+        Gs = G.GpuSetup(P.spec, P._keep)
+        wall_s += time.perf_counter() - t1
+        dev_ms.append(Gs.ms)
+        if k == 0:
+            a, b = P.arrays(), Gs.arrays()
+            same = all(np.array_equal(a[key].view(np.uint8), b[key].view(np.uint8)) for key in a)
+            rp, ci = sets[0].pattern()
+            grp, gci = Gs.pattern()
+            same = same and np.array_equal(rp, grp) and np.array_equal(ci, gci)
+        Gs.close()
+    nv = len(problems)
+    log("set-up per variant: host %.1f ms (problem %.1f + set %.1f), GPU %.2f ms device, %.1f ms wall; variant 0 "
+        "identical: %s" % (1e3 * (host_problem_s + host_set_s) / nv, 1e3 * host_problem_s / nv, 1e3 * host_set_s / nv,
+                           float(np.median(dev_ms)), 1e3 * wall_s / nv, same))
+    return {"gpu_device_ms_per_variant_median": float(np.median(dev_ms)),
+            "gpu_wall_ms_per_variant": 1e3 * wall_s / nv,
+            "host_ms_per_variant": 1e3 * (host_problem_s + host_set_s) / nv,
+            "host_problem_ms_per_variant": 1e3 * host_problem_s / nv,
+            "host_set_create_ms_per_variant": 1e3 * host_set_s / nv,
+            "variant0_identical_to_host": bool(same),
+            "note": "GPU: mgpu_setup_build (device time by HIP events; wall includes allocation and the host "
+                    "copies behind its descriptor). Host: mgpu_problem_build on the host's threads + "
+                    "mgpu_set_create (pattern and uploads). Not part of the timed assembly."}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -349,10 +381,15 @@ def main():
 
     t0 = time.time()
     problems, sets, bufs = [], [], []
+    host_problem_s, host_set_s = 0.0, 0.0
     for k in range(args.variants):
         spec, keep = W.elasticity_spec(args.n, W.sphere_array(k))
+        t1 = time.perf_counter()
         P = M.Problem(spec, keep)
+        t2 = time.perf_counter()
         S = M.Set(P.desc, keepalive=P)
+        host_problem_s += t2 - t1
+        host_set_s += time.perf_counter() - t2
         L.attach(S, net)
         u = np.random.default_rng(SEED + 100 + k).uniform(-1.0, 1.0, S.n_rows)
         problems.append((P, u))
@@ -364,6 +401,7 @@ def main():
     nnz = sum(S.nnz for S in sets)
     log("setup %.1fs: %d variants, elements %d cut %d sub-cells %d nnz %d" % (time.time() - t0, args.variants, n_el,
                                                                               n_cut, n_ig, nnz))
+    setup = gpu_setup_leg(problems, sets, args, host_problem_s, host_set_s)
 
     def run_mode(prec, quad, label):
         for S, (ub, vb, rb) in zip(sets, bufs):
@@ -501,6 +539,7 @@ def main():
         "event_ms_per_replay": evl,
         "roofline": roof,
         "kernels": kernels,
+        "setup": setup,
     }
     if not args.no_cpu:
         log("cpu baseline ...")
