@@ -286,17 +286,6 @@ def run_lattice(args):
     print(json.dumps(out), flush=True)
 
 
-def gauss_prior_network():
-    """A network whose output is the HEX_3x3x3 Gauss weights / 8 for every input (all weights zero,
-    last bias the Gauss weights): the moment fit then starts from the volume-scaled Gauss rule."""
-    _, _, g = L.operators()
-    net = L.MLP(seed=1)
-    p = np.zeros(net.n_params, np.float32)
-    p[-27:] = (g / 8.0).astype(np.float32)
-    net.params = p
-    return net
-
-
 def timed_replays(B, steps, warmup):
     for _ in range(warmup):
         B.launch()
@@ -435,7 +424,7 @@ def main():
     # the same path with the network replaced by the volume-scaled Gauss prior (a constant network:
     # HEX_3x3x3 weights, scaled to each element's volume moment by the fit), to show what the trained
     # network changes: acceptance, time, error
-    gauss = gauss_prior_network()
+    gauss = L.gauss_prior()
     for S in sets:
         L.attach(S, gauss)
     sec_g, _, _, _, st_g = run_mode(M.PREC_MIXED, M.QUAD_LEARNED, "mixed + Gauss prior")

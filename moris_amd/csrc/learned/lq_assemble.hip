@@ -3,8 +3,7 @@
 // Per cut element (hex8 IP cell):
 //   (MLP forward)    8 -> 64 -> 64 -> 27 on BF16 MFMA (mlp.hip), its inputs the 8 nodal level-set
 //                    values over their largest magnitude: a prior for the 27 weights
-//   k_lq_moments     one lane per XTK sub-tet (the regular-subdivision tets the interface left
-//                    untouched add constant moments instead), elements packed into wavefronts by 16-lane
+//   k_lq_moments     one lane per XTK sub-tet, elements packed into wavefronts by 16-lane
 //                    quarters (a 48-lane element beside a 16-lane one, two 32-lane ones, ...):
 //                    the 23 moments of the material sub-cells in FP64 (TET_11 on the sub-tet,
 //                    its FP64 Jacobian taken once for its 11 points, reduction through LDS)
@@ -40,9 +39,9 @@ __global__ __launch_bounds__(64) void k_lq_moments(DevView s, LqView lv) {
     int own[4], sub0[4], nsubq[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        own[k] = lv.mom_owner[16 * blockIdx.x + k];
-        sub0[k] = lv.mom_owner[16 * blockIdx.x + 4 + k];
-        nsubq[k] = lv.mom_owner[16 * blockIdx.x + 8 + k];
+        own[k] = lv.mom_owner[12 * blockIdx.x + k];
+        sub0[k] = lv.mom_owner[12 * blockIdx.x + 4 + k];
+        nsubq[k] = lv.mom_owner[12 * blockIdx.x + 8 + k];
     }
     // (selects, not indexing, so that own[] stays in registers)
     auto own_at = [&](int k) { return k == 0 ? own[0] : (k == 1 ? own[1] : (k == 2 ? own[2] : own[3])); };
@@ -62,7 +61,7 @@ __global__ __launch_bounds__(64) void k_lq_moments(DevView s, LqView lv) {
     const int ig0 = (quarter == 0 ? sub0[0] : (quarter == 1 ? sub0[1] : (quarter == 2 ? sub0[2] : sub0[3])));
     const int nsub = c < 0 ? 0 : (quarter == 0 ? nsubq[0] : (quarter == 1 ? nsubq[1] : (quarter == 2 ? nsubq[2] : nsubq[3])));
     for (int t = lane - 16 * q0; t < nsub; t += width) {
-        const double2* src2 = reinterpret_cast<const double2*>(s.ig_xi + (size_t)lv.mom_gen[ig0 + t] * 12);
+        const double2* src2 = reinterpret_cast<const double2*>(s.ig_xi + (size_t)(ig0 + t) * 12);
         double X[12];
 #pragma unroll
         for (int h = 0; h < 6; ++h) {
@@ -134,9 +133,6 @@ __global__ __launch_bounds__(64) void k_lq_moments(DevView s, LqView lv) {
 #pragma unroll
         for (int qq = 0; qq < 4; ++qq)
             if (qq >= qs && own[qq] == cs) v += part[k * 4 + qq];
-        // the element's regular-subdivision tets: constant moments
-        for (uint32_t m = (uint32_t)lv.mom_owner[16 * blockIdx.x + 12 + qs]; m; m &= m - 1)
-            v += lv.reg_mom[24 * __builtin_ctz(m) + k];
         lv.mom[(size_t)cs * kLqMomentsPad + k] = k < kLqMoments ? v : 0.0;
     }
 }

@@ -21,11 +21,8 @@ struct LqView {
     float* w_pred;                  // [n_cut][27] MLP output (weights / 8)
     double* mom;                    // [n_cut][24] moments of the material sub-cells (23 + padding)
     int32_t n_mom_waves;            // wavefronts of the moment kernel
-    const int32_t* mom_owner;       // [n_mom_waves][16]: per 16-lane quarter the cut element (or
-                                    // -1), its first entry in mom_gen, its count there, and the
-                                    // element's regular-subdivision tets (bit t: tet t)
-    const int32_t* mom_gen;         // sub-cells (indices into ig_xi) the moment kernel integrates
-    const double* reg_mom;          // [24][24] moments of the 24 regular-subdivision tets
+    const int32_t* mom_owner;       // [n_mom_waves][12]: per 16-lane quarter the cut element (or
+                                    // -1), its first sub-cell and its sub-cell count
     float* w_fit;                   // [n_cut][27] fitted parametric weights
     int32_t* flag;                  // [n_cut] 1 learned, 0 fallback
     const double* Vt;               // [27][23]  V^T

@@ -302,22 +302,30 @@ __global__ __launch_bounds__(64) void k_cut_elasticity_mfma(DevView s) {
 }
 
 // ------------------------------------------------------------------------------------------------
-// k_records: the stencil's cell records, once per set (the material and the cut are fixed): one
-// thread per lattice cell writes its record at its lower-corner node, the material folded in: k
+// k_prep: per-call inputs of the stencil kernel, one thread per lattice node / cell.  Each node
+// gathers u of its adofs into the planar u_node (0 where the node has no adof); each cell writes
+// one record (void / uncut material / cut index) with the material folded in: k
 // (CM_Diffusion_Linear_Isotropic) or the three distinct entries of the isotropic D
 // (CM_Struc_Linear_Isotropic::full_3d / full_plane_stress).  The reference moments of an uncut
 // cell are compile-time constants (kRefGrad), scaled by the stencil itself.
-// k_prep: the per-call input, one thread per lattice node: u of the node's adofs into the planar
-// u_node (0 where the node has no adof).
 // ------------------------------------------------------------------------------------------------
 template <int DIM, int NF>
-__global__ __launch_bounds__(256) void k_records(DevView s) {
+__global__ __launch_bounds__(256) void k_prep(DevView s, const double* u) {
+    constexpr int NB = DIM == 3 ? 8 : 4;
+    constexpr int NG = DIM * NB;
     constexpr bool DIFF = NF == 1;
+    constexpr int NM = DIFF ? NB * NB : NG * NG;
+    constexpr int NQ = DIM == 3 ? kHex2Points : kQuad2Points;
     const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t n_cells = (int64_t)s.nx * s.ny * s.nz;
+    const int64_t n_nodes = (int64_t)(s.nx + 1) * (s.ny + 1) * (DIM == 3 ? s.nz + 1 : 1);
+    if (c < n_nodes) {
+        // u gathered per lattice node so that rows read it without the node -> adof hop
+        const int a = s.node_adof[c];
+#pragma unroll
+        for (int f = 0; f < NF; ++f) s.u_node[(size_t)f * n_nodes + c] = a >= 0 ? u[(size_t)f * s.N + a] : 0.0;
+    }
     if (c >= n_cells) return;
-    const int ci = (int)(c % s.nx), cj = (int)((c / s.nx) % s.ny), ck = DIM == 3 ? (int)(c / ((int64_t)s.nx * s.ny)) : 0;
-    const int64_t at = s.pad + ci + (int64_t)(s.nx + 1) * (cj + (int64_t)(s.ny + 1) * ck);
     const int e = s.cell_elem[c];
     if (DIFF) {
         double v = 0.0;
@@ -325,7 +333,7 @@ __global__ __launch_bounds__(256) void k_records(DevView s) {
             const int cut = s.elem_cut[e];
             v = cut >= 0 ? -(double)(cut + 1) : s.material[e];
         }
-        s.cell_rec[at] = v;
+        s.cell_rec[c] = v;
     } else {
         double r[4] = {0.0, 0.0, 0.0, 0.0};
         if (e >= 0) {
@@ -342,18 +350,8 @@ __global__ __launch_bounds__(256) void k_records(DevView s) {
                 r[3] = D[NS - 1][NS - 1];
             }
         }
-        for (int k = 0; k < 4; ++k) s.cell_rec[4 * at + k] = r[k];
+        for (int k = 0; k < 4; ++k) s.cell_rec[4 * c + k] = r[k];
     }
-}
-
-template <int NF>
-__global__ __launch_bounds__(256) void k_prep(DevView s, const double* u, int64_t n_nodes) {
-    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= n_nodes) return;
-    // u gathered per lattice node so that rows read it without the node -> adof hop
-    const int a = s.node_adof[c];
-#pragma unroll
-    for (int f = 0; f < NF; ++f) s.u_node[(size_t)f * s.u_stride + s.pad + c] = a >= 0 ? u[(size_t)f * s.N + a] : 0.0;
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -408,18 +406,22 @@ __device__ inline bool stencil_row(const DevView& s, const double (*sc)[3], int 
 
     constexpr int NC = DIM == 3 ? 8 : 4;
     // the row's CSR range and the records of its (up to NC) cells, all requested before any is
-    // used: a cell beyond the lattice reads the zero record of a node that is no cell's corner
+    // used: out-of-lattice cells read cell 0 and are masked afterwards
     const int row = AL * s.N + r;
     const int start = s.row_ptr[row];
     const int next = s.row_ptr[row + 1];
+    bool inside[NC];
     double recf[NC];
     double4 rec4[NC];
 #pragma unroll
     for (int cc = 0; cc < NC; ++cc) {
-        const int lower = node + ((cc & 1) ? 0 : -1) + (nx + 1) * (((cc & 2) ? 0 : -1) +
-                                                                    (DIM == 3 ? (ny + 1) * ((cc & 4) ? 0 : -1) : 0));
-        if (DIFF) recf[cc] = s.cell_rec[s.pad + lower];
-        else rec4[cc] = *reinterpret_cast<const double4*>(s.cell_rec + 4 * (size_t)(s.pad + lower));
+        const int ci = i + ((cc & 1) ? 0 : -1);
+        const int cj = j + ((cc & 2) ? 0 : -1);
+        const int ck = DIM == 3 ? k + ((cc & 4) ? 0 : -1) : 0;
+        inside[cc] = !(ci < 0 || ci >= nx || cj < 0 || cj >= ny || (DIM == 3 && (ck < 0 || ck >= nz)));
+        const int cell = inside[cc] ? ci + nx * (cj + ny * ck) : 0;
+        if (DIFF) recf[cc] = s.cell_rec[cell];
+        else rec4[cc] = *reinterpret_cast<const double4*>(s.cell_rec + 4 * (size_t)cell);
     }
 
     // u at the 27 neighbours.  Where the registers allow (FP32 K_e, or diffusion) they are
@@ -429,14 +431,15 @@ __device__ inline bool stencil_row(const DevView& s, const double (*sc)[3], int 
     // outside the stencil reads the row's own node).  The mask selects the use.
     // (node indices are int32: the lattice has fewer than 2^31 nodes)
     constexpr bool kEarlyU = sizeof(KT) == 4 || DIFF;
-    const double* ub = s.u_node + (size_t)BE * s.u_stride + s.pad;
+    const int n_nodes = (nx + 1) * (ny + 1) * (DIM == 3 ? nz + 1 : 1);
+    const double* ub = s.u_node + (size_t)BE * n_nodes;
     double un[NS27];
     auto load_u = [&](uint32_t m) {
 #pragma unroll
         for (int q = 0; q < NS27; ++q) {
             const int dx = q % 3 - 1, dy = (q / 3) % 3 - 1, dz = DIM == 3 ? q / 9 - 1 : 0;
             const int nb = node + dx + (nx + 1) * (dy + (ny + 1) * dz);
-            const int at = kEarlyU ? nb : ((m & (1u << q)) ? nb : node);
+            const int at = kEarlyU ? min(max(nb, 0), n_nodes - 1) : ((m & (1u << q)) ? nb : node);
             un[q] = ub[at];
         }
     };
@@ -449,11 +452,12 @@ __device__ inline bool stencil_row(const DevView& s, const double (*sc)[3], int 
         const int cx = (cc & 1) ? 0 : -1;
         const int cy = (cc & 2) ? 0 : -1;
         const int cz = DIM == 3 ? ((cc & 4) ? 0 : -1) : 0;
-        const double flag = DIFF ? recf[cc] : rec4[cc].x;
+        const double flag = !inside[cc] ? 0.0 : (DIFF ? recf[cc] : rec4[cc].x);
         if (flag == 0.0) continue;
 #pragma unroll
         for (int b = 0; b < NB; ++b) {
-                        const int q = (cx + vertex_off<DIM>(b, 0) + 1) + 3 * (cy + vertex_off<DIM>(b, 1) + 1) + (DIM == 3 ? 9 * (cz + vertex_off<DIM>(b, 2) + 1) : 0);
+            const int* off = DIM == 3 ? kHex8Off[b] : kQuad4Off[b];
+            const int q = (cx + off[0] + 1) + 3 * (cy + off[1] + 1) + (DIM == 3 ? 9 * (cz + off[2] + 1) : 0);
             mask |= 1u << q;
         }
     }
@@ -500,7 +504,7 @@ __device__ inline bool stencil_row(const DevView& s, const double (*sc)[3], int 
         constexpr int a_tab[8] = {6, 7, 5, 4, 2, 3, 1, 0};   // vertex with offset (-cx,-cy,-cz)
         constexpr int a_tab2[4] = {2, 3, 1, 0};
         const int a_loc = DIM == 3 ? a_tab[cc] : a_tab2[cc];
-        const double flag = DIFF ? recf[cc] : rec4[cc].x;
+        const double flag = !inside[cc] ? 0.0 : (DIFF ? recf[cc] : rec4[cc].x);
         if (flag == 0.0) continue;
         double c0 = 0.0, c1 = 0.0, cs = 0.0;
         if (!DIFF) {
@@ -513,7 +517,8 @@ __device__ inline bool stencil_row(const DevView& s, const double (*sc)[3], int 
             ke_seg(cc, flag, kv);
 #pragma unroll
             for (int b = 0; b < NB; ++b) {
-                                const int q = (cx + vertex_off<DIM>(b, 0) + 1) + 3 * (cy + vertex_off<DIM>(b, 1) + 1) + (DIM == 3 ? 9 * (cz + vertex_off<DIM>(b, 2) + 1) : 0);
+                const int* off = DIM == 3 ? kHex8Off[b] : kQuad4Off[b];
+                const int q = (cx + off[0] + 1) + 3 * (cy + off[1] + 1) + (DIM == 3 ? 9 * (cz + off[2] + 1) : 0);
                 acc[q] += kv[b];
             }
         } else if (DIFF) {
@@ -523,7 +528,8 @@ __device__ inline bool stencil_row(const DevView& s, const double (*sc)[3], int 
             for (int ii = 0; ii < DIM; ++ii) ks[ii] = flag * sc[ii][ii];
 #pragma unroll
             for (int b = 0; b < NB; ++b) {
-                                const int q = (cx + vertex_off<DIM>(b, 0) + 1) + 3 * (cy + vertex_off<DIM>(b, 1) + 1) + (DIM == 3 ? 9 * (cz + vertex_off<DIM>(b, 2) + 1) : 0);
+                const int* off = DIM == 3 ? kHex8Off[b] : kQuad4Off[b];
+                const int q = (cx + off[0] + 1) + 3 * (cy + off[1] + 1) + (DIM == 3 ? 9 * (cz + off[2] + 1) : 0);
                 double v = 0.0;
 #pragma unroll
                 for (int ii = 0; ii < DIM; ++ii) v += ks[ii] * kRefGrad<DIM>.v[ii][a_loc][ii][b];
@@ -540,7 +546,8 @@ __device__ inline bool stencil_row(const DevView& s, const double (*sc)[3], int 
             for (int ii = 0; ii < DIM; ++ii) css[ii] = cs * (AL == BE ? sc[ii][ii] : sc[BE][AL]);
 #pragma unroll
             for (int b = 0; b < NB; ++b) {
-                                const int q = (cx + vertex_off<DIM>(b, 0) + 1) + 3 * (cy + vertex_off<DIM>(b, 1) + 1) + (DIM == 3 ? 9 * (cz + vertex_off<DIM>(b, 2) + 1) : 0);
+                const int* off = DIM == 3 ? kHex8Off[b] : kQuad4Off[b];
+                const int q = (cx + off[0] + 1) + 3 * (cy + off[1] + 1) + (DIM == 3 ? 9 * (cz + off[2] + 1) : 0);
                 double v = cab * kRefGrad<DIM>.v[AL][a_loc][BE][b];
                 if (AL == BE) {
 #pragma unroll
@@ -789,9 +796,10 @@ mgpu_status launch_assemble(mgpu_set* S, const double* u, double* vals, double* 
     {
         const int64_t n_nodes = (int64_t)(S->v.nx + 1) * (S->v.ny + 1) * (dim == 3 ? S->v.nz + 1 : 1);
         dim3 gp((unsigned)((n_nodes + 255) / 256)), bp(256);
-        if (nf == 1) hipLaunchKernelGGL((k_prep<1>), gp, bp, 0, S->stream, S->v, u, n_nodes);
-        else if (nf == 2) hipLaunchKernelGGL((k_prep<2>), gp, bp, 0, S->stream, S->v, u, n_nodes);
-        else hipLaunchKernelGGL((k_prep<3>), gp, bp, 0, S->stream, S->v, u, n_nodes);
+        if (dim == 3 && nf == 1) hipLaunchKernelGGL((k_prep<3, 1>), gp, bp, 0, S->stream, S->v, u);
+        else if (dim == 3 && nf == 3) hipLaunchKernelGGL((k_prep<3, 3>), gp, bp, 0, S->stream, S->v, u);
+        else if (dim == 2 && nf == 1) hipLaunchKernelGGL((k_prep<2, 1>), gp, bp, 0, S->stream, S->v, u);
+        else hipLaunchKernelGGL((k_prep<2, 2>), gp, bp, 0, S->stream, S->v, u);
         HIP_TRY(hipGetLastError());
         if (ev) HIP_TRY(hipEventRecord(ev[4], S->stream));
         const int var = S->stencil_variant < 0 ? 1 : S->stencil_variant;
@@ -839,20 +847,13 @@ mgpu_status lq_prepare(mgpu_set* S) {
     lv.w_fit = static_cast<float*>(p);
     if ((rc = alloc((size_t)nc * sizeof(int32_t), &p)) != MGPU_OK) return rc;
     lv.flag = static_cast<int32_t*>(p);
-    // work list of the moment kernel over each cut element's sub-tets other than the regular-
-    // subdivision tets (whose moments are the constants reg_mom, added by the element's slot):
-    // each cut element takes ceil(sub-tets / 16) 16-lane quarters of a wavefront (at most 4; a
-    // 4-quarter element loops); 4-quarter elements go alone, 3-quarter ones beside a 1-quarter
-    // one, 2-quarter ones in pairs, 1-quarter ones in fours
+    // work list of the moment kernel: each cut element takes ceil(sub-tets / 16) 16-lane quarters
+    // of a wavefront (at most 4; a 4-quarter element loops); 4-quarter elements go alone,
+    // 3-quarter ones beside a 1-quarter one, 2-quarter ones in pairs, 1-quarter ones in fours
     {
-        if ((rc = upload(S, S->cut_gen.data(), S->cut_gen.size(), const_cast<int32_t**>(&lv.mom_gen))) != MGPU_OK)
-            return rc;
-        std::vector<double> reg(24 * 24);
-        regular_subtet_moments(reg.data());
-        if ((rc = upload(S, reg.data(), reg.size(), const_cast<double**>(&lv.reg_mom))) != MGPU_OK) return rc;
         std::vector<int32_t> byq[5];
         for (int32_t c = 0; c < nc; ++c) {
-            const int nsub = S->cut_gen_ptr[(size_t)c + 1] - S->cut_gen_ptr[(size_t)c];
+            const int nsub = S->cut_nip[c] / kTet11Points;
             byq[std::min(4, std::max(1, (nsub + 15) / 16))].push_back(c);
         }
         std::vector<int32_t> owner;
@@ -873,18 +874,17 @@ mgpu_status lq_prepare(mgpu_set* S) {
             const int32_t a = take1(), b = take1(), c = take1(), d = take1();
             owner.insert(owner.end(), {a, b, c, d});
         }
-        // records of 16: the 4 owners, then each quarter's first entry in mom_gen and its count,
-        // then the owners' regular-tet masks
+        // records of 12: the 4 owners, then each quarter's first sub-cell and sub-cell count (so
+        // that the kernel's first dependent load is the sub-cell coordinates)
         lv.n_mom_waves = (int32_t)(owner.size() / 4);
-        std::vector<int32_t> rec((size_t)lv.n_mom_waves * 16, 0);
+        std::vector<int32_t> rec((size_t)lv.n_mom_waves * 12, 0);
         for (int32_t w = 0; w < lv.n_mom_waves; ++w)
             for (int k = 0; k < 4; ++k) {
                 const int32_t c = owner[(size_t)w * 4 + k];
-                rec[(size_t)w * 16 + k] = c;
+                rec[(size_t)w * 12 + k] = c;
                 if (c >= 0) {
-                    rec[(size_t)w * 16 + 4 + k] = S->cut_gen_ptr[(size_t)c];
-                    rec[(size_t)w * 16 + 8 + k] = S->cut_gen_ptr[(size_t)c + 1] - S->cut_gen_ptr[(size_t)c];
-                    rec[(size_t)w * 16 + 12 + k] = (int32_t)S->cut_regmask[(size_t)c];
+                    rec[(size_t)w * 12 + 4 + k] = S->cut_ig0[(size_t)c];
+                    rec[(size_t)w * 12 + 8 + k] = S->cut_nip[(size_t)c] / kTet11Points;
                 }
             }
         if ((rc = upload(S, rec.data(), rec.size(), const_cast<int32_t**>(&lv.mom_owner))) != MGPU_OK) return rc;
@@ -997,20 +997,6 @@ extern "C" mgpu_status mgpu_set_create(const mgpu_set_desc* desc, mgpu_set** out
         S->vphi_cut.resize(h.cut_elems.size() * 8);
         for (size_t c = 0; c < h.cut_elems.size(); ++c)
             for (int k = 0; k < 8; ++k) S->vphi_cut[c * 8 + k] = desc->vertex_phi[(size_t)h.cut_elems[c] * 8 + k];
-        const size_t nc = h.cut_elems.size();
-        S->cut_regmask.assign(nc, 0u);
-        S->cut_gen_ptr.assign(nc + 1, 0);
-        S->cut_gen.clear();
-        S->cut_gen.reserve(n_ig);
-        for (size_t c = 0; c < nc; ++c) {
-            const int e = h.cut_elems[c];
-            for (int32_t t = desc->ig_ptr[e]; t < desc->ig_ptr[e + 1]; ++t) {
-                const int r = regular_subtet_index(desc->ig_xi + (size_t)t * 12);
-                if (r >= 0 && !(S->cut_regmask[c] & (1u << r))) S->cut_regmask[c] |= 1u << r;
-                else S->cut_gen.push_back(t);
-            }
-            S->cut_gen_ptr[c + 1] = (int32_t)S->cut_gen.size();
-        }
     }
     if (v.n_cut > 0) {
         void* p = nullptr;
@@ -1044,26 +1030,12 @@ extern "C" mgpu_status mgpu_set_create(const mgpu_set_desc* desc, mgpu_set** out
         HIP_TRY(hipMemset(p, 0, sizeof(int32_t)));
         v.err_flag = static_cast<int32_t*>(p);
         const size_t n_cells = (size_t)v.nx * v.ny * v.nz;
-        v.pad = h.dim == 3 ? (v.nx + 1) * (v.ny + 1) + (v.nx + 1) + 1 : (v.nx + 1) + 1;
-        v.u_stride = (int64_t)h.n_nodes + 2 * (int64_t)v.pad;
-        const size_t rs = h.nf == 1 ? 1 : 4;
-        HIP_TRY(hipMalloc(&p, (size_t)v.u_stride * rs * sizeof(double)));
+        HIP_TRY(hipMalloc(&p, std::max<size_t>(n_cells, 1) * (h.nf == 1 ? 1 : 4) * sizeof(double)));
         S->allocs.push_back(p);
         v.cell_rec = static_cast<double*>(p);
-        HIP_TRY(hipMemset(p, 0, (size_t)v.u_stride * rs * sizeof(double)));
-        HIP_TRY(hipMalloc(&p, (size_t)v.u_stride * h.nf * sizeof(double)));
+        HIP_TRY(hipMalloc(&p, std::max<size_t>((size_t)h.n_nodes, 1) * h.nf * sizeof(double)));
         S->allocs.push_back(p);
         v.u_node = static_cast<double*>(p);
-        HIP_TRY(hipMemset(p, 0, (size_t)v.u_stride * h.nf * sizeof(double)));
-        if (n_cells > 0) {
-            dim3 g((unsigned)((n_cells + 255) / 256)), b(256);
-            if (h.dim == 3 && h.nf == 1) hipLaunchKernelGGL((k_records<3, 1>), g, b, 0, S->stream, v);
-            else if (h.dim == 3) hipLaunchKernelGGL((k_records<3, 3>), g, b, 0, S->stream, v);
-            else if (h.nf == 1) hipLaunchKernelGGL((k_records<2, 1>), g, b, 0, S->stream, v);
-            else hipLaunchKernelGGL((k_records<2, 2>), g, b, 0, S->stream, v);
-            HIP_TRY(hipGetLastError());
-            HIP_TRY(hipStreamSynchronize(S->stream));
-        }
     }
     *out = S.release();
     return MGPU_OK;

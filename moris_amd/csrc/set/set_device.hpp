@@ -28,17 +28,9 @@ struct DevView {
     double* ke_cut;             // [n_cut][na][na]: rows (vertex, field) node-major, columns (field, vertex)
     float* ke_cut_f;            // the same in FP32 (mixed precision: FP32 integrands)
     int32_t* err_flag;          // [1]
-    // Lattice records, indexed by lattice node with `pad` guard entries before and after, so that
-    // every cell and node a stencil row touches has an index (cells and nodes beyond the lattice
-    // read zero records / are masked):
-    int32_t pad;                // (nx+1)(ny+1) + (nx+1) + 1 in 3D, (nx+1) + 1 in 2D
-    int64_t u_stride;           // n_nodes + 2 pad
-    double* cell_rec;           // per cell at [pad + its lower-corner node], static per set:
-                                // diffusion [1] {k | -(cut+1) | 0 void}, elasticity [4] {flag,
-                                // D_nn, D_nn', D_shear}; zero at every node that is no cell's
-                                // lower corner (void)
-    double* u_node;             // per call: u of each node's adofs, planar [nf][u_stride] at
-                                // [f][pad + node] (0 where the node has no adof, and in the guards)
+    double* cell_rec;           // per lattice cell: diffusion [1] {k | -(cut+1) | 0 void},
+                                // elasticity [4] {flag, D_nn, D_nn', D_shear}
+    double* u_node;             // per lattice node: u of its adofs, planar [nf][n_nodes] (0 if unused)
     const int32_t* skip;        // [n_cut] nonzero: element integrated elsewhere (learned), or NULL
     const int32_t* cut_order;   // [n_cut] cut element processed by block b (longest first), or NULL
 };
@@ -56,15 +48,6 @@ __host__ __device__ constexpr int lagrange_vertex_sign(int dim, int a, int d) {
     // HEX8: 0 (-,-,-) 1 (+,-,-) 2 (+,+,-) 3 (-,+,-) 4..7 the same at z = +1; QUAD4: 0..3 as z = -1
     return d == 0 ? ((a % 4 == 1 || a % 4 == 2) ? 1 : -1)
                   : d == 1 ? ((a % 4 == 2 || a % 4 == 3) ? 1 : -1) : (dim == 3 && a >= 4 ? 1 : -1);
-}
-
-// Lattice offset (0 or 1) along axis d of vertex b of the HEX8 / QUAD4 IP cell: the values of
-// kHex8Off / kQuad4Off as a constant expression, so that in unrolled loops the indices they make
-// are compile-time constants (register arrays stay in registers; a table in constant memory is
-// read at run time and pushes such arrays to scratch).
-template <int DIM>
-__host__ __device__ constexpr int vertex_off(int b, int d) {
-    return (lagrange_vertex_sign(DIM, b, d) + 1) / 2;
 }
 
 template <int DIM>

@@ -6,7 +6,6 @@
 //   CSR pattern        Sparse_Matrix_EpetraFECrs::build_graph, cl_Sparse_Matrix_EpetraFECrs.cpp:195-240
 //                      (union over elements of topology x topology), rows/cols ascending
 #include <algorithm>
-#include <array>
 #include <cmath>
 #include <atomic>
 #include <cstdlib>
@@ -481,80 +480,4 @@ mgpu_status build_set_host(const mgpu_set_desc& d, SetHost& h, std::string* err)
     return MGPU_OK;
 }
 
-// ------------------------------------------------------------------------------------------------
-// Regular-subdivision sub-tets (see set_internal.hpp)
-// ------------------------------------------------------------------------------------------------
-namespace {
-// subdivision vertex of a parametric point: 0-7 corners, 8-13 face centres, 14 the centre, -1 none
-int subdivision_vertex(const double* x) {
-    for (int v = 0; v < 8; ++v)
-        if (x[0] == kHex8XiHat[v][0] && x[1] == kHex8XiHat[v][1] && x[2] == kHex8XiHat[v][2]) return v;
-    for (int f = 0; f < 6; ++f)
-        if (x[0] == kHexFaceXi[f][0] && x[1] == kHexFaceXi[f][1] && x[2] == kHexFaceXi[f][2]) return 8 + f;
-    if (x[0] == 0.0 && x[1] == 0.0 && x[2] == 0.0) return 14;
-    return -1;
-}
-
-double subtet_det6(const double* X) {
-    double J[3][3];
-    const int rr[3] = {0, 1, 3};
-    for (int r = 0; r < 3; ++r)
-        for (int d = 0; d < 3; ++d) J[r][d] = X[rr[r] * 3 + d] - X[6 + d];
-    return (J[0][0] * (J[1][1] * J[2][2] - J[2][1] * J[1][2]) - J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
-            J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0])) / 6.0;
-}
-}  // namespace
-
-int regular_subtet_index(const double* x12) {
-    for (int q = 0; q < 12; ++q)   // an intersection vertex has a coordinate off {-1, 0, 1}
-        if (x12[q] != 0.0 && x12[q] != 1.0 && x12[q] != -1.0) return -1;
-    // vertex sets of the 24 tets as bit masks over the 15 subdivision vertices
-    static const auto keys = [] {
-        std::array<uint32_t, 24> k{};
-        for (int t = 0; t < 24; ++t)
-            for (int v = 0; v < 4; ++v) k[t] |= 1u << kHex24Tets[t][v];
-        return k;
-    }();
-    uint32_t key = 0;
-    for (int v = 0; v < 4; ++v) {
-        const int id = subdivision_vertex(x12 + 3 * v);
-        if (id < 0) return -1;
-        key |= 1u << id;
-    }
-    for (int t = 0; t < 24; ++t)
-        if (keys[t] == key) return subtet_det6(x12) > 0.0 ? t : -1;
-    return -1;
-}
-
-void regular_subtet_moments(double* m) {
-    for (int t = 0; t < 24; ++t) {
-        double X[12];
-        for (int v = 0; v < 4; ++v) {
-            const int sv = kHex24Tets[t][v];
-            for (int d = 0; d < 3; ++d)
-                X[3 * v + d] = sv < 8 ? kHex8XiHat[sv][d] : (sv < 14 ? kHexFaceXi[sv - 8][d] : 0.0);
-        }
-        if (subtet_det6(X) < 0.0) std::swap_ranges(X, X + 3, X + 3);   // vertices 0 <-> 1
-        const double detJ = subtet_det6(X);
-        double acc[24] = {0.0};
-        for (int q = 0; q < kTet11Points; ++q) {
-            double xip[3];
-            for (int d = 0; d < 3; ++d)
-                xip[d] = X[6 + d] + kTet11Xi[q][0] * (X[d] - X[6 + d]) + kTet11Xi[q][1] * (X[3 + d] - X[6 + d]) +
-                         kTet11Xi[q][2] * (X[9 + d] - X[6 + d]);
-            const double wq = kTet11W[q] * detJ;
-            const double xv[3] = {1.0, xip[0], xip[0] * xip[0]};
-            const double yv[3] = {1.0, xip[1], xip[1] * xip[1]};
-            const double zv[3] = {1.0, xip[2], xip[2] * xip[2]};
-            int k = 0;
-            for (int cc = 0; cc <= 2; ++cc)
-                for (int bb = 0; bb <= 2; ++bb)
-                    for (int aa = 0; aa <= 2; ++aa)
-                        if (aa + bb + cc <= 4) acc[k++] += wq * xv[aa] * yv[bb] * zv[cc];
-        }
-        for (int k = 0; k < 24; ++k) m[24 * t + k] = acc[k];
-    }
-}
-
 }  // namespace mgpu
-

@@ -75,14 +75,4 @@ struct SetHost {
 // code with *err set.
 mgpu_status build_set_host(const mgpu_set_desc& d, SetHost& h, std::string* err);
 
-// Sub-tets of a HEX8 cut cell that the interface left untouched are tets of the regular
-// subdivision (cl_XTK_Regular_Subdivision_Interface.hpp:110-360): their four vertices are
-// subdivision vertices (corners, face centres, centre).  Index 0..23 of the regular tet whose
-// vertex set the sub-tet x12 ([4][3] parametric coordinates) has, if positively oriented (the
-// orientation test of the moment kernel: rows v0 - v2, v1 - v2, v3 - v2), else -1.
-int regular_subtet_index(const double* x12);
-// The 23 learned-quadrature moments (TET_11, w* = w detJ) of each of the 24 regular tets taken
-// positively oriented, in the moment kernel's monomial order, [24][24] (zero-padded).
-void regular_subtet_moments(double* m);
-
 }  // namespace mgpu

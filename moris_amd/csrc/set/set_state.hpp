@@ -40,10 +40,6 @@ struct mgpu_set {
     const int32_t* d_cut_order = nullptr;   // device permutation (built on first use)
     // learned cut-cell quadrature
     std::vector<double> vphi_cut;      // host copy of the cut elements' nodal level sets
-    // per cut element: the regular-subdivision tets among its sub-tets (bit t: tet t), and its
-    // other sub-tets (indices into ig_xi, in order) -- the moment kernel integrates only these
-    std::vector<uint32_t> cut_regmask;
-    std::vector<int32_t> cut_gen_ptr, cut_gen;
     mgpu::LqView lv{};
     const mgpu_mlp* mlp = nullptr;
     bool lq_ready = false;

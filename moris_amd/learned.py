@@ -139,6 +139,18 @@ class MLP:
             pass
 
 
+def gauss_prior():
+    """A network whose output is the HEX_3x3x3 Gauss weights / 8 for every input (all weights zero,
+    last bias the Gauss weights): the moment fit then starts from the volume-scaled Gauss rule.
+    Stands in for the trained network to show what the network changes."""
+    _, _, g = operators()
+    net = MLP(seed=1)
+    p = np.zeros(net.n_params, np.float32)
+    p[-N_POINTS:] = (g / 8.0).astype(np.float32)
+    net.params = p
+    return net
+
+
 def attach(set_, mlp, tol=1e-6, iters=0, kappa=4.0):
     """Attach a network to a Set (keeps a reference so it outlives the set's calls).  Elements
     stay learned when their FP32 weights reproduce the moments to tol and sum |w| <= kappa * volume."""

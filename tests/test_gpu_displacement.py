@@ -7,12 +7,13 @@ oracle, from the plain FP64 GPU path and from the mixed-precision (+ learned qua
 GPU path.
 
 The error is measured in the L2 norm over the material domain with lumped-mass weights (each
-element's material volume split equally among its nodes): ||du||_L2 / ||u||_L2.  Adofs supported
-only by sliver cut cells carry almost no stiffness (no ghost stabilisation in a bulk-only set), so
-their displacements are large and ill-determined; the Euclidean norm over adofs is dominated by
-them and is printed beside the L2 norm but not held to the bar.  Bars: the FP64 GPU path's K
-within 1e-12 of the oracle's and its displacement within 1e-6 (round-off amplified by the
-sliver-cell conditioning); mixed and mixed + learned K within 1e-6 and displacement within 1e-5.
+element's material volume split equally among its nodes), ||du||_L2 / ||u||_L2, and in the
+Euclidean norm over all adofs, ||du|| / ||u||.  Adofs supported only by sliver cut cells carry
+almost no stiffness (no ghost stabilisation in a bulk-only set), so their displacements are large
+and ill-determined and dominate the Euclidean norm; FP64 round-off in K alone moves it by ~5e-6 on
+the 3D case.  Bars: the FP64 GPU path's K within 1e-12 of the oracle's, its displacement within
+1e-6 (L2) and 1e-5 (Euclidean); mixed and mixed + learned (the committed network) K within 1e-6,
+displacement within 1e-5 in both norms.
 """
 import numpy as np
 import pytest
@@ -22,6 +23,7 @@ import scipy.sparse.linalg as spla
 import moris_amd as M
 from helpers import adof_coords, spec_circle2
 from moris_amd import learned as L
+from moris_amd import train_lq
 from moris_amd import workloads as W
 from oracle import py as O
 
@@ -108,14 +110,14 @@ def test_displacement_error_2d_plane_stress_circle_hole():
         "mixed": lambda S: S.options(M.PREC_MIXED, M.QUAD_SUBDIVISION),
     })
     print("2D displacement errors", e)
-    assert e["fp64"]["K"] <= 1e-12 and e["fp64"]["L2"] <= 1e-6
-    assert e["mixed"]["K"] <= 1e-6 and e["mixed"]["L2"] <= 1e-5
+    assert e["fp64"]["K"] <= 1e-12 and e["fp64"]["L2"] <= 1e-6 and e["fp64"]["euclid"] <= 1e-5
+    assert e["mixed"]["K"] <= 1e-6 and e["mixed"]["L2"] <= 1e-5 and e["mixed"]["euclid"] <= 1e-5
 
 
 def test_displacement_error_3d_sphere_array_learned():
     n = 16
     spec, keep = W.elasticity_spec(n, W.sphere_array(0, per_axis=2, radius=0.2))
-    net = L.MLP(seed=12)
+    net = train_lq.load() or L.MLP(seed=12)
 
     def learned(S):
         L.attach(S, net)
@@ -127,6 +129,6 @@ def test_displacement_error_3d_sphere_array_learned():
         "mixed+learned": learned,
     })
     print("3D displacement errors", e)
-    assert e["fp64"]["K"] <= 1e-12 and e["fp64"]["L2"] <= 1e-6
-    assert e["mixed"]["K"] <= 1e-6 and e["mixed"]["L2"] <= 1e-5
-    assert e["mixed+learned"]["K"] <= 1e-6 and e["mixed+learned"]["L2"] <= 1e-5
+    assert e["fp64"]["K"] <= 1e-12 and e["fp64"]["L2"] <= 1e-6 and e["fp64"]["euclid"] <= 1e-5
+    for mode in ("mixed", "mixed+learned"):
+        assert e[mode]["K"] <= 1e-6 and e[mode]["L2"] <= 1e-5 and e[mode]["euclid"] <= 1e-5, mode

@@ -6,9 +6,13 @@
 * training with the project's kernels lowers the loss;
 * learned assembly and the FP32 (mixed) subdivision assembly against the FP64 oracle at the
   1e-6 bar of the mixed path;
-* what the trained network gains: the fit keeps the 23 moments of degree <= 4 exact from any prior;
-  the network prior also carries part of the degree-5/6 tri-quadratic moments, which the
-  volume-scaled Gauss prior does not (median error at least 25 % lower).
+* what the network changes, stated as measured: the fit sets the 23 moments of degree <= 4 from
+  any prior, so K (trilinear elements) does not depend on the network.  With the volume-scaled
+  Gauss prior in its place K and R agree with the trained network's to the mixed path's bar and
+  the fraction of cut elements kept on the learned rule is the same within 2 points (the bench's
+  Gauss-prior leg shows the same on the 64^3 workload: 94.8 % against 94.6 %, equal time).  The
+  network does learn its training objective: on held-out cells its weights are closer to the
+  exact 27-point rule than the Gauss prior's (a check of the training kernels, not a gain in K).
 """
 import numpy as np
 import pytest
@@ -132,34 +136,42 @@ def test_learned_fallback_goes_through_subdivision():
     assert rel(vals, vo) <= 1e-6 and rel(R, Ro) <= 1e-6
 
 
-def test_trained_network_gain_on_high_moments():
-    net = train_lq.load()   # the checkpoint trained on 1M cells by moris_amd.train_lq
+def test_trained_network_learns_its_objective_on_held_out_cells():
+    """The committed checkpoint (1M cells) against the volume-scaled Gauss prior on held-out cells:
+    mean squared error of the predicted 27 weights against the exact 27-point rule of the cell
+    (the training target).  Measured 1.7e-5 against 1.6e-4."""
+    net = train_lq.load()
     if net is None:
         x, y, _ = L.dataset(200000, SEED + 30)
         net = L.MLP(seed=8)
         net.train(x, y, epochs=12, batch=1024, lr=3e-3)
-    xt, yt, mt = L.dataset(3000, SEED + 31, with_moments=True)
-    V, Pm, g = L.operators()
-    mask = L.MASK23
-    wp = net.forward(xt).astype(np.float64) * 8.0
-    # point coordinates in the library's order: the monomials xi, eta, zeta are rows 1, 3, 9 of V
-    pts = np.stack([V[1], V[3], V[9]], axis=1)
-    V27 = np.stack([[xi ** a * eta ** b * zeta ** c for c in range(3) for b in range(3) for a in range(3)]
-                    for (xi, eta, zeta) in pts], axis=1)
-    err_mlp, err_gauss = [], []
-    for i in range(xt.shape[0]):
-        m23 = mt[i, mask]
-        w_m, r_m = L.fit(m23, wp[i])
-        w_g, r_g = L.fit(m23, g * mt[i, 0] / 8.0)
-        # one FP64 projection step: residual at roundoff of the correction (acceptance tolerance 1e-7)
-        assert r_m < 1e-9 and r_g < 1e-9
-        hi = ~mask
-        scale = max(abs(mt[i, 0]), 1e-12)
-        err_mlp.append(np.abs(V27[hi] @ w_m - mt[i, hi]).max() / scale)
-        err_gauss.append(np.abs(V27[hi] @ w_g - mt[i, hi]).max() / scale)
-    em, eg = np.median(err_mlp), np.median(err_gauss)
-    print("median degree-5/6 moment error: network prior %.3e, Gauss prior %.3e" % (em, eg))
-    # measured with the 1M-cell checkpoint: 2.55e-3 against 4.15e-3 (the fit fixes the 23 moments
-    # of degree <= 4 from any prior; the network decides the remaining 4 directions of the rule)
-    assert em < 0.75 * eg
+    xt, yt, _ = L.dataset(3000, SEED + 31)
+    _, _, g = L.operators()
+    mse_net = float(np.mean((net.forward(xt).astype(np.float64) - yt) ** 2))
+    mse_gauss = float(np.mean((np.tile(g / 8.0, (xt.shape[0], 1)) - yt) ** 2))
+    print("held-out weight MSE: network %.3e, Gauss prior %.3e" % (mse_net, mse_gauss))
+    assert mse_net < 0.5 * mse_gauss
 
+
+def test_gauss_prior_in_place_of_the_network_changes_neither_K_nor_acceptance():
+    net = train_lq.load()
+    if net is None:
+        pytest.skip("no committed checkpoint")
+    spec, keep = M.make_spec(3, (24, 24, 24), iwg=M.IWG_STRUC_LINEAR_BULK, spheres=random_spheres(24, SEED + 40))
+    P = M.Problem(spec, keep)
+    S = M.Set(P.desc, keepalive=P)
+    u = np.random.default_rng(SEED + 41).uniform(-1.0, 1.0, S.n_rows)
+    S.options(M.PREC_FP64, M.QUAD_SUBDIVISION)
+    v64, R64 = S.compute_jacobian_and_residual(u)
+    out = {}
+    for name, m in (("network", net), ("gauss", L.gauss_prior())):
+        L.attach(S, m)
+        S.options(M.PREC_MIXED, M.QUAD_LEARNED)
+        vals, R = S.compute_jacobian_and_residual(u)
+        out[name] = (vals, R, float(L.result(S)[2].mean()))
+        assert rel(vals, v64) <= 1e-6 and rel(R, R64) <= 1e-6
+    acc_n, acc_g = out["network"][2], out["gauss"][2]
+    print("learned fraction of cut elements: network %.4f, Gauss prior %.4f; K network vs Gauss %.2e" % (
+        acc_n, acc_g, rel(out["network"][0], out["gauss"][0])))
+    assert abs(acc_n - acc_g) <= 0.02
+    assert rel(out["network"][0], out["gauss"][0]) <= 2e-6
