@@ -17,7 +17,7 @@
 //   k_cut          one thread per intersected cell: the 15 subdivision vertices (face nodes from the
 //                  lowest intersected cell sharing the face, as the host does), the cut into a slot
 //   k_elem_fill    element list in lattice order, IP boxes, material, vertex level set, sub-cell counts
-//   k_copy_sub     slots -> ig_xi
+//   k_copy_sub     slots -> ig_xi, one block per intersected cell
 //   k_basis_used, k_basis_adof   bases some element uses, numbered in ascending basis index
 //   k_trows        T rows and element topology
 //   k_rows         CSR row lengths, then columns (union of the topologies around each basis)
@@ -141,7 +141,7 @@ __global__ void k_compact_cut(Lat L, const int32_t* is_cut, const int32_t* cut_p
 // One thread per intersected cell: the cut into its slot (kMaxSubTets sub-tets of 12 doubles).
 __global__ void k_cut(Lat L, const double* phi, const int8_t* cls, const int32_t* cut_cell, int32_t n_int,
                       double* slots, int32_t* n_sub, int32_t* status) {
-    const int t = blockIdx.x * kT + threadIdx.x;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n_int) return;
     const int64_t c = cut_cell[t];
     int ijk[3];
@@ -248,15 +248,18 @@ __global__ void k_elem_fill(Lat L, const double* phi, const int8_t* cls, const i
     }
 }
 
-__global__ void k_copy_sub(int32_t n_el, int dim, const int32_t* elem_slot, const int32_t* ig_ptr, const double* slots,
-                           double* ig_xi) {
-    const int32_t e = blockIdx.x * kT + threadIdx.x;
-    if (e >= n_el || elem_slot[e] < 0) return;
+// One block per intersected cell that is an element: its sub-cells from the slot into ig_xi, the
+// block's lanes striding over the values (coalesced on both sides).
+__global__ void k_copy_sub(int dim, const int32_t* cut_cell, const int32_t* cell_elem, const int32_t* n_sub,
+                           const int32_t* ig_ptr, const double* slots, double* ig_xi) {
+    const int t = blockIdx.x;
+    const int32_t e = cell_elem[cut_cell[t]];
+    if (e < 0) return;
     const int per = (dim + 1) * dim;
-    const double* src = slots + (size_t)elem_slot[e] * kMaxSubTets * 12;
+    const double* src = slots + (size_t)t * kMaxSubTets * 12;
     double* dst = ig_xi + (size_t)ig_ptr[e] * per;
-    const int n = (ig_ptr[e + 1] - ig_ptr[e]) * per;
-    for (int q = 0; q < n; ++q) dst[q] = src[q];
+    const int n = n_sub[t] * per;
+    for (int q = threadIdx.x; q < n; q += blockDim.x) dst[q] = src[q];
 }
 
 // Basis lattice: p = 1 the lattice nodes (n + 1 per axis), p = 2 the quadratic B-splines (n + 2 per
@@ -524,7 +527,9 @@ extern "C" mgpu_status mgpu_setup_build(const mgpu_problem_spec* spec, void* str
     int32_t* n_sub;
     if ((rc = dalloc(S.get(), (size_t)n_int * kMaxSubTets * 12, &slots)) != MGPU_OK) return rc;
     if ((rc = dalloc(S.get(), (size_t)n_int, &n_sub)) != MGPU_OK) return rc;
-    hipLaunchKernelGGL(su::k_cut, dim3(blocks(n_int)), dim3(su::kT), 0, st, L, phi, cls, cut_cell, n_int, slots, n_sub,
+    // one thread per intersected cell in blocks of 64, so that every CU takes some of the cells
+    hipLaunchKernelGGL(su::k_cut, dim3((unsigned)std::max<int64_t>(1, (n_int + 63) / 64)), dim3(64), 0, st, L, phi, cls,
+                       cut_cell, n_int, slots, n_sub,
                        status);
     SU_TRY(hipGetLastError());
     // 3. elements in lattice order, their boxes, material and sub-cells
@@ -552,8 +557,9 @@ extern "C" mgpu_status mgpu_setup_build(const mgpu_problem_spec* spec, void* str
     S->n_ig = n_ig;
     const int per_sub = (L.dim + 1) * L.dim;
     if ((rc = dalloc(S.get(), (size_t)n_ig * per_sub, &S->ig_xi)) != MGPU_OK) return rc;
-    hipLaunchKernelGGL(su::k_copy_sub, dim3(blocks(n_el)), dim3(su::kT), 0, st, n_el, L.dim, elem_slot, S->ig_ptr, slots,
-                       S->ig_xi);
+    if (n_int > 0)
+        hipLaunchKernelGGL(su::k_copy_sub, dim3((unsigned)n_int), dim3(64), 0, st, L.dim, cut_cell, cell_elem, n_sub,
+                           S->ig_ptr, slots, S->ig_xi);
     SU_TRY(hipGetLastError());
     // 4. adof numbering of the used bases, T rows, topology
     const int64_t n_basis = (int64_t)L.nbs[0] * L.nbs[1] * L.nbs[2];
