@@ -5,10 +5,11 @@
  * Linear_Solver_Aztec::solve_linear_system (projects/SOL/DLA/src/cl_DLA_Linear_Solver_Aztec.cpp
  * :88-137: AztecOO::Iterate( AZ_max_iter, rel_residual ), Aztec's default convergence test
  * ||r_k|| <= rel_residual * ||r_0||, solver and preconditioner chosen by AZ_solver / AZ_precond,
- * :260-356; AZ_cg with AZ_Jacobi among its options).  These entry points solve symmetric positive
- * definite systems held in device memory on one GPU: conjugate gradients with a Jacobi
- * preconditioner, every reduction in a fixed order (bit-identical results run to run), and the
- * convergence test of Aztec's default.  Strong Dirichlet conditions, which the reference's weak
+ * :260-356; AZ_gmres (the default), AZ_cg and AZ_Jacobi among its options).  These entry points
+ * solve systems held in device memory on one GPU: conjugate gradients (symmetric positive definite
+ * K) and restarted GMRES (any non-singular K), both with a Jacobi preconditioner, every reduction
+ * in a fixed order (bit-identical results run to run), and the convergence test of Aztec's
+ * default.  Strong Dirichlet conditions, which the reference's weak
  * (Nitsche) formulation does not need, can be applied to the system first.  Same conventions as
  * moris_gpu.h; every pointer below is a DEVICE pointer unless said otherwise.
  */
@@ -53,6 +54,18 @@ mgpu_status mgpu_cg_solve(int32_t n_rows, const int32_t* row_ptr, const int32_t*
    ends; larger systems are refused with MGPU_ERR_INVALID_ARG.  All calls on one GPU share that
    array: calls that overlap in time on one GPU (several host threads or streams) race and must be
    serialised by the caller.  Results equal those of mgpu_cg_solve bit for bit.                */
+/* Restarted GMRES(restart) with a right Jacobi preconditioner for K x = b, K any non-singular CSR
+   matrix with sorted columns (the non-symmetric systems of UNSYMMETRIC Nitsche side sets): the
+   reference's default solver, AztecOO AZ_gmres with AZ_kspace = restart (cl_DLA_Linear_Solver_
+   Aztec.cpp:88-137, :260-356).  Classical Gram-Schmidt with one reorthogonalisation, Givens
+   rotations; stops when ||r_k|| <= rtol ||r_0|| (r_0 of the initial guess) or after max_iter
+   Arnoldi steps; restart in 1 .. 64.  x holds the initial guess on entry and the solution on
+   return; info as for CG (iterations = Arnoldi steps).  The library allocates (restart + 2)
+   n_rows doubles for the call; synchronous; bit-identical run to run.                           */
+mgpu_status mgpu_gmres_solve(int32_t n_rows, const int32_t* row_ptr, const int32_t* col_idx, const double* vals,
+                             const double* b, double* x, double rtol, int32_t restart, int32_t max_iter,
+                             void* stream, mgpu_cg_info* info);
+
 #define MGPU_CG_NOALLOC_MAX_ROWS 2500000
 mgpu_status mgpu_cg_solve_noalloc(int32_t n_rows, const int32_t* row_ptr, const int32_t* col_idx,
                                   const double* vals, const double* b, double* x, double rtol, int32_t max_iter,

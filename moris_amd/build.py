@@ -32,6 +32,7 @@ HIP_SOURCES = [
     "learned/lq_assemble.hip",
     "learned/mlp.hip",
     "solve/cg.hip",
+    "solve/gmres.hip",
     "setup/gpu_setup.hip",
 ]
 

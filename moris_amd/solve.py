@@ -2,7 +2,8 @@
 
 The reference solves K u = R after assembly with AztecOO (projects/SOL/DLA/src/
 cl_DLA_Linear_Solver_Aztec.cpp:88-137); here conjugate gradients with a Jacobi preconditioner run
-on the CSR values the assembly left in device memory, with Aztec's default convergence test
+on the CSR values the assembly left in device memory (restarted GMRES, the reference's default, for
+the non-symmetric systems of unsymmetric Nitsche side sets), with Aztec's default convergence test
 ||r_k|| <= rtol ||r_0||.  No CPU fallback: every call goes through libmoris_gpu.so.
 """
 import ctypes
@@ -27,6 +28,9 @@ def _lib():
         lib.mgpu_cg_solve.argtypes = [ctypes.c_int32, _vp, _vp, _vp, _vp, _vp, ctypes.c_double, ctypes.c_int32, _vp,
                                       ctypes.POINTER(CgInfo)]
         lib.mgpu_cg_solve.restype = ctypes.c_int32
+        lib.mgpu_gmres_solve.argtypes = [ctypes.c_int32, _vp, _vp, _vp, _vp, _vp, ctypes.c_double, ctypes.c_int32,
+                                         ctypes.c_int32, _vp, ctypes.POINTER(CgInfo)]
+        lib.mgpu_gmres_solve.restype = ctypes.c_int32
         lib._solve_bound = True
     return lib
 
@@ -66,3 +70,15 @@ def cg(A, b, x0=None, rtol=1e-10, max_iter=20000):
     _check(_lib().mgpu_cg_solve(A.n_rows, A.row_ptr.ptr, A.col_idx.ptr, A.vals.ptr, bb.ptr, x.ptr, rtol, max_iter,
                                 None, ctypes.byref(info)), "mgpu_cg_solve")
     return x.to_array(np.float64, A.n_rows), info
+
+
+def gmres(A, b, x0=None, rtol=1e-10, restart=30, max_iter=20000):
+    """Solve A x = b (any non-singular A) by restarted GMRES with a right Jacobi preconditioner on
+    the GPU; returns (x as a host array, CgInfo with iterations = Arnoldi steps)."""
+    bb = b if isinstance(b, DeviceBuffer) else DeviceBuffer.from_array(np.ascontiguousarray(b, np.float64))
+    x = DeviceBuffer.from_array(np.zeros(A.n_rows) if x0 is None else np.ascontiguousarray(x0, np.float64))
+    info = CgInfo()
+    _check(_lib().mgpu_gmres_solve(A.n_rows, A.row_ptr.ptr, A.col_idx.ptr, A.vals.ptr, bb.ptr, x.ptr, rtol, restart,
+                                   max_iter, None, ctypes.byref(info)), "mgpu_gmres_solve")
+    return x.to_array(np.float64, A.n_rows), info
+

@@ -5,7 +5,8 @@
   and Neumann, on the interface and on domain boundary faces (TRI3 sub-cell faces, QUAD4 hex faces,
   2D LINE2 sides), diffusion and elasticity, random u; a second call is bit-identical.
 * The reference's known answer (UT_MDL_XTK_DiffusionElement.cpp:91-340) assembled on the GPU:
-  T = 5 + 20 z at every node of the 1 x 1 x 4 lattice cut at z = 3.1.
+  T = 5 + 20 z at every node of the 1 x 1 x 4 lattice cut at z = 3.1, solved by a direct solver
+  and by GPU GMRES (the system is not symmetric).
 * C1 with the reference's weak boundary conditions: 64 x 64 QUAD4 plane stress around the circular
   hole, left edge clamped by SYMMETRIC Nitsche (alpha 100), unit traction (1, 0) on the right edge
   by Neumann; K u = f solved on the GPU (CG) for K of the oracle, of the FP64 GPU path and of the
@@ -91,6 +92,10 @@ def test_known_answer_xtk_cut_diffusion_5_plus_20z_on_gpu():
     T = spla.spsolve(K.tocsc(), -R0)
     z = np.repeat(np.arange(5.0), 4)
     assert np.abs(T - (5.0 + 20.0 * z)).max() < 1e-8, T
+    # the same system (non-symmetric: unsymmetric Nitsche) solved on the GPU by GMRES
+    Tg, info = SV.gmres(SV.DeviceCSR(rp, ci, vals), -R0, rtol=1e-14, restart=30)
+    assert info.converged == 1
+    assert np.abs(Tg - (5.0 + 20.0 * z)).max() < 1e-8, Tg
 
 
 def test_c1_weak_boundary_conditions_displacement():

@@ -27,6 +27,7 @@ import scipy.sparse.linalg as spla
 import moris_amd as M
 from helpers import SEED, adof_coords, rel
 from moris_amd import learned as L
+from moris_amd import sideset as SS
 from moris_amd import solve as SV
 from moris_amd import train_lq
 from moris_amd import workloads as W
@@ -112,6 +113,58 @@ def test_cg_reports_breakdown_on_indefinite_matrix():
     SV.apply_dirichlet(A, rb, fixed)
     with pytest.raises(M.MgpuError):
         SV.cg(A, rb, rtol=1e-10)
+
+
+def _unsymmetric_nitsche_system(n=10):
+    """3D diffusion around a sphere with UNSYMMETRIC Nitsche u = 1 on x = 0 (alpha 2) and a Neumann
+    flux on the interface: K is not symmetric."""
+    spec, keep = M.make_spec(3, (n, n, n), spheres=[[0.5, 0.5, 0.5, 0.3]], k_sigma=0.5, seed=SEED + 5)
+    P = M.Problem(spec, keep)
+    S = M.Set(P.desc, keepalive=P)
+    sides = []
+    for geom in (SS.GEOM_QUAD4, SS.GEOM_TRI3):
+        e, x, nrm = SS.facets(P, SS.SIDE_XMIN, geom)
+        if len(e):
+            sides.append(SS.SideSet(S, SS.IWG_SPATIALDIFF_DIRICHLET_UNSYMMETRIC_NITSCHE, geom, e, x, nrm, alpha=2.0,
+                                    value=[1.0]))
+    e, x, nrm = SS.facets(P, SS.SIDE_INTERFACE, SS.GEOM_TRI3)
+    sides.append(SS.SideSet(S, SS.IWG_SPATIALDIFF_NEUMANN, SS.GEOM_TRI3, e, x, nrm, value=[0.5]))
+    vals, R0 = SS.assemble(S, sides, np.zeros(S.n_rows))
+    rp, ci = S.pattern()
+    return P, S, sides, rp, ci, vals, -R0
+
+
+def test_gmres_solves_the_unsymmetric_nitsche_system():
+    P, S, sides, rp, ci, vals, b = _unsymmetric_nitsche_system()
+    K = sp.csr_matrix((vals, ci, rp), shape=(S.n_rows, S.n_rows))
+    assert abs(K - K.T).max() > 1e-3 * abs(K).max()
+    x_ref = spla.spsolve(K.tocsc(), b)
+    A = SV.DeviceCSR(rp, ci, vals)
+    x, info = SV.gmres(A, b, rtol=1e-12, restart=30)
+    print("gmres(30) steps", info.iterations, "rel residual %.2e true %.2e" % (info.rel_residual,
+                                                                           info.true_rel_residual))
+    assert info.converged == 1 and info.true_rel_residual <= 1e-11
+    assert rel(x, x_ref) <= 1e-9
+    x2, info2 = SV.gmres(A, b, rtol=1e-12, restart=30)
+    assert np.array_equal(x.view(np.uint64), x2.view(np.uint64)) and info2.iterations == info.iterations
+    x3, info3 = SV.gmres(A, b, rtol=1e-12, restart=30, max_iter=7)
+    assert info3.iterations == 7 and info3.converged == 0
+    # another restart length: other iterates, the same solution
+    x4, info4 = SV.gmres(A, b, rtol=1e-12, restart=8)
+    assert info4.converged == 1 and rel(x4, x_ref) <= 1e-9
+    # on a symmetric positive definite system GMRES and CG agree
+    Ssym = M.Set(P.desc, keepalive=P)
+    sym = []
+    for geom in (SS.GEOM_QUAD4, SS.GEOM_TRI3):
+        e, xf, nrm = SS.facets(P, SS.SIDE_XMIN, geom)
+        if len(e):
+            sym.append(SS.SideSet(Ssym, SS.IWG_SPATIALDIFF_DIRICHLET_SYMMETRIC_NITSCHE, geom, e, xf, nrm, alpha=2.0,
+                                  value=[1.0]))
+    vs, Rs = SS.assemble(Ssym, sym, np.zeros(Ssym.n_rows))
+    As = SV.DeviceCSR(rp, ci, vs)
+    xg, ig = SV.gmres(As, -Rs, rtol=1e-12)
+    xc, ic = SV.cg(As, -Rs, rtol=1e-12)
+    assert ig.converged == 1 and ic.converged == 1 and rel(xg, xc) <= 1e-9
 
 
 def _traction_problem(P, n):
