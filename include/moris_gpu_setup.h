@@ -48,6 +48,13 @@ mgpu_status mgpu_setup_copy(const mgpu_gpu_setup* s, int32_t* elem_cell, int32_t
                             double* material, double* ip_box, double* vertex_phi, int32_t* t_ptr, int32_t* t_adof,
                             double* t_weight, int32_t* topo, int32_t* row_ptr, int32_t* col_idx);
 mgpu_status mgpu_setup_desc(mgpu_gpu_setup* s, mgpu_set_desc* desc);
+/* A set made directly from the set-up's device arrays, with no host round trip: the same set as
+   mgpu_set_create(mgpu_setup_desc(...)) for bspline_order 1 (nodal T rows; others return
+   MGPU_ERR_UNSUPPORTED).  The set uses the set-up's device arrays in place, so the set-up must
+   outlive the set.  Its host copies of the pattern and topology (mgpu_pattern_get,
+   mgpu_element_topology, side sets) are fetched from the device on first use.  No counterpart in
+   the reference (its set-up is on the host); for design loops that re-mesh per design.        */
+mgpu_status mgpu_set_create_from_setup(mgpu_gpu_setup* s, mgpu_set** out);
 
 #ifdef __cplusplus
 }

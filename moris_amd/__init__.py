@@ -304,38 +304,52 @@ class DeviceBuffer:
 class Set:
     """fem::Set of the bulk blocks of a problem, on the GPU."""
 
-    def __init__(self, desc, keepalive=None):
+    def __init__(self, desc, keepalive=None, handle=None, n_elements=None, n_cut=None):
+        """A set from a descriptor (mgpu_set_create), or around a handle made elsewhere (the GPU
+        set-up's mgpu_set_create_from_setup), given its element and cut-element counts."""
         lib = load_library()
         self._keep = keepalive
         self.desc = desc
-        self._h = ctypes.c_void_p()
-        _check(lib.mgpu_set_create(ctypes.byref(desc), ctypes.byref(self._h)), "mgpu_set_create")
+        if handle is None:
+            self._h = ctypes.c_void_p()
+            _check(lib.mgpu_set_create(ctypes.byref(desc), ctypes.byref(self._h)), "mgpu_set_create")
+            self.n_elements = desc.n_elements
+            ne = desc.n_elements
+            ig = np.ctypeslib.as_array(desc.ig_ptr, shape=(ne + 1,)) if ne else np.zeros(1, np.int32)
+            self.n_cut = int(np.count_nonzero(np.diff(ig)))
+        else:
+            self._h = handle
+            self.n_elements = n_elements
+            self.n_cut = n_cut
         nr = ctypes.c_int32()
         nnz = ctypes.c_int64()
-        rp = _c_i32p()
-        ci = _c_i32p()
-        _check(lib.mgpu_pattern_get(self._h, ctypes.byref(nr), ctypes.byref(nnz), ctypes.byref(rp),
-                                    ctypes.byref(ci)), "mgpu_pattern_get")
+        _check(lib.mgpu_pattern_get(self._h, ctypes.byref(nr), ctypes.byref(nnz), None, None), "mgpu_pattern_get")
         self.n_rows = nr.value
         self.nnz = nnz.value
-        self._rp, self._ci = rp, ci
-        ne = desc.n_elements
-        ig = np.ctypeslib.as_array(desc.ig_ptr, shape=(ne + 1,)) if ne else np.zeros(1, np.int32)
-        self.n_cut = int(np.count_nonzero(np.diff(ig)))
+        self._rp = self._ci = None
+
+    def _pattern_ptrs(self):
+        if self._rp is None:
+            rp, ci = _c_i32p(), _c_i32p()
+            _check(load_library().mgpu_pattern_get(self._h, None, None, ctypes.byref(rp), ctypes.byref(ci)),
+                   "mgpu_pattern_get")
+            self._rp, self._ci = rp, ci
+        return self._rp, self._ci
 
     def options(self, precision=PREC_FP64, quadrature=QUAD_SUBDIVISION):
         _check(load_library().mgpu_set_options(self._h, precision, quadrature), "mgpu_set_options")
 
     def pattern(self):
-        rp = np.ctypeslib.as_array(self._rp, shape=(self.n_rows + 1,)).copy()
-        ci = np.ctypeslib.as_array(self._ci, shape=(max(self.nnz, 1),))[: self.nnz].copy()
+        prp, pci = self._pattern_ptrs()
+        rp = np.ctypeslib.as_array(prp, shape=(self.n_rows + 1,)).copy()
+        ci = np.ctypeslib.as_array(pci, shape=(max(self.nnz, 1),))[: self.nnz].copy()
         return rp, ci
 
     def topology(self):
         n = ctypes.c_int32()
         p = _c_i32p()
         _check(load_library().mgpu_element_topology(self._h, ctypes.byref(n), ctypes.byref(p)), "topology")
-        ne = self.desc.n_elements
+        ne = self.n_elements
         if ne == 0:
             return np.zeros((0, n.value), dtype=np.int32)
         return np.ctypeslib.as_array(p, shape=(ne * n.value,)).reshape(ne, n.value).copy()
@@ -345,7 +359,7 @@ class Set:
         off = ctypes.POINTER(ctypes.c_int64)()
         p = _c_i32p()
         _check(load_library().mgpu_element_topology_csr(self._h, ctypes.byref(off), ctypes.byref(p)), "topology_csr")
-        ne = self.desc.n_elements
+        ne = self.n_elements
         if ne == 0:
             return np.zeros(1, np.int64)
         return np.ctypeslib.as_array(off, shape=(ne + 1,)).copy()
@@ -355,7 +369,7 @@ class Set:
         off = ctypes.POINTER(ctypes.c_int64)()
         p = _c_i32p()
         _check(load_library().mgpu_element_topology_csr(self._h, ctypes.byref(off), ctypes.byref(p)), "topology_csr")
-        ne = self.desc.n_elements
+        ne = self.n_elements
         if ne == 0:
             return []
         o = np.ctypeslib.as_array(off, shape=(ne + 1,)).copy()

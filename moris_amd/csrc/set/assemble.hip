@@ -950,6 +950,57 @@ mgpu_status set_error(mgpu_status code, const std::string& msg) {
 
 extern "C" const char* mgpu_last_error_string(void) { return g_last_error.c_str(); }
 
+namespace mgpu {
+// The per-set buffers of a lattice set after its inputs are on the device: the cut elements'
+// matrices, their sub-cell counts and longest-first order (from the host copy of ig_ptr), the
+// error flag, the cell records and the planar u.
+mgpu_status finish_lattice_set(mgpu_set* S, const int32_t* ig_ptr) {
+    SetHost& h = S->h;
+    DevView& v = S->v;
+    mgpu_status rc;
+    if (v.n_cut > 0) {
+        void* p = nullptr;
+        HIP_TRY(hipMalloc(&p, (size_t)v.n_cut * h.na * h.na * sizeof(double)));
+        S->allocs.push_back(p);
+        v.ke_cut = static_cast<double*>(p);
+        HIP_TRY(hipMalloc(&p, (size_t)v.n_cut * h.na * h.na * sizeof(float)));
+        S->allocs.push_back(p);
+        v.ke_cut_f = static_cast<float*>(p);
+        S->cut_nip.resize(v.n_cut);
+        S->cut_ig0.resize(v.n_cut);
+        for (int32_t c = 0; c < v.n_cut; ++c) {
+            const int32_t ce = h.cut_elems[c];
+            S->cut_nip[c] = (ig_ptr[ce + 1] - ig_ptr[ce]) * (h.dim == 3 ? kTet11Points : kTri7Points);
+            S->cut_ig0[c] = ig_ptr[ce];
+        }
+        // longest-first processing order of the cut elements (sub-cell count descending, stable)
+        std::vector<int32_t> order(v.n_cut);
+        for (int32_t c = 0; c < v.n_cut; ++c) order[c] = c;
+        std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
+            const int32_t ex = h.cut_elems[x], ey = h.cut_elems[y];
+            return ig_ptr[ex + 1] - ig_ptr[ex] > ig_ptr[ey + 1] - ig_ptr[ey];
+        });
+        if ((rc = upload(S, order.data(), order.size(), const_cast<int32_t**>(&S->d_cut_order))) != MGPU_OK)
+            return rc;
+    }
+    {
+        void* p = nullptr;
+        HIP_TRY(hipMalloc(&p, sizeof(int32_t)));
+        S->allocs.push_back(p);
+        HIP_TRY(hipMemset(p, 0, sizeof(int32_t)));
+        v.err_flag = static_cast<int32_t*>(p);
+        const size_t n_cells = (size_t)v.nx * v.ny * v.nz;
+        HIP_TRY(hipMalloc(&p, std::max<size_t>(n_cells, 1) * (h.nf == 1 ? 1 : 4) * sizeof(double)));
+        S->allocs.push_back(p);
+        v.cell_rec = static_cast<double*>(p);
+        HIP_TRY(hipMalloc(&p, std::max<size_t>((size_t)h.n_nodes, 1) * h.nf * sizeof(double)));
+        S->allocs.push_back(p);
+        v.u_node = static_cast<double*>(p);
+    }
+    return MGPU_OK;
+}
+}  // namespace mgpu
+
 extern "C" mgpu_status mgpu_set_create(const mgpu_set_desc* desc, mgpu_set** out) {
     MGPU_GUARD_BEGIN
     if (!desc || !out) return set_error(MGPU_ERR_INVALID_ARG, "null argument");
@@ -998,45 +1049,7 @@ extern "C" mgpu_status mgpu_set_create(const mgpu_set_desc* desc, mgpu_set** out
         for (size_t c = 0; c < h.cut_elems.size(); ++c)
             for (int k = 0; k < 8; ++k) S->vphi_cut[c * 8 + k] = desc->vertex_phi[(size_t)h.cut_elems[c] * 8 + k];
     }
-    if (v.n_cut > 0) {
-        void* p = nullptr;
-        HIP_TRY(hipMalloc(&p, (size_t)v.n_cut * h.na * h.na * sizeof(double)));
-        S->allocs.push_back(p);
-        v.ke_cut = static_cast<double*>(p);
-        HIP_TRY(hipMalloc(&p, (size_t)v.n_cut * h.na * h.na * sizeof(float)));
-        S->allocs.push_back(p);
-        v.ke_cut_f = static_cast<float*>(p);
-        S->cut_nip.resize(v.n_cut);
-        S->cut_ig0.resize(v.n_cut);
-        for (int32_t c = 0; c < v.n_cut; ++c) {
-            const int32_t ce = h.cut_elems[c];
-            S->cut_nip[c] = (desc->ig_ptr[ce + 1] - desc->ig_ptr[ce]) * (h.dim == 3 ? kTet11Points : kTri7Points);
-            S->cut_ig0[c] = desc->ig_ptr[ce];
-        }
-        // longest-first processing order of the cut elements (sub-cell count descending, stable)
-        std::vector<int32_t> order(v.n_cut);
-        for (int32_t c = 0; c < v.n_cut; ++c) order[c] = c;
-        std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
-            const int32_t ex = h.cut_elems[x], ey = h.cut_elems[y];
-            return desc->ig_ptr[ex + 1] - desc->ig_ptr[ex] > desc->ig_ptr[ey + 1] - desc->ig_ptr[ey];
-        });
-        if ((rc = upload(S.get(), order.data(), order.size(), const_cast<int32_t**>(&S->d_cut_order))) != MGPU_OK)
-            return rc;
-    }
-    {
-        void* p = nullptr;
-        HIP_TRY(hipMalloc(&p, sizeof(int32_t)));
-        S->allocs.push_back(p);
-        HIP_TRY(hipMemset(p, 0, sizeof(int32_t)));
-        v.err_flag = static_cast<int32_t*>(p);
-        const size_t n_cells = (size_t)v.nx * v.ny * v.nz;
-        HIP_TRY(hipMalloc(&p, std::max<size_t>(n_cells, 1) * (h.nf == 1 ? 1 : 4) * sizeof(double)));
-        S->allocs.push_back(p);
-        v.cell_rec = static_cast<double*>(p);
-        HIP_TRY(hipMalloc(&p, std::max<size_t>((size_t)h.n_nodes, 1) * h.nf * sizeof(double)));
-        S->allocs.push_back(p);
-        v.u_node = static_cast<double*>(p);
-    }
+    if ((rc = finish_lattice_set(S.get(), desc->ig_ptr)) != MGPU_OK) return rc;
     *out = S.release();
     return MGPU_OK;
     MGPU_GUARD_END
@@ -1067,8 +1080,10 @@ extern "C" mgpu_status mgpu_set_stream(mgpu_set* set, void* stream) {
 extern "C" mgpu_status mgpu_pattern_get(const mgpu_set* set, int32_t* n_rows, int64_t* nnz,
                                         const int32_t** row_ptr, const int32_t** col_idx) {
     if (!set) return set_error(MGPU_ERR_INVALID_ARG, "null set");
+    if (row_ptr || col_idx)
+        if (mgpu_status rc = ensure_host(const_cast<mgpu_set*>(set)); rc != MGPU_OK) return rc;
     if (n_rows) *n_rows = set->h.n_rows;
-    if (nnz) *nnz = (int64_t)set->h.col_idx.size();
+    if (nnz) *nnz = set->h.nnz;
     if (row_ptr) *row_ptr = set->h.row_ptr.data();
     if (col_idx) *col_idx = set->h.col_idx.data();
     return MGPU_OK;
@@ -1076,6 +1091,7 @@ extern "C" mgpu_status mgpu_pattern_get(const mgpu_set* set, int32_t* n_rows, in
 
 extern "C" mgpu_status mgpu_element_topology(const mgpu_set* set, int32_t* n_per_element, const int32_t** ids) {
     if (!set) return set_error(MGPU_ERR_INVALID_ARG, "null set");
+    if (mgpu_status rc = ensure_host(const_cast<mgpu_set*>(set)); rc != MGPU_OK) return rc;
     if (set->h.generic) {
         const auto& na_t = set->h.na_t;
         for (size_t e = 1; e < na_t.size(); ++e)
@@ -1092,6 +1108,7 @@ extern "C" mgpu_status mgpu_element_topology(const mgpu_set* set, int32_t* n_per
 
 extern "C" mgpu_status mgpu_element_topology_csr(const mgpu_set* set, const int64_t** offsets, const int32_t** ids) {
     if (!set) return set_error(MGPU_ERR_INVALID_ARG, "null set");
+    if (mgpu_status rc = ensure_host(const_cast<mgpu_set*>(set)); rc != MGPU_OK) return rc;
     if (offsets) *offsets = set->h.topo_ptr.data();
     if (ids) *ids = set->h.topo.data();
     return MGPU_OK;
@@ -1117,7 +1134,7 @@ extern "C" mgpu_status mgpu_set_assemble(mgpu_set* set, const double* u, double*
 extern "C" mgpu_status mgpu_set_assemble_host(mgpu_set* set, const double* u, double* vals, double* rhs) {
     MGPU_GUARD_BEGIN
     if (!set) return set_error(MGPU_ERR_INVALID_ARG, "null set");
-    const size_t nr = (size_t)set->h.n_rows, nnz = set->h.col_idx.size();
+    const size_t nr = (size_t)set->h.n_rows, nnz = (size_t)set->h.nnz;
     if (nr == 0) return MGPU_OK;
     if (!u || !vals || !rhs) return set_error(MGPU_ERR_INVALID_ARG, "null buffer");
     if (!set->d_u_tmp) {

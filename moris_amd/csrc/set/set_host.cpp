@@ -113,6 +113,7 @@ mgpu_status build_generic(const mgpu_set_desc& d, SetHost& h, std::string* err) 
             h.row_ptr[(size_t)f * N + a + 1] = (int32_t)nnz;
         }
     h.col_idx.resize((size_t)nnz);
+    h.nnz = (int64_t)nnz;
     parallel_for(N, [&](int64_t a) {
         const auto& v = nbr[(size_t)a];
         for (int f = 0; f < nf; ++f) {
@@ -381,6 +382,7 @@ mgpu_status build_set_host(const mgpu_set_desc& d, SetHost& h, std::string* err)
             h.row_ptr[(size_t)f * N + a + 1] = (int32_t)nnz;
         }
     h.col_idx.resize((size_t)nnz);
+    h.nnz = (int64_t)nnz;
     parallel_for(N, [&](int64_t a) {
         int32_t buf[8 * 27];
         int n = 0;

@@ -40,6 +40,7 @@ struct SetHost {
                                      // unique per type, global ids, types concatenated
     std::vector<int32_t> row_ptr;    // [n_rows + 1]
     std::vector<int32_t> col_idx;    // [nnz]
+    int64_t nnz = 0;                 // CSR values (also while col_idx is not fetched)
     // element vertex b -> position of its adof in the sorted per-type topology
     std::vector<uint8_t> vert_pos;   // [n_el][nb]
     // lattice (stencil) path

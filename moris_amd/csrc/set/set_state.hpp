@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -43,6 +44,9 @@ struct mgpu_set {
     mgpu::LqView lv{};
     const mgpu_mlp* mlp = nullptr;
     bool lq_ready = false;
+    // a set made from the GPU set-up holds its pattern, topology and cell map on the device only;
+    // this fills the host copies (h.row_ptr, col_idx, topo, vert_pos, cell_elem) on first need
+    std::function<mgpu_status(mgpu_set*)> fetch_host;
 
     mgpu_set() = default;
     mgpu_set(const mgpu_set&) = delete;
@@ -57,6 +61,16 @@ struct mgpu_set {
 };
 
 namespace mgpu {
+// The host copies of the pattern and topology, fetched from the device the first time a host
+// entry point needs them (sets made from the GPU set-up).
+inline mgpu_status ensure_host(mgpu_set* S) {
+    if (!S->fetch_host) return MGPU_OK;
+    auto f = std::move(S->fetch_host);
+    S->fetch_host = nullptr;
+    return f(S);
+}
+// Buffers of a lattice set once its inputs are on the device (assemble.hip).
+mgpu_status finish_lattice_set(mgpu_set* S, const int32_t* ig_ptr_host);
 // Enqueue one assemble call of the set on S->stream (no synchronisation, no allocation, no memset:
 // safe to capture into a graph).  ev (optional, 6 events): [0] start, [1] after the learned
 // inputs/network, [2] after the moment fit, [3] after the cut-element integrands, [4] after the

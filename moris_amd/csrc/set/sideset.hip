@@ -266,6 +266,7 @@ extern "C" mgpu_status mgpu_sideset_create(mgpu_set* bulk, const mgpu_sideset_de
     MGPU_GUARD_BEGIN
     if (!bulk || !d || !out) return set_error(MGPU_ERR_INVALID_ARG, "null argument");
     *out = nullptr;
+    if (mgpu_status rc = ensure_host(bulk); rc != MGPU_OK) return rc;
     const SetHost& h = bulk->h;
     const int dim = h.dim, nf = h.nf, nb = h.nb, na = nf * nb;
     const bool diff = h.iwg == MGPU_IWG_SPATIALDIFF_BULK;

@@ -38,6 +38,7 @@
 #include "../common/fem_tables.hpp"
 #include "../common/levelset.hpp"
 #include "../common/xtk_cut.hpp"
+#include "../set/set_state.hpp"
 #include "internal.hpp"
 
 #define SU_TRY(expr)                                                                               \
@@ -382,6 +383,30 @@ __global__ void k_rows(Lat L, const int32_t* basis_of, int32_t N, const int32_t*
     }
 }
 
+// Sub-cells the assembly kernels skip (detJ < -1e-6, cl_FEM_Element_Bulk.cpp:291), counted with the
+// host set-up's formula: rows X_r - X_2 scaled by half the cell extent.
+__global__ void k_count_skip(Lat L, int32_t n_ig, const double* ig_xi, unsigned long long* count) {
+    const int32_t t = blockIdx.x * kT + threadIdx.x;
+    if (t >= n_ig) return;
+    double detJ;
+    if (L.dim == 3) {
+        const double* X = ig_xi + (size_t)t * 12;
+        double J[3][3];
+        const int rr[3] = {0, 1, 3};
+        for (int r = 0; r < 3; ++r)
+            for (int c = 0; c < 3; ++c) J[r][c] = (X[rr[r] * 3 + c] - X[2 * 3 + c]) * (0.5 * L.h[c]);
+        detJ = (J[0][0] * (J[1][1] * J[2][2] - J[2][1] * J[1][2]) - J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0])) / 6.0;
+    } else {
+        const double* X = ig_xi + (size_t)t * 6;
+        double J[2][2];
+        for (int r = 0; r < 2; ++r)
+            for (int c = 0; c < 2; ++c) J[r][c] = (X[r * 2 + c] - X[2 * 2 + c]) * (0.5 * L.h[c]);
+        detJ = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) / 2.0;
+    }
+    if (detJ < kDetJLowerLimit) atomicAdd(count, 1ull);
+}
+
 }  // namespace su
 }  // namespace mgpu
 
@@ -397,6 +422,8 @@ struct mgpu_gpu_setup {
     // device arrays
     int32_t *elem_cell = nullptr, *ig_ptr = nullptr, *t_ptr = nullptr, *t_adof = nullptr, *topo = nullptr;
     int32_t *row_ptr = nullptr, *col_idx = nullptr;
+    int32_t *cell_elem = nullptr, *node_adof = nullptr, *basis_of = nullptr;   // lattice maps (p = 1: nodes)
+    int64_t n_skip = 0;                                                           // sub-cells with detJ < -1e-6
     double *ig_xi = nullptr, *material = nullptr, *ip_box = nullptr, *t_weight = nullptr, *vertex_phi = nullptr;
     std::vector<void*> allocs;
     // host copies behind mgpu_setup_desc
@@ -537,6 +564,7 @@ extern "C" mgpu_status mgpu_setup_build(const mgpu_problem_spec* spec, void* str
     if ((rc = dalloc(S.get(), (size_t)L.n_cells, &is_elem)) != MGPU_OK) return rc;
     if ((rc = dalloc(S.get(), (size_t)L.n_cells + 1, &elem_pos)) != MGPU_OK) return rc;
     if ((rc = dalloc(S.get(), (size_t)L.n_cells, &cell_elem)) != MGPU_OK) return rc;
+    S->cell_elem = cell_elem;
     hipLaunchKernelGGL(su::k_elem_flag, dim3(blocks(L.n_cells)), dim3(su::kT), 0, st, L, cls, cut_pos, n_sub, is_elem);
     int32_t n_el = 0;
     if ((rc = scan(S.get(), is_elem, elem_pos, L.n_cells, &n_el)) != MGPU_OK) return rc;
@@ -561,6 +589,11 @@ extern "C" mgpu_status mgpu_setup_build(const mgpu_problem_spec* spec, void* str
         hipLaunchKernelGGL(su::k_copy_sub, dim3((unsigned)n_int), dim3(64), 0, st, L.dim, cut_cell, cell_elem, n_sub,
                            S->ig_ptr, slots, S->ig_xi);
     SU_TRY(hipGetLastError());
+    unsigned long long* skip_count;
+    if ((rc = dalloc(S.get(), 1, &skip_count)) != MGPU_OK) return rc;
+    SU_TRY(hipMemsetAsync(skip_count, 0, sizeof(unsigned long long), st));
+    if (n_ig > 0)
+        hipLaunchKernelGGL(su::k_count_skip, dim3(blocks(n_ig)), dim3(su::kT), 0, st, L, n_ig, S->ig_xi, skip_count);
     // 4. adof numbering of the used bases, T rows, topology
     const int64_t n_basis = (int64_t)L.nbs[0] * L.nbs[1] * L.nbs[2];
     if (n_basis >= INT32_MAX) return set_error(MGPU_ERR_INVALID_ARG, "too many bases for 32-bit ids");
@@ -574,6 +607,8 @@ extern "C" mgpu_status mgpu_setup_build(const mgpu_problem_spec* spec, void* str
     if ((rc = scan(S.get(), used, bpos, n_basis, &N)) != MGPU_OK) return rc;
     S->N = N;
     if ((rc = dalloc(S.get(), (size_t)N, &basis_of)) != MGPU_OK) return rc;
+    S->node_adof = badof;
+    S->basis_of = basis_of;
     hipLaunchKernelGGL(su::k_basis_adof, dim3(blocks(n_basis)), dim3(su::kT), 0, st, n_basis, used, bpos, badof,
                        basis_of);
     const int per_row = order == 1 ? 1 : (L.dim == 3 ? 8 : 4);
@@ -616,6 +651,9 @@ extern "C" mgpu_status mgpu_setup_build(const mgpu_problem_spec* spec, void* str
     S->ms = ms;
     int32_t bad = 0;
     SU_TRY(hipMemcpy(&bad, status, sizeof(int32_t), hipMemcpyDeviceToHost));
+    unsigned long long nsk = 0;
+    SU_TRY(hipMemcpy(&nsk, skip_count, sizeof(nsk), hipMemcpyDeviceToHost));
+    S->n_skip = (int64_t)nsk;
     if (bad) return set_error(MGPU_ERR_GEOMETRY, "node-hierarchy template not found in a cell");
     {   // cut elements: elements with sub-cells
         std::vector<int32_t> ns((size_t)n_el + 1);
@@ -723,3 +761,116 @@ extern "C" mgpu_status mgpu_setup_desc(mgpu_gpu_setup* s, mgpu_set_desc* d) {
     return MGPU_OK;
     MGPU_GUARD_END
 }
+
+namespace {
+template <class T>
+mgpu_status set_upload(mgpu_set* S, const T* src, size_t n, T** dst) {
+    *dst = nullptr;
+    if (n == 0) return MGPU_OK;
+    void* p = nullptr;
+    SU_TRY(hipMalloc(&p, n * sizeof(T)));
+    S->allocs.push_back(p);
+    SU_TRY(hipMemcpy(p, src, n * sizeof(T), hipMemcpyHostToDevice));
+    *dst = static_cast<T*>(p);
+    return MGPU_OK;
+}
+}  // namespace
+
+extern "C" mgpu_status mgpu_set_create_from_setup(mgpu_gpu_setup* G, mgpu_set** out) {
+    MGPU_GUARD_BEGIN
+    if (!G || !out) return set_error(MGPU_ERR_INVALID_ARG, "null argument");
+    *out = nullptr;
+    if (G->L.order != 1)
+        return set_error(MGPU_ERR_UNSUPPORTED, "a set made from the GPU set-up needs nodal T rows (bspline_order 1); "
+                                               "use mgpu_setup_desc and mgpu_set_create");
+    const su::Lat& L = G->L;
+    auto S = std::make_unique<mgpu_set>();
+    SetHost& h = S->h;
+    h.dim = L.dim;
+    h.iwg = L.iwg;
+    h.nf = G->nf;
+    h.nb = L.nb;
+    h.na = h.nf * h.nb;
+    h.nm = h.nf == 1 ? 1 : 2;
+    h.n_el = G->n_el;
+    h.n_adofs_type = G->N;
+    h.n_rows = (int32_t)G->n_rows;
+    h.nnz = G->nnz;
+    h.lattice = true;
+    for (int k = 0; k < 3; ++k) {
+        h.lat.n[k] = L.n[k];
+        h.lat.lo[k] = L.lo[k];
+        h.lat.h[k] = L.h[k];
+    }
+    h.n_cells = L.n_cells;
+    h.n_nodes = L.n_nodes;
+    const int npts = L.dim == 3 ? kTet11Points : kTri7Points;
+    h.n_skip_subcells = G->n_skip;
+    h.n_skip_points = G->n_skip * npts;
+    // cut elements from the sub-cell offsets (the one host copy the set needs up front)
+    std::vector<int32_t> igp((size_t)G->n_el + 1);
+    SU_TRY(hipMemcpy(igp.data(), G->ig_ptr, igp.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+    h.elem_cut.assign((size_t)G->n_el, -1);
+    for (int32_t e = 0; e < G->n_el; ++e)
+        if (igp[(size_t)e + 1] > igp[(size_t)e]) {
+            h.elem_cut[(size_t)e] = (int32_t)h.cut_elems.size();
+            h.cut_elems.push_back(e);
+        }
+    DevView& v = S->v;
+    v.n_el = h.n_el;
+    v.n_cut = (int32_t)h.cut_elems.size();
+    v.N = h.n_adofs_type;
+    v.n_rows = h.n_rows;
+    v.nx = L.n[0];
+    v.ny = L.n[1];
+    v.nz = L.dim == 3 ? L.n[2] : 1;
+    for (int d = 0; d < 3; ++d) v.h[d] = L.h[d];
+    S->n_ip_cut = (int64_t)G->n_ig * npts;
+    // the set-up's device arrays, used in place (the set-up outlives the set)
+    v.material = G->material;
+    v.cell_elem = G->cell_elem;
+    v.node_adof = G->node_adof;
+    v.row_node = G->basis_of;
+    v.row_ptr = G->row_ptr;
+    v.ig_ptr = G->ig_ptr;
+    v.ig_xi = G->ig_xi;
+    mgpu_status rc;
+    if ((rc = set_upload(S.get(), h.elem_cut.data(), h.elem_cut.size(), const_cast<int32_t**>(&v.elem_cut))) != MGPU_OK)
+        return rc;
+    if ((rc = set_upload(S.get(), h.cut_elems.data(), h.cut_elems.size(), const_cast<int32_t**>(&v.cut_elems))) != MGPU_OK)
+        return rc;
+    if (L.dim == 3 && v.n_cut > 0) {
+        std::vector<double> vphi((size_t)G->n_el * 8);
+        SU_TRY(hipMemcpy(vphi.data(), G->vertex_phi, vphi.size() * sizeof(double), hipMemcpyDeviceToHost));
+        S->vphi_cut.resize((size_t)v.n_cut * 8);
+        for (int32_t c = 0; c < v.n_cut; ++c)
+            for (int k = 0; k < 8; ++k) S->vphi_cut[(size_t)c * 8 + k] = vphi[(size_t)h.cut_elems[(size_t)c] * 8 + k];
+    }
+    if ((rc = finish_lattice_set(S.get(), igp.data())) != MGPU_OK) return rc;
+    // host copies of the pattern, topology and cell map only when a host entry point asks
+    S->fetch_host = [G](mgpu_set* T) -> mgpu_status {
+        SetHost& hh = T->h;
+        hh.row_ptr.resize((size_t)hh.n_rows + 1);
+        hh.col_idx.resize((size_t)hh.nnz);
+        hh.topo.resize((size_t)hh.n_el * hh.na);
+        hh.cell_elem.resize((size_t)hh.n_cells);
+        SU_TRY(hipMemcpy(hh.row_ptr.data(), G->row_ptr, hh.row_ptr.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+        if (hh.nnz > 0)
+            SU_TRY(hipMemcpy(hh.col_idx.data(), G->col_idx, hh.col_idx.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+        if (!hh.topo.empty())
+            SU_TRY(hipMemcpy(hh.topo.data(), G->topo, hh.topo.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+        SU_TRY(hipMemcpy(hh.cell_elem.data(), G->cell_elem, hh.cell_elem.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+        hh.topo_ptr.resize((size_t)hh.n_el + 1);
+        for (int64_t e = 0; e <= hh.n_el; ++e) hh.topo_ptr[(size_t)e] = e * hh.na;
+        // lattice vertex -> position of its adof among the element's adofs (ascending node index)
+        static const uint8_t pos3[8] = {0, 1, 3, 2, 4, 5, 7, 6};
+        hh.vert_pos.resize((size_t)hh.n_el * hh.nb);
+        for (int64_t e = 0; e < hh.n_el; ++e)
+            for (int b = 0; b < hh.nb; ++b) hh.vert_pos[(size_t)e * hh.nb + b] = pos3[b];
+        return MGPU_OK;
+    };
+    *out = S.release();
+    return MGPU_OK;
+    MGPU_GUARD_END
+}
+

@@ -13,7 +13,7 @@ _i32p = ctypes.POINTER(ctypes.c_int32)
 _f64p = ctypes.POINTER(ctypes.c_double)
 
 _NAMES = ("mgpu_setup_build", "mgpu_setup_destroy", "mgpu_setup_counts", "mgpu_setup_time", "mgpu_setup_copy",
-          "mgpu_setup_desc")
+          "mgpu_setup_desc", "mgpu_set_create_from_setup")
 
 
 def _lib():
@@ -27,6 +27,7 @@ def _lib():
         lib.mgpu_setup_copy.argtypes = [_vp, _i32p, _i32p, _f64p, _f64p, _f64p, _f64p, _i32p, _i32p, _f64p, _i32p,
                                         _i32p, _i32p]
         lib.mgpu_setup_desc.argtypes = [_vp, ctypes.POINTER(SetDesc)]
+        lib.mgpu_set_create_from_setup.argtypes = [_vp, ctypes.POINTER(_vp)]
         for n in _NAMES:
             getattr(lib, n).restype = ctypes.c_int32
         lib._setup_bound = True
@@ -53,8 +54,24 @@ class GpuSetup:
         ms = ctypes.c_double()
         _check(lib.mgpu_setup_time(self._h, ctypes.byref(ms)), "mgpu_setup_time")
         self.ms = ms.value
-        self.desc = SetDesc()
-        _check(lib.mgpu_setup_desc(self._h, ctypes.byref(self.desc)), "mgpu_setup_desc")
+        self._desc = None
+
+    @property
+    def desc(self):
+        """SetDesc over host copies of the device arrays (made on first use)."""
+        if self._desc is None:
+            d = SetDesc()
+            _check(_lib().mgpu_setup_desc(self._h, ctypes.byref(d)), "mgpu_setup_desc")
+            self._desc = d
+        return self._desc
+
+    def make_set(self):
+        """A Set made from the device arrays with no host round trip (bspline_order 1); it keeps
+        this set-up alive."""
+        from . import Set
+        h = _vp()
+        _check(_lib().mgpu_set_create_from_setup(self._h, ctypes.byref(h)), "mgpu_set_create_from_setup")
+        return Set(None, keepalive=self, handle=h, n_elements=self.n_elements, n_cut=self.n_cut)
 
     @property
     def dim(self):

@@ -9,8 +9,12 @@ cl_HMR_T_Matrix_Base.cpp:306-457 (T rows) and cl_Sparse_Matrix_EpetraFECrs.cpp:1
   math library); p = 1 and p = 2, 2D and 3D, phase 0 and 1.
 * Gyroid (sin / cos from the device library): integer arrays identical, coordinates within 1e-13.
 * A Set made from the GPU set-up's descriptor assembles K and R bit-identical to the host one.
+* A Set made directly from the set-up's device arrays (mgpu_set_create_from_setup) equals the set
+  mgpu_set_create makes from host copies of the same arrays: K and R bit-identical (FP64, mixed,
+  mixed + learned), pattern, topology and statistics equal, a side set on it equal.
 * The headline batch's set-up (32 sphere-array variants, 64^3 elasticity): variants 0, 13 and 31
-  bit-identical, device time per variant printed beside the host set-up's wall time.
+  bit-identical, device time per variant printed beside the host set-up's wall time, and the
+  set made from the device arrays.
 """
 import time
 
@@ -20,6 +24,9 @@ import pytest
 import moris_amd as M
 from helpers import SEED
 from moris_amd import gpu_setup as G
+from moris_amd import learned as L
+from moris_amd import sideset as SS
+from moris_amd import train_lq
 from moris_amd import workloads as W
 
 pytestmark = pytest.mark.gpu
@@ -106,6 +113,57 @@ def test_set_from_gpu_setup_assembles_identically():
     assert np.array_equal(r1.view(np.uint64), r2.view(np.uint64))
 
 
+def _same(a, b):
+    return np.array_equal(np.asarray(a).view(np.uint64), np.asarray(b).view(np.uint64))
+
+
+@pytest.mark.parametrize("case", ["spheres3_elast_p1", "sphere3_diff_p1", "circle2_elast_p1", "plane3_through_nodes"])
+def test_set_from_device_arrays_equals_host_set(case):
+    spec, keep = CASES[case]()
+    P = M.Problem(spec, keep)
+    Gs = G.GpuSetup(spec, keep)
+    # the reference: a set made through mgpu_set_create from host copies of the same set-up's arrays
+    # (the host set-up's own set differs only where the device math library rounds the material)
+    S = M.Set(Gs.desc, keepalive=Gs)
+    D = Gs.make_set()
+    assert (D.n_rows, D.nnz, D.n_cut) == (S.n_rows, S.nnz, S.n_cut)
+    u = np.random.default_rng(SEED + 9).uniform(-1.0, 1.0, S.n_rows)
+    modes = [(M.PREC_FP64, M.QUAD_SUBDIVISION), (M.PREC_MIXED, M.QUAD_SUBDIVISION)]
+    if spec.dim == 3 and spec.iwg == M.IWG_STRUC_LINEAR_BULK:
+        net = train_lq.load() or L.MLP(seed=3)
+        L.attach(S, net)
+        L.attach(D, net)
+        modes.append((M.PREC_MIXED, M.QUAD_LEARNED))
+    for prec, quad in modes:
+        S.options(prec, quad)
+        D.options(prec, quad)
+        v1, r1 = S.compute_jacobian_and_residual(u)
+        v2, r2 = D.compute_jacobian_and_residual(u)
+        assert _same(v1, v2) and _same(r1, r2), (prec, quad)
+    assert np.array_equal(S.statistics(), D.statistics())
+    rp, ci = S.pattern()
+    rp2, ci2 = D.pattern()
+    assert np.array_equal(rp, rp2) and np.array_equal(ci, ci2)
+    assert np.array_equal(S.topology(), D.topology())
+    # a side set on each: the same values
+    geom = SS.GEOM_TRI3 if spec.dim == 3 else SS.GEOM_LINE2
+    iwg = SS.IWG_SPATIALDIFF_NEUMANN if spec.iwg == M.IWG_SPATIALDIFF_BULK else SS.IWG_STRUC_LINEAR_NEUMANN
+    e, x, nrm = SS.facets(P, SS.SIDE_INTERFACE, geom)
+    val = [0.3, -0.2, 0.1][: (1 if spec.iwg == M.IWG_SPATIALDIFF_BULK else spec.dim)]
+    S.options(M.PREC_FP64, M.QUAD_SUBDIVISION)
+    D.options(M.PREC_FP64, M.QUAD_SUBDIVISION)
+    a1 = SS.assemble(S, [SS.SideSet(S, iwg, geom, e, x, nrm, value=val)], u)
+    a2 = SS.assemble(D, [SS.SideSet(D, iwg, geom, e, x, nrm, value=val)], u)
+    assert _same(a1[0], a2[0]) and _same(a1[1], a2[1])
+
+
+def test_set_from_device_arrays_needs_nodal_rows():
+    spec, keep = CASES["sphere3_diff_p2"]()
+    Gs = G.GpuSetup(spec, keep)
+    with pytest.raises(M.MgpuError):
+        Gs.make_set()
+
+
 def test_rejects_unsupported_specs():
     for kw in (dict(lagrange_order=2, bspline_order=2), dict(refine_levels=1, bspline_order=2), dict(bspline_order=3)):
         spec, keep = M.make_spec(3, (4, 4, 4), iwg=M.IWG_STRUC_LINEAR_BULK, spheres=[[0.5, 0.5, 0.5, 0.3]], **kw)
@@ -133,6 +191,19 @@ def test_headline_batch_setup_variants():
         rp, ci = S.pattern()
         grp, gci = Gs.pattern()
         assert np.array_equal(rp, grp) and np.array_equal(ci, gci)
+        # the same variant end to end on the device: set-up, set from its arrays, one assembly
+        M.synchronize()
+        t0 = time.perf_counter()
+        G2 = G.GpuSetup(spec, keep)
+        D = G2.make_set()
+        ub = M.DeviceBuffer.from_array(np.zeros(D.n_rows))
+        vb, rb = M.DeviceBuffer(8 * D.nnz), M.DeviceBuffer(8 * D.n_rows)
+        D.assemble_device(ub, vb, rb)
+        M.synchronize()
+        dev_s = time.perf_counter() - t0
+        v1, _ = S.compute_jacobian_and_residual(np.zeros(S.n_rows))
+        assert _same(v1, vb.to_array(np.float64, D.nnz))
         print("variant %d: elements %d cut %d sub-tets %d nnz %d | host set-up %.3f s, GPU set-up %.3f s wall, "
-              "%.3f ms device" % (k, Gs.n_elements, Gs.n_cut, Gs.n_ig, Gs.nnz, host_s, gpu_s, Gs.ms), flush=True)
-        del Gs, S, P
+              "%.3f ms device; GPU set-up + set from its arrays + first assembly %.3f s" % (
+                  k, Gs.n_elements, Gs.n_cut, Gs.n_ig, Gs.nnz, host_s, gpu_s, Gs.ms, dev_s), flush=True)
+        del D, G2, Gs, S, P
