@@ -70,8 +70,9 @@ typedef struct mgpu_sideset_desc {
 
 typedef struct mgpu_sideset mgpu_sideset;
 
-/* A side set on the bulk set `bulk` (which must outlive it).  The bulk set's elements must have
-   nodal T rows (one adof of weight 1 per basis); others are refused with MGPU_ERR_UNSUPPORTED. */
+/* A side set on the bulk set `bulk` (which must outlive it): lattice sets with nodal T rows, and
+   generic-path sets (p = 2 B-spline rows on HEX8 / QUAD4 cells, HEX27 cells, THB rows) with up to
+   64 adofs per dof type and element.                                                            */
 mgpu_status mgpu_sideset_create(mgpu_set* bulk, const mgpu_sideset_desc* desc, mgpu_sideset** out);
 mgpu_status mgpu_sideset_destroy(mgpu_sideset* ss);
 /* ADD the side set's K and R into the bulk set's CSR values and R (DEVICE arrays; call after

@@ -202,6 +202,257 @@ __global__ __launch_bounds__(64) void k_side(SideView s, const double* u) {
     }
 }
 
+// ------------------------------------------------------------------------------------------------
+// Side sets on generic-path sets (T rows with several entries: p = 2 B-splines on HEX8 / QUAD4
+// cells, HEX27 cells, THB rows).  One block per side cluster (the facets of one element): the
+// facet matrix K_p over the IP cell's NB bases is summed over the cluster's facets and points,
+// then projected, K_e = T^T K_p T and R_e = T^T (K_p T u_e - r_p), as
+// Equation_Object::get_egn_obj_jacobian_and_residual (cl_MSI_Equation_Object.cpp:789-842) does for
+// the reference's side clusters.  HEX27 bases are the tensor products of the 1D quadratic
+// Lagrange polynomials (kHex27Half node order).
+// ------------------------------------------------------------------------------------------------
+constexpr int kSideMaxNat = 64;   // adofs per dof type of a side cluster's element
+
+struct SideGenView {
+    int32_t n_clusters, N, nm;
+    double beta, alpha, value[3];
+    int32_t neumann;
+    const int32_t* cl_elem;    // [n_cl] element of each side cluster
+    const int32_t* cl_fptr;    // [n_cl + 1] its facets
+    const double* xi;          // [n facets][nvs][dim], in cluster order
+    const double* normal;      // [n facets][dim]
+    const double* hlen;        // [n_cl] length measure of the cluster
+    const int64_t* cl_aptr;    // [n_cl + 1] into cl_adof
+    const int32_t* cl_adof;    // the cluster element's adofs (global rows, types concatenated)
+    const int64_t* ke_off;     // [n_cl + 1] pool offsets, (nf na_t)^2 per cluster
+    const int64_t* re_off;     // [n_cl + 1] nf na_t per cluster
+    double* ke;
+    double* re;
+};
+
+template <int DIM, int NB>
+__device__ inline void side_basis(int a, const double* xi, const double* ext, double& N, double* g) {
+    if constexpr (NB == 27) {
+        double l[3][3], dl[3][3];
+        for (int d = 0; d < 3; ++d) lagrange_quadratic_1d(xi[d], l[d], dl[d]);
+        const int i0 = kHex27Half[a][0], i1 = kHex27Half[a][1], i2 = kHex27Half[a][2];
+        N = l[0][i0] * l[1][i1] * l[2][i2];
+        g[0] = dl[0][i0] * l[1][i1] * l[2][i2] * 2.0 / ext[0];
+        g[1] = l[0][i0] * dl[1][i1] * l[2][i2] * 2.0 / ext[1];
+        g[2] = l[0][i0] * l[1][i1] * dl[2][i2] * 2.0 / ext[2];
+    } else {
+        double f[DIM];
+        for (int d = 0; d < DIM; ++d) f[d] = 1.0 + (DIM == 3 ? kHex8XiHat[a][d] : kQuad4XiHat[a][d]) * xi[d];
+        N = 1.0;
+        for (int d = 0; d < DIM; ++d) N *= f[d];
+        N /= (double)(1 << DIM);
+        for (int i = 0; i < DIM; ++i) {
+            double p = (DIM == 3 ? kHex8XiHat[a][i] : kQuad4XiHat[a][i]) / (double)(1 << DIM);
+            for (int d = 0; d < DIM; ++d)
+                if (d != i) p *= f[d];
+            g[i] = p * 2.0 / ext[i];
+        }
+    }
+}
+
+template <int DIM, int NF, int GEOM, int NB>
+__global__ __launch_bounds__(256) void k_side_gen(SideGenView s, GenView gv, const double* u) {
+    constexpr int NA = NF * NB;
+    constexpr int NVS = GEOM == MGPU_GEOM_TRI3 ? 3 : (GEOM == MGPU_GEOM_QUAD4 ? 4 : 2);
+    constexpr int NQ = GEOM == MGPU_GEOM_TRI3 ? kTri7Points : (GEOM == MGPU_GEOM_QUAD4 ? kQuad2Points : kBar3Points);
+    constexpr int M = kSideMaxNat;
+    __shared__ double Kp[NA * NA];
+    __shared__ double rd[NA];
+    __shared__ double Nq[NB], Gq[NB * DIM];
+    __shared__ double Tm[NB * M];
+    __shared__ double W[NB * M];
+    __shared__ double ue[NF * M], up[NA];
+    __shared__ double pt[DIM + 1];
+    const int c = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int e = s.cl_elem[c];
+    double lo[DIM], ext[DIM];
+    for (int d = 0; d < DIM; ++d) {
+        lo[d] = gv.ip_box[(size_t)e * 2 * DIM + d];
+        ext[d] = gv.ip_box[(size_t)e * 2 * DIM + DIM + d];
+    }
+    const double m0 = gv.material[(size_t)e * s.nm], m1 = s.nm > 1 ? gv.material[(size_t)e * s.nm + 1] : 0.0;
+    double lam = 0.0, mu = 0.0;
+    if constexpr (NF > 1) {
+        constexpr int NS = DIM == 3 ? 6 : 3;
+        double D[NS][NS];
+        elastic_D<DIM>(m0, m1, D);
+        lam = D[0][1];
+        mu = D[NS - 1][NS - 1];
+    }
+    const double gamma = s.neumann ? 0.0 : s.alpha * m0 / s.hlen[c];
+    const double beta = s.beta;
+    for (int t = tid; t < NA * NA; t += 256) Kp[t] = 0.0;
+    for (int t = tid; t < NA; t += 256) rd[t] = 0.0;
+    for (int fct = s.cl_fptr[c]; fct < s.cl_fptr[c + 1]; ++fct) {
+        double X[NVS][DIM], n[DIM];
+        for (int k = 0; k < NVS; ++k)
+            for (int d = 0; d < DIM; ++d) X[k][d] = s.xi[((size_t)fct * NVS + k) * DIM + d];
+        for (int d = 0; d < DIM; ++d) n[d] = s.normal[(size_t)fct * DIM + d];
+        for (int q = 0; q < NQ; ++q) {
+            __syncthreads();   // the previous point's N and gradients are no longer read
+            if (tid == 0) {
+                double phi[NVS], detJ;
+                if constexpr (GEOM == MGPU_GEOM_TRI3) {
+                    phi[0] = kTri7Xi[q][0];
+                    phi[1] = kTri7Xi[q][1];
+                    phi[2] = 1.0 - kTri7Xi[q][0] - kTri7Xi[q][1];
+                    double e1[3], e2[3];
+                    for (int d = 0; d < 3; ++d) {
+                        e1[d] = (X[0][d] - X[2][d]) * 0.5 * ext[d];
+                        e2[d] = (X[1][d] - X[2][d]) * 0.5 * ext[d];
+                    }
+                    const double c0 = e1[1] * e2[2] - e1[2] * e2[1], c1 = e1[2] * e2[0] - e1[0] * e2[2],
+                                 c2 = e1[0] * e2[1] - e1[1] * e2[0];
+                    detJ = sqrt(c0 * c0 + c1 * c1 + c2 * c2) / 2.0;
+                } else if constexpr (GEOM == MGPU_GEOM_QUAD4) {
+                    const double ss = kQuad2Xi[q][0], tt = kQuad2Xi[q][1];
+                    phi[0] = 0.25 * (1.0 - ss) * (1.0 - tt);
+                    phi[1] = 0.25 * (1.0 + ss) * (1.0 - tt);
+                    phi[2] = 0.25 * (1.0 + ss) * (1.0 + tt);
+                    phi[3] = 0.25 * (1.0 - ss) * (1.0 + tt);
+                    double js[3], jt[3];
+                    for (int d = 0; d < 3; ++d) {
+                        js[d] = 0.25 * (-(1.0 - tt) * X[0][d] + (1.0 - tt) * X[1][d] + (1.0 + tt) * X[2][d] -
+                                        (1.0 + tt) * X[3][d]) * 0.5 * ext[d];
+                        jt[d] = 0.25 * (-(1.0 - ss) * X[0][d] - (1.0 + ss) * X[1][d] + (1.0 + ss) * X[2][d] +
+                                        (1.0 - ss) * X[3][d]) * 0.5 * ext[d];
+                    }
+                    const double c0 = js[1] * jt[2] - js[2] * jt[1], c1 = js[2] * jt[0] - js[0] * jt[2],
+                                 c2 = js[0] * jt[1] - js[1] * jt[0];
+                    detJ = sqrt(c0 * c0 + c1 * c1 + c2 * c2);
+                } else {
+                    const double ss = kBar3Xi[q];
+                    phi[0] = 0.5 * (1.0 - ss);
+                    phi[1] = 0.5 * (1.0 + ss);
+                    double l2 = 0.0;
+                    for (int d = 0; d < DIM; ++d) {
+                        const double t = (X[1][d] - X[0][d]) * 0.5 * ext[d];
+                        l2 += t * t;
+                    }
+                    detJ = 0.5 * sqrt(l2);
+                }
+                for (int d = 0; d < DIM; ++d) {
+                    double v = 0.0;
+                    for (int k = 0; k < NVS; ++k) v += phi[k] * X[k][d];
+                    pt[d] = v;
+                }
+                pt[DIM] = (GEOM == MGPU_GEOM_TRI3 ? kTri7W[q] : (GEOM == MGPU_GEOM_QUAD4 ? kQuad2W[q] : kBar3W[q])) * detJ;
+            }
+            __syncthreads();
+            if (tid < NB) {
+                double xi[DIM], g[DIM], N;
+                for (int d = 0; d < DIM; ++d) xi[d] = pt[d];
+                side_basis<DIM, NB>(tid, xi, ext, N, g);
+                Nq[tid] = N;
+                for (int d = 0; d < DIM; ++d) Gq[tid * DIM + d] = g[d];
+            }
+            __syncthreads();
+            const double w = pt[DIM];
+            for (int pr = tid; pr < NB * NB; pr += 256) {
+                const int a = pr / NB, b = pr % NB;
+                const double Na = Nq[a], Nb = Nq[b];
+                const double* ga = Gq + a * DIM;
+                const double* gb = Gq + b * DIM;
+                double nga = 0.0, ngb = 0.0;
+                for (int d = 0; d < DIM; ++d) {
+                    nga += n[d] * ga[d];
+                    ngb += n[d] * gb[d];
+                }
+                if (NF == 1) {
+                    const double k = m0;
+                    if (!s.neumann) Kp[a * NA + b] += w * (-Na * k * ngb + beta * k * nga * Nb + gamma * Na * Nb);
+                    if (b == 0) rd[a] += s.neumann ? w * Na * s.value[0] : w * (beta * k * nga + gamma * Na) * s.value[0];
+                } else {
+                    if (!s.neumann)
+                        for (int f = 0; f < NF; ++f)
+                            for (int g2 = 0; g2 < NF; ++g2) {
+                                const double Tfgb = lam * n[f] * gb[g2] + mu * n[g2] * gb[f] + (f == g2 ? mu * ngb : 0.0);
+                                const double Tgfa = lam * n[g2] * ga[f] + mu * n[f] * ga[g2] + (f == g2 ? mu * nga : 0.0);
+                                Kp[(f * NB + a) * NA + g2 * NB + b] +=
+                                    w * (-Na * Tfgb + beta * Tgfa * Nb + (f == g2 ? gamma * Na * Nb : 0.0));
+                            }
+                    if (b == 0)
+                        for (int f = 0; f < NF; ++f) {
+                            double t;
+                            if (s.neumann) {
+                                t = Na * s.value[f];
+                            } else {
+                                t = gamma * Na * s.value[f];
+                                for (int g2 = 0; g2 < NF; ++g2) {
+                                    const double Tgfa = lam * n[g2] * ga[f] + mu * n[f] * ga[g2] + (f == g2 ? mu * nga : 0.0);
+                                    t += beta * Tgfa * s.value[g2];
+                                }
+                            }
+                            rd[f * NB + a] += w * t;
+                        }
+                }
+            }
+        }
+    }
+    // the element's T (NB x na, element-local columns) and its adof values
+    const int na = gv.na_t[e];
+    for (int t = tid; t < NB * M; t += 256) Tm[t] = 0.0;
+    __syncthreads();
+    if (tid < NB)
+        for (int k = gv.t_ptr[(size_t)e * NB + tid]; k < gv.t_ptr[(size_t)e * NB + tid + 1]; ++k)
+            Tm[tid * M + gv.t_col[k]] += gv.t_w[k];
+    const int64_t a0 = s.cl_aptr[c];
+    for (int t = tid; t < NF * na; t += 256) ue[t] = u[s.cl_adof[a0 + t]];
+    __syncthreads();
+    // u_p = T u_e per field, then r_p = K_p u_p - rd (into up, after a barrier)
+    double upv = 0.0;
+    if (tid < NA) {
+        const int f = tid / NB, b = tid % NB;
+        for (int x = 0; x < na; ++x) upv += Tm[b * M + x] * ue[f * na + x];
+    }
+    __syncthreads();
+    if (tid < NA) up[tid] = upv;
+    __syncthreads();
+    double rp = 0.0;
+    if (tid < NA) {
+        if (!s.neumann)
+            for (int j = 0; j < NA; ++j) rp += Kp[tid * NA + j] * up[j];
+        rp -= rd[tid];
+    }
+    __syncthreads();
+    if (tid < NA) up[tid] = rp;
+    __syncthreads();
+    double* re = s.re + s.re_off[c];
+    for (int t = tid; t < NF * na; t += 256) {
+        const int f = t / na, x = t % na;
+        double v = 0.0;
+        for (int b = 0; b < NB; ++b) v += Tm[b * M + x] * up[f * NB + b];
+        re[t] = v;
+    }
+    if (s.neumann) return;
+    // K_e = T^T K_p T, one (f, g) block at a time through W = K_p(f, g) T
+    double* ke = s.ke + s.ke_off[c];
+    const int nat = NF * na;
+    for (int f = 0; f < NF; ++f)
+        for (int g2 = 0; g2 < NF; ++g2) {
+            __syncthreads();
+            for (int t = tid; t < NB * na; t += 256) {
+                const int b = t / na, y = t % na;
+                double v = 0.0;
+                for (int b2 = 0; b2 < NB; ++b2) v += Kp[(f * NB + b) * NA + g2 * NB + b2] * Tm[b2 * M + y];
+                W[b * M + y] = v;
+            }
+            __syncthreads();
+            for (int t = tid; t < na * na; t += 256) {
+                const int x = t / na, y = t % na;
+                double v = 0.0;
+                for (int b = 0; b < NB; ++b) v += Tm[b * M + x] * W[b * M + y];
+                ke[(size_t)(f * na + x) * nat + g2 * na + y] = v;
+            }
+        }
+}
+
 // out[dst[t]] += sum of pool[src[k]], k in [ptr[t], ptr[t+1]) (one thread per target)
 __global__ __launch_bounds__(256) void k_side_gather(int64_t n, const int64_t* ptr, const int64_t* dst,
                                                      const int64_t* src, const double* pool, double* out) {
@@ -219,7 +470,9 @@ using namespace mgpu;
 struct mgpu_sideset {
     mgpu_set* bulk = nullptr;
     int dim = 3, nf = 1, nb = 8, geom = MGPU_GEOM_TRI3;
+    bool generic = false;      // side clusters of a generic-path set (k_side_gen)
     SideView v{};
+    SideGenView gv{};
     int64_t n_kt = 0, n_rt = 0;
     const int64_t *kt_ptr = nullptr, *kt_dst = nullptr, *kt_src = nullptr;
     const int64_t *rt_ptr = nullptr, *rt_dst = nullptr, *rt_src = nullptr;
@@ -262,6 +515,143 @@ void build_gather(std::vector<std::pair<int64_t, int64_t>>& pairs, std::vector<i
 }
 }  // namespace
 
+namespace {
+// Side set of a generic-path bulk set: facets grouped into side clusters (one per element, in
+// element order), the cluster's length measure, its element's adofs, pool offsets and the gather
+// lists from each K_e / R_e entry to its CSR value / row.
+mgpu_status create_generic_side(mgpu_set* bulk, const mgpu_sideset_desc* d, int kind, double beta, int nvs,
+                                mgpu_sideset** out) {
+    const SetHost& h = bulk->h;
+    const int dim = h.dim, nf = h.nf, nb = h.nb;
+    if (!((dim == 3 && (nb == 8 || nb == 27)) || (dim == 2 && nb == 4)))
+        return set_error(MGPU_ERR_UNSUPPORTED, "side sets on generic sets need HEX8 / HEX27 / QUAD4 IP cells");
+    const int32_t nfc = d->n_facets;
+    for (int32_t k = 0; k < nfc; ++k)
+        if (h.na_t[(size_t)d->facet_elem[k]] > kSideMaxNat)
+            return set_error(MGPU_ERR_UNSUPPORTED, "side cluster element with more than 64 adofs per dof type");
+    auto S = std::make_unique<mgpu_sideset>();
+    S->bulk = bulk;
+    S->dim = dim;
+    S->nf = nf;
+    S->nb = nb;
+    S->geom = d->side_geometry;
+    S->generic = true;
+    SideGenView& v = S->gv;
+    v.N = h.n_adofs_type;
+    v.nm = h.nm;
+    v.beta = beta;
+    v.alpha = d->nitsche_alpha;
+    v.neumann = kind == 1;
+    for (int k = 0; k < 3; ++k) v.value[k] = d->value[k];
+    // facets in element order (stable), grouped into clusters
+    std::vector<int32_t> order((size_t)nfc);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) { return d->facet_elem[x] < d->facet_elem[y]; });
+    std::vector<double> box((size_t)h.n_el * 2 * dim);
+    if (h.n_el > 0)
+        HIP_TRY(hipMemcpy(box.data(), bulk->g.ip_box, box.size() * sizeof(double), hipMemcpyDeviceToHost));
+    std::vector<int32_t> cl_elem, cl_fptr(1, 0);
+    std::vector<double> xi((size_t)nfc * nvs * dim), nrm((size_t)nfc * dim), hlen;
+    double A = 0.0;
+    for (int32_t k = 0; k < nfc; ++k) {
+        const int32_t f = order[(size_t)k], e = d->facet_elem[f];
+        if (k == 0 || e != cl_elem.back()) {
+            if (k > 0) {
+                cl_fptr.push_back(k);
+                hlen.push_back(dim == 3 ? std::sqrt(4.0 * A / 3.141592653589793) : A);
+            }
+            cl_elem.push_back(e);
+            A = 0.0;
+        }
+        std::copy(d->facet_xi + (size_t)f * nvs * dim, d->facet_xi + (size_t)(f + 1) * nvs * dim, xi.data() + (size_t)k * nvs * dim);
+        std::copy(d->facet_normal + (size_t)f * dim, d->facet_normal + (size_t)(f + 1) * dim, nrm.data() + (size_t)k * dim);
+        const double* X = d->facet_xi + (size_t)f * nvs * dim;
+        const double* ext = box.data() + (size_t)e * 2 * dim + dim;
+        auto px = [&](int vtx, int q) { return (X[vtx * dim + q] + 1.0) * 0.5 * ext[q]; };
+        if (nvs == 2) {
+            double l2 = 0.0;
+            for (int q = 0; q < dim; ++q) l2 += (px(1, q) - px(0, q)) * (px(1, q) - px(0, q));
+            A += std::sqrt(l2);
+        } else {
+            for (int t = 0; t + 2 < nvs; ++t) {
+                double e1[3], e2[3];
+                for (int q = 0; q < 3; ++q) {
+                    e1[q] = px(t + 1, q) - px(0, q);
+                    e2[q] = px(t + 2, q) - px(0, q);
+                }
+                const double c0 = e1[1] * e2[2] - e1[2] * e2[1], c1 = e1[2] * e2[0] - e1[0] * e2[2],
+                             c2 = e1[0] * e2[1] - e1[1] * e2[0];
+                A += 0.5 * std::sqrt(c0 * c0 + c1 * c1 + c2 * c2);
+            }
+        }
+    }
+    if (nfc > 0) {
+        cl_fptr.push_back(nfc);
+        hlen.push_back(dim == 3 ? std::sqrt(4.0 * A / 3.141592653589793) : A);
+    }
+    const int32_t ncl = (int32_t)cl_elem.size();
+    v.n_clusters = ncl;
+    for (int32_t c = 0; c < ncl; ++c)
+        if (kind == 0 && !(hlen[(size_t)c] > 0.0)) return set_error(MGPU_ERR_GEOMETRY, "a Dirichlet side cluster has zero measure");
+    // adofs, pool offsets and gather lists per cluster
+    std::vector<int64_t> aptr(1, 0), koff(1, 0), roff(1, 0);
+    std::vector<int32_t> adof;
+    std::vector<std::pair<int64_t, int64_t>> kp, rp;
+    for (int32_t c = 0; c < ncl; ++c) {
+        const int32_t e = cl_elem[(size_t)c];
+        const int64_t t0 = h.topo_ptr[(size_t)e], nat = h.topo_ptr[(size_t)e + 1] - t0;
+        for (int64_t r = 0; r < nat; ++r) {
+            const int32_t row = h.topo[(size_t)(t0 + r)];
+            adof.push_back(row);
+            rp.push_back({row, roff.back() + r});
+            if (kind == 1) continue;
+            const int32_t* cb = h.col_idx.data() + h.row_ptr[(size_t)row];
+            const int32_t* ce = h.col_idx.data() + h.row_ptr[(size_t)row + 1];
+            for (int64_t cc = 0; cc < nat; ++cc) {
+                const int32_t col = h.topo[(size_t)(t0 + cc)];
+                const int32_t* it = std::lower_bound(cb, ce, col);
+                if (it == ce || *it != col) return set_error(MGPU_ERR_STATE, "side entry outside the bulk set's pattern");
+                kp.push_back({(int64_t)(it - h.col_idx.data()), koff.back() + r * nat + cc});
+            }
+        }
+        aptr.push_back((int64_t)adof.size());
+        koff.push_back(koff.back() + (kind == 1 ? 0 : nat * nat));
+        roff.push_back(roff.back() + nat);
+    }
+    std::vector<int64_t> kptr, kdst, ksrc, rptr, rdst, rsrc;
+    build_gather(kp, kptr, kdst, ksrc);
+    build_gather(rp, rptr, rdst, rsrc);
+    S->n_kt = (int64_t)kdst.size();
+    S->n_rt = (int64_t)rdst.size();
+    mgpu_status rc;
+    if ((rc = side_upload(S.get(), cl_elem.data(), cl_elem.size(), &v.cl_elem)) != MGPU_OK) return rc;
+    if ((rc = side_upload(S.get(), cl_fptr.data(), cl_fptr.size(), &v.cl_fptr)) != MGPU_OK) return rc;
+    if ((rc = side_upload(S.get(), xi.data(), xi.size(), &v.xi)) != MGPU_OK) return rc;
+    if ((rc = side_upload(S.get(), nrm.data(), nrm.size(), &v.normal)) != MGPU_OK) return rc;
+    if ((rc = side_upload(S.get(), hlen.data(), hlen.size(), &v.hlen)) != MGPU_OK) return rc;
+    if ((rc = side_upload(S.get(), aptr.data(), aptr.size(), &v.cl_aptr)) != MGPU_OK) return rc;
+    if ((rc = side_upload(S.get(), adof.data(), adof.size(), &v.cl_adof)) != MGPU_OK) return rc;
+    if ((rc = side_upload(S.get(), koff.data(), koff.size(), &v.ke_off)) != MGPU_OK) return rc;
+    if ((rc = side_upload(S.get(), roff.data(), roff.size(), &v.re_off)) != MGPU_OK) return rc;
+    if ((rc = side_upload(S.get(), kptr.data(), kptr.size(), &S->kt_ptr)) != MGPU_OK) return rc;
+    if ((rc = side_upload(S.get(), kdst.data(), kdst.size(), &S->kt_dst)) != MGPU_OK) return rc;
+    if ((rc = side_upload(S.get(), ksrc.data(), ksrc.size(), &S->kt_src)) != MGPU_OK) return rc;
+    if ((rc = side_upload(S.get(), rptr.data(), rptr.size(), &S->rt_ptr)) != MGPU_OK) return rc;
+    if ((rc = side_upload(S.get(), rdst.data(), rdst.size(), &S->rt_dst)) != MGPU_OK) return rc;
+    if ((rc = side_upload(S.get(), rsrc.data(), rsrc.size(), &S->rt_src)) != MGPU_OK) return rc;
+    void* p = nullptr;
+    HIP_TRY(hipMalloc(&p, std::max<int64_t>(koff.back(), 1) * sizeof(double)));
+    S->allocs.push_back(p);
+    v.ke = static_cast<double*>(p);
+    HIP_TRY(hipMalloc(&p, std::max<int64_t>(roff.back(), 1) * sizeof(double)));
+    S->allocs.push_back(p);
+    v.re = static_cast<double*>(p);
+    S->v.n_facets = nfc;
+    *out = S.release();
+    return MGPU_OK;
+}
+}  // namespace
+
 extern "C" mgpu_status mgpu_sideset_create(mgpu_set* bulk, const mgpu_sideset_desc* d, mgpu_sideset** out) {
     MGPU_GUARD_BEGIN
     if (!bulk || !d || !out) return set_error(MGPU_ERR_INVALID_ARG, "null argument");
@@ -285,7 +675,6 @@ extern "C" mgpu_status mgpu_sideset_create(mgpu_set* bulk, const mgpu_sideset_de
     const bool geom_ok = dim == 3 ? (d->side_geometry == MGPU_GEOM_TRI3 || d->side_geometry == MGPU_GEOM_QUAD4)
                                   : d->side_geometry == MGPU_GEOM_LINE2;
     if (!geom_ok) return set_error(MGPU_ERR_INVALID_ARG, "side geometry must be TRI3 / QUAD4 in 3D, LINE2 in 2D");
-    if (h.generic) return set_error(MGPU_ERR_UNSUPPORTED, "side sets are implemented for lattice sets with nodal T rows");
     if (d->n_facets < 0 || (d->n_facets > 0 && (!d->facet_elem || !d->facet_xi || !d->facet_normal)))
         return set_error(MGPU_ERR_INVALID_ARG, "bad facet arrays");
     if (kind == 0 && !(d->nitsche_alpha >= 0.0)) return set_error(MGPU_ERR_INVALID_ARG, "nitsche_alpha must be >= 0");
@@ -293,6 +682,7 @@ extern "C" mgpu_status mgpu_sideset_create(mgpu_set* bulk, const mgpu_sideset_de
     const int32_t nfc = d->n_facets;
     for (int32_t k = 0; k < nfc; ++k)
         if (d->facet_elem[k] < 0 || d->facet_elem[k] >= h.n_el) return set_error(MGPU_ERR_INVALID_ARG, "facet element out of range");
+    if (h.generic) return create_generic_side(bulk, d, kind, beta, nvs, out);
     auto S = std::make_unique<mgpu_sideset>();
     S->bulk = bulk;
     S->dim = dim;
@@ -422,6 +812,32 @@ void launch_side(mgpu_sideset* S, const double* u, hipStream_t st) {
         hipLaunchKernelGGL((k_side<2, NF, MGPU_GEOM_LINE2>), g, b, 0, st, S->v, u);
     }
 }
+template <int DIM, int NF, int NB>
+void launch_side_gen_nb(mgpu_sideset* S, const double* u, hipStream_t st) {
+    const dim3 g((unsigned)S->gv.n_clusters), b(256);
+    if constexpr (DIM == 3) {
+        if (S->geom == MGPU_GEOM_TRI3)
+            hipLaunchKernelGGL((k_side_gen<3, NF, MGPU_GEOM_TRI3, NB>), g, b, 0, st, S->gv, S->bulk->g, u);
+        else
+            hipLaunchKernelGGL((k_side_gen<3, NF, MGPU_GEOM_QUAD4, NB>), g, b, 0, st, S->gv, S->bulk->g, u);
+    } else {
+        hipLaunchKernelGGL((k_side_gen<2, NF, MGPU_GEOM_LINE2, NB>), g, b, 0, st, S->gv, S->bulk->g, u);
+    }
+}
+
+void launch_side_gen(mgpu_sideset* S, const double* u, hipStream_t st) {
+    if (S->gv.n_clusters == 0) return;
+    if (S->dim == 3 && S->nb == 27) {
+        if (S->nf == 1) launch_side_gen_nb<3, 1, 27>(S, u, st);
+        else launch_side_gen_nb<3, 3, 27>(S, u, st);
+    } else if (S->dim == 3) {
+        if (S->nf == 1) launch_side_gen_nb<3, 1, 8>(S, u, st);
+        else launch_side_gen_nb<3, 3, 8>(S, u, st);
+    } else {
+        if (S->nf == 1) launch_side_gen_nb<2, 1, 4>(S, u, st);
+        else launch_side_gen_nb<2, 2, 4>(S, u, st);
+    }
+}
 }  // namespace
 
 extern "C" mgpu_status mgpu_sideset_assemble(mgpu_sideset* S, const double* u, double* vals, double* rhs) {
@@ -430,17 +846,21 @@ extern "C" mgpu_status mgpu_sideset_assemble(mgpu_sideset* S, const double* u, d
     if (S->v.n_facets == 0) return MGPU_OK;
     if (!u || !vals || !rhs) return set_error(MGPU_ERR_INVALID_ARG, "null buffer");
     hipStream_t st = S->bulk->stream;
-    if (S->dim == 3 && S->nf == 1) launch_side<3, 1>(S, u, st);
+    if (S->generic) {
+        launch_side_gen(S, u, st);
+    } else if (S->dim == 3 && S->nf == 1) launch_side<3, 1>(S, u, st);
     else if (S->dim == 3) launch_side<3, 3>(S, u, st);
     else if (S->nf == 1) launch_side<2, 1>(S, u, st);
     else launch_side<2, 2>(S, u, st);
     HIP_TRY(hipGetLastError());
+    const double* kpool = S->generic ? S->gv.ke : S->v.ke;
+    const double* rpool = S->generic ? S->gv.re : S->v.re;
     if (S->n_kt > 0)
         hipLaunchKernelGGL(k_side_gather, dim3((unsigned)((S->n_kt + 255) / 256)), dim3(256), 0, st, S->n_kt, S->kt_ptr,
-                           S->kt_dst, S->kt_src, S->v.ke, vals);
+                           S->kt_dst, S->kt_src, kpool, vals);
     if (S->n_rt > 0)
         hipLaunchKernelGGL(k_side_gather, dim3((unsigned)((S->n_rt + 255) / 256)), dim3(256), 0, st, S->n_rt, S->rt_ptr,
-                           S->rt_dst, S->rt_src, S->v.re, rhs);
+                           S->rt_dst, S->rt_src, rpool, rhs);
     HIP_TRY(hipGetLastError());
     return MGPU_OK;
     MGPU_GUARD_END

@@ -830,7 +830,6 @@ extern "C" mgpu_status mgpu_problem_facets(const mgpu_problem* p, int32_t where,
     if (where < MGPU_SIDE_INTERFACE || where > 2 * dim) return mgpu::set_error(MGPU_ERR_INVALID_ARG, "bad facet location");
     const bool ok_geom = dim == 3 ? (geom == MGPU_GEOM_TRI3 || geom == MGPU_GEOM_QUAD4) : geom == MGPU_GEOM_LINE2;
     if (!ok_geom) return mgpu::set_error(MGPU_ERR_INVALID_ARG, "side geometry must be TRI3 / QUAD4 in 3D, LINE2 in 2D");
-    if (p->nb_ip != p->nb) return mgpu::set_error(MGPU_ERR_UNSUPPORTED, "side facets are implemented for HEX8 / QUAD4 IP cells");
     if (!p->refined.empty() && where != MGPU_SIDE_INTERFACE)
         return mgpu::set_error(MGPU_ERR_UNSUPPORTED, "boundary facets are implemented for level-0 lattices");
     auto F = std::make_unique<mgpu_facets>();

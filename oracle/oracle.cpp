@@ -1344,7 +1344,7 @@ void side_assemble(const mgpu_set_desc& d, int iwg, int nvs, int64_t nfc, const 
         }
         // IP cell vertices
         const double* box = d.ip_box + (size_t)e * 2 * D;
-        double X[8][3];
+        double X[27][3];
         for (int v = 0; v < nb; ++v) {
             double xv[3];
             basis_xi(D, nb, v, xv);

@@ -98,7 +98,7 @@ def oracle_desc(Q, s):
     dim = Q.dim
     d.spatial_dim = dim
     d.iwg = Q.iwg
-    d.ip_geometry = M.GEOM_HEX8 if dim == 3 else M.GEOM_QUAD4
+    d.ip_geometry = (M.GEOM_HEX27 if Q.nb == 27 else M.GEOM_HEX8) if dim == 3 else M.GEOM_QUAD4
     d.ig_geometry = M.GEOM_TET4 if dim == 3 else M.GEOM_TRI3
     d.n_elements = Q.n_elements
     d.n_bases = Q.nb

@@ -3,7 +3,8 @@
 * Bulk set plus side sets assembled on the GPU against the oracle's bulk assembly plus its own
   facets and side integrands, at 1e-12 for K and R: Nitsche Dirichlet (SYMMETRIC and UNSYMMETRIC)
   and Neumann, on the interface and on domain boundary faces (TRI3 sub-cell faces, QUAD4 hex faces,
-  2D LINE2 sides), diffusion and elasticity, random u; a second call is bit-identical.
+  2D LINE2 sides), diffusion and elasticity, random u; a second call is bit-identical.  The same
+  on generic-path sets: p = 2 B-splines on HEX8 and QUAD4 cells, HEX27 cells, one THB level.
 * The reference's known answer (UT_MDL_XTK_DiffusionElement.cpp:91-340) assembled on the GPU:
   T = 5 + 20 z at every node of the 1 x 1 x 4 lattice cut at z = 3.1, solved by a direct solver
   and by GPU GMRES (the system is not symmetric).
@@ -52,6 +53,34 @@ CASES = {
 }
 
 
+# side sets on generic-path sets: p = 2 B-splines on HEX8 / QUAD4, HEX27 cells, THB rows
+GENERIC_CASES = {
+    "p2_hex8_diff_sphere": (lambda: M.make_spec(3, (8, 8, 8), spheres=[[0.5, 0.5, 0.5, 0.3]], k_sigma=0.5,
+                                                seed=SEED + 3, bspline_order=2),
+                            [(SS.IWG_SPATIALDIFF_NEUMANN, SS.SIDE_INTERFACE, SS.GEOM_TRI3, 0.0, [1.5]),
+                             (SS.IWG_SPATIALDIFF_DIRICHLET_UNSYMMETRIC_NITSCHE, SS.SIDE_XMIN, SS.GEOM_QUAD4, 3.0, [0.4])]),
+    "hex27_elast_sphere": (lambda: M.make_spec(3, (6, 6, 6), iwg=1, spheres=[[0.5, 0.5, 0.5, 0.3]], bspline_order=2,
+                                               lagrange_order=2),
+                           [(SS.IWG_STRUC_LINEAR_DIRICHLET_SYMMETRIC_NITSCHE, SS.SIDE_INTERFACE, SS.GEOM_TRI3, 10.0,
+                             [0.01, -0.02, 0.005]),
+                            (SS.IWG_STRUC_LINEAR_NEUMANN, SS.SIDE_XMAX, SS.GEOM_QUAD4, 0.0, [1.0, 0.5, -0.25])]),
+    "hex27_diff_face_sphere": (lambda: M.make_spec(3, (6, 6, 6), spheres=[[0.0, 0.5, 0.5, 0.3]], k_sigma=0.3,
+                                                   seed=SEED, bspline_order=2, lagrange_order=2),
+                               [(SS.IWG_SPATIALDIFF_DIRICHLET_SYMMETRIC_NITSCHE, SS.SIDE_XMIN, SS.GEOM_TRI3, 2.0, [1.0]),
+                                (SS.IWG_SPATIALDIFF_DIRICHLET_SYMMETRIC_NITSCHE, SS.SIDE_XMIN, SS.GEOM_QUAD4, 2.0,
+                                 [1.0])]),
+    "thb_elast_sphere": (lambda: M.make_spec(3, (4, 4, 4), iwg=1, spheres=[[0.5, 0.5, 0.5, 0.3]], bspline_order=2,
+                                             lagrange_order=2, refine_levels=1),
+                         [(SS.IWG_STRUC_LINEAR_DIRICHLET_UNSYMMETRIC_NITSCHE, SS.SIDE_INTERFACE, SS.GEOM_TRI3, 5.0,
+                           [0.0, 0.1, 0.0])]),
+    "p2_quad4_elast_circle": (lambda: M.make_spec(2, (16, 16), iwg=1, spheres=[[0.5, 0.5, 0.0, 0.25]],
+                                                  bspline_order=2),
+                              [(SS.IWG_STRUC_LINEAR_DIRICHLET_SYMMETRIC_NITSCHE, SS.SIDE_XMIN, SS.GEOM_LINE2, 100.0,
+                                [0.0, 0.0]),
+                               (SS.IWG_STRUC_LINEAR_NEUMANN, SS.SIDE_INTERFACE, SS.GEOM_LINE2, 0.0, [0.1, -0.3])]),
+}
+
+
 def _gpu_sets(spec, keep, sides):
     P = M.Problem(spec, keep)
     S = M.Set(P.desc, keepalive=P)
@@ -69,6 +98,23 @@ def test_sidesets_match_oracle(case):
     P, S, side_sets = _gpu_sets(spec, keep, sides)
     assert all(ss.n_facets > 0 for ss in side_sets), [ss.n_facets for ss in side_sets]
     u = np.random.default_rng(SEED + 90).uniform(-1.0, 1.0, S.n_rows)
+    vals, R = SS.assemble(S, side_sets, u)
+    Q, s, vo, Ro = oracle_side_system(spec, sides, u)
+    rp, ci = S.pattern()
+    assert np.array_equal(rp, s["row_ptr"]) and np.array_equal(ci, s["col_idx"])
+    print(case, "facets", [ss.n_facets for ss in side_sets], "|dK|/|K| %.2e |dR|/|R| %.2e" % (rel(vals, vo), rel(R, Ro)))
+    assert rel(vals, vo) <= 1e-12 and rel(R, Ro) <= 1e-12
+    v2, R2 = SS.assemble(S, side_sets, u)
+    assert np.array_equal(v2.view(np.uint64), vals.view(np.uint64)) and np.array_equal(R2.view(np.uint64), R.view(np.uint64))
+
+
+@pytest.mark.parametrize("case", sorted(GENERIC_CASES))
+def test_generic_sidesets_match_oracle(case):
+    mk, sides = GENERIC_CASES[case]
+    spec, keep = mk()
+    P, S, side_sets = _gpu_sets(spec, keep, sides)
+    assert all(ss.n_facets > 0 for ss in side_sets), [ss.n_facets for ss in side_sets]
+    u = np.random.default_rng(SEED + 91).uniform(-1.0, 1.0, S.n_rows)
     vals, R = SS.assemble(S, side_sets, u)
     Q, s, vo, Ro = oracle_side_system(spec, sides, u)
     rp, ci = S.pattern()

@@ -166,3 +166,75 @@ def test_known_answer_xtk_cut_diffusion_5_plus_20z():
     z = np.repeat(np.arange(5.0), 4)
     assert nr == 20
     assert np.abs(T - (5.0 + 20.0 * z)).max() < 1e-8, T
+
+
+GENERIC = {
+    "p2_hex8_diff": lambda: M.make_spec(3, (5, 5, 5), spheres=[[0.5, 0.5, 0.5, 0.3]], k_sigma=0.4, seed=SEED,
+                                        bspline_order=2),
+    "p2_hex27_elast": lambda: M.make_spec(3, (4, 4, 4), iwg=1, spheres=[[0.5, 0.5, 0.5, 0.3]], bspline_order=2,
+                                          lagrange_order=2),
+    "thb_elast": lambda: M.make_spec(3, (4, 4, 4), iwg=1, spheres=[[0.5, 0.5, 0.5, 0.3]], bspline_order=2,
+                                     lagrange_order=2, refine_levels=1),
+    "p2_quad4_elast": lambda: M.make_spec(2, (8, 8), iwg=1, spheres=[[0.5, 0.5, 0.0, 0.25]], bspline_order=2),
+}
+
+
+@pytest.mark.parametrize("case", sorted(GENERIC))
+def test_oracle_side_sets_on_generic_rows(case):
+    """Side IWGs over T rows with several entries (p = 2 B-splines, HEX27 cells, THB): a Neumann
+    load sums to -t_N times the interface measure per field (partition of unity of the splines);
+    the SYMMETRIC Nitsche matrix is symmetric, the UNSYMMETRIC one is not; a field equal to u_D
+    (the same coefficient on every spline) has zero Nitsche residual."""
+    spec, keep = GENERIC[case]()
+    Q = O.OracleProblem(spec)
+    s = Q.setup()
+    d, k = _desc_of(Q, s)
+    rp, ci = s["row_ptr"], s["col_idx"]
+    nr = Q.n_rows
+    dim = spec.dim
+    nvs = 3 if dim == 3 else 2
+    nf = dim if spec.iwg == 1 else 1
+    e, x, n = O.facets(Q, SS.SIDE_INTERFACE, nvs)
+    assert len(e) > 0
+    # interface measure from the facets and their elements' boxes
+    box = s["ip_box"].reshape(-1, 2 * dim)[e]
+    X = box[:, None, :dim] + 0.5 * (x + 1.0) * box[:, None, dim:]
+    if nvs == 2:
+        area = np.linalg.norm(X[:, 1] - X[:, 0], axis=1).sum()
+    else:
+        area = 0.5 * np.linalg.norm(np.cross(X[:, 1] - X[:, 0], X[:, 2] - X[:, 0]), axis=1).sum()
+    t = [1.5, -0.5, 2.0][:nf]
+    neu = SS.IWG_STRUC_LINEAR_NEUMANN if nf > 1 else SS.IWG_SPATIALDIFF_NEUMANN
+    vals, R = np.zeros(Q.nnz), np.zeros(nr)
+    O.side_assemble(d, rp, ci, neu, nvs, e, x, n, 0.0, t, np.zeros(nr), vals, R)
+    N = nr // nf
+    for f, tf in enumerate(t):
+        assert abs(R[f * N:(f + 1) * N].sum() + tf * area) < 1e-12 * max(1.0, area)
+    sym = SS.IWG_STRUC_LINEAR_DIRICHLET_SYMMETRIC_NITSCHE if nf > 1 else SS.IWG_SPATIALDIFF_DIRICHLET_SYMMETRIC_NITSCHE
+    value = [0.7, -0.2, 0.4][:nf]
+    u = np.concatenate([np.full(N, v) for v in value])
+    for unsym in (False, True):
+        code = sym + (1 if unsym else 0)
+        vals, R = np.zeros(Q.nnz), np.zeros(nr)
+        O.side_assemble(d, rp, ci, code, nvs, e, x, n, 2.0, value, np.zeros(nr), vals, R)
+        K = sp.csr_matrix((vals, ci, rp), shape=(nr, nr))
+        asym = abs(K - K.T).max() / abs(K).max()
+        assert (asym > 1e-3) if unsym else (asym < 1e-13), (unsym, asym)
+        v2, R2 = np.zeros(Q.nnz), np.zeros(nr)
+        O.side_assemble(d, rp, ci, code, nvs, e, x, n, 2.0, value, u, v2, R2)
+        assert np.abs(R2).max() < 1e-12 * np.abs(R).max()
+
+
+def test_hex27_facets_equal_hex8_facets():
+    """The facets of HEX27 problems are those of the same cut on HEX8 cells (the cut is that of the
+    cell box)."""
+    for where, geom in ((SS.SIDE_INTERFACE, SS.GEOM_TRI3), (SS.SIDE_XMIN, SS.GEOM_QUAD4), (SS.SIDE_XMIN, SS.GEOM_TRI3)):
+        got = []
+        for lag in (1, 2):
+            spec, keep = M.make_spec(3, (5, 5, 5), iwg=1, spheres=[[0.0, 0.5, 0.5, 0.35]], bspline_order=2,
+                                     lagrange_order=lag)
+            P = M.Problem(spec, keep)
+            got.append(SS.facets(P, where, geom))
+        for a, b in zip(*got):
+            assert np.array_equal(np.asarray(a).view(np.uint8), np.asarray(b).view(np.uint8))
+
