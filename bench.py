@@ -230,6 +230,61 @@ def run_hex27(args):
     print(json.dumps(out), flush=True)
 
 
+def run_hex27_blocks(args):
+    """C3's whole domain on one GPU: the n^3-base lattice of the unit cube (one THB level on the
+    intersected cells, HEX27 cells, p = 2 rows, gyroid of period 4 cells) split into blocks^3
+    sub-domains of (n / blocks)^3 base cells, each assembled as its own set and freed before the
+    next: the blocks a multi-GPU run puts one per GPU, here in sequence on one GPU.  Rows of adofs on
+    block faces hold each block's partial sums (a global matrix adds them).  The time per call is
+    the sum over the blocks.  One JSON line."""
+    n, nbk = args.n, args.blocks
+    if n % nbk:
+        raise SystemExit("--n must be a multiple of --blocks")
+    m = n // nbk
+    hcell = 1.0 / n
+    totals = {"fp64": 0.0, "mixed": 0.0}
+    elements = cut = subtets = nnz = 0
+    t_setup = 0.0
+    for bk in range(nbk ** 3):
+        ijk = (bk % nbk, (bk // nbk) % nbk, bk // (nbk * nbk))
+        lo = tuple(c * m * hcell for c in ijk)
+        hi = tuple(l + m * hcell for l in lo)
+        spec, keep = M.make_spec(3, (m, m, m), lo=lo, hi=hi, iwg=M.IWG_STRUC_LINEAR_BULK, ls_kind=M.LS_GYROID,
+                                 gyroid_scale=2 * np.pi * n / 4.0, bspline_order=2, lagrange_order=2,
+                                 refine_levels=args.refine)
+        t0 = time.time()
+        P = M.Problem(spec, keep)
+        S = M.Set(P.desc, keepalive=P)
+        t_setup += time.time() - t0
+        u = np.random.default_rng(SEED + 27 + bk).uniform(-1.0, 1.0, S.n_rows)
+        ub, vb, rb = M.DeviceBuffer.from_array(u), M.DeviceBuffer(8 * S.nnz), M.DeviceBuffer(8 * S.n_rows)
+        line = []
+        for prec, name in ((M.PREC_FP64, "fp64"), (M.PREC_MIXED, "mixed")):
+            S.options(prec, M.QUAD_SUBDIVISION)
+            per, _ = S.time_assemble(ub, vb, rb, max(2, args.steps // 5))
+            totals[name] += per
+            line.append("%s %.1f ms" % (name, per))
+        elements += P.n_elements
+        cut += P.n_cut
+        subtets += P.n_ig
+        nnz += S.nnz
+        log("block %d/%d %s: elements %d, cut %d, nnz %d, set-up %.1fs; %s" % (
+            bk + 1, nbk ** 3, ijk, P.n_elements, P.n_cut, S.nnz, time.time() - t0, ", ".join(line)))
+        del ub, vb, rb, S, P
+    out = {"metric": "elements/s for K+R assembly, HEX27 p=2 elasticity (gyroid cut)%s, whole domain as blocks" % (
+               ", one THB level" if args.refine else ""),
+           "unit": "elements/s", "n_gpus": 1, "steps": args.steps, "higher_is_better": True, "data": "synthetic",
+           "config": {"workload": "%d^3 HEX27 base lattice%s, p=2 B-splines, gyroid period 4 cells, as %d^3 blocks "
+                                  "of %d^3 base cells assembled in sequence" % (
+                                      n, ", one THB level on the intersected cells" if args.refine else "", nbk, m),
+                      "elements": elements, "cut_elements": cut, "ig_subcells": subtets, "nnz_sum_over_blocks": nnz,
+                      "host_setup_s": t_setup},
+           "value": elements / (totals["mixed"] * 1e-3), "dtype": "f32+f64",
+           "fp64": {"value": elements / (totals["fp64"] * 1e-3), "ms_per_call": totals["fp64"]},
+           "mixed": {"value": elements / (totals["mixed"] * 1e-3), "ms_per_call": totals["mixed"]}}
+    print(json.dumps(out), flush=True)
+
+
 def run_lattice(args):
     """One lattice set (stencil path) at a parity configuration: c1 = 64 x 64 QUAD4 plane-stress
     elasticity around a circular hole, c2 = 128^3 HEX8 diffusion around a spherical hole with a
@@ -357,6 +412,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--refine", type=int, default=0, choices=(0, 1),
                     help="hex27 workload: one THB level on the intersected cells")
+    ap.add_argument("--blocks", type=int, default=1,
+                    help="hex27 workload: split the n^3 domain into blocks^3 sets assembled in sequence")
     ap.add_argument("--workload", choices=("variants", "hex27", "c1", "c2"), default="variants",
                     help="variants: the headline batch (default); hex27: the quadratic HEX27 configuration; "
                          "c1: 64x64 QUAD4 plane-stress elasticity; c2: 128^3 HEX8 diffusion")
@@ -369,7 +426,10 @@ def main():
         print("bench.py: no GPU visible", file=sys.stderr)
         sys.exit(1)
     if args.workload == "hex27":
-        run_hex27(args)
+        if args.blocks > 1:
+            run_hex27_blocks(args)
+        else:
+            run_hex27(args)
         return
     if args.workload in ("c1", "c2"):
         run_lattice(args)
