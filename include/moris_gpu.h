@@ -102,6 +102,16 @@ typedef struct mgpu_problem_spec {
                                   level-1 cells; truncated hierarchical quadratic B-splines
                                   (cl_HMR_T_Matrix_Base.cpp:127-258); such sets have no lattice
                                   hint and take the generic path                                */
+    int32_t enrich;            /* 0: one element per material cell (a cut cell's material
+                                  sub-cells form one element; neighbouring cells share bases);
+                                  1: XTK basis-cluster enrichment (cl_XTK_Enrichment.cpp:276-400,
+                                  level-0 lattices): a cut cell's material sub-cells split into
+                                  subphases (face-connected components), one element each, and
+                                  every basis is replicated once per connected component of the
+                                  material subphases in its support, so material pieces that do
+                                  not touch inside a basis' support share no adof.  Sets with a
+                                  replicated basis or two elements in one cell take the generic
+                                  path.                                                          */
 } mgpu_problem_spec;
 
 /* Uniform lattice the elements of a set sit on (HMR level 0).  n[0] == 0: no lattice. */

@@ -53,6 +53,7 @@ class ProblemSpec(ctypes.Structure):
         ("bspline_order", ctypes.c_int32),
         ("lagrange_order", ctypes.c_int32),
         ("refine_levels", ctypes.c_int32),
+        ("enrich", ctypes.c_int32),
     ]
 
 
@@ -163,7 +164,8 @@ def _ptr(a, ctype):
 
 def make_spec(dim, n, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), iwg=IWG_SPATIALDIFF_BULK, spheres=None,
               ls_kind=None, gyroid_scale=1.0, plane=(0.0, 0.0, 0.0, 0.0), material_phase=1, E=1.0, nu=0.3,
-              k_mu=0.0, k_sigma=0.0, seed=20261202, bspline_order=1, lagrange_order=1, refine_levels=0):
+              k_mu=0.0, k_sigma=0.0, seed=20261202, bspline_order=1, lagrange_order=1, refine_levels=0,
+              enrich=0):
     """Build a ProblemSpec; returns (spec, keepalive) -- keep the second value alive with the spec."""
     s = ProblemSpec()
     s.dim = dim
@@ -188,6 +190,7 @@ def make_spec(dim, n, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), iwg=IWG_SPATIALDIF
     s.bspline_order = bspline_order
     s.lagrange_order = lagrange_order
     s.refine_levels = refine_levels
+    s.enrich = enrich
     return s, keep
 
 

@@ -21,6 +21,7 @@ CXX = os.environ.get("CXX", "g++")
 
 HOST_SOURCES = [
     "setup/problem.cpp",
+    "setup/enrich.cpp",
     "set/set_host.cpp",
     "learned/dataset.cpp",
 ]

@@ -315,7 +315,38 @@ mgpu_status build_set_host(const mgpu_set_desc& d, SetHost& h, std::string* err)
     for (int64_t r = 0; r < n_trows && nodal; ++r)
         nodal = (d.t_ptr[r + 1] - d.t_ptr[r] == 1) && d.t_weight[d.t_ptr[r]] == 1.0;
     const bool has_lattice = d.lattice.n[0] > 0 && d.elem_cell != nullptr;
-    if (!nodal || !has_lattice || hex27) {
+    // the stencil also needs one element per cell and one adof per lattice node, ascending in node
+    // order; enriched sets (several subphase elements in a cell, a node replicated per level) and
+    // other numberings take the generic path
+    bool one_per_node = true;
+    if (nodal && has_lattice && !hex27) {
+        const int64_t lx = d.lattice.n[0], ly = d.lattice.n[1], lz = d.spatial_dim == 3 ? d.lattice.n[2] : 1;
+        const int64_t cells = lx * ly * lz;
+        std::vector<char> seen((size_t)std::max<int64_t>(cells, 0), 0);
+        std::vector<int32_t> nadof((size_t)(d.spatial_dim == 3 ? (lx + 1) * (ly + 1) * (lz + 1) : (lx + 1) * (ly + 1)), -1);
+        for (int32_t e = 0; e < d.n_elements && one_per_node; ++e) {
+            const int64_t c = d.elem_cell[e];
+            if (c < 0 || c >= cells) break;   // reported by the stencil set-up
+            if (seen[(size_t)c]) { one_per_node = false; break; }
+            seen[(size_t)c] = 1;
+            const int64_t i = c % lx, j = (c / lx) % ly, k = c / (lx * ly);
+            for (int b = 0; b < nb; ++b) {
+                const int* off = d.spatial_dim == 3 ? kHex8Off[b] : kQuad4Off[b];
+                const int64_t nid = d.spatial_dim == 3 ? (i + off[0]) + (lx + 1) * ((j + off[1]) + (ly + 1) * (k + off[2]))
+                                                       : (i + off[0]) + (lx + 1) * (j + off[1]);
+                const int32_t a = d.t_adof[d.t_ptr[(int64_t)e * nb + b]];
+                if (nadof[(size_t)nid] >= 0 && nadof[(size_t)nid] != a) { one_per_node = false; break; }
+                nadof[(size_t)nid] = a;
+            }
+        }
+        int32_t last = -1;
+        for (size_t nid = 0; nid < nadof.size() && one_per_node; ++nid) {
+            if (nadof[nid] < 0) continue;
+            if (nadof[nid] <= last) one_per_node = false;
+            last = nadof[nid];
+        }
+    }
+    if (!nodal || !has_lattice || hex27 || !one_per_node) {
         h.generic = true;
         const mgpu_status rc = build_generic(d, h, err);
         // the HEX27 kernel stages T and T^T K_p T blocks of 27 x na_t in LDS beside K_p

@@ -478,6 +478,8 @@ extern "C" mgpu_status mgpu_setup_build(const mgpu_problem_spec* spec, void* str
         return set_error(MGPU_ERR_UNSUPPORTED, "iwg not supported");
     if (spec->lagrange_order > 1 || spec->refine_levels != 0)
         return set_error(MGPU_ERR_UNSUPPORTED, "the GPU set-up covers HEX8 / QUAD4 IP cells without refinement");
+    if (spec->enrich != 0)
+        return set_error(MGPU_ERR_UNSUPPORTED, "the GPU set-up numbers one element per cell (no enrichment)");
     const int order = spec->bspline_order <= 1 ? 1 : spec->bspline_order;
     if (order > 2) return set_error(MGPU_ERR_UNSUPPORTED, "bspline_order must be 1 or 2");
     for (int d = 0; d < spec->dim; ++d)

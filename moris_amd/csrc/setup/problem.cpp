@@ -17,6 +17,7 @@
 //                                  the cut is that of the cell's box, as for HEX8
 #include <algorithm>
 #include <memory>
+#include <numeric>
 #include <cstring>
 #include <string>
 #include <thread>
@@ -25,6 +26,7 @@
 #include "../common/fem_tables.hpp"
 #include "../common/levelset.hpp"
 #include "../common/xtk_cut.hpp"
+#include "enrich.hpp"
 #include "internal.hpp"
 
 using namespace mgpu;
@@ -43,7 +45,9 @@ struct mgpu_problem {
     int64_t n_cut = 0, n_void = 0;
 
     std::vector<double> node_phi;
-    std::vector<int32_t> cell_elem;
+    std::vector<int32_t> cell_elem;      // first element of a level-0 cell, -1 if none
+    std::vector<int32_t> cell_nelem;     // elements of a level-0 cell (subphases when enriched)
+    std::vector<uint8_t> elem_corner;    // enrichment: strictly-material corners per element
 
     std::vector<int32_t> elem_cell;
     std::vector<int8_t> elem_level;      // 0, or 1 for a THB level-1 cell
@@ -117,6 +121,10 @@ struct CellOut {
     bool cut;
     std::vector<double> xi;   // sub-cell vertex parametric coords
     int n_sub;
+    // enrichment: sub-cells per subphase (xi grouped by subphase) and each subphase's
+    // strictly-material corners
+    std::vector<int32_t> sp_count;
+    std::vector<uint8_t> sp_corner;
 };
 
 // Process a contiguous range of cells; material cells appended in order.
@@ -212,6 +220,7 @@ int process_cells(const mgpu_problem& p, int64_t c0, int64_t c1, std::vector<Cel
         }
         if (n_sub == 0) { ++n_void; continue; }
         co.n_sub = n_sub;
+        if (p.spec.enrich) split_subphases(p.dim, n_sub, co.xi.data(), regions, phase, co.sp_count, co.sp_corner);
         out.push_back(std::move(co));
     }
     return MGPU_OK;
@@ -382,15 +391,24 @@ extern "C" mgpu_status mgpu_problem_build(const mgpu_problem_spec* spec, mgpu_pr
         if (rcs[t] != MGPU_OK) return mgpu::set_error(rcs[t], errs[t]);
 
     // 3. element arrays
+    // with enrichment a cut cell contributes one element per material subphase
+    const bool enrich = spec->enrich != 0;
+    if (enrich && thb) return mgpu::set_error(MGPU_ERR_UNSUPPORTED, "enrichment is implemented for level-0 lattices");
+    auto n_elems_of = [&](const CellOut& co) { return enrich && co.cut ? (int64_t)co.sp_count.size() : (int64_t)1; };
     int64_t n_el = 0, n_ig = 0;
     for (int t = 0; t < T; ++t) {
-        n_el += (int64_t)parts[t].size();
         p->n_void += voids[t];
-        for (const auto& lo : parts[t]) n_ig += lo.co.n_sub;
+        for (const auto& lo : parts[t]) {
+            n_el += n_elems_of(lo.co);
+            n_ig += lo.co.n_sub;
+        }
     }
+    if (n_el >= INT32_MAX) return mgpu::set_error(MGPU_ERR_INVALID_ARG, "too many elements");
     const int d2 = 2 * p->dim;
     const int nv_ig = p->dim == 3 ? 4 : 3;
     p->cell_elem.assign((size_t)p->n_cells, -1);
+    p->cell_nelem.assign((size_t)p->n_cells, 0);
+    if (enrich) p->elem_corner.resize((size_t)n_el);
     p->elem_cell.resize((size_t)n_el);
     p->elem_level.resize((size_t)n_el);
     p->elem_lcell.resize((size_t)n_el);
@@ -412,45 +430,55 @@ extern "C" mgpu_status mgpu_problem_build(const mgpu_problem_spec* spec, mgpu_pr
             const int ijk[3] = {i, j, k};
             // level-0 cell of the element (its parent for a level-1 cell)
             const int64_t c0 = lo.level == 0 ? c : (i / 2) + (int64_t)p->n[0] * ((j / 2) + (int64_t)p->n[1] * (k / 2));
-            if (lo.level == 0) p->cell_elem[(size_t)c] = (int32_t)e;
-            p->elem_cell[(size_t)e] = (int32_t)c0;
-            p->elem_level[(size_t)e] = (int8_t)lo.level;
-            p->elem_lcell[(size_t)e] = c;
-            for (int d = 0; d < p->dim; ++d) {
-                p->ip_box[(size_t)e * d2 + d] = L.lo[d] + (double)ijk[d] * L.h[d];
-                p->ip_box[(size_t)e * d2 + p->dim + d] = L.h[d];
+            if (lo.level == 0) {
+                p->cell_elem[(size_t)c] = (int32_t)e;
+                p->cell_nelem[(size_t)c] = (int32_t)n_elems_of(co);
             }
-            if (p->nb_ip == 27) {
-                // level set at the 27 HEX27 nodes (the cut itself uses the 8 corner values)
-                for (int v = 0; v < 27; ++v) {
-                    double x[3];
-                    for (int d = 0; d < 3; ++d) x[d] = L.lo[d] + ((double)ijk[d] + 0.5 * kHex27Half[v][d]) * L.h[d];
-                    p->vertex_phi[(size_t)e * 27 + v] = p->ls(x);
+            const int64_t n_sp = n_elems_of(co);
+            for (int64_t q = 0; q < n_sp; ++q, ++e) {
+                if (enrich) p->elem_corner[(size_t)e] = co.cut ? co.sp_corner[(size_t)q] : (uint8_t)((1u << (1 << p->dim)) - 1u);
+                p->elem_cell[(size_t)e] = (int32_t)c0;
+                p->elem_level[(size_t)e] = (int8_t)lo.level;
+                p->elem_lcell[(size_t)e] = c;
+                for (int d = 0; d < p->dim; ++d) {
+                    p->ip_box[(size_t)e * d2 + d] = L.lo[d] + (double)ijk[d] * L.h[d];
+                    p->ip_box[(size_t)e * d2 + p->dim + d] = L.h[d];
                 }
-            } else {
-                for (int v = 0; v < p->nb; ++v) {
-                    const int64_t nid = p->dim == 3 ? node_index(L, i + kHex8Off[v][0], j + kHex8Off[v][1], k + kHex8Off[v][2])
-                                                    : node_index(L, i + kQuad4Off[v][0], j + kQuad4Off[v][1], 0);
-                    p->vertex_phi[(size_t)e * p->nb + v] = L.node_phi[(size_t)nid];
+                if (p->nb_ip == 27) {
+                    // level set at the 27 HEX27 nodes (the cut itself uses the 8 corner values)
+                    for (int v = 0; v < 27; ++v) {
+                        double x[3];
+                        for (int d = 0; d < 3; ++d) x[d] = L.lo[d] + ((double)ijk[d] + 0.5 * kHex27Half[v][d]) * L.h[d];
+                        p->vertex_phi[(size_t)e * 27 + v] = p->ls(x);
+                    }
+                } else {
+                    for (int v = 0; v < p->nb; ++v) {
+                        const int64_t nid = p->dim == 3 ? node_index(L, i + kHex8Off[v][0], j + kHex8Off[v][1], k + kHex8Off[v][2])
+                                                        : node_index(L, i + kQuad4Off[v][0], j + kQuad4Off[v][1], 0);
+                        p->vertex_phi[(size_t)e * p->nb + v] = L.node_phi[(size_t)nid];
+                    }
+                }
+                if (diff) {
+                    // per-element conductivity seeded by the cell id (level-1 cells: n_cells + fine id)
+                    const uint64_t sid = lo.level == 0 ? (uint64_t)c : (uint64_t)(p->n_cells + c);
+                    p->material[(size_t)e] = spec->k_sigma == 0.0
+                                                 ? std::exp(spec->k_mu)
+                                                 : std::exp(spec->k_mu + spec->k_sigma * normal_at(spec->seed, sid));
+                } else {
+                    p->material[(size_t)e * 2] = spec->E;
+                    p->material[(size_t)e * 2 + 1] = spec->nu;
+                }
+                p->ig_ptr[(size_t)e] = (int32_t)ig;
+                if (co.cut) {
+                    // the subphase's sub-cells (all of them without enrichment)
+                    const int64_t first = enrich ? std::accumulate(co.sp_count.begin(), co.sp_count.begin() + q, (int64_t)0) : 0;
+                    const int64_t cnt = enrich ? co.sp_count[(size_t)q] : co.n_sub;
+                    std::memcpy(&p->ig_xi[(size_t)ig * nv_ig * p->dim], co.xi.data() + (size_t)first * nv_ig * p->dim,
+                                (size_t)cnt * nv_ig * p->dim * sizeof(double));
+                    ig += cnt;
+                    ++p->n_cut;
                 }
             }
-            if (diff) {
-                // per-element conductivity seeded by the cell id (level-1 cells: n_cells + fine id)
-                const uint64_t sid = lo.level == 0 ? (uint64_t)c : (uint64_t)(p->n_cells + c);
-                p->material[(size_t)e] = spec->k_sigma == 0.0
-                                             ? std::exp(spec->k_mu)
-                                             : std::exp(spec->k_mu + spec->k_sigma * normal_at(spec->seed, sid));
-            } else {
-                p->material[(size_t)e * 2] = spec->E;
-                p->material[(size_t)e * 2 + 1] = spec->nu;
-            }
-            p->ig_ptr[(size_t)e] = (int32_t)ig;
-            if (co.cut) {
-                std::memcpy(&p->ig_xi[(size_t)ig * nv_ig * p->dim], co.xi.data(), co.xi.size() * sizeof(double));
-                ig += co.n_sub;
-                ++p->n_cut;
-            }
-            ++e;
         }
     }
     p->ig_ptr[(size_t)n_el] = (int32_t)ig;
@@ -474,6 +502,27 @@ extern "C" mgpu_status mgpu_problem_build(const mgpu_problem_spec* spec, mgpu_pr
         ijk[2] = p->dim == 3 ? (int)(c / ((int64_t)p->n[0] * p->n[1])) : 0;
     };
     const size_t n_rows = (size_t)n_el * p->nb_ip;
+    // enrichment: the enriched adof of every (element, basis of its cell)
+    std::vector<int32_t> eadof;
+    int32_t n_eadofs = 0;
+    const int escheme = order == 2 ? kEnrichBspline2 : p->nb_ip == 27 ? kEnrichLagrange27 : kEnrichLagrange1;
+    const int enloc = enrich_local_count(p->dim, escheme);
+    if (enrich) {
+        EnrichInput ein{};
+        ein.dim = p->dim;
+        ein.scheme = escheme;
+        for (int d = 0; d < 3; ++d) ein.n[d] = p->n[d];
+        ein.cell_first = p->cell_elem.data();
+        ein.cell_count = p->cell_nelem.data();
+        ein.elem_corner = p->elem_corner.data();
+        ein.n_el = n_el;
+        std::string eerr;
+        if (!enrich_numbering(ein, eadof, &n_eadofs, &eerr)) return mgpu::set_error(MGPU_ERR_UNSUPPORTED, eerr);
+    }
+    // enriched adof of the basis at per-axis offsets off[] from the element's cell
+    auto enriched = [&](int64_t ee, const int* off) {
+        return eadof[(size_t)ee * enloc + enrich_local_index(p->dim, escheme, off)];
+    };
     if (thb) {
         // Truncated hierarchical quadratic B-splines, two levels (T_Matrix_Base::
         // calculate_truncated_t_matrix, cl_HMR_T_Matrix_Base.cpp:127-258, applied at the Lagrange
@@ -625,7 +674,7 @@ extern "C" mgpu_status mgpu_problem_build(const mgpu_problem_spec* spec, mgpu_pr
         int32_t next = 0;
         for (int64_t nid = 0; nid < p->n_nodes; ++nid)
             if (node_adof[(size_t)nid] == 0) node_adof[(size_t)nid] = next++;
-        p->n_adofs_per_type = next;
+        p->n_adofs_per_type = enrich ? n_eadofs : next;
         p->t_ptr.resize(n_rows + 1);
         p->t_adof.resize(n_rows);
         p->t_weight.assign(n_rows, 1.0);
@@ -633,11 +682,13 @@ extern "C" mgpu_status mgpu_problem_build(const mgpu_problem_spec* spec, mgpu_pr
             int ijk[3];
             elem_ijk(ee, ijk);
             for (int v = 0; v < p->nb; ++v) {
-                const int64_t nid = p->dim == 3 ? node_index(*p, ijk[0] + kHex8Off[v][0], ijk[1] + kHex8Off[v][1], ijk[2] + kHex8Off[v][2])
-                                                : node_index(*p, ijk[0] + kQuad4Off[v][0], ijk[1] + kQuad4Off[v][1], 0);
+                const int* off = p->dim == 3 ? kHex8Off[v] : kQuad4Off[v];
+                const int64_t nid = p->dim == 3 ? node_index(*p, ijk[0] + off[0], ijk[1] + off[1], ijk[2] + off[2])
+                                                : node_index(*p, ijk[0] + off[0], ijk[1] + off[1], 0);
                 const size_t r = (size_t)ee * p->nb + v;
                 p->t_ptr[r] = (int32_t)r;
-                p->t_adof[r] = node_adof[(size_t)nid];
+                const int off3[3] = {off[0], off[1], p->dim == 3 ? off[2] : 0};
+                p->t_adof[r] = enrich ? enriched(ee, off3) : node_adof[(size_t)nid];
             }
         }
         p->t_ptr[n_rows] = (int32_t)n_rows;
@@ -659,7 +710,7 @@ extern "C" mgpu_status mgpu_problem_build(const mgpu_problem_spec* spec, mgpu_pr
         int32_t next = 0;
         for (auto& a : node_adof)
             if (a == 0) a = next++;
-        p->n_adofs_per_type = next;
+        p->n_adofs_per_type = enrich ? n_eadofs : next;
         p->t_ptr.resize(n_rows + 1);
         p->t_adof.resize(n_rows);
         p->t_weight.assign(n_rows, 1.0);
@@ -669,7 +720,7 @@ extern "C" mgpu_status mgpu_problem_build(const mgpu_problem_spec* spec, mgpu_pr
                 basis_half(ijk[0], ijk[1], ijk[2], v, g);
                 const size_t r = (size_t)ee * 27 + v;
                 p->t_ptr[r] = (int32_t)r;
-                p->t_adof[r] = node_adof[gid(g)];
+                p->t_adof[r] = enrich ? enriched(ee, kHex27Half[v]) : node_adof[gid(g)];
             }
         }
         p->t_ptr[n_rows] = (int32_t)n_rows;
@@ -721,22 +772,33 @@ extern "C" mgpu_status mgpu_problem_build(const mgpu_problem_spec* spec, mgpu_pr
         int32_t next = 0;
         for (auto& a : spline_adof)
             if (a == 0) a = next++;
-        p->n_adofs_per_type = next;
+        p->n_adofs_per_type = enrich ? n_eadofs : next;
         if (n_entries >= (size_t)INT32_MAX) return mgpu::set_error(MGPU_ERR_INVALID_ARG, "too many T entries");
         p->t_ptr.resize(n_rows + 1);
         p->t_adof.resize(n_entries);
         p->t_weight.resize(n_entries);
         size_t pos = 0;
-        for (int64_t ee = 0; ee < n_el; ++ee)
+        for (int64_t ee = 0; ee < n_el; ++ee) {
+            int ijk[3];
+            elem_ijk(ee, ijk);
             for (int v = 0; v < p->nb_ip; ++v) {
                 const int nq = row(ee, v, sp, wt);
                 p->t_ptr[(size_t)ee * p->nb_ip + v] = (int32_t)pos;
                 for (int q = 0; q < nq; ++q) {
-                    p->t_adof[pos] = spline_adof[(size_t)sp[q]];
+                    if (enrich) {
+                        // the spline's offsets from the element's cell (0..2 per axis)
+                        const int64_t sidx[3] = {sp[q] % sx, (sp[q] / sx) % sy, sp[q] / (sx * sy)};
+                        int off[3] = {0, 0, 0};
+                        for (int d = 0; d < p->dim; ++d) off[d] = (int)(sidx[d] - ijk[d]);
+                        p->t_adof[pos] = enriched(ee, off);
+                    } else {
+                        p->t_adof[pos] = spline_adof[(size_t)sp[q]];
+                    }
                     p->t_weight[pos] = wt[q];
                     ++pos;
                 }
             }
+        }
         p->t_ptr[n_rows] = (int32_t)pos;
     }
 

@@ -773,6 +773,209 @@ void thb_rows(OracleProblem& p) {
         }
 }
 
+// ------------------------------------------------------------------------------------------------
+// XTK basis-cluster enrichment (XTK/src/cl_XTK_Enrichment.cpp:276-400, 622-1510), restated from
+// the geometry: subphases are the face-connected components of a cell's material sub-cells; two
+// subphases of face-adjacent cells are neighbours when a material facet of one on the common face
+// has its centroid inside a material facet of the other (the facets are compared in physical
+// coordinates); a basis' support is the set of elements whose T rows hold it, and its enrichment
+// levels are the components of that support under the neighbour relation, numbered in order of
+// their first element.  Enriched adofs: basis by basis in ascending adof, level by level.
+// ------------------------------------------------------------------------------------------------
+
+// Components of the material sub-cells of one cell, [n_sub][D+1][D] in `sub`: sub-cells with D
+// common vertices share a face.  Components in order of their first sub-cell.
+std::vector<std::vector<int>> subphase_groups(int D, const std::vector<double>& sub) {
+    const int nv = D + 1;
+    const int n = (int)(sub.size() / (size_t)(nv * D));
+    auto vtx_eq = [&](int s, int a, int t, int b) {
+        for (int d = 0; d < D; ++d)
+            if (sub[((size_t)s * nv + a) * D + d] != sub[((size_t)t * nv + b) * D + d]) return false;
+        return true;
+    };
+    std::vector<int> comp(n, -1);
+    std::vector<std::vector<int>> groups;
+    for (int s0 = 0; s0 < n; ++s0) {
+        if (comp[s0] >= 0) continue;
+        const int g = (int)groups.size();
+        groups.push_back({});
+        std::vector<int> stack{s0};
+        comp[s0] = g;
+        while (!stack.empty()) {
+            const int s = stack.back();
+            stack.pop_back();
+            for (int t = 0; t < n; ++t) {
+                if (comp[t] >= 0) continue;
+                int common = 0;
+                for (int a = 0; a < nv; ++a)
+                    for (int b = 0; b < nv; ++b) common += vtx_eq(s, a, t, b) ? 1 : 0;
+                if (common == D) {
+                    comp[t] = g;
+                    stack.push_back(t);
+                }
+            }
+        }
+    }
+    for (int s = 0; s < n; ++s) groups[comp[s]].push_back(s);
+    return groups;
+}
+
+// Material facets of an element on face (axis d, side +-1) of its cell, as physical vertex
+// coordinates [facet][vertex][3]; an uncut element's face is the whole face (two triangles, or one
+// segment in 2D).
+std::vector<std::array<std::array<double, 3>, 3>> face_facets(const OracleProblem& p, size_t e, int d, double side) {
+    const int D = p.dim, nv = D + 1;
+    const double* box = p.ip_box.data() + e * 2 * D;
+    auto phys = [&](const double* xi, std::array<double, 3>& x) {
+        x = {0.0, 0.0, 0.0};
+        for (int q = 0; q < D; ++q) x[q] = box[q] + 0.5 * (xi[q] + 1.0) * box[D + q];
+    };
+    std::vector<std::array<std::array<double, 3>, 3>> out;
+    const int32_t s0 = p.ig_ptr[e], s1 = p.ig_ptr[e + 1];
+    if (s0 == s1) {
+        // the face's corners in cyclic order
+        int ax[2] = {0, 0}, na = 0;
+        for (int q = 0; q < D; ++q)
+            if (q != d) ax[na++] = q;
+        double cyc[4][3];
+        const double pat[4][2] = {{-1, -1}, {1, -1}, {1, 1}, {-1, 1}};
+        for (int k = 0; k < 4; ++k) {
+            cyc[k][d] = side;
+            cyc[k][ax[0]] = pat[k][0];
+            if (D == 3) cyc[k][ax[1]] = pat[k][1];
+        }
+        std::array<std::array<double, 3>, 3> f{};
+        if (D == 3) {
+            const int tri[2][3] = {{0, 1, 2}, {0, 2, 3}};
+            for (auto& t : tri) {
+                for (int k = 0; k < 3; ++k) phys(cyc[t[k]], f[k]);
+                out.push_back(f);
+            }
+        } else {
+            phys(cyc[0], f[0]);
+            phys(cyc[1], f[1]);
+            out.push_back(f);
+        }
+        return out;
+    }
+    for (int32_t s = s0; s < s1; ++s) {
+        const double* X = p.ig_xi.data() + (size_t)s * nv * D;
+        for (int omit = 0; omit < nv; ++omit) {
+            std::array<std::array<double, 3>, 3> f{};
+            int m = 0;
+            bool on = true;
+            for (int k = 0; k < nv; ++k) {
+                if (k == omit) continue;
+                on = on && std::fabs(X[k * D + d] - side) < 1e-12;
+                phys(X + k * D, f[m++]);
+            }
+            if (on) out.push_back(f);
+        }
+    }
+    return out;
+}
+
+// Point (on the face plane normal to axis d) inside a facet of that plane, with a tolerance.
+bool in_facet(int D, int d, const std::array<double, 3>& x, const std::array<std::array<double, 3>, 3>& f) {
+    int ax[2] = {0, 0}, na = 0;
+    for (int q = 0; q < D; ++q)
+        if (q != d) ax[na++] = q;
+    if (D == 2) {
+        const double a = f[0][ax[0]], b = f[1][ax[0]], lo = std::min(a, b), hi = std::max(a, b);
+        const double tol = 1e-9 * (hi - lo);
+        return x[ax[0]] > lo - tol && x[ax[0]] < hi + tol;
+    }
+    auto P = [&](const std::array<double, 3>& v, int k) { return v[ax[k]]; };
+    const double x1 = P(f[0], 0), y1 = P(f[0], 1), x2 = P(f[1], 0), y2 = P(f[1], 1), x3 = P(f[2], 0), y3 = P(f[2], 1);
+    const double det = (y2 - y3) * (x1 - x3) + (x3 - x2) * (y1 - y3);
+    if (std::fabs(det) == 0.0) return false;
+    const double l1 = ((y2 - y3) * (P(x, 0) - x3) + (x3 - x2) * (P(x, 1) - y3)) / det;
+    const double l2 = ((y3 - y1) * (P(x, 0) - x3) + (x1 - x3) * (P(x, 1) - y3)) / det;
+    const double l3 = 1.0 - l1 - l2;
+    const double tol = 1e-9;
+    return l1 > -tol && l2 > -tol && l3 > -tol;
+}
+
+// Enrich the T rows of p in place: the adof of each row entry becomes (basis, level).
+void enrich_rows(OracleProblem& p) {
+    const int D = p.dim;
+    const size_t n_el = p.elem_cell.size();
+    // elements of each cell
+    std::map<int64_t, std::vector<size_t>> cell_elems;
+    for (size_t e = 0; e < n_el; ++e) cell_elems[p.elem_cell[e]].push_back(e);
+    // neighbour pairs across the faces between cells c and c + e_d
+    std::vector<std::vector<size_t>> nbr(n_el);
+    for (const auto& kv : cell_elems) {
+        const int64_t c = kv.first;
+        const int64_t ijk[3] = {c % p.n[0], (c / p.n[0]) % p.n[1], c / ((int64_t)p.n[0] * p.n[1])};
+        for (int d = 0; d < D; ++d) {
+            if (ijk[d] + 1 >= p.n[d]) continue;
+            const int64_t cn = c + (d == 0 ? 1 : d == 1 ? p.n[0] : (int64_t)p.n[0] * p.n[1]);
+            auto it = cell_elems.find(cn);
+            if (it == cell_elems.end()) continue;
+            for (size_t a : kv.second) {
+                const auto fa = face_facets(p, a, d, 1.0);
+                for (size_t b : it->second) {
+                    const auto fb = face_facets(p, b, d, -1.0);
+                    bool touch = false;
+                    for (const auto& f : fa) {
+                        std::array<double, 3> ctr{0.0, 0.0, 0.0};
+                        for (int k = 0; k < D; ++k)
+                            for (int q = 0; q < 3; ++q) ctr[q] += f[k][q] / D;
+                        for (const auto& g : fb)
+                            if (in_facet(D, d, ctr, g)) { touch = true; break; }
+                        if (touch) break;
+                    }
+                    if (touch) {
+                        nbr[a].push_back(b);
+                        nbr[b].push_back(a);
+                    }
+                }
+            }
+        }
+    }
+    // support (elements) of each adof, ascending
+    const int nb = p.nb_ip;
+    std::vector<std::vector<size_t>> support(p.N);
+    for (size_t e = 0; e < n_el; ++e)
+        for (int32_t q = p.t_ptr[e * nb]; q < p.t_ptr[(e + 1) * nb]; ++q) {
+            auto& s = support[p.t_adof[q]];
+            if (s.empty() || s.back() != e) s.push_back(e);
+        }
+    // levels per (adof, element) and the enriched ids
+    std::vector<std::map<size_t, int>> level(p.N);
+    std::vector<int32_t> base(p.N, 0);
+    int32_t next = 0;
+    for (int32_t a = 0; a < p.N; ++a) {
+        const auto& s = support[a];
+        int n_lev = 0;
+        for (size_t e : s) {
+            if (level[a].count(e)) continue;
+            // flood the component of e within the support
+            std::vector<size_t> stack{e};
+            level[a][e] = n_lev;
+            while (!stack.empty()) {
+                const size_t x = stack.back();
+                stack.pop_back();
+                for (size_t y : nbr[x]) {
+                    if (!std::binary_search(s.begin(), s.end(), y) || level[a].count(y)) continue;
+                    level[a][y] = n_lev;
+                    stack.push_back(y);
+                }
+            }
+            ++n_lev;
+        }
+        base[a] = next;
+        next += n_lev;
+    }
+    for (size_t e = 0; e < n_el; ++e)
+        for (int32_t q = p.t_ptr[e * nb]; q < p.t_ptr[(e + 1) * nb]; ++q) {
+            const int32_t a = p.t_adof[q];
+            p.t_adof[q] = base[a] + level[a].at(e);
+        }
+    p.N = next;
+}
+
 void build(OracleProblem& p, const mgpu_problem_spec& s) {
     p.spec = s;
     p.dim = s.dim;
@@ -888,7 +1091,17 @@ void build(OracleProblem& p, const mgpu_problem_spec& s) {
         } else if (it->second.empty()) {
             continue;
         }
-        add_element(0, c, c, it == cut.end() ? nullptr : &it->second);
+        if (s.enrich && it != cut.end()) {
+            // one element per subphase, its sub-cells in their original order
+            const size_t stride = (size_t)nv * p.dim;
+            for (const auto& g : subphase_groups(p.dim, it->second)) {
+                std::vector<double> part;
+                for (int q : g) part.insert(part.end(), it->second.begin() + q * stride, it->second.begin() + (q + 1) * stride);
+                add_element(0, c, c, &part);
+            }
+        } else {
+            add_element(0, c, c, it == cut.end() ? nullptr : &it->second);
+        }
         for (int v = 0; v < p.nb; ++v) used[nodes[v]] = 1;
     }
     const int order = s.bspline_order <= 1 ? 1 : s.bspline_order;
@@ -932,6 +1145,10 @@ void build(OracleProblem& p, const mgpu_problem_spec& s) {
         bspline2_rows(p);
     }
     p.t_ptr.push_back((int32_t)p.t_adof.size());
+    if (s.enrich) {
+        if (thb) throw std::runtime_error("enrichment is implemented for level-0 lattices");
+        enrich_rows(p);
+    }
 }
 
 // element topology: per type sorted unique adof ids (global)

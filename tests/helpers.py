@@ -38,6 +38,19 @@ def random_spheres(n_spheres, seed, rmin=0.05, rmax=0.12):
     return np.hstack([c, r])
 
 
+def two_bodies(dim, n, r_cells=2.5, gap_cells=0.5):
+    """Two spheres (circles) on the diagonal through the centre of cell n//2 (each side), material
+    inside: the cell holds one corner cap of each, so it splits into two subphases."""
+    h = 1.0 / n
+    ctr = np.full(dim, (n // 2 + 0.5) * h)
+    u = np.ones(dim) / np.sqrt(dim)
+    r = r_cells * h
+    a = ctr - (gap_cells * h + r) * u
+    b = ctr + (gap_cells * h + r) * u
+    pad = [0.0] * (3 - dim)
+    return [list(a) + pad + [r], list(b) + pad + [r]]
+
+
 def adof_coords(problem_arrays, dim, n):
     """Coordinates of every adof (used lattice nodes in ascending node order) of a unit-box lattice."""
     ec = problem_arrays["elem_cell"].astype(np.int64)

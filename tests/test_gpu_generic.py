@@ -19,7 +19,7 @@ import numpy as np
 import pytest
 
 import moris_amd as M
-from helpers import SEED, rel
+from helpers import SEED, rel, two_bodies
 from moris_amd import learned as L
 from oracle import py as O
 
@@ -56,6 +56,18 @@ CASES = {
                                                         lagrange_order=2, refine_levels=1),
     "thb_sphere_4_hex27_diffusion": lambda: M.make_spec(3, (4, 4, 4), spheres=[[0.5, 0.5, 0.5, 0.3]], k_sigma=0.4,
                                                         bspline_order=2, lagrange_order=2, refine_levels=1),
+    # XTK enrichment: a cell split into two subphase elements, bases replicated per level; nodal
+    # p = 1 rows that leave the lattice stencil for the generic path
+    "enriched_two_spheres_p1_elasticity": lambda: M.make_spec(3, (10, 10, 10), iwg=M.IWG_STRUC_LINEAR_BULK,
+                                                              spheres=two_bodies(3, 10), material_phase=0, enrich=1),
+    "enriched_two_circles_p1_plane_stress": lambda: M.make_spec(2, (16, 16), iwg=M.IWG_STRUC_LINEAR_BULK,
+                                                                spheres=two_bodies(2, 16), material_phase=0,
+                                                                enrich=1),
+    "enriched_gyroid_7_p1_diffusion": lambda: M.make_spec(3, (7, 7, 7), ls_kind=M.LS_GYROID, gyroid_scale=9.0,
+                                                          k_sigma=0.5, enrich=1),
+    "enriched_gyroid_6_p2_hex27_elasticity": lambda: M.make_spec(3, (6, 6, 6), iwg=M.IWG_STRUC_LINEAR_BULK,
+                                                                 ls_kind=M.LS_GYROID, gyroid_scale=2 * np.pi * 1.6,
+                                                                 bspline_order=2, lagrange_order=2, enrich=1),
 }
 
 

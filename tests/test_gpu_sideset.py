@@ -19,7 +19,7 @@ import scipy.sparse as sp
 import scipy.sparse.linalg as spla
 
 import moris_amd as M
-from helpers import NV, SEED, oracle_side_system, rel
+from helpers import NV, SEED, oracle_side_system, rel, two_bodies
 from moris_amd import sideset as SS
 from moris_amd import solve as SV
 
@@ -78,6 +78,11 @@ GENERIC_CASES = {
                               [(SS.IWG_STRUC_LINEAR_DIRICHLET_SYMMETRIC_NITSCHE, SS.SIDE_XMIN, SS.GEOM_LINE2, 100.0,
                                 [0.0, 0.0]),
                                (SS.IWG_STRUC_LINEAR_NEUMANN, SS.SIDE_INTERFACE, SS.GEOM_LINE2, 0.0, [0.1, -0.3])]),
+    # XTK enrichment: interface facets of two bodies that share a cell, on subphase elements
+    "enriched_two_spheres_elast": (lambda: M.make_spec(3, (10, 10, 10), iwg=1, spheres=two_bodies(3, 10),
+                                                       material_phase=0, enrich=1),
+                                   [(SS.IWG_STRUC_LINEAR_DIRICHLET_SYMMETRIC_NITSCHE, SS.SIDE_INTERFACE, SS.GEOM_TRI3,
+                                     10.0, [0.01, -0.02, 0.005])]),
 }
 
 
