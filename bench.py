@@ -182,11 +182,11 @@ def run_hex27(args):
     ub, vb, rb = M.DeviceBuffer.from_array(u), M.DeviceBuffer(8 * S.nnz), M.DeviceBuffer(8 * S.n_rows)
     ig = np.diff(P.arrays()["ig_ptr"])
     npts = np.where(ig > 0, ig * 35, 27)
-    # executed MFMA work: per 32-point chunk the 21 upper 16 x 16 tiles of M x 8 k-steps of 4 points;
+    # executed MFMA work: per 64-point chunk the 21 upper 16 x 16 tiles of M x 16 k-steps of 4 points;
     # per element and dof-type block the 2 n_t + n_t^2 tiles of K_p T and T^T (K_p T) x 7 k-steps
     # (n_t = ceil(adofs per type / 16))
     n_t = (np.diff(S.topology_offsets()) // 3 + 15) // 16
-    mfma_flops = float(np.sum(np.ceil(npts / 32.0)) * 21 * 8 * 2 * 16 * 16 * 4 +
+    mfma_flops = float(np.sum(np.ceil(npts / 64.0)) * 21 * 16 * 2 * 16 * 16 * 4 +
                        np.sum(9 * (2 * n_t + n_t * n_t)) * 7 * 2 * 16 * 16 * 4)
     native, _, _ = elasticity_flops(int(npts.sum()), P.n_elements, nb=27)
     out = {"metric": "elements/s for K+R assembly, HEX27 p=2 elasticity (gyroid cut), generic path",
@@ -200,6 +200,7 @@ def run_hex27(args):
     res = {}
     for prec, name in ((M.PREC_FP64, "fp64"), (M.PREC_MIXED, "mixed")):
         S.options(prec, M.QUAD_SUBDIVISION)
+        S.autotune(ub, vb, rb, iters=2)   # element order of the gather mode
         per, st = S.time_assemble(ub, vb, rb, max(3, args.steps))
         res[name] = (vb.to_array(np.float64, S.nnz), rb.to_array(np.float64, S.n_rows))
         out[name] = {"value": P.n_elements / (per * 1e-3), "ms_per_call": per,
@@ -261,6 +262,7 @@ def run_hex27_blocks(args):
         line = []
         for prec, name in ((M.PREC_FP64, "fp64"), (M.PREC_MIXED, "mixed")):
             S.options(prec, M.QUAD_SUBDIVISION)
+            S.autotune(ub, vb, rb, iters=1)
             per, _ = S.time_assemble(ub, vb, rb, max(2, args.steps // 5))
             totals[name] += per
             line.append("%s %.1f ms" % (name, per))

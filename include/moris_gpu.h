@@ -247,6 +247,9 @@ mgpu_status mgpu_set_time_assemble(mgpu_set* set, const double* u_adof, double* 
    (0 natural, 1 longest first by sub-cell count).  Stencil variants differ in rows per workgroup
    and in how a row reaches HBM: the first half of the variants stage the workgroup's CSR range
    in LDS and write it coalesced, the second half (same row counts) store each row directly.
+   Generic-path sets have no stencil: their shape is the order of the element workgroups of the
+   gather mode (0 element order, 1 most sub-cells first; scatter mode orders each colour most
+   sub-cells first itself), and the autotuner times only (0, 0) and (0, 1).
    Results are bit-identical for every shape.                                                   */
 mgpu_status mgpu_set_tuning(mgpu_set* set, int32_t stencil_variant, int32_t cut_order);
 mgpu_status mgpu_set_get_tuning(const mgpu_set* set, int32_t* stencil_variant, int32_t* cut_order);
@@ -255,7 +258,7 @@ mgpu_status mgpu_set_get_tuning(const mgpu_set* set, int32_t* stencil_variant, i
 mgpu_status mgpu_set_tuning_variants(const mgpu_set* set, int32_t* n_variants, int32_t* rows);
 /* Time every launch shape with `iters` assemble calls on the given device buffers and keep the
    fastest.  The choice is cached per process by set shape (dimension, fields, rows, cut count
-   rounded to a power of two, precision, quadrature); flags bit 0 ignores the cache.
+   rounded to a power of two, precision, quadrature, path); flags bit 0 ignores the cache.
    ms[2 * n_variants] (optional) receives the per-call mean of shape (variant, order) at
    [2 * variant + order], or -1 where the cache answered.                                       */
 mgpu_status mgpu_set_autotune(mgpu_set* set, const double* u_adof, double* csr_vals, double* rhs,

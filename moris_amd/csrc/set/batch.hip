@@ -26,20 +26,28 @@ using namespace mgpu;
 
 namespace {
 
-using TuneKey = std::tuple<int, int, int, int, int, int>;
+using TuneKey = std::tuple<int, int, int, int, int, int, int>;
 std::mutex g_tune_mu;
 std::map<TuneKey, std::pair<int, int>> g_tune_cache;
 
 TuneKey tune_key(const mgpu_set* S) {
-    const int n_cut = S->v.n_cut;
+    const int n_cut = (int)S->h.cut_elems.size();
     const int bucket = n_cut == 0 ? 0 : 32 - __builtin_clz((unsigned)n_cut);
-    return TuneKey{S->h.dim, S->h.nf, S->h.n_rows, bucket, S->precision, S->quadrature};
+    return TuneKey{S->h.dim, S->h.nf, S->h.n_rows, bucket, S->precision, S->quadrature, S->h.generic ? 1 : 0};
+}
+
+// Generic sets have one launch shape per element order (no stencil); the order applies to the
+// gather mode's element workgroups (scatter mode orders within each colour itself).
+bool order_choice(const mgpu_set* S) {
+    if (S->h.generic) return !S->h.scatter && !S->h.cut_elems.empty() && S->d_gen_order;
+    return S->v.n_cut > 0;
 }
 
 void apply_tuning(mgpu_set* S, int variant, int order) {
     S->stencil_variant = variant;
     S->cut_order = order;
     S->v.cut_order = order ? S->d_cut_order : nullptr;
+    S->g.order = order && S->h.generic && !S->h.scatter ? S->d_gen_order : nullptr;
 }
 
 }  // namespace
@@ -84,6 +92,7 @@ extern "C" mgpu_status mgpu_set_autotune(mgpu_set* set, const double* u, double*
     if (ms)
         for (int k = 0; k < 2 * nv; ++k) ms[k] = -1.0;
     if (set->h.n_rows == 0) return MGPU_OK;
+    if (set->h.generic) nv = 1;   // shapes of a generic set: the element order only
     if (!u || !vals || !rhs) return set_error(MGPU_ERR_INVALID_ARG, "null buffer");
     const TuneKey key = tune_key(set);
     if (!(flags & 1)) {
@@ -102,7 +111,7 @@ extern "C" mgpu_status mgpu_set_autotune(mgpu_set* set, const double* u, double*
     mgpu_status rc = MGPU_OK;
     for (int v = 0; v < nv && rc == MGPU_OK; ++v)
         for (int o = 0; o < 2 && rc == MGPU_OK; ++o) {
-            if (o == 1 && set->v.n_cut == 0) continue;
+            if (o == 1 && !order_choice(set)) continue;
             apply_tuning(set, v, o);
             if ((rc = launch_assemble(set, u, vals, rhs, nullptr)) != MGPU_OK) break;
             if (hipEventRecord(t0, set->stream) != hipSuccess) { rc = set_error(MGPU_ERR_HIP, "event record"); break; }

@@ -14,7 +14,7 @@
 //                       cl_MSI_Equation_Object.cpp:426-493, :789-842), in FP64 through LDS, written
 //                       to the element's slot of the K_e pool.
 //   k_generic_hex27     HEX27 IP cells (HEX_3x3x3 / TET_35): one block of four wavefronts per
-//                       element, the moments on MFMA (FP64, or FP32 per 32-point chunk summed in
+//                       element, the moments on MFMA (FP64, or FP32 per 64-point chunk summed in
 //                       FP64), K_p and K_e = T^T K_p T in FP64 through LDS.  In scatter mode
 //                       (sets too large for a K_e pool) one launch per colour class, each
 //                       element adding its K_e into the CSR values after k_generic_zero.
@@ -33,6 +33,21 @@
 namespace mgpu {
 
 namespace {
+
+template <class T>
+mgpu_status gupload(mgpu_set* S, const T* src, size_t n, T** dst);
+
+// the gather sources in 32 bits when every K_e entry index fits, else in 64 bits
+mgpu_status upload_gather_src(mgpu_set* S, const std::vector<int64_t>& src, GenView& g) {
+    g.gather_src = nullptr;
+    g.gather_src32 = nullptr;
+    const size_t n_ke = S->h.ke_off.empty() ? 0 : (size_t)S->h.ke_off.back();
+    if (n_ke < ((size_t)1 << 31)) {
+        std::vector<int32_t> s32(src.begin(), src.end());
+        return gupload(S, s32.data(), s32.size(), const_cast<int32_t**>(&g.gather_src32));
+    }
+    return gupload(S, src.data(), src.size(), const_cast<int64_t**>(&g.gather_src));
+}
 
 template <class T>
 mgpu_status gupload(mgpu_set* S, const T* src, size_t n, T** dst) {
@@ -114,7 +129,7 @@ __global__ __launch_bounds__(64) void k_generic_element(GenView g) {
     constexpr int GP = NG + 1;
     static_assert(2 * 64 * GP * sizeof(T) <= kGenLds * sizeof(double), "LDS phases");
     __shared__ double lds[kGenLds];
-    const int e = blockIdx.x;
+    const int e = g.order ? g.order[blockIdx.x] : (int)blockIdx.x;
     const int lane = threadIdx.x;
     double hh[DIM];
     for (int d = 0; d < DIM; ++d) hh[d] = g.ip_box[(size_t)e * 2 * DIM + DIM + d];
@@ -203,21 +218,21 @@ __global__ __launch_bounds__(64) void k_generic_element(GenView g) {
 }
 
 // HEX27 IP cells: one block of 4 wavefronts per element.  Points (HEX_3x3x3 uncut, TET_35 on the
-// sub-tets, w* = w detJ, detJ < -1e-6 skipped) are staged 32 at a time as records of w* and the
+// sub-tets, w* = w detJ, detJ < -1e-6 skipped) are staged 64 at a time as records of w* and the
 // 1D quadratic Lagrange values and derivatives per axis, in the MFMA operand type (two LDS buffers,
 // one barrier per chunk);
 // each lane forms its MFMA operands, the physical gradients dN_a/dx_i (times w* for A), from them
 // in registers, and the moments
 //   M[(i,a),(j,b)] = sum_ip w* dN_a/dx_i dN_b/dx_j      (81 x 81, padded to 96 x 96)
 // accumulate on MFMA: FP64 (v_mfma_f64_16x16x4f64) on the plain path; on the mixed path FP32
-// products (v_mfma_f32_16x16x4f32) summed per 32-point chunk and added into FP64 registers.  M
+// products (v_mfma_f32_16x16x4f32) summed per 64-point chunk and added into FP64 registers.  M
 // being symmetric, the four wavefronts compute its 21 upper tiles of 16 x 16 (6, 5, 5, 5) and
 // mirror them.  Then K_p from
 // M and the material in LDS, and K_e = T^T K_p T per (dof type, dof type) block in FP64.  The LDS
 // (80 KB: K_p, T and one block of K_p T for up to 64 adofs per type, e.g. truncated hierarchical
 // rows) uses gfx950's 160 KB per workgroup.
-constexpr int kH27Chunk = 32;                    // points per staging round
-static_assert((kH27Chunk & (kH27Chunk - 1)) == 0 && 4 * kH27Chunk <= 256, "4 threads per point of a chunk");
+constexpr int kH27Chunk = 64;                    // points per staging round
+static_assert((kH27Chunk & (kH27Chunk - 1)) == 0 && kH27Chunk >= 64, "4 slots per point, whole wavefronts");
 constexpr int kH27PtStride = 20;                 // w*, then l[3][3], dl[3][3] of the point
 constexpr int kH27Lds = 81 * 81 + 2 * 27 * kHex27MaxAdofsPerType;   // doubles: K_p, T, W of one block (80 KB)
 static_assert(2 * kH27Chunk * kH27PtStride <= 81 * 81, "two point-record buffers fit the K_p region");
@@ -246,7 +261,7 @@ template <> struct H27Tiles<3> {
     static constexpr int mt[6] = {3, 3, 3, 4, 4, 0}, nt[6] = {3, 4, 5, 4, 5, 0};
 };
 
-// M tiles += Gw^T G over one 32-point chunk (A: row lane % 16 of the tile, k = lane / 16; B: k =
+// M tiles += Gw^T G over one chunk of points (A: row lane % 16 of the tile, k = lane / 16; B: k =
 // lane / 16, column lane % 16), FP64 MFMA or FP32 MFMA summed into the FP64 accumulators, straight
 // from the chunk's point records: the lane forms the operands of its tiles in registers (column (i, a) of point p is ((2/h_i dl_i) l_j) l_k, times w* for A), so that no
 // staged gradient table and no barrier sit between the point records and the MFMA work.  go / gs:
@@ -309,7 +324,7 @@ __global__ __launch_bounds__(256) void k_generic_hex27(GenView g, int c0, double
     constexpr int NB = 27, NG = 81, NP = NF * NB;
     __shared__ double lds[kH27Lds];
     // scatter mode: block b of a launch takes element b of the colour starting at c0
-    const int e = SCATTER ? g.colour_el[c0 + blockIdx.x] : blockIdx.x;
+    const int e = SCATTER ? g.colour_el[c0 + blockIdx.x] : (g.order ? g.order[blockIdx.x] : (int)blockIdx.x);
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     double hh[3];
     for (int d = 0; d < 3; ++d) hh[d] = g.ip_box[(size_t)e * 6 + 3 + d];
@@ -344,8 +359,8 @@ __global__ __launch_bounds__(256) void k_generic_hex27(GenView g, int c0, double
         T* pts = reinterpret_cast<T*>(lds) + buf * kH27Chunk * kH27PtStride;
         // the chunk's points: w* and the 1D quadratic Lagrange values / derivatives per axis, by 4
         // threads per point (one per axis, one for the weight)
-        if (tid < 4 * kH27Chunk) {
-            const int pl = tid & (kH27Chunk - 1), role = tid / kH27Chunk;
+        for (int slot = tid; slot < 4 * kH27Chunk; slot += 256) {
+            const int pl = slot & (kH27Chunk - 1), role = slot / kH27Chunk;
             const int ip = p0 + pl;
             T* pp = pts + pl * kH27PtStride;
             const int t = ip / kTet35Points, q = ip - kTet35Points * t;
@@ -542,11 +557,12 @@ __global__ void k_generic_zero(double* vals, int64_t n) {
         __builtin_nontemporal_store(0.0, vals + p);   // an output: nothing reads it back here
 }
 
-__global__ void k_generic_gather(GenView g, double* vals) {
+template <typename I>
+__global__ void k_generic_gather(GenView g, const I* src, double* vals) {
     const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= g.nnz) return;
     double v = 0.0;
-    for (int64_t k = g.gather_ptr[p]; k < g.gather_ptr[p + 1]; ++k) v += g.ke[g.gather_src[k]];
+    for (int64_t k = g.gather_ptr[p]; k < g.gather_ptr[p + 1]; ++k) v += g.ke[src[k]];
     __builtin_nontemporal_store(v, vals + p);   // an output: nothing reads it back here
 }
 
@@ -583,7 +599,7 @@ mgpu_status generic_upload(mgpu_set* S, const mgpu_set_desc& d) {
         (rc = gupload(S, h.na_t.data(), h.na_t.size(), const_cast<int32_t**>(&g.na_t))) ||
         (rc = gupload(S, h.ke_off.data(), h.ke_off.size(), const_cast<int64_t**>(&g.ke_off))) ||
         (rc = gupload(S, h.gather_ptr.data(), h.gather_ptr.size(), const_cast<int64_t**>(&g.gather_ptr))) ||
-        (rc = gupload(S, h.gather_src.data(), h.gather_src.size(), const_cast<int64_t**>(&g.gather_src))) ||
+        (rc = upload_gather_src(S, h.gather_src, g)) ||
         (rc = gupload(S, h.row_ptr.data(), h.row_ptr.size(), const_cast<int32_t**>(&g.row_ptr))) ||
         (rc = gupload(S, h.col_idx.data(), h.col_idx.size(), const_cast<int32_t**>(&g.col_idx))))
         return rc;
@@ -595,6 +611,17 @@ mgpu_status generic_upload(mgpu_set* S, const mgpu_set_desc& d) {
             (rc = gupload(S, h.rank.data(), h.rank.size(), const_cast<uint16_t**>(&g.rank))))
             return rc;
         return MGPU_OK;   // no K_e pool
+    }
+    // the elements with most sub-cells first (stable), so that a launch does not end on one long
+    // element; used when the tuning selects it
+    {
+        std::vector<int32_t> order((size_t)h.n_el);
+        for (int32_t e = 0; e < h.n_el; ++e) order[(size_t)e] = e;
+        std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
+            return d.ig_ptr[x + 1] - d.ig_ptr[x] > d.ig_ptr[y + 1] - d.ig_ptr[y];
+        });
+        if (h.n_el > 0 && (rc = gupload(S, order.data(), order.size(), const_cast<int32_t**>(&S->d_gen_order))))
+            return rc;
     }
     const size_t n_ke = h.ke_off.empty() ? 0 : (size_t)h.ke_off.back();
     if (n_ke > 0) {
@@ -654,9 +681,13 @@ mgpu_status launch_generic(mgpu_set* S, const double* u, double* vals, double* r
         HIP_TRY(hipGetLastError());
     }
     if (ev) HIP_TRY(hipEventRecord(ev[3], S->stream));
-    if (S->g.nnz > 0 && !h.scatter)
-        hipLaunchKernelGGL(k_generic_gather, dim3((unsigned)((S->g.nnz + 255) / 256)), dim3(256), 0, S->stream, S->g,
-                           vals);
+    if (S->g.nnz > 0 && !h.scatter) {
+        const dim3 gg((unsigned)((S->g.nnz + 255) / 256));
+        if (S->g.gather_src32)
+            hipLaunchKernelGGL(k_generic_gather<int32_t>, gg, dim3(256), 0, S->stream, S->g, S->g.gather_src32, vals);
+        else
+            hipLaunchKernelGGL(k_generic_gather<int64_t>, gg, dim3(256), 0, S->stream, S->g, S->g.gather_src, vals);
+    }
     HIP_TRY(hipGetLastError());
     if (ev) HIP_TRY(hipEventRecord(ev[4], S->stream));
     if (h.n_rows > 0)

@@ -92,12 +92,16 @@ struct GenView {
     double* ke;                  // element matrices, (nf na_t)^2 per element, row-major
     const int64_t* gather_ptr;   // [nnz + 1]
     const int64_t* gather_src;   // K_e entries summed into each CSR value
+    const int32_t* gather_src32; // the same in 32 bits when the pool has fewer than 2^31 entries
     const int32_t* row_ptr;      // [n_rows + 1]
     const int32_t* col_idx;      // [nnz]
     // scatter mode (HEX27 sets without a K_e pool)
     const int32_t* topo;         // rows of each element, nf * na_t from topo_ptr[e]
     const int64_t* topo_ptr;     // [n_el + 1]
     const int32_t* colour_el;    // [n_el] elements by colour
+    // gather mode: the processing order of the element workgroups (most sub-cells first), or
+    // null for element order; the autotuner picks one
+    const int32_t* order;
     const int64_t* rank_off;     // [n_el + 1]
     const uint16_t* rank;        // [na_t][na_t] per element: column of adof y among adof x's neighbours
 };

@@ -39,6 +39,7 @@ struct mgpu_set {
     int stencil_variant = -1;               // index into the stencil row-block sizes, -1 default
     int cut_order = 0;                      // 1: cut elements longest-first
     const int32_t* d_cut_order = nullptr;   // device permutation (built on first use)
+    const int32_t* d_gen_order = nullptr;   // generic sets, gather mode: elements most sub-cells first
     // learned cut-cell quadrature
     std::vector<double> vphi_cut;      // host copy of the cut elements' nodal level sets
     mgpu::LqView lv{};

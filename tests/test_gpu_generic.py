@@ -125,6 +125,39 @@ def test_generic_path_reproduces_lattice_stencil(iwg):
     assert rel(v2, v1) <= 1e-12 and rel(R2, R1) <= 1e-12
 
 
+@pytest.mark.parametrize("lagrange_order", [1, 2])
+def test_generic_element_order_and_autotune(lagrange_order):
+    """Generic sets in gather mode have two launch shapes: element workgroups in element order, or
+    most sub-cells first.  Both give bit-identical K and R (each K_e has its own pool slot and the
+    gather sums in a fixed order); the autotuner times both, keeps the faster, and the result still
+    matches the oracle."""
+    spec, keep = M.make_spec(3, (6, 6, 6), iwg=M.IWG_STRUC_LINEAR_BULK, ls_kind=M.LS_GYROID,
+                             gyroid_scale=2 * np.pi * 1.5, bspline_order=2, lagrange_order=lagrange_order)
+    P = M.Problem(spec, keep)
+    S = M.Set(P.desc, keepalive=P)
+    u = np.random.default_rng(SEED + 5).uniform(-1.0, 1.0, S.n_rows)
+    ub, vb, rb = M.DeviceBuffer.from_array(u), M.DeviceBuffer(8 * S.nnz), M.DeviceBuffer(8 * S.n_rows)
+    out = []
+    for order in (0, 1):
+        S.tuning(cut_order=order)
+        S.assemble_device(ub, vb, rb)
+        M.synchronize()
+        out.append((vb.to_array(np.float64, S.nnz), rb.to_array(np.float64, S.n_rows)))
+    assert np.array_equal(out[0][0].view(np.uint64), out[1][0].view(np.uint64))
+    assert np.array_equal(out[0][1].view(np.uint64), out[1][1].view(np.uint64))
+    ms = S.autotune(ub, vb, rb, iters=2, ignore_cache=True)
+    assert set(ms) == {(0, 0), (0, 1)} and all(t > 0 for t in ms.values()), ms
+    _, order = S.tuning()
+    assert ms[(0, order)] == min(ms.values())
+    S.assemble_device(ub, vb, rb)
+    M.synchronize()
+    assert np.array_equal(vb.to_array(np.float64, S.nnz).view(np.uint64), out[0][0].view(np.uint64))
+    Q = O.OracleProblem(spec)
+    s = Q.setup()
+    vo, Ro, _ = O.assemble_desc(P.desc, s["row_ptr"], s["col_idx"], u)
+    assert rel(out[0][0], vo) <= 1e-12 and rel(out[0][1], Ro) <= 1e-12
+
+
 def test_generic_batch_replays_and_learned_refused():
     sets, bufs, eager = [], [], []
     for k, r in enumerate((0.28, 0.31)):
