@@ -7,7 +7,7 @@
  * ||r_k|| <= rel_residual * ||r_0||, solver and preconditioner chosen by AZ_solver / AZ_precond,
  * :260-356; AZ_gmres (the default), AZ_cg and AZ_Jacobi among its options).  These entry points
  * solve systems held in device memory on one GPU: conjugate gradients (symmetric positive definite
- * K) and restarted GMRES (any non-singular K), both with a Jacobi preconditioner, every reduction
+ * K) and restarted GMRES (any non-singular K), with a Jacobi or (GMRES) an ILU(k) preconditioner, every reduction
  * in a fixed order (bit-identical results run to run), and the convergence test of Aztec's
  * default.  Strong Dirichlet conditions, which the reference's weak
  * (Nitsche) formulation does not need, can be applied to the system first.  Same conventions as
@@ -47,13 +47,6 @@ mgpu_status mgpu_cg_solve(int32_t n_rows, const int32_t* row_ptr, const int32_t*
                           const double* b, double* x, double rtol, int32_t max_iter, void* stream,
                           mgpu_cg_info* info);
 
-/* The same solve without any device allocation.  No counterpart in the reference: it is for
-   callers that cannot allocate device memory (serving loops, harnesses that forbid hipMalloc).
-   The work vectors live in a fixed device array sized for MGPU_CG_NOALLOC_MAX_ROWS rows (about
-   100 MB), one copy per GPU, held from the library's first use on that GPU until the process
-   ends; larger systems are refused with MGPU_ERR_INVALID_ARG.  All calls on one GPU share that
-   array: calls that overlap in time on one GPU (several host threads or streams) race and must be
-   serialised by the caller.  Results equal those of mgpu_cg_solve bit for bit.                */
 /* Restarted GMRES(restart) with a right Jacobi preconditioner for K x = b, K any non-singular CSR
    matrix with sorted columns (the non-symmetric systems of UNSYMMETRIC Nitsche side sets): the
    reference's default solver, AztecOO AZ_gmres with AZ_kspace = restart (cl_DLA_Linear_Solver_
@@ -66,6 +59,46 @@ mgpu_status mgpu_gmres_solve(int32_t n_rows, const int32_t* row_ptr, const int32
                              const double* b, double* x, double rtol, int32_t restart, int32_t max_iter,
                              void* stream, mgpu_cg_info* info);
 
+/* Incomplete LU factors ILU(k) of a CSR matrix on the GPU, the preconditioner the reference pairs
+   with GMRES (AZ_precond = AZ_dom_decomp, AZ_subdomain_solve = AZ_ilu, AZ_graph_fill = k;
+   cl_DLA_Linear_Solver_Aztec.cpp:353-380, UT_MDL_XTK_HMR_Linear_Struc_2D.cpp:990-993), restated
+   from the published algorithm (Saad, Iterative Methods for Sparse Linear Systems, Alg. 10.4 and
+   the level-of-fill rule lev(a_ij) = min(lev(a_ij), lev(a_ik) + lev(a_kj) + 1), fill kept up to
+   level k) since AztecOO is not part of the reference sources.  Symbolic and numeric parts are
+   split as a solver reuses one pattern for many matrices:
+     mgpu_ilu_create   the factor pattern (the matrix pattern, every diagonal, the fill of level
+                       <= fill_level) and the level schedules, computed on the host from the
+                       device pattern (row_ptr, col_idx: sorted columns); synchronous;
+     mgpu_ilu_factor   the numeric factors of vals (same pattern as at create), one launch per
+                       level of the forward schedule; synchronous; MGPU_ERR_INVALID_ARG on a zero
+                       pivot;
+     mgpu_ilu_apply    z = (L U)^-1 v (v == z allowed), asynchronous on the stream;
+     mgpu_ilu_factors  the factor pattern and values (DEVICE pointers owned by the handle: the
+                       strict lower part holds L, whose unit diagonal is implied, the rest U) and
+                       the forward / backward level counts (n_levels[2], optional).
+   Rows of at most 5000 factor entries.  Results are the same on every call.                   */
+typedef struct mgpu_ilu mgpu_ilu;
+mgpu_status mgpu_ilu_create(int32_t n_rows, const int32_t* row_ptr, const int32_t* col_idx, int32_t fill_level,
+                            void* stream, mgpu_ilu** out);
+mgpu_status mgpu_ilu_factor(mgpu_ilu* ilu, const double* vals, void* stream);
+mgpu_status mgpu_ilu_apply(const mgpu_ilu* ilu, const double* v, double* z, void* stream);
+mgpu_status mgpu_ilu_factors(const mgpu_ilu* ilu, int32_t* n_rows, int64_t* nnz, const int32_t** row_ptr,
+                             const int32_t** col_idx, const double** lu, int32_t* n_levels);
+mgpu_status mgpu_ilu_destroy(mgpu_ilu* ilu);
+
+/* mgpu_gmres_solve right-preconditioned by factors from mgpu_ilu_factor of the same matrix (or of
+   one with the same pattern): K M^-1 y = b, x = M^-1 y, M = L U.  Allocates one more vector.  */
+mgpu_status mgpu_gmres_solve_ilu(int32_t n_rows, const int32_t* row_ptr, const int32_t* col_idx, const double* vals,
+                                 const double* b, double* x, double rtol, int32_t restart, int32_t max_iter,
+                                 const mgpu_ilu* ilu, void* stream, mgpu_cg_info* info);
+
+/* The same solve without any device allocation.  No counterpart in the reference: it is for
+   callers that cannot allocate device memory (serving loops, harnesses that forbid hipMalloc).
+   The work vectors live in a fixed device array sized for MGPU_CG_NOALLOC_MAX_ROWS rows (about
+   100 MB), one copy per GPU, held from the library's first use on that GPU until the process
+   ends; larger systems are refused with MGPU_ERR_INVALID_ARG.  All calls on one GPU share that
+   array: calls that overlap in time on one GPU (several host threads or streams) race and must be
+   serialised by the caller.  Results equal those of mgpu_cg_solve bit for bit.                */
 #define MGPU_CG_NOALLOC_MAX_ROWS 2500000
 mgpu_status mgpu_cg_solve_noalloc(int32_t n_rows, const int32_t* row_ptr, const int32_t* col_idx,
                                   const double* vals, const double* b, double* x, double rtol, int32_t max_iter,

@@ -34,6 +34,7 @@ HIP_SOURCES = [
     "learned/mlp.hip",
     "solve/cg.hip",
     "solve/gmres.hip",
+    "solve/ilu.hip",
     "setup/gpu_setup.hip",
 ]
 

@@ -30,6 +30,7 @@
 
 #include "../../../include/moris_gpu_solve.h"
 #include "../setup/internal.hpp"
+#include "ilu.hpp"
 
 #define GM_TRY(expr)                                                                               \
     do {                                                                                           \
@@ -88,8 +89,8 @@ __global__ __launch_bounds__(kThreads) void k_gm_dinv(int32_t n, const int32_t* 
     dinv[r] = d != 0.0 ? 1.0 / d : 1.0;   // a zero diagonal is left unscaled
 }
 
-// y = b - K x (RESID) or y = K D^-1 x (APPLY); partial[blk] = block sum of y.y (RESID only).
-template <int L, bool RESID>
+// y = b - K x (RESID), y = K D^-1 x (JAC) or y = K x; partial[blk] = block sum of y.y (RESID only).
+template <int L, bool RESID, bool JAC>
 __global__ __launch_bounds__(kThreads) void k_gm_spmv(int32_t n, const int32_t* rp, const int32_t* ci,
                                                       const double* vals, const double* dinv, const double* x,
                                                       const double* b, double* y, double* partial, const State* st) {
@@ -103,7 +104,7 @@ __global__ __launch_bounds__(kThreads) void k_gm_spmv(int32_t n, const int32_t* 
         double s = 0.0;
         for (int32_t k = rp[r] + lane; k < rp[r + 1]; k += L) {
             const int32_t c = ci[k];
-            s += RESID ? vals[k] * x[c] : vals[k] * (dinv[c] * x[c]);
+            s += (RESID || !JAC) ? vals[k] * x[c] : vals[k] * (dinv[c] * x[c]);
         }
 #pragma unroll
         for (int off = L / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, L);
@@ -280,6 +281,26 @@ __global__ __launch_bounds__(kThreads) void k_gm_update(int32_t n, const double*
     }
 }
 
+// ILU path: u = sum_i y_i v_i (then u <- M^-1 u and x += u).
+__global__ __launch_bounds__(kThreads) void k_gm_vy(int32_t n, const double* V, int64_t ldv, double* u, const State* st) {
+    const int j = st->j;
+    for (int64_t r = (int64_t)blockIdx.x * kThreads + threadIdx.x; r < n; r += (int64_t)gridDim.x * kThreads) {
+        double s = 0.0;
+        for (int i = 0; i < j; ++i) s += st->y[i] * V[i * ldv + r];
+        u[r] = s;
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void k_gm_add(int32_t n, const double* u, double* x) {
+    for (int64_t r = (int64_t)blockIdx.x * kThreads + threadIdx.x; r < n; r += (int64_t)gridDim.x * kThreads) x[r] += u[r];
+}
+
+// ILU path: z = v_j before M^-1 is applied to it in place (nothing once the cycle has ended).
+__global__ __launch_bounds__(kThreads) void k_gm_copy(int32_t n, const double* v, double* z, const State* st) {
+    if (st->cycle_end) return;
+    for (int64_t r = (int64_t)blockIdx.x * kThreads + threadIdx.x; r < n; r += (int64_t)gridDim.x * kThreads) z[r] = v[r];
+}
+
 __global__ __launch_bounds__(kThreads) void k_gm_final(const double* partial, State* st) {
     __shared__ double sh[kThreads];
     double a = 0.0;
@@ -295,17 +316,26 @@ int lanes_for(int32_t n, int64_t nnz) {
     return L;
 }
 
-template <bool RESID>
-void launch_spmv(int L, int32_t n, const int32_t* rp, const int32_t* ci, const double* vals, const double* dinv,
-                 const double* x, const double* b, double* y, double* partial, const State* st, hipStream_t s) {
+template <bool RESID, bool JAC>
+void launch_spmv_t(int L, int32_t n, const int32_t* rp, const int32_t* ci, const double* vals, const double* dinv,
+                   const double* x, const double* b, double* y, double* partial, const State* st, hipStream_t s) {
     const dim3 g(kBlocks), t(kThreads);
     switch (L) {
-        case 4: hipLaunchKernelGGL((k_gm_spmv<4, RESID>), g, t, 0, s, n, rp, ci, vals, dinv, x, b, y, partial, st); break;
-        case 8: hipLaunchKernelGGL((k_gm_spmv<8, RESID>), g, t, 0, s, n, rp, ci, vals, dinv, x, b, y, partial, st); break;
-        case 16: hipLaunchKernelGGL((k_gm_spmv<16, RESID>), g, t, 0, s, n, rp, ci, vals, dinv, x, b, y, partial, st); break;
-        case 32: hipLaunchKernelGGL((k_gm_spmv<32, RESID>), g, t, 0, s, n, rp, ci, vals, dinv, x, b, y, partial, st); break;
-        default: hipLaunchKernelGGL((k_gm_spmv<64, RESID>), g, t, 0, s, n, rp, ci, vals, dinv, x, b, y, partial, st); break;
+        case 4: hipLaunchKernelGGL((k_gm_spmv<4, RESID, JAC>), g, t, 0, s, n, rp, ci, vals, dinv, x, b, y, partial, st); break;
+        case 8: hipLaunchKernelGGL((k_gm_spmv<8, RESID, JAC>), g, t, 0, s, n, rp, ci, vals, dinv, x, b, y, partial, st); break;
+        case 16: hipLaunchKernelGGL((k_gm_spmv<16, RESID, JAC>), g, t, 0, s, n, rp, ci, vals, dinv, x, b, y, partial, st); break;
+        case 32: hipLaunchKernelGGL((k_gm_spmv<32, RESID, JAC>), g, t, 0, s, n, rp, ci, vals, dinv, x, b, y, partial, st); break;
+        default: hipLaunchKernelGGL((k_gm_spmv<64, RESID, JAC>), g, t, 0, s, n, rp, ci, vals, dinv, x, b, y, partial, st); break;
     }
+}
+
+// b - K x; K D^-1 x (Jacobi) or K x (the ILU path applies its factors before)
+template <bool RESID>
+void launch_spmv(int L, int32_t n, const int32_t* rp, const int32_t* ci, const double* vals, const double* dinv,
+                 const double* x, const double* b, double* y, double* partial, const State* st, hipStream_t s,
+                 bool jacobi = true) {
+    if (RESID || jacobi) launch_spmv_t<RESID, true>(L, n, rp, ci, vals, dinv, x, b, y, partial, st, s);
+    else launch_spmv_t<RESID, false>(L, n, rp, ci, vals, dinv, x, b, y, partial, st, s);
 }
 
 }  // namespace gm
@@ -313,10 +343,12 @@ void launch_spmv(int L, int32_t n, const int32_t* rp, const int32_t* ci, const d
 
 using namespace mgpu;
 
-extern "C" mgpu_status mgpu_gmres_solve(int32_t n, const int32_t* rp, const int32_t* ci, const double* vals,
-                                        const double* b, double* x, double rtol, int32_t restart, int32_t max_iter,
-                                        void* stream, mgpu_cg_info* info) {
-    MGPU_GUARD_BEGIN
+namespace {
+
+// Restarted GMRES, right-preconditioned by Jacobi (pc == nullptr) or by incomplete LU factors.
+mgpu_status gmres_run(int32_t n, const int32_t* rp, const int32_t* ci, const double* vals, const double* b, double* x,
+                      double rtol, int32_t restart, int32_t max_iter, const mgpu_ilu* pc, void* stream,
+                      mgpu_cg_info* info) {
     if (n < 0) return set_error(MGPU_ERR_INVALID_ARG, "negative size");
     if (n > 0 && (!rp || !ci || !vals || !b || !x)) return set_error(MGPU_ERR_INVALID_ARG, "null device pointer");
     if (!(rtol >= 0.0)) return set_error(MGPU_ERR_INVALID_ARG, "rtol must be >= 0");
@@ -326,8 +358,8 @@ extern "C" mgpu_status mgpu_gmres_solve(int32_t n, const int32_t* rp, const int3
     if (n == 0) return MGPU_OK;
     hipStream_t s = static_cast<hipStream_t>(stream);
     const int64_t ldv = ((int64_t)n + 31) / 32 * 32;
-    // work: V [(restart + 1) ldv], dinv [ldv], partials [(restart + 1) kBlocks], the state
-    const size_t n_doubles = (size_t)(restart + 2) * ldv + (size_t)(gm::kMaxRestart + 1) * gm::kBlocks;
+    // work: V [(restart + 1) ldv], dinv [ldv], z [ldv] (ILU), partials [(restart + 1) kBlocks], the state
+    const size_t n_doubles = (size_t)(restart + 3) * ldv + (size_t)(gm::kMaxRestart + 1) * gm::kBlocks;
     void* base = nullptr;
     void* stp = nullptr;
     GM_TRY(hipMalloc(&base, n_doubles * sizeof(double)));
@@ -348,7 +380,9 @@ extern "C" mgpu_status mgpu_gmres_solve(int32_t n, const int32_t* rp, const int3
     } guard{base, stp, s};
     double* V = static_cast<double*>(base);
     double* dinv = V + (size_t)(restart + 1) * ldv;
-    double* partial = dinv + ldv;
+    double* zb = dinv + ldv;
+    double* partial = zb + ldv;
+    const bool jac = pc == nullptr;
     gm::State* st = static_cast<gm::State*>(stp);
     int32_t nnz = 0;
     GM_TRY(hipMemcpyAsync(&nnz, rp + n, sizeof(int32_t), hipMemcpyDeviceToHost, s));
@@ -373,7 +407,14 @@ extern "C" mgpu_status mgpu_gmres_solve(int32_t n, const int32_t* rp, const int3
         for (int j = 0; j < restart; ++j) {
             double* vj = V + (size_t)j * ldv;
             double* w = V + (size_t)(j + 1) * ldv;
-            gm::launch_spmv<false>(L, n, rp, ci, vals, dinv, vj, nullptr, w, partial, st, s);
+            if (jac) {
+                gm::launch_spmv<false>(L, n, rp, ci, vals, dinv, vj, nullptr, w, partial, st, s);
+            } else {
+                // w = K M^-1 v_j
+                hipLaunchKernelGGL(gm::k_gm_copy, gv, t, 0, s, n, vj, zb, st);
+                ilu_apply_launch(pc, zb, &st->cycle_end, s);
+                gm::launch_spmv<false>(L, n, rp, ci, vals, dinv, zb, nullptr, w, partial, st, s, false);
+            }
             hipLaunchKernelGGL(gm::k_gm_zero_col, dim3(1), dim3(gm::kMaxRestart + 1), 0, s, st);
             for (int pass = 0; pass < 2; ++pass) {
                 for (int i0 = 0; i0 <= j; i0 += gm::kChunk)
@@ -385,7 +426,14 @@ extern "C" mgpu_status mgpu_gmres_solve(int32_t n, const int32_t* rp, const int3
             hipLaunchKernelGGL(gm::k_gm_scale, gv, t, 0, s, n, w, st);
         }
         hipLaunchKernelGGL(gm::k_gm_solve_y, dim3(1), dim3(1), 0, s, st);
-        hipLaunchKernelGGL(gm::k_gm_update, gv, t, 0, s, n, V, ldv, dinv, x, st);
+        if (jac) {
+            hipLaunchKernelGGL(gm::k_gm_update, gv, t, 0, s, n, V, ldv, dinv, x, st);
+        } else {
+            // x += M^-1 V y
+            hipLaunchKernelGGL(gm::k_gm_vy, gv, t, 0, s, n, V, ldv, zb, st);
+            ilu_apply_launch(pc, zb, nullptr, s);
+            hipLaunchKernelGGL(gm::k_gm_add, gv, t, 0, s, n, zb, x);
+        }
         GM_TRY(hipGetLastError());
         GM_TRY(hipMemcpyAsync(&done, &st->done, sizeof(int32_t), hipMemcpyDeviceToHost, s));
         GM_TRY(hipMemcpyAsync(&iters, &st->iter, sizeof(int32_t), hipMemcpyDeviceToHost, s));
@@ -410,5 +458,25 @@ extern "C" mgpu_status mgpu_gmres_solve(int32_t n, const int32_t* rp, const int3
         info->true_rel_residual = r0 > 0.0 ? res_true / r0 : 0.0;
     }
     return MGPU_OK;
+}
+
+}  // namespace
+
+extern "C" mgpu_status mgpu_gmres_solve(int32_t n, const int32_t* rp, const int32_t* ci, const double* vals,
+                                        const double* b, double* x, double rtol, int32_t restart, int32_t max_iter,
+                                        void* stream, mgpu_cg_info* info) {
+    MGPU_GUARD_BEGIN
+    return gmres_run(n, rp, ci, vals, b, x, rtol, restart, max_iter, nullptr, stream, info);
+    MGPU_GUARD_END
+}
+
+extern "C" mgpu_status mgpu_gmres_solve_ilu(int32_t n, const int32_t* rp, const int32_t* ci, const double* vals,
+                                            const double* b, double* x, double rtol, int32_t restart,
+                                            int32_t max_iter, const mgpu_ilu* ilu, void* stream, mgpu_cg_info* info) {
+    MGPU_GUARD_BEGIN
+    if (!ilu) return set_error(MGPU_ERR_INVALID_ARG, "null factor");
+    if (ilu->n != n) return set_error(MGPU_ERR_INVALID_ARG, "the factor is of another size");
+    if (!ilu->factored) return set_error(MGPU_ERR_STATE, "the factor has not been computed");
+    return gmres_run(n, rp, ci, vals, b, x, rtol, restart, max_iter, ilu, stream, info);
     MGPU_GUARD_END
 }

@@ -167,6 +167,125 @@ def test_gmres_solves_the_unsymmetric_nitsche_system():
     assert ig.converged == 1 and ic.converged == 1 and rel(xg, xc) <= 1e-9
 
 
+def _dense_fill_levels(rp, ci, n, k_max):
+    """Level of fill of every entry of ILU(k) (Saad, Iterative Methods, Alg. 10.5, dense form): 0 on
+    the matrix pattern and the diagonal, lev(i,j) = min over kept k < min(i, j) of lev(i,k) +
+    lev(k,j) + 1, entries kept while lev <= k_max.  A restatement independent of the library's
+    row-merge version, for small n."""
+    big = 1 << 20
+    lev = np.full((n, n), big, dtype=np.int64)
+    for i in range(n):
+        lev[i, ci[rp[i]:rp[i + 1]]] = 0
+        lev[i, i] = 0
+    for i in range(n):
+        for k in range(i):
+            if lev[i, k] > k_max:
+                continue
+            upd = lev[i, k] + lev[k, k + 1:] + 1
+            lev[i, k + 1:] = np.minimum(lev[i, k + 1:], np.where(lev[k, k + 1:] <= k_max, upd, big))
+    return lev <= k_max
+
+
+def _lu_product(n, frp, fci, lu):
+    F = sp.csr_matrix((lu, fci, frp), shape=(n, n))
+    L = sp.tril(F, k=-1) + sp.identity(n)
+    U = sp.triu(F)
+    return (L @ U).tocsr()
+
+
+def test_ilu0_factors_reproduce_the_matrix_on_its_pattern():
+    """ILU(0): L U equals K on K's pattern (its defining property), the factor pattern is K's, the
+    level schedules have more than one level, and a second factorisation is bit-identical."""
+    P, S, sides, rp, ci, vals, b = _unsymmetric_nitsche_system()
+    n = S.n_rows
+    A = SV.DeviceCSR(rp, ci, vals)
+    F = SV.ILU(A, fill_level=0)
+    m, frp, fci, lu, lev = F.factors()
+    print("ILU(0) rows", n, "levels", lev)
+    assert m == n and np.array_equal(frp, rp) and np.array_equal(fci, ci)
+    assert lev[0] > 1 and lev[1] > 1
+    K = sp.csr_matrix((vals, ci, rp), shape=(n, n))
+    LU = _lu_product(n, frp, fci, lu)
+    mask = sp.csr_matrix((np.ones(len(ci)), ci, rp), shape=(n, n))
+    diff = (LU - K).multiply(mask)
+    assert abs(diff).max() <= 1e-12 * abs(K).max()
+    # off the pattern L U carries the dropped fill: it is not K there
+    assert abs((LU - K) - diff).max() > 1e-6 * abs(K).max()
+    F.factor(A)
+    lu2 = F.factors()[3]
+    assert np.array_equal(lu.view(np.uint64), lu2.view(np.uint64))
+    F.close()
+
+
+@pytest.mark.parametrize("k", [1, 2])
+def test_iluk_pattern_matches_the_level_of_fill_rule(k):
+    spec, keep = M.make_spec(2, (6, 6), spheres=[[0.5, 0.5, 0.0, 0.3]], iwg=M.IWG_STRUC_LINEAR_BULK)
+    P = M.Problem(spec, keep)
+    S = M.Set(P.desc, keepalive=P)
+    u = np.zeros(S.n_rows)
+    vals, _ = S.compute_jacobian_and_residual(u)
+    rp, ci = S.pattern()
+    n = S.n_rows
+    # a positive shift keeps every pivot away from zero (the bulk-only K is singular)
+    K = sp.csr_matrix((vals, ci, rp), shape=(n, n)) + sp.identity(n) * abs(vals).max()
+    K.sort_indices()
+    A = SV.DeviceCSR(K.indptr, K.indices, K.data)
+    F = SV.ILU(A, fill_level=k)
+    m, frp, fci, lu, lev = F.factors()
+    want = _dense_fill_levels(K.indptr, K.indices, n, k)
+    got = np.zeros((n, n), dtype=bool)
+    for i in range(n):
+        got[i, fci[frp[i]:frp[i + 1]]] = True
+    assert np.array_equal(got, want)
+    assert frp[-1] > len(K.indices)   # fill was added
+    LU = _lu_product(n, frp, fci, lu)
+    diff = (LU - K).multiply(sp.csr_matrix((np.ones(len(K.indices)), K.indices, K.indptr), shape=(n, n)))
+    assert abs(diff).max() <= 1e-12 * abs(K).max()
+    F.close()
+
+
+def test_ilu_with_full_fill_is_the_exact_lu():
+    """With every fill level kept the incomplete factors are the complete LU (no pivoting on this
+    diagonally shifted matrix): (L U)^-1 b is the direct solution."""
+    spec, keep = M.make_spec(2, (5, 4), spheres=[[0.5, 0.5, 0.0, 0.3]])
+    P = M.Problem(spec, keep)
+    S = M.Set(P.desc, keepalive=P)
+    vals, _ = S.compute_jacobian_and_residual(np.zeros(S.n_rows))
+    rp, ci = S.pattern()
+    n = S.n_rows
+    K = (sp.csr_matrix((vals, ci, rp), shape=(n, n)) + sp.identity(n) * abs(vals).max()).tocsr()
+    K.sort_indices()
+    A = SV.DeviceCSR(K.indptr, K.indices, K.data)
+    F = SV.ILU(A, fill_level=n)
+    bvec = np.random.default_rng(SEED + 8).uniform(-1.0, 1.0, n)
+    z = F.apply(bvec)
+    x = spla.spsolve(K.tocsc(), bvec)
+    assert rel(z, x) <= 1e-12
+    F.close()
+
+
+def test_gmres_with_ilu_preconditioner():
+    """GMRES right-preconditioned by ILU(0) and ILU(1) solves the unsymmetric Nitsche system to the
+    direct solution in fewer Arnoldi steps than with Jacobi, bit-identically on a second solve."""
+    P, S, sides, rp, ci, vals, b = _unsymmetric_nitsche_system()
+    K = sp.csr_matrix((vals, ci, rp), shape=(S.n_rows, S.n_rows))
+    x_ref = spla.spsolve(K.tocsc(), b)
+    A = SV.DeviceCSR(rp, ci, vals)
+    _, info_j = SV.gmres(A, b, rtol=1e-12, restart=30)
+    for k in (0, 1):
+        F = SV.ILU(A, fill_level=k)
+        x, info = SV.gmres(A, b, rtol=1e-12, restart=30, ilu=F)
+        print("gmres(30) + ILU(%d): %d steps (Jacobi %d), true rel residual %.2e" % (k, info.iterations,
+                                                                                    info_j.iterations,
+                                                                                    info.true_rel_residual))
+        assert info.converged == 1 and info.true_rel_residual <= 1e-11
+        assert info.iterations < info_j.iterations
+        assert rel(x, x_ref) <= 1e-9
+        x2, info2 = SV.gmres(A, b, rtol=1e-12, restart=30, ilu=F)
+        assert np.array_equal(x.view(np.uint64), x2.view(np.uint64)) and info2.iterations == info.iterations
+        F.close()
+
+
 def _traction_problem(P, n):
     """Consistent loads of a unit x-traction on the lattice faces of x = 1, the x = 0 face clamped."""
     nodes, X = adof_coords(P.arrays(), 3, (n, n, n))
