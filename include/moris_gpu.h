@@ -179,6 +179,20 @@ mgpu_status mgpu_problem_desc(const mgpu_problem* p, mgpu_set_desc* out);
 mgpu_status mgpu_problem_counts(const mgpu_problem* p, int64_t counts[6]);
 mgpu_status mgpu_problem_destroy(mgpu_problem* p);
 
+/* Elemental extraction operators (T-matrices from the integration cells to the adofs) of a set
+   descriptor, written to a file: the content of the reference's T-matrix output
+   (Integration_Mesh::save_elemental_T_matrices_to_file, projects/MTK/src/mesh/
+   cl_MTK_Integration_Mesh_write_T_matrices.cpp:38-160, 684-809; Workflow_HMR_XTK::
+   output_T_matrices, projects/WRK/src/cl_WRK_Workflow_HMR_XTK.cpp:542-575), which stores HDF5.
+   For every integration cell c (a cut element's sub-cells in order, or an uncut element's IP-cell
+   box as one cell with its 2^dim corners) the weights E_c[v][a] = sum_b N_b(xi_v) T[b][a] map the
+   element's adofs to the cell's vertices.  Host arrays only; synchronous.  Little-endian layout:
+     char magic[8] = "MGPUEXT1"; int32 dim, n_bases, n_adofs_per_type, 0; int64 n_cells;
+     per cell: int32 element, n_vertices, n_adofs; int32 adof[n_adofs] (ascending, type-local);
+               double E[n_vertices][n_adofs].
+   *n_cells (optional) receives the cell count.                                                 */
+mgpu_status mgpu_write_extraction_operators(const mgpu_set_desc* desc, const char* path, int64_t* n_cells);
+
 /* Set life cycle (fem::Set + MSI::Equation_Set scratch + SOL graph; cl_FEM_Set.cpp:35-100,
    :132 initialize_set, cl_DLA_Solver_Interface.cpp:20-54 build_graph).                          */
 mgpu_status mgpu_set_create(const mgpu_set_desc* desc, mgpu_set** out);

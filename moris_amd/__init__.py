@@ -86,6 +86,7 @@ class SetDesc(ctypes.Structure):
 # every symbol include/moris_gpu.h declares
 EXPORTED = [
     "mgpu_problem_build", "mgpu_problem_desc", "mgpu_problem_counts", "mgpu_problem_destroy",
+    "mgpu_write_extraction_operators",
     "mgpu_set_create", "mgpu_set_destroy", "mgpu_set_options", "mgpu_set_stream", "mgpu_pattern_get",
     "mgpu_element_topology", "mgpu_element_topology_csr", "mgpu_set_assemble", "mgpu_set_assemble_host", "mgpu_set_stats",
     "mgpu_set_statistics",
@@ -250,6 +251,38 @@ class Problem:
             self.close()
         except Exception:
             pass
+
+
+def write_extraction_operators(desc, path):
+    """mgpu_write_extraction_operators: the elemental extraction operators (T-matrices from the
+    integration cells to the adofs) of a set descriptor to `path`; returns the cell count."""
+    lib = load_library()
+    lib.mgpu_write_extraction_operators.argtypes = [ctypes.POINTER(SetDesc), ctypes.c_char_p,
+                                                    ctypes.POINTER(ctypes.c_int64)]
+    n = ctypes.c_int64()
+    _check(lib.mgpu_write_extraction_operators(ctypes.byref(desc), os.fsencode(path), ctypes.byref(n)),
+           "mgpu_write_extraction_operators")
+    return n.value
+
+
+def read_extraction_operators(path):
+    """Read a file of write_extraction_operators: (header dict, list of (element, adof ids [na],
+    weights [n_vertices][na]))."""
+    raw = open(path, "rb").read()
+    if raw[:8] != b"MGPUEXT1":
+        raise MgpuError("not an extraction-operator file: %s" % path)
+    dim, nb, n_adofs, _ = np.frombuffer(raw, np.int32, 4, 8)
+    n_cells = int(np.frombuffer(raw, np.int64, 1, 24)[0])
+    off, cells = 32, []
+    for _ in range(n_cells):
+        e, nv, na = (int(x) for x in np.frombuffer(raw, np.int32, 3, off))
+        off += 12
+        ids = np.frombuffer(raw, np.int32, na, off).copy()
+        off += 4 * na
+        E = np.frombuffer(raw, np.float64, nv * na, off).reshape(nv, na).copy()
+        off += 8 * nv * na
+        cells.append((e, ids, E))
+    return {"dim": int(dim), "n_bases": int(nb), "n_adofs_per_type": int(n_adofs), "n_cells": n_cells}, cells
 
 
 def desc_from_arrays(desc, arrays):

@@ -23,6 +23,7 @@ HOST_SOURCES = [
     "setup/problem.cpp",
     "setup/enrich.cpp",
     "set/set_host.cpp",
+    "set/extraction.cpp",
     "learned/dataset.cpp",
 ]
 HIP_SOURCES = [
