@@ -112,6 +112,11 @@ typedef struct mgpu_problem_spec {
                                   not touch inside a basis' support share no adof.  Sets with a
                                   replicated basis or two elements in one cell take the generic
                                   path.                                                          */
+    int32_t ghost;             /* 1: the set descriptor couples the two elements of every ghost
+                                  facet (mgpu_problem_facets with MGPU_SIDE_GHOST) in its pattern,
+                                  so that a ghost side set can be assembled into it; such sets
+                                  take the generic path (level-0 lattices, HEX8 / QUAD4, without
+                                  enrichment)                                                    */
 } mgpu_problem_spec;
 
 /* Uniform lattice the elements of a set sit on (HMR level 0).  n[0] == 0: no lattice. */
@@ -166,6 +171,14 @@ typedef struct mgpu_set_desc {
     /* Per-element level-set values at the IP-cell vertices, [n_elements][n_bases] (optional,
        needed by MGPU_QUAD_LEARNED).                                                             */
     const double*  vertex_phi;
+
+    /* Element pairs whose adofs are coupled in the pattern beyond the elements themselves: the
+       leader and follower of each facet of a double-sided side set (ghost facets), [n][2]
+       (optional; the graph of Solver_Interface::build_graph covers every equation object, single-
+       and double-sided, cl_Sparse_Matrix_EpetraFECrs.cpp:195-240).  Sets with couplings take the
+       generic path.                                                                              */
+    int32_t        n_couplings;
+    const int32_t* couplings;
 } mgpu_set_desc;
 
 typedef struct mgpu_problem mgpu_problem;   /* host-side workload (cut mesh, numbering)        */

@@ -37,6 +37,18 @@ extern "C" {
 #define MGPU_IWG_STRUC_LINEAR_DIRICHLET_SYMMETRIC_NITSCHE   5
 #define MGPU_IWG_STRUC_LINEAR_DIRICHLET_UNSYMMETRIC_NITSCHE 6
 #define MGPU_IWG_STRUC_LINEAR_NEUMANN                       7
+/* Double-sided ghost stabilisation on the faces between a cut element and its neighbours
+   (IWG_Diffusion_Virtual_Work_Ghost, projects/FEM/INT/src/IWG/cl_FEM_IWG_Diffusion_Virtual_Work_
+   Ghost.cpp; IWG_Isotropic_Struc_Linear_Virtual_Work_Ghost, cl_FEM_IWG_Isotropic_Struc_Linear_
+   Virtual_Work_Ghost.cpp:50-255), order 1, with SP_Ghost_Virtual_Work gamma = alpha h
+   (projects/FEM/INT/src/SP/cl_FEM_SP_Ghost_Virtual_Work.cpp:30-41; h the cell length measure of
+   the double-sided cluster, sqrt(4 A / pi) in 3D, A in 2D, A the face measure):
+     R_L += w* gamma T_L^T [[t]],  R_F -= w* gamma T_F^T [[t]],  [[t]] = N_n (D_L B_L u_L - D_F B_F u_F),
+   T_X = N_n B_X the test traction of side X without its material (diffusion: n^T grad N_X,
+   [[t]] = n^T (k_L grad u_L - k_F grad u_F)), N_n the Voigt traction operator of the leader's
+   outward normal n.                                                                             */
+#define MGPU_IWG_SPATIALDIFF_GHOST                          8
+#define MGPU_IWG_STRUC_LINEAR_GHOST                         9
 
 /* Side geometry: the facet kind of one side set (mtk side geometry of the IG cells). */
 #define MGPU_GEOM_LINE2  5   /* 2D: a side of a QUAD4 or TRI3 IG cell, BAR_3                          */
@@ -52,6 +64,10 @@ extern "C" {
 #define MGPU_SIDE_YMAX       4
 #define MGPU_SIDE_ZMIN       5
 #define MGPU_SIDE_ZMAX       6
+#define MGPU_SIDE_GHOST     -1   /* ghost facets: the faces between two material elements of a level-0
+                                    lattice of which at least one is cut and whose shared face holds a
+                                    lattice node strictly in the material; leader the lower cell index,
+                                    the facet the leader's face (QUAD4 / LINE2), n its outward normal */
 
 /* A side set: facets of one geometry, each on an element of the bulk set.  Facet vertices are given
    in the parametric coordinates of the element's IP cell, [n_facets][n_vertices][spatial_dim]
@@ -64,15 +80,21 @@ typedef struct mgpu_sideset_desc {
     const int32_t* facet_elem; /* [n_facets] element of the bulk set                               */
     const double*  facet_xi;   /* [n_facets][n_vertices][dim]                                       */
     const double*  facet_normal; /* [n_facets][dim]                                                  */
-    double nitsche_alpha;      /* Dirichlet: SP parameter alpha of gamma = alpha * material / h      */
+    double nitsche_alpha;      /* Dirichlet: SP parameter alpha of gamma = alpha * material / h;
+                                  ghost: alpha of gamma = alpha h                                   */
     double value[3];           /* Dirichlet: prescribed u_D per field; Neumann: flux / traction t_N  */
+    const int32_t* facet_follower; /* ghost IWGs: [n_facets] the follower element.  The facet is the
+                                  whole face the two IP cells share: equal boxes, the follower's
+                                  the leader's shifted along the facet normal (an axis)             */
 } mgpu_sideset_desc;
 
 typedef struct mgpu_sideset mgpu_sideset;
 
 /* A side set on the bulk set `bulk` (which must outlive it): lattice sets with nodal T rows, and
    generic-path sets (p = 2 B-spline rows on HEX8 / QUAD4 cells, HEX27 cells, THB rows) with up to
-   64 adofs per dof type and element.                                                            */
+   64 adofs per dof type and element.  Ghost side sets need HEX8 / QUAD4 cells, at most 64 adofs
+   per dof type on the two elements of a facet together, and a bulk set whose descriptor couples
+   every facet's two elements (mgpu_set_desc.couplings).                                         */
 mgpu_status mgpu_sideset_create(mgpu_set* bulk, const mgpu_sideset_desc* desc, mgpu_sideset** out);
 mgpu_status mgpu_sideset_destroy(mgpu_sideset* ss);
 /* ADD the side set's K and R into the bulk set's CSR values and R (DEVICE arrays; call after
@@ -90,6 +112,8 @@ mgpu_status mgpu_problem_facets(const mgpu_problem* p, int32_t where, int32_t si
 mgpu_status mgpu_facets_get(const mgpu_facets* f, int32_t* n_facets, const int32_t** facet_elem,
                             const double** facet_xi, const double** facet_normal);
 mgpu_status mgpu_facets_destroy(mgpu_facets* f);
+/* The follower element of every facet of a MGPU_SIDE_GHOST facet list (NULL for other lists). */
+mgpu_status mgpu_facets_followers(const mgpu_facets* f, const int32_t** facet_follower);
 
 #ifdef __cplusplus
 }

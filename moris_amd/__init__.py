@@ -54,6 +54,7 @@ class ProblemSpec(ctypes.Structure):
         ("lagrange_order", ctypes.c_int32),
         ("refine_levels", ctypes.c_int32),
         ("enrich", ctypes.c_int32),
+        ("ghost", ctypes.c_int32),
     ]
 
 
@@ -80,6 +81,8 @@ class SetDesc(ctypes.Structure):
         ("lattice", Lattice),
         ("elem_cell", _c_i32p),
         ("vertex_phi", _c_f64p),
+        ("n_couplings", ctypes.c_int32),
+        ("couplings", _c_i32p),
     ]
 
 
@@ -166,7 +169,7 @@ def _ptr(a, ctype):
 def make_spec(dim, n, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), iwg=IWG_SPATIALDIFF_BULK, spheres=None,
               ls_kind=None, gyroid_scale=1.0, plane=(0.0, 0.0, 0.0, 0.0), material_phase=1, E=1.0, nu=0.3,
               k_mu=0.0, k_sigma=0.0, seed=20261202, bspline_order=1, lagrange_order=1, refine_levels=0,
-              enrich=0):
+              enrich=0, ghost=0):
     """Build a ProblemSpec; returns (spec, keepalive) -- keep the second value alive with the spec."""
     s = ProblemSpec()
     s.dim = dim
@@ -192,6 +195,7 @@ def make_spec(dim, n, lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), iwg=IWG_SPATIALDIF
     s.lagrange_order = lagrange_order
     s.refine_levels = refine_levels
     s.enrich = enrich
+    s.ghost = ghost
     return s, keep
 
 

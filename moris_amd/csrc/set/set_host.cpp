@@ -93,12 +93,33 @@ mgpu_status build_generic(const mgpu_set_desc& d, SetHost& h, std::string* err) 
         for (int32_t e = 0; e < h.n_el; ++e)
             for (int32_t a : loc[(size_t)e]) a_el[(size_t)fill[(size_t)a]++] = e;
     }
+    // coupled element pairs (double-sided sets): each adof of one element of a pair is also a
+    // neighbour of every adof of the other
+    const int32_t ncp = d.couplings ? d.n_couplings : 0;
+    std::vector<int64_t> c_ptr((size_t)N + 1, 0);
+    std::vector<int32_t> c_el;
+    if (ncp > 0) {
+        for (int32_t k = 0; k < ncp; ++k)
+            for (int s2 = 0; s2 < 2; ++s2)
+                for (int32_t a : loc[(size_t)d.couplings[2 * k + s2]]) c_ptr[(size_t)a + 1]++;
+        for (int32_t a = 0; a < N; ++a) c_ptr[(size_t)a + 1] += c_ptr[(size_t)a];
+        c_el.resize((size_t)c_ptr[(size_t)N]);
+        std::vector<int64_t> fill(c_ptr.begin(), c_ptr.end() - 1);
+        for (int32_t k = 0; k < ncp; ++k)
+            for (int s2 = 0; s2 < 2; ++s2)
+                for (int32_t a : loc[(size_t)d.couplings[2 * k + s2]])
+                    c_el[(size_t)fill[(size_t)a]++] = d.couplings[2 * k + 1 - s2];   // the partner element
+    }
     // neighbours of each type-local adof
     std::vector<std::vector<int32_t>> nbr((size_t)N);
     parallel_for(N, [&](int64_t a) {
         std::vector<int32_t>& v = nbr[(size_t)a];
         for (int64_t q = a_ptr[(size_t)a]; q < a_ptr[(size_t)a + 1]; ++q) {
             const auto& u = loc[(size_t)a_el[(size_t)q]];
+            v.insert(v.end(), u.begin(), u.end());
+        }
+        for (int64_t q = c_ptr[(size_t)a]; ncp > 0 && q < c_ptr[(size_t)a + 1]; ++q) {
+            const auto& u = loc[(size_t)c_el[(size_t)q]];
             v.insert(v.end(), u.begin(), u.end());
         }
         std::sort(v.begin(), v.end());
@@ -346,7 +367,16 @@ mgpu_status build_set_host(const mgpu_set_desc& d, SetHost& h, std::string* err)
             last = nadof[nid];
         }
     }
-    if (!nodal || !has_lattice || hex27 || !one_per_node) {
+    if (d.n_couplings < 0 || (d.n_couplings > 0 && !d.couplings)) {
+        *err = "bad coupling arrays";
+        return MGPU_ERR_INVALID_ARG;
+    }
+    for (int32_t k = 0; k < 2 * d.n_couplings; ++k)
+        if (d.couplings[k] < 0 || d.couplings[k] >= h.n_el) {
+            *err = "coupled element out of range";
+            return MGPU_ERR_INVALID_ARG;
+        }
+    if (!nodal || !has_lattice || hex27 || !one_per_node || d.n_couplings > 0) {
         h.generic = true;
         const mgpu_status rc = build_generic(d, h, err);
         // the HEX27 kernel stages T and T^T K_p T blocks of 27 x na_t in LDS beside K_p

@@ -20,6 +20,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <iterator>
 #include <cmath>
 #include <memory>
 #include <numeric>
@@ -463,6 +464,218 @@ __global__ __launch_bounds__(256) void k_side_gather(int64_t n, const int64_t* p
     out[dst[t]] += v;
 }
 
+// ------------------------------------------------------------------------------------------------
+// Ghost facets (double-sided): one block per facet, the leader's and the follower's NB bases side by
+// side (2 NB columns per dof type), K_p in LDS from the face points (QUAD_2x2 / BAR_3 on the shared
+// face), then both elements' T rows against the union of their adofs.
+//   K_p[(f,X,a),(g,Y,b)] += w* gamma s_X s_Y t_{X a f} . T_{Y b g},   s_L = 1, s_F = -1,
+//   t_{X a f} = N_n eps(N_{X a} e_f)         (test traction, no material),
+//   T_{Y b g} = N_n D_Y eps(N_{Y b} e_g)     (traction of side Y; diffusion: t = n . grad N,
+//                                              T = k_Y n . grad N)
+// ------------------------------------------------------------------------------------------------
+struct GhostView {
+    int32_t n_facets, N, nm;
+    const int32_t* leader;
+    const int32_t* follower;
+    const int32_t* axis;       // [n] axis of the face normal
+    const double* side;        // [n] sign of the leader's outward normal along it
+    const double* gamma;       // [n] alpha h
+    const int64_t* cl_aptr;    // [n + 1] into cl_adof
+    const int32_t* cl_adof;    // union of the two elements' adofs (global rows, types concatenated)
+    const int64_t* map_ptr;    // [n + 1] into map
+    const int32_t* map;        // element-local column -> union column: leader's, then follower's
+    const int64_t* ke_off;
+    const int64_t* re_off;
+    double* ke;
+    double* re;
+};
+
+template <int DIM, int NF>
+__global__ __launch_bounds__(256) void k_ghost(GhostView s, GenView gv, const double* u) {
+    constexpr int NB = DIM == 3 ? 8 : 4;
+    constexpr int NB2 = 2 * NB;
+    constexpr int NA2 = NF * NB2;
+    constexpr int NQ = DIM == 3 ? kQuad2Points : kBar3Points;
+    constexpr int NS = DIM == 3 ? 6 : 3;
+    constexpr int M = kSideMaxNat;
+    __shared__ double Kp[NA2 * NA2];
+    __shared__ double Gq[NB2 * DIM];
+    __shared__ double Tm[NB2 * M];
+    __shared__ double W[NB2 * M];
+    __shared__ double ue[NF * M], up[NA2];
+    __shared__ double Dm[2][NS * NS];
+    __shared__ double km[2];
+    const int c = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int eX[2] = {s.leader[c], s.follower[c]};
+    const int ax = s.axis[c];
+    const double sd = s.side[c];
+    double ext[DIM];
+    for (int d = 0; d < DIM; ++d) ext[d] = gv.ip_box[(size_t)eX[0] * 2 * DIM + DIM + d];
+    double n[DIM];
+    for (int d = 0; d < DIM; ++d) n[d] = d == ax ? sd : 0.0;
+    if (tid < 2) {
+        const int e = eX[tid];
+        if constexpr (NF == 1) {
+            km[tid] = gv.material[(size_t)e * s.nm];
+        } else {
+            double D[NS][NS];
+            elastic_D<DIM>(gv.material[(size_t)e * s.nm], gv.material[(size_t)e * s.nm + 1], D);
+            for (int i = 0; i < NS; ++i)
+                for (int j = 0; j < NS; ++j) Dm[tid][i * NS + j] = D[i][j];
+        }
+    }
+    for (int t = tid; t < NA2 * NA2; t += 256) Kp[t] = 0.0;
+    // the face's other axes (ascending) and the point's weight w * detJ of the face
+    int oa[2] = {0, 0}, no = 0;
+    for (int d = 0; d < DIM; ++d)
+        if (d != ax) oa[no++] = d;
+    double face_det = 1.0;
+    for (int k = 0; k < DIM - 1; ++k) face_det *= 0.5 * ext[oa[k]];
+    const double gamma = s.gamma[c];
+    for (int q = 0; q < NQ; ++q) {
+        __syncthreads();   // the previous point's gradients are no longer read (and Dm / km are set)
+        double w;
+        double xi[DIM];
+        if constexpr (DIM == 3) {
+            xi[oa[0]] = kQuad2Xi[q][0];
+            xi[oa[1]] = kQuad2Xi[q][1];
+            w = kQuad2W[q] * face_det;
+        } else {
+            xi[oa[0]] = kBar3Xi[q];
+            w = kBar3W[q] * face_det;
+        }
+        if (tid < NB2) {
+            const int X = tid / NB, a = tid % NB;
+            double x[DIM], g[DIM], N;
+            for (int d = 0; d < DIM; ++d) x[d] = xi[d];
+            x[ax] = X == 0 ? sd : -sd;   // the leader's face, and the same face seen from the follower
+            side_basis<DIM, NB>(a, x, ext, N, g);
+            for (int d = 0; d < DIM; ++d) Gq[tid * DIM + d] = g[d];
+        }
+        __syncthreads();
+        for (int pr = tid; pr < NB2 * NB2; pr += 256) {
+            const int i = pr / NB2, j = pr % NB2;   // (X, a), (Y, b)
+            const int X = i / NB, Y = j / NB;
+            const double sg = (X == Y ? 1.0 : -1.0) * w * gamma;
+            const double* ga = Gq + i * DIM;
+            const double* gb = Gq + j * DIM;
+            if constexpr (NF == 1) {
+                double na = 0.0, nb = 0.0;
+                for (int d = 0; d < DIM; ++d) {
+                    na += n[d] * ga[d];
+                    nb += n[d] * gb[d];
+                }
+                Kp[i * NA2 + j] += sg * na * km[Y] * nb;
+            } else {
+                // t (no material) of (X, a, f) and T (side Y's material) of (Y, b, g)
+                double t[NF][DIM], T[NF][DIM];
+                for (int f = 0; f < NF; ++f) {
+                    double ea[NS], eb[NS];
+                    for (int k = 0; k < NS; ++k) ea[k] = eb[k] = 0.0;
+                    // engineering strain of N e_f: [xx, yy, zz, yz, xz, xy] (2D: [xx, yy, xy])
+                    if constexpr (DIM == 3) {
+                        const int sh1 = f == 0 ? 5 : (f == 1 ? 5 : 4), sh2 = f == 0 ? 4 : 3;
+                        const int o1 = f == 0 ? 1 : 0, o2 = f == 2 ? 1 : 2;
+                        ea[f] = ga[f];
+                        eb[f] = gb[f];
+                        ea[sh1] = ga[o1];
+                        eb[sh1] = gb[o1];
+                        ea[sh2] = ga[o2];
+                        eb[sh2] = gb[o2];
+                    } else {
+                        ea[f] = ga[f];
+                        eb[f] = gb[f];
+                        ea[2] = ga[1 - f];
+                        eb[2] = gb[1 - f];
+                    }
+                    double sb[NS];
+                    for (int r = 0; r < NS; ++r) {
+                        double v = 0.0;
+                        for (int k = 0; k < NS; ++k) v += Dm[Y][r * NS + k] * eb[k];
+                        sb[r] = v;
+                    }
+                    // N_n v: the Voigt traction operator of n
+                    if constexpr (DIM == 3) {
+                        t[f][0] = n[0] * ea[0] + n[2] * ea[4] + n[1] * ea[5];
+                        t[f][1] = n[1] * ea[1] + n[2] * ea[3] + n[0] * ea[5];
+                        t[f][2] = n[2] * ea[2] + n[1] * ea[3] + n[0] * ea[4];
+                        T[f][0] = n[0] * sb[0] + n[2] * sb[4] + n[1] * sb[5];
+                        T[f][1] = n[1] * sb[1] + n[2] * sb[3] + n[0] * sb[5];
+                        T[f][2] = n[2] * sb[2] + n[1] * sb[3] + n[0] * sb[4];
+                    } else {
+                        t[f][0] = n[0] * ea[0] + n[1] * ea[2];
+                        t[f][1] = n[1] * ea[1] + n[0] * ea[2];
+                        T[f][0] = n[0] * sb[0] + n[1] * sb[2];
+                        T[f][1] = n[1] * sb[1] + n[0] * sb[2];
+                    }
+                }
+                for (int f = 0; f < NF; ++f)
+                    for (int g2 = 0; g2 < NF; ++g2) {
+                        double v = 0.0;
+                        for (int d = 0; d < DIM; ++d) v += t[f][d] * T[g2][d];
+                        Kp[(f * NB2 + i) * NA2 + g2 * NB2 + j] += sg * v;
+                    }
+            }
+        }
+    }
+    // both elements' T against the union of their adofs, and the union's adof values
+    const int64_t m0 = s.map_ptr[c];
+    const int nL = gv.na_t[eX[0]];
+    const int64_t a0 = s.cl_aptr[c];
+    const int nu = (int)((s.cl_aptr[c + 1] - a0) / NF);
+    for (int t = tid; t < NB2 * M; t += 256) Tm[t] = 0.0;
+    __syncthreads();
+    if (tid < NB2) {
+        const int X = tid / NB, b = tid % NB, e = eX[X];
+        const int64_t mo = m0 + (X == 0 ? 0 : nL);
+        for (int k = gv.t_ptr[(size_t)e * NB + b]; k < gv.t_ptr[(size_t)e * NB + b + 1]; ++k)
+            Tm[tid * M + s.map[mo + gv.t_col[k]]] += gv.t_w[k];
+    }
+    for (int t = tid; t < NF * nu; t += 256) ue[t] = u[s.cl_adof[a0 + t]];
+    __syncthreads();
+    double upv = 0.0;
+    if (tid < NA2) {
+        const int f = tid / NB2, b = tid % NB2;
+        for (int x = 0; x < nu; ++x) upv += Tm[b * M + x] * ue[f * nu + x];
+    }
+    __syncthreads();
+    if (tid < NA2) up[tid] = upv;
+    __syncthreads();
+    double rp = 0.0;
+    if (tid < NA2)
+        for (int j = 0; j < NA2; ++j) rp += Kp[tid * NA2 + j] * up[j];
+    __syncthreads();
+    if (tid < NA2) up[tid] = rp;
+    __syncthreads();
+    double* re = s.re + s.re_off[c];
+    for (int t = tid; t < NF * nu; t += 256) {
+        const int f = t / nu, x = t % nu;
+        double v = 0.0;
+        for (int b = 0; b < NB2; ++b) v += Tm[b * M + x] * up[f * NB2 + b];
+        re[t] = v;
+    }
+    double* ke = s.ke + s.ke_off[c];
+    const int nat = NF * nu;
+    for (int f = 0; f < NF; ++f)
+        for (int g2 = 0; g2 < NF; ++g2) {
+            __syncthreads();
+            for (int t = tid; t < NB2 * nu; t += 256) {
+                const int b = t / nu, y = t % nu;
+                double v = 0.0;
+                for (int b2 = 0; b2 < NB2; ++b2) v += Kp[(f * NB2 + b) * NA2 + g2 * NB2 + b2] * Tm[b2 * M + y];
+                W[b * M + y] = v;
+            }
+            __syncthreads();
+            for (int t = tid; t < nu * nu; t += 256) {
+                const int x = t / nu, y = t % nu;
+                double v = 0.0;
+                for (int b = 0; b < NB2; ++b) v += Tm[b * M + x] * W[b * M + y];
+                ke[(size_t)(f * nu + x) * nat + g2 * nu + y] = v;
+            }
+        }
+}
+
 }  // namespace mgpu
 
 using namespace mgpu;
@@ -471,8 +684,10 @@ struct mgpu_sideset {
     mgpu_set* bulk = nullptr;
     int dim = 3, nf = 1, nb = 8, geom = MGPU_GEOM_TRI3;
     bool generic = false;      // side clusters of a generic-path set (k_side_gen)
+    bool ghost = false;        // double-sided ghost facets (k_ghost)
     SideView v{};
     SideGenView gv{};
+    GhostView hv{};
     int64_t n_kt = 0, n_rt = 0;
     const int64_t *kt_ptr = nullptr, *kt_dst = nullptr, *kt_src = nullptr;
     const int64_t *rt_ptr = nullptr, *rt_dst = nullptr, *rt_src = nullptr;
@@ -652,6 +867,126 @@ mgpu_status create_generic_side(mgpu_set* bulk, const mgpu_sideset_desc* d, int 
 }
 }  // namespace
 
+namespace {
+// Ghost facets of a generic-path bulk set whose descriptor coupled every facet's two elements: per
+// facet the union of the elements' adofs, the element-local -> union column maps, gamma = alpha h,
+// pool offsets and the gather lists from each K_e / R_e entry to its CSR value / row.
+mgpu_status create_ghost_side(mgpu_set* bulk, const mgpu_sideset_desc* d, mgpu_sideset** out) {
+    const SetHost& h = bulk->h;
+    const int dim = h.dim, nf = h.nf, nb = h.nb;
+    if (!((dim == 3 && nb == 8) || (dim == 2 && nb == 4)))
+        return set_error(MGPU_ERR_UNSUPPORTED, "ghost facets need HEX8 / QUAD4 IP cells");
+    if (!h.generic) return set_error(MGPU_ERR_STATE, "the bulk set does not couple the ghost facets' elements");
+    if (!(d->nitsche_alpha >= 0.0)) return set_error(MGPU_ERR_INVALID_ARG, "the ghost parameter alpha must be >= 0");
+    const int32_t nfc = d->n_facets;
+    std::vector<double> box((size_t)h.n_el * 2 * dim);
+    if (h.n_el > 0)
+        HIP_TRY(hipMemcpy(box.data(), bulk->g.ip_box, box.size() * sizeof(double), hipMemcpyDeviceToHost));
+    auto S = std::make_unique<mgpu_sideset>();
+    S->bulk = bulk;
+    S->dim = dim;
+    S->nf = nf;
+    S->nb = nb;
+    S->geom = d->side_geometry;
+    S->ghost = true;
+    GhostView& v = S->hv;
+    v.n_facets = nfc;
+    v.N = h.n_adofs_type;
+    v.nm = h.nm;
+    const int32_t N = h.n_adofs_type;
+    std::vector<int32_t> axis((size_t)nfc), map;
+    std::vector<double> side((size_t)nfc), gamma((size_t)nfc);
+    std::vector<int64_t> aptr(1, 0), mptr(1, 0), koff(1, 0), roff(1, 0);
+    std::vector<int32_t> adof, uni;
+    std::vector<std::pair<int64_t, int64_t>> kp, rp;
+    for (int32_t k = 0; k < nfc; ++k) {
+        const int32_t eL = d->facet_elem[k], eF = d->facet_follower[k];
+        if (eF < 0 || eF >= h.n_el || eF == eL) return set_error(MGPU_ERR_INVALID_ARG, "bad ghost follower element");
+        const double* n = d->facet_normal + (size_t)k * dim;
+        int ax = -1;
+        for (int q = 0; q < dim; ++q)
+            if (std::fabs(n[q]) == 1.0) ax = q;
+        for (int q = 0; q < dim && ax >= 0; ++q)
+            if (q != ax && n[q] != 0.0) ax = -1;
+        if (ax < 0) return set_error(MGPU_ERR_INVALID_ARG, "a ghost facet normal must be a lattice axis");
+        const double* bl = box.data() + (size_t)eL * 2 * dim;
+        const double* bf = box.data() + (size_t)eF * 2 * dim;
+        for (int q = 0; q < dim; ++q) {
+            const double shift = q == ax ? n[ax] * bl[dim + q] : 0.0;
+            if (bf[dim + q] != bl[dim + q] || std::fabs(bf[q] - (bl[q] + shift)) > 1e-12 * (1.0 + std::fabs(bl[q])))
+                return set_error(MGPU_ERR_INVALID_ARG, "ghost facet elements are not equal neighbouring boxes");
+        }
+        axis[(size_t)k] = ax;
+        side[(size_t)k] = n[ax];
+        double A = 1.0;
+        for (int q = 0; q < dim; ++q)
+            if (q != ax) A *= bl[dim + q];
+        gamma[(size_t)k] = d->nitsche_alpha * (dim == 3 ? std::sqrt(4.0 * A / 3.141592653589793) : A);
+        // union of the two elements' type-local adofs (the type-0 rows of their topologies)
+        const int32_t* tl = h.topo.data() + h.topo_ptr[(size_t)eL];
+        const int32_t* tf = h.topo.data() + h.topo_ptr[(size_t)eF];
+        const int nL = h.na_t[(size_t)eL], nF = h.na_t[(size_t)eF];
+        uni.clear();
+        std::set_union(tl, tl + nL, tf, tf + nF, std::back_inserter(uni));
+        const int nu = (int)uni.size();
+        if (nu > kSideMaxNat) return set_error(MGPU_ERR_UNSUPPORTED, "ghost facet with more than 64 adofs per dof type");
+        for (int x = 0; x < nL; ++x) map.push_back((int32_t)(std::lower_bound(uni.begin(), uni.end(), tl[x]) - uni.begin()));
+        for (int x = 0; x < nF; ++x) map.push_back((int32_t)(std::lower_bound(uni.begin(), uni.end(), tf[x]) - uni.begin()));
+        mptr.push_back((int64_t)map.size());
+        const int64_t nat = (int64_t)nf * nu;
+        for (int64_t r = 0; r < nat; ++r) {
+            const int32_t row = uni[(size_t)(r % nu)] + (int32_t)(r / nu) * N;
+            adof.push_back(row);
+            rp.push_back({row, roff.back() + r});
+            const int32_t* cb = h.col_idx.data() + h.row_ptr[(size_t)row];
+            const int32_t* ce = h.col_idx.data() + h.row_ptr[(size_t)row + 1];
+            for (int64_t cc = 0; cc < nat; ++cc) {
+                const int32_t col = uni[(size_t)(cc % nu)] + (int32_t)(cc / nu) * N;
+                const int32_t* it = std::lower_bound(cb, ce, col);
+                if (it == ce || *it != col) return set_error(MGPU_ERR_STATE, "ghost entry outside the bulk set's pattern");
+                kp.push_back({(int64_t)(it - h.col_idx.data()), koff.back() + r * nat + cc});
+            }
+        }
+        aptr.push_back((int64_t)adof.size());
+        koff.push_back(koff.back() + nat * nat);
+        roff.push_back(roff.back() + nat);
+    }
+    std::vector<int64_t> kptr, kdst, ksrc, rptr, rdst, rsrc;
+    build_gather(kp, kptr, kdst, ksrc);
+    build_gather(rp, rptr, rdst, rsrc);
+    S->n_kt = (int64_t)kdst.size();
+    S->n_rt = (int64_t)rdst.size();
+    mgpu_status rc;
+    if ((rc = side_upload(S.get(), d->facet_elem, (size_t)nfc, &v.leader)) != MGPU_OK) return rc;
+    if ((rc = side_upload(S.get(), d->facet_follower, (size_t)nfc, &v.follower)) != MGPU_OK) return rc;
+    if ((rc = side_upload(S.get(), axis.data(), axis.size(), &v.axis)) != MGPU_OK) return rc;
+    if ((rc = side_upload(S.get(), side.data(), side.size(), &v.side)) != MGPU_OK) return rc;
+    if ((rc = side_upload(S.get(), gamma.data(), gamma.size(), &v.gamma)) != MGPU_OK) return rc;
+    if ((rc = side_upload(S.get(), aptr.data(), aptr.size(), &v.cl_aptr)) != MGPU_OK) return rc;
+    if ((rc = side_upload(S.get(), adof.data(), adof.size(), &v.cl_adof)) != MGPU_OK) return rc;
+    if ((rc = side_upload(S.get(), mptr.data(), mptr.size(), &v.map_ptr)) != MGPU_OK) return rc;
+    if ((rc = side_upload(S.get(), map.data(), map.size(), &v.map)) != MGPU_OK) return rc;
+    if ((rc = side_upload(S.get(), koff.data(), koff.size(), &v.ke_off)) != MGPU_OK) return rc;
+    if ((rc = side_upload(S.get(), roff.data(), roff.size(), &v.re_off)) != MGPU_OK) return rc;
+    if ((rc = side_upload(S.get(), kptr.data(), kptr.size(), &S->kt_ptr)) != MGPU_OK) return rc;
+    if ((rc = side_upload(S.get(), kdst.data(), kdst.size(), &S->kt_dst)) != MGPU_OK) return rc;
+    if ((rc = side_upload(S.get(), ksrc.data(), ksrc.size(), &S->kt_src)) != MGPU_OK) return rc;
+    if ((rc = side_upload(S.get(), rptr.data(), rptr.size(), &S->rt_ptr)) != MGPU_OK) return rc;
+    if ((rc = side_upload(S.get(), rdst.data(), rdst.size(), &S->rt_dst)) != MGPU_OK) return rc;
+    if ((rc = side_upload(S.get(), rsrc.data(), rsrc.size(), &S->rt_src)) != MGPU_OK) return rc;
+    void* p = nullptr;
+    HIP_TRY(hipMalloc(&p, std::max<int64_t>(koff.back(), 1) * sizeof(double)));
+    S->allocs.push_back(p);
+    v.ke = static_cast<double*>(p);
+    HIP_TRY(hipMalloc(&p, std::max<int64_t>(roff.back(), 1) * sizeof(double)));
+    S->allocs.push_back(p);
+    v.re = static_cast<double*>(p);
+    S->v.n_facets = nfc;
+    *out = S.release();
+    return MGPU_OK;
+}
+}  // namespace
+
 extern "C" mgpu_status mgpu_sideset_create(mgpu_set* bulk, const mgpu_sideset_desc* d, mgpu_sideset** out) {
     MGPU_GUARD_BEGIN
     if (!bulk || !d || !out) return set_error(MGPU_ERR_INVALID_ARG, "null argument");
@@ -669,6 +1004,8 @@ extern "C" mgpu_status mgpu_sideset_create(mgpu_set* bulk, const mgpu_sideset_de
         case MGPU_IWG_STRUC_LINEAR_DIRICHLET_SYMMETRIC_NITSCHE: kind = diff ? -1 : 0; beta = -1.0; break;
         case MGPU_IWG_STRUC_LINEAR_DIRICHLET_UNSYMMETRIC_NITSCHE: kind = diff ? -1 : 0; beta = 1.0; break;
         case MGPU_IWG_STRUC_LINEAR_NEUMANN: kind = diff ? -1 : 1; break;
+        case MGPU_IWG_SPATIALDIFF_GHOST: kind = diff ? 2 : -1; break;
+        case MGPU_IWG_STRUC_LINEAR_GHOST: kind = diff ? -1 : 2; break;
         default: return set_error(MGPU_ERR_UNSUPPORTED, "unknown side IWG");
     }
     if (kind < 0) return set_error(MGPU_ERR_INVALID_ARG, "side IWG does not match the bulk set's field");
@@ -682,6 +1019,12 @@ extern "C" mgpu_status mgpu_sideset_create(mgpu_set* bulk, const mgpu_sideset_de
     const int32_t nfc = d->n_facets;
     for (int32_t k = 0; k < nfc; ++k)
         if (d->facet_elem[k] < 0 || d->facet_elem[k] >= h.n_el) return set_error(MGPU_ERR_INVALID_ARG, "facet element out of range");
+    if (kind == 2) {
+        if (d->side_geometry != (dim == 3 ? MGPU_GEOM_QUAD4 : MGPU_GEOM_LINE2))
+            return set_error(MGPU_ERR_INVALID_ARG, "ghost facets are whole cell faces: QUAD4 in 3D, LINE2 in 2D");
+        if (nfc > 0 && !d->facet_follower) return set_error(MGPU_ERR_INVALID_ARG, "ghost facets need facet_follower");
+        return create_ghost_side(bulk, d, out);
+    }
     if (h.generic) return create_generic_side(bulk, d, kind, beta, nvs, out);
     auto S = std::make_unique<mgpu_sideset>();
     S->bulk = bulk;
@@ -846,15 +1189,21 @@ extern "C" mgpu_status mgpu_sideset_assemble(mgpu_sideset* S, const double* u, d
     if (S->v.n_facets == 0) return MGPU_OK;
     if (!u || !vals || !rhs) return set_error(MGPU_ERR_INVALID_ARG, "null buffer");
     hipStream_t st = S->bulk->stream;
-    if (S->generic) {
+    if (S->ghost) {
+        const dim3 g((unsigned)S->hv.n_facets), b(256);
+        if (S->dim == 3 && S->nf == 1) hipLaunchKernelGGL((k_ghost<3, 1>), g, b, 0, st, S->hv, S->bulk->g, u);
+        else if (S->dim == 3) hipLaunchKernelGGL((k_ghost<3, 3>), g, b, 0, st, S->hv, S->bulk->g, u);
+        else if (S->nf == 1) hipLaunchKernelGGL((k_ghost<2, 1>), g, b, 0, st, S->hv, S->bulk->g, u);
+        else hipLaunchKernelGGL((k_ghost<2, 2>), g, b, 0, st, S->hv, S->bulk->g, u);
+    } else if (S->generic) {
         launch_side_gen(S, u, st);
     } else if (S->dim == 3 && S->nf == 1) launch_side<3, 1>(S, u, st);
     else if (S->dim == 3) launch_side<3, 3>(S, u, st);
     else if (S->nf == 1) launch_side<2, 1>(S, u, st);
     else launch_side<2, 2>(S, u, st);
     HIP_TRY(hipGetLastError());
-    const double* kpool = S->generic ? S->gv.ke : S->v.ke;
-    const double* rpool = S->generic ? S->gv.re : S->v.re;
+    const double* kpool = S->ghost ? S->hv.ke : (S->generic ? S->gv.ke : S->v.ke);
+    const double* rpool = S->ghost ? S->hv.re : (S->generic ? S->gv.re : S->v.re);
     if (S->n_kt > 0)
         hipLaunchKernelGGL(k_side_gather, dim3((unsigned)((S->n_kt + 255) / 256)), dim3(256), 0, st, S->n_kt, S->kt_ptr,
                            S->kt_dst, S->kt_src, kpool, vals);

@@ -48,6 +48,8 @@ struct mgpu_problem {
     std::vector<int32_t> cell_elem;      // first element of a level-0 cell, -1 if none
     std::vector<int32_t> cell_nelem;     // elements of a level-0 cell (subphases when enriched)
     std::vector<uint8_t> elem_corner;    // enrichment: strictly-material corners per element
+    std::vector<int32_t> ghost;          // ghost facets: (leader, follower, axis) triples
+    std::vector<int32_t> couplings;      // spec.ghost: (leader, follower) pairs for the descriptor
 
     std::vector<int32_t> elem_cell;
     std::vector<int8_t> elem_level;      // 0, or 1 for a THB level-1 cell
@@ -224,6 +226,44 @@ int process_cells(const mgpu_problem& p, int64_t c0, int64_t c1, std::vector<Cel
         out.push_back(std::move(co));
     }
     return MGPU_OK;
+}
+
+// Ghost facets of a level-0 problem (XTK's ghost stabilisation facets between a cut cell's material
+// and its neighbours): faces between two material elements, at least one of them cut, whose
+// shared face holds a lattice node strictly in the material (so that the material of both sides
+// meets there); leader the lower cell, elements ascending, then axes ascending.
+void ghost_facets(const mgpu_problem& p, std::vector<int32_t>& out) {
+    out.clear();
+    const int dim = p.dim;
+    const int phase = p.spec.material_phase;
+    const size_t n_el = p.elem_cell.size();
+    for (size_t e = 0; e < n_el; ++e) {
+        const int64_t c = p.elem_cell[e];
+        const int ijk[3] = {(int)(c % p.n[0]), (int)((c / p.n[0]) % p.n[1]),
+                            dim == 3 ? (int)(c / ((int64_t)p.n[0] * p.n[1])) : 0};
+        const bool cut_e = p.ig_ptr[e + 1] > p.ig_ptr[e];
+        for (int d = 0; d < dim; ++d) {
+            if (ijk[d] + 1 >= p.n[d]) continue;
+            const int64_t cn = c + (d == 0 ? 1 : d == 1 ? p.n[0] : (int64_t)p.n[0] * p.n[1]);
+            const int32_t en = p.cell_elem[(size_t)cn];
+            if (en < 0) continue;
+            const bool cut_n = p.ig_ptr[(size_t)en + 1] > p.ig_ptr[(size_t)en];
+            if (!cut_e && !cut_n) continue;
+            bool material = false;
+            const int nv = dim == 3 ? 8 : 4;
+            for (int v = 0; v < nv && !material; ++v) {
+                const int* off = dim == 3 ? kHex8Off[v] : kQuad4Off[v];
+                if (off[d] != 1) continue;
+                const int64_t nid = dim == 3 ? node_index(p, ijk[0] + off[0], ijk[1] + off[1], ijk[2] + off[2])
+                                             : node_index(p, ijk[0] + off[0], ijk[1] + off[1], 0);
+                material = ls_region(p.node_phi[(size_t)nid]) == phase;
+            }
+            if (!material) continue;
+            out.push_back((int32_t)e);
+            out.push_back(en);
+            out.push_back(d);
+        }
+    }
 }
 
 int n_threads() {
@@ -802,6 +842,15 @@ extern "C" mgpu_status mgpu_problem_build(const mgpu_problem_spec* spec, mgpu_pr
         p->t_ptr[n_rows] = (int32_t)pos;
     }
 
+    if (spec->ghost) {
+        if (thb || p->nb_ip == 27 || enrich)
+            return mgpu::set_error(MGPU_ERR_UNSUPPORTED, "ghost facets are implemented for HEX8 / QUAD4 level-0 sets without enrichment");
+        ghost_facets(*p, p->ghost);
+        for (size_t k = 0; k < p->ghost.size(); k += 3) {
+            p->couplings.push_back(p->ghost[k]);
+            p->couplings.push_back(p->ghost[k + 1]);
+        }
+    }
     *out = p.release();
     return MGPU_OK;
     MGPU_GUARD_END
@@ -833,6 +882,8 @@ extern "C" mgpu_status mgpu_problem_desc(const mgpu_problem* p, mgpu_set_desc* d
     if (!p->refined.empty()) d->lattice.n[0] = 0;
     d->elem_cell = p->elem_cell.data();
     d->vertex_phi = p->vertex_phi.data();
+    d->n_couplings = (int32_t)(p->couplings.size() / 2);
+    d->couplings = p->couplings.empty() ? nullptr : p->couplings.data();
     return MGPU_OK;
 }
 
@@ -869,6 +920,7 @@ extern "C" mgpu_status mgpu_problem_destroy(mgpu_problem* p) {
 struct mgpu_facets {
     int32_t dim = 3, nv = 3;
     std::vector<int32_t> elem;
+    std::vector<int32_t> follower;   // ghost facets only
     std::vector<double> xi;      // [n][nv][dim]
     std::vector<double> normal;  // [n][dim]
 };
@@ -889,6 +941,41 @@ extern "C" mgpu_status mgpu_problem_facets(const mgpu_problem* p, int32_t where,
     if (!p || !out) return mgpu::set_error(MGPU_ERR_INVALID_ARG, "null argument");
     *out = nullptr;
     const int dim = p->dim;
+    if (where == MGPU_SIDE_GHOST) {
+        if (!(dim == 3 ? geom == MGPU_GEOM_QUAD4 : geom == MGPU_GEOM_LINE2))
+            return mgpu::set_error(MGPU_ERR_INVALID_ARG, "ghost facets are whole cell faces: QUAD4 in 3D, LINE2 in 2D");
+        if (!p->refined.empty() || p->nb_ip == 27 || p->spec.enrich)
+            return mgpu::set_error(MGPU_ERR_UNSUPPORTED, "ghost facets are implemented for HEX8 / QUAD4 level-0 sets without enrichment");
+        std::vector<int32_t> g;
+        if (p->spec.ghost) g = p->ghost;
+        else ghost_facets(*p, g);
+        auto F = std::make_unique<mgpu_facets>();
+        F->dim = dim;
+        F->nv = dim == 3 ? 4 : 2;
+        const double cyc[4][2] = {{-1, -1}, {1, -1}, {1, 1}, {-1, 1}};
+        for (size_t k = 0; k < g.size(); k += 3) {
+            const int d = g[k + 2];
+            F->elem.push_back(g[k]);
+            F->follower.push_back(g[k + 1]);
+            int ax[2] = {0, 0}, na = 0;
+            for (int q = 0; q < dim; ++q)
+                if (q != d) ax[na++] = q;
+            for (int v = 0; v < F->nv; ++v) {
+                double x[3] = {0, 0, 0};
+                x[d] = 1.0;
+                if (dim == 3) {
+                    x[ax[0]] = cyc[v][0];
+                    x[ax[1]] = cyc[v][1];
+                } else {
+                    x[ax[0]] = v == 0 ? -1.0 : 1.0;
+                }
+                for (int q = 0; q < dim; ++q) F->xi.push_back(x[q]);
+            }
+            for (int q = 0; q < dim; ++q) F->normal.push_back(q == d ? 1.0 : 0.0);
+        }
+        *out = F.release();
+        return MGPU_OK;
+    }
     if (where < MGPU_SIDE_INTERFACE || where > 2 * dim) return mgpu::set_error(MGPU_ERR_INVALID_ARG, "bad facet location");
     const bool ok_geom = dim == 3 ? (geom == MGPU_GEOM_TRI3 || geom == MGPU_GEOM_QUAD4) : geom == MGPU_GEOM_LINE2;
     if (!ok_geom) return mgpu::set_error(MGPU_ERR_INVALID_ARG, "side geometry must be TRI3 / QUAD4 in 3D, LINE2 in 2D");
@@ -1016,6 +1103,12 @@ extern "C" mgpu_status mgpu_facets_get(const mgpu_facets* f, int32_t* n, const i
     if (elem) *elem = f->elem.data();
     if (xi) *xi = f->xi.data();
     if (normal) *normal = f->normal.data();
+    return MGPU_OK;
+}
+
+extern "C" mgpu_status mgpu_facets_followers(const mgpu_facets* f, const int32_t** follower) {
+    if (!f) return mgpu::set_error(MGPU_ERR_INVALID_ARG, "null facets");
+    if (follower) *follower = f->follower.empty() ? nullptr : f->follower.data();
     return MGPU_OK;
 }
 

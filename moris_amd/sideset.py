@@ -1,5 +1,6 @@
-"""Python view of include/moris_gpu_sideset.h: side-set IWGs (Nitsche Dirichlet, Neumann) on the
-facets of a bulk set, assembled on the GPU into the bulk set's CSR values and residual
+"""Python view of include/moris_gpu_sideset.h: side-set IWGs (Nitsche Dirichlet, Neumann, and the
+double-sided ghost stabilisation) on the facets of a bulk set, assembled on the GPU into the bulk
+set's CSR values and residual
 (Element_Sideset::compute_jacobian_and_residual, projects/FEM/INT/src/ELEM/cl_FEM_Element_Sideset.cpp
 :245-330).  No CPU fallback: every call goes through libmoris_gpu.so.
 """
@@ -19,6 +20,9 @@ IWG_SPATIALDIFF_NEUMANN = 4
 IWG_STRUC_LINEAR_DIRICHLET_SYMMETRIC_NITSCHE = 5
 IWG_STRUC_LINEAR_DIRICHLET_UNSYMMETRIC_NITSCHE = 6
 IWG_STRUC_LINEAR_NEUMANN = 7
+IWG_SPATIALDIFF_GHOST = 8
+IWG_STRUC_LINEAR_GHOST = 9
+SIDE_GHOST = -1
 GEOM_TRI3, GEOM_QUAD4, GEOM_LINE2 = 2, 0, 5
 SIDE_INTERFACE, SIDE_XMIN, SIDE_XMAX, SIDE_YMIN, SIDE_YMAX, SIDE_ZMIN, SIDE_ZMAX = range(7)
 N_VERTS = {GEOM_TRI3: 3, GEOM_QUAD4: 4, GEOM_LINE2: 2}
@@ -27,7 +31,7 @@ N_VERTS = {GEOM_TRI3: 3, GEOM_QUAD4: 4, GEOM_LINE2: 2}
 class SideSetDesc(ctypes.Structure):
     _fields_ = [("iwg", ctypes.c_int32), ("side_geometry", ctypes.c_int32), ("n_facets", ctypes.c_int32),
                 ("facet_elem", _i32p), ("facet_xi", _f64p), ("facet_normal", _f64p),
-                ("nitsche_alpha", ctypes.c_double), ("value", ctypes.c_double * 3)]
+                ("nitsche_alpha", ctypes.c_double), ("value", ctypes.c_double * 3), ("facet_follower", _i32p)]
 
 
 def _lib():
@@ -39,8 +43,9 @@ def _lib():
         lib.mgpu_problem_facets.argtypes = [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(_vp)]
         lib.mgpu_facets_get.argtypes = [_vp, _i32p, ctypes.POINTER(_i32p), ctypes.POINTER(_f64p), ctypes.POINTER(_f64p)]
         lib.mgpu_facets_destroy.argtypes = [_vp]
+        lib.mgpu_facets_followers.argtypes = [_vp, ctypes.POINTER(_i32p)]
         for n in ("mgpu_sideset_create", "mgpu_sideset_destroy", "mgpu_sideset_assemble", "mgpu_problem_facets",
-                  "mgpu_facets_get", "mgpu_facets_destroy"):
+                  "mgpu_facets_get", "mgpu_facets_destroy", "mgpu_facets_followers"):
             getattr(lib, n).restype = ctypes.c_int32
         lib._side_bound = True
     return lib
@@ -68,11 +73,40 @@ def facets(problem, where, geometry):
         lib.mgpu_facets_destroy(h)
 
 
-def make_desc(iwg, geometry, elem, xi, normal, alpha=0.0, value=(0.0, 0.0, 0.0)):
-    """(SideSetDesc, keepalive) from facet arrays."""
+def ghost_facets(problem):
+    """Ghost facets of a Problem (MGPU_SIDE_GHOST): (leader [n], follower [n], xi [n][nv][dim] on the
+    leader, normal [n][dim]) as numpy copies."""
+    lib = _lib()
+    geometry = GEOM_QUAD4 if problem.dim == 3 else GEOM_LINE2
+    h = _vp()
+    _check(lib.mgpu_problem_facets(problem._h, SIDE_GHOST, geometry, ctypes.byref(h)), "mgpu_problem_facets")
+    try:
+        n = ctypes.c_int32()
+        e, x, nr, fo = _i32p(), _f64p(), _f64p(), _i32p()
+        _check(lib.mgpu_facets_get(h, ctypes.byref(n), ctypes.byref(e), ctypes.byref(x), ctypes.byref(nr)),
+               "mgpu_facets_get")
+        _check(lib.mgpu_facets_followers(h, ctypes.byref(fo)), "mgpu_facets_followers")
+        dim = problem.dim
+        nv = N_VERTS[geometry]
+        if n.value == 0:
+            return np.zeros(0, np.int32), np.zeros(0, np.int32), np.zeros((0, nv, dim)), np.zeros((0, dim))
+        leader = np.ctypeslib.as_array(e, shape=(n.value,)).copy()
+        follower = np.ctypeslib.as_array(fo, shape=(n.value,)).copy()
+        xi = np.ctypeslib.as_array(x, shape=(n.value * nv * dim,)).reshape(n.value, nv, dim).copy()
+        normal = np.ctypeslib.as_array(nr, shape=(n.value * dim,)).reshape(n.value, dim).copy()
+        return leader, follower, xi, normal
+    finally:
+        lib.mgpu_facets_destroy(h)
+
+
+def make_desc(iwg, geometry, elem, xi, normal, alpha=0.0, value=(0.0, 0.0, 0.0), follower=None):
+    """(SideSetDesc, keepalive) from facet arrays (follower: the ghost IWGs' follower elements)."""
     d = SideSetDesc()
     keep = [np.ascontiguousarray(elem, np.int32), np.ascontiguousarray(xi, np.float64),
             np.ascontiguousarray(normal, np.float64)]
+    if follower is not None:
+        keep.append(np.ascontiguousarray(follower, np.int32))
+        d.facet_follower = keep[3].ctypes.data_as(_i32p) if len(keep[3]) else None
     d.iwg, d.side_geometry, d.n_facets = iwg, geometry, len(keep[0])
     d.facet_elem = keep[0].ctypes.data_as(_i32p) if len(keep[0]) else None
     d.facet_xi = keep[1].ctypes.data_as(_f64p) if len(keep[0]) else None
@@ -86,11 +120,11 @@ def make_desc(iwg, geometry, elem, xi, normal, alpha=0.0, value=(0.0, 0.0, 0.0))
 class SideSet:
     """A side set on a bulk Set: its K and R are added into the bulk set's CSR values and R."""
 
-    def __init__(self, bulk, iwg, geometry, elem, xi, normal, alpha=0.0, value=(0.0, 0.0, 0.0)):
+    def __init__(self, bulk, iwg, geometry, elem, xi, normal, alpha=0.0, value=(0.0, 0.0, 0.0), follower=None):
         if not isinstance(bulk, Set):
             raise TypeError("bulk must be a moris_amd.Set")
         self.bulk = bulk
-        self.desc, self._keep = make_desc(iwg, geometry, elem, xi, normal, alpha, value)
+        self.desc, self._keep = make_desc(iwg, geometry, elem, xi, normal, alpha, value, follower)
         self.n_facets = self.desc.n_facets
         self._h = _vp()
         _check(_lib().mgpu_sideset_create(bulk._h, ctypes.byref(self.desc), ctypes.byref(self._h)),

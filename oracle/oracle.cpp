@@ -38,6 +38,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <iterator>
 #include <map>
 #include <memory>
 #include <stdexcept>
@@ -383,6 +384,8 @@ struct OracleProblem {
     std::vector<double> t_weight;
     int32_t N = 0;
     int64_t n_cut = 0;
+    // ghost stabilisation (spec.ghost): (leader, follower) element pairs coupled in the graph
+    std::vector<int32_t> couplings;
     // graph
     std::vector<int32_t> row_ptr, col_idx;
 };
@@ -976,6 +979,8 @@ void enrich_rows(OracleProblem& p) {
     p.N = next;
 }
 
+void ghost_facets(const OracleProblem& p, std::vector<int32_t>& out);
+
 void build(OracleProblem& p, const mgpu_problem_spec& s) {
     p.spec = s;
     p.dim = s.dim;
@@ -1149,6 +1154,15 @@ void build(OracleProblem& p, const mgpu_problem_spec& s) {
         if (thb) throw std::runtime_error("enrichment is implemented for level-0 lattices");
         enrich_rows(p);
     }
+    if (s.ghost) {
+        if (thb || p.nb_ip == 27 || s.enrich) throw std::runtime_error("ghost facets need HEX8 / QUAD4 level-0 sets");
+        std::vector<int32_t> g;
+        ghost_facets(p, g);
+        for (size_t k = 0; k < g.size(); k += 3) {
+            p.couplings.push_back(g[k]);
+            p.couplings.push_back(g[k + 1]);
+        }
+    }
 }
 
 // element topology: per type sorted unique adof ids (global)
@@ -1169,6 +1183,14 @@ void build_graph(const mgpu_set_desc& d, int nf, std::vector<int32_t>& row_ptr, 
     std::vector<int32_t> ids;
     for (int64_t e = 0; e < d.n_elements; ++e) {
         element_topology(d, nf, e, ids);
+        for (int32_t r : ids) rows[r].insert(rows[r].end(), ids.begin(), ids.end());
+    }
+    // double-sided clusters (ghost facets) couple their two elements
+    std::vector<int32_t> other;
+    for (int32_t k = 0; d.couplings && k < d.n_couplings; ++k) {
+        element_topology(d, nf, d.couplings[2 * k], ids);
+        element_topology(d, nf, d.couplings[2 * k + 1], other);
+        ids.insert(ids.end(), other.begin(), other.end());
         for (int32_t r : ids) rows[r].insert(rows[r].end(), ids.begin(), ids.end());
     }
     row_ptr.assign(nrows + 1, 0);
@@ -1779,6 +1801,204 @@ void problem_facets(const OracleProblem& p, int where, int nvs, std::vector<int3
 
 thread_local std::string g_err;
 
+// Ghost facets (XTK ghost stabilisation): every pair of face-neighbouring material elements of the
+// level-0 lattice, at least one cut, whose shared face has a corner node strictly in the material;
+// leader the element of the lower cell; (leader, follower, axis) triples in leader order.
+void ghost_facets(const OracleProblem& p, std::vector<int32_t>& out) {
+    out.clear();
+    std::map<int64_t, size_t> elem_of;
+    for (size_t e = 0; e < p.elem_cell.size(); ++e) elem_of[p.elem_cell[e]] = e;
+    for (size_t e = 0; e < p.elem_cell.size(); ++e) {
+        const int64_t c = p.elem_cell[e];
+        int64_t nodes[8];
+        cell_nodes(p, c, nodes);
+        const int64_t ijk[3] = {c % p.n[0], (c / p.n[0]) % p.n[1], c / ((int64_t)p.n[0] * p.n[1])};
+        for (int d = 0; d < p.dim; ++d) {
+            if (ijk[d] + 1 >= p.n[d]) continue;
+            const int64_t cn = c + (d == 0 ? 1 : d == 1 ? p.n[0] : (int64_t)p.n[0] * p.n[1]);
+            auto it = elem_of.find(cn);
+            if (it == elem_of.end()) continue;
+            const size_t en = it->second;
+            const bool cut = p.ig_ptr[e + 1] > p.ig_ptr[e] || p.ig_ptr[en + 1] > p.ig_ptr[en];
+            if (!cut) continue;
+            // the face x_d = upper of cell c: its corner nodes are those of the neighbour's lower face
+            int64_t nn[8];
+            cell_nodes(p, cn, nn);
+            bool material = false;
+            for (int v = 0; v < p.nb && !material; ++v)
+                for (int w = 0; w < p.nb && !material; ++w)
+                    if (nodes[v] == nn[w]) material = region(p.node_phi[nodes[v]]) == p.spec.material_phase;
+            if (!material) continue;
+            out.push_back((int32_t)e);
+            out.push_back((int32_t)en);
+            out.push_back(d);
+        }
+    }
+}
+
+// Ghost facets on descriptor d (IWG_Diffusion_Virtual_Work_Ghost / IWG_Isotropic_Struc_Linear_
+// Virtual_Work_Ghost, order 1, SP_Ghost_Virtual_Work gamma = alpha h): per facet the leader's and
+// the follower's B at each face point (the follower's parametric point from the physical one),
+//   K_XY += w* gamma s_X s_Y (n_flat B_X)^T (n_flat D_Y B_Y),  s_L = 1, s_F = -1
+// (diffusion: (n^T B_X)^T k_Y n^T B_Y), then T^T K T over the union of the two topologies, R = K u.
+void ghost_assemble(const mgpu_set_desc& d, int64_t nfc, const int32_t* fl, const int32_t* ff, const double* fx,
+                    const double* fnrm, double alpha, const double* u, const int32_t* row_ptr,
+                    const int32_t* col_idx, double* vals, double* rhs) {
+    const int D = d.spatial_dim, nb = d.n_bases;
+    const int nvs = D == 3 ? 4 : 2;
+    const bool elast = d.iwg == MGPU_IWG_STRUC_LINEAR_BULK;
+    const int nf = elast ? D : 1, np = nf * nb, ns = elast ? (D == 3 ? 6 : 3) : D;
+    for (int64_t k = 0; k < nfc; ++k) {
+        const int32_t eX[2] = {fl[k], ff[k]};
+        std::vector<int32_t> idL, idF, U;
+        element_topology(d, nf, eX[0], idL);
+        element_topology(d, nf, eX[1], idF);
+        std::set_union(idL.begin(), idL.end(), idF.begin(), idF.end(), std::back_inserter(U));
+        const int nu = (int)U.size();
+        // T of both sides against U: rows (X, f, b)
+        std::vector<double> T((size_t)2 * np * nu, 0.0);
+        for (int X = 0; X < 2; ++X)
+            for (int b = 0; b < nb; ++b)
+                for (int q = d.t_ptr[eX[X] * nb + b]; q < d.t_ptr[eX[X] * nb + b + 1]; ++q)
+                    for (int f = 0; f < nf; ++f) {
+                        const int32_t id = d.t_adof[q] + f * d.n_adofs_per_type;
+                        const int col = (int)(std::lower_bound(U.begin(), U.end(), id) - U.begin());
+                        T[(size_t)(X * np + f * nb + b) * nu + col] += d.t_weight[q];
+                    }
+        // materials
+        double Dm[2][6][6] = {}, kc[2] = {0.0, 0.0};
+        for (int X = 0; X < 2; ++X) {
+            const int32_t e = eX[X];
+            if (elast) {
+                const double E = d.material[2 * e], nu_ = d.material[2 * e + 1];
+                if (D == 3) {
+                    const double pre = E / (1.0 + nu_) / (1.0 - 2.0 * nu_);
+                    for (int i = 0; i < 3; ++i)
+                        for (int j = 0; j < 3; ++j) Dm[X][i][j] = i == j ? pre * (1.0 - nu_) : pre * nu_;
+                    for (int i = 3; i < 6; ++i) Dm[X][i][i] = pre * (1.0 - 2.0 * nu_) / 2.0;
+                } else {
+                    const double pre = E / (1.0 - nu_ * nu_);
+                    Dm[X][0][0] = pre; Dm[X][1][1] = pre; Dm[X][0][1] = pre * nu_; Dm[X][1][0] = pre * nu_;
+                    Dm[X][2][2] = pre * 0.5 * (1.0 - nu_);
+                }
+            } else {
+                kc[X] = d.material[e];
+            }
+        }
+        const double* nv = fnrm + (size_t)k * D;
+        double nfl[3][6] = {{0}};
+        if (elast && D == 3) {
+            nfl[0][0] = nv[0]; nfl[1][1] = nv[1]; nfl[2][2] = nv[2];
+            nfl[0][4] = nv[2]; nfl[0][5] = nv[1]; nfl[1][3] = nv[2];
+            nfl[1][5] = nv[0]; nfl[2][3] = nv[1]; nfl[2][4] = nv[0];
+        } else if (elast) {
+            nfl[0][0] = nv[0]; nfl[0][2] = nv[1]; nfl[1][1] = nv[1]; nfl[1][2] = nv[0];
+        } else {
+            for (int c = 0; c < D; ++c) nfl[0][c] = nv[c];
+        }
+        // face points and the cluster's length measure
+        const double* boxL = d.ip_box + (size_t)eX[0] * 2 * D;
+        double Xf[4][3], phi[7][4], w[7], dJ[7];
+        for (int v = 0; v < nvs; ++v)
+            for (int c = 0; c < 3; ++c)
+                Xf[v][c] = c < D ? boxL[c] + 0.5 * (fx[((size_t)k * nvs + v) * D + c] + 1.0) * boxL[D + c] : 0.0;
+        const int nq = side_points(D, nvs, Xf, phi, w, dJ);
+        double A = 0.0;
+        for (int q = 0; q < nq; ++q) A += w[q] * dJ[q];
+        const double gamma = alpha * (D == 3 ? std::pow(4.0 * A / 3.141592653589793, 0.5) : A);
+        std::vector<double> Kp((size_t)4 * np * np, 0.0);
+        std::vector<double> B[2], tq[2], Tq[2];
+        for (int q = 0; q < nq; ++q) {
+            const double ws = w[q] * dJ[q];
+            double x[3] = {0, 0, 0};
+            for (int c = 0; c < D; ++c)
+                for (int v = 0; v < nvs; ++v) x[c] += phi[q][v] * Xf[v][c];
+            for (int X = 0; X < 2; ++X) {
+                const double* box = d.ip_box + (size_t)eX[X] * 2 * D;
+                double xip[3] = {0, 0, 0};
+                for (int c = 0; c < D; ++c) xip[c] = 2.0 * (x[c] - box[c]) / box[D + c] - 1.0;
+                double dN[3][27];
+                basis_dN(D, nb, xip, dN);
+                double dNdx[3][27];
+                for (int r = 0; r < D; ++r)
+                    for (int a = 0; a < nb; ++a) dNdx[r][a] = dN[r][a] * 2.0 / box[D + r];
+                B[X].assign((size_t)ns * np, 0.0);
+                if (elast && D == 3) {
+                    for (int a = 0; a < nb; ++a) {
+                        B[X][0 * np + a] = dNdx[0][a]; B[X][4 * np + a] = dNdx[2][a]; B[X][5 * np + a] = dNdx[1][a];
+                        B[X][1 * np + nb + a] = dNdx[1][a]; B[X][3 * np + nb + a] = dNdx[2][a]; B[X][5 * np + nb + a] = dNdx[0][a];
+                        B[X][2 * np + 2 * nb + a] = dNdx[2][a]; B[X][3 * np + 2 * nb + a] = dNdx[1][a];
+                        B[X][4 * np + 2 * nb + a] = dNdx[0][a];
+                    }
+                } else if (elast) {
+                    for (int a = 0; a < nb; ++a) {
+                        B[X][0 * np + a] = dNdx[0][a]; B[X][2 * np + a] = dNdx[1][a];
+                        B[X][1 * np + nb + a] = dNdx[1][a]; B[X][2 * np + nb + a] = dNdx[0][a];
+                    }
+                } else {
+                    for (int i = 0; i < D; ++i)
+                        for (int a = 0; a < nb; ++a) B[X][(size_t)i * np + a] = dNdx[i][a];
+                }
+                // test traction n_flat B and traction n_flat D B (diffusion: n^T B, k n^T B)
+                tq[X].assign((size_t)nf * np, 0.0);
+                Tq[X].assign((size_t)nf * np, 0.0);
+                for (int f = 0; f < nf; ++f)
+                    for (int c = 0; c < np; ++c) {
+                        double t = 0.0, tt = 0.0;
+                        for (int i = 0; i < ns; ++i) {
+                            t += nfl[f][i] * B[X][(size_t)i * np + c];
+                            double db = 0.0;
+                            if (elast)
+                                for (int j = 0; j < ns; ++j) db += Dm[X][i][j] * B[X][(size_t)j * np + c];
+                            else
+                                db = kc[X] * B[X][(size_t)i * np + c];
+                            tt += nfl[f][i] * db;
+                        }
+                        tq[X][(size_t)f * np + c] = t;
+                        Tq[X][(size_t)f * np + c] = tt;
+                    }
+            }
+            for (int X = 0; X < 2; ++X)
+                for (int Y = 0; Y < 2; ++Y) {
+                    const double sg = (X == Y ? 1.0 : -1.0) * ws * gamma;
+                    for (int i = 0; i < np; ++i)
+                        for (int j = 0; j < np; ++j) {
+                            double v = 0.0;
+                            for (int f = 0; f < nf; ++f) v += tq[X][(size_t)f * np + i] * Tq[Y][(size_t)f * np + j];
+                            Kp[(size_t)(X * np + i) * 2 * np + Y * np + j] += sg * v;
+                        }
+                }
+        }
+        // K_e = T^T K_p T, R_e = K_e u_U, added into the CSR values / R
+        std::vector<double> KT((size_t)2 * np * nu, 0.0), Ke((size_t)nu * nu, 0.0);
+        for (int i = 0; i < 2 * np; ++i)
+            for (int b = 0; b < nu; ++b) {
+                double v = 0.0;
+                for (int j = 0; j < 2 * np; ++j) v += Kp[(size_t)i * 2 * np + j] * T[(size_t)j * nu + b];
+                KT[(size_t)i * nu + b] = v;
+            }
+        for (int a = 0; a < nu; ++a)
+            for (int b = 0; b < nu; ++b) {
+                double v = 0.0;
+                for (int i = 0; i < 2 * np; ++i) v += T[(size_t)i * nu + a] * KT[(size_t)i * nu + b];
+                Ke[(size_t)a * nu + b] = v;
+            }
+        for (int a = 0; a < nu; ++a) {
+            const int32_t r = U[a];
+            double rr = 0.0;
+            for (int b = 0; b < nu; ++b) {
+                rr += Ke[(size_t)a * nu + b] * u[U[b]];
+                const int32_t* cb = col_idx + row_ptr[r];
+                const int32_t* ce = col_idx + row_ptr[r + 1];
+                const int32_t* it = std::lower_bound(cb, ce, U[b]);
+                if (it == ce || *it != U[b]) throw std::runtime_error("ghost entry outside the graph");
+                vals[it - col_idx] += Ke[(size_t)a * nu + b];
+            }
+            rhs[r] += rr;
+        }
+    }
+}
+
 mgpu_set_desc desc_of(const OracleProblem& p) {
     mgpu_set_desc d;
     std::memset(&d, 0, sizeof(d));
@@ -1797,6 +2017,8 @@ mgpu_set_desc desc_of(const OracleProblem& p) {
     d.ig_ptr = p.ig_ptr.data();
     d.ig_xi = p.ig_xi.data();
     d.elem_cell = p.elem_cell.data();
+    d.n_couplings = (int32_t)(p.couplings.size() / 2);
+    d.couplings = p.couplings.empty() ? nullptr : p.couplings.data();
     return d;
 }
 
@@ -2016,6 +2238,30 @@ int oracle_side_assemble(const mgpu_set_desc* d, const int32_t* row_ptr, const i
         g_err = ex.what();
         return 1;
     }
+}
+
+int oracle_ghost_assemble(const mgpu_set_desc* d, const int32_t* row_ptr, const int32_t* col_idx, int64_t n_facets,
+                          const int32_t* leader, const int32_t* follower, const double* xi, const double* normal,
+                          double alpha, const double* u, double* vals, double* rhs) {
+    try {
+        ghost_assemble(*d, n_facets, leader, follower, xi, normal, alpha, u, row_ptr, col_idx, vals, rhs);
+        return 0;
+    } catch (const std::exception& ex) {
+        g_err = ex.what();
+        return 1;
+    }
+}
+
+// ghost facets of an oracle problem: (leader, follower, axis) triples; returns the count (at most
+// `max` written)
+int64_t oracle_ghost_facets(void* h, int32_t* trip, int64_t max) {
+    const OracleProblem& p = *static_cast<OracleProblem*>(h);
+    std::vector<int32_t> g;
+    ghost_facets(p, g);
+    const int64_t n = (int64_t)(g.size() / 3);
+    for (int64_t k = 0; k < std::min(n, max); ++k)
+        for (int c = 0; c < 3; ++c) trip[3 * k + c] = g[3 * k + c];
+    return n;
 }
 
 int oracle_rule(int rule, double* xi, double* w) {

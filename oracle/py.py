@@ -257,3 +257,36 @@ def side_assemble(desc, row_ptr, col_idx, iwg, nvs, elem, xi, normal, alpha, val
     if lib.oracle_side_assemble(ctypes.byref(desc), _p(row_ptr), _p(col_idx), iwg, nvs, len(elem), _p(elem), _p(xi),
                                 _p(normal), alpha, _p(value), _p(u), _p(vals), _p(rhs)) != 0:
         raise RuntimeError("oracle_side_assemble: " + lib.oracle_last_error().decode())
+
+
+def ghost_facets(problem):
+    """Ghost facets of an OracleProblem (its own extraction): (leader, follower, axis) arrays."""
+    lib = load()
+    lib.oracle_ghost_facets.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]
+    lib.oracle_ghost_facets.restype = ctypes.c_int64
+    n = lib.oracle_ghost_facets(problem._h, None, 0)
+    trip = np.zeros(max(n, 1) * 3, np.int32)
+    lib.oracle_ghost_facets(problem._h, _p(trip), n)
+    trip = trip[: 3 * n].reshape(n, 3)
+    return trip[:, 0].copy(), trip[:, 1].copy(), trip[:, 2].copy()
+
+
+def ghost_assemble(desc, row_ptr, col_idx, leader, follower, xi, normal, alpha, u, vals, rhs):
+    """Reference algorithm of ghost facets on descriptor `desc` (xi: the leader's face vertices,
+    normal: its outward normal), ADDED into vals / rhs (numpy, in place)."""
+    lib = load()
+    lib.oracle_ghost_assemble.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
+                                          ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                          ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    lib.oracle_ghost_assemble.restype = ctypes.c_int
+    row_ptr = np.ascontiguousarray(row_ptr, np.int32)
+    col_idx = np.ascontiguousarray(col_idx, np.int32)
+    leader = np.ascontiguousarray(leader, np.int32)
+    follower = np.ascontiguousarray(follower, np.int32)
+    xi = np.ascontiguousarray(xi, np.float64)
+    normal = np.ascontiguousarray(normal, np.float64)
+    u = np.ascontiguousarray(u, np.float64)
+    assert vals.dtype == np.float64 and rhs.dtype == np.float64 and vals.flags.c_contiguous and rhs.flags.c_contiguous
+    if lib.oracle_ghost_assemble(ctypes.byref(desc), _p(row_ptr), _p(col_idx), len(leader), _p(leader), _p(follower),
+                                 _p(xi), _p(normal), alpha, _p(u), _p(vals), _p(rhs)) != 0:
+        raise RuntimeError("oracle_ghost_assemble: " + lib.oracle_last_error().decode())
