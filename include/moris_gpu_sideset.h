@@ -38,7 +38,8 @@ extern "C" {
 #define MGPU_IWG_STRUC_LINEAR_DIRICHLET_UNSYMMETRIC_NITSCHE 6
 #define MGPU_IWG_STRUC_LINEAR_NEUMANN                       7
 /* Double-sided ghost stabilisation on the faces between a cut element and its neighbours
-   (IWG_Diffusion_Virtual_Work_Ghost, projects/FEM/INT/src/IWG/cl_FEM_IWG_Diffusion_Virtual_Work_
+   (SPATIALDIFF_VW_GHOST and STRUC_LINEAR_VW_GHOST, cl_FEM_IWG_Factory.cpp:183, 223;
+   IWG_Diffusion_Virtual_Work_Ghost, projects/FEM/INT/src/IWG/cl_FEM_IWG_Diffusion_Virtual_Work_
    Ghost.cpp; IWG_Isotropic_Struc_Linear_Virtual_Work_Ghost, cl_FEM_IWG_Isotropic_Struc_Linear_
    Virtual_Work_Ghost.cpp:50-255), order 1, with SP_Ghost_Virtual_Work gamma = alpha h
    (projects/FEM/INT/src/SP/cl_FEM_SP_Ghost_Virtual_Work.cpp:30-41; h the cell length measure of

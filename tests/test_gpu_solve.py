@@ -362,3 +362,75 @@ def test_full_size_displacement_error(case):
     assert out["mixed+learned"]["K"] <= 1e-6
     assert out["mixed+learned"]["L2"] <= 1e-5
     assert out["mixed+learned"]["euclid"] <= (5e-5 if case.startswith("C5") else 1e-5)
+
+
+def _embed(rp, ci, rpg, cig):
+    """Positions of the entries of pattern (rp, ci) inside the wider pattern (rpg, cig); both CSR
+    patterns have ascending columns per row."""
+    n = len(rp) - 1
+    key = np.repeat(np.arange(n, dtype=np.int64), np.diff(rp)) * n + ci
+    keyg = np.repeat(np.arange(n, dtype=np.int64), np.diff(rpg)) * n + cig
+    pos = np.searchsorted(keyg, key)
+    assert np.array_equal(keyg[pos], key)
+    return pos
+
+
+@pytest.mark.timeout(900)
+@pytest.mark.parametrize("case", ["C5 variant 0", "C4 random spheres"])
+def test_full_size_displacement_error_with_ghost(case):
+    """The same traction problems with the ghost stabilisation of the cut elements (virtual-work ghost,
+    gamma = 0.01 h) added to the bulk K on the oracle's side and on the GPU's: the oracle's
+    bulk + ghost K against the GPU's FP64 and mixed + learned bulk K, each plus the GPU's ghost K on
+    the pattern that couples every ghost facet's two elements."""
+    n, alpha = 64, 0.01
+    spheres = W.sphere_array(0) if case.startswith("C5") else W.random_spheres(64)
+    spec, keep = W.elasticity_spec(n, spheres)
+    P = M.Problem(spec, keep)
+    S = M.Set(P.desc, keepalive=P)
+    rp, ci = S.pattern()
+    spec.ghost = 1
+    Pg = M.Problem(spec, keep)
+    Sg = M.Set(Pg.desc, keepalive=Pg)              # the coupled pattern (generic path)
+    rpg, cig = Sg.pattern()
+    assert Sg.n_rows == S.n_rows
+    pos = _embed(rp, ci, rpg, cig)
+    Lf, Ff, xi, nrm = SS.ghost_facets(Pg)
+    G = SS.SideSet(Sg, SS.IWG_STRUC_LINEAR_GHOST, SS.GEOM_QUAD4, Lf, xi, nrm, alpha=alpha, follower=Ff)
+    u0 = np.zeros(S.n_rows)
+    ub = M.DeviceBuffer.from_array(u0)
+    vb = M.DeviceBuffer.from_array(np.zeros(len(cig)))
+    rb = M.DeviceBuffer.from_array(np.zeros(S.n_rows))
+    G.assemble_device(ub, vb, rb)
+    M.load_library().mgpu_synchronize()
+    kg = vb.to_array(np.float64, len(cig))
+    vgo = np.zeros(len(cig))
+    O.ghost_assemble(Pg.desc, rpg, cig, Lf, Ff, xi, nrm, alpha, u0, vgo, np.zeros(S.n_rows))
+    print(case, "ghost facets", len(Lf), "nnz", len(ci), "->", len(cig),
+          "GPU ghost K vs oracle %.2e" % rel(kg, vgo), flush=True)
+    assert rel(kg, vgo) <= 1e-12
+    f, fixed = _traction_problem(P, n)
+    wl = _lumped_weights(P, n)
+    vo, _, _ = O.assemble_desc(P.desc, rp, ci, u0, n_threads=16)
+    Ko = vgo.copy()
+    Ko[pos] += vo
+    u_ref, info = _solve(rpg, cig, Ko, f, fixed)
+    print(case, "oracle bulk + ghost K: cg iterations", info.iterations,
+          "true rel residual %.2e" % info.true_rel_residual, flush=True)
+    assert info.converged == 1
+    v64, _ = S.compute_jacobian_and_residual(u0)
+    L.attach(S, train_lq.load())
+    S.options(M.PREC_MIXED, M.QUAD_LEARNED)
+    vl, _ = S.compute_jacobian_and_residual(u0)
+    out = {}
+    for name, v in (("fp64", v64), ("mixed+learned", vl)):
+        K = kg.copy()
+        K[pos] += v
+        u, inf = _solve(rpg, cig, K, f, fixed)
+        du = u - u_ref
+        out[name] = {"K": rel(K, Ko), "euclid": rel(u, u_ref),
+                     "L2": float(np.sqrt(np.sum(wl * du * du) / np.sum(wl * u_ref * u_ref))),
+                     "iterations": inf.iterations}
+    print(case, "displacement errors with ghost", out, flush=True)
+    assert out["fp64"]["K"] <= 1e-12 and out["fp64"]["euclid"] <= 1e-8
+    assert out["mixed+learned"]["K"] <= 1e-6
+    assert out["mixed+learned"]["L2"] <= 1e-5 and out["mixed+learned"]["euclid"] <= 1e-5
