@@ -62,6 +62,12 @@ typedef int32_t mgpu_status;
 #define MGPU_QUAD_SUBDIVISION  0   /* XTK regular subdivision + node hierarchy, TET_11 / TRI_7       */
 #define MGPU_QUAD_LEARNED      1   /* MLP weights on the HEX_3x3x3 Gauss points, moment-checked,
                                       falling back to subdivision per element                      */
+#define MGPU_QUAD_MOMENT_FIT   2   /* HEX27 cells (generic path): a cut cell's TET_35 points replaced
+                                      by the 5 x 5 x 5 Gauss-Legendre nodes, weighted by the cell's
+                                      degree-4 Lagrange moments over those points; the same sum as the
+                                      subdivision for integrands of degree <= 4 per axis (the box
+                                      HEX27 stiffness), i.e. the same K up to rounding, from 125
+                                      points per cut cell                                            */
 
 /* Level-set kinds (GEN analytic geometries). */
 #define MGPU_LS_NONE     0   /* no geometry: every cell is uncut material                           */

@@ -25,7 +25,7 @@ IWG_SPATIALDIFF_BULK = 0
 IWG_STRUC_LINEAR_BULK = 1
 GEOM_QUAD4, GEOM_HEX8, GEOM_TRI3, GEOM_TET4, GEOM_HEX27 = 0, 1, 2, 3, 4
 PREC_FP64, PREC_MIXED = 0, 1
-QUAD_SUBDIVISION, QUAD_LEARNED = 0, 1
+QUAD_SUBDIVISION, QUAD_LEARNED, QUAD_MOMENT_FIT = 0, 1, 2
 LS_NONE, LS_SPHERES, LS_GYROID, LS_PLANE = 0, 1, 2, 3
 
 _c_i32p = ctypes.POINTER(ctypes.c_int32)

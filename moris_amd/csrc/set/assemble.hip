@@ -1066,9 +1066,12 @@ extern "C" mgpu_status mgpu_set_destroy(mgpu_set* set) {
 extern "C" mgpu_status mgpu_set_options(mgpu_set* set, int32_t precision, int32_t quadrature) {
     if (!set) return set_error(MGPU_ERR_INVALID_ARG, "null set");
     if (precision != MGPU_PREC_FP64 && precision != MGPU_PREC_MIXED) return set_error(MGPU_ERR_INVALID_ARG, "bad precision");
-    if (quadrature != MGPU_QUAD_SUBDIVISION && quadrature != MGPU_QUAD_LEARNED) return set_error(MGPU_ERR_INVALID_ARG, "bad quadrature");
+    if (quadrature != MGPU_QUAD_SUBDIVISION && quadrature != MGPU_QUAD_LEARNED && quadrature != MGPU_QUAD_MOMENT_FIT)
+        return set_error(MGPU_ERR_INVALID_ARG, "bad quadrature");
     if (quadrature == MGPU_QUAD_LEARNED && !set->mlp)
         return set_error(MGPU_ERR_STATE, "learned quadrature needs a model (mgpu_set_quadrature_model)");
+    if (quadrature == MGPU_QUAD_MOMENT_FIT && !(set->h.generic && set->h.nb == 27))
+        return set_error(MGPU_ERR_UNSUPPORTED, "moment-fitted quadrature is implemented for HEX27 cells");
     set->precision = precision;
     set->quadrature = quadrature;
     return MGPU_OK;

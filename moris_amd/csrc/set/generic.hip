@@ -15,9 +15,12 @@
 //                       to the element's slot of the K_e pool.
 //   k_generic_hex27     HEX27 IP cells (HEX_3x3x3 / TET_35): one block of four wavefronts per
 //                       element, the moments on MFMA (FP64, or FP32 per 64-point chunk summed in
-//                       FP64), K_p and K_e = T^T K_p T in FP64 through LDS.  In scatter mode
-//                       (sets too large for a K_e pool) one launch per colour class, each
-//                       element adding its K_e into the CSR values after k_generic_zero.
+//                       FP64), K_p and K_e = T^T K_p T in FP64 through LDS (blocks h >= f,
+//                       mirrored).  With MGPU_QUAD_MOMENT_FIT a cut element's TET_35 points only
+//                       fit the 125 weights of a 5 x 5 x 5 Gauss-Legendre rule (FP64 MFMA), and M
+//                       comes from those weights by sum factorisation (the same M up to rounding).
+//                       In scatter mode (sets too large for a K_e pool) one launch per colour
+//                       class, each element adding its K_e into the CSR values after k_generic_zero.
 //   k_generic_gather    one thread per CSR value: the sum of its K_e entries in ascending element
 //                       order (deterministic SumIntoGlobalValues, cl_Sparse_Matrix_EpetraFECrs.cpp
 //                       :103-133).
@@ -240,6 +243,146 @@ static_assert(2 * kH27Chunk * kH27PtStride <= 81 * 81, "two point-record buffers
 typedef double doublex4_t __attribute__((ext_vector_type(4)));
 typedef float floatx4_t __attribute__((ext_vector_type(4)));
 
+// Moment-fitted rule of a cut HEX27 cell (MGPU_QUAD_MOMENT_FIT): the 5 x 5 x 5 Gauss-Legendre
+// nodes xi_ijk with the weights
+//   W_ijk = sum_q w*_q L_i(xi_q,x) L_j(xi_q,y) L_k(xi_q,z)
+// over the cell's TET_35 points q, L the degree-4 Lagrange polynomials through the 5 nodes.  A
+// polynomial f of degree <= 4 in each axis equals its interpolant sum_ijk f(xi_ijk) L_i L_j L_k, so
+// sum_ijk W_ijk f(xi_ijk) = sum_q w*_q f(xi_q), the subdivision rule's sum.  The moment integrand
+// dN_a/dx_i dN_b/dx_j of a box HEX27 cell (two triquadratic factors, one differentiated each) has
+// degree <= 4 in each axis, so M and K equal the subdivision's up to rounding, from 125 points
+// instead of 35 per sub-tet.
+static constexpr double kGl5Host[5] = {-0.90617984593866399279762687829939, -0.53846931010568309103631442070021,
+                                       0.0, 0.53846931010568309103631442070021, 0.90617984593866399279762687829939};
+constexpr double gl5_inv_den(int i) {
+    double d = 1.0;
+    for (int m = 0; m < 5; ++m)
+        if (m != i) d *= kGl5Host[i] - kGl5Host[m];
+    return 1.0 / d;
+}
+static __constant__ const double kGl5[5] = {kGl5Host[0], kGl5Host[1], kGl5Host[2], kGl5Host[3], kGl5Host[4]};
+static __constant__ const double kGl5InvDen[5] = {gl5_inv_den(0), gl5_inv_den(1), gl5_inv_den(2), gl5_inv_den(3),
+                                                  gl5_inv_den(4)};
+constexpr int kMfitPoints = 125;
+constexpr int kMfitStride = 17;   // point record of the fit: w*, then L[3][5] (odd: the lanes of a staging wavefront, one point each, write distinct LDS banks)
+static_assert(2 * kH27Chunk * kMfitStride <= 81 * 81, "two fit record buffers fit the K_p region");
+static_assert(81 * 81 + 5 * 128 <= kH27Lds, "fitted weights and four partial sums after the K_p region");
+
+// L_i(x), i < 5, through the Gauss-Legendre nodes (products of the other nodes' factors)
+__device__ inline void lagrange5(double x, double* L) {
+    double d[5];
+#pragma unroll
+    for (int m = 0; m < 5; ++m) d[m] = x - kGl5[m];
+    double pre = 1.0;
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+        L[i] = pre * kGl5InvDen[i];
+        pre *= d[i];
+    }
+    double suf = 1.0;
+#pragma unroll
+    for (int i = 4; i >= 0; --i) {
+        L[i] *= suf;
+        suf *= d[i];
+    }
+}
+
+// The fitted weights of cut element (ig0, nsub) into Wfit[125] (index (i 5 + j) 5 + k): the TET_35
+// table in LDS once, the element's sub-tet vertices and their w-scale detJ (0 when detJ < -1e-6)
+// copied to LDS up to kMfitTetBatch sub-tets at a time; the TET_35 points staged from them 64 at a
+// time as FP64 records of w* and L[axis][5]; then
+//   W[(i,j)][k] += (w* L_i(x) L_j(y)) L_k(z)
+// on FP64 MFMA (rows (i, j) in two 16-row tiles, column k < 5; each wavefront over 16 points of a
+// chunk), and the wavefronts' partial sums added in wavefront order (the same sum on every call).
+constexpr int kMfitTetBatch = 256;
+constexpr int kMfitXs = 2 * kH27Chunk * kMfitStride;     // LDS offsets (doubles): staged vertices,
+constexpr int kMfitDj = kMfitXs + 12 * kMfitTetBatch;     // detJ per staged sub-tet,
+constexpr int kMfitT35 = kMfitDj + kMfitTetBatch;         // TET_35 [q][xi0, xi1, xi2, w]
+static_assert(kMfitT35 + 4 * kTet35Points <= 81 * 81, "records, staged sub-tets and the rule fit the K_p region");
+__device__ inline void mfit_weights(const GenView& g, int ig0, int nsub, const double* hh, double* lds, double* Wfit,
+                                    int tid, int wave, int col16, int kq) {
+    doublex4_t acc[2] = {doublex4_t{0.0, 0.0, 0.0, 0.0}, doublex4_t{0.0, 0.0, 0.0, 0.0}};
+    int ai[2], aj[2];
+#pragma unroll
+    for (int rt = 0; rt < 2; ++rt) {
+        const int r = rt * 16 + col16;
+        ai[rt] = r < 25 ? r / 5 : -1;
+        aj[rt] = r < 25 ? r - 5 * (r / 5) : 0;
+    }
+    double* Xs = lds + kMfitXs;
+    double* Dj = lds + kMfitDj;
+    double* T35 = lds + kMfitT35;
+    if (tid < 4 * kTet35Points) {
+        const int q = tid >> 2, c = tid & 3;
+        T35[tid] = c < 3 ? kTet35Xi[q][c] : kTet35W[q];
+    }
+    for (int t0 = 0; t0 < nsub; t0 += kMfitTetBatch) {
+        const int nt = min(kMfitTetBatch, nsub - t0), npts = nt * kTet35Points;
+        if (t0 > 0) __syncthreads();   // the previous batch's records and vertices are read
+        const double* src = g.ig_xi + (size_t)(ig0 + t0) * 12;
+        for (int idx = tid; idx < nt * 12; idx += 256) Xs[idx] = src[idx];
+        if (tid < nt) {
+            const double* X = src + (size_t)tid * 12;
+            double J[3][3];
+            const int rr[3] = {0, 1, 3};
+            for (int r = 0; r < 3; ++r)
+                for (int d = 0; d < 3; ++d) J[r][d] = (X[rr[r] * 3 + d] - X[2 * 3 + d]) * (0.5 * hh[d]);
+            const double detJ = (J[0][0] * (J[1][1] * J[2][2] - J[2][1] * J[1][2]) -
+                                 J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                                 J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0])) / 6.0;
+            Dj[tid] = detJ < kDetJLowerLimit ? 0.0 : detJ;
+        }
+        __syncthreads();
+        for (int p0 = 0, buf = 0; p0 < npts; p0 += kH27Chunk, buf ^= 1) {
+            double* pts = lds + buf * kH27Chunk * kMfitStride;
+            for (int slot = tid; slot < 4 * kH27Chunk; slot += 256) {
+                const int pl = slot & (kH27Chunk - 1), role = slot / kH27Chunk;
+                const int ip = p0 + pl;
+                double* pp = pts + pl * kMfitStride;
+                const bool in = ip < npts;
+                const int t = in ? ip / kTet35Points : 0, q = in ? ip - kTet35Points * t : 0;
+                const double* X = Xs + t * 12;
+                const double* tq = T35 + 4 * q;
+                if (role < 3) {
+                    const int d = role;
+                    const double xd = in ? tq[0] * X[d] + tq[1] * X[3 + d] + (1.0 - tq[0] - tq[1] - tq[2]) * X[6 + d] +
+                                               tq[2] * X[9 + d]
+                                         : 0.0;
+                    double L[5];
+                    lagrange5(xd, L);
+#pragma unroll
+                    for (int k = 0; k < 5; ++k) pp[1 + 5 * d + k] = L[k];
+                } else {
+                    pp[0] = in ? tq[3] * Dj[t] : 0.0;
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int s = 0; s < kH27Chunk / 16; ++s) {
+                const double* pp = pts + (wave * (kH27Chunk / 4) + 4 * s + kq) * kMfitStride;
+                const double b = col16 < 5 ? pp[11 + col16] : 0.0;
+#pragma unroll
+                for (int rt = 0; rt < 2; ++rt) {
+                    const double a = ai[rt] >= 0 ? (pp[0] * pp[1 + ai[rt]]) * pp[6 + aj[rt]] : 0.0;
+                    acc[rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[rt], 0, 0, 0);
+                }
+            }
+        }
+    }
+    // partial sums of the wavefronts after the 128 fitted weights (outside the record buffers)
+    double* Wp = Wfit + 128 + wave * 128;
+#pragma unroll
+    for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int r = rt * 16 + kq + 4 * j;
+            if (r < 25 && col16 < 5) Wp[r * 5 + col16] = acc[rt][j];
+        }
+    __syncthreads();
+    if (tid < kMfitPoints) Wfit[tid] = ((Wfit[128 + tid] + Wfit[256 + tid]) + Wfit[384 + tid]) + Wfit[512 + tid];
+    __syncthreads();
+}
+
 // M is symmetric: only the 21 tiles (mt <= nt) of the 6 x 6 tile grid are computed, the rest
 // mirrored.  Wavefront W owns H27Tiles<W> (rows sharing an A operand, columns a B operand):
 // W0 row 0; W1 row 1; W2 row 2 and (5,5); W3 rows 3 and 4.
@@ -319,7 +462,77 @@ __device__ inline void h27_store(double* Ms, const doublex4_t* acc, int col16, i
         }
 }
 
-template <int NF, typename T, bool SCATTER>
+// M of a cut element from its fitted weights, by sum factorisation over the tensor grid of nodes:
+//   M[(i,a),(j,b)] = s_i s_j sum_{x,y,z} W_xyz P_0(a_0,b_0; x) P_1(a_1,b_1; y) P_2(a_2,b_2; z),
+// (a_0, a_1, a_2) the 1D indices of HEX27 basis a (kHex27Half), s_i = 2 / h_i, and
+// P_d(alpha, beta; m) = F_alpha(xi_m) F'_beta(xi_m), F the quadratic Lagrange value, differentiated
+// on the gradient's axis (d == i for alpha, d == j for beta): four factor types tau = 2 [d == i] +
+// [d == j] per axis.  Contracted over z, then y, then x in FP64 (about 38k multiply-adds where 125
+// points x 81 x 81 would take 820k); the six blocks i <= j are computed, the other three mirrored.
+// LDS after the K_p region (doubles): the fitted weights, then the tables below.
+constexpr int kSfP = 128;                   // P[tau][alpha][beta][m], 4 x 3 x 3 x 5
+constexpr int kSfZ = kSfP + 180;            // Z[zt][x 5 + y][alpha_2 beta_2], tau_2 = {0, 1, 3}[zt]
+constexpr int kSfY = kSfZ + 3 * 25 * 9;     // Y[block][x][alpha_1 beta_1][alpha_2 beta_2]
+constexpr int kSfInv = kSfY + 6 * 5 * 81;   // int [27]: HEX27 basis of the 1D indices (a_0 9 + a_1 3 + a_2)
+static_assert(81 * 81 + kSfInv + 14 <= kH27Lds, "sum factorisation tables fit after the K_p region");
+static_assert(kSfP >= kMfitPoints, "the tables follow the fitted weights");
+__device__ inline void mfit_moments(double* lds, const double* hh, int tid) {
+    double* base = lds + 81 * 81;
+    const double* W = base;
+    double* P = base + kSfP;
+    double* Z = base + kSfZ;
+    double* Y = base + kSfY;
+    int* inv = reinterpret_cast<int*>(base + kSfInv);
+    if (tid < 180) {
+        const int tau = tid / 45, r = tid - 45 * tau, al = r / 15, be = (r / 5) % 3, m = r % 5;
+        double l[3], dl[3];
+        lagrange_quadratic_1d(kGl5[m], l, dl);
+        const double fa = (tau & 2) ? (al == 0 ? dl[0] : (al == 1 ? dl[1] : dl[2])) : (al == 0 ? l[0] : (al == 1 ? l[1] : l[2]));
+        const double fb = (tau & 1) ? (be == 0 ? dl[0] : (be == 1 ? dl[1] : dl[2])) : (be == 0 ? l[0] : (be == 1 ? l[1] : l[2]));
+        P[tid] = fa * fb;
+    } else if (tid >= 192 && tid < 192 + 27) {
+        const int a = tid - 192;
+        inv[(kHex27Half[a][0] * 3 + kHex27Half[a][1]) * 3 + kHex27Half[a][2]] = a;
+    }
+    __syncthreads();
+    for (int idx = tid; idx < 3 * 225; idx += 256) {
+        const int zt = idx / 225, r = idx - 225 * zt, xy = r / 9, ab = r - 9 * xy;
+        const double* Pp = P + ((zt == 2 ? 3 : zt) * 9 + ab) * 5;
+        const double* Wp = W + xy * 5;
+        double v = 0.0;
+        for (int m = 0; m < 5; ++m) v += Wp[m] * Pp[m];
+        Z[idx] = v;
+    }
+    __syncthreads();
+    // block p: (i, j) = (0,0) (0,1) (0,2) (1,1) (1,2) (2,2)
+    for (int idx = tid; idx < 6 * 405; idx += 256) {
+        const int p = idx / 405, r = idx - 405 * p, x = r / 81, ab12 = r - 81 * x, ab1 = ab12 / 9, ab2 = ab12 - 9 * ab1;
+        const int i = (p >= 3) + (p >= 5), j = p < 3 ? p : (p < 5 ? p - 2 : 2);
+        const int tau1 = 2 * (i == 1) + (j == 1), tau2 = 2 * (i == 2) + (j == 2);
+        const double* Zp = Z + (tau2 == 3 ? 2 : tau2) * 225 + x * 45 + ab2;
+        const double* Pp = P + (tau1 * 9 + ab1) * 5;
+        double v = 0.0;
+        for (int y = 0; y < 5; ++y) v += Zp[9 * y] * Pp[y];
+        Y[idx] = v;
+    }
+    __syncthreads();
+    for (int idx = tid; idx < 6 * 729; idx += 256) {
+        const int p = idx / 729, r = idx - 729 * p, ab0 = r / 81, ab12 = r - 81 * ab0;
+        const int i = (p >= 3) + (p >= 5), j = p < 3 ? p : (p < 5 ? p - 2 : 2);
+        const int tau0 = 2 * (i == 0) + (j == 0);
+        const double* Yp = Y + p * 405 + ab12;
+        const double* Pp = P + (tau0 * 9 + ab0) * 5;
+        double v = 0.0;
+        for (int x = 0; x < 5; ++x) v += Yp[81 * x] * Pp[x];
+        v *= (2.0 / hh[i]) * (2.0 / hh[j]);
+        const int ab1 = ab12 / 9, ab2 = ab12 - 9 * ab1;
+        const int a = inv[((ab0 / 3) * 3 + ab1 / 3) * 3 + ab2 / 3], b = inv[((ab0 % 3) * 3 + ab1 % 3) * 3 + ab2 % 3];
+        lds[(i * 27 + a) * 81 + j * 27 + b] = v;
+        if (i != j) lds[(j * 27 + b) * 81 + i * 27 + a] = v;
+    }
+}
+
+template <int NF, typename T, bool SCATTER, bool MFIT>
 __global__ __launch_bounds__(256) void k_generic_hex27(GenView g, int c0, double* vals) {
     constexpr int NB = 27, NG = 81, NP = NF * NB;
     __shared__ double lds[kH27Lds];
@@ -330,9 +543,14 @@ __global__ __launch_bounds__(256) void k_generic_hex27(GenView g, int c0, double
     for (int d = 0; d < 3; ++d) hh[d] = g.ip_box[(size_t)e * 6 + 3 + d];
     const int ig0 = g.ig_ptr[e], nsub = g.ig_ptr[e + 1] - ig0;
     const bool cut = nsub > 0;
-    const int npts = cut ? nsub * kTet35Points : kHex3Points;
+    // moment-fitted cut elements take M from the fitted weights (mfit_moments), not from points
+    const bool fit = MFIT && cut;
+    const int npts = fit ? 0 : (cut ? nsub * kTet35Points : kHex3Points);
 
     const int col16 = lane & 15, kq = lane >> 4;
+    // moment-fitted rule (cut elements): the 125 weights after the K_p region
+    if constexpr (MFIT)
+        if (fit) mfit_weights(g, ig0, nsub, hh, lds, lds + NG * NG, tid, wave, col16, kq);
     // the lane's operand columns cg * 16 + lane % 16, cg < 6: (i, a) = (col / 27, col % 27), the
     // record offsets of dl_i(xi_i), l(xi_j), l(xi_k) and 2 / h_i (zero beyond column 80)
     int go[6];
@@ -416,13 +634,18 @@ __global__ __launch_bounds__(256) void k_generic_hex27(GenView g, int c0, double
     }
     __syncthreads();   // the last chunk's records are read before M overwrites them
 
-    // M to LDS: the wavefront's tiles and their mirror images
+    // M to LDS: the wavefront's tiles and their mirror images, or (moment-fitted cut elements) the
+    // sum factorisation of the fitted weights
     double* Ms = lds;
-    switch (wave) {
-        case 0: h27_store<0, T>(Ms, acc, col16, kq); break;
-        case 1: h27_store<1, T>(Ms, acc, col16, kq); break;
-        case 2: h27_store<2, T>(Ms, acc, col16, kq); break;
-        default: h27_store<3, T>(Ms, acc, col16, kq); break;
+    if (fit) {
+        mfit_moments(lds, hh, tid);
+    } else {
+        switch (wave) {
+            case 0: h27_store<0, T>(Ms, acc, col16, kq); break;
+            case 1: h27_store<1, T>(Ms, acc, col16, kq); break;
+            case 2: h27_store<2, T>(Ms, acc, col16, kq); break;
+            default: h27_store<3, T>(Ms, acc, col16, kq); break;
+        }
     }
     __syncthreads();
 
@@ -481,16 +704,17 @@ __global__ __launch_bounds__(256) void k_generic_hex27(GenView g, int c0, double
     // K_e = T^T K_p T per (f, h) block on FP64 MFMA (16 x 16 tiles, k in steps of 4, operands
     // from LDS; A: row lane % 16, k = lane / 16; B: k = lane / 16, column lane % 16; result
     // register j of a lane: row lane / 16 + 4 j, column lane % 16): W = K_p[f][h] T (27 x na_t),
-    // then T^T W.  The wavefronts take the tiles in turn.  K_e is stored to the element's slot of
-    // the pool, or (scatter mode) added into the CSR values: entry (x, y) of block (f, h) sits in
-    // row topo[f na_t + x], field block h of that row, at the rank of adof y among the row adof's
-    // neighbours.  No other element of the colour touches the row.
+    // then T^T W.  The wavefronts take the tiles in turn.  K_p being symmetric, so is K_e: only the
+    // blocks h >= f are computed, block (h, f) is their mirror image.  K_e is stored to the
+    // element's slot of the pool, or (scatter mode) added into the CSR values: entry (x, y) of block
+    // (f, h) sits in row topo[f na_t + x], field block h of that row, at the rank of adof y among the
+    // row adof's neighbours.  No other element of the colour touches the row.
     double* ke = SCATTER ? nullptr : g.ke + g.ke_off[e];
     const int32_t* rows = SCATTER ? g.topo + g.topo_ptr[e] : nullptr;
     const uint16_t* rk = SCATTER ? g.rank + g.rank_off[e] : nullptr;
     const int n_nt = (na_t + 15) / 16;   // 16-column tiles of T, W and K_e
     for (int f = 0; f < NF; ++f)
-        for (int hb = 0; hb < NF; ++hb) {
+        for (int hb = f; hb < NF; ++hb) {
             for (int tile = wave; tile < 2 * n_nt; tile += 4) {
                 const int mt = tile / n_nt, nt = tile - n_nt * (tile / n_nt);
                 const int arow = mt * 16 + col16, bcol = nt * 16 + col16;   // a; y
@@ -512,21 +736,28 @@ __global__ __launch_bounds__(256) void k_generic_hex27(GenView g, int c0, double
             for (int tile = wave; tile < n_nt * n_nt; tile += 4) {
                 const int mt = tile / n_nt, nt = tile - n_nt * (tile / n_nt);
                 const int xrow = mt * 16 + col16, ycol = nt * 16 + col16;   // x; y
-                // scatter mode: the tile's CSR positions and old values are requested before its
-                // MFMA work, which hides their chain of dependent loads
-                [[maybe_unused]] int pos[4];
-                [[maybe_unused]] double old[4];
+                // scatter mode: the tile's CSR positions and old values (and those of its mirror
+                // image in block (h, f)) are requested before its MFMA work, which hides their
+                // chain of dependent loads
+                [[maybe_unused]] int pos[4], pos2[4];
+                [[maybe_unused]] double old[4], old2[4];
                 if constexpr (SCATTER) {
 #pragma unroll
                     for (int j = 0; j < 4; ++j) {
                         const int x = mt * 16 + kq + 4 * j;
-                        pos[j] = -1;
-                        old[j] = 0.0;
+                        pos[j] = pos2[j] = -1;
+                        old[j] = old2[j] = 0.0;
                         if (x < na_t && ycol < na_t) {
                             const int32_t row = rows[f * na_t + x];
                             const int32_t r0 = g.row_ptr[row], len = (g.row_ptr[row + 1] - r0) / NF;
                             pos[j] = r0 + hb * len + rk[x * na_t + ycol];
                             old[j] = vals[pos[j]];
+                            if (hb != f) {
+                                const int32_t row2 = rows[hb * na_t + ycol];
+                                const int32_t r2 = g.row_ptr[row2], len2 = (g.row_ptr[row2 + 1] - r2) / NF;
+                                pos2[j] = r2 + f * len2 + rk[ycol * na_t + x];
+                                old2[j] = vals[pos2[j]];
+                            }
                         }
                     }
                 }
@@ -543,8 +774,10 @@ __global__ __launch_bounds__(256) void k_generic_hex27(GenView g, int c0, double
                     const int x = mt * 16 + kq + 4 * j;
                     if constexpr (SCATTER) {
                         if (pos[j] >= 0) vals[pos[j]] = old[j] + c4[j];
+                        if (pos2[j] >= 0) vals[pos2[j]] = old2[j] + c4[j];
                     } else if (x < na_t && ycol < na_t) {
                         ke[(size_t)(f * na_t + x) * NA + hb * na_t + ycol] = c4[j];
+                        if (hb != f) ke[(size_t)(hb * na_t + ycol) * NA + f * na_t + x] = c4[j];
                     }
                 }
             }
@@ -634,16 +867,22 @@ mgpu_status generic_upload(mgpu_set* S, const mgpu_set_desc& d) {
 }
 
 namespace {
-template <bool SCATTER>
-void launch_hex27(mgpu_set* S, bool f32, dim3 gr, int c0, double* vals) {
+template <bool SCATTER, bool MFIT>
+void launch_hex27_q(mgpu_set* S, bool f32, dim3 gr, int c0, double* vals) {
     const dim3 bl(256);
     if (S->h.nf == 1) {
-        if (f32) hipLaunchKernelGGL((k_generic_hex27<1, float, SCATTER>), gr, bl, 0, S->stream, S->g, c0, vals);
-        else hipLaunchKernelGGL((k_generic_hex27<1, double, SCATTER>), gr, bl, 0, S->stream, S->g, c0, vals);
+        if (f32) hipLaunchKernelGGL((k_generic_hex27<1, float, SCATTER, MFIT>), gr, bl, 0, S->stream, S->g, c0, vals);
+        else hipLaunchKernelGGL((k_generic_hex27<1, double, SCATTER, MFIT>), gr, bl, 0, S->stream, S->g, c0, vals);
     } else {
-        if (f32) hipLaunchKernelGGL((k_generic_hex27<3, float, SCATTER>), gr, bl, 0, S->stream, S->g, c0, vals);
-        else hipLaunchKernelGGL((k_generic_hex27<3, double, SCATTER>), gr, bl, 0, S->stream, S->g, c0, vals);
+        if (f32) hipLaunchKernelGGL((k_generic_hex27<3, float, SCATTER, MFIT>), gr, bl, 0, S->stream, S->g, c0, vals);
+        else hipLaunchKernelGGL((k_generic_hex27<3, double, SCATTER, MFIT>), gr, bl, 0, S->stream, S->g, c0, vals);
     }
+}
+
+template <bool SCATTER>
+void launch_hex27(mgpu_set* S, bool f32, dim3 gr, int c0, double* vals) {
+    if (S->quadrature == MGPU_QUAD_MOMENT_FIT) launch_hex27_q<SCATTER, true>(S, f32, gr, c0, vals);
+    else launch_hex27_q<SCATTER, false>(S, f32, gr, c0, vals);
 }
 }  // namespace
 

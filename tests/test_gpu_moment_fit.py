@@ -1,0 +1,107 @@
+"""GPU tests of the moment-fitted quadrature of cut HEX27 cells (MGPU_QUAD_MOMENT_FIT): each cut
+cell's TET_35 points replaced by the 5 x 5 x 5 Gauss-Legendre nodes weighted by the cell's degree-4
+Lagrange moments over those points, which gives the subdivision's sum for every integrand of degree
+<= 4 per axis, the box HEX27 stiffness among them.
+
+* against the oracle (TET_35 on every sub-tet) at 1e-12 on the FP64 path and 1e-6 on the mixed
+  path: p = 2 B-spline and nodal rows, diffusion and elasticity, one THB level, an enriched set;
+  pattern unchanged; repeated calls bit-identical;
+* scatter mode (colour classes) to the same bars;
+* size-independent on a 16^3 gyroid HEX27 set and a 12^3-base THB set: within 1e-12 of the
+  subdivision path on the GPU;
+* refused with MGPU_ERR_UNSUPPORTED on sets without HEX27 cells.
+"""
+import numpy as np
+import pytest
+
+import moris_amd as M
+from helpers import SEED, rel, two_bodies
+from oracle import py as O
+
+pytestmark = pytest.mark.gpu
+
+CASES = {
+    "hex27_p2_gyroid_6_elasticity": lambda: M.make_spec(3, (6, 6, 6), iwg=M.IWG_STRUC_LINEAR_BULK,
+                                                        ls_kind=M.LS_GYROID, gyroid_scale=2 * np.pi * 1.5,
+                                                        bspline_order=2, lagrange_order=2),
+    "hex27_p2_sphere_6_diffusion": lambda: M.make_spec(3, (6, 5, 4), spheres=[[0.5, 0.5, 0.5, 0.3]], k_sigma=0.5,
+                                                       bspline_order=2, lagrange_order=2),
+    "hex27_lagrange_sphere_5_elasticity": lambda: M.make_spec(3, (5, 5, 5), iwg=M.IWG_STRUC_LINEAR_BULK,
+                                                              spheres=[[0.4, 0.5, 0.6, 0.35]], lagrange_order=2),
+    "thb_plane_5_hex27_elasticity": lambda: M.make_spec(3, (5, 5, 5), iwg=M.IWG_STRUC_LINEAR_BULK, ls_kind=M.LS_PLANE,
+                                                        plane=[0.3, 0.5, 0.8, -0.6], bspline_order=2,
+                                                        lagrange_order=2, refine_levels=1),
+    "thb_sphere_4_hex27_diffusion": lambda: M.make_spec(3, (4, 4, 4), spheres=[[0.5, 0.5, 0.5, 0.3]], k_sigma=0.4,
+                                                        bspline_order=2, lagrange_order=2, refine_levels=1),
+    "enriched_two_spheres_6_hex27_elasticity": lambda: M.make_spec(3, (6, 6, 6), iwg=M.IWG_STRUC_LINEAR_BULK,
+                                                                   spheres=two_bodies(3, 6), material_phase=0,
+                                                                   bspline_order=2, lagrange_order=2, enrich=1),
+}
+
+
+def _check_against_oracle(S, spec):
+    Q = O.OracleProblem(spec)
+    s = Q.setup()
+    rp, ci = S.pattern()
+    assert np.array_equal(rp, s["row_ptr"]) and np.array_equal(ci, s["col_idx"])
+    u = np.random.default_rng(SEED + 21).uniform(-1.0, 1.0, S.n_rows)
+    vo, Ro = Q.assemble(u)
+    S.options(M.PREC_FP64, M.QUAD_MOMENT_FIT)
+    vals, R = S.compute_jacobian_and_residual(u)
+    print("FP64 moment fit vs oracle |dK|/|K| %.2e |dR|/|R| %.2e" % (rel(vals, vo), rel(R, Ro)), flush=True)
+    assert rel(vals, vo) <= 1e-12 and rel(R, Ro) <= 1e-12
+    v2, R2 = S.compute_jacobian_and_residual(u)
+    assert np.array_equal(v2.view(np.uint64), vals.view(np.uint64)) and np.array_equal(R2.view(np.uint64), R.view(np.uint64))
+    S.options(M.PREC_MIXED, M.QUAD_MOMENT_FIT)
+    vm, Rm = S.compute_jacobian_and_residual(u)
+    print("mixed moment fit vs oracle |dK|/|K| %.2e" % rel(vm, vo), flush=True)
+    assert rel(vm, vo) <= 1e-6 and rel(Rm, Ro) <= 1e-6
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_moment_fit_matches_oracle(case):
+    print("case", case, flush=True)
+    spec, keep = CASES[case]()
+    P = M.Problem(spec, keep)
+    S = M.Set(P.desc, keepalive=P)
+    _check_against_oracle(S, spec)
+
+
+@pytest.mark.parametrize("case", ["hex27_p2_gyroid_6_elasticity", "thb_sphere_4_hex27_diffusion"])
+def test_moment_fit_scatter_mode_matches_oracle(case, monkeypatch):
+    print("case", case, flush=True)
+    spec, keep = CASES[case]()
+    P = M.Problem(spec, keep)
+    monkeypatch.setenv("MGPU_GENERIC_SCATTER", "1")
+    S = M.Set(P.desc, keepalive=P)
+    monkeypatch.delenv("MGPU_GENERIC_SCATTER")
+    _check_against_oracle(S, spec)
+
+
+@pytest.mark.parametrize("refine", [0, 1])
+def test_moment_fit_equals_subdivision_on_larger_sets(refine):
+    n = 16 if refine == 0 else 12
+    spec, keep = M.make_spec(3, (n, n, n), iwg=M.IWG_STRUC_LINEAR_BULK, ls_kind=M.LS_GYROID,
+                             gyroid_scale=2 * np.pi * n / 4.0, bspline_order=2, lagrange_order=2,
+                             refine_levels=refine)
+    P = M.Problem(spec, keep)
+    S = M.Set(P.desc, keepalive=P)
+    u = np.random.default_rng(SEED + 22).uniform(-1.0, 1.0, S.n_rows)
+    v_sub, R_sub = S.compute_jacobian_and_residual(u)
+    S.options(M.PREC_FP64, M.QUAD_MOMENT_FIT)
+    v_fit, R_fit = S.compute_jacobian_and_residual(u)
+    S.options(M.PREC_MIXED, M.QUAD_MOMENT_FIT)
+    v_mix, _ = S.compute_jacobian_and_residual(u)
+    print("elements", P.n_elements, "fit vs subdivision %.2e, mixed fit %.2e" % (rel(v_fit, v_sub), rel(v_mix, v_sub)),
+          flush=True)
+    assert rel(v_fit, v_sub) <= 1e-12 and rel(R_fit, R_sub) <= 1e-12
+    assert rel(v_mix, v_sub) <= 1e-6
+
+
+def test_moment_fit_refused_without_hex27_cells():
+    for kw in ({}, {"bspline_order": 2}):
+        spec, keep = M.make_spec(3, (6, 6, 6), iwg=M.IWG_STRUC_LINEAR_BULK, spheres=[[0.5, 0.5, 0.5, 0.3]], **kw)
+        P = M.Problem(spec, keep)
+        S = M.Set(P.desc, keepalive=P)
+        with pytest.raises(M.MgpuError, match="HEX27"):
+            S.options(M.PREC_FP64, M.QUAD_MOMENT_FIT)
