@@ -265,7 +265,6 @@ static __constant__ const double kGl5InvDen[5] = {gl5_inv_den(0), gl5_inv_den(1)
                                                   gl5_inv_den(4)};
 constexpr int kMfitPoints = 125;
 constexpr int kMfitStride = 17;   // point record of the fit: w*, then L[3][5] (odd: the lanes of a staging wavefront, one point each, write distinct LDS banks)
-static_assert(2 * kH27Chunk * kMfitStride <= 81 * 81, "two fit record buffers fit the K_p region");
 static_assert(81 * 81 + 5 * 128 <= kH27Lds, "fitted weights and four partial sums after the K_p region");
 
 // L_i(x), i < 5, through the Gauss-Legendre nodes (products of the other nodes' factors)
@@ -294,21 +293,27 @@ __device__ inline void lagrange5(double x, double* L) {
 //   W[(i,j)][k] += (w* L_i(x) L_j(y)) L_k(z)
 // on FP64 MFMA (rows (i, j) in two 16-row tiles, column k < 5; each wavefront over 16 points of a
 // chunk), and the wavefronts' partial sums added in wavefront order (the same sum on every call).
-constexpr int kMfitTetBatch = 256;
-constexpr int kMfitXs = 2 * kH27Chunk * kMfitStride;     // LDS offsets (doubles): staged vertices,
+constexpr int kMfitChunk = 128;                           // points per staging round of the fit
+constexpr int kMfitTetBatch = 128;
+constexpr int kMfitXs = 2 * kMfitChunk * kMfitStride;    // LDS offsets (doubles): staged vertices,
 constexpr int kMfitDj = kMfitXs + 12 * kMfitTetBatch;     // detJ per staged sub-tet,
 constexpr int kMfitT35 = kMfitDj + kMfitTetBatch;         // TET_35 [q][xi0, xi1, xi2, w]
 static_assert(kMfitT35 + 4 * kTet35Points <= 81 * 81, "records, staged sub-tets and the rule fit the K_p region");
 __device__ inline void mfit_weights(const GenView& g, int ig0, int nsub, const double* hh, double* lds, double* Wfit,
                                     int tid, int wave, int col16, int kq) {
     doublex4_t acc[2] = {doublex4_t{0.0, 0.0, 0.0, 0.0}, doublex4_t{0.0, 0.0, 0.0, 0.0}};
-    int ai[2], aj[2];
+    // operand roles of the lane, as record offsets and 0 / 1 masks, so that every LDS read is
+    // unconditional (a branch around a read keeps the compiler from batching the reads of a chunk)
+    int ao[2], bo;
+    double am[2];
 #pragma unroll
     for (int rt = 0; rt < 2; ++rt) {
-        const int r = rt * 16 + col16;
-        ai[rt] = r < 25 ? r / 5 : -1;
-        aj[rt] = r < 25 ? r - 5 * (r / 5) : 0;
+        const int r = rt * 16 + col16, rc = r < 25 ? r : 0;
+        ao[rt] = (1 + rc / 5) | ((6 + rc % 5) << 8);
+        am[rt] = r < 25 ? 1.0 : 0.0;
     }
+    bo = 11 + (col16 < 5 ? col16 : 0);
+    const double bm = col16 < 5 ? 1.0 : 0.0;
     double* Xs = lds + kMfitXs;
     double* Dj = lds + kMfitDj;
     double* T35 = lds + kMfitT35;
@@ -333,21 +338,25 @@ __device__ inline void mfit_weights(const GenView& g, int ig0, int nsub, const d
             Dj[tid] = detJ < kDetJLowerLimit ? 0.0 : detJ;
         }
         __syncthreads();
-        for (int p0 = 0, buf = 0; p0 < npts; p0 += kH27Chunk, buf ^= 1) {
-            double* pts = lds + buf * kH27Chunk * kMfitStride;
-            for (int slot = tid; slot < 4 * kH27Chunk; slot += 256) {
-                const int pl = slot & (kH27Chunk - 1), role = slot / kH27Chunk;
-                const int ip = p0 + pl;
+        // software-pipelined: the MFMA work of chunk c is issued before chunk c + 1 is staged into
+        // the other buffer, so the staging arithmetic overlaps the matrix work; one barrier per chunk
+        const int nch = (npts + kMfitChunk - 1) / kMfitChunk;
+        auto stage = [&](int c) {
+            double* pts = lds + (c & 1) * kMfitChunk * kMfitStride;
+            for (int slot = tid; slot < 4 * kMfitChunk; slot += 256) {
+                const int pl = slot & (kMfitChunk - 1), role = slot / kMfitChunk;
+                const int ip = c * kMfitChunk + pl;
                 double* pp = pts + pl * kMfitStride;
+                // points past the end read point 0's data and get weight 0
                 const bool in = ip < npts;
-                const int t = in ? ip / kTet35Points : 0, q = in ? ip - kTet35Points * t : 0;
+                const int ipc = in ? ip : 0;
+                const int t = ipc / kTet35Points, q = ipc - kTet35Points * t;
                 const double* X = Xs + t * 12;
                 const double* tq = T35 + 4 * q;
                 if (role < 3) {
                     const int d = role;
-                    const double xd = in ? tq[0] * X[d] + tq[1] * X[3 + d] + (1.0 - tq[0] - tq[1] - tq[2]) * X[6 + d] +
-                                               tq[2] * X[9 + d]
-                                         : 0.0;
+                    const double xd = tq[0] * X[d] + tq[1] * X[3 + d] + (1.0 - tq[0] - tq[1] - tq[2]) * X[6 + d] +
+                                      tq[2] * X[9 + d];
                     double L[5];
                     lagrange5(xd, L);
 #pragma unroll
@@ -356,17 +365,23 @@ __device__ inline void mfit_weights(const GenView& g, int ig0, int nsub, const d
                     pp[0] = in ? tq[3] * Dj[t] : 0.0;
                 }
             }
-            __syncthreads();
+        };
+        stage(0);
+        __syncthreads();
+        for (int c = 0; c < nch; ++c) {
+            const double* pts = lds + (c & 1) * kMfitChunk * kMfitStride;
 #pragma unroll
-            for (int s = 0; s < kH27Chunk / 16; ++s) {
-                const double* pp = pts + (wave * (kH27Chunk / 4) + 4 * s + kq) * kMfitStride;
-                const double b = col16 < 5 ? pp[11 + col16] : 0.0;
+            for (int s = 0; s < kMfitChunk / 16; ++s) {
+                const double* pp = pts + (wave * (kMfitChunk / 4) + 4 * s + kq) * kMfitStride;
+                const double b = bm * pp[bo];
 #pragma unroll
                 for (int rt = 0; rt < 2; ++rt) {
-                    const double a = ai[rt] >= 0 ? (pp[0] * pp[1 + ai[rt]]) * pp[6 + aj[rt]] : 0.0;
+                    const double a = am[rt] * ((pp[0] * pp[ao[rt] & 255]) * pp[ao[rt] >> 8]);
                     acc[rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[rt], 0, 0, 0);
                 }
             }
+            if (c + 1 < nch) stage(c + 1);
+            __syncthreads();
         }
     }
     // partial sums of the wavefronts after the 128 fitted weights (outside the record buffers)
