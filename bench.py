@@ -171,6 +171,7 @@ def run_hex27(args):
     log("hex27 problem %.1fs: %d^3, elements %d, cut %d, sub-tets %d" % (time.time() - t0, n, P.n_elements, P.n_cut,
                                                                          P.n_ig))
     S = M.Set(P.desc, keepalive=P)
+    t_host = time.time() - t0
     # sets whose element matrices exceed 2^28 entries assemble in scatter mode (no K_e pool);
     # MGPU_GENERIC_SCATTER=1 / =0 forces it on / off
     ke_entries = int(np.sum(np.diff(S.topology_offsets()).astype(np.int64) ** 2))
@@ -236,6 +237,18 @@ def run_hex27(args):
         out["roofline"]["traffic"] = THB48_SCATTER_TRAFFIC_BYTES
         out["roofline"]["traffic_source"] = ("rocprofv3 --pmc FETCH_SIZE (x2.0) + WRITE_SIZE (x1.0), summed over "
                                              "the colour launches of one call")
+    if not args.refine:
+        # the set-up of the same set on the GPU (cut, numbering, T rows at the 27 nodes, pattern)
+        from moris_amd import gpu_setup as G
+        Gs = G.GpuSetup(spec, keep)
+        Gs.close()
+        t1 = time.time()
+        Gs = G.GpuSetup(spec, keep)
+        wall = time.time() - t1
+        out["setup"] = {"gpu_device_ms": Gs.ms, "gpu_wall_ms": 1e3 * wall, "host_problem_and_set_s": t_host,
+                        "same_counts_as_host": [Gs.n_elements, Gs.n_ig, Gs.nnz] == [P.n_elements, P.n_ig, S.nnz]}
+        log("hex27 GPU set-up: %.2f ms device, %.2f ms wall (host problem + set %.2f s)" % (Gs.ms, 1e3 * wall, t_host))
+        Gs.close()
     if not args.no_cpu:
         log("hex27 cpu baseline ...")
         out["cpu_baseline"] = cpu_baseline(P, S, u, budget_s=15.0, probe_n=200, rule="TET_35")

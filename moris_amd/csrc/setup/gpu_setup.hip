@@ -1,7 +1,8 @@
 // gpu_setup.hip -- the set-up of a lattice set on the GPU: level set at the lattice nodes, XTK cut of
 // every intersected cell, element list, adof numbering, T rows (p = 1 nodal, p = 2 level-0
 // B-splines), element topology and CSR pattern.  It produces the same arrays as the host path
-// (setup/problem.cpp + set/set_host.cpp) for HEX8 / QUAD4 IP cells without refinement.
+// (setup/problem.cpp + set/set_host.cpp) for HEX8 / QUAD4 IP cells without refinement, and for
+// HEX27 IP cells with p = 2 rows (the cut on the cell's corners, the rows at the 27 nodes).
 //
 //   intersected cells          Geometry_Engine::is_intersected, cl_GEN_Geometry_Engine.cpp:278-300
 //   cut of one cell            common/xtk_cut.hpp (regular subdivision + node hierarchy,
@@ -52,9 +53,11 @@ namespace mgpu {
 namespace su {
 
 constexpr int kT = 256;
+constexpr int kHex27RowEntries = 343;   // T entries of the 27 rows of a HEX27 element, p = 2
 
 struct Lat {
     int dim, nb, order, phase, iwg;
+    int nb_ip;   // IP-cell bases: nb, or 27 for HEX27 cells
     int n[3];
     double lo[3], h[3];
     int64_t n_cells, n_nodes, n_faces;
@@ -240,7 +243,16 @@ __global__ void k_elem_fill(Lat L, const double* phi, const int8_t* cls, const i
         ip_box[(size_t)e * d2 + d] = L.lo[d] + (double)ijk[d] * L.h[d];
         ip_box[(size_t)e * d2 + L.dim + d] = L.h[d];
     }
-    for (int v = 0; v < L.nb; ++v) vertex_phi[(size_t)e * L.nb + v] = phi[cell_node(L, c, v)];
+    if (L.nb_ip == 27) {
+        // level set at the 27 HEX27 nodes (the cut itself uses the 8 corner values)
+        for (int v = 0; v < 27; ++v) {
+            double x[3];
+            for (int d = 0; d < 3; ++d) x[d] = L.lo[d] + ((double)ijk[d] + 0.5 * kHex27Half[v][d]) * L.h[d];
+            vertex_phi[(size_t)e * 27 + v] = L.ls(x);
+        }
+    } else {
+        for (int v = 0; v < L.nb; ++v) vertex_phi[(size_t)e * L.nb + v] = phi[cell_node(L, c, v)];
+    }
     if (L.iwg == MGPU_IWG_SPATIALDIFF_BULK) {
         material[e] = L.k_sigma == 0.0 ? exp(L.k_mu) : exp(L.k_mu + L.k_sigma * normal_at(L.seed, (uint64_t)c));
     } else {
@@ -291,6 +303,9 @@ __global__ void k_basis_adof(int64_t n, const int32_t* used, const int32_t* pos,
 // T rows and topology of element e.  p = 1: one entry of weight 1 per vertex (its node).  p = 2:
 // a vertex at lattice node m of an axis carries the splines m and m + 1 with 1/2 each; the row is
 // the tensor product, x fastest, weight the product of the axis weights (1/8 in 3D, 1/4 in 2D).
+// HEX27 cells (p = 2): a node at half-step coordinate g of an axis carries the splines m, m + 1 with
+// 1/2 each when g = 2 m, and m, m + 1, m + 2 with 1/8, 3/4, 1/8 when g = 2 m + 1 (the cell centre);
+// the 27 rows of an element hold (2 + 3 + 2)^3 = 343 entries.
 // Topology: the element's bases in ascending index per dof type, type f offset by f N.
 __global__ void k_trows(Lat L, int32_t n_el, const int32_t* elem_cell, const int32_t* basis_adof, int32_t N, int nf,
                         int32_t* t_ptr, int32_t* t_adof, double* t_weight, int32_t* topo) {
@@ -298,6 +313,36 @@ __global__ void k_trows(Lat L, int32_t n_el, const int32_t* elem_cell, const int
     if (e >= n_el) return;
     int ijk[3];
     cell_ijk(L, elem_cell[e], ijk);
+    if (L.nb_ip == 27) {
+        int64_t pos = (int64_t)e * kHex27RowEntries;
+        for (int v = 0; v < 27; ++v) {
+            t_ptr[(int64_t)e * 27 + v] = (int32_t)pos;
+            int na[3], sm[3];
+            double wa[3][3];
+            for (int d = 0; d < 3; ++d) {
+                const int g = 2 * ijk[d] + kHex27Half[v][d];
+                sm[d] = g / 2;
+                if ((g & 1) == 0) {
+                    na[d] = 2;
+                    wa[d][0] = wa[d][1] = 0.5;
+                    wa[d][2] = 0.0;
+                } else {
+                    na[d] = 3;
+                    wa[d][0] = 0.125;
+                    wa[d][1] = 0.75;
+                    wa[d][2] = 0.125;
+                }
+            }
+            for (int c = 0; c < na[2]; ++c)
+                for (int b = 0; b < na[1]; ++b)
+                    for (int a = 0; a < na[0]; ++a) {
+                        t_adof[pos] = basis_adof[basis_index(L, sm[0] + a, sm[1] + b, sm[2] + c)];
+                        t_weight[pos] = wa[0][a] * wa[1][b] * wa[2][c];
+                        ++pos;
+                    }
+        }
+        if (e == n_el - 1) t_ptr[(int64_t)n_el * 27] = (int32_t)((int64_t)n_el * kHex27RowEntries);
+    } else {
     const int per_row = L.order == 1 ? 1 : (L.dim == 3 ? 8 : 4);
     for (int v = 0; v < L.nb; ++v) {
         const int ox = L.dim == 3 ? kHex8Off[v][0] : kQuad4Off[v][0];
@@ -321,6 +366,7 @@ __global__ void k_trows(Lat L, int32_t n_el, const int32_t* elem_cell, const int
                 }
     }
     if (e == n_el - 1) t_ptr[(int64_t)n_el * L.nb] = (int32_t)((int64_t)n_el * L.nb * per_row);
+    }
     const int per = L.order + 1, pz = L.dim == 3 ? per : 1;
     const int na_t = per * per * pz;
     int32_t* out = topo + (int64_t)e * nf * na_t;
@@ -476,8 +522,10 @@ extern "C" mgpu_status mgpu_setup_build(const mgpu_problem_spec* spec, void* str
     if (spec->dim != 2 && spec->dim != 3) return set_error(MGPU_ERR_INVALID_ARG, "dim must be 2 or 3");
     if (spec->iwg != MGPU_IWG_SPATIALDIFF_BULK && spec->iwg != MGPU_IWG_STRUC_LINEAR_BULK)
         return set_error(MGPU_ERR_UNSUPPORTED, "iwg not supported");
-    if (spec->lagrange_order > 1 || spec->refine_levels != 0)
-        return set_error(MGPU_ERR_UNSUPPORTED, "the GPU set-up covers HEX8 / QUAD4 IP cells without refinement");
+    if (spec->lagrange_order > 2 || spec->refine_levels != 0)
+        return set_error(MGPU_ERR_UNSUPPORTED, "the GPU set-up covers HEX8 / QUAD4 and HEX27 IP cells without refinement");
+    if (spec->lagrange_order == 2 && (spec->dim != 3 || spec->bspline_order != 2))
+        return set_error(MGPU_ERR_UNSUPPORTED, "the GPU set-up takes HEX27 cells in 3D with bspline_order 2");
     if (spec->enrich != 0)
         return set_error(MGPU_ERR_UNSUPPORTED, "the GPU set-up numbers one element per cell (no enrichment)");
     const int order = spec->bspline_order <= 1 ? 1 : spec->bspline_order;
@@ -493,6 +541,7 @@ extern "C" mgpu_status mgpu_setup_build(const mgpu_problem_spec* spec, void* str
     su::Lat& L = S->L;
     L.dim = spec->dim;
     L.nb = L.dim == 3 ? 8 : 4;
+    L.nb_ip = spec->lagrange_order == 2 ? 27 : L.nb;
     L.order = order;
     L.phase = spec->material_phase;
     L.iwg = spec->iwg;
@@ -578,7 +627,7 @@ extern "C" mgpu_status mgpu_setup_build(const mgpu_problem_spec* spec, void* str
     if ((rc = dalloc(S.get(), (size_t)n_el * d2, &S->ip_box)) != MGPU_OK) return rc;
     if ((rc = dalloc(S.get(), (size_t)n_el * (nf == 1 ? 1 : 2), &S->material)) != MGPU_OK) return rc;
     if ((rc = dalloc(S.get(), (size_t)n_el + 1, &S->ig_ptr)) != MGPU_OK) return rc;
-    if ((rc = dalloc(S.get(), (size_t)n_el * L.nb, &S->vertex_phi)) != MGPU_OK) return rc;
+    if ((rc = dalloc(S.get(), (size_t)n_el * L.nb_ip, &S->vertex_phi)) != MGPU_OK) return rc;
     hipLaunchKernelGGL(su::k_elem_fill, dim3(blocks(L.n_cells)), dim3(su::kT), 0, st, L, phi, cls, cut_pos, n_sub,
                        is_elem, elem_pos, S->elem_cell, cell_elem, elem_nsub, elem_slot, S->ip_box, S->material,
                        S->vertex_phi);
@@ -614,13 +663,13 @@ extern "C" mgpu_status mgpu_setup_build(const mgpu_problem_spec* spec, void* str
     hipLaunchKernelGGL(su::k_basis_adof, dim3(blocks(n_basis)), dim3(su::kT), 0, st, n_basis, used, bpos, badof,
                        basis_of);
     const int per_row = order == 1 ? 1 : (L.dim == 3 ? 8 : 4);
-    S->n_t = (int64_t)n_el * L.nb * per_row;
+    S->n_t = L.nb_ip == 27 ? (int64_t)n_el * su::kHex27RowEntries : (int64_t)n_el * L.nb * per_row;
     const int per = order + 1;
     S->na_t = (int64_t)per * per * (L.dim == 3 ? per : 1);
     const int64_t w = 2 * per - 1;
     if (S->n_t >= INT32_MAX || (int64_t)nf * N * nf * w * w * (L.dim == 3 ? w : 1) >= INT32_MAX)
         return set_error(MGPU_ERR_INVALID_ARG, "T rows or pattern exceed 32-bit offsets");
-    if ((rc = dalloc(S.get(), (size_t)n_el * L.nb + 1, &S->t_ptr)) != MGPU_OK) return rc;
+    if ((rc = dalloc(S.get(), (size_t)n_el * L.nb_ip + 1, &S->t_ptr)) != MGPU_OK) return rc;
     if ((rc = dalloc(S.get(), (size_t)S->n_t, &S->t_adof)) != MGPU_OK) return rc;
     if ((rc = dalloc(S.get(), (size_t)S->n_t, &S->t_weight)) != MGPU_OK) return rc;
     if ((rc = dalloc(S.get(), (size_t)n_el * nf * S->na_t, &S->topo)) != MGPU_OK) return rc;
@@ -683,7 +732,7 @@ extern "C" mgpu_status mgpu_setup_counts(const mgpu_gpu_setup* s, int64_t c[8]) 
     c[4] = s->n_rows;
     c[5] = s->nnz;
     c[6] = s->n_t;
-    c[7] = s->L.nb;
+    c[7] = s->L.nb_ip;
     return MGPU_OK;
 }
 
@@ -710,8 +759,8 @@ extern "C" mgpu_status mgpu_setup_copy(const mgpu_gpu_setup* s, int32_t* elem_ce
     if ((rc = cp(ig_xi, s->ig_xi, (size_t)s->n_ig * (dim + 1) * dim * 8)) != MGPU_OK) return rc;
     if ((rc = cp(material, s->material, (size_t)s->n_el * nm * 8)) != MGPU_OK) return rc;
     if ((rc = cp(ip_box, s->ip_box, (size_t)s->n_el * 2 * dim * 8)) != MGPU_OK) return rc;
-    if ((rc = cp(vertex_phi, s->vertex_phi, (size_t)s->n_el * s->L.nb * 8)) != MGPU_OK) return rc;
-    if ((rc = cp(t_ptr, s->t_ptr, ((size_t)s->n_el * s->L.nb + 1) * 4)) != MGPU_OK) return rc;
+    if ((rc = cp(vertex_phi, s->vertex_phi, (size_t)s->n_el * s->L.nb_ip * 8)) != MGPU_OK) return rc;
+    if ((rc = cp(t_ptr, s->t_ptr, ((size_t)s->n_el * s->L.nb_ip + 1) * 4)) != MGPU_OK) return rc;
     if ((rc = cp(t_adof, s->t_adof, (size_t)s->n_t * 4)) != MGPU_OK) return rc;
     if ((rc = cp(t_weight, s->t_weight, (size_t)s->n_t * 8)) != MGPU_OK) return rc;
     if ((rc = cp(topo, s->topo, (size_t)s->n_el * s->nf * s->na_t * 4)) != MGPU_OK) return rc;
@@ -730,8 +779,8 @@ extern "C" mgpu_status mgpu_setup_desc(mgpu_gpu_setup* s, mgpu_set_desc* d) {
     s->h_ig_xi.resize((size_t)s->n_ig * (dim + 1) * dim);
     s->h_material.resize((size_t)s->n_el * nm);
     s->h_ip_box.resize((size_t)s->n_el * 2 * dim);
-    s->h_vertex_phi.resize((size_t)s->n_el * s->L.nb);
-    s->h_t_ptr.resize((size_t)s->n_el * s->L.nb + 1);
+    s->h_vertex_phi.resize((size_t)s->n_el * s->L.nb_ip);
+    s->h_t_ptr.resize((size_t)s->n_el * s->L.nb_ip + 1);
     s->h_t_adof.resize((size_t)s->n_t);
     s->h_t_weight.resize((size_t)s->n_t);
     mgpu_status rc = mgpu_setup_copy(s, s->h_elem_cell.data(), s->h_ig_ptr.data(), s->h_ig_xi.data(), s->h_material.data(),
@@ -741,10 +790,10 @@ extern "C" mgpu_status mgpu_setup_desc(mgpu_gpu_setup* s, mgpu_set_desc* d) {
     std::memset(d, 0, sizeof(*d));
     d->spatial_dim = dim;
     d->iwg = s->L.iwg;
-    d->ip_geometry = dim == 3 ? MGPU_GEOM_HEX8 : MGPU_GEOM_QUAD4;
+    d->ip_geometry = s->L.nb_ip == 27 ? MGPU_GEOM_HEX27 : (dim == 3 ? MGPU_GEOM_HEX8 : MGPU_GEOM_QUAD4);
     d->ig_geometry = dim == 3 ? MGPU_GEOM_TET4 : MGPU_GEOM_TRI3;
     d->n_elements = s->n_el;
-    d->n_bases = s->L.nb;
+    d->n_bases = s->L.nb_ip;
     d->n_adofs_per_type = s->N;
     d->ip_box = s->h_ip_box.data();
     d->material = s->h_material.data();

@@ -6,9 +6,10 @@ cl_HMR_T_Matrix_Base.cpp:306-457 (T rows) and cl_Sparse_Matrix_EpetraFECrs.cpp:1
 * Planes and spheres: every integer array (elements, sub-cell offsets, T rows, topology, CSR pattern)
   and every coordinate array (sub-cell vertices, boxes, vertex level set, T weights) bit-identical;
   material within 2 ulp (exp, log and cos of the per-element conductivity come from the device
-  math library); p = 1 and p = 2, 2D and 3D, phase 0 and 1.
+  math library); p = 1 and p = 2, 2D and 3D, phase 0 and 1, and HEX27 cells with p = 2 rows.
 * Gyroid (sin / cos from the device library): integer arrays identical, coordinates within 1e-13.
-* A Set made from the GPU set-up's descriptor assembles K and R bit-identical to the host one.
+* A Set made from the GPU set-up's descriptor assembles K and R bit-identical to the host one (HEX8
+  p = 1; HEX27 p = 2 with the subdivision and the moment-fitted rule).
 * A Set made directly from the set-up's device arrays (mgpu_set_create_from_setup) equals the set
   mgpu_set_create makes from host copies of the same arrays: K and R bit-identical (FP64, mixed,
   mixed + learned), pattern, topology and statistics equal, a side set on it equal.
@@ -52,6 +53,11 @@ CASES = {
                                                   seed=SEED, material_phase=0, bspline_order=2),
     "circle2_elast_p2": lambda: M.make_spec(2, (16, 16), iwg=M.IWG_STRUC_LINEAR_BULK, spheres=[[0.5, 1.0, 0.0, 0.3]],
                                             bspline_order=2),
+    # HEX27 IP cells with p = 2 rows at the 27 nodes (the quadratic configuration without refinement)
+    "sphere3_elast_hex27_p2": lambda: M.make_spec(3, (10, 9, 8), iwg=M.IWG_STRUC_LINEAR_BULK,
+                                                  spheres=[[0.45, 0.5, 0.55, 0.3]], bspline_order=2, lagrange_order=2),
+    "plane3_diff_hex27_p2": lambda: M.make_spec(3, (7, 7, 7), ls_kind=M.LS_PLANE, plane=[0.3, 0.5, 0.8, -0.6],
+                                                k_sigma=0.4, seed=SEED + 3, bspline_order=2, lagrange_order=2),
 }
 
 
@@ -157,6 +163,23 @@ def test_set_from_device_arrays_equals_host_set(case):
     assert _same(a1[0], a2[0]) and _same(a1[1], a2[1])
 
 
+def test_hex27_set_from_gpu_setup_assembles_identically():
+    """A HEX27 set (generic path) made from the GPU set-up's arrays equals the host set-up's set bit
+    for bit, with the subdivision rule and with the moment-fitted rule."""
+    spec, keep = CASES["sphere3_elast_hex27_p2"]()
+    P = M.Problem(spec, keep)
+    S = M.Set(P.desc, keepalive=P)
+    Gs = G.GpuSetup(spec, keep)
+    S2 = M.Set(Gs.desc, keepalive=Gs)
+    u = np.random.default_rng(SEED + 6).uniform(-1.0, 1.0, S.n_rows)
+    for quad in (M.QUAD_SUBDIVISION, M.QUAD_MOMENT_FIT):
+        S.options(M.PREC_FP64, quad)
+        S2.options(M.PREC_FP64, quad)
+        v1, r1 = S.compute_jacobian_and_residual(u)
+        v2, r2 = S2.compute_jacobian_and_residual(u)
+        assert _same(v1, v2) and _same(r1, r2), quad
+
+
 def test_set_from_device_arrays_needs_nodal_rows():
     spec, keep = CASES["sphere3_diff_p2"]()
     Gs = G.GpuSetup(spec, keep)
@@ -165,7 +188,8 @@ def test_set_from_device_arrays_needs_nodal_rows():
 
 
 def test_rejects_unsupported_specs():
-    for kw in (dict(lagrange_order=2, bspline_order=2), dict(refine_levels=1, bspline_order=2), dict(bspline_order=3)):
+    for kw in (dict(lagrange_order=2), dict(refine_levels=1, bspline_order=2), dict(bspline_order=3),
+               dict(enrich=1)):
         spec, keep = M.make_spec(3, (4, 4, 4), iwg=M.IWG_STRUC_LINEAR_BULK, spheres=[[0.5, 0.5, 0.5, 0.3]], **kw)
         with pytest.raises(M.MgpuError):
             G.GpuSetup(spec, keep)
