@@ -9,6 +9,9 @@ Lagrange moments over those points, which gives the subdivision's sum for every 
 * scatter mode (colour classes) to the same bars;
 * size-independent on a 16^3 gyroid HEX27 set and a 12^3-base THB set: within 1e-12 of the
   subdivision path on the GPU;
+* an element given as 750 sub-tets tiling its box (several staging batches of the weight fit): fit,
+  subdivision and oracle within 1e-12, and within 1e-8 of the uncut HEX_3x3x3 element (the TET_35
+  sum over the tiling differs from it by 9e-10 on M, computed on the CPU);
 * refused with MGPU_ERR_UNSUPPORTED on sets without HEX27 cells.
 """
 import numpy as np
@@ -105,3 +108,54 @@ def test_moment_fit_refused_without_hex27_cells():
         S = M.Set(P.desc, keepalive=P)
         with pytest.raises(M.MgpuError, match="HEX27"):
             S.options(M.PREC_FP64, M.QUAD_MOMENT_FIT)
+
+
+def _box_tets(m):
+    """The box [-1, 1]^3 as m^3 sub-cubes of 6 tets each (Kuhn split), every tet positively oriented
+    in the kernels' convention (rows X_0 - X_2, X_1 - X_2, X_3 - X_2): [m^3 6][4][3]."""
+    kuhn = [(0, 1, 3, 7), (0, 1, 5, 7), (0, 2, 3, 7), (0, 2, 6, 7), (0, 4, 5, 7), (0, 4, 6, 7)]
+    corner = np.array([[(v >> 0) & 1, (v >> 1) & 1, (v >> 2) & 1] for v in range(8)], float)
+    out = []
+    h = 2.0 / m
+    for k in range(m):
+        for j in range(m):
+            for i in range(m):
+                base = np.array([-1.0 + i * h, -1.0 + j * h, -1.0 + k * h])
+                for t in kuhn:
+                    X = base + h * corner[list(t)]
+                    J = np.stack([X[0] - X[2], X[1] - X[2], X[3] - X[2]])
+                    if np.linalg.det(J) < 0:
+                        X = X[[1, 0, 2, 3]]
+                    out.append(X)
+    return np.array(out)
+
+
+def test_moment_fit_element_with_many_sub_tets():
+    """An element whose box is given as 750 sub-tets (more than one staging batch of the weight fit)
+    next to uncut elements: moment fit, subdivision and the oracle agree."""
+    spec, keep = M.make_spec(3, (3, 2, 2), iwg=M.IWG_STRUC_LINEAR_BULK, bspline_order=2, lagrange_order=2)
+    P = M.Problem(spec, keep)
+    d = M.SetDesc()
+    M.ctypes.memmove(M.ctypes.byref(d), M.ctypes.byref(P.desc), M.ctypes.sizeof(M.SetDesc))
+    tets = _box_tets(5)
+    ig_ptr = np.zeros(P.n_elements + 1, np.int32)
+    ig_ptr[1:] = len(tets)                       # element 0 holds every sub-tet, the others none
+    ig_xi = np.ascontiguousarray(tets.reshape(-1))
+    d.ig_ptr = ig_ptr.ctypes.data_as(M.ctypes.POINTER(M.ctypes.c_int32))
+    d.ig_xi = ig_xi.ctypes.data_as(M.ctypes.POINTER(M.ctypes.c_double))
+    keepalive = (P, ig_ptr, ig_xi)
+    S = M.Set(d, keepalive=keepalive)
+    rp, ci = S.pattern()
+    u = np.random.default_rng(SEED + 23).uniform(-1.0, 1.0, S.n_rows)
+    vo, Ro, _ = O.assemble_desc(d, rp, ci, u)
+    v_sub, R_sub = S.compute_jacobian_and_residual(u)
+    S.options(M.PREC_FP64, M.QUAD_MOMENT_FIT)
+    v_fit, R_fit = S.compute_jacobian_and_residual(u)
+    print("750 sub-tets: subdivision vs oracle %.2e, fit vs oracle %.2e" % (rel(v_sub, vo), rel(v_fit, vo)), flush=True)
+    assert rel(v_sub, vo) <= 1e-12 and rel(R_sub, Ro) <= 1e-12
+    assert rel(v_fit, vo) <= 1e-12 and rel(R_fit, Ro) <= 1e-12
+    # the sub-tets tile the box: the cut element's K equals the uncut rule's up to the TET_35
+    # error on this degree-10 integrand, and exactly where the box is integrated by HEX_3x3x3
+    Su = M.Set(P.desc, keepalive=P)
+    v_box, _ = Su.compute_jacobian_and_residual(u)
+    assert rel(v_fit, v_box) <= 1e-8
