@@ -67,7 +67,11 @@ typedef int32_t mgpu_status;
                                       degree-4 Lagrange moments over those points; the same sum as the
                                       subdivision for integrands of degree <= 4 per axis (the box
                                       HEX27 stiffness), i.e. the same K up to rounding, from 125
-                                      points per cut cell                                            */
+                                      points per cut cell.  3D lattice (hex8) sets: the 27 HEX_3x3x3
+                                      points weighted by the degree-2 Lagrange moments over the
+                                      TET_11 points (exact for the trilinear stiffness), no model,
+                                      no fallback; the 27-point product runs as in
+                                      MGPU_QUAD_LEARNED (FP32 MFMA)                                  */
 
 /* Level-set kinds (GEN analytic geometries). */
 #define MGPU_LS_NONE     0   /* no geometry: every cell is uncut material                           */

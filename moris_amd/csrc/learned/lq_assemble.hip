@@ -137,6 +137,147 @@ __global__ __launch_bounds__(64) void k_lq_moments(DevView s, LqView lv) {
     }
 }
 
+// k_mfit8_weights (MGPU_QUAD_MOMENT_FIT on lattice sets): the same work list and sub-tet loop as
+// k_lq_moments, and each element's 27 weights at the HEX_3x3x3 points:
+//   W_g = sum_q w*_q L_i(xi_q) L_j(eta_q) L_k(zeta_q),   (i, j, k) the 1D nodes of point g,
+// L the quadratic Lagrange polynomials through the rule's 1D nodes -s, 0, s (s its literal).  A
+// polynomial of degree <= 2 in each axis equals its interpolant at the 27 points, so the rule
+// sum_g W_g f(xi_g) is the subdivision's TET_11 sum for every such f; the trilinear stiffness
+// dN_a/dx_i dN_b/dx_j is one.  The points accumulate the 27 monomial moments x^a y^b z^c (a, b,
+// c <= 2) as (w* x^a)(y^b z^c); per element they become the weights through the coefficients of
+// L_i = sum_a C_ia x^a: W_ijk = sum_abc C_ia C_jb C_kc m_abc.  No fit, no network, no fallback:
+// flag = 1 for every element.
+constexpr int kMf8 = 27;
+__global__ __launch_bounds__(64) void k_mfit8_weights(DevView s, LqView lv) {
+    const int lane = threadIdx.x;
+    int own[4], sub0[4], nsubq[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        own[k] = lv.mom_owner[12 * blockIdx.x + k];
+        sub0[k] = lv.mom_owner[12 * blockIdx.x + 4 + k];
+        nsubq[k] = lv.mom_owner[12 * blockIdx.x + 8 + k];
+    }
+    auto own_at = [&](int k) { return k == 0 ? own[0] : (k == 1 ? own[1] : (k == 2 ? own[2] : own[3])); };
+    const int quarter = lane >> 4;
+    const int c = own_at(quarter);
+    int q0 = 0, q1 = 3;
+#pragma unroll
+    for (int k = 1; k < 4; ++k)
+        if (k <= quarter && own[k] != own[k - 1]) q0 = k;
+#pragma unroll
+    for (int k = 2; k >= 0; --k)
+        if (k >= quarter && own[k] != own[k + 1]) q1 = k;
+    const int width = 16 * (q1 - q0 + 1);
+    // 1D nodes -s, 0, s: L_0 = x (x - s) / (2 s^2), L_1 = 1 - x^2 / s^2, L_2 = x (x + s) / (2 s^2)
+    const double sg = kHex3Xi[1][0];
+    double acc[kMf8];
+#pragma unroll
+    for (int k = 0; k < kMf8; ++k) acc[k] = 0.0;
+    const int ig0 = (quarter == 0 ? sub0[0] : (quarter == 1 ? sub0[1] : (quarter == 2 ? sub0[2] : sub0[3])));
+    const int nsub = c < 0 ? 0 : (quarter == 0 ? nsubq[0] : (quarter == 1 ? nsubq[1] : (quarter == 2 ? nsubq[2] : nsubq[3])));
+    for (int t = lane - 16 * q0; t < nsub; t += width) {
+        const double2* src2 = reinterpret_cast<const double2*>(s.ig_xi + (size_t)(ig0 + t) * 12);
+        double X[12];
+#pragma unroll
+        for (int h = 0; h < 6; ++h) {
+            const double2 v = src2[h];
+            X[2 * h] = v.x;
+            X[2 * h + 1] = v.y;
+        }
+        double J[3][3];
+        const int rr[3] = {0, 1, 3};
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int d = 0; d < 3; ++d) J[r][d] = X[rr[r] * 3 + d] - X[6 + d];
+        const double detJ = (J[0][0] * (J[1][1] * J[2][2] - J[2][1] * J[1][2]) -
+                             J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                             J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0])) / 6.0;
+        if (detJ < kDetJLowerLimit) continue;
+#pragma unroll
+        for (int q = 0; q < kTet11Points; ++q) {
+            double xip[3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+                xip[d] = X[6 + d] + kTet11Xi[q][0] * J[0][d] + kTet11Xi[q][1] * J[1][d] + kTet11Xi[q][2] * J[2][d];
+            const double wq = kTet11W[q] * detJ;
+            const double wx[3] = {wq, wq * xip[0], wq * (xip[0] * xip[0])};
+            const double yv[3] = {1.0, xip[1], xip[1] * xip[1]};
+            const double zv[3] = {1.0, xip[2], xip[2] * xip[2]};
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+                for (int b = 0; b < 3; ++b) {
+                    const double yz = yv[b] * zv[c];
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) acc[(c * 3 + b) * 3 + a] += wx[a] * yz;
+                }
+        }
+    }
+    // reduction through LDS in four chunks of 8 moments (the last padded), then per slot
+    __shared__ double red[8 * 65];
+    __shared__ double part[32 * 4];
+#pragma unroll
+    for (int ch = 0; ch < 4; ++ch) {
+        if (ch > 0) __syncthreads();
+#pragma unroll
+        for (int k = 0; k < 8; ++k) red[k * 65 + lane] = (8 * ch + k < kMf8) ? acc[8 * ch + k] : 0.0;
+        __syncthreads();
+        if (lane < 32) {
+            const int k = lane / 4, q = lane % 4;
+            double v = 0.0;
+#pragma unroll
+            for (int l = 0; l < 16; ++l) v += red[k * 65 + 16 * q + l];
+            part[(8 * ch + k) * 4 + q] = v;
+        }
+    }
+    __syncthreads();
+    // each slot's moments (its quarters summed in order), then its weights
+    __shared__ double mtot[4 * 32];
+    for (int t = lane; t < 4 * 32; t += 64) {
+        const int qs = t / 32, k = t % 32;
+        const int cs = own_at(qs);
+        double v = 0.0;
+        if (cs >= 0 && k < kMf8 && !(qs > 0 && own_at(qs - 1) == cs)) {
+#pragma unroll
+            for (int qq = 0; qq < 4; ++qq)
+                if (qq >= qs && own[qq] == cs) v += part[k * 4 + qq];
+        }
+        mtot[t] = v;
+    }
+    __syncthreads();
+    const double Cf[3][3] = {{0.0, -0.5 / sg, 0.5 / (sg * sg)}, {1.0, 0.0, -1.0 / (sg * sg)}, {0.0, 0.5 / sg, 0.5 / (sg * sg)}};
+    for (int t = lane; t < 4 * 32; t += 64) {
+        const int qs = t / 32, g = t % 32;
+        const int cs = own_at(qs);
+        if (g >= kMf8 || cs < 0 || (qs > 0 && own_at(qs - 1) == cs)) continue;
+        // point g of the HEX_3x3x3 table: its 1D node per axis (-s, 0, s -> 0, 1, 2)
+        int nd[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const double x = kHex3Xi[g][d];
+            nd[d] = x < -0.5 * sg ? 0 : (x > 0.5 * sg ? 2 : 1);
+        }
+        double ci[3], cj[3], ck[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            ci[a] = nd[0] == 0 ? Cf[0][a] : (nd[0] == 1 ? Cf[1][a] : Cf[2][a]);
+            cj[a] = nd[1] == 0 ? Cf[0][a] : (nd[1] == 1 ? Cf[1][a] : Cf[2][a]);
+            ck[a] = nd[2] == 0 ? Cf[0][a] : (nd[2] == 1 ? Cf[1][a] : Cf[2][a]);
+        }
+        const double* m = mtot + qs * 32;
+        double v = 0.0;
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int b = 0; b < 3; ++b)
+#pragma unroll
+                for (int a = 0; a < 3; ++a) v += (ci[a] * cj[b]) * ck[c] * m[(c * 3 + b) * 3 + a];
+        lv.w_fit[(size_t)cs * kLqPoints + g] = (float)v;
+        if (g == 0) lv.flag[cs] = 1;
+    }
+}
+
 typedef double doublex4_t __attribute__((ext_vector_type(4)));
 
 // The lane's fragments of the fit operators, loaded once per wavefront: V (A operand of the
@@ -384,6 +525,9 @@ void lq_launch_fit(const DevView& s, const LqView& lv, hipStream_t st) {
         hipLaunchKernelGGL(k_lq_moments, dim3(lv.n_mom_waves), dim3(64), 0, st, s, lv);
         hipLaunchKernelGGL(k_lq_project, dim3((lv.n_cut + 15) / 16), dim3(64), 0, st, lv);
     }
+}
+void lq_launch_mfit8(const DevView& s, const LqView& lv, hipStream_t st) {
+    if (lv.n_cut > 0) hipLaunchKernelGGL(k_mfit8_weights, dim3(lv.n_mom_waves), dim3(64), 0, st, s, lv);
 }
 void lq_launch_element(const DevView& s, const LqView& lv, int nf, bool f32_store, hipStream_t st) {
     if (lv.n_cut == 0) return;

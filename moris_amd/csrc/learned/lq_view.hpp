@@ -31,6 +31,9 @@ struct LqView {
 };
 
 void lq_launch_fit(const DevView& s, const LqView& lv, hipStream_t st);
+// MGPU_QUAD_MOMENT_FIT on lattice sets: the 27 weights from the sub-tets' Lagrange moments, every
+// element accepted
+void lq_launch_mfit8(const DevView& s, const LqView& lv, hipStream_t st);
 void lq_launch_element(const DevView& s, const LqView& lv, int nf, bool f32_store, hipStream_t st);
 void mlp_forward_launch(const mgpu_mlp* m, const float* x, int64_t n, float* y, hipStream_t st);
 // the same with the inputs formed from the cells' 8 nodal level-set values (vphi[n][8])

@@ -774,10 +774,15 @@ mgpu_status launch_assemble(mgpu_set* S, const double* u, double* vals, double* 
     const int dim = h.dim, nf = h.nf;
     if (ev) HIP_TRY(hipEventRecord(ev[0], S->stream));
     const bool learned = S->v.n_cut > 0 && S->quadrature == MGPU_QUAD_LEARNED;
+    const bool mfit = S->v.n_cut > 0 && S->quadrature == MGPU_QUAD_MOMENT_FIT;
     if (learned) {
         mlp_forward_phi_launch(S->mlp, S->lv.vphi, S->lv.n_cut, S->lv.w_pred, S->stream);
         if (ev) HIP_TRY(hipEventRecord(ev[1], S->stream));
         lq_launch_fit(S->v, S->lv, S->stream);
+        HIP_TRY(hipGetLastError());
+    } else if (mfit) {
+        if (ev) HIP_TRY(hipEventRecord(ev[1], S->stream));
+        lq_launch_mfit8(S->v, S->lv, S->stream);
         HIP_TRY(hipGetLastError());
     } else if (ev) {
         HIP_TRY(hipEventRecord(ev[1], S->stream));
@@ -787,6 +792,9 @@ mgpu_status launch_assemble(mgpu_set* S, const double* u, double* vals, double* 
         if (learned) {
             lq_launch_element(S->v, S->lv, nf, S->precision == MGPU_PREC_MIXED, S->stream);
             launch_subdivision(S, S->lv.flag);
+        } else if (mfit) {
+            // every element takes the fitted rule: no subdivision launch
+            lq_launch_element(S->v, S->lv, nf, S->precision == MGPU_PREC_MIXED, S->stream);
         } else {
             launch_subdivision(S, nullptr);
         }
@@ -816,17 +824,22 @@ mgpu_status launch_assemble(mgpu_set* S, const double* u, double* vals, double* 
 }  // namespace mgpu
 
 namespace {
-// Buffers and constant tables of the learned quadrature (lattice hex8 sets).
-mgpu_status lq_prepare(mgpu_set* S) {
-    if (S->lq_ready) return MGPU_OK;
+// Buffers and constant tables of the learned quadrature (lattice hex8 sets); with need_phi the
+// cut cells' nodal level-set values too (the network's inputs; the moment fit does not use them).
+mgpu_status lq_prepare(mgpu_set* S, bool need_phi) {
     const SetHost& h = S->h;
     const int nc = S->v.n_cut;
-    if (h.dim != 3) return set_error(MGPU_ERR_UNSUPPORTED, "learned quadrature is implemented for 3D hex8 sets");
+    const char* what = need_phi ? "learned quadrature" : "moment-fitted quadrature";
+    if (h.dim != 3) return set_error(MGPU_ERR_UNSUPPORTED, std::string(what) + " is implemented for 3D hex8 sets");
     if (h.generic)
-        return set_error(MGPU_ERR_UNSUPPORTED, "learned quadrature is implemented for lattice sets with nodal T rows");
-    if (nc > 0 && S->vphi_cut.empty())
+        return set_error(MGPU_ERR_UNSUPPORTED, std::string(what) + " is implemented for lattice sets with nodal T rows");
+    if (need_phi && nc > 0 && S->vphi_cut.empty())
         return set_error(MGPU_ERR_INVALID_ARG, "learned quadrature needs vertex_phi in the set descriptor");
     LqView& lv = S->lv;
+    mgpu_status rc;
+    if (need_phi && !lv.vphi && nc > 0)
+        if ((rc = upload(S, S->vphi_cut.data(), S->vphi_cut.size(), const_cast<double**>(&lv.vphi))) != MGPU_OK) return rc;
+    if (S->lq_ready) return MGPU_OK;
     lv.n_cut = nc;
     if (lv.tol == 0.0) lv.tol = 1e-6;
     if (lv.kappa == 0.0) lv.kappa = 4.0;
@@ -836,9 +849,7 @@ mgpu_status lq_prepare(mgpu_set* S) {
         S->allocs.push_back(*p);
         return MGPU_OK;
     };
-    mgpu_status rc;
     void* p;
-    if ((rc = upload(S, S->vphi_cut.data(), S->vphi_cut.size(), const_cast<double**>(&lv.vphi))) != MGPU_OK) return rc;
     if ((rc = alloc((size_t)nc * kLqPoints * sizeof(float), &p)) != MGPU_OK) return rc;
     lv.w_pred = static_cast<float*>(p);
     if ((rc = alloc((size_t)nc * kLqMomentsPad * sizeof(double), &p)) != MGPU_OK) return rc;
@@ -1070,8 +1081,13 @@ extern "C" mgpu_status mgpu_set_options(mgpu_set* set, int32_t precision, int32_
         return set_error(MGPU_ERR_INVALID_ARG, "bad quadrature");
     if (quadrature == MGPU_QUAD_LEARNED && !set->mlp)
         return set_error(MGPU_ERR_STATE, "learned quadrature needs a model (mgpu_set_quadrature_model)");
-    if (quadrature == MGPU_QUAD_MOMENT_FIT && !(set->h.generic && set->h.nb == 27))
-        return set_error(MGPU_ERR_UNSUPPORTED, "moment-fitted quadrature is implemented for HEX27 cells");
+    if (quadrature == MGPU_QUAD_MOMENT_FIT) {
+        if (set->h.generic && set->h.nb != 27)
+            return set_error(MGPU_ERR_UNSUPPORTED, "moment-fitted quadrature is implemented for HEX27 cells and 3D "
+                                                   "lattice sets");
+        if (!set->h.generic)
+            if (mgpu_status rc = lq_prepare(set, false); rc != MGPU_OK) return rc;
+    }
     set->precision = precision;
     set->quadrature = quadrature;
     return MGPU_OK;
@@ -1161,7 +1177,8 @@ extern "C" mgpu_status mgpu_set_assemble_host(mgpu_set* set, const double* u, do
 extern "C" mgpu_status mgpu_set_stats(const mgpu_set* set, int64_t stats[5]) {
     if (!set || !stats) return set_error(MGPU_ERR_INVALID_ARG, "null argument");
     for (int i = 0; i < 5; ++i) stats[i] = set->stats[i];
-    if (set->quadrature == MGPU_QUAD_LEARNED && set->v.n_cut > 0) {
+    if ((set->quadrature == MGPU_QUAD_LEARNED || (set->quadrature == MGPU_QUAD_MOMENT_FIT && !set->h.generic)) &&
+        set->v.n_cut > 0) {
         std::vector<int32_t> f((size_t)set->v.n_cut);
         HIP_TRY(hipStreamSynchronize(set->stream));
         HIP_TRY(hipMemcpy(f.data(), set->lv.flag, f.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
@@ -1198,7 +1215,7 @@ extern "C" mgpu_status mgpu_set_quadrature_model(mgpu_set* set, const mgpu_mlp* 
     if (!set || !mlp) return set_error(MGPU_ERR_INVALID_ARG, "null argument");
     if (!(tol > 0.0) || iters < 0 || !(kappa >= 1.0))
         return set_error(MGPU_ERR_INVALID_ARG, "bad tolerance / iteration count / cancellation bound");
-    mgpu_status rc = lq_prepare(set);
+    mgpu_status rc = lq_prepare(set, true);
     if (rc != MGPU_OK) return rc;
     set->lv.tol = tol;
     set->lv.iters = iters;

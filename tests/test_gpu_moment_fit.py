@@ -12,13 +12,17 @@ Lagrange moments over those points, which gives the subdivision's sum for every 
 * an element given as 750 sub-tets tiling its box (several staging batches of the weight fit): fit,
   subdivision and oracle within 1e-12, and within 1e-8 of the uncut HEX_3x3x3 element (the TET_35
   sum over the tiling differs from it by 9e-10 on M, computed on the CPU);
-* refused with MGPU_ERR_UNSUPPORTED on sets without HEX27 cells.
+* 3D lattice (hex8) sets: the 27 HEX_3x3x3 weights from the degree-2 Lagrange moments; K and R
+  against the oracle at 1e-6 (FP32 MFMA product), every cut element accepted, the weights summing
+  to each element's material volume; refused in 2D;
+* refused with MGPU_ERR_UNSUPPORTED on generic sets without HEX27 cells.
 """
 import numpy as np
 import pytest
 
 import moris_amd as M
 from helpers import SEED, rel, two_bodies
+from moris_amd import learned as L
 from oracle import py as O
 
 pytestmark = pytest.mark.gpu
@@ -101,8 +105,8 @@ def test_moment_fit_equals_subdivision_on_larger_sets(refine):
     assert rel(v_mix, v_sub) <= 1e-6
 
 
-def test_moment_fit_refused_without_hex27_cells():
-    for kw in ({}, {"bspline_order": 2}):
+def test_moment_fit_refused_on_generic_sets_without_hex27_cells():
+    for kw in ({"bspline_order": 2},):
         spec, keep = M.make_spec(3, (6, 6, 6), iwg=M.IWG_STRUC_LINEAR_BULK, spheres=[[0.5, 0.5, 0.5, 0.3]], **kw)
         P = M.Problem(spec, keep)
         S = M.Set(P.desc, keepalive=P)
@@ -159,3 +163,52 @@ def test_moment_fit_element_with_many_sub_tets():
     Su = M.Set(P.desc, keepalive=P)
     v_box, _ = Su.compute_jacobian_and_residual(u)
     assert rel(v_fit, v_box) <= 1e-8
+
+
+LATTICE = {
+    "sphere_16_elasticity": lambda: M.make_spec(3, (16, 16, 16), iwg=M.IWG_STRUC_LINEAR_BULK,
+                                                spheres=[[0.45, 0.5, 0.55, 0.31]]),
+    "spheres_12_diffusion": lambda: M.make_spec(3, (12, 12, 12), spheres=[[0.3, 0.3, 0.5, 0.2], [0.7, 0.6, 0.5, 0.22]],
+                                                k_sigma=0.5),
+}
+
+
+@pytest.mark.parametrize("case", sorted(LATTICE))
+def test_lattice_moment_fit_matches_oracle(case):
+    """3D lattice sets: the 27 weights from the degree-2 Lagrange moments reproduce the subdivision
+    for the trilinear stiffness; K and R against the oracle within the FP32 MFMA product's 1e-6,
+    every cut element on the fitted rule, the weights summing to the material's parametric volume."""
+    print("case", case, flush=True)
+    spec, keep = LATTICE[case]()
+    P = M.Problem(spec, keep)
+    S = M.Set(P.desc, keepalive=P)
+    Q = O.OracleProblem(spec)
+    s = Q.setup()
+    u = np.random.default_rng(SEED + 24).uniform(-1.0, 1.0, S.n_rows)
+    vo, Ro = Q.assemble(u)
+    for prec in (M.PREC_FP64, M.PREC_MIXED):
+        S.options(prec, M.QUAD_MOMENT_FIT)
+        vals, R = S.compute_jacobian_and_residual(u)
+        print(case, "precision", prec, "|dK|/|K| %.2e |dR|/|R| %.2e" % (rel(vals, vo), rel(R, Ro)), flush=True)
+        assert rel(vals, vo) <= 1e-6 and rel(R, Ro) <= 1e-6
+    st = S.statistics()
+    assert st[2] == P.n_cut and st[1] == 0          # every cut element accepted, none on the subdivision
+    _, w_fit, flags = L.result(S)
+    assert np.all(flags == 1)
+    a = P.arrays()
+    ig = a["ig_ptr"]
+    X = a["ig_xi"].reshape(-1, 4, 3)
+    J = np.stack([X[:, 0] - X[:, 2], X[:, 1] - X[:, 2], X[:, 3] - X[:, 2]], axis=1)
+    vol = np.abs(np.linalg.det(J)) / 6.0
+    csum = np.concatenate([[0.0], np.cumsum(vol)])
+    cut = np.flatnonzero(np.diff(ig) > 0)
+    v_el = csum[ig[cut + 1]] - csum[ig[cut]]
+    assert np.abs(w_fit.sum(axis=1) - v_el).max() <= 1e-5 * 8.0
+
+
+def test_lattice_moment_fit_refused_in_2d():
+    spec, keep = M.make_spec(2, (16, 16), iwg=M.IWG_STRUC_LINEAR_BULK, spheres=[[0.5, 0.5, 0.0, 0.3]])
+    P = M.Problem(spec, keep)
+    S = M.Set(P.desc, keepalive=P)
+    with pytest.raises(M.MgpuError, match="3D"):
+        S.options(M.PREC_MIXED, M.QUAD_MOMENT_FIT)

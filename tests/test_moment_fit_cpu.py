@@ -8,7 +8,9 @@ Pins:
   * on the oracle's TET_35 points of sliver-to-regular sub-tets of a cell: random polynomials of
     degree <= 4 per axis agree to 1e-14, a degree-5 monomial does not (the bound is sharp);
   * the HEX27 moment integrand dN_a/dxi_i dN_b/dxi_j (the oracle's shape functions) summed over the
-    fitted nodes equals its TET_35 sum to 1e-13 for every (a, i, b, j).
+    fitted nodes equals its TET_35 sum to 1e-13 for every (a, i, b, j);
+  * the lattice (hex8) variant: 3 nodes per axis (the HEX_3x3x3 rule's literal nodes), TET_11
+    points: degree <= 2 per axis reproduced, degree 3 not, and the trilinear moment matrix equal.
 """
 import numpy as np
 from numpy.polynomial import legendre
@@ -87,5 +89,49 @@ def test_hex27_moment_integrand_on_the_fitted_nodes():
     Aq = dNq.reshape(len(P), 81)
     Ag = dNg.reshape(len(G), 81)
     Mq = Aq.T @ (w[:, None] * Aq)
+    Mg = Ag.T @ (Wg[:, None] * Ag)
+    assert np.abs(Mg - Mq).max() <= 1e-13 * np.abs(Mq).max()
+
+
+# ---- 3D lattice sets: 3 nodes per axis (the HEX_3x3x3 rule's), TET_11 points ----------------------
+
+def _lagrange3(x, s):
+    x = np.asarray(x, float)
+    return np.stack([x * (x - s) / (2 * s * s), 1.0 - x * x / (s * s), x * (x + s) / (2 * s * s)], axis=1)
+
+
+def test_hex8_fit_on_hex3_nodes_reproduces_tensor_degree_2():
+    """With the HEX_3x3x3 rule's literal nodes s = 0.774596669241483, the weights W_g = sum_q w_q
+    L_i(x_q) L_j(y_q) L_k(z_q) over TET_11 points reproduce every polynomial of degree <= 2 per
+    axis, and hence the trilinear moment matrix sum w g g^T (the oracle's HEX8 shape functions);
+    degree 3 in one axis is not reproduced."""
+    x3, w3 = O.rule("HEX_3x3x3")
+    s = np.abs(x3[:, 0]).max()
+    xi11, w11 = O.rule("TET_11")
+    rng = np.random.default_rng(9)
+    P, W = [], []
+    for t in range(10):
+        V = rng.uniform(-1, 1, (4, 3))
+        J = np.stack([V[0] - V[2], V[1] - V[2], V[3] - V[2]])
+        det = abs(np.linalg.det(J)) / 6.0
+        bary = np.stack([xi11[:, 0], xi11[:, 1], 1.0 - xi11.sum(axis=1), xi11[:, 2]], axis=1)
+        P.append(bary @ V)
+        W.append(w11 * det)
+    P, W = np.concatenate(P), np.concatenate(W)
+    Lx, Ly, Lz = _lagrange3(P[:, 0], s), _lagrange3(P[:, 1], s), _lagrange3(P[:, 2], s)
+    Wt = np.einsum("q,qi,qj,qk->ijk", W, Lx, Ly, Lz)
+    # the rule's points in its own order: node index per axis (-s, 0, s -> 0, 1, 2)
+    idx = np.where(x3 < -0.5 * s, 0, np.where(x3 > 0.5 * s, 2, 1))
+    Wg = Wt[idx[:, 0], idx[:, 1], idx[:, 2]]
+    C = rng.uniform(-1, 1, (3, 3, 3))
+    f = lambda X: np.einsum("abc,qa,qb,qc->q", C, X[:, :1] ** np.arange(3), X[:, 1:2] ** np.arange(3),
+                            X[:, 2:3] ** np.arange(3))
+    assert abs(Wg @ f(x3) - W @ f(P)) <= 1e-14 * max(1.0, abs(W @ f(P))) * 10
+    g3 = lambda X: X[:, 0] ** 3 * X[:, 1]
+    assert abs(Wg @ g3(x3) - W @ g3(P)) > 1e-6 * abs(W @ g3(P))
+    _, dNq = O.shape(3, P)
+    _, dNg = O.shape(3, x3)
+    Aq, Ag = dNq.reshape(len(P), 24), dNg.reshape(len(x3), 24)
+    Mq = Aq.T @ (W[:, None] * Aq)
     Mg = Ag.T @ (Wg[:, None] * Ag)
     assert np.abs(Mg - Mq).max() <= 1e-13 * np.abs(Mq).max()
