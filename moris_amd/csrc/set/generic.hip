@@ -20,7 +20,8 @@
 //                       fit the 125 weights of a 5 x 5 x 5 Gauss-Legendre rule (FP64 MFMA), and M
 //                       comes from those weights by sum factorisation (the same M up to rounding).
 //                       In scatter mode (sets too large for a K_e pool) one launch per colour
-//                       class, each element adding its K_e into the CSR values after k_generic_zero.
+//                       class, each element adding its K_e into the CSR values after k_generic_zero
+//                       (FP64 atomics without return; one element per value within a colour).
 //   k_generic_gather    one thread per CSR value: the sum of its K_e entries in ascending element
 //                       order (deterministic SumIntoGlobalValues, cl_Sparse_Matrix_EpetraFECrs.cpp
 //                       :103-133).
@@ -723,7 +724,10 @@ __global__ __launch_bounds__(256) void k_generic_hex27(GenView g, int c0, double
     // blocks h >= f are computed, block (h, f) is their mirror image.  K_e is stored to the
     // element's slot of the pool, or (scatter mode) added into the CSR values: entry (x, y) of block
     // (f, h) sits in row topo[f na_t + x], field block h of that row, at the rank of adof y among the
-    // row adof's neighbours.  No other element of the colour touches the row.
+    // row adof's neighbours.  No other element of the colour touches the row, so the adds are
+    // hardware FP64 atomics without return (global_atomic_add_f64, done at the L2: no old value
+    // travels to the CU) that still reach every value in colour order: the same sums as a
+    // read-modify-write, deterministic.
     double* ke = SCATTER ? nullptr : g.ke + g.ke_off[e];
     const int32_t* rows = SCATTER ? g.topo + g.topo_ptr[e] : nullptr;
     const uint16_t* rk = SCATTER ? g.rank + g.rank_off[e] : nullptr;
@@ -751,27 +755,23 @@ __global__ __launch_bounds__(256) void k_generic_hex27(GenView g, int c0, double
             for (int tile = wave; tile < n_nt * n_nt; tile += 4) {
                 const int mt = tile / n_nt, nt = tile - n_nt * (tile / n_nt);
                 const int xrow = mt * 16 + col16, ycol = nt * 16 + col16;   // x; y
-                // scatter mode: the tile's CSR positions and old values (and those of its mirror
-                // image in block (h, f)) are requested before its MFMA work, which hides their
-                // chain of dependent loads
+                // scatter mode: the tile's CSR positions (and those of its mirror image in block
+                // (h, f)) are requested before its MFMA work, which hides their chain of dependent
+                // loads
                 [[maybe_unused]] int pos[4], pos2[4];
-                [[maybe_unused]] double old[4], old2[4];
                 if constexpr (SCATTER) {
 #pragma unroll
                     for (int j = 0; j < 4; ++j) {
                         const int x = mt * 16 + kq + 4 * j;
                         pos[j] = pos2[j] = -1;
-                        old[j] = old2[j] = 0.0;
                         if (x < na_t && ycol < na_t) {
                             const int32_t row = rows[f * na_t + x];
                             const int32_t r0 = g.row_ptr[row], len = (g.row_ptr[row + 1] - r0) / NF;
                             pos[j] = r0 + hb * len + rk[x * na_t + ycol];
-                            old[j] = vals[pos[j]];
                             if (hb != f) {
                                 const int32_t row2 = rows[hb * na_t + ycol];
                                 const int32_t r2 = g.row_ptr[row2], len2 = (g.row_ptr[row2 + 1] - r2) / NF;
                                 pos2[j] = r2 + f * len2 + rk[ycol * na_t + x];
-                                old2[j] = vals[pos2[j]];
                             }
                         }
                     }
@@ -788,8 +788,8 @@ __global__ __launch_bounds__(256) void k_generic_hex27(GenView g, int c0, double
                 for (int j = 0; j < 4; ++j) {
                     const int x = mt * 16 + kq + 4 * j;
                     if constexpr (SCATTER) {
-                        if (pos[j] >= 0) vals[pos[j]] = old[j] + c4[j];
-                        if (pos2[j] >= 0) vals[pos2[j]] = old2[j] + c4[j];
+                        if (pos[j] >= 0) unsafeAtomicAdd(vals + pos[j], c4[j]);
+                        if (pos2[j] >= 0) unsafeAtomicAdd(vals + pos2[j], c4[j]);
                     } else if (x < na_t && ycol < na_t) {
                         ke[(size_t)(f * na_t + x) * NA + hb * na_t + ycol] = c4[j];
                         if (hb != f) ke[(size_t)(hb * na_t + ycol) * NA + f * na_t + x] = c4[j];
