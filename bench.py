@@ -55,9 +55,10 @@ STENCIL_TRAFFIC_BYTES = (97555 * 2 + 445915) * 1024
 # The same for k_generic_hex27<3,double,false> on the 32^3 HEX27 gyroid set (gather mode):
 # 109,644 KiB fetched, 1,470,336 KiB written (the K_e pool).
 HEX27_32_TRAFFIC_BYTES = (109644 * 2 + 1470336) * 1024
-# The same for k_generic_hex27<3,double,true> on the 48^3-base THB gyroid set (scatter mode), summed
-# over the 42 colour launches of one call: 54,099,334 KiB fetched, 62,130,510 KiB written.
-THB48_SCATTER_TRAFFIC_BYTES = (54099334 * 2 + 62130510) * 1024
+# The same for k_generic_hex27<3,double,true,false> on the 48^3-base THB gyroid set (scatter mode,
+# FP64 atomic adds), summed over the 42 colour launches of one call: 4,423,474 KiB fetched,
+# 70,966,197 KiB written (the atomics count as write requests only).
+THB48_SCATTER_TRAFFIC_BYTES = (4423474 * 2 + 70966197) * 1024
 # The same for the FP64 k_stencil<3,1,ROWS,DIRECT,double> of the 128^3 diffusion set, per stencil
 # variant (the autotuner's choice): (FETCH_SIZE, WRITE_SIZE) KiB per launch.
 C2_STENCIL_TRAFFIC_KIB = {0: (100537, 428126), 1: (94277, 433217), 2: (68008, 426869),
@@ -236,7 +237,8 @@ def run_hex27(args):
     elif n == 48 and args.refine == 1 and mode == "scatter":
         out["roofline"]["traffic"] = THB48_SCATTER_TRAFFIC_BYTES
         out["roofline"]["traffic_source"] = ("rocprofv3 --pmc FETCH_SIZE (x2.0) + WRITE_SIZE (x1.0), summed over "
-                                             "the colour launches of one call")
+                                             "the colour launches of one call; the atomic adds count as write "
+                                             "requests only, so this is a lower bound")
     if not args.refine:
         # the set-up of the same set on the GPU (cut, numbering, T rows at the 27 nodes, pattern)
         from moris_amd import gpu_setup as G

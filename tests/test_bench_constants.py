@@ -52,7 +52,7 @@ def test_profiles_name_their_commands():
 
 def test_thb48_scatter_traffic_matches_its_profile():
     f, w = _row("k_generic_hex27_scatter_elasticity_p2_thb_gyroid_48_hbm_traffic.md",
-                "k_generic_hex27<3,double,true>, 42 launches (FP64)")
+                "k_generic_hex27<3,double,true,false>, 42 launches (FP64 subdivision)")
     assert bench.THB48_SCATTER_TRAFFIC_BYTES == (2 * f + w) * 1024
     # the zero fill writes the 949,141,152 FP64 values once (0.11 % over, measured), to within 0.5 %
     _, wz = _row("k_generic_hex27_scatter_elasticity_p2_thb_gyroid_48_hbm_traffic.md", "k_generic_zero")
