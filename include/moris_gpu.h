@@ -246,7 +246,9 @@ mgpu_status mgpu_element_topology_csr(const mgpu_set* set, const int64_t** offse
    stream: a kernel-side failure (a stencil row whose length disagrees with the pattern, whose
    values are then left unwritten) is reported by the next synchronising call on the set
    (mgpu_set_assemble_host, mgpu_set_time_assemble, mgpu_set_autotune, mgpu_batch_synchronize),
-   once; the flag is cleared when it is reported.                                               */
+   once; the flag is cleared when it is reported.  csr_vals must be ordinary device memory
+   (hipMalloc): HEX27 sets in scatter mode add into it with FP64 atomics, which fine-grained or
+   host-mapped memory does not take.                                                            */
 mgpu_status mgpu_set_assemble(mgpu_set* set, const double* u_adof, double* csr_vals, double* rhs);
 /* Same with HOST arrays; copies in and out, synchronous. */
 mgpu_status mgpu_set_assemble_host(mgpu_set* set, const double* u_adof, double* csr_vals, double* rhs);
