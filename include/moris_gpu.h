@@ -67,7 +67,9 @@ typedef int32_t mgpu_status;
                                       degree-4 Lagrange moments over those points; the same sum as the
                                       subdivision for integrands of degree <= 4 per axis (the box
                                       HEX27 stiffness), i.e. the same K up to rounding, from 125
-                                      points per cut cell.  3D lattice (hex8) sets: the 27 HEX_3x3x3
+                                      points per cut cell.  Generic HEX8 / QUAD4 cells: the 3^dim
+                                      tensor Gauss points, degree-2 Lagrange moments of the TET_11 /
+                                      TRI_7 points.  3D lattice (hex8) sets: the 27 HEX_3x3x3
                                       points weighted by the degree-2 Lagrange moments over the
                                       TET_11 points (exact for the trilinear stiffness), no model,
                                       no fallback; the 27-point product runs as in

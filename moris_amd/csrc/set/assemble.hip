@@ -1082,9 +1082,6 @@ extern "C" mgpu_status mgpu_set_options(mgpu_set* set, int32_t precision, int32_
     if (quadrature == MGPU_QUAD_LEARNED && !set->mlp)
         return set_error(MGPU_ERR_STATE, "learned quadrature needs a model (mgpu_set_quadrature_model)");
     if (quadrature == MGPU_QUAD_MOMENT_FIT) {
-        if (set->h.generic && set->h.nb != 27)
-            return set_error(MGPU_ERR_UNSUPPORTED, "moment-fitted quadrature is implemented for HEX27 cells and 3D "
-                                                   "lattice sets");
         if (!set->h.generic)
             if (mgpu_status rc = lq_prepare(set, false); rc != MGPU_OK) return rc;
     }

@@ -125,7 +125,81 @@ __device__ inline double generic_point(const GenView& g, int e, bool cut, int ip
 
 constexpr int kGenLds = 24 * 24 + 8 * kGenericMaxAdofsPerType + 24 * 3 * kGenericMaxAdofsPerType;
 
-template <int DIM, int NF, typename T>
+// Moment-fitted rule of a cut HEX8 / QUAD4 cell (MGPU_QUAD_MOMENT_FIT on the generic path): the
+// 3^DIM tensor points of the 3-point Gauss rule with weights W_g = sum_q w*_q L_i(x_q) L_j(y_q)
+// (L_k(z_q)) over the cell's TET_11 / TRI_7 points, L the quadratics through the nodes -s, 0, s.
+// The (bi)trilinear moment integrand has degree <= 2 per axis, so the rule gives the subdivision's
+// sum.  One wavefront: lanes over the points, then a reduction in lane order into Wg.
+constexpr int kGenFitW = kGenLds - 32;   // LDS offset of the fitted weights (doubles)
+template <int DIM>
+__device__ inline void gen_mfit_weights(const GenView& g, int e, const double* hh, double* lds, int lane) {
+    constexpr int NV = DIM + 1;
+    constexpr int NQ = DIM == 3 ? kTet11Points : kTri7Points;
+    constexpr int NW = DIM == 3 ? 27 : 9;
+    static_assert(NW * 65 <= kGenFitW, "reduction rows and the fitted weights fit the LDS");
+    const double sg = kHex3Xi[1][0];   // the 3-point Gauss node (the rule's literal)
+    const int npts = (g.ig_ptr[e + 1] - g.ig_ptr[e]) * NQ;
+    double acc[NW];
+#pragma unroll
+    for (int k = 0; k < NW; ++k) acc[k] = 0.0;
+    for (int ip = lane; ip < npts; ip += 64) {
+        const int sc = g.ig_ptr[e] + ip / NQ, q = ip % NQ;
+        const double* X = g.ig_xi + (size_t)sc * NV * DIM;
+        double N[NV];
+        if constexpr (DIM == 3) {
+            N[0] = kTet11Xi[q][0];
+            N[1] = kTet11Xi[q][1];
+            N[2] = 1.0 - kTet11Xi[q][0] - kTet11Xi[q][1] - kTet11Xi[q][2];
+            N[3] = kTet11Xi[q][2];
+        } else {
+            N[0] = kTri7Xi[q][0];
+            N[1] = kTri7Xi[q][1];
+            N[2] = 1.0 - kTri7Xi[q][0] - kTri7Xi[q][1];
+        }
+        double L[DIM][3];
+        for (int d = 0; d < DIM; ++d) {
+            double x = 0.0;
+            for (int k = 0; k < NV; ++k) x += N[k] * X[k * DIM + d];
+            L[d][0] = x * (x - sg) / (2.0 * sg * sg);
+            L[d][1] = 1.0 - x * x / (sg * sg);
+            L[d][2] = x * (x + sg) / (2.0 * sg * sg);
+        }
+        double detJ;
+        if constexpr (DIM == 3) {
+            double J[3][3];
+            const int rr[3] = {0, 1, 3};
+            for (int r = 0; r < 3; ++r)
+                for (int d = 0; d < 3; ++d) J[r][d] = (X[rr[r] * 3 + d] - X[2 * 3 + d]) * (0.5 * hh[d]);
+            detJ = (J[0][0] * (J[1][1] * J[2][2] - J[2][1] * J[1][2]) -
+                    J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                    J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0])) / 6.0;
+        } else {
+            double J[2][2];
+            for (int r = 0; r < 2; ++r)
+                for (int d = 0; d < 2; ++d) J[r][d] = (X[r * 2 + d] - X[2 * 2 + d]) * (0.5 * hh[d]);
+            detJ = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) / 2.0;
+        }
+        const double w = detJ < kDetJLowerLimit ? 0.0 : (DIM == 3 ? kTet11W[q] : kTri7W[q]) * detJ;
+#pragma unroll
+        for (int k = 0; k < NW; ++k) {
+            const int ix = k % 3, iy = (k / 3) % 3, iz = k / 9;
+            double v = w * L[0][ix] * L[1][iy];
+            if constexpr (DIM == 3) v *= L[2][iz];
+            acc[k] += v;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < NW; ++k) lds[k * 65 + lane] = acc[k];
+    __syncthreads();
+    if (lane < NW) {
+        double v = 0.0;
+        for (int l = 0; l < 64; ++l) v += lds[lane * 65 + l];
+        lds[kGenFitW + lane] = v;
+    }
+    __syncthreads();
+}
+
+template <int DIM, int NF, typename T, bool MFIT>
 __global__ __launch_bounds__(64) void k_generic_element(GenView g) {
     constexpr int NB = DIM == 3 ? 8 : 4;
     constexpr int NG = DIM * NB;
@@ -138,8 +212,12 @@ __global__ __launch_bounds__(64) void k_generic_element(GenView g) {
     double hh[DIM];
     for (int d = 0; d < DIM; ++d) hh[d] = g.ip_box[(size_t)e * 2 * DIM + DIM + d];
     const bool cut = g.ig_ptr[e + 1] > g.ig_ptr[e];
-    const int npts = cut ? (g.ig_ptr[e + 1] - g.ig_ptr[e]) * (DIM == 3 ? kTet11Points : kTri7Points)
-                         : (DIM == 3 ? kHex2Points : kQuad2Points);
+    const bool fit = MFIT && cut;
+    const int npts = fit ? (DIM == 3 ? 27 : 9)
+                         : (cut ? (g.ig_ptr[e + 1] - g.ig_ptr[e]) * (DIM == 3 ? kTet11Points : kTri7Points)
+                                : (DIM == 3 ? kHex2Points : kQuad2Points));
+    if constexpr (MFIT)
+        if (fit) gen_mfit_weights<DIM>(g, e, hh, lds, lane);
 
     // phase 1: moments, lane (a, b) owns M[(i,a),(j,b)], i, j < DIM
     T* sgw = reinterpret_cast<T*>(lds);
@@ -153,7 +231,21 @@ __global__ __launch_bounds__(64) void k_generic_element(GenView g) {
         const int ip = p0 + lane;
         if (ip < npts) {
             double grad[DIM][NB];
-            const double w = generic_point<DIM>(g, e, cut, ip, hh, grad);
+            double w;
+            if (fit) {
+                // tensor point ip of the 3-point Gauss rule, x fastest, and its fitted weight
+                const double nd[3] = {-kHex3Xi[1][0], 0.0, kHex3Xi[1][0]};
+                double xi[DIM];
+                xi[0] = nd[ip % 3];
+                xi[1] = nd[(ip / 3) % 3];
+                if constexpr (DIM == 3) xi[2] = nd[ip / 9];
+                lagrange_dNdxi<DIM>(xi, grad);
+                for (int i = 0; i < DIM; ++i)
+                    for (int c = 0; c < NB; ++c) grad[i][c] *= 2.0 / hh[i];
+                w = lds[kGenFitW + ip];
+            } else {
+                w = generic_point<DIM>(g, e, cut, ip, hh, grad);
+            }
             for (int i = 0; i < DIM; ++i)
                 for (int c = 0; c < NB; ++c) {
                     sgw[lane * GP + i * NB + c] = (T)(w * grad[i][c]);
@@ -894,6 +986,24 @@ void launch_hex27_q(mgpu_set* S, bool f32, dim3 gr, int c0, double* vals) {
     }
 }
 
+template <bool MFIT>
+void launch_generic_element(mgpu_set* S, bool f32, dim3 gr, dim3 bl) {
+    const SetHost& h = S->h;
+    if (h.dim == 3 && h.nf == 1) {
+        if (f32) hipLaunchKernelGGL((k_generic_element<3, 1, float, MFIT>), gr, bl, 0, S->stream, S->g);
+        else hipLaunchKernelGGL((k_generic_element<3, 1, double, MFIT>), gr, bl, 0, S->stream, S->g);
+    } else if (h.dim == 3) {
+        if (f32) hipLaunchKernelGGL((k_generic_element<3, 3, float, MFIT>), gr, bl, 0, S->stream, S->g);
+        else hipLaunchKernelGGL((k_generic_element<3, 3, double, MFIT>), gr, bl, 0, S->stream, S->g);
+    } else if (h.nf == 1) {
+        if (f32) hipLaunchKernelGGL((k_generic_element<2, 1, float, MFIT>), gr, bl, 0, S->stream, S->g);
+        else hipLaunchKernelGGL((k_generic_element<2, 1, double, MFIT>), gr, bl, 0, S->stream, S->g);
+    } else {
+        if (f32) hipLaunchKernelGGL((k_generic_element<2, 2, float, MFIT>), gr, bl, 0, S->stream, S->g);
+        else hipLaunchKernelGGL((k_generic_element<2, 2, double, MFIT>), gr, bl, 0, S->stream, S->g);
+    }
+}
+
 template <bool SCATTER>
 void launch_hex27(mgpu_set* S, bool f32, dim3 gr, int c0, double* vals) {
     if (S->quadrature == MGPU_QUAD_MOMENT_FIT) launch_hex27_q<SCATTER, true>(S, f32, gr, c0, vals);
@@ -919,18 +1029,10 @@ mgpu_status launch_generic(mgpu_set* S, const double* u, double* vals, double* r
             }
         } else if (h.nb == 27) {
             launch_hex27<false>(S, f32, gr, 0, vals);
-        } else if (h.dim == 3 && h.nf == 1) {
-            if (f32) hipLaunchKernelGGL((k_generic_element<3, 1, float>), gr, bl, 0, S->stream, S->g);
-            else hipLaunchKernelGGL((k_generic_element<3, 1, double>), gr, bl, 0, S->stream, S->g);
-        } else if (h.dim == 3) {
-            if (f32) hipLaunchKernelGGL((k_generic_element<3, 3, float>), gr, bl, 0, S->stream, S->g);
-            else hipLaunchKernelGGL((k_generic_element<3, 3, double>), gr, bl, 0, S->stream, S->g);
-        } else if (h.nf == 1) {
-            if (f32) hipLaunchKernelGGL((k_generic_element<2, 1, float>), gr, bl, 0, S->stream, S->g);
-            else hipLaunchKernelGGL((k_generic_element<2, 1, double>), gr, bl, 0, S->stream, S->g);
+        } else if (S->quadrature == MGPU_QUAD_MOMENT_FIT) {
+            launch_generic_element<true>(S, f32, gr, bl);
         } else {
-            if (f32) hipLaunchKernelGGL((k_generic_element<2, 2, float>), gr, bl, 0, S->stream, S->g);
-            else hipLaunchKernelGGL((k_generic_element<2, 2, double>), gr, bl, 0, S->stream, S->g);
+            launch_generic_element<false>(S, f32, gr, bl);
         }
         HIP_TRY(hipGetLastError());
     }

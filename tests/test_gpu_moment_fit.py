@@ -15,7 +15,8 @@ Lagrange moments over those points, which gives the subdivision's sum for every 
 * 3D lattice (hex8) sets: the 27 HEX_3x3x3 weights from the degree-2 Lagrange moments; K and R
   against the oracle at 1e-6 (FP32 MFMA product), every cut element accepted, the weights summing
   to each element's material volume; refused in 2D;
-* refused with MGPU_ERR_UNSUPPORTED on generic sets without HEX27 cells.
+* generic-path HEX8 / QUAD4 cells (p = 2 and THB rows): 3 nodes per axis, TET_11 / TRI_7 points,
+  against the oracle at 1e-12 (FP64) and 1e-6 (mixed).
 """
 import numpy as np
 import pytest
@@ -105,13 +106,29 @@ def test_moment_fit_equals_subdivision_on_larger_sets(refine):
     assert rel(v_mix, v_sub) <= 1e-6
 
 
-def test_moment_fit_refused_on_generic_sets_without_hex27_cells():
-    for kw in ({"bspline_order": 2},):
-        spec, keep = M.make_spec(3, (6, 6, 6), iwg=M.IWG_STRUC_LINEAR_BULK, spheres=[[0.5, 0.5, 0.5, 0.3]], **kw)
-        P = M.Problem(spec, keep)
-        S = M.Set(P.desc, keepalive=P)
-        with pytest.raises(M.MgpuError, match="HEX27"):
-            S.options(M.PREC_FP64, M.QUAD_MOMENT_FIT)
+GENERIC_LINEAR = {
+    "p2_sphere_8_hex8_elasticity": lambda: M.make_spec(3, (8, 8, 8), iwg=M.IWG_STRUC_LINEAR_BULK,
+                                                       spheres=[[0.5, 0.5, 0.5, 0.3]], bspline_order=2),
+    "p2_sphere_10_hex8_diffusion": lambda: M.make_spec(3, (10, 10, 10), spheres=[[0.45, 0.5, 0.55, 0.33]],
+                                                       k_sigma=0.5, bspline_order=2),
+    "p2_circle_16x12_quad4_plane_stress": lambda: M.make_spec(2, (16, 12), iwg=M.IWG_STRUC_LINEAR_BULK,
+                                                              spheres=[[0.5, 0.5, 0.0, 0.3]], bspline_order=2),
+    "thb_sphere_6_hex8_elasticity": lambda: M.make_spec(3, (6, 6, 6), iwg=M.IWG_STRUC_LINEAR_BULK,
+                                                        spheres=[[0.5, 0.5, 0.5, 0.3]], bspline_order=2,
+                                                        refine_levels=1),
+}
+
+
+@pytest.mark.parametrize("case", sorted(GENERIC_LINEAR))
+def test_generic_hex8_quad4_moment_fit_matches_oracle(case):
+    """Generic-path HEX8 / QUAD4 cells (p = 2 and THB rows): the 3^dim tensor Gauss points with the
+    degree-2 Lagrange moments of the TET_11 / TRI_7 points as weights; K and R against the oracle at
+    1e-12 (FP64) and 1e-6 (mixed)."""
+    print("case", case, flush=True)
+    spec, keep = GENERIC_LINEAR[case]()
+    P = M.Problem(spec, keep)
+    S = M.Set(P.desc, keepalive=P)
+    _check_against_oracle(S, spec)
 
 
 def _box_tets(m):
