@@ -251,9 +251,11 @@ def test_hex27_scatter_mode_matches_oracle_and_gather(case, monkeypatch):
     assert rel(vm, vo) <= 1e-6 and rel(Rm, Ro) <= 1e-6, (rel(vm, vo), rel(Rm, Ro))
 
 
-def test_hex27_scatter_mode_graph_replays(monkeypatch):
+@pytest.mark.parametrize("quad", [M.QUAD_SUBDIVISION, M.QUAD_MOMENT_FIT])
+def test_hex27_scatter_mode_graph_replays(monkeypatch, quad):
     """Graph replays of scatter-mode sets zero the values themselves (a fill kernel, no memset
-    node) and are bit-identical to eager calls, whatever the buffers held before."""
+    node) and are bit-identical to eager calls, whatever the buffers held before; with the
+    subdivision and with the moment-fitted rule."""
     monkeypatch.setenv("MGPU_GENERIC_SCATTER", "1")
     sets, bufs, eager = [], [], []
     for k, n in enumerate((5, 6)):
@@ -262,6 +264,7 @@ def test_hex27_scatter_mode_graph_replays(monkeypatch):
                                  refine_levels=k)
         P = M.Problem(spec, keep)
         S = M.Set(P.desc, keepalive=P)
+        S.options(M.PREC_FP64, quad)
         u = np.random.default_rng(k).uniform(-1.0, 1.0, S.n_rows)
         ub, vb, rb = M.DeviceBuffer.from_array(u), M.DeviceBuffer(8 * S.nnz), M.DeviceBuffer(8 * S.n_rows)
         S.assemble_device(ub, vb, rb)
