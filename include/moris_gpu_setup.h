@@ -41,7 +41,7 @@ mgpu_status mgpu_setup_time(const mgpu_gpu_setup* s, double* ms);
 /* Host copies of the device arrays (any pointer may be NULL; sizes from mgpu_setup_counts):
    elem_cell[n_el], ig_ptr[n_el + 1], ig_xi[n_ig][dim + 1][dim], material[n_el][1 | 2],
    ip_box[n_el][2 dim], vertex_phi[n_el][nb], t_ptr[n_el nb + 1], t_adof[n_t], t_weight[n_t],
-   topo[n_el][nf na] (na = (p + 1)^dim bases per dof type, ascending; type f offset by f N),
+   topo[n_el][nf na] (na = (p + 1)^dim bases per dof type, 27 on HEX27 cells, ascending; type f offset by f N),
    row_ptr[rows + 1], col_idx[nnz].  The descriptor (host pointers into copies owned by the
    handle, valid until destroy) feeds mgpu_set_create.                                          */
 mgpu_status mgpu_setup_copy(const mgpu_gpu_setup* s, int32_t* elem_cell, int32_t* ig_ptr, double* ig_xi,
@@ -49,7 +49,7 @@ mgpu_status mgpu_setup_copy(const mgpu_gpu_setup* s, int32_t* elem_cell, int32_t
                             double* t_weight, int32_t* topo, int32_t* row_ptr, int32_t* col_idx);
 mgpu_status mgpu_setup_desc(mgpu_gpu_setup* s, mgpu_set_desc* desc);
 /* A set made directly from the set-up's device arrays, with no host round trip: the same set as
-   mgpu_set_create(mgpu_setup_desc(...)) for bspline_order 1 (nodal T rows; others return
+   mgpu_set_create(mgpu_setup_desc(...)) for bspline_order 1 on HEX8 / QUAD4 cells (nodal T rows; others return
    MGPU_ERR_UNSUPPORTED).  The set uses the set-up's device arrays in place, so the set-up must
    outlive the set.  Its host copies of the pattern and topology (mgpu_pattern_get,
    mgpu_element_topology, side sets) are fetched from the device on first use.  No counterpart in

@@ -58,6 +58,9 @@ constexpr int kHex27RowEntries = 343;   // T entries of the 27 rows of a HEX27 e
 struct Lat {
     int dim, nb, order, phase, iwg;
     int nb_ip;   // IP-cell bases: nb, or 27 for HEX27 cells
+    // bases of a cell per axis (p + 1, or 3 for nodal HEX27 rows) and the basis-index step of a
+    // cell (1; 2 for nodal HEX27 rows, whose bases are the nodes of the half-step grid)
+    int bper, bstride;
     int n[3];
     double lo[3], h[3];
     int64_t n_cells, n_nodes, n_faces;
@@ -287,10 +290,10 @@ __global__ void k_basis_used(Lat L, int32_t n_el, const int32_t* elem_cell, int3
     if (e >= n_el) return;
     int ijk[3];
     cell_ijk(L, elem_cell[e], ijk);
-    const int per = L.order + 1, pz = L.dim == 3 ? per : 1;
+    const int per = L.bper, pz = L.dim == 3 ? per : 1, st = L.bstride;
     for (int c = 0; c < pz; ++c)
         for (int b = 0; b < per; ++b)
-            for (int a = 0; a < per; ++a) used[basis_index(L, ijk[0] + a, ijk[1] + b, ijk[2] + c)] = 1;
+            for (int a = 0; a < per; ++a) used[basis_index(L, st * ijk[0] + a, st * ijk[1] + b, st * ijk[2] + c)] = 1;
 }
 
 __global__ void k_basis_adof(int64_t n, const int32_t* used, const int32_t* pos, int32_t* adof, int32_t* basis_of) {
@@ -313,7 +316,17 @@ __global__ void k_trows(Lat L, int32_t n_el, const int32_t* elem_cell, const int
     if (e >= n_el) return;
     int ijk[3];
     cell_ijk(L, elem_cell[e], ijk);
-    if (L.nb_ip == 27) {
+    if (L.nb_ip == 27 && L.order == 1) {
+        // nodal HEX27 rows: node v of the element is the half-step grid node 2 ijk + kHex27Half[v]
+        for (int v = 0; v < 27; ++v) {
+            const int64_t r = (int64_t)e * 27 + v;
+            t_ptr[r] = (int32_t)r;
+            t_adof[r] = basis_adof[basis_index(L, 2 * ijk[0] + kHex27Half[v][0], 2 * ijk[1] + kHex27Half[v][1],
+                                               2 * ijk[2] + kHex27Half[v][2])];
+            t_weight[r] = 1.0;
+        }
+        if (e == n_el - 1) t_ptr[(int64_t)n_el * 27] = (int32_t)((int64_t)n_el * 27);
+    } else if (L.nb_ip == 27) {
         int64_t pos = (int64_t)e * kHex27RowEntries;
         for (int v = 0; v < 27; ++v) {
             t_ptr[(int64_t)e * 27 + v] = (int32_t)pos;
@@ -367,22 +380,23 @@ __global__ void k_trows(Lat L, int32_t n_el, const int32_t* elem_cell, const int
     }
     if (e == n_el - 1) t_ptr[(int64_t)n_el * L.nb] = (int32_t)((int64_t)n_el * L.nb * per_row);
     }
-    const int per = L.order + 1, pz = L.dim == 3 ? per : 1;
+    const int per = L.bper, pz = L.dim == 3 ? per : 1, st = L.bstride;
     const int na_t = per * per * pz;
     int32_t* out = topo + (int64_t)e * nf * na_t;
     int q = 0;
     for (int c = 0; c < pz; ++c)
         for (int b = 0; b < per; ++b)
             for (int a = 0; a < per; ++a, ++q) {
-                const int32_t id = basis_adof[basis_index(L, ijk[0] + a, ijk[1] + b, ijk[2] + c)];
+                const int32_t id = basis_adof[basis_index(L, st * ijk[0] + a, st * ijk[1] + b, st * ijk[2] + c)];
                 for (int f = 0; f < nf; ++f) out[f * na_t + q] = id + f * N;
             }
 }
 
 // Row of basis s (every dof type): the bases sharing a material element with s, i.e. the union of
-// the topologies of the elements around s.  The cells around s are s - per + 1 .. s per axis and
-// their bases lie in the window s - per + 1 .. s + per - 1 (w = 2 per - 1 per axis, w^dim <= 125
-// bits).  COUNT: row lengths; else the columns, ascending: type g's window bases offset by g N.
+// the topologies of the elements around s.  Cell i carries the bases st i .. st i + per - 1 (st the
+// basis step of a cell), so the cells around s are those with st i in s - per + 1 .. s, and their
+// bases lie in the window s - per + 1 .. s + per - 1 (w = 2 per - 1 per axis, w^dim <= 125 bits).
+// COUNT: row lengths; else the columns, ascending: type g's window bases offset by g N.
 template <bool COUNT>
 __global__ void k_rows(Lat L, const int32_t* basis_of, int32_t N, const int32_t* cell_elem, const int32_t* basis_adof,
                        int nf, int32_t* row_len, const int32_t* row_ptr, int32_t* col_idx) {
@@ -391,14 +405,17 @@ __global__ void k_rows(Lat L, const int32_t* basis_of, int32_t N, const int32_t*
     const int64_t s = basis_of[a];
     const int si[3] = {(int)(s % L.nbs[0]), (int)((s / L.nbs[0]) % L.nbs[1]),
                        L.dim == 3 ? (int)(s / ((int64_t)L.nbs[0] * L.nbs[1])) : 0};
-    const int per = L.order + 1, pz = L.dim == 3 ? per : 1;
+    const int per = L.bper, pz = L.dim == 3 ? per : 1, st = L.bstride;
     const int w = 2 * per - 1, wz = L.dim == 3 ? w : 1;
     uint32_t mask[4] = {0u, 0u, 0u, 0u};
     for (int dz = 0; dz < pz; ++dz)
         for (int dy = 0; dy < per; ++dy)
             for (int dx = 0; dx < per; ++dx) {
-                const int ci = si[0] - dx, cj = si[1] - dy, ck = si[2] - dz;
-                if (ci < 0 || cj < 0 || ck < 0 || ci >= L.n[0] || cj >= L.n[1] || ck >= L.n[2]) continue;
+                // the cell whose first basis is s - d per axis (when a cell starts there)
+                const int bx = si[0] - dx, by = si[1] - dy, bz = si[2] - dz;
+                if (bx < 0 || by < 0 || bz < 0 || bx % st || by % st || bz % st) continue;
+                const int ci = bx / st, cj = by / st, ck = bz / st;
+                if (ci >= L.n[0] || cj >= L.n[1] || ck >= L.n[2]) continue;
                 if (cell_elem[ci + (int64_t)L.n[0] * (cj + (int64_t)L.n[1] * ck)] < 0) continue;
                 // the cell's bases ci + q .. in window coordinates (origin s - per + 1)
                 for (int c = 0; c < pz; ++c)
@@ -524,8 +541,8 @@ extern "C" mgpu_status mgpu_setup_build(const mgpu_problem_spec* spec, void* str
         return set_error(MGPU_ERR_UNSUPPORTED, "iwg not supported");
     if (spec->lagrange_order > 2 || spec->refine_levels != 0)
         return set_error(MGPU_ERR_UNSUPPORTED, "the GPU set-up covers HEX8 / QUAD4 and HEX27 IP cells without refinement");
-    if (spec->lagrange_order == 2 && (spec->dim != 3 || spec->bspline_order != 2))
-        return set_error(MGPU_ERR_UNSUPPORTED, "the GPU set-up takes HEX27 cells in 3D with bspline_order 2");
+    if (spec->lagrange_order == 2 && spec->dim != 3)
+        return set_error(MGPU_ERR_UNSUPPORTED, "the GPU set-up takes HEX27 cells in 3D");
     if (spec->enrich != 0)
         return set_error(MGPU_ERR_UNSUPPORTED, "the GPU set-up numbers one element per cell (no enrichment)");
     const int order = spec->bspline_order <= 1 ? 1 : spec->bspline_order;
@@ -543,13 +560,16 @@ extern "C" mgpu_status mgpu_setup_build(const mgpu_problem_spec* spec, void* str
     L.nb = L.dim == 3 ? 8 : 4;
     L.nb_ip = spec->lagrange_order == 2 ? 27 : L.nb;
     L.order = order;
+    const bool nodal27 = L.nb_ip == 27 && order == 1;
+    L.bper = nodal27 ? 3 : order + 1;
+    L.bstride = nodal27 ? 2 : 1;
     L.phase = spec->material_phase;
     L.iwg = spec->iwg;
     for (int d = 0; d < 3; ++d) {
         L.n[d] = d < L.dim ? spec->n[d] : 1;
         L.lo[d] = d < L.dim ? spec->lo[d] : 0.0;
         L.h[d] = d < L.dim ? (spec->hi[d] - spec->lo[d]) / (double)spec->n[d] : 1.0;
-        L.nbs[d] = d < L.dim ? L.n[d] + (order == 1 ? 1 : 2) : 1;
+        L.nbs[d] = d < L.dim ? (nodal27 ? 2 * L.n[d] + 1 : L.n[d] + (order == 1 ? 1 : 2)) : 1;
     }
     L.n_cells = (int64_t)L.n[0] * L.n[1] * L.n[2];
     L.n_nodes = L.dim == 3 ? (int64_t)(L.n[0] + 1) * (L.n[1] + 1) * (L.n[2] + 1) : (int64_t)(L.n[0] + 1) * (L.n[1] + 1);
@@ -663,8 +683,9 @@ extern "C" mgpu_status mgpu_setup_build(const mgpu_problem_spec* spec, void* str
     hipLaunchKernelGGL(su::k_basis_adof, dim3(blocks(n_basis)), dim3(su::kT), 0, st, n_basis, used, bpos, badof,
                        basis_of);
     const int per_row = order == 1 ? 1 : (L.dim == 3 ? 8 : 4);
-    S->n_t = L.nb_ip == 27 ? (int64_t)n_el * su::kHex27RowEntries : (int64_t)n_el * L.nb * per_row;
-    const int per = order + 1;
+    S->n_t = nodal27 ? (int64_t)n_el * 27
+                     : (L.nb_ip == 27 ? (int64_t)n_el * su::kHex27RowEntries : (int64_t)n_el * L.nb * per_row);
+    const int per = L.bper;
     S->na_t = (int64_t)per * per * (L.dim == 3 ? per : 1);
     const int64_t w = 2 * per - 1;
     if (S->n_t >= INT32_MAX || (int64_t)nf * N * nf * w * w * (L.dim == 3 ? w : 1) >= INT32_MAX)
@@ -831,9 +852,9 @@ extern "C" mgpu_status mgpu_set_create_from_setup(mgpu_gpu_setup* G, mgpu_set** 
     MGPU_GUARD_BEGIN
     if (!G || !out) return set_error(MGPU_ERR_INVALID_ARG, "null argument");
     *out = nullptr;
-    if (G->L.order != 1)
-        return set_error(MGPU_ERR_UNSUPPORTED, "a set made from the GPU set-up needs nodal T rows (bspline_order 1); "
-                                               "use mgpu_setup_desc and mgpu_set_create");
+    if (G->L.order != 1 || G->L.nb_ip != G->L.nb)
+        return set_error(MGPU_ERR_UNSUPPORTED, "a set made from the GPU set-up needs HEX8 / QUAD4 cells with nodal T "
+                                               "rows (bspline_order 1); use mgpu_setup_desc and mgpu_set_create");
     const su::Lat& L = G->L;
     auto S = std::make_unique<mgpu_set>();
     SetHost& h = S->h;

@@ -84,7 +84,9 @@ class GpuSetup:
     def _copy(self):
         ne, dim, nb, nf = self.n_elements, self.dim, self.n_bases, self.n_fields
         order = 2 if self.spec.bspline_order == 2 else 1
-        na = (order + 1) ** dim
+        # adofs per type and element: (p + 1)^dim splines or nodes, 27 on HEX27 cells (p = 2 splines
+        # or the 27 nodes)
+        na = 27 if nb == 27 else (order + 1) ** dim
         out = {
             "elem_cell": np.zeros(ne, np.int32), "ig_ptr": np.zeros(ne + 1, np.int32),
             "ig_xi": np.zeros(self.n_ig * (dim + 1) * dim), "material": np.zeros(ne * (2 if nf > 1 else 1)),

@@ -67,18 +67,18 @@ def test_create_rejects_bad_material_before_touching_a_device():
 
 
 def test_gpu_setup_validates_the_spec_before_touching_a_device():
-    """mgpu_setup_build refuses nodal HEX27 rows, THB, p > 2 and enrichment with MGPU_ERR_UNSUPPORTED and a
-    null spec with MGPU_ERR_INVALID_ARG before it looks for a device; a valid spec (HEX8 or HEX27 with
-    p = 2) without a device returns MGPU_ERR_NO_DEVICE (no CPU fallback)."""
+    """mgpu_setup_build refuses THB, p > 2, Lagrange order > 2 and enrichment with MGPU_ERR_UNSUPPORTED and
+    a null spec with MGPU_ERR_INVALID_ARG before it looks for a device; a valid spec (HEX8, HEX27 with
+    p = 2 or nodal rows) without a device returns MGPU_ERR_NO_DEVICE (no CPU fallback)."""
     from moris_amd import gpu_setup as G
     lib = G._lib()
     out = ctypes.c_void_p()
     assert lib.mgpu_setup_build(None, None, ctypes.byref(out)) == 1
-    for kw in (dict(lagrange_order=2), dict(lagrange_order=2, bspline_order=2, refine_levels=1),
+    for kw in (dict(lagrange_order=3), dict(lagrange_order=2, bspline_order=2, refine_levels=1),
                dict(refine_levels=1, bspline_order=2), dict(bspline_order=3), dict(enrich=1)):
         spec, keep = M.make_spec(3, (4, 4, 4), iwg=M.IWG_STRUC_LINEAR_BULK, spheres=[[0.5, 0.5, 0.5, 0.3]], **kw)
         assert lib.mgpu_setup_build(ctypes.byref(spec), None, ctypes.byref(out)) == 3, kw
     if M.device_count() == 0:
-        for kw in ({}, dict(lagrange_order=2, bspline_order=2)):
+        for kw in ({}, dict(lagrange_order=2, bspline_order=2), dict(lagrange_order=2)):
             spec, keep = M.make_spec(3, (4, 4, 4), spheres=[[0.5, 0.5, 0.5, 0.3]], **kw)
             assert lib.mgpu_setup_build(ctypes.byref(spec), None, ctypes.byref(out)) == 4, kw

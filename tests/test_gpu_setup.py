@@ -6,7 +6,7 @@ cl_HMR_T_Matrix_Base.cpp:306-457 (T rows) and cl_Sparse_Matrix_EpetraFECrs.cpp:1
 * Planes and spheres: every integer array (elements, sub-cell offsets, T rows, topology, CSR pattern)
   and every coordinate array (sub-cell vertices, boxes, vertex level set, T weights) bit-identical;
   material within 2 ulp (exp, log and cos of the per-element conductivity come from the device
-  math library); p = 1 and p = 2, 2D and 3D, phase 0 and 1, and HEX27 cells with p = 2 rows.
+  math library); p = 1 and p = 2, 2D and 3D, phase 0 and 1, and HEX27 cells with p = 2 or nodal rows.
 * Gyroid (sin / cos from the device library): integer arrays identical, coordinates within 1e-13.
 * A Set made from the GPU set-up's descriptor assembles K and R bit-identical to the host one (HEX8
   p = 1; HEX27 p = 2 with the subdivision and the moment-fitted rule).
@@ -58,6 +58,9 @@ CASES = {
                                                   spheres=[[0.45, 0.5, 0.55, 0.3]], bspline_order=2, lagrange_order=2),
     "plane3_diff_hex27_p2": lambda: M.make_spec(3, (7, 7, 7), ls_kind=M.LS_PLANE, plane=[0.3, 0.5, 0.8, -0.6],
                                                 k_sigma=0.4, seed=SEED + 3, bspline_order=2, lagrange_order=2),
+    # HEX27 IP cells with nodal rows (adofs on the half-step node grid)
+    "sphere3_elast_hex27_nodal": lambda: M.make_spec(3, (7, 6, 5), iwg=M.IWG_STRUC_LINEAR_BULK,
+                                                     spheres=[[0.45, 0.5, 0.55, 0.32]], lagrange_order=2),
 }
 
 
@@ -188,7 +191,7 @@ def test_set_from_device_arrays_needs_nodal_rows():
 
 
 def test_rejects_unsupported_specs():
-    for kw in (dict(lagrange_order=2), dict(refine_levels=1, bspline_order=2), dict(bspline_order=3),
+    for kw in (dict(lagrange_order=3), dict(refine_levels=1, bspline_order=2), dict(bspline_order=3),
                dict(enrich=1)):
         spec, keep = M.make_spec(3, (4, 4, 4), iwg=M.IWG_STRUC_LINEAR_BULK, spheres=[[0.5, 0.5, 0.5, 0.3]], **kw)
         with pytest.raises(M.MgpuError):
