@@ -215,28 +215,56 @@ mgpu_status build_generic(const mgpu_set_desc& d, SetHost& h, std::string* err) 
         });
         return MGPU_OK;
     }
-    // gather lists: K_e entry (x, y) of element e goes to CSR value (topo[x], topo[y])
+    // gather lists: K_e entry (x, y) of element e goes to CSR value (topo[x], topo[y]).  Built per
+    // CSR row, rows in parallel: the values of row f N + a take entries only from the elements
+    // holding adof a (a_el, ascending), each element at most one entry per value (its topology has
+    // no repeated adof), so visiting those elements in ascending order lists each value's entries
+    // in ascending k = ke_off[e] + x na + y, the order of summation.  The element's topology and
+    // the row's columns are both ascending: one merge per (row, element) finds every column.
     if (n_src >= ((int64_t)1 << 40)) { *err = "element matrices too large"; return MGPU_ERR_INVALID_ARG; }
-    std::vector<int64_t> pos((size_t)n_src);
-    parallel_for(h.n_el, [&](int64_t e) {
-        const int32_t* t = h.topo.data() + h.topo_ptr[(size_t)e];
-        const int64_t na = h.topo_ptr[(size_t)e + 1] - h.topo_ptr[(size_t)e];
-        for (int64_t x = 0; x < na; ++x) {
-            const int32_t row = t[x];
-            const int32_t* c0 = h.col_idx.data() + h.row_ptr[(size_t)row];
-            const int32_t* c1 = h.col_idx.data() + h.row_ptr[(size_t)row + 1];
-            for (int64_t y = 0; y < na; ++y)
-                pos[(size_t)(h.ke_off[(size_t)e] + x * na + y)] = (int64_t)(std::lower_bound(c0, c1, t[y]) - h.col_idx.data());
+    // visit(row, fn): fn(p, k) for every entry k of the row's values p, elements ascending
+    auto visit = [&](int64_t row, auto&& fn) {
+        const int32_t a = (int32_t)(row % N);
+        const int f = (int)(row / N);
+        const int32_t* cols = h.col_idx.data();
+        const int64_t c0 = h.row_ptr[(size_t)row], c1 = h.row_ptr[(size_t)row + 1];
+        for (int64_t q = a_ptr[(size_t)a]; q < a_ptr[(size_t)a + 1]; ++q) {
+            const int32_t e = a_el[(size_t)q];
+            const auto& u = loc[(size_t)e];
+            const int64_t nat = (int64_t)u.size(), na = (int64_t)nf * nat;
+            const int64_t x = f * nat + (int64_t)(std::lower_bound(u.begin(), u.end(), a) - u.begin());
+            const int32_t* t = h.topo.data() + h.topo_ptr[(size_t)e];
+            const int64_t k0 = h.ke_off[(size_t)e] + x * na;
+            int64_t p = c0;
+            for (int64_t y = 0; y < na; ++y) {
+                while (p < c1 && cols[p] < t[y]) ++p;
+                fn(p, k0 + y);
+            }
+        }
+    };
+    h.gather_ptr.assign((size_t)nnz + 1, 0);
+    parallel_for(h.n_rows, [&](int64_t row) { visit(row, [&](int64_t p, int64_t) { h.gather_ptr[(size_t)p + 1]++; }); });
+    // prefix sums: row totals in order, then each row's values from its base, rows in parallel
+    std::vector<int64_t> row_base((size_t)h.n_rows + 1, 0);
+    parallel_for(h.n_rows, [&](int64_t row) {
+        int64_t s = 0;
+        for (int64_t p = h.row_ptr[(size_t)row]; p < h.row_ptr[(size_t)row + 1]; ++p) s += h.gather_ptr[(size_t)p + 1];
+        row_base[(size_t)row + 1] = s;
+    });
+    for (int32_t r = 0; r < h.n_rows; ++r) row_base[(size_t)r + 1] += row_base[(size_t)r];
+    parallel_for(h.n_rows, [&](int64_t row) {
+        int64_t s = row_base[(size_t)row];
+        for (int64_t p = h.row_ptr[(size_t)row]; p < h.row_ptr[(size_t)row + 1]; ++p) {
+            s += h.gather_ptr[(size_t)p + 1];
+            h.gather_ptr[(size_t)p + 1] = s;
         }
     });
-    h.gather_ptr.assign((size_t)nnz + 1, 0);
-    for (int64_t k = 0; k < n_src; ++k) h.gather_ptr[(size_t)pos[(size_t)k] + 1]++;
-    for (int64_t p = 0; p < nnz; ++p) h.gather_ptr[(size_t)p + 1] += h.gather_ptr[(size_t)p];
     h.gather_src.resize((size_t)n_src);
-    {
-        std::vector<int64_t> fill(h.gather_ptr.begin(), h.gather_ptr.end() - 1);
-        for (int64_t k = 0; k < n_src; ++k) h.gather_src[(size_t)fill[(size_t)pos[(size_t)k]]++] = k;
-    }
+    parallel_for(h.n_rows, [&](int64_t row) {
+        const int64_t c0 = h.row_ptr[(size_t)row], c1 = h.row_ptr[(size_t)row + 1];
+        std::vector<int64_t> fill(h.gather_ptr.begin() + c0, h.gather_ptr.begin() + c1);
+        visit(row, [&](int64_t p, int64_t k) { h.gather_src[(size_t)fill[(size_t)(p - c0)]++] = k; });
+    });
     return MGPU_OK;
 }
 
